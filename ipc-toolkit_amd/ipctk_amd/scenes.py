@@ -1,0 +1,141 @@
+"""Seeded synthetic contact scenes (the workloads the benchmarks quote).
+
+Every generator is deterministic given its seed and returns float64
+positions (nV, 3), int32 edges (nE, 2) and faces (nF, 3), plus the scene
+parameters.  Edges are the unique undirected face edges, each stored as
+(min, max) and sorted lexicographically (the set ``igl::edges`` returns).
+
+Sheets are n x n vertex grids over the unit square, vertex (i, j) at
+(j h, i h) with h = 1/(n-1); each grid cell is split along its
+(i,j)-(i+1,j+1) diagonal into two counter-clockwise triangles.
+"""
+import math
+
+import numpy as np
+
+
+def grid_sheet(n, z=0.0, offset=(0.0, 0.0), size=1.0):
+    h = size / (n - 1)
+    jj, ii = np.meshgrid(np.arange(n), np.arange(n))
+    V = np.zeros((n * n, 3))
+    V[:, 0] = jj.reshape(-1) * h + offset[0]
+    V[:, 1] = ii.reshape(-1) * h + offset[1]
+    V[:, 2] = z
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    v00 = (i * n + j).reshape(-1)
+    v01 = v00 + 1
+    v10 = v00 + n
+    v11 = v10 + 1
+    F = np.empty((2 * (n - 1) ** 2, 3), dtype=np.int64)
+    F[0::2] = np.stack([v00, v01, v11], 1)
+    F[1::2] = np.stack([v00, v11, v10], 1)
+    return V, F.astype(np.int32)
+
+
+def edges_from_faces(F):
+    F = np.asarray(F, dtype=np.int64)
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    e.sort(axis=1)
+    e = np.unique(e, axis=0)
+    return e.astype(np.int32)
+
+
+def merge(meshes):
+    Vs, Fs, off = [], [], 0
+    for V, F in meshes:
+        Vs.append(V)
+        Fs.append(F + off)
+        off += len(V)
+    V = np.concatenate(Vs)
+    F = np.concatenate(Fs).astype(np.int32)
+    return V, edges_from_faces(F), F
+
+
+class Scene:
+    """Positions at t0 (V), the step target V1, topology and parameters."""
+
+    def __init__(self, name, V, E, F, V1, dhat, kappa, dmin=0.0, rest=None, **meta):
+        self.name = name
+        self.V = np.ascontiguousarray(V, dtype=np.float64)
+        self.E = np.ascontiguousarray(E, dtype=np.int32)
+        self.F = np.ascontiguousarray(F, dtype=np.int32)
+        self.V1 = np.ascontiguousarray(V1, dtype=np.float64)
+        self.rest = self.V if rest is None else np.ascontiguousarray(rest, np.float64)
+        self.dhat = float(dhat)
+        self.kappa = float(kappa)
+        self.dmin = float(dmin)
+        self.meta = meta
+
+    @property
+    def sizes(self):
+        return len(self.V), len(self.E), len(self.F)
+
+
+def two_sheets(n=64, seed=0, dhat=1e-3, kappa=1e3):
+    """Two stacked n x n sheets: bottom at z = 0, top at
+    z = 0.5 dhat + U(-0.25, 0.25) dhat per vertex.  Step: the top sheet
+    moves by (0, 0, -1e-3) + U(-1e-4, 1e-4)^3 per vertex."""
+    rng = np.random.default_rng(seed)
+    Vb, Fb = grid_sheet(n, 0.0)
+    Vt, Ft = grid_sheet(n, 0.0)
+    Vt[:, 2] = 0.5 * dhat + rng.uniform(-0.25, 0.25, len(Vt)) * dhat
+    V, E, F = merge([(Vb, Fb), (Vt, Ft)])
+    V1 = V.copy()
+    nb = len(Vb)
+    disp = np.zeros((len(Vt), 3))
+    disp[:, 2] = -1e-3
+    disp += rng.uniform(-1e-4, 1e-4, (len(Vt), 3))
+    V1[nb:] += disp
+    return Scene(f"two_sheets_{n}", V, E, F, V1, dhat, kappa, seed=seed)
+
+
+def stacked_sheets(n=512, sheets=8, seed=0, dhat=1e-3, kappa=1e4, spacing=0.6,
+                   jitter=0.2, inplane_sigma=0.5, dz=-0.8):
+    """`sheets` n x n sheets stacked in z, sheet k at k * spacing * dhat plus a
+    per-vertex z-jitter U(-jitter, jitter) dhat.  "Interleaved": odd sheets
+    are shifted by half a grid cell in x and y, so no vertex sits exactly
+    above a vertex of the neighbouring sheets.  Step: per-vertex in-plane
+    N(0, (inplane_sigma dhat)^2) and a uniform dz * dhat in z."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (n - 1)
+    meshes = []
+    for k in range(sheets):
+        off = (0.5 * h, 0.5 * h) if (k % 2) else (0.0, 0.0)
+        Vk, Fk = grid_sheet(n, 0.0, offset=off)
+        Vk[:, 2] = k * spacing * dhat + rng.uniform(-jitter, jitter, len(Vk)) * dhat
+        meshes.append((Vk, Fk))
+    V, E, F = merge(meshes)
+    V1 = V.copy()
+    V1[:, :2] += rng.normal(0.0, inplane_sigma * dhat, (len(V), 2))
+    V1[:, 2] += dz * dhat
+    return Scene(f"stacked_sheets_{sheets}x{n}", V, E, F, V1, dhat, kappa, seed=seed,
+                 interleaved="odd sheets shifted by half a cell in x and y")
+
+
+def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+         (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9),
+         (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2),
+         (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    V = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    for _ in range(subdiv):
+        cache = {}
+        newF = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[key] = len(V) - 1
+            return cache[key]
+
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            newF += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = newF
+    V = np.array(V) * radius + np.asarray(center, dtype=np.float64)
+    return V, np.array(F, dtype=np.int32)

@@ -1,0 +1,491 @@
+/* Broad phase restatement: conservative AABBs, Morton codes, LBVH build
+ * (Apetrei 2014 as in lbvh.cpp) and candidate traversal.
+ * TEST INFRASTRUCTURE ONLY (see ipc_oracle.h). */
+#include "ipco_internal.h"
+
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void ipco_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int ipco_get_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---- boxes (aabb.cpp:35-126) --------------------------------------------- */
+typedef struct box64 {
+    double lo[3], hi[3];
+} box64;
+
+/* AABB::from_point + conservative_inflation: nextafter(v -/+ r, -/+inf) */
+static void point_box(const double *p, double r, box64 *b)
+{
+    for (int k = 0; k < 3; k++) {
+        b->lo[k] = nextafter(p[k] - r, -INFINITY);
+        b->hi[k] = nextafter(p[k] + r, INFINITY);
+    }
+}
+
+static void box_union(const box64 *a, const box64 *b, box64 *r)
+{
+    for (int k = 0; k < 3; k++) {
+        r->lo[k] = b->lo[k] < a->lo[k] ? b->lo[k] : a->lo[k];
+        r->hi[k] = a->hi[k] < b->hi[k] ? b->hi[k] : a->hi[k];
+    }
+}
+
+static box64 *vertex_boxes64(const double *V0, const double *V1, int nV,
+                             double r)
+{
+    box64 *vb = (box64 *)malloc(sizeof(box64) * (size_t)(nV > 0 ? nV : 1));
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nV; i++) {
+        point_box(V0 + 3 * (size_t)i, r, &vb[i]);
+        if (V1) {
+            box64 b1;
+            point_box(V1 + 3 * (size_t)i, r, &b1);
+            box_union(&vb[i], &b1, &vb[i]);
+        }
+    }
+    return vb;
+}
+
+static box64 *element_boxes64(const box64 *vb, const int32_t *I, int n,
+                              int arity)
+{
+    box64 *eb = (box64 *)malloc(sizeof(box64) * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        box64 b = vb[I[(size_t)arity * i]];
+        for (int j = 1; j < arity; j++)
+            box_union(&b, &vb[I[(size_t)arity * i + j]], &b);
+        eb[i] = b;
+    }
+    return eb;
+}
+
+/* lbvh.cpp:29-41 assign_inflated_aabb */
+static void box_to_f32(const box64 *b, float *f)
+{
+    for (int k = 0; k < 3; k++) {
+        f[k] = nextafterf((float)b->lo[k], -INFINITY);
+        f[3 + k] = nextafterf((float)b->hi[k], INFINITY);
+    }
+}
+
+void ipco_build_boxes_f32(const double *V0, const double *V1, int nV,
+                          const int32_t *E, int nE, const int32_t *F, int nF,
+                          double inflation, float *vbox, float *ebox,
+                          float *fbox)
+{
+    box64 *vb = vertex_boxes64(V0, V1, nV, inflation);
+    if (vbox)
+        for (int i = 0; i < nV; i++) box_to_f32(&vb[i], vbox + 6 * (size_t)i);
+    if (ebox && nE > 0) {
+        box64 *eb = element_boxes64(vb, E, nE, 2);
+        for (int i = 0; i < nE; i++) box_to_f32(&eb[i], ebox + 6 * (size_t)i);
+        free(eb);
+    }
+    if (fbox && nF > 0) {
+        box64 *fb = element_boxes64(vb, F, nF, 3);
+        for (int i = 0; i < nF; i++) box_to_f32(&fb[i], fbox + 6 * (size_t)i);
+        free(fb);
+    }
+    free(vb);
+}
+
+/* ---- Morton codes (math/morton.hpp:24-63) -------------------------------- */
+static uint64_t expand_bits_2(uint64_t v)
+{
+    v = (v | v << 32) & 0x1F00000000FFFFull;
+    v = (v | v << 16) & 0x1F0000FF0000FFull;
+    v = (v | v << 8) & 0x100F00F00F00F00Full;
+    v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+static double clampd(double v, double lo, double hi)
+{
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+
+static uint64_t morton_3d(double x, double y, double z)
+{
+    const double scale = 2097152.0; /* 1 << 21 */
+    x = clampd(x * scale, 0.0, scale - 1);
+    y = clampd(y * scale, 0.0, scale - 1);
+    z = clampd(z * scale, 0.0, scale - 1);
+    return (expand_bits_2((uint64_t)x) << 2) | (expand_bits_2((uint64_t)y) << 1)
+        | expand_bits_2((uint64_t)z);
+}
+
+/* stable LSD radix sort of (key, val) by key */
+void ipco_sort_u64_pairs(uint64_t *key, int64_t *val, int64_t n)
+{
+    uint64_t *k2 = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t *v2 = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    uint64_t all_or = 0;
+    for (int64_t i = 0; i < n; i++) all_or |= key[i];
+    for (int shift = 0; shift < 64; shift += 8) {
+        if (((all_or >> shift) & 0xFF) == 0) continue;
+        int64_t cnt[257];
+        memset(cnt, 0, sizeof(cnt));
+        for (int64_t i = 0; i < n; i++) cnt[((key[i] >> shift) & 0xFF) + 1]++;
+        for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t dst = cnt[(key[i] >> shift) & 0xFF]++;
+            k2[dst] = key[i];
+            v2[dst] = val[i];
+        }
+        memcpy(key, k2, sizeof(uint64_t) * (size_t)n);
+        memcpy(val, v2, sizeof(int64_t) * (size_t)n);
+    }
+    free(k2);
+    free(v2);
+}
+
+/* ---- LBVH (lbvh.cpp:99-330) ---------------------------------------------- */
+typedef struct lbvh {
+    int n;           /* leaves */
+    float *box;      /* (2n-1) x 6 */
+    int32_t *left;   /* inner: left child; leaf: primitive id */
+    int32_t *right;  /* inner: right child (never 0); leaf: 0 */
+    int32_t *rml;    /* rightmost leaf (sorted position) per node */
+} lbvh;
+
+static void lbvh_free(lbvh *t)
+{
+    free(t->box);
+    free(t->left);
+    free(t->right);
+    free(t->rml);
+    memset(t, 0, sizeof(*t));
+}
+
+static int delta(const uint64_t *codes, int n, int i, uint64_t ci, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const uint64_t cj = codes[j];
+    if (ci == cj) return 32 + __builtin_clz((unsigned)(i ^ j));
+    return __builtin_clzll(ci ^ cj);
+}
+
+static void lbvh_build(const box64 *boxes, int n, const double *mesh_min,
+                       const double *mesh_max, lbvh *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->n = n;
+    if (n <= 0) return;
+    const int nn = 2 * n - 1;
+    t->box = (float *)malloc(sizeof(float) * 6 * (size_t)nn);
+    t->left = (int32_t *)calloc((size_t)nn, sizeof(int32_t));
+    t->right = (int32_t *)calloc((size_t)nn, sizeof(int32_t));
+    t->rml = (int32_t *)malloc(sizeof(int32_t) * (size_t)nn);
+
+    uint64_t *codes = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+    int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    double width[3];
+    for (int k = 0; k < 3; k++) width[k] = mesh_max[k] - mesh_min[k];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double m[3];
+        for (int k = 0; k < 3; k++) {
+            const double c = 0.5 * (boxes[i].lo[k] + boxes[i].hi[k]);
+            m[k] = (c - mesh_min[k]) / width[k];
+        }
+        codes[i] = morton_3d(m[0], m[1], m[2]);
+        ids[i] = i;
+    }
+    /* The reference sorts by code only with tbb::parallel_sort (ties in any
+     * order); this restatement breaks ties by primitive id (stable sort). */
+    ipco_sort_u64_pairs(codes, ids, n);
+
+    const int LEAF = n - 1;
+    int *visits = (int *)calloc((size_t)nn, sizeof(int));
+    int32_t *lrange = (int32_t *)malloc(sizeof(int32_t) * (size_t)nn);
+    int32_t *rrange = (int32_t *)malloc(sizeof(int32_t) * (size_t)nn);
+    int root = (n == 1) ? 0 : -1;
+
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        box_to_f32(&boxes[ids[i]], t->box + 6 * (size_t)(LEAF + i));
+        t->left[LEAF + i] = (int32_t)ids[i];
+        t->right[LEAF + i] = 0;
+        t->rml[LEAF + i] = i;
+        if (n == 1) continue;
+        int lk = i, rk = i, cur = LEAF + i;
+        for (;;) {
+            const int is_a = (lk == 0)
+                || (rk != n - 1
+                    && delta(codes, n, rk, codes[rk], rk + 1)
+                        > delta(codes, n, lk - 1, codes[lk - 1], lk));
+            const int parent = is_a ? rk : lk - 1;
+            if (is_a) {
+                __atomic_store_n(&t->left[parent], cur, __ATOMIC_RELAXED);
+                __atomic_store_n(&lrange[parent], lk, __ATOMIC_RELAXED);
+            } else {
+                __atomic_store_n(&t->right[parent], cur, __ATOMIC_RELAXED);
+                __atomic_store_n(&rrange[parent], rk, __ATOMIC_RELAXED);
+            }
+            if (__atomic_fetch_add(&visits[parent], 1, __ATOMIC_ACQ_REL) == 0)
+                break;
+            const int l = t->left[parent], r = t->right[parent];
+            const float *bl = t->box + 6 * (size_t)l, *br = t->box + 6 * (size_t)r;
+            float *bp = t->box + 6 * (size_t)parent;
+            for (int k = 0; k < 3; k++) {
+                bp[k] = br[k] < bl[k] ? br[k] : bl[k];
+                bp[3 + k] = bl[3 + k] < br[3 + k] ? br[3 + k] : bl[3 + k];
+            }
+            t->rml[parent] = t->rml[l] > t->rml[r] ? t->rml[l] : t->rml[r];
+            lk = lrange[parent];
+            rk = rrange[parent];
+            cur = parent;
+            if (lk == 0 && rk == n - 1) {
+                root = parent;
+                break;
+            }
+        }
+    }
+    /* move the root to index 0 (lbvh.cpp:306-329) */
+    if (root > 0) {
+        float tb[6];
+        memcpy(tb, t->box, sizeof(tb));
+        memcpy(t->box, t->box + 6 * (size_t)root, sizeof(tb));
+        memcpy(t->box + 6 * (size_t)root, tb, sizeof(tb));
+        int32_t x;
+        x = t->left[0]; t->left[0] = t->left[root]; t->left[root] = x;
+        x = t->right[0]; t->right[0] = t->right[root]; t->right[root] = x;
+        x = t->rml[0]; t->rml[0] = t->rml[root]; t->rml[root] = x;
+        for (int i = 0; i < nn; i++)
+            if (t->right[i] != 0 && t->left[i] == 0) t->left[i] = root;
+    }
+    free(visits);
+    free(lrange);
+    free(rrange);
+    free(codes);
+    free(ids);
+}
+
+static inline int box_overlap(const float *a, const float *b)
+{
+    return a[0] <= b[3] && a[1] <= b[4] && a[2] <= b[5] && b[0] <= a[3]
+        && b[1] <= a[4] && b[2] <= a[5];
+}
+
+/* ---- candidate filters (lbvh.cpp:810-873, collision_filter.hpp) ---------- */
+typedef struct filt {
+    const int32_t *E, *F, *group;
+} filt;
+
+static inline int can_vv(const filt *f, int a, int b)
+{
+    return f->group == NULL || f->group[a] != f->group[b];
+}
+
+static int can_ee(const filt *f, int ea, int eb)
+{
+    const int a0 = f->E[2 * (size_t)ea], a1 = f->E[2 * (size_t)ea + 1];
+    const int b0 = f->E[2 * (size_t)eb], b1 = f->E[2 * (size_t)eb + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return 0;
+    return can_vv(f, a0, b0) || can_vv(f, a0, b1) || can_vv(f, a1, b0)
+        || can_vv(f, a1, b1);
+}
+
+static int can_fv(const filt *f, int fi, int vi)
+{
+    const int f0 = f->F[3 * (size_t)fi], f1 = f->F[3 * (size_t)fi + 1],
+              f2 = f->F[3 * (size_t)fi + 2];
+    if (vi == f0 || vi == f1 || vi == f2) return 0;
+    return can_vv(f, vi, f0) || can_vv(f, vi, f1) || can_vv(f, vi, f2);
+}
+
+typedef struct pairvec {
+    int64_t n, cap;
+    int32_t *ij;
+} pairvec;
+
+static void pv_push(pairvec *v, int32_t a, int32_t b)
+{
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 1024;
+        v->ij = (int32_t *)realloc(v->ij, sizeof(int32_t) * 2 * (size_t)v->cap);
+    }
+    v->ij[2 * v->n] = a;
+    v->ij[2 * v->n + 1] = b;
+    v->n++;
+}
+
+/* traverse_lbvh (lbvh.cpp:368-455): kind 0 = EE (triangular),
+ * 1 = FV (query = vertex leaf, swap order -> (face, vertex)) */
+static void traverse(const float *qbox, int qprim, int qpos, const lbvh *t,
+                     int kind, const filt *f, pairvec *out)
+{
+    const int nn = 2 * t->n - 1;
+    if (nn == 1) {
+        if (kind == 0) return;
+        if (box_overlap(t->box, qbox) && can_fv(f, t->left[0], qprim))
+            pv_push(out, t->left[0], qprim);
+        return;
+    }
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    int node = 0;
+    do {
+        const int l = t->left[node], r = t->right[node];
+        int il = box_overlap(t->box + 6 * (size_t)l, qbox);
+        int ir = box_overlap(t->box + 6 * (size_t)r, qbox);
+        if (kind == 0) {
+            if (il && t->rml[l] <= qpos) il = 0;
+            if (ir && t->rml[r] <= qpos) ir = 0;
+        }
+        const int leaf_l = t->right[l] == 0, leaf_r = t->right[r] == 0;
+        if (il && leaf_l) {
+            const int j = t->left[l];
+            if (kind == 0 ? can_ee(f, qprim, j) : can_fv(f, j, qprim))
+                kind == 0 ? pv_push(out, qprim, j) : pv_push(out, j, qprim);
+        }
+        if (ir && leaf_r) {
+            const int j = t->left[r];
+            if (kind == 0 ? can_ee(f, qprim, j) : can_fv(f, j, qprim))
+                kind == 0 ? pv_push(out, qprim, j) : pv_push(out, j, qprim);
+        }
+        const int tl = il && !leaf_l, tr = ir && !leaf_r;
+        if (!tl && !tr) {
+            node = stack[--sp];
+        } else {
+            node = tl ? l : r;
+            if (tl && tr) {
+                if (sp >= 128) {
+                    fprintf(stderr, "ipco: traversal stack overflow\n");
+                    abort();
+                }
+                stack[sp++] = r;
+            }
+        }
+    } while (node != 0);
+}
+
+static void detect(const lbvh *src, const lbvh *dst, int kind, const filt *f,
+                   ipco_pairs *out)
+{
+    out->n = 0;
+    out->ij = NULL;
+    if (src->n == 0 || dst->n == 0) return;
+    if (kind == 0 && dst->n <= 1) return;
+    const int LEAF = src->n - 1;
+    int nt = ipco_get_max_threads();
+    pairvec *loc = (pairvec *)calloc((size_t)nt, sizeof(pairvec));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < src->n; i++)
+            traverse(src->box + 6 * (size_t)(LEAF + i), src->left[LEAF + i], i,
+                     dst, kind, f, &loc[tid]);
+    }
+    int64_t tot = 0;
+    for (int k = 0; k < nt; k++) tot += loc[k].n;
+    out->n = tot;
+    out->ij = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(tot > 0 ? tot : 1));
+    int64_t o = 0;
+    for (int k = 0; k < nt; k++) {
+        if (loc[k].n)
+            memcpy(out->ij + 2 * o, loc[k].ij, sizeof(int32_t) * 2 * (size_t)loc[k].n);
+        o += loc[k].n;
+        free(loc[k].ij);
+    }
+    free(loc);
+}
+
+int ipco_candidates(const double *V0, const double *V1, int nV,
+                    const int32_t *E, int nE, const int32_t *F, int nF,
+                    double inflation, const int32_t *vertex_group,
+                    ipco_pairs *ee, ipco_pairs *fv)
+{
+    ee->n = fv->n = 0;
+    ee->ij = fv->ij = NULL;
+    if (nV <= 0) return 0;
+    box64 *vb = vertex_boxes64(V0, V1, nV, inflation);
+    box64 *eb = nE > 0 ? element_boxes64(vb, E, nE, 2) : NULL;
+    box64 *fb = nF > 0 ? element_boxes64(vb, F, nF, 3) : NULL;
+    /* broad_phase.cpp:93-125 compute_mesh_aabb over vertex boxes */
+    double mn[3] = { INFINITY, INFINITY, INFINITY };
+    double mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = 0; i < nV; i++)
+        for (int k = 0; k < 3; k++) {
+            if (vb[i].lo[k] < mn[k]) mn[k] = vb[i].lo[k];
+            if (mx[k] < vb[i].hi[k]) mx[k] = vb[i].hi[k];
+        }
+    lbvh tv, te, tf;
+    lbvh_build(vb, nV, mn, mx, &tv);
+    lbvh_build(eb, nE, mn, mx, &te);
+    lbvh_build(fb, nF, mn, mx, &tf);
+    filt f = { E, F, vertex_group };
+    detect(&te, &te, 0, &f, ee); /* lbvh.cpp:742-754 */
+    detect(&tv, &tf, 1, &f, fv); /* lbvh.cpp:756-769 */
+    lbvh_free(&tv);
+    lbvh_free(&te);
+    lbvh_free(&tf);
+    free(vb);
+    free(eb);
+    free(fb);
+    return 0;
+}
+
+void ipco_pairs_free(ipco_pairs *p)
+{
+    free(p->ij);
+    p->ij = NULL;
+    p->n = 0;
+}
+
+int ipco_faces_to_edges(const int32_t *E, int nE, const int32_t *F, int nF,
+                        int32_t *F2E)
+{
+    /* key (min,max) -> edge id via sort + binary search */
+    uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(nE > 0 ? nE : 1));
+    int64_t *id = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nE > 0 ? nE : 1));
+    for (int i = 0; i < nE; i++) {
+        uint32_t a = (uint32_t)E[2 * i], b = (uint32_t)E[2 * i + 1];
+        if (a > b) { uint32_t t = a; a = b; b = t; }
+        key[i] = ((uint64_t)a << 32) | b;
+        id[i] = i;
+    }
+    ipco_sort_u64_pairs(key, id, nE);
+    int rc = 0;
+    for (int f = 0; f < nF; f++)
+        for (int j = 0; j < 3; j++) {
+            uint32_t a = (uint32_t)F[3 * f + j], b = (uint32_t)F[3 * f + (j + 1) % 3];
+            if (a > b) { uint32_t t = a; a = b; b = t; }
+            const uint64_t k = ((uint64_t)a << 32) | b;
+            int64_t lo = 0, hi = nE;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (key[mid] < k) lo = mid + 1; else hi = mid;
+            }
+            if (lo < nE && key[lo] == k) F2E[3 * f + j] = (int32_t)id[lo];
+            else { F2E[3 * f + j] = -1; rc = -1; }
+        }
+    free(key);
+    free(id);
+    return rc;
+}

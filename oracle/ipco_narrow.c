@@ -1,0 +1,465 @@
+/* Narrow phase (collision-set build) and barrier potential restatement.
+ * TEST INFRASTRUCTURE ONLY (see ipc_oracle.h). */
+#include "ipco_internal.h"
+
+enum { T_VV = 0, T_EV = 1, T_EE = 2, T_FV = 3 };
+
+typedef struct rec {
+    int32_t type; /* -1 inactive */
+    int32_t a, b; /* VV (min,max); EV (edge,vertex); EE (ea,eb); FV (f,v) */
+    int32_t dtype;
+    double eps_x;
+} rec;
+
+static void vv_rec(rec *r, int32_t u, int32_t v)
+{
+    r->type = T_VV;
+    r->a = u < v ? u : v;
+    r->b = u < v ? v : u;
+}
+
+static void ev_rec(rec *r, int32_t e, int32_t v)
+{
+    r->type = T_EV;
+    r->a = e;
+    r->b = v;
+}
+
+static inline const double *vx(const double *V, int i)
+{
+    return V + 3 * (size_t)i;
+}
+
+/* normal_collisions_builder.cpp:239-336 add_face_vertex_collision */
+static void fv_candidate(const double *V, const int32_t *F, const int32_t *F2E,
+                         int32_t fi, int32_t vi, double active2, rec *r)
+{
+    r->type = -1;
+    const int32_t f0 = F[3 * (size_t)fi], f1 = F[3 * (size_t)fi + 1],
+                  f2 = F[3 * (size_t)fi + 2];
+    const double *p = vx(V, vi), *t0 = vx(V, f0), *t1 = vx(V, f1),
+                 *t2 = vx(V, f2);
+    const int dt = ipco_point_triangle_distance_type(p, t0, t1, t2);
+    const double d = ipco_point_triangle_distance(p, t0, t1, t2, dt);
+    if (!(d < active2)) return;
+    switch (dt) {
+    case IPCO_PT_P_T0: vv_rec(r, vi, f0); break;
+    case IPCO_PT_P_T1: vv_rec(r, vi, f1); break;
+    case IPCO_PT_P_T2: vv_rec(r, vi, f2); break;
+    case IPCO_PT_P_E0: ev_rec(r, F2E[3 * (size_t)fi], vi); break;
+    case IPCO_PT_P_E1: ev_rec(r, F2E[3 * (size_t)fi + 1], vi); break;
+    case IPCO_PT_P_E2: ev_rec(r, F2E[3 * (size_t)fi + 2], vi); break;
+    default:
+        r->type = T_FV;
+        r->a = fi;
+        r->b = vi;
+    }
+}
+
+/* normal_collisions_builder.cpp:140-237 add_edge_edge_collision */
+static void ee_candidate(const double *V, const double *Vr, const int32_t *E,
+                         int32_t ea, int32_t eb, double active2, rec *r)
+{
+    r->type = -1;
+    const int32_t a0 = E[2 * (size_t)ea], a1 = E[2 * (size_t)ea + 1];
+    const int32_t b0 = E[2 * (size_t)eb], b1 = E[2 * (size_t)eb + 1];
+    const double *pa0 = vx(V, a0), *pa1 = vx(V, a1), *pb0 = vx(V, b0),
+                 *pb1 = vx(V, b1);
+    const int actual = ipco_edge_edge_distance_type(pa0, pa1, pb0, pb1);
+    const double d = ipco_edge_edge_distance(pa0, pa1, pb0, pb1, actual);
+    if (!(d < active2)) return;
+    const double eps_x = ipco_edge_edge_mollifier_threshold(
+        vx(Vr, a0), vx(Vr, a1), vx(Vr, b0), vx(Vr, b1));
+    const double x = ipco_edge_edge_cross_squarednorm(pa0, pa1, pb0, pb1);
+    const int dt = x < eps_x ? IPCO_EE_EA_EB : actual;
+    switch (dt) {
+    case IPCO_EE_EA0_EB0: vv_rec(r, a0, b0); break;
+    case IPCO_EE_EA0_EB1: vv_rec(r, a0, b1); break;
+    case IPCO_EE_EA1_EB0: vv_rec(r, a1, b0); break;
+    case IPCO_EE_EA1_EB1: vv_rec(r, a1, b1); break;
+    case IPCO_EE_EA_EB0: ev_rec(r, ea, b0); break;
+    case IPCO_EE_EA_EB1: ev_rec(r, ea, b1); break;
+    case IPCO_EE_EA0_EB: ev_rec(r, eb, a0); break;
+    case IPCO_EE_EA1_EB: ev_rec(r, eb, a1); break;
+    default:
+        r->type = T_EE;
+        r->a = ea;
+        r->b = eb;
+        r->dtype = actual;
+        r->eps_x = eps_x;
+    }
+}
+
+static uint64_t rec_key(const rec *r)
+{
+    if (r->type == T_EE) {
+        const uint32_t lo = (uint32_t)(r->a < r->b ? r->a : r->b);
+        const uint32_t hi = (uint32_t)(r->a < r->b ? r->b : r->a);
+        /* edge ids < 2^28 assumed by this packing; dtype in the low 4 bits */
+        return ((uint64_t)lo << 36) | ((uint64_t)hi << 4) | (uint64_t)r->dtype;
+    }
+    return ((uint64_t)(uint32_t)r->a << 32) | (uint32_t)r->b;
+}
+
+int ipco_build_collisions(const double *V, int nV, const double *V_rest,
+                          const int32_t *E, int nE, const int32_t *F, int nF,
+                          const int32_t *F2E, const ipco_pairs *ee_cand,
+                          const ipco_pairs *fv_cand, double dhat, double dmin,
+                          ipco_collisions *out)
+{
+    (void)nV;
+    (void)nE;
+    (void)nF;
+    memset(out, 0, sizeof(*out));
+    out->dmin = dmin;
+    const double off = dmin + dhat;
+    const double active2 = off * off; /* normal_collisions.cpp:52-54 */
+    const int64_t nfv = fv_cand ? fv_cand->n : 0;
+    const int64_t nee = ee_cand ? ee_cand->n : 0;
+    const int64_t n = nfv + nee;
+    rec *R = (rec *)malloc(sizeof(rec) * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nee; i++)
+        ee_candidate(V, V_rest, E, ee_cand->ij[2 * i], ee_cand->ij[2 * i + 1],
+                     active2, &R[i]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nfv; i++)
+        fv_candidate(V, F, F2E, fv_cand->ij[2 * i], fv_cand->ij[2 * i + 1],
+                     active2, &R[nee + i]);
+
+    /* per-type sort by key, then merge duplicates summing weights
+     * (normal_collisions_builder.cpp:547-600, unit weights) */
+    for (int t = 0; t < 4; t++) {
+        int64_t m = 0;
+        for (int64_t i = 0; i < n; i++) m += R[i].type == t;
+        uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(m > 0 ? m : 1));
+        int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+        int64_t j = 0;
+        for (int64_t i = 0; i < n; i++)
+            if (R[i].type == t) {
+                key[j] = rec_key(&R[i]);
+                idx[j++] = i;
+            }
+        ipco_sort_u64_pairs(key, idx, m);
+        int64_t u = 0;
+        for (int64_t i = 0; i < m; i++)
+            if (i == 0 || key[i] != key[i - 1] || t == T_FV) u++;
+        int32_t *ids = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(u > 0 ? u : 1));
+        double *w = (double *)malloc(sizeof(double) * (size_t)(u > 0 ? u : 1));
+        double *eps = NULL;
+        int32_t *dts = NULL;
+        if (t == T_EE) {
+            eps = (double *)malloc(sizeof(double) * (size_t)(u > 0 ? u : 1));
+            dts = (int32_t *)malloc(sizeof(int32_t) * (size_t)(u > 0 ? u : 1));
+        }
+        int64_t k = -1;
+        for (int64_t i = 0; i < m; i++) {
+            const rec *r = &R[idx[i]];
+            if (i == 0 || key[i] != key[i - 1] || t == T_FV) {
+                k++;
+                ids[2 * k] = r->a;
+                ids[2 * k + 1] = r->b;
+                w[k] = 1.0;
+                if (t == T_EE) {
+                    eps[k] = r->eps_x;
+                    dts[k] = r->dtype;
+                }
+            } else {
+                w[k] += 1.0;
+            }
+        }
+        free(key);
+        free(idx);
+        switch (t) {
+        case T_VV: out->nvv = u; out->vv = ids; out->vv_w = w; break;
+        case T_EV: out->nev = u; out->ev = ids; out->ev_w = w; break;
+        case T_EE:
+            out->nee = u;
+            out->ee = ids;
+            out->ee_w = w;
+            out->ee_eps_x = eps;
+            out->ee_dtype = dts;
+            break;
+        default: out->nfv = u; out->fv = ids; out->fv_w = w; break;
+        }
+    }
+    free(R);
+    return 0;
+}
+
+void ipco_collisions_free(ipco_collisions *c)
+{
+    free(c->vv); free(c->vv_w);
+    free(c->ev); free(c->ev_w);
+    free(c->ee); free(c->ee_w); free(c->ee_eps_x); free(c->ee_dtype);
+    free(c->fv); free(c->fv_w);
+    memset(c, 0, sizeof(*c));
+}
+
+/* ========================================================================= */
+/* Barrier potential                                                         */
+/* ========================================================================= */
+
+typedef struct stencil {
+    int kind;  /* ipco_stencil_distance_derivs kind */
+    int dtype; /* fixed dtype used for E/grad/hess */
+    int nv;
+    int32_t ids[4];
+    double w, eps_x; /* eps_x < 0: not mollified */
+} stencil;
+
+static int64_t n_collisions(const ipco_collisions *c)
+{
+    return c->nvv + c->nev + c->nee + c->nfv;
+}
+
+/* CollisionStencil::vertex_ids and known_dtype: VV (v0,v1) point-point;
+ * EV (v, e0, e1) P_E; EE (ea0,ea1,eb0,eb1) stored dtype; FV (v,f0,f1,f2) P_T
+ * (face_vertex.hpp:28-32, edge_vertex.hpp:28-32, edge_edge.hpp:96). */
+static void get_stencil(const ipco_collisions *c, const int32_t *E,
+                        const int32_t *F, int64_t i, stencil *s)
+{
+    s->eps_x = -1;
+    if (i < c->nvv) {
+        s->kind = 3; s->dtype = 0; s->nv = 2;
+        s->ids[0] = c->vv[2 * i]; s->ids[1] = c->vv[2 * i + 1];
+        s->w = c->vv_w[i];
+        return;
+    }
+    i -= c->nvv;
+    if (i < c->nev) {
+        const int32_t e = c->ev[2 * i];
+        s->kind = 2; s->dtype = IPCO_PE_P_E; s->nv = 3;
+        s->ids[0] = c->ev[2 * i + 1];
+        s->ids[1] = E[2 * (size_t)e]; s->ids[2] = E[2 * (size_t)e + 1];
+        s->w = c->ev_w[i];
+        return;
+    }
+    i -= c->nev;
+    if (i < c->nee) {
+        const int32_t ea = c->ee[2 * i], eb = c->ee[2 * i + 1];
+        s->kind = 1; s->dtype = c->ee_dtype[i]; s->nv = 4;
+        s->ids[0] = E[2 * (size_t)ea]; s->ids[1] = E[2 * (size_t)ea + 1];
+        s->ids[2] = E[2 * (size_t)eb]; s->ids[3] = E[2 * (size_t)eb + 1];
+        s->w = c->ee_w[i];
+        s->eps_x = c->ee_eps_x[i];
+        return;
+    }
+    i -= c->nee;
+    const int32_t f = c->fv[2 * i];
+    s->kind = 0; s->dtype = IPCO_PT_P_T; s->nv = 4;
+    s->ids[0] = c->fv[2 * i + 1];
+    s->ids[1] = F[3 * (size_t)f]; s->ids[2] = F[3 * (size_t)f + 1];
+    s->ids[3] = F[3 * (size_t)f + 2];
+    s->w = c->fv_w[i];
+}
+
+static void gather(const double *V, const stencil *s, double *x)
+{
+    for (int i = 0; i < s->nv; i++)
+        for (int k = 0; k < 3; k++) x[3 * i + k] = V[3 * (size_t)s->ids[i] + k];
+}
+
+/* barrier_potential.cpp:62-99 */
+static double f0(double d2, double dmin, double dhat, double kappa)
+{
+    return kappa * ipco_barrier(d2 - dmin * dmin, (2 * dmin + dhat) * dhat);
+}
+static double f1(double d2, double dmin, double dhat, double kappa)
+{
+    return kappa
+        * ipco_barrier_first_derivative(d2 - dmin * dmin, (2 * dmin + dhat) * dhat);
+}
+static double f2(double d2, double dmin, double dhat, double kappa)
+{
+    return kappa
+        * ipco_barrier_second_derivative(d2 - dmin * dmin, (2 * dmin + dhat) * dhat);
+}
+
+/* normal_potential.cpp:127-136 */
+static double local_energy(const stencil *s, const double *x, double dmin,
+                           double dhat, double kappa)
+{
+    const double d = ipco_stencil_distance_derivs(s->kind, s->dtype, x, NULL, NULL);
+    const double m = s->eps_x >= 0 ? ipco_edge_edge_mollifier(x, s->eps_x, NULL, NULL) : 1.0;
+    return s->w * m * f0(d, dmin, dhat, kappa);
+}
+
+double ipco_barrier_energy(const ipco_collisions *c, const double *V, int nV,
+                           const int32_t *E, const int32_t *F, double dhat,
+                           double kappa)
+{
+    (void)nV;
+    const int64_t n = n_collisions(c);
+    double sum = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sum)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        sum += local_energy(&s, x, c->dmin, dhat, kappa);
+    }
+    return sum;
+}
+
+/* normal_potential.cpp:138-172 */
+static void local_gradient(const stencil *s, const double *x, double dmin,
+                           double dhat, double kappa, double *g)
+{
+    const int nd = 3 * s->nv;
+    double gm[12];
+    const double m = s->eps_x >= 0 ? ipco_edge_edge_mollifier(x, s->eps_x, gm, NULL) : 1.0;
+    for (int i = 0; i < nd; i++) g[i] = 0;
+    if (m <= 0) return;
+    double gd[12];
+    const double d = ipco_stencil_distance_derivs(s->kind, s->dtype, x, gd, NULL);
+    const double f = f0(d, dmin, dhat, kappa), fp = f1(d, dmin, dhat, kappa);
+    if (s->eps_x < 0) {
+        for (int i = 0; i < nd; i++) g[i] = (s->w * fp) * gd[i];
+        return;
+    }
+    for (int i = 0; i < nd; i++)
+        g[i] = (s->w * f) * gm[i] + (s->w * m * fp) * gd[i];
+}
+
+void ipco_barrier_gradient(const ipco_collisions *c, const double *V, int nV,
+                           const int32_t *E, const int32_t *F, double dhat,
+                           double kappa, double *grad)
+{
+    const int64_t n = n_collisions(c);
+    memset(grad, 0, sizeof(double) * 3 * (size_t)nV);
+    /* serial scatter in collision order (the reference sums per-thread
+     * copies; sums agree to rounding) */
+    double *loc = (double *)malloc(sizeof(double) * 12 * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        local_gradient(&s, x, c->dmin, dhat, kappa, loc + 12 * i);
+    }
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        get_stencil(c, E, F, i, &s);
+        for (int a = 0; a < s.nv; a++)
+            for (int k = 0; k < 3; k++)
+                grad[3 * (size_t)s.ids[a] + k] += loc[12 * i + 3 * a + k];
+    }
+    free(loc);
+}
+
+/* normal_potential.cpp:174-232; H row-major nd x nd */
+static int local_hessian(const stencil *s, const double *x, double dmin,
+                         double dhat, double kappa, int psd, double *H)
+{
+    const int nd = 3 * s->nv;
+    double gd[12], hdd[144], gm[12], hm[144];
+    const double d = ipco_stencil_distance_derivs(s->kind, s->dtype, x, gd, hdd);
+    const int moll = s->eps_x >= 0;
+    const double m = moll ? ipco_edge_edge_mollifier(x, s->eps_x, gm, hm) : 1.0;
+    const double w = s->w;
+    if (moll && m <= 0) {
+        const double f = f0(d, dmin, dhat, kappa);
+        for (int i = 0; i < nd * nd; i++) H[i] = (w * f) * hm[i];
+        return 0;
+    }
+    const double fp = f1(d, dmin, dhat, kappa), fpp = f2(d, dmin, dhat, kappa);
+    if (!moll) {
+        for (int i = 0; i < nd; i++)
+            for (int j = 0; j < nd; j++)
+                H[i * nd + j] = (w * fpp) * gd[i] * gd[j] + (w * fp) * hdd[i * nd + j];
+    } else {
+        const double f = f0(d, dmin, dhat, kappa);
+        const double wm = w * m;
+        for (int i = 0; i < nd; i++)
+            for (int j = 0; j < nd; j++) {
+                const double gfgm_ij = (w * fp) * gd[i] * gm[j];
+                const double gfgm_ji = (w * fp) * gd[j] * gm[i];
+                H[i * nd + j] = (w * f) * hm[i * nd + j] + gfgm_ij + gfgm_ji
+                    + (wm * fpp) * gd[i] * gd[j] + (wm * fp) * hdd[i * nd + j];
+            }
+    }
+    return ipco_project_to_psd(H, nd, psd);
+}
+
+int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
+                         const int32_t *E, const int32_t *F, double dhat,
+                         double kappa, int psd, ipco_csr *out)
+{
+    const int64_t n = n_collisions(c);
+    const int64_t ndof = 3 * (int64_t)nV;
+    memset(out, 0, sizeof(*out));
+    out->n = (int32_t)ndof;
+    double *loc = (double *)malloc(sizeof(double) * 144 * (size_t)(n > 0 ? n : 1));
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        if (local_hessian(&s, x, c->dmin, dhat, kappa, psd, loc + 144 * i) != 0)
+            rc = -1;
+    }
+    /* triplets, skipping exact zeros (local_to_global.hpp:289-301) */
+    int64_t nt = 0;
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        get_stencil(c, E, F, i, &s);
+        const int nd = 3 * s.nv;
+        for (int q = 0; q < nd * nd; q++) nt += loc[144 * i + q] != 0;
+    }
+    uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(nt > 0 ? nt : 1));
+    int64_t *src = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nt > 0 ? nt : 1));
+    int64_t t = 0;
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        get_stencil(c, E, F, i, &s);
+        const int nd = 3 * s.nv;
+        for (int a = 0; a < s.nv; a++)
+            for (int b = 0; b < s.nv; b++)
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) {
+                        const int q = (3 * a + k) * nd + 3 * b + l;
+                        if (loc[144 * i + q] == 0) continue;
+                        const uint64_t row = 3 * (uint64_t)s.ids[a] + k;
+                        const uint64_t col = 3 * (uint64_t)s.ids[b] + l;
+                        key[t] = row * (uint64_t)ndof + col;
+                        src[t++] = 144 * i + q;
+                    }
+    }
+    ipco_sort_u64_pairs(key, src, nt);
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < nt; i++) nnz += (i == 0 || key[i] != key[i - 1]);
+    out->nnz = nnz;
+    out->rowptr = (int64_t *)calloc((size_t)ndof + 1, sizeof(int64_t));
+    out->col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    out->val = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    int64_t k = -1;
+    for (int64_t i = 0; i < nt; i++) {
+        if (i == 0 || key[i] != key[i - 1]) {
+            k++;
+            const uint64_t row = key[i] / (uint64_t)ndof;
+            out->col[k] = (int32_t)(key[i] % (uint64_t)ndof);
+            out->val[k] = loc[src[i]];
+            out->rowptr[row + 1]++;
+        } else {
+            out->val[k] += loc[src[i]];
+        }
+    }
+    for (int64_t r = 0; r < ndof; r++) out->rowptr[r + 1] += out->rowptr[r];
+    free(key);
+    free(src);
+    free(loc);
+    return rc;
+}
+
+void ipco_csr_free(ipco_csr *m)
+{
+    free(m->rowptr);
+    free(m->col);
+    free(m->val);
+    memset(m, 0, sizeof(*m));
+}

@@ -1,0 +1,479 @@
+"""ctypes front end of the CPU restatement (oracle/libipc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check in
+__graft_entry__.py and by the cpu_baseline leg of bench.py, always as the
+checker or the timed CPU baseline, never as the product path.
+
+Positions are (nV, 3) float64 C-contiguous arrays (AoS), edges (nE, 2) and
+faces (nF, 3) int32.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libipc_oracle.so")
+_lib = None
+
+PSD = {"none": 0, "clamp": 1, "abs": 2}
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+class _Pairs(C.Structure):
+    _fields_ = [("n", C.c_int64), ("ij", C.POINTER(C.c_int32))]
+
+
+class _Collisions(C.Structure):
+    _fields_ = [
+        ("nvv", C.c_int64), ("nev", C.c_int64), ("nee", C.c_int64), ("nfv", C.c_int64),
+        ("vv", C.POINTER(C.c_int32)), ("vv_w", C.POINTER(C.c_double)),
+        ("ev", C.POINTER(C.c_int32)), ("ev_w", C.POINTER(C.c_double)),
+        ("ee", C.POINTER(C.c_int32)), ("ee_w", C.POINTER(C.c_double)),
+        ("ee_eps_x", C.POINTER(C.c_double)), ("ee_dtype", C.POINTER(C.c_int32)),
+        ("fv", C.POINTER(C.c_int32)), ("fv_w", C.POINTER(C.c_double)),
+        ("dmin", C.c_double),
+    ]
+
+
+class _CSR(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nnz", C.c_int64), ("rowptr", C.POINTER(C.c_int64)),
+                ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
+
+
+class CCDParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_long),
+                ("conservative_rescaling", C.c_double)]
+
+
+DEFAULT_CCD = CCDParams(1e-6, 10_000_000, 0.8)
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+_F = C.POINTER(C.c_float)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        for name in ("ipco_point_edge_distance_type",):
+            getattr(L, name).argtypes = [_D, _D, _D]
+            getattr(L, name).restype = C.c_int
+        for name in ("ipco_point_triangle_distance_type", "ipco_edge_edge_distance_type"):
+            getattr(L, name).argtypes = [_D, _D, _D, _D]
+            getattr(L, name).restype = C.c_int
+        L.ipco_point_point_distance.argtypes = [_D, _D]
+        L.ipco_point_line_distance.argtypes = [_D, _D, _D]
+        L.ipco_point_plane_distance.argtypes = [_D, _D, _D, _D]
+        L.ipco_line_line_distance.argtypes = [_D, _D, _D, _D]
+        L.ipco_point_edge_distance.argtypes = [_D, _D, _D, C.c_int]
+        L.ipco_point_triangle_distance.argtypes = [_D, _D, _D, _D, C.c_int]
+        L.ipco_edge_edge_distance.argtypes = [_D, _D, _D, _D, C.c_int]
+        L.ipco_stencil_distance_derivs.argtypes = [C.c_int, C.c_int, _D, _D, _D]
+        L.ipco_edge_edge_cross_squarednorm.argtypes = [_D, _D, _D, _D]
+        L.ipco_edge_edge_mollifier_threshold.argtypes = [_D, _D, _D, _D]
+        L.ipco_edge_edge_mollifier.argtypes = [_D, C.c_double, _D, _D]
+        for name in ("ipco_point_point_distance", "ipco_point_line_distance",
+                     "ipco_point_plane_distance", "ipco_line_line_distance",
+                     "ipco_point_edge_distance", "ipco_point_triangle_distance",
+                     "ipco_edge_edge_distance", "ipco_stencil_distance_derivs",
+                     "ipco_edge_edge_cross_squarednorm",
+                     "ipco_edge_edge_mollifier_threshold", "ipco_edge_edge_mollifier",
+                     "ipco_barrier", "ipco_barrier_first_derivative",
+                     "ipco_barrier_second_derivative"):
+            getattr(L, name).restype = C.c_double
+        for name in ("ipco_barrier", "ipco_barrier_first_derivative",
+                     "ipco_barrier_second_derivative"):
+            getattr(L, name).argtypes = [C.c_double, C.c_double]
+        L.ipco_project_to_psd.argtypes = [_D, C.c_int, C.c_int]
+        L.ipco_symmetric_eigenvalues.argtypes = [_D, C.c_int, _D]
+        L.ipco_faces_to_edges.argtypes = [_I, C.c_int, _I, C.c_int, _I]
+        L.ipco_build_boxes_f32.argtypes = [_D, _D, C.c_int, _I, C.c_int, _I, C.c_int,
+                                           C.c_double, _F, _F, _F]
+        L.ipco_candidates.argtypes = [_D, _D, C.c_int, _I, C.c_int, _I, C.c_int,
+                                      C.c_double, _I, C.POINTER(_Pairs), C.POINTER(_Pairs)]
+        L.ipco_pairs_free.argtypes = [C.POINTER(_Pairs)]
+        L.ipco_build_collisions.argtypes = [_D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I,
+                                            C.POINTER(_Pairs), C.POINTER(_Pairs),
+                                            C.c_double, C.c_double, C.POINTER(_Collisions)]
+        L.ipco_collisions_free.argtypes = [C.POINTER(_Collisions)]
+        L.ipco_barrier_energy.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
+                                          C.c_double, C.c_double]
+        L.ipco_barrier_energy.restype = C.c_double
+        L.ipco_barrier_gradient.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
+                                            C.c_double, C.c_double, _D]
+        L.ipco_barrier_hessian.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
+                                           C.c_double, C.c_double, C.c_int, C.POINTER(_CSR)]
+        L.ipco_csr_free.argtypes = [C.POINTER(_CSR)]
+        L.ipco_ticcd.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_long, C.c_int, _D, _D]
+        L.ipco_stencil_ccd.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double,
+                                       C.POINTER(CCDParams), _D]
+        L.ipco_candidates_stepsize.argtypes = [_D, _D, C.c_int, _I, _I, C.POINTER(_Pairs),
+                                               C.POINTER(_Pairs), C.c_double,
+                                               C.POINTER(CCDParams), C.c_int]
+        L.ipco_candidates_stepsize.restype = C.c_double
+        L.ipco_compute_collision_free_stepsize.argtypes = [
+            _D, _D, C.c_int, _I, C.c_int, _I, C.c_int, C.c_double, C.POINTER(CCDParams),
+            C.c_int]
+        L.ipco_compute_collision_free_stepsize.restype = C.c_double
+        L.ipco_is_step_collision_free.argtypes = [_D, _D, C.c_int, _I, C.c_int, _I, C.c_int,
+                                                  C.c_double, C.POINTER(CCDParams)]
+        L.ipco_candidates_toi.argtypes = [_D, _D, _I, _I, C.POINTER(_Pairs), C.c_int,
+                                          C.c_double, C.POINTER(CCDParams), _D]
+        L.ipco_set_num_threads.argtypes = [C.c_int]
+        L.ipco_get_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_I)
+
+
+def set_num_threads(n):
+    lib().ipco_set_num_threads(int(n))
+
+
+def get_max_threads():
+    return lib().ipco_get_max_threads()
+
+
+# ---- primitives ------------------------------------------------------------
+
+def point_edge_distance_type(p, e0, e1):
+    return lib().ipco_point_edge_distance_type(_d(p)[1], _d(e0)[1], _d(e1)[1])
+
+
+def point_triangle_distance_type(p, t0, t1, t2):
+    a = [_d(x) for x in (p, t0, t1, t2)]
+    return lib().ipco_point_triangle_distance_type(*[x[1] for x in a])
+
+
+def edge_edge_distance_type(ea0, ea1, eb0, eb1):
+    a = [_d(x) for x in (ea0, ea1, eb0, eb1)]
+    return lib().ipco_edge_edge_distance_type(*[x[1] for x in a])
+
+
+def point_point_distance(p0, p1):
+    return lib().ipco_point_point_distance(_d(p0)[1], _d(p1)[1])
+
+
+def point_line_distance(p, e0, e1):
+    return lib().ipco_point_line_distance(_d(p)[1], _d(e0)[1], _d(e1)[1])
+
+
+def point_plane_distance(p, t0, t1, t2):
+    a = [_d(x) for x in (p, t0, t1, t2)]
+    return lib().ipco_point_plane_distance(*[x[1] for x in a])
+
+
+def line_line_distance(ea0, ea1, eb0, eb1):
+    a = [_d(x) for x in (ea0, ea1, eb0, eb1)]
+    return lib().ipco_line_line_distance(*[x[1] for x in a])
+
+
+def point_edge_distance(p, e0, e1, dtype=-1):
+    return lib().ipco_point_edge_distance(_d(p)[1], _d(e0)[1], _d(e1)[1], dtype)
+
+
+def point_triangle_distance(p, t0, t1, t2, dtype=-1):
+    a = [_d(x) for x in (p, t0, t1, t2)]
+    return lib().ipco_point_triangle_distance(*[x[1] for x in a], dtype)
+
+
+def edge_edge_distance(ea0, ea1, eb0, eb1, dtype=-1):
+    a = [_d(x) for x in (ea0, ea1, eb0, eb1)]
+    return lib().ipco_edge_edge_distance(*[x[1] for x in a], dtype)
+
+
+STENCIL_NDOF = {0: 12, 1: 12, 2: 9, 3: 6}
+
+
+def stencil_distance_derivs(kind, x, dtype=-1):
+    """kind 0 point-triangle, 1 edge-edge, 2 point-edge, 3 point-point.
+    Returns (value, gradient, hessian)."""
+    n = STENCIL_NDOF[kind]
+    x, px = _d(np.asarray(x, dtype=np.float64).reshape(-1))
+    g = np.zeros(n)
+    H = np.zeros((n, n))
+    v = lib().ipco_stencil_distance_derivs(kind, dtype, px, g.ctypes.data_as(_D),
+                                           H.ctypes.data_as(_D))
+    return v, g, H
+
+
+def edge_edge_cross_squarednorm(ea0, ea1, eb0, eb1):
+    a = [_d(x) for x in (ea0, ea1, eb0, eb1)]
+    return lib().ipco_edge_edge_cross_squarednorm(*[x[1] for x in a])
+
+
+def edge_edge_mollifier_threshold(ea0, ea1, eb0, eb1):
+    a = [_d(x) for x in (ea0, ea1, eb0, eb1)]
+    return lib().ipco_edge_edge_mollifier_threshold(*[x[1] for x in a])
+
+
+def edge_edge_mollifier(x12, eps_x):
+    x, px = _d(np.asarray(x12, dtype=np.float64).reshape(12))
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    m = lib().ipco_edge_edge_mollifier(px, eps_x, g.ctypes.data_as(_D), H.ctypes.data_as(_D))
+    return m, g, H
+
+
+def barrier(d, dhat):
+    return lib().ipco_barrier(d, dhat)
+
+
+def barrier_first_derivative(d, dhat):
+    return lib().ipco_barrier_first_derivative(d, dhat)
+
+
+def barrier_second_derivative(d, dhat):
+    return lib().ipco_barrier_second_derivative(d, dhat)
+
+
+def project_to_psd(A, method="clamp"):
+    A = np.array(A, dtype=np.float64, order="C")
+    rc = lib().ipco_project_to_psd(A.ctypes.data_as(_D), A.shape[0], PSD[method])
+    if rc != 0:
+        raise RuntimeError("unable to project matrix onto positive semi-definite cone")
+    return A
+
+
+def symmetric_eigenvalues(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    lam = np.zeros(A.shape[0])
+    lib().ipco_symmetric_eigenvalues(A.ctypes.data_as(_D), A.shape[0], lam.ctypes.data_as(_D))
+    return lam
+
+
+def faces_to_edges(E, F):
+    E, pE = _i(E)
+    F, pF = _i(F)
+    F2E = np.zeros((len(F), 3), dtype=np.int32)
+    if lib().ipco_faces_to_edges(pE, len(E), pF, len(F), F2E.ctypes.data_as(_I)) != 0:
+        raise RuntimeError("Unable to find edge!")
+    return F2E
+
+
+# ---- broad phase -----------------------------------------------------------
+
+def boxes_f32(V0, E, F, inflation=0.0, V1=None):
+    V0, p0 = _d(V0)
+    p1 = None
+    if V1 is not None:
+        V1, p1 = _d(V1)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    vb = np.zeros((len(V0), 6), np.float32)
+    eb = np.zeros((len(E), 6), np.float32)
+    fb = np.zeros((len(F), 6), np.float32)
+    lib().ipco_build_boxes_f32(p0, p1, len(V0), pE, len(E), pF, len(F), inflation,
+                               vb.ctypes.data_as(_F), eb.ctypes.data_as(_F),
+                               fb.ctypes.data_as(_F))
+    return vb, eb, fb
+
+
+def _take_pairs(p):
+    n = p.n
+    out = np.ctypeslib.as_array(p.ij, shape=(n * 2,)).reshape(n, 2).copy() if n else \
+        np.zeros((0, 2), np.int32)
+    lib().ipco_pairs_free(C.byref(p))
+    return out
+
+
+def _pairs_struct(a):
+    a = np.ascontiguousarray(a, dtype=np.int32).reshape(-1, 2)
+    s = _Pairs(len(a), a.ctypes.data_as(_I))
+    return s, a
+
+
+def candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None):
+    """(ee, fv) candidate arrays, LBVH semantics: ee rows (query, other)."""
+    V0, p0 = _d(V0)
+    p1 = None
+    if V1 is not None:
+        V1, p1 = _d(V1)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    pg = None
+    if vertex_group is not None:
+        vertex_group, pg = _i(vertex_group)
+    ee, fv = _Pairs(), _Pairs()
+    lib().ipco_candidates(p0, p1, len(V0), pE, len(E), pF, len(F), inflation, pg,
+                          C.byref(ee), C.byref(fv))
+    return _take_pairs(ee), _take_pairs(fv)
+
+
+# ---- narrow phase + potentials ---------------------------------------------
+
+class Collisions:
+    """Collision set (canonical order) holding the C struct for potentials."""
+
+    def __init__(self, s, keep):
+        self._s = s
+        self._keep = keep
+
+        def arr(p, n, k=1, dt=np.int32):
+            if n == 0:
+                return np.zeros((0, k) if k > 1 else (0,), dt)
+            a = np.ctypeslib.as_array(p, shape=(n * k,)).copy()
+            return a.reshape(n, k) if k > 1 else a
+
+        self.vv = arr(s.vv, s.nvv, 2)
+        self.vv_w = arr(s.vv_w, s.nvv, 1, np.float64)
+        self.ev = arr(s.ev, s.nev, 2)
+        self.ev_w = arr(s.ev_w, s.nev, 1, np.float64)
+        self.ee = arr(s.ee, s.nee, 2)
+        self.ee_w = arr(s.ee_w, s.nee, 1, np.float64)
+        self.ee_eps_x = arr(s.ee_eps_x, s.nee, 1, np.float64)
+        self.ee_dtype = arr(s.ee_dtype, s.nee, 1)
+        self.fv = arr(s.fv, s.nfv, 2)
+        self.fv_w = arr(s.fv_w, s.nfv, 1, np.float64)
+
+    def __len__(self):
+        return len(self.vv) + len(self.ev) + len(self.ee) + len(self.fv)
+
+    def __del__(self):
+        try:
+            lib().ipco_collisions_free(C.byref(self._s))
+        except Exception:
+            pass
+
+
+def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None):
+    V, pV = _d(V)
+    if V_rest is None:
+        V_rest = V
+    V_rest, pR = _d(V_rest)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    F2E = faces_to_edges(E, F)
+    F2E, pF2E = _i(F2E)
+    ee_s, ee_a = _pairs_struct(ee_cand)
+    fv_s, fv_a = _pairs_struct(fv_cand)
+    out = _Collisions()
+    lib().ipco_build_collisions(pV, len(V), pR, pE, len(E), pF, len(F), pF2E,
+                                C.byref(ee_s), C.byref(fv_s), dhat, dmin, C.byref(out))
+    return Collisions(out, (E, F))
+
+
+def barrier_energy(coll, V, E, F, dhat, kappa):
+    V, pV = _d(V)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    return lib().ipco_barrier_energy(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa)
+
+
+def barrier_gradient(coll, V, E, F, dhat, kappa):
+    V, pV = _d(V)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    g = np.zeros(3 * len(V))
+    lib().ipco_barrier_gradient(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa,
+                                g.ctypes.data_as(_D))
+    return g
+
+
+def barrier_hessian(coll, V, E, F, dhat, kappa, psd="clamp"):
+    """scipy.sparse.csr_matrix (3nV x 3nV)."""
+    import scipy.sparse as sp
+    V, pV = _d(V)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    m = _CSR()
+    rc = lib().ipco_barrier_hessian(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa,
+                                    PSD[psd], C.byref(m))
+    n, nnz = m.n, m.nnz
+    rowptr = np.ctypeslib.as_array(m.rowptr, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(m.col, shape=(max(nnz, 1),))[:nnz].copy()
+    val = np.ctypeslib.as_array(m.val, shape=(max(nnz, 1),))[:nnz].copy()
+    lib().ipco_csr_free(C.byref(m))
+    if rc != 0:
+        raise RuntimeError("unable to project matrix onto positive semi-definite cone")
+    return sp.csr_matrix((val, col, rowptr), shape=(n, n))
+
+
+# ---- CCD -------------------------------------------------------------------
+
+def ticcd(is_vf, x_t0, x_t1, ms=0.0, tolerance=1e-6, t_max=1.0, max_itr=10_000_000,
+          no_zero_toi=False, err=-1.0):
+    x0, p0 = _d(np.asarray(x_t0, dtype=np.float64).reshape(12))
+    x1, p1 = _d(np.asarray(x_t1, dtype=np.float64).reshape(12))
+    toi = C.c_double(0)
+    otol = C.c_double(0)
+    hit = lib().ipco_ticcd(int(is_vf), p0, p1, err, ms, tolerance, t_max, max_itr,
+                           int(no_zero_toi), C.byref(toi), C.byref(otol))
+    return bool(hit), toi.value, otol.value
+
+
+def stencil_ccd(kind, x_t0, x_t1, min_distance=0.0, tmax=1.0, params=DEFAULT_CCD):
+    """kind 0 point-triangle, 1 edge-edge, 2 point-edge, 3 point-point.
+    Returns (hit, toi)."""
+    x0, p0 = _d(np.asarray(x_t0, dtype=np.float64).reshape(-1))
+    x1, p1 = _d(np.asarray(x_t1, dtype=np.float64).reshape(-1))
+    toi = C.c_double(np.inf)
+    hit = lib().ipco_stencil_ccd(kind, p0, p1, min_distance, tmax, C.byref(params),
+                                 C.byref(toi))
+    return bool(hit), toi.value
+
+
+def candidates_toi(V0, V1, E, F, cand, is_ee, min_distance=0.0, params=DEFAULT_CCD):
+    V0, p0 = _d(V0)
+    V1, p1 = _d(V1)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    s, a = _pairs_struct(cand)
+    out = np.zeros(len(a))
+    lib().ipco_candidates_toi(p0, p1, pE, pF, C.byref(s), int(is_ee), min_distance,
+                              C.byref(params), out.ctypes.data_as(_D))
+    return out
+
+
+def candidates_stepsize(V0, V1, E, F, ee, fv, min_distance=0.0, params=DEFAULT_CCD,
+                        running_tmax=False):
+    V0, p0 = _d(V0)
+    V1, p1 = _d(V1)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    es, ea = _pairs_struct(ee)
+    fs, fa = _pairs_struct(fv)
+    return lib().ipco_candidates_stepsize(p0, p1, len(V0), pE, pF, C.byref(es),
+                                          C.byref(fs), min_distance, C.byref(params),
+                                          int(running_tmax))
+
+
+def compute_collision_free_stepsize(V0, V1, E, F, min_distance=0.0, params=DEFAULT_CCD,
+                                    running_tmax=False):
+    V0, p0 = _d(V0)
+    V1, p1 = _d(V1)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    return lib().ipco_compute_collision_free_stepsize(p0, p1, len(V0), pE, len(E), pF,
+                                                      len(F), min_distance,
+                                                      C.byref(params), int(running_tmax))
+
+
+def is_step_collision_free(V0, V1, E, F, min_distance=0.0, params=DEFAULT_CCD):
+    V0, p0 = _d(V0)
+    V1, p1 = _d(V1)
+    E, pE = _i(E)
+    F, pF = _i(F)
+    return bool(lib().ipco_is_step_collision_free(p0, p1, len(V0), pE, len(E), pF, len(F),
+                                                  min_distance, C.byref(params)))
