@@ -1,0 +1,160 @@
+/*
+ * ipcg.h -- C ABI of the MI355X (gfx950) contact hot path of IPC Toolkit.
+ *
+ * One context owns one HIP stream and all device memory of one collision
+ * mesh: topology, positions at t0 (slot 0) and t1 (slot 1), broad-phase
+ * trees, candidate lists, the collision set, the gradient and the Hessian.
+ * Every call returns an ipcg_status (0 = success) and never throws; the last
+ * error message of a context is available from ipcg_last_error().  Calls on
+ * one context must not overlap; distinct contexts are independent.
+ *
+ * Layouts: positions are float64 with IPCG_LAYOUT_AOS (row i = x,y,z, a
+ * C-contiguous numpy (nV,3) array) or IPCG_LAYOUT_SOA (all x, then all y,
+ * then all z: the column-major Eigen::MatrixXd the reference takes).
+ * Edges (nE,2) and faces (nF,3) are int32 row-major.  Gradients are RowMajor
+ * [x0,y0,z0,x1,...] (reference config.hpp.in:45).  Hessians are CSR with
+ * int64 row pointers, int32 columns, float64 values; the matrix is symmetric,
+ * so CSR equals the reference's column-major CSC.
+ *
+ * Citations name the reference interface (IPC Toolkit v1.6.0) each entry
+ * point stands behind.
+ */
+#ifndef IPCG_H
+#define IPCG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ipcg_status {
+    IPCG_OK = 0,
+    IPCG_ERR_ARG = 1,       /* invalid argument (sizes, ids, layout)        */
+    IPCG_ERR_HIP = 2,       /* HIP runtime error                            */
+    IPCG_ERR_OOM = 3,       /* device allocation failed                     */
+    IPCG_ERR_STATE = 4,     /* call out of order (e.g. no mesh uploaded)     */
+    IPCG_ERR_NUMERIC = 5,   /* PSD eigensolver failed / CCD work overflow   */
+    IPCG_ERR_NO_DEVICE = 6  /* no HIP device                                */
+} ipcg_status;
+
+enum { IPCG_LAYOUT_AOS = 0, IPCG_LAYOUT_SOA = 1 };
+enum { IPCG_BP_LBVH = 0, IPCG_BP_SAP = 1 };
+/* utils/eigen_ext.hpp:202-206 PSDProjectionMethod */
+enum { IPCG_PSD_NONE = 0, IPCG_PSD_CLAMP = 1, IPCG_PSD_ABS = 2 };
+/* candidate / collision kinds */
+enum { IPCG_VV = 0, IPCG_EV = 1, IPCG_EE = 2, IPCG_FV = 3 };
+
+typedef struct ipcg_ctx ipcg_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int ipcg_ctx_create(int device, ipcg_ctx** out);
+int ipcg_ctx_destroy(ipcg_ctx* ctx);
+const char* ipcg_last_error(const ipcg_ctx* ctx);
+/* wait for all work of the context's stream */
+int ipcg_ctx_synchronize(ipcg_ctx* ctx);
+/* the context's hipStream_t (as void*), for callers that order their own
+ * work against the library's */
+void* ipcg_ctx_stream(ipcg_ctx* ctx);
+
+/* ---- collision mesh (collision_mesh.hpp:14-126, collision_filter.hpp) ---
+ * rest_positions: nV x 3 in `layout`; used for the edge-edge mollifier
+ * threshold (edge_edge_mollifier.cpp:194-202).  faces_to_edges is built
+ * on upload (collision_mesh.cpp:510-541; IPCG_ERR_ARG if a face edge is
+ * missing from `edges`).  vertex_patch may be NULL (CollisionMesh's default
+ * filter: every vertex pair may collide) or nV patch ids, giving
+ * can_collide(i,j) = patch[i] != patch[j] (make_vertex_patches_filter,
+ * collision_filter.hpp:113-118). */
+int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest_positions,
+                     int layout, int32_t nE, const int32_t* edges, int32_t nF,
+                     const int32_t* faces, const int32_t* vertex_patch);
+
+/* ---- positions ----------------------------------------------------------
+ * slot 0: the configuration x (static queries, potentials, CCD start);
+ * slot 1: the CCD end configuration.  _device takes a device pointer
+ * (same layout rules) and copies on the context stream. */
+int ipcg_positions_upload(ipcg_ctx* ctx, int slot, const double* V, int layout);
+int ipcg_positions_upload_device(ipcg_ctx* ctx, int slot, const double* dV,
+                                 int layout);
+
+/* ---- broad phase (broad_phase.hpp:19-133, lbvh.cpp, sweep_and_prune.cpp) -
+ * swept = 0: boxes of slot 0 inflated by `inflation`
+ *            (BroadPhase::build(vertices, ...), Candidates::build static);
+ * swept = 1: boxes of the segment slot0 -> slot1
+ *            (BroadPhase::build(vertices_t0, vertices_t1, ...)).
+ * method IPCG_BP_LBVH (default, lbvh.cpp) or IPCG_BP_SAP (sweep and prune).
+ * Produces the 3D candidate lists (broad_phase.cpp:79-91): edge-edge pairs,
+ * each unordered pair once as (lower Morton position, higher), and
+ * face-vertex pairs (face, vertex). */
+int ipcg_broad_phase(ipcg_ctx* ctx, int swept, double inflation, int method);
+int ipcg_candidates_count(ipcg_ctx* ctx, int64_t* n_ee, int64_t* n_fv);
+/* ee: 2*n_ee int32, fv: 2*n_fv int32 (either may be NULL) */
+int ipcg_candidates_download(ipcg_ctx* ctx, int32_t* ee, int32_t* fv);
+
+/* ---- collision set (normal_collisions.cpp:20-158) ------------------------
+ * build: static broad phase with inflation 0.5*(dhat+dmin) followed by the
+ * narrow phase (NormalCollisions::build(mesh, V, dhat, dmin, broad_phase)).
+ * build_from_candidates: narrow phase over the current static candidate
+ * lists (NormalCollisions::build(candidates, mesh, V, dhat, dmin)). */
+int ipcg_collisions_build(ipcg_ctx* ctx, double dhat, double dmin, int method);
+int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmin);
+/* counts[4] = {vv, ev, ee, fv} */
+int ipcg_collisions_count(ipcg_ctx* ctx, int64_t counts[4]);
+/* Canonical order: vv by (min,max); ev by (edge,vertex); ee by
+ * (min edge, max edge) with ids as oriented by the candidate; fv by
+ * (face,vertex).  Any pointer may be NULL. */
+int ipcg_collisions_download(ipcg_ctx* ctx, int32_t* vv, double* vv_w,
+                             int32_t* ev, double* ev_w, int32_t* ee,
+                             double* ee_w, double* ee_eps_x, int32_t* ee_dtype,
+                             int32_t* fv, double* fv_w);
+
+/* ---- barrier potential (potential.cpp:36-222, barrier_potential.cpp) ----
+ * Over the current collision set at slot-0 positions. */
+int ipcg_barrier_energy(ipcg_ctx* ctx, double dhat, double kappa, double* out);
+/* grad: host array of 3*nV doubles (RowMajor), or NULL to keep it on device */
+int ipcg_barrier_gradient(ipcg_ctx* ctx, double dhat, double kappa, double* grad);
+/* assembles the CSR on device; *nnz receives the entry count */
+int ipcg_barrier_hessian(ipcg_ctx* ctx, double dhat, double kappa, int psd,
+                         int64_t* nnz);
+/* rowptr: 3nV+1 int64, col: nnz int32, val: nnz float64 */
+int ipcg_hessian_download(ipcg_ctx* ctx, int64_t* rowptr, int32_t* col, double* val);
+/* device pointers of the last gradient / CSR (valid until the next call) */
+int ipcg_device_results(ipcg_ctx* ctx, const double** grad, const int64_t** rowptr,
+                        const int32_t** col, const double** val);
+
+/* ---- CCD (ipc.cpp:45-101, candidates.cpp:252-292, tight_inclusion_ccd) --
+ * compute_collision_free_stepsize: swept broad phase (inflation
+ * 0.5*min_distance) + Tight-Inclusion CCD of every candidate, slot 0 ->
+ * slot 1.  Every candidate is solved on [0,1] and outcomes at or above the
+ * running minimum are left unexplored, which yields the minimum of the
+ * per-candidate times of impact independently of scheduling. */
+int ipcg_compute_collision_free_stepsize(ipcg_ctx* ctx, double min_distance,
+                                         double tolerance, int64_t max_iterations,
+                                         double conservative_rescaling, int method,
+                                         double* step);
+/* same narrow phase over the current (swept) candidate lists */
+int ipcg_candidates_stepsize(ipcg_ctx* ctx, double min_distance, double tolerance,
+                             int64_t max_iterations, double conservative_rescaling,
+                             double* step);
+/* is_step_collision_free (ipc.cpp:20-41): 1 if no candidate reports a hit */
+int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double tolerance,
+                                int64_t max_iterations, double conservative_rescaling,
+                                int method, int* collision_free);
+/* per-candidate times of impact of the current swept lists, each on
+ * [0,1] with no bound (+inf when no hit); toi_ee: n_ee, toi_fv: n_fv */
+int ipcg_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
+                        int64_t max_iterations, double conservative_rescaling,
+                        double* toi_ee, double* toi_fv);
+
+/* ---- timing of the last call's kernels (for benchmarks) ----------------- */
+/* Kernel time of named stages recorded with HIP events on the context
+ * stream since the last reset; names are stage names, ms in milliseconds.
+ * Returns the number of stages (up to max). */
+int ipcg_stage_times(ipcg_ctx* ctx, int max, const char** names, double* ms,
+                     int64_t* calls);
+int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

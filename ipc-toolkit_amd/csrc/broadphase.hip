@@ -1,0 +1,501 @@
+// Broad phase on the GPU: conservative boxes, 63-bit Morton codes, LBVH
+// (Apetrei 2014 single bottom-up pass) and candidate traversal.
+//
+// Boxes follow aabb.cpp:35-126 (nextafter(p -/+ r) outward in FP64, unions
+// for edges and faces) and lbvh.cpp:29-41 (FP32 with one more outward ulp).
+// Morton codes follow lbvh.cpp:148-169 (FP64 box centre mapped by the mesh
+// box of the vertex boxes, morton.hpp:53-63) and are sorted stably, so a
+// pair's orientation (query = lower sorted position, lbvh.cpp:413-424) is
+// the one the CPU restatement produces.  The candidate set itself is every
+// leaf pair whose FP32 boxes overlap and passes the filters, independent of
+// the tree shape.
+//
+// Tree layout: internal node j (splitting sorted keys j | j+1) stores both
+// child boxes, child links and the children's rightmost leaf positions in
+// one 64-byte line, so a traversal step is one line load.
+#include "context.hpp"
+#include "geometry.hpp"
+#include "sort.hpp"
+
+namespace ipcg {
+
+namespace {
+
+constexpr int BP_BLOCK = 256;
+constexpr int TR_BLOCK = 128;
+constexpr int TR_STACK = 64; // lbvh.cpp:378 MAX_STACK_SIZE
+
+__device__ __forceinline__ d3 ldx(const double* X, int nV, int i)
+{
+    return { X[i], X[nV + i], X[2 * (int64_t)nV + i] };
+}
+
+// AABB::from_point (+ swept union), aabb.cpp:21-55
+__device__ __forceinline__ void vbox(const double* X0, const double* X1, int nV, int i, double r,
+                                     double* lo, double* hi)
+{
+    const d3 p = ldx(X0, nV, i);
+    lo[0] = nextafter(p.x - r, -INFINITY);
+    lo[1] = nextafter(p.y - r, -INFINITY);
+    lo[2] = nextafter(p.z - r, -INFINITY);
+    hi[0] = nextafter(p.x + r, INFINITY);
+    hi[1] = nextafter(p.y + r, INFINITY);
+    hi[2] = nextafter(p.z + r, INFINITY);
+    if (X1) {
+        const d3 q = ldx(X1, nV, i);
+        const double l1[3] = { nextafter(q.x - r, -INFINITY), nextafter(q.y - r, -INFINITY),
+                               nextafter(q.z - r, -INFINITY) };
+        const double h1[3] = { nextafter(q.x + r, INFINITY), nextafter(q.y + r, INFINITY),
+                               nextafter(q.z + r, INFINITY) };
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = l1[k] < lo[k] ? l1[k] : lo[k];
+            hi[k] = hi[k] < h1[k] ? h1[k] : hi[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(BP_BLOCK) void k_mesh_aabb_partial(const double* X0,
+                                                                const double* X1, int nV,
+                                                                double r, double* partial)
+{
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = blockIdx.x * BP_BLOCK + threadIdx.x; i < nV; i += gridDim.x * BP_BLOCK) {
+        double l[3], h[3];
+        vbox(X0, X1, nV, i, r, l, h);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], l[k]);
+            hi[k] = fmax(hi[k], h[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, WAVE));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, WAVE));
+        }
+    __shared__ double s[BP_BLOCK / WAVE][6];
+    const int w = threadIdx.x / WAVE;
+    if (threadIdx.x % WAVE == 0)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            s[w][k] = lo[k];
+            s[w][3 + k] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = s[0][threadIdx.x];
+        for (int j = 1; j < BP_BLOCK / WAVE; j++)
+            v = threadIdx.x < 3 ? fmin(v, s[j][threadIdx.x]) : fmax(v, s[j][threadIdx.x]);
+        partial[blockIdx.x * 6 + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_mesh_aabb_final(const double* partial, int nb, double* aabb)
+{
+    if (threadIdx.x < 6) {
+        double v = partial[threadIdx.x];
+        for (int j = 1; j < nb; j++)
+            v = threadIdx.x < 3 ? fmin(v, partial[j * 6 + threadIdx.x])
+                                : fmax(v, partial[j * 6 + threadIdx.x]);
+        aabb[threadIdx.x] = v;
+    }
+}
+
+__device__ __forceinline__ uint64_t expand_bits_2(uint64_t v)
+{
+    v = (v | v << 32) & 0x1F00000000FFFFull;
+    v = (v | v << 16) & 0x1F0000FF0000FFull;
+    v = (v | v << 8) & 0x100F00F00F00F00Full;
+    v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t morton3(double x, double y, double z)
+{
+    const double scale = 2097152.0, top = scale - 1;
+    x = x * scale;
+    y = y * scale;
+    z = z * scale;
+    x = x < 0.0 ? 0.0 : (top < x ? top : x);
+    y = y < 0.0 ? 0.0 : (top < y ? top : y);
+    z = z < 0.0 ? 0.0 : (top < z ? top : z);
+    return (expand_bits_2((uint64_t)x) << 2) | (expand_bits_2((uint64_t)y) << 1)
+        | expand_bits_2((uint64_t)z);
+}
+
+// per primitive (ARITY vertices): FP64 box, Morton code, FP32 box
+template <int ARITY>
+__global__ __launch_bounds__(BP_BLOCK) void k_prim_boxes(
+    const double* X0, const double* X1, int nV, const int32_t* I, int n, double r,
+    const double* aabb, float* box32, uint64_t* keys, uint32_t* ids)
+{
+    const int p = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    double lo[3], hi[3];
+    vbox(X0, X1, nV, ARITY == 1 ? p : I[(int64_t)ARITY * p], r, lo, hi);
+#pragma unroll
+    for (int j = 1; j < ARITY; j++) {
+        double l[3], h[3];
+        vbox(X0, X1, nV, I[(int64_t)ARITY * p + j], r, l, h);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = l[k] < lo[k] ? l[k] : lo[k];
+            hi[k] = hi[k] < h[k] ? h[k] : hi[k];
+        }
+    }
+    double m[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double c = 0.5 * (lo[k] + hi[k]);
+        m[k] = (c - aabb[k]) / (aabb[3 + k] - aabb[k]);
+    }
+    keys[p] = morton3(m[0], m[1], m[2]);
+    ids[p] = (uint32_t)p;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        box32[(int64_t)6 * p + k] = nextafterf(__double2float_rn(lo[k]), -INFINITY);
+        box32[(int64_t)6 * p + 3 + k] = nextafterf(__double2float_rn(hi[k]), INFINITY);
+    }
+}
+
+__device__ __forceinline__ int delta(const uint64_t* codes, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    const uint64_t ci = codes[i], cj = codes[j];
+    if (ci == cj) return 32 + __clz((unsigned)(i ^ j));
+    return __clzll(ci ^ cj);
+}
+
+struct BuildInfo {
+    int32_t visits, lrange, rrange, pad;
+};
+
+// Apetrei 2014 bottom-up build (lbvh.cpp:197-302)
+__global__ __launch_bounds__(BP_BLOCK) void k_lbvh_build(const uint64_t* codes,
+                                                         const uint32_t* ids, const float* box32,
+                                                         int n, Node* nodes, BuildInfo* info,
+                                                         int32_t* root)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], hi[3];
+    const uint32_t pid = ids[i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        lo[k] = box32[(int64_t)6 * pid + k];
+        hi[k] = box32[(int64_t)6 * pid + 3 + k];
+    }
+    if (n == 1) {
+        *root = -1;
+        return;
+    }
+    int lk = i, rk = i, cur = ~i;
+    for (int step = 0;; step++) {
+        if (step >= n) { // a walk is at most n - 1 steps; a longer one is a broken build
+            atomicOr(&info[0].pad, 1);
+            return;
+        }
+        const bool is_a = (lk == 0) || (rk != n - 1 && delta(codes, n, rk, rk + 1)
+                                                           > delta(codes, n, lk - 1, lk));
+        const int parent = is_a ? rk : lk - 1;
+        Node* P = nodes + parent;
+        if (is_a) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                P->lo0[k] = lo[k];
+                P->hi0[k] = hi[k];
+            }
+            P->c0 = cur;
+            P->r0 = rk;
+            info[parent].lrange = lk;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                P->lo1[k] = lo[k];
+                P->hi1[k] = hi[k];
+            }
+            P->c1 = cur;
+            P->r1 = rk;
+            info[parent].rrange = rk;
+        }
+        __threadfence();
+        if (atomicAdd(&info[parent].visits, 1) == 0) return;
+        __threadfence();
+        const volatile Node* VP = P;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float sl = is_a ? VP->lo1[k] : VP->lo0[k];
+            const float sh = is_a ? VP->hi1[k] : VP->hi0[k];
+            lo[k] = fminf(lo[k], sl);
+            hi[k] = fmaxf(hi[k], sh);
+        }
+        lk = ((volatile BuildInfo*)info)[parent].lrange;
+        rk = ((volatile BuildInfo*)info)[parent].rrange;
+        cur = parent;
+        if (lk == 0 && rk == n - 1) {
+            *root = parent;
+            return;
+        }
+    }
+}
+
+__device__ __forceinline__ bool overlap(const float* q, const float* lo, const float* hi)
+{
+    return q[0] <= hi[0] && q[1] <= hi[1] && q[2] <= hi[2] && lo[0] <= q[3] && lo[1] <= q[4]
+        && lo[2] <= q[5];
+}
+
+struct Filter {
+    const int32_t* E;
+    const int32_t* F;
+    const int32_t* patch; // may be null
+};
+
+__device__ __forceinline__ bool can_vv(const Filter& f, int a, int b)
+{
+    return f.patch == nullptr || f.patch[a] != f.patch[b];
+}
+
+// lbvh.cpp:826-849 can_edges_collide
+__device__ __forceinline__ bool can_ee(const Filter& f, int ea, int eb)
+{
+    const int a0 = f.E[2 * (int64_t)ea], a1 = f.E[2 * (int64_t)ea + 1];
+    const int b0 = f.E[2 * (int64_t)eb], b1 = f.E[2 * (int64_t)eb + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+    return can_vv(f, a0, b0) || can_vv(f, a0, b1) || can_vv(f, a1, b0) || can_vv(f, a1, b1);
+}
+
+// lbvh.cpp:851-858 can_face_vertex_collide
+__device__ __forceinline__ bool can_fv(const Filter& f, int fi, int vi)
+{
+    const int f0 = f.F[3 * (int64_t)fi], f1 = f.F[3 * (int64_t)fi + 1],
+              f2 = f.F[3 * (int64_t)fi + 2];
+    if (vi == f0 || vi == f1 || vi == f2) return false;
+    return can_vv(f, vi, f0) || can_vv(f, vi, f1) || can_vv(f, vi, f2);
+}
+
+// One lane per query leaf (in the query set's Morton order).  TRI: edge-edge
+// self traversal reporting (query, other) for other at a later sorted
+// position; otherwise vertex queries against the face tree reporting
+// (face, vertex).
+template <bool TRI>
+__global__ __launch_bounds__(TR_BLOCK) void k_traverse(
+    const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
+    const uint32_t* tids, const float* tbox, int nleaves, Filter flt, int2* out,
+    unsigned long long* counter, int64_t cap, int* status)
+{
+    __shared__ int stack[TR_STACK][TR_BLOCK];
+    const int t = threadIdx.x;
+    const int qpos = blockIdx.x * TR_BLOCK + t;
+    bool active = qpos < nq;
+    float q[6] = { 0, 0, 0, 0, 0, 0 };
+    int qprim = 0;
+    if (active) {
+        qprim = (int)qids[qpos];
+#pragma unroll
+        for (int k = 0; k < 6; k++) q[k] = qbox[(int64_t)6 * qprim + k];
+    }
+    const int root = *root_p;
+    const int nin = nleaves - 1; // internal nodes
+    if (root < -1 || root >= (nin > 0 ? nin : 1)) { // build did not finish: refuse to walk
+        if (t == 0) atomicOr(status, 2);
+        return;
+    }
+    if (root == -1) { // single target leaf (lbvh.cpp:386-396)
+        int hit = -1;
+        if (!TRI && active) {
+            const int fp = (int)tids[0];
+            const float* b = tbox + (int64_t)6 * fp;
+            if (overlap(q, b, b + 3) && can_fv(flt, fp, qprim)) hit = fp;
+        }
+        const unsigned long long slot = wave_append(counter, hit >= 0 ? 1u : 0u);
+        if (hit >= 0 && (int64_t)slot < cap) out[slot] = make_int2(hit, qprim);
+        return;
+    }
+    int node = root, sp = 0, steps = 0;
+    const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
+    while (__any(active)) {
+        if (active && ++steps > max_steps) {
+            atomicOr(status, 2);
+            active = false;
+        }
+        int h0 = -1, h1 = -1;
+        bool nd_ok = true;
+        if (active) {
+            const Node nd = nodes[node];
+            // links must name an internal node or a leaf position
+            if (nd.c0 >= nin || nd.c1 >= nin || nd.c0 < -nleaves || nd.c1 < -nleaves) {
+                atomicOr(status, 2);
+                active = false;
+                nd_ok = false;
+            }
+            bool o0 = nd_ok && overlap(q, nd.lo0, nd.hi0), o1 = nd_ok && overlap(q, nd.lo1, nd.hi1);
+            if (TRI) {
+                if (o0 && nd.r0 <= qpos) o0 = false;
+                if (o1 && nd.r1 <= qpos) o1 = false;
+            }
+            const bool l0 = nd.c0 < 0, l1 = nd.c1 < 0;
+            if (o0 && l0) {
+                const int j = (int)tids[~nd.c0];
+                if (TRI ? can_ee(flt, qprim, j) : can_fv(flt, j, qprim)) h0 = j;
+            }
+            if (o1 && l1) {
+                const int j = (int)tids[~nd.c1];
+                if (TRI ? can_ee(flt, qprim, j) : can_fv(flt, j, qprim)) h1 = j;
+            }
+            const bool t0 = o0 && !l0, t1 = o1 && !l1;
+            if (!nd_ok) {
+            } else if (!t0 && !t1) {
+                if (sp == 0) active = false;
+                else node = stack[--sp][t];
+            } else {
+                node = t0 ? nd.c0 : nd.c1;
+                if (t0 && t1) {
+                    if (sp < TR_STACK) stack[sp++][t] = nd.c1;
+                    else {
+                        atomicOr(status, 1);
+                        active = false;
+                    }
+                }
+            }
+        }
+        const unsigned cnt = (h0 >= 0 ? 1u : 0u) + (h1 >= 0 ? 1u : 0u);
+        unsigned long long slot = wave_append(counter, cnt);
+        if (h0 >= 0) {
+            if ((int64_t)slot < cap) out[slot] = TRI ? make_int2(qprim, h0) : make_int2(h0, qprim);
+            slot++;
+        }
+        if (h1 >= 0 && (int64_t)slot < cap)
+            out[slot] = TRI ? make_int2(qprim, h1) : make_int2(h1, qprim);
+    }
+}
+
+void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const double* X0,
+                const double* X1, double r, bool with_nodes)
+{
+    hipStream_t s = c.stream;
+    T.n = n;
+    if (n <= 0) return;
+    T.box.reserve(sizeof(float) * 6 * (size_t)n);
+    T.keys.reserve(sizeof(uint64_t) * (size_t)n);
+    T.keys2.reserve(sizeof(uint64_t) * (size_t)n);
+    T.ids.reserve(sizeof(uint32_t) * (size_t)n);
+    T.ids2.reserve(sizeof(uint32_t) * (size_t)n);
+    T.root.reserve(sizeof(int32_t));
+    const double* aabb = c.aabb.as<double>();
+    const unsigned g = grid_for(n, BP_BLOCK);
+    if (arity == 1)
+        hipLaunchKernelGGL(k_prim_boxes<1>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
+                           aabb, T.box.as<float>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+    else if (arity == 2)
+        hipLaunchKernelGGL(k_prim_boxes<2>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
+                           aabb, T.box.as<float>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_prim_boxes<3>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
+                           aabb, T.box.as<float>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+    IPCG_CHECK_LAUNCH();
+    c.sort_tmp.reserve(radix_tmp_bytes(n));
+    c.stage_begin("bp_sort");
+    radix_sort_pairs(T.keys.as<uint64_t>(), T.ids.as<uint32_t>(), T.keys2.as<uint64_t>(),
+                     T.ids2.as<uint32_t>(), n, 63, c.sort_tmp.p, s);
+    c.stage_end();
+    if (!with_nodes) return;
+    T.nodes.reserve(sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1));
+    T.info.reserve(sizeof(BuildInfo) * (size_t)(n > 1 ? n - 1 : 1));
+    IPCG_HIP(hipMemsetAsync(T.info.p, 0, sizeof(BuildInfo) * (size_t)(n > 1 ? n - 1 : 1), s));
+    IPCG_HIP(hipMemsetAsync(T.root.p, 0x80, sizeof(int32_t), s)); // "unfinished" sentinel
+    c.stage_begin("bp_build");
+    hipLaunchKernelGGL(k_lbvh_build, dim3(g), dim3(BP_BLOCK), 0, s, T.keys.as<uint64_t>(),
+                       T.ids.as<uint32_t>(), T.box.as<float>(), n, T.nodes.as<Node>(),
+                       T.info.as<BuildInfo>(), T.root.as<int32_t>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+}
+
+} // namespace
+
+void bp_build(Ctx& c, bool swept, double inflation, int method)
+{
+    c.require_mesh();
+    c.require_pos(0);
+    if (swept) c.require_pos(1);
+    if (method != IPCG_BP_LBVH) throw Error(IPCG_ERR_ARG, "unsupported broad phase method");
+    hipStream_t s = c.stream;
+    const double* X0 = c.X[0].as<double>();
+    const double* X1 = swept ? c.X[1].as<double>() : nullptr;
+    c.cand_valid = false;
+    c.n_ee = c.n_fv = 0;
+    if (c.nV == 0) {
+        c.cand_valid = true;
+        c.cand_swept = swept;
+        return;
+    }
+    c.stage_begin("bp_boxes");
+    // mesh box of the vertex boxes (broad_phase.cpp:93-125)
+    const int nb = (int)grid_for(c.nV, BP_BLOCK, 1024);
+    c.aabb_partial.reserve(sizeof(double) * 6 * nb);
+    c.aabb.reserve(sizeof(double) * 6);
+    hipLaunchKernelGGL(k_mesh_aabb_partial, dim3(nb), dim3(BP_BLOCK), 0, s, X0, X1, c.nV,
+                       inflation, c.aabb_partial.as<double>());
+    hipLaunchKernelGGL(k_mesh_aabb_final, dim3(1), dim3(64), 0, s, c.aabb_partial.as<double>(),
+                       nb, c.aabb.as<double>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    // vertex order (queries of the face tree), edge tree, face tree
+    build_tree(c, c.tv, nullptr, 1, c.nV, X0, X1, inflation, false);
+    build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, true);
+    build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true);
+
+    c.counters.reserve(sizeof(unsigned long long) * 4);
+    Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr };
+    unsigned long long* cnt = c.counters.as<unsigned long long>();
+    int* status = reinterpret_cast<int*>(cnt + 2);
+    unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (c.cap_ee == 0) c.cap_ee = std::max<int64_t>(1024, 16 * (int64_t)c.nE);
+        if (c.cap_fv == 0) c.cap_fv = std::max<int64_t>(1024, 8 * (int64_t)c.nV);
+        c.cand_ee.reserve(sizeof(int2) * (size_t)c.cap_ee);
+        c.cand_fv.reserve(sizeof(int2) * (size_t)c.cap_fv);
+        IPCG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 4, s));
+        c.stage_begin("bp_traverse");
+        if (c.nE >= 2)
+            hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0,
+                               s, c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
+                               c.te.nodes.as<Node>(), c.te.root.as<int32_t>(),
+                               c.te.ids.as<uint32_t>(), c.te.box.as<float>(), c.nE, flt,
+                               c.cand_ee.as<int2>(), cnt, c.cap_ee, status);
+        if (c.nF >= 1)
+            hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(c.nV, TR_BLOCK)), dim3(TR_BLOCK),
+                               0, s, c.tv.box.as<float>(), c.tv.ids.as<uint32_t>(), c.nV,
+                               c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(),
+                               c.tf.ids.as<uint32_t>(), c.tf.box.as<float>(), c.nF, flt,
+                               c.cand_fv.as<int2>(), cnt + 1, c.cap_fv, status);
+        IPCG_CHECK_LAUNCH();
+        c.stage_end();
+        IPCG_HIP(hipMemcpyAsync(h, cnt, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        if ((int)h[2] & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
+        if ((int)h[2] & 1)
+            throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
+        const int64_t ne = (int64_t)h[0], nf = (int64_t)h[1];
+        bool again = false;
+        if (ne > c.cap_ee) {
+            c.cap_ee = ne + ne / 8 + 1024;
+            again = true;
+        }
+        if (nf > c.cap_fv) {
+            c.cap_fv = nf + nf / 8 + 1024;
+            again = true;
+        }
+        c.n_ee = ne;
+        c.n_fv = nf;
+        if (!again) break;
+    }
+    c.cand_valid = true;
+    c.cand_swept = swept;
+}
+
+} // namespace ipcg

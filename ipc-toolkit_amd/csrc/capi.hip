@@ -1,0 +1,474 @@
+// C ABI (include/ipcg.h): context lifetime, uploads, and the status-code
+// wrappers around the stage entry points.  Nothing here throws across the
+// ABI: every entry point catches and records the message in the context.
+#include "context.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+using namespace ipcg;
+
+struct ipcg_ctx {
+    Ctx c;
+};
+
+namespace ipcg {
+
+void Ctx::require_mesh() const
+{
+    if (!has_mesh) throw Error(IPCG_ERR_STATE, "no collision mesh uploaded");
+}
+
+void Ctx::require_pos(int slot) const
+{
+    if (!has_pos[slot])
+        throw Error(IPCG_ERR_STATE, "positions slot " + std::to_string(slot) + " not uploaded");
+}
+
+void Ctx::stage_begin(const char* name)
+{
+    if (!timer.enabled) return;
+    hipEvent_t e;
+    IPCG_HIP(hipEventCreate(&e));
+    IPCG_HIP(hipEventRecord(e, stream));
+    open_stage.push_back({ e, name });
+}
+
+void Ctx::stage_end()
+{
+    if (!timer.enabled || open_stage.empty()) return;
+    hipEvent_t e;
+    IPCG_HIP(hipEventCreate(&e));
+    IPCG_HIP(hipEventRecord(e, stream));
+    auto st = open_stage.back();
+    open_stage.pop_back();
+    timer.pending.push_back({ st.first, e, st.second });
+}
+
+} // namespace ipcg
+
+namespace {
+
+template <class F> int guard(ipcg_ctx* ctx, F&& f)
+{
+    if (!ctx) return IPCG_ERR_ARG;
+    try {
+        f(ctx->c);
+        ctx->c.err.clear();
+        return IPCG_OK;
+    } catch (const Error& e) {
+        ctx->c.err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->c.err = e.what();
+        return IPCG_ERR_HIP;
+    }
+}
+
+__global__ void k_aos_to_soa(const double* in, double* out, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        out[i] = in[3 * (int64_t)i];
+        out[n + i] = in[3 * (int64_t)i + 1];
+        out[2 * (int64_t)n + i] = in[3 * (int64_t)i + 2];
+    }
+}
+
+__global__ void k_gather_pairs(const int2* in, int64_t n, int32_t* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        out[2 * i] = in[i].x;
+        out[2 * i + 1] = in[i].y;
+    }
+}
+
+void upload_positions(Ctx& c, DevBuf& dst, const double* src, int layout, bool device_src)
+{
+    const size_t bytes = sizeof(double) * 3 * (size_t)c.nV;
+    dst.reserve(bytes > 0 ? bytes : 8);
+    if (c.nV == 0) return;
+    const hipMemcpyKind kind = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (layout == IPCG_LAYOUT_SOA) {
+        IPCG_HIP(hipMemcpyAsync(dst.p, src, bytes, kind, c.stream));
+    } else if (layout == IPCG_LAYOUT_AOS) {
+        c.np_tmp.reserve(bytes);
+        IPCG_HIP(hipMemcpyAsync(c.np_tmp.p, src, bytes, kind, c.stream));
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(c.nV, 256)), dim3(256), 0, c.stream,
+                           c.np_tmp.as<double>(), dst.as<double>(), c.nV);
+        IPCG_CHECK_LAUNCH();
+    } else {
+        throw Error(IPCG_ERR_ARG, "unknown position layout");
+    }
+    if (!device_src) IPCG_HIP(hipStreamSynchronize(c.stream));
+}
+
+} // namespace
+
+extern "C" {
+
+int ipcg_ctx_create(int device, ipcg_ctx** out)
+{
+    if (!out) return IPCG_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return IPCG_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) return IPCG_ERR_ARG;
+    ipcg_ctx* ctx = new ipcg_ctx();
+    const int rc = guard(ctx, [&](Ctx& c) {
+        c.device = device;
+        IPCG_HIP(hipSetDevice(device));
+        IPCG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        IPCG_HIP(hipHostMalloc(&c.pinned, 4096, hipHostMallocDefault));
+        c.counters.reserve(sizeof(unsigned long long) * 16);
+    });
+    if (rc != IPCG_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return IPCG_OK;
+}
+
+int ipcg_ctx_destroy(ipcg_ctx* ctx)
+{
+    if (!ctx) return IPCG_ERR_ARG;
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    DevBuf* bufs[] = { &c.E, &c.F, &c.F2E, &c.patch, &c.rest, &c.X[0], &c.X[1],
+                       &c.tv.box, &c.tv.keys, &c.tv.keys2, &c.tv.ids, &c.tv.ids2, &c.tv.nodes, &c.tv.info, &c.tv.root,
+                       &c.te.box, &c.te.keys, &c.te.keys2, &c.te.ids, &c.te.ids2, &c.te.nodes, &c.te.info, &c.te.root,
+                       &c.tf.box, &c.tf.keys, &c.tf.keys2, &c.tf.ids, &c.tf.ids2, &c.tf.nodes, &c.tf.info, &c.tf.root,
+                       &c.aabb_partial, &c.aabb, &c.sort_tmp, &c.cand_ee, &c.cand_fv, &c.counters,
+                       &c.c_ids[0], &c.c_ids[1], &c.c_ids[2], &c.c_ids[3], &c.c_w[0], &c.c_w[1], &c.c_w[2], &c.c_w[3],
+                       &c.c_eps, &c.c_dtype, &c.np_keys, &c.np_keys2, &c.np_vals, &c.np_vals2, &c.np_ukeys,
+                       &c.np_start, &c.np_tmp, &c.np_raw, &c.red_partial, &c.red_out, &c.grad, &c.hq,
+                       &c.blk_keys, &c.blk_keys2, &c.blk_vals, &c.blk_vals2, &c.blk_ukeys, &c.blk_start,
+                       &c.blk_tmp, &c.blk_mask, &c.row_cnt, &c.rowptr, &c.col, &c.val, &c.ccd_bound,
+                       &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status };
+    for (DevBuf* b : bufs) b->release();
+    for (auto& p : c.timer.pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    if (c.pinned) (void)hipHostFree(c.pinned);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+    return IPCG_OK;
+}
+
+const char* ipcg_last_error(const ipcg_ctx* ctx)
+{
+    return ctx ? ctx->c.err.c_str() : "null context";
+}
+
+int ipcg_ctx_synchronize(ipcg_ctx* ctx)
+{
+    return guard(ctx, [&](Ctx& c) { IPCG_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+void* ipcg_ctx_stream(ipcg_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, int32_t nE,
+                     const int32_t* edges, int32_t nF, const int32_t* faces,
+                     const int32_t* vertex_patch)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (nV < 0 || nE < 0 || nF < 0 || (nV > 0 && !rest) || (nE > 0 && !edges)
+            || (nF > 0 && !faces))
+            throw Error(IPCG_ERR_ARG, "invalid mesh sizes or null arrays");
+        IPCG_HIP(hipSetDevice(c.device));
+        for (int64_t i = 0; i < 2 * (int64_t)nE; i++)
+            if (edges[i] < 0 || edges[i] >= nV) throw Error(IPCG_ERR_ARG, "edge vertex id out of range");
+        for (int64_t i = 0; i < 3 * (int64_t)nF; i++)
+            if (faces[i] < 0 || faces[i] >= nV) throw Error(IPCG_ERR_ARG, "face vertex id out of range");
+        // faces_to_edges (collision_mesh.cpp:510-541)
+        std::vector<std::pair<uint64_t, int32_t>> ek((size_t)nE);
+        for (int32_t e = 0; e < nE; e++) {
+            uint32_t a = (uint32_t)edges[2 * e], b = (uint32_t)edges[2 * e + 1];
+            if (a > b) std::swap(a, b);
+            ek[e] = { ((uint64_t)a << 32) | b, e };
+        }
+        std::sort(ek.begin(), ek.end());
+        std::vector<int32_t> f2e(3 * (size_t)nF);
+        for (int32_t f = 0; f < nF; f++)
+            for (int j = 0; j < 3; j++) {
+                uint32_t a = (uint32_t)faces[3 * f + j], b = (uint32_t)faces[3 * f + (j + 1) % 3];
+                if (a > b) std::swap(a, b);
+                const uint64_t k = ((uint64_t)a << 32) | b;
+                auto it = std::lower_bound(ek.begin(), ek.end(), std::make_pair(k, INT32_MIN));
+                if (it == ek.end() || it->first != k)
+                    throw Error(IPCG_ERR_ARG, "Unable to find edge!");
+                f2e[3 * (size_t)f + j] = it->second;
+            }
+        c.nV = nV;
+        c.nE = nE;
+        c.nF = nF;
+        c.E.reserve(sizeof(int32_t) * 2 * (size_t)std::max(nE, 1));
+        c.F.reserve(sizeof(int32_t) * 3 * (size_t)std::max(nF, 1));
+        c.F2E.reserve(sizeof(int32_t) * 3 * (size_t)std::max(nF, 1));
+        if (nE) IPCG_HIP(hipMemcpyAsync(c.E.p, edges, sizeof(int32_t) * 2 * nE, hipMemcpyHostToDevice, c.stream));
+        if (nF) IPCG_HIP(hipMemcpyAsync(c.F.p, faces, sizeof(int32_t) * 3 * nF, hipMemcpyHostToDevice, c.stream));
+        if (nF) IPCG_HIP(hipMemcpyAsync(c.F2E.p, f2e.data(), sizeof(int32_t) * 3 * nF, hipMemcpyHostToDevice, c.stream));
+        c.has_patch = vertex_patch != nullptr;
+        if (c.has_patch) {
+            c.patch.reserve(sizeof(int32_t) * (size_t)std::max(nV, 1));
+            if (nV) IPCG_HIP(hipMemcpyAsync(c.patch.p, vertex_patch, sizeof(int32_t) * nV, hipMemcpyHostToDevice, c.stream));
+        }
+        upload_positions(c, c.rest, rest, layout, false);
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+        c.has_mesh = true;
+        c.has_pos[0] = c.has_pos[1] = false;
+        c.cand_valid = c.coll_valid = c.hess_valid = false;
+        c.cap_ee = c.cap_fv = 0;
+    });
+}
+
+int ipcg_positions_upload(ipcg_ctx* ctx, int slot, const double* V, int layout)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.require_mesh();
+        if (slot < 0 || slot > 1 || (c.nV > 0 && !V)) throw Error(IPCG_ERR_ARG, "invalid slot or array");
+        upload_positions(c, c.X[slot], V, layout, false);
+        c.has_pos[slot] = true;
+    });
+}
+
+int ipcg_positions_upload_device(ipcg_ctx* ctx, int slot, const double* dV, int layout)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.require_mesh();
+        if (slot < 0 || slot > 1 || (c.nV > 0 && !dV)) throw Error(IPCG_ERR_ARG, "invalid slot or array");
+        upload_positions(c, c.X[slot], dV, layout, true);
+        c.has_pos[slot] = true;
+    });
+}
+
+int ipcg_broad_phase(ipcg_ctx* ctx, int swept, double inflation, int method)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(inflation >= 0)) throw Error(IPCG_ERR_ARG, "inflation radius must be >= 0");
+        bp_build(c, swept != 0, inflation, method);
+    });
+}
+
+int ipcg_candidates_count(ipcg_ctx* ctx, int64_t* n_ee, int64_t* n_fv)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.cand_valid) throw Error(IPCG_ERR_STATE, "no candidate set");
+        if (n_ee) *n_ee = c.n_ee;
+        if (n_fv) *n_fv = c.n_fv;
+    });
+}
+
+int ipcg_candidates_download(ipcg_ctx* ctx, int32_t* ee, int32_t* fv)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.cand_valid) throw Error(IPCG_ERR_STATE, "no candidate set");
+        if (ee && c.n_ee)
+            IPCG_HIP(hipMemcpyAsync(ee, c.cand_ee.p, sizeof(int2) * c.n_ee, hipMemcpyDeviceToHost, c.stream));
+        if (fv && c.n_fv)
+            IPCG_HIP(hipMemcpyAsync(fv, c.cand_fv.p, sizeof(int2) * c.n_fv, hipMemcpyDeviceToHost, c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_collisions_build(ipcg_ctx* ctx, double dhat, double dmin, int method)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(dhat > 0) || !(dmin >= 0)) throw Error(IPCG_ERR_ARG, "dhat must be > 0 and dmin >= 0");
+        bp_build(c, false, 0.5 * (dhat + dmin), method);
+        np_build(c, dhat, dmin);
+    });
+}
+
+int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmin)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(dhat > 0) || !(dmin >= 0)) throw Error(IPCG_ERR_ARG, "dhat must be > 0 and dmin >= 0");
+        np_build(c, dhat, dmin);
+    });
+}
+
+int ipcg_collisions_count(ipcg_ctx* ctx, int64_t counts[4])
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.coll_valid) throw Error(IPCG_ERR_STATE, "no collision set");
+        for (int k = 0; k < 4; k++) counts[k] = c.ncoll[k];
+    });
+}
+
+int ipcg_collisions_download(ipcg_ctx* ctx, int32_t* vv, double* vv_w, int32_t* ev, double* ev_w,
+                             int32_t* ee, double* ee_w, double* ee_eps_x, int32_t* ee_dtype,
+                             int32_t* fv, double* fv_w)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.coll_valid) throw Error(IPCG_ERR_STATE, "no collision set");
+        int32_t* ids[4] = { vv, ev, ee, fv };
+        double* w[4] = { vv_w, ev_w, ee_w, fv_w };
+        for (int k = 0; k < 4; k++) {
+            const int64_t n = c.ncoll[k];
+            if (!n) continue;
+            if (ids[k]) IPCG_HIP(hipMemcpyAsync(ids[k], c.c_ids[k].p, sizeof(int2) * n, hipMemcpyDeviceToHost, c.stream));
+            if (w[k]) IPCG_HIP(hipMemcpyAsync(w[k], c.c_w[k].p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        }
+        if (c.ncoll[2]) {
+            if (ee_eps_x) IPCG_HIP(hipMemcpyAsync(ee_eps_x, c.c_eps.p, sizeof(double) * c.ncoll[2], hipMemcpyDeviceToHost, c.stream));
+            if (ee_dtype) IPCG_HIP(hipMemcpyAsync(ee_dtype, c.c_dtype.p, sizeof(int32_t) * c.ncoll[2], hipMemcpyDeviceToHost, c.stream));
+        }
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_barrier_energy(ipcg_ctx* ctx, double dhat, double kappa, double* out)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!out) throw Error(IPCG_ERR_ARG, "null output");
+        *out = pot_energy(c, dhat, kappa);
+    });
+}
+
+int ipcg_barrier_gradient(ipcg_ctx* ctx, double dhat, double kappa, double* grad)
+{
+    return guard(ctx, [&](Ctx& c) {
+        pot_gradient(c, dhat, kappa);
+        if (grad && c.nV) {
+            IPCG_HIP(hipMemcpyAsync(grad, c.grad.p, sizeof(double) * 3 * c.nV, hipMemcpyDeviceToHost, c.stream));
+            IPCG_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int ipcg_barrier_hessian(ipcg_ctx* ctx, double dhat, double kappa, int psd, int64_t* nnz)
+{
+    return guard(ctx, [&](Ctx& c) {
+        pot_hessian(c, dhat, kappa, psd);
+        if (nnz) *nnz = c.nnz;
+    });
+}
+
+int ipcg_hessian_download(ipcg_ctx* ctx, int64_t* rowptr, int32_t* col, double* val)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.hess_valid) throw Error(IPCG_ERR_STATE, "no Hessian assembled");
+        if (rowptr) IPCG_HIP(hipMemcpyAsync(rowptr, c.rowptr.p, sizeof(int64_t) * (3 * (size_t)c.nV + 1), hipMemcpyDeviceToHost, c.stream));
+        if (col && c.nnz) IPCG_HIP(hipMemcpyAsync(col, c.col.p, sizeof(int32_t) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+        if (val && c.nnz) IPCG_HIP(hipMemcpyAsync(val, c.val.p, sizeof(double) * c.nnz, hipMemcpyDeviceToHost, c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_device_results(ipcg_ctx* ctx, const double** grad, const int64_t** rowptr,
+                        const int32_t** col, const double** val)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (grad) *grad = c.grad.as<double>();
+        if (rowptr) *rowptr = c.rowptr.as<int64_t>();
+        if (col) *col = c.col.as<int32_t>();
+        if (val) *val = c.val.as<double>();
+    });
+}
+
+static void check_ccd_args(double min_distance, double tol, double rescale)
+{
+    if (!(min_distance >= 0) || !(tol > 0) || !(rescale > 0 && rescale <= 1))
+        throw Error(IPCG_ERR_ARG, "invalid CCD parameters");
+}
+
+int ipcg_compute_collision_free_stepsize(ipcg_ctx* ctx, double min_distance, double tolerance,
+                                         int64_t max_iterations, double rescale, int method,
+                                         double* step)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!step) throw Error(IPCG_ERR_ARG, "null output");
+        bp_build(c, true, 0.5 * min_distance, method);
+        *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr, nullptr);
+    });
+}
+
+int ipcg_candidates_stepsize(ipcg_ctx* ctx, double min_distance, double tolerance,
+                             int64_t max_iterations, double rescale, double* step)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!step) throw Error(IPCG_ERR_ARG, "null output");
+        *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr, nullptr);
+    });
+}
+
+int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double tolerance,
+                                int64_t max_iterations, double rescale, int method,
+                                int* collision_free)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!collision_free) throw Error(IPCG_ERR_ARG, "null output");
+        bp_build(c, true, 0.5 * min_distance, method);
+        const double r = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, true, nullptr, nullptr);
+        *collision_free = r == 1.0 ? 1 : 0;
+    });
+}
+
+int ipcg_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
+                        int64_t max_iterations, double rescale, double* toi_ee, double* toi_fv)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!toi_ee && !toi_fv) throw Error(IPCG_ERR_ARG, "null outputs");
+        ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, toi_ee, toi_fv);
+    });
+}
+
+int ipcg_stage_times(ipcg_ctx* ctx, int max, const char** names, double* ms, int64_t* calls)
+{
+    if (!ctx) return -IPCG_ERR_ARG;
+    Ctx& c = ctx->c;
+    try {
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+        for (auto& p : c.timer.pending) {
+            float t = 0;
+            IPCG_HIP(hipEventElapsedTime(&t, p.a, p.b));
+            auto& a = c.timer.acc[p.name];
+            a.first += t;
+            a.second += 1;
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        c.timer.pending.clear();
+    } catch (const Error& e) {
+        c.err = e.what();
+        return -e.code;
+    }
+    int i = 0;
+    for (auto& kv : c.timer.acc) {
+        if (i >= max) break;
+        if (names) names[i] = kv.first.c_str();
+        if (ms) ms[i] = kv.second.first;
+        if (calls) calls[i] = kv.second.second;
+        i++;
+    }
+    return i;
+}
+
+int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable)
+{
+    return guard(ctx, [&](Ctx& c) {
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+        for (auto& p : c.timer.pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        c.timer.pending.clear();
+        c.timer.acc.clear();
+        c.timer.enabled = enable != 0;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,690 @@
+// Continuous collision detection over the swept candidate lists:
+// Tight-Inclusion per candidate inside the IPC Toolkit strategy
+// (tight_inclusion_ccd.cpp:33-336) and the step-size minimum
+// (candidates.cpp:252-292).
+//
+// Two paths share one device restatement of the algorithm (ti.hpp):
+//  * lane path: persistent lanes pull candidates from an atomic counter and
+//    run the whole breadth-first search in a private 3 x CAP box slab.  Most
+//    candidates are rejected at the root box or resolve in a few levels.
+//  * wavefront path, for candidates whose levels outgrow the slab: all of
+//    them advance one level per round of short kernels.  Every box of the
+//    level is evaluated by its own lane (the inclusion test is the work),
+//    then one lane per candidate replays the sequential per-level rules over
+//    its boxes in order, then children are emitted through a scan and each
+//    candidate's next level is ordered (merge of the lower / upper t halves).
+// The running minimum lives in device memory as the bit pattern of a
+// non-negative double (unsigned atomic min); every search hides outcomes at
+// or above it, which leaves the final minimum independent of scheduling.
+#include "context.hpp"
+#include "sort.hpp"
+#include "ti.hpp"
+
+#include <algorithm>
+
+namespace ipcg {
+
+namespace {
+
+constexpr int CB = 64;
+constexpr int LANE_CAP = 256;
+constexpr int LANES = 32768;
+constexpr int WF_BATCH = 4096;
+constexpr int64_t WF_MAX_BOXES = 1ll << 26;
+constexpr double SMALL_TOI = 1e-6; // tight_inclusion_ccd.hpp:14
+
+__device__ __forceinline__ d3 ldx(const double* X, int nV, int i)
+{
+    return { X[i], X[nV + i], X[2 * (int64_t)nV + i] };
+}
+
+struct CCDArgs {
+    const double* X0;
+    const double* X1;
+    int nV;
+    const int32_t* E;
+    const int32_t* F;
+    const int2* ee;
+    int64_t n_ee;
+    const int2* fv;
+    int64_t n_fv;
+    double min_distance, tolerance, rescale;
+    long max_iter;
+    bool bounded;          // hide outcomes >= running minimum
+    double* bound;         // running minimum (double bits)
+    double* toi_out;       // per-candidate toi (null: not recorded)
+    int* any_hit;          // early exit flag (null: no early exit)
+};
+
+__device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
+{
+    atomicMin(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ void report(const CCDArgs& A, int64_t ci, bool hit, double toi)
+{
+    if (hit) {
+        if (A.bounded) atomic_min_nonneg(A.bound, toi);
+        if (A.any_hit) atomicOr(A.any_hit, 1);
+    }
+    if (A.toi_out) A.toi_out[ci] = hit ? toi : INFINITY;
+}
+
+// Strategy prologue (ccd_strategy + *_ccd of tight_inclusion_ccd.cpp):
+// gathers the stencil, returns 1 (hit at toi = 0), 0 (miss) or -1 (search
+// needed, q / eff / adj set).
+__device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double& eff, double& adj)
+{
+    int ids[4];
+    const bool is_ee = ci < A.n_ee;
+    if (is_ee) {
+        const int2 c = A.ee[ci];
+        ids[0] = A.E[2 * (int64_t)c.x];
+        ids[1] = A.E[2 * (int64_t)c.x + 1];
+        ids[2] = A.E[2 * (int64_t)c.y];
+        ids[3] = A.E[2 * (int64_t)c.y + 1];
+    } else {
+        const int2 c = A.fv[ci - A.n_ee];
+        ids[0] = c.y;
+        ids[1] = A.F[3 * (int64_t)c.x];
+        ids[2] = A.F[3 * (int64_t)c.x + 1];
+        ids[3] = A.F[3 * (int64_t)c.x + 2];
+    }
+    d3 p0[4];
+    bool moving = false;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        p0[j] = ldx(A.X0, A.nV, ids[j]);
+        const d3 p1 = ldx(A.X1, A.nV, ids[j]);
+        q.s[3 * j] = p0[j].x;
+        q.s[3 * j + 1] = p0[j].y;
+        q.s[3 * j + 2] = p0[j].z;
+        q.e[3 * j] = p1.x;
+        q.e[3 * j + 1] = p1.y;
+        q.e[3 * j + 2] = p1.z;
+        moving |= p1.x != p0[j].x || p1.y != p0[j].y || p1.z != p0[j].z;
+    }
+    q.vf = !is_ee;
+    const double d2 = is_ee
+        ? ee_dist2(p0[0], p0[1], p0[2], p0[3], edge_edge_type(p0[0], p0[1], p0[2], p0[3]))
+        : pt_dist2(p0[0], p0[1], p0[2], p0[3], point_triangle_type(p0[0], p0[1], p0[2], p0[3]));
+    const double d0 = sqrt(d2);
+    const double dmin = A.min_distance;
+    if (!(d0 > dmin)) return 1; // check_initial_distance.hpp:7-21 (also without motion)
+    if (!moving) return 0;
+    adj = 0.5 * d0 < A.tolerance ? 0.5 * d0 : A.tolerance;
+    eff = (1.0 - A.rescale) * (d0 - dmin);
+    eff = eff < 1e-4 ? eff : 1e-4;
+    eff += dmin;
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// lane path
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, TIBox* slab,
+                                                  unsigned long long* work, uint32_t* heavy,
+                                                  unsigned long long* n_heavy)
+{
+    const int64_t lane = (int64_t)blockIdx.x * CB + threadIdx.x;
+    TISlab S;
+    S.cap = LANE_CAP;
+    S.cur = slab + lane * 3 * (int64_t)LANE_CAP;
+    S.lo = S.cur + LANE_CAP;
+    S.hi = S.lo + LANE_CAP;
+    const int64_t n = A.n_ee + A.n_fv;
+    for (;;) {
+        if (A.any_hit && __hip_atomic_load(A.any_hit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+        const int64_t ci = (int64_t)atomicAdd(work, 1ull);
+        if (ci >= n) break;
+        TIQuery q;
+        double eff = 0, adj = 0, toi = INFINITY;
+        const int pre = candidate_setup(A, ci, q, eff, adj);
+        if (pre >= 0) {
+            report(A, ci, pre == 1, 0.0);
+            continue;
+        }
+        TIBound B1 { A.bounded ? A.bound : nullptr, 1.0, SMALL_TOI };
+        int r = ti_query(q, eff, adj, A.max_iter, false, B1, S, toi);
+        if (r == TI_HIT && toi < SMALL_TOI) {
+            TIBound B2 { A.bounded ? A.bound : nullptr, (1.0 / A.rescale) * (1 + 1e-12), 0.0 };
+            r = ti_query(q, A.min_distance, adj, -1, true, B2, S, toi);
+            if (r == TI_HIT) toi *= A.rescale;
+        }
+        if (r == TI_OVERFLOW) {
+            heavy[atomicAdd(n_heavy, 1ull)] = (uint32_t)ci;
+            continue;
+        }
+        report(A, ci, r == TI_HIT, toi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// wavefront path
+// ---------------------------------------------------------------------------
+struct WFBox {
+    double t, u, v;
+    int32_t owner, pad;
+};
+
+struct WFState {
+    TIQuery q;
+    int64_t ci;
+    double ms, delta, adj, eff;
+    double wt, wu, wv;
+    double toi_skip, temp_toi, temp_out_tol, result;
+    int64_t refine;
+    long max_itr;
+    int stage, nz_iter, levels, axis;
+    int use_skip, done, hit, restart;
+    int64_t seg_start, seg_count;   // boxes of the current level
+    int64_t next_start, next_count; // boxes of the next level
+};
+
+enum : uint8_t { F_ZERO = 1, F_BOXIN = 2, F_TOL = 4, F_SKIP = 8 };
+
+__device__ void wf_begin_query(WFState& S, double ms, double delta, long max_itr)
+{
+    S.ms = ms;
+    S.delta = delta;
+    S.max_itr = max_itr;
+    ti_param_tol(S.q, delta);
+    ti_filter(S.q, ms > 0);
+    S.wt = S.wu = S.wv = 1.0;
+    S.toi_skip = 4;
+    S.temp_toi = INFINITY;
+    S.temp_out_tol = delta;
+    S.use_skip = 0;
+    S.refine = 0;
+    S.levels = 0;
+    S.restart = 1;
+}
+
+__global__ void k_wf_init(CCDArgs A, const uint32_t* heavy, int64_t first, int64_t nb,
+                          WFState* st, WFBox* boxes)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    WFState S;
+    S.ci = heavy[first + i];
+    double eff = 0, adj = 0;
+    const int pre = candidate_setup(A, S.ci, S.q, eff, adj);
+    S.done = 0;
+    S.hit = 0;
+    S.result = INFINITY;
+    S.eff = eff;
+    S.adj = adj;
+    S.stage = 1;
+    S.nz_iter = 0;
+    S.axis = 0;
+    S.seg_start = i;
+    S.seg_count = 1;
+    S.next_start = S.next_count = 0;
+    if (pre >= 0) { // resolved by the prologue (cannot happen for lane overflows)
+        S.done = 1;
+        S.hit = pre == 1;
+        S.result = 0.0;
+        S.restart = 0;
+    } else {
+        wf_begin_query(S, eff, adj, A.max_iter);
+        S.restart = 0;
+    }
+    boxes[i] = { 0.0, 0.0, 0.0, (int32_t)i, 0 };
+    st[i] = S;
+}
+
+__device__ __forceinline__ double wf_bound(const CCDArgs& A, const WFState& S)
+{
+    if (!A.bounded) return INFINITY;
+    const double g = __hip_atomic_load(A.bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (S.stage == 1) return g > SMALL_TOI ? g : SMALL_TOI;
+    return g * ((1.0 / A.rescale) * (1 + 1e-12));
+}
+
+__global__ void k_wf_eval(CCDArgs A, const WFState* st, const WFBox* boxes, int64_t n,
+                          uint8_t* flags, uint8_t* cmask)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    cmask[j] = 0;
+    const WFBox b = boxes[j];
+    const WFState& S = st[b.owner];
+    uint8_t f = 0;
+    if (S.done || b.t >= wf_bound(A, S)) {
+        f = F_SKIP;
+    } else {
+        bool box_in;
+        double tw[3];
+        const TIBox tb { b.t, b.u, b.v };
+        if (ti_inclusion(S.q, tb, S.wt, S.wu, S.wv, S.ms, box_in, tw)) {
+            f = F_ZERO;
+            if (box_in) f |= F_BOXIN;
+            if (tw[0] <= S.delta && tw[1] <= S.delta && tw[2] <= S.delta) f |= F_TOL;
+            // max_itr bookkeeping needs the widest inclusion; it is recomputed
+            // by the deciding lane only for the first zero-in box of a level
+        }
+    }
+    flags[j] = f;
+}
+
+// Ends the current search of a candidate with (hit, toi) and applies the
+// strategy: retry with no_zero_toi, its halvings, or completion.
+__device__ void wf_search_done(const CCDArgs& A, WFState& S, bool hit, double toi, double out_tol)
+{
+    if (S.stage == 1) {
+        if (hit && toi < SMALL_TOI) {
+            S.stage = 2;
+            S.nz_iter = 0;
+            wf_begin_query(S, A.min_distance, S.adj, -1);
+            return;
+        }
+        S.done = 1;
+        S.hit = hit;
+        S.result = toi;
+        return;
+    }
+    S.nz_iter++;
+    if (hit && toi == 0 && S.nz_iter < TI_MAX_NO_ZERO_TOI_ITER && !(out_tol > S.delta)) {
+        double ms = S.ms, delta = S.delta;
+        if (10 * delta < ms) ms *= 0.5;
+        else delta *= 0.5;
+        wf_begin_query(S, ms, delta, -1);
+        return;
+    }
+    S.done = 1;
+    S.hit = hit;
+    S.result = hit ? toi * A.rescale : toi;
+}
+
+// one lane per candidate: replay ti_bfs's per-level rules over the level
+__global__ void k_wf_decide(CCDArgs A, WFState* st, int64_t nb, const WFBox* boxes,
+                            const uint8_t* flags, uint8_t* cmask, int* error)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    WFState S = st[i];
+    S.restart = 0;
+    const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
+    if (S.done) {
+        for (int64_t j = s0; j < s1; j++) cmask[j] = 0;
+        st[i] = S;
+        return;
+    }
+    if (++S.levels > TI_MAX_LEVELS) {
+        atomicOr(error, 1);
+        S.done = 1;
+        for (int64_t j = s0; j < s1; j++) cmask[j] = 0;
+        st[i] = S;
+        return;
+    }
+    const double bound = wf_bound(A, S);
+    const TIQuery& q = S.q;
+    const bool c1 = S.wt <= q.tol[0] && S.wu <= q.tol[1] && S.wv <= q.tol[2];
+    int axis = 0;
+    {
+        double best = -INFINITY;
+        const double w3[3] = { S.wt, S.wu, S.wv };
+        for (int k = 0; k < 3; k++) {
+            const double r = w3[k] > q.tol[k] ? w3[k] / q.tol[k] : -INFINITY;
+            if (r > best) {
+                best = r;
+                axis = k;
+            }
+        }
+    }
+    bool level_less_tol = true, level_root = false;
+    bool finished = false, fhit = false;
+    double ftoi = INFINITY, fout = S.delta;
+    int64_t nchild = 0;
+    int64_t j = s0;
+    for (; j < s1; j++) {
+        const uint8_t f = flags[j];
+        const WFBox b = boxes[j];
+        cmask[j] = 0;
+        if ((f & F_SKIP) || b.t >= S.toi_skip || b.t >= bound) continue;
+        S.refine++;
+        if (!(f & F_ZERO)) continue;
+        const bool tol_cond = f & F_TOL;
+        const bool box_in = f & F_BOXIN;
+        const bool c2 = box_in && level_less_tol;
+        if (!tol_cond) level_less_tol = false;
+        if (c1 || c2 || level_less_tol) {
+            finished = true;
+            fhit = true;
+            ftoi = b.t;
+            break;
+        }
+        if (S.max_itr > 0) {
+            if (!level_root) {
+                bool bi;
+                double tw[3];
+                const TIBox tb { b.t, b.u, b.v };
+                ti_inclusion(q, tb, S.wt, S.wu, S.wv, S.ms, bi, tw);
+                double m = S.delta;
+                for (int k = 0; k < 3; k++) m = tw[k] > m ? tw[k] : m;
+                S.temp_toi = b.t;
+                S.temp_out_tol = m;
+                level_root = true;
+            }
+            if (S.refine > S.max_itr) {
+                finished = true;
+                fhit = true;
+                ftoi = S.temp_toi;
+                fout = S.temp_out_tol;
+                break;
+            }
+        }
+        if (tol_cond || box_in) {
+            S.toi_skip = b.t < S.toi_skip ? b.t : S.toi_skip;
+            S.use_skip = 1;
+            continue;
+        }
+        uint8_t m = 0;
+        if (axis == 0) {
+            m = 3;
+        } else {
+            const double hu = axis == 1 ? b.u + 0.5 * S.wu : b.u;
+            const double hv = axis == 2 ? b.v + 0.5 * S.wv : b.v;
+            if (!q.vf || hu + hv <= 1) m |= 1; // upper half
+            if (!q.vf || b.u + b.v <= 1) m |= 2; // lower half
+        }
+        cmask[j] = m;
+        nchild += __popc(m);
+    }
+    for (j = j + 1; j < s1; j++) cmask[j] = 0;
+    if (finished) {
+        for (int64_t k = s0; k < s1; k++) cmask[k] = 0;
+        wf_search_done(A, S, fhit, ftoi, fout);
+    } else if (nchild == 0) {
+        wf_search_done(A, S, S.use_skip != 0, S.use_skip ? S.toi_skip : INFINITY, S.delta);
+    } else {
+        S.axis = axis;
+    }
+    st[i] = S;
+}
+
+__global__ void k_wf_counts(const uint8_t* cmask, int64_t n, uint32_t* cnt)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j < n) cnt[j] = (uint32_t)__popc(cmask[j]);
+}
+
+__global__ void k_wf_emit(const WFState* st, const WFBox* boxes, const uint8_t* cmask,
+                          const uint32_t* off, int64_t n, WFBox* out)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const uint8_t m = cmask[j];
+    if (!m) return;
+    const WFBox b = boxes[j];
+    const WFState& S = st[b.owner];
+    WFBox h = b;
+    if (S.axis == 0) h.t = b.t + 0.5 * S.wt;
+    else if (S.axis == 1) h.u = b.u + 0.5 * S.wu;
+    else h.v = b.v + 0.5 * S.wv;
+    uint32_t p = off[j];
+    if (m & 1) out[p++] = h; // split_and_push pushes the upper half first
+    if (m & 2) out[p] = b;
+}
+
+// next-level segment of every candidate; restarts get one root box after
+// all children
+__global__ void k_wf_segments(WFState* st, int64_t nb, const uint32_t* off, int64_t nbox,
+                              unsigned long long* nroot, int64_t total_children, WFBox* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    WFState& S = st[i];
+    if (S.done) {
+        S.next_start = 0;
+        S.next_count = 0;
+        return;
+    }
+    if (S.restart) {
+        const int64_t p = total_children + (int64_t)atomicAdd(nroot, 1ull);
+        out[p] = { 0.0, 0.0, 0.0, (int32_t)i, 0 };
+        S.next_start = p;
+        S.next_count = 1;
+        return;
+    }
+    const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
+    const int64_t a = off[s0], b = s1 < nbox ? (int64_t)off[s1] : total_children;
+    S.next_start = a;
+    S.next_count = b - a;
+}
+
+// order each candidate's next level: a t split interleaves (upper, lower)
+// pairs; merge the two sorted halves (ties: upper half first, it was pushed
+// first); other splits are already in order.
+__global__ void k_wf_order(WFState* st, int64_t nb, const WFBox* in, WFBox* out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    WFState& S = st[i];
+    const int64_t s = S.next_start, n = S.next_count;
+    const bool was_split = !S.done && !S.restart;
+    if (was_split && S.axis == 0) {
+        int64_t a = 0, b = 0, k = s; // a: uppers at s+2a, b: lowers at s+2b+1
+        const int64_t half = n / 2;
+        while (a < half && b < half) {
+            const WFBox hu = in[s + 2 * a], lo = in[s + 2 * b + 1];
+            if (hu.t <= lo.t) {
+                out[k++] = hu;
+                a++;
+            } else {
+                out[k++] = lo;
+                b++;
+            }
+        }
+        while (a < half) out[k++] = in[s + 2 * a++];
+        while (b < half) out[k++] = in[s + 2 * b++ + 1];
+    } else {
+        for (int64_t k = s; k < s + n; k++) out[k] = in[k];
+    }
+    if (was_split) {
+        if (S.axis == 0) S.wt *= 0.5;
+        else if (S.axis == 1) S.wu *= 0.5;
+        else S.wv *= 0.5;
+    }
+    S.seg_start = s;
+    S.seg_count = n;
+}
+
+__global__ void k_wf_finish(CCDArgs A, const WFState* st, int64_t nb)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    const WFState& S = st[i];
+    report(A, S.ci, S.hit != 0, S.result);
+}
+
+// n[0]: candidates still searching, n[1]: of those, restarting at a root box
+__global__ void k_wf_pending(const WFState* st, int64_t nb, unsigned long long* n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < nb && !st[i].done) {
+        atomicAdd(n, 1ull);
+        if (st[i].restart) atomicAdd(n + 1, 1ull);
+    }
+}
+
+
+// Runs one batch of heavy candidates to completion.  Returns false, with
+// nothing reported, when a level exceeds the box budget.
+bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, int64_t nb,
+              WFState* states, bool early_exit, int* any_hit, int* error,
+              unsigned long long* nroot, unsigned long long* npend, unsigned long long* h)
+{
+    hipStream_t s = c.stream;
+    int64_t nbox = nb;
+    c.blk_keys.reserve(sizeof(WFBox) * (size_t)std::max<int64_t>(nbox, 1024));
+    WFBox* cur = reinterpret_cast<WFBox*>(c.blk_keys.p);
+    hipLaunchKernelGGL(k_wf_init, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, heavy, first, nb,
+                       states, cur);
+    IPCG_CHECK_LAUNCH();
+    for (;;) {
+        if (early_exit) {
+            IPCG_HIP(hipMemcpyAsync(h, any_hit, sizeof(int), hipMemcpyDeviceToHost, s));
+            IPCG_HIP(hipStreamSynchronize(s));
+            if (*reinterpret_cast<int*>(h)) break;
+        }
+        c.blk_vals.reserve(sizeof(uint8_t) * 2 * (size_t)std::max<int64_t>(nbox, 1) + 64);
+        uint8_t* flags = c.blk_vals.as<uint8_t>();
+        uint8_t* cmask = flags + ((nbox + 63) / 64) * 64;
+        c.blk_start.reserve(sizeof(uint32_t) * (size_t)(nbox + 1));
+        c.blk_vals2.reserve(sizeof(uint32_t) * (size_t)(nbox + 1));
+        uint32_t* cnt = c.blk_start.as<uint32_t>();
+        uint32_t* off = c.blk_vals2.as<uint32_t>();
+        c.blk_tmp.reserve(scan_tmp_bytes(nbox + 1));
+        hipLaunchKernelGGL(k_wf_eval, dim3(grid_for(nbox, 256, INT32_MAX)), dim3(256), 0, s, A,
+                           states, cur, nbox, flags, cmask);
+        hipLaunchKernelGGL(k_wf_decide, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb,
+                           cur, flags, cmask, error);
+        hipLaunchKernelGGL(k_wf_counts, dim3(grid_for(nbox, 256, INT32_MAX)), dim3(256), 0, s,
+                           cmask, nbox, cnt);
+        exclusive_scan_u32(cnt, off, nbox, off + nbox, c.blk_tmp.p, s);
+        IPCG_CHECK_LAUNCH();
+        IPCG_HIP(hipMemcpyAsync(h, off + nbox, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipMemsetAsync(npend, 0, 2 * sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_wf_pending, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, npend);
+        IPCG_HIP(hipMemcpyAsync(h + 1, npend, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        const int64_t children = reinterpret_cast<uint32_t*>(h)[0];
+        const int64_t pending = (int64_t)h[1], restarts = (int64_t)h[2];
+        if (pending == 0) break;
+        const int64_t nnext = children + restarts; // a restart is one root box
+        if (nnext > WF_MAX_BOXES) return false;
+        c.blk_keys2.reserve(sizeof(WFBox) * (size_t)std::max<int64_t>(nnext, 1));
+        c.blk_ukeys.reserve(sizeof(WFBox) * (size_t)std::max<int64_t>(nnext, 1));
+        WFBox* nxt = reinterpret_cast<WFBox*>(c.blk_keys2.p);
+        WFBox* ord = reinterpret_cast<WFBox*>(c.blk_ukeys.p);
+        IPCG_HIP(hipMemsetAsync(nroot, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_wf_emit, dim3(grid_for(nbox, 256, INT32_MAX)), dim3(256), 0, s, states,
+                           cur, cmask, off, nbox, nxt);
+        hipLaunchKernelGGL(k_wf_segments, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, off,
+                           nbox, nroot, children, nxt);
+        hipLaunchKernelGGL(k_wf_order, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, nxt,
+                           ord);
+        IPCG_CHECK_LAUNCH();
+        std::swap(c.blk_keys, c.blk_ukeys);
+        cur = reinterpret_cast<WFBox*>(c.blk_keys.p);
+        nbox = nnext;
+    }
+    hipLaunchKernelGGL(k_wf_finish, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb);
+    IPCG_CHECK_LAUNCH();
+    return true;
+}
+
+} // namespace
+
+double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, double rescale,
+                    bool early_exit, double* toi_ee, double* toi_fv)
+{
+    c.require_mesh();
+    c.require_pos(0);
+    c.require_pos(1);
+    if (!c.cand_valid || !c.cand_swept)
+        throw Error(IPCG_ERR_STATE, "no swept candidate set (run the swept broad phase)");
+    hipStream_t s = c.stream;
+    const int64_t n = c.n_ee + c.n_fv;
+    const bool per_cand = toi_ee || toi_fv;
+    if (n == 0) return 1.0;
+    if (n >= (1ll << 32)) throw Error(IPCG_ERR_ARG, "too many CCD candidates");
+    // device scalars: bound, work counter, heavy count, hit flag, error, roots
+    c.ccd_status.reserve(512);
+    char* st = c.ccd_status.as<char>();
+    double* bound = reinterpret_cast<double*>(st);
+    unsigned long long* work = reinterpret_cast<unsigned long long*>(st + 64);
+    unsigned long long* n_heavy = reinterpret_cast<unsigned long long*>(st + 128);
+    int* any_hit = reinterpret_cast<int*>(st + 192);
+    int* error = reinterpret_cast<int*>(st + 256);
+    unsigned long long* nroot = reinterpret_cast<unsigned long long*>(st + 320);
+    unsigned long long* npend = reinterpret_cast<unsigned long long*>(st + 384); // [2]
+    const double one = 1.0;
+    IPCG_HIP(hipMemsetAsync(st, 0, 512, s));
+    IPCG_HIP(hipMemcpyAsync(bound, &one, sizeof(double), hipMemcpyHostToDevice, s));
+    c.ccd_overflow.reserve(sizeof(uint32_t) * (size_t)n);
+    uint32_t* heavy = c.ccd_overflow.as<uint32_t>();
+    if (per_cand) c.ccd_toi.reserve(sizeof(double) * (size_t)n);
+    c.ccd_slab.reserve(sizeof(TIBox) * 3 * (size_t)LANE_CAP * LANES);
+
+    CCDArgs A;
+    A.X0 = c.X[0].as<double>();
+    A.X1 = c.X[1].as<double>();
+    A.nV = c.nV;
+    A.E = c.E.as<int32_t>();
+    A.F = c.F.as<int32_t>();
+    A.ee = c.cand_ee.as<int2>();
+    A.n_ee = c.n_ee;
+    A.fv = c.cand_fv.as<int2>();
+    A.n_fv = c.n_fv;
+    A.min_distance = min_distance;
+    A.tolerance = tol;
+    A.rescale = rescale;
+    A.max_iter = (long)max_iter;
+    A.bounded = !per_cand;
+    A.bound = bound;
+    A.toi_out = per_cand ? c.ccd_toi.as<double>() : nullptr;
+    A.any_hit = early_exit ? any_hit : nullptr;
+    unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
+
+    c.stage_begin("ccd_lanes");
+    const int64_t ln = std::min<int64_t>(LANES, ((n + CB - 1) / CB) * CB);
+    hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A,
+                       c.ccd_slab.as<TIBox>(), work, heavy, n_heavy);
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    IPCG_HIP(hipMemcpyAsync(h, n_heavy, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    const int64_t nh = (int64_t)h[0];
+
+    if (nh > 0) {
+        c.stage_begin("ccd_wavefront");
+        DevBuf& wst = c.ccd_bound; // per-candidate wavefront state
+        wst.reserve(sizeof(WFState) * (size_t)WF_BATCH);
+        WFState* states = wst.as<WFState>();
+        // Batches of heavy candidates advance together.  A batch whose level
+        // outgrows the box budget is abandoned (results are only reported at
+        // the end of a batch, so this has no side effect) and retried with a
+        // quarter of the candidates.
+        int64_t batch = WF_BATCH;
+        for (int64_t first = 0; first < nh;) {
+            const int64_t nb = std::min<int64_t>(batch, nh - first);
+            const bool ok = wf_batch(c, A, heavy, first, nb, states, early_exit, any_hit, error,
+                                     nroot, npend, h);
+            if (!ok) {
+                if (nb == 1)
+                    throw Error(IPCG_ERR_NUMERIC, "CCD wavefront level exceeds its box budget");
+                batch = std::max<int64_t>(1, nb / 4);
+                continue;
+            }
+            first += nb;
+            batch = std::min<int64_t>(WF_BATCH, 2 * batch);
+        }
+        c.stage_end();
+        IPCG_HIP(hipMemcpyAsync(h, error, sizeof(int), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        if (*reinterpret_cast<int*>(h))
+            throw Error(IPCG_ERR_NUMERIC, "CCD root search exceeded the level limit");
+    }
+    if (per_cand) {
+        if (toi_ee && c.n_ee)
+            IPCG_HIP(hipMemcpyAsync(toi_ee, c.ccd_toi.p, sizeof(double) * c.n_ee,
+                                    hipMemcpyDeviceToHost, s));
+        if (toi_fv && c.n_fv)
+            IPCG_HIP(hipMemcpyAsync(toi_fv, c.ccd_toi.as<double>() + c.n_ee,
+                                    sizeof(double) * c.n_fv, hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        return 1.0;
+    }
+    double* hd = static_cast<double*>(c.pinned);
+    IPCG_HIP(hipMemcpyAsync(hd, bound, sizeof(double), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipMemcpyAsync(hd + 1, any_hit, sizeof(int), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    if (early_exit) return reinterpret_cast<int*>(hd + 1)[0] ? 0.0 : 1.0;
+    return hd[0];
+}
+
+} // namespace ipcg

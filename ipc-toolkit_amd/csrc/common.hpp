@@ -1,0 +1,100 @@
+// Shared host/device plumbing: status codes, HIP error checks, grow-only
+// device buffers, wave-level helpers for wave64 (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ipcg.h"
+
+namespace ipcg {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define IPCG_HIP(expr)                                                                  \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            throw ::ipcg::Error(IPCG_ERR_HIP, std::string(#expr) + ": "                 \
+                                                  + hipGetErrorString(_e));             \
+    } while (0)
+
+#define IPCG_CHECK_LAUNCH() IPCG_HIP(hipGetLastError())
+
+constexpr int WAVE = 64;
+
+inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_blocks) g = max_blocks;
+    return (unsigned)g;
+}
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n)
+    {
+        if (n <= bytes) return;
+        if (p) IPCG_HIP(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        size_t want = n + n / 4 + 256;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw Error(IPCG_ERR_OOM, "device allocation of " + std::to_string(want)
+                                          + " bytes failed");
+        }
+        bytes = want;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+// ---- wave64 helpers -------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// exclusive prefix count of `pred` among lower lanes, and the wave total
+__device__ __forceinline__ unsigned wave_prefix(bool pred, unsigned& total)
+{
+    const unsigned long long m = __ballot(pred);
+    total = (unsigned)__popcll(m);
+    return (unsigned)__popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+// wave-aggregated append of `cnt` items; returns this lane's first slot
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long* counter,
+                                                          unsigned cnt)
+{
+    // inclusive scan of cnt over active lanes via shuffles
+    unsigned x = cnt;
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const unsigned y = __shfl_up(x, o, WAVE);
+        if (l >= o) x += y;
+    }
+    const unsigned total = __shfl(x, WAVE - 1, WAVE);
+    unsigned long long base = 0;
+    if (l == WAVE - 1 && total) base = atomicAdd(counter, (unsigned long long)total);
+    base = __shfl(base, WAVE - 1, WAVE);
+    return base + (x - cnt);
+}
+
+} // namespace ipcg

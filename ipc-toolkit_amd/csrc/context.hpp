@@ -1,0 +1,101 @@
+// Device-resident state of one collision mesh and the host-side stage entry
+// points that enqueue its kernels on the context stream.
+#pragma once
+
+#include "common.hpp"
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace ipcg {
+
+struct StageTimer {
+    bool enabled = false;
+    struct Rec {
+        hipEvent_t a, b;
+        std::string name;
+    };
+    std::vector<Rec> pending;
+    std::map<std::string, std::pair<double, int64_t>> acc;
+};
+
+// LBVH of one primitive set: internal node j splits sorted keys j | j+1.
+struct Node {
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    int32_t c0, c1; // child: >= 0 internal node, < 0 leaf ~position
+    int32_t r0, r1; // rightmost leaf position of each child
+};
+static_assert(sizeof(Node) == 64, "Node is one 64-byte line");
+
+struct Tree {
+    int32_t n = 0;       // leaves
+    DevBuf box;          // float[6] per primitive (original order)
+    DevBuf keys, keys2;  // uint64 Morton codes (sorted in place)
+    DevBuf ids, ids2;    // uint32 primitive id per sorted position
+    DevBuf nodes;        // Node[n-1]
+    DevBuf info;         // construction counters / ranges
+    DevBuf root;         // int32 root node (-1: single leaf)
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // mesh
+    int32_t nV = 0, nE = 0, nF = 0;
+    bool has_mesh = false, has_patch = false;
+    DevBuf E, F, F2E, patch, rest;
+    DevBuf X[2];
+    bool has_pos[2] = { false, false };
+
+    // broad phase
+    Tree tv, te, tf;
+    DevBuf aabb_partial, aabb; // mesh bounds
+    DevBuf sort_tmp;
+    bool cand_valid = false, cand_swept = false;
+    DevBuf cand_ee, cand_fv;   // int2 pairs
+    DevBuf counters;           // uint64 counters (device)
+    int64_t n_ee = 0, n_fv = 0;
+    int64_t cap_ee = 0, cap_fv = 0;
+
+    // collision set
+    bool coll_valid = false;
+    double coll_dmin = 0;
+    int64_t ncoll[4] = { 0, 0, 0, 0 };
+    DevBuf c_ids[4], c_w[4], c_eps, c_dtype; // ids: int2 per collision
+    DevBuf np_keys, np_keys2, np_vals, np_vals2, np_ukeys, np_start, np_tmp, np_raw;
+
+    // potential
+    DevBuf red_partial, red_out, grad, hq;
+    DevBuf blk_keys, blk_keys2, blk_vals, blk_vals2, blk_ukeys, blk_start, blk_tmp;
+    DevBuf blk_mask, row_cnt, rowptr, col, val;
+    int64_t nnz = 0;
+    bool hess_valid = false;
+
+    // ccd
+    DevBuf ccd_bound, ccd_slab, ccd_overflow, ccd_toi, ccd_status;
+
+    // host pinned staging
+    void* pinned = nullptr;
+
+    StageTimer timer;
+    void stage_begin(const char* name);
+    void stage_end();
+    std::vector<std::pair<hipEvent_t, std::string>> open_stage;
+
+    void require_mesh() const;
+    void require_pos(int slot) const;
+};
+
+// ---- stage entry points (one translation unit each) -------------------------
+void bp_build(Ctx& c, bool swept, double inflation, int method);
+void np_build(Ctx& c, double dhat, double dmin);
+double pot_energy(Ctx& c, double dhat, double kappa);
+void pot_gradient(Ctx& c, double dhat, double kappa);
+void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
+double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
+                    double rescale, bool early_exit, double* toi_ee, double* toi_fv);
+
+} // namespace ipcg

@@ -1,0 +1,273 @@
+// Narrow phase: classify every candidate's distance type, test activity
+// d^2 < (dhat + dmin)^2 and reduce it to a VV / EV / EE / FV collision
+// (normal_collisions_builder.cpp:140-336), then merge duplicates summing
+// their unit weights (normal_collisions_builder.cpp:547-600) by sorting the
+// collision keys and run-length encoding them.  Output order is canonical:
+// vv by (min, max), ev by (edge, vertex), ee by (min edge, max edge), fv by
+// (face, vertex).
+#include "context.hpp"
+#include "geometry.hpp"
+#include "sort.hpp"
+
+namespace ipcg {
+
+namespace {
+
+constexpr int NP_BLOCK = 256;
+
+__device__ __forceinline__ d3 ldx(const double* X, int nV, int i)
+{
+    return { X[i], X[nV + i], X[2 * (int64_t)nV + i] };
+}
+
+struct EERec {
+    int32_t ea, eb, dtype, pad;
+    double eps_x;
+};
+
+struct Streams {
+    uint64_t* vv;
+    uint64_t* ev;
+    EERec* ee;
+    uint64_t* fv;
+    unsigned long long* count; // [4]
+};
+
+__device__ __forceinline__ uint64_t vv_key(int a, int b)
+{
+    const uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+    return ((uint64_t)lo << 32) | hi;
+}
+
+__device__ __forceinline__ uint64_t pair_key(int a, int b)
+{
+    return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+// candidates: [0, n_ee) edge-edge, [n_ee, n_ee + n_fv) face-vertex
+__global__ __launch_bounds__(NP_BLOCK) void k_classify(
+    const double* X, const double* R, int nV, const int32_t* E, const int32_t* F,
+    const int32_t* F2E, const int2* ee, int64_t n_ee, const int2* fv, int64_t n_fv,
+    double active2, Streams S)
+{
+    const int64_t i = (int64_t)blockIdx.x * NP_BLOCK + threadIdx.x;
+    int kind = -1; // 0 vv, 1 ev, 2 ee, 3 fv
+    uint64_t key = 0;
+    EERec er { 0, 0, 0, 0, 0.0 };
+    if (i < n_ee) {
+        const int2 c = ee[i];
+        const int a0 = E[2 * (int64_t)c.x], a1 = E[2 * (int64_t)c.x + 1];
+        const int b0 = E[2 * (int64_t)c.y], b1 = E[2 * (int64_t)c.y + 1];
+        const d3 pa0 = ldx(X, nV, a0), pa1 = ldx(X, nV, a1), pb0 = ldx(X, nV, b0),
+                 pb1 = ldx(X, nV, b1);
+        const int actual = edge_edge_type(pa0, pa1, pb0, pb1);
+        const double d = ee_dist2(pa0, pa1, pb0, pb1, actual);
+        if (d < active2) {
+            const double eps_x = ee_mollifier_threshold(ldx(R, nV, a0), ldx(R, nV, a1),
+                                                        ldx(R, nV, b0), ldx(R, nV, b1));
+            const double x = ee_cross2(pa0, pa1, pb0, pb1);
+            const int dt = x < eps_x ? (int)EE_EA_EB : actual;
+            switch (dt) {
+            case EE_EA0_EB0: kind = 0; key = vv_key(a0, b0); break;
+            case EE_EA0_EB1: kind = 0; key = vv_key(a0, b1); break;
+            case EE_EA1_EB0: kind = 0; key = vv_key(a1, b0); break;
+            case EE_EA1_EB1: kind = 0; key = vv_key(a1, b1); break;
+            case EE_EA_EB0: kind = 1; key = pair_key(c.x, b0); break;
+            case EE_EA_EB1: kind = 1; key = pair_key(c.x, b1); break;
+            case EE_EA0_EB: kind = 1; key = pair_key(c.y, a0); break;
+            case EE_EA1_EB: kind = 1; key = pair_key(c.y, a1); break;
+            default:
+                kind = 2;
+                er = { c.x, c.y, actual, 0, eps_x };
+            }
+        }
+    } else if (i < n_ee + n_fv) {
+        const int2 c = fv[i - n_ee];
+        const int fi = c.x, vi = c.y;
+        const int f0 = F[3 * (int64_t)fi], f1 = F[3 * (int64_t)fi + 1],
+                  f2 = F[3 * (int64_t)fi + 2];
+        const d3 p = ldx(X, nV, vi), t0 = ldx(X, nV, f0), t1 = ldx(X, nV, f1),
+                 t2 = ldx(X, nV, f2);
+        const int dt = point_triangle_type(p, t0, t1, t2);
+        const double d = pt_dist2(p, t0, t1, t2, dt);
+        if (d < active2) {
+            switch (dt) {
+            case PT_P_T0: kind = 0; key = vv_key(vi, f0); break;
+            case PT_P_T1: kind = 0; key = vv_key(vi, f1); break;
+            case PT_P_T2: kind = 0; key = vv_key(vi, f2); break;
+            case PT_P_E0: kind = 1; key = pair_key(F2E[3 * (int64_t)fi], vi); break;
+            case PT_P_E1: kind = 1; key = pair_key(F2E[3 * (int64_t)fi + 1], vi); break;
+            case PT_P_E2: kind = 1; key = pair_key(F2E[3 * (int64_t)fi + 2], vi); break;
+            default: kind = 3; key = pair_key(fi, vi);
+            }
+        }
+    }
+    // one wave-aggregated append per stream
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const unsigned long long slot = wave_append(S.count + k, kind == k ? 1u : 0u);
+        if (kind == k) {
+            if (k == 0) S.vv[slot] = key;
+            else if (k == 1) S.ev[slot] = key;
+            else if (k == 2) S.ee[slot] = er;
+            else S.fv[slot] = key;
+        }
+    }
+}
+
+__global__ void k_iota(uint32_t* v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+__global__ void k_ee_keys(const EERec* r, int64_t n, uint64_t* keys, uint32_t* vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        keys[i] = vv_key(r[i].ea, r[i].eb);
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_ee_gather(const EERec* r, const uint32_t* perm, int64_t n, int2* ids, double* w,
+                            double* eps, int32_t* dtype)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const EERec e = r[perm[i]];
+        ids[i] = make_int2(e.ea, e.eb);
+        w[i] = 1.0;
+        eps[i] = e.eps_x;
+        dtype[i] = e.dtype;
+    }
+}
+
+__global__ void k_keys_to_pairs(const uint64_t* keys, int64_t n, int2* ids, double* w)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        ids[i] = make_int2((int)(keys[i] >> 32), (int)(keys[i] & 0xFFFFFFFFu));
+        w[i] = 1.0;
+    }
+}
+
+__global__ void k_runs_to_weights(const uint64_t* ukeys, const uint32_t* start, const uint32_t* nu,
+                                  int2* ids, double* w)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (int64_t)*nu) {
+        ids[i] = make_int2((int)(ukeys[i] >> 32), (int)(ukeys[i] & 0xFFFFFFFFu));
+        w[i] = (double)(start[i + 1] - start[i]);
+    }
+}
+
+int key_bits_for(int64_t max_id)
+{
+    int b = 1;
+    while ((1ll << b) <= max_id) b++;
+    return 32 + b;
+}
+
+} // namespace
+
+void np_build(Ctx& c, double dhat, double dmin)
+{
+    c.require_mesh();
+    c.require_pos(0);
+    if (!c.cand_valid || c.cand_swept)
+        throw Error(IPCG_ERR_STATE, "no static candidate set (run the static broad phase)");
+    hipStream_t s = c.stream;
+    c.coll_valid = false;
+    const int64_t n = c.n_ee + c.n_fv;
+    const int64_t cap = n > 0 ? n : 1;
+    c.np_raw.reserve(sizeof(uint64_t) * 2 * cap + sizeof(EERec) * (size_t)(c.n_ee > 0 ? c.n_ee : 1)
+                     + sizeof(uint64_t) * cap + 256);
+    char* base = c.np_raw.as<char>();
+    Streams S;
+    S.vv = reinterpret_cast<uint64_t*>(base);
+    S.ev = S.vv + cap;
+    S.fv = S.ev + cap;
+    S.ee = reinterpret_cast<EERec*>(S.fv + cap);
+    c.counters.reserve(sizeof(unsigned long long) * 8);
+    S.count = c.counters.as<unsigned long long>() + 4;
+    IPCG_HIP(hipMemsetAsync(S.count, 0, sizeof(unsigned long long) * 4, s));
+    const double off = dmin + dhat;
+    c.stage_begin("np_classify");
+    if (n > 0)
+        hipLaunchKernelGGL(k_classify, dim3(grid_for(n, NP_BLOCK, INT32_MAX)), dim3(NP_BLOCK), 0, s,
+                           c.X[0].as<double>(), c.rest.as<double>(), c.nV, c.E.as<int32_t>(),
+                           c.F.as<int32_t>(), c.F2E.as<int32_t>(), c.cand_ee.as<int2>(), c.n_ee,
+                           c.cand_fv.as<int2>(), c.n_fv, off * off, S);
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
+    IPCG_HIP(hipMemcpyAsync(h, S.count, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    const int64_t nraw[4] = { (int64_t)h[0], (int64_t)h[1], (int64_t)h[2], (int64_t)h[3] };
+
+    c.stage_begin("np_merge");
+    int64_t mx = 1;
+    for (int k = 0; k < 4; k++) mx = std::max(mx, nraw[k]);
+    c.np_keys.reserve(sizeof(uint64_t) * mx);
+    c.np_keys2.reserve(sizeof(uint64_t) * mx);
+    c.np_vals.reserve(sizeof(uint32_t) * mx);
+    c.np_vals2.reserve(sizeof(uint32_t) * mx);
+    c.np_ukeys.reserve(sizeof(uint64_t) * mx);
+    c.np_start.reserve(sizeof(uint32_t) * (mx + 1));
+    c.np_tmp.reserve(std::max(radix_tmp_bytes(mx), rle_tmp_bytes(mx)));
+    uint64_t* keys = c.np_keys.as<uint64_t>();
+    uint32_t* vals = c.np_vals.as<uint32_t>();
+    uint32_t* nu_dev = reinterpret_cast<uint32_t*>(c.counters.as<unsigned long long>() + 3);
+    const int vbits = key_bits_for(std::max<int64_t>(c.nV, c.nE));
+    const int fbits = key_bits_for(std::max<int64_t>(c.nF, 1));
+    for (int k = 0; k < 4; k++) {
+        const int64_t m = nraw[k];
+        c.c_ids[k].reserve(sizeof(int2) * (size_t)std::max<int64_t>(m, 1));
+        c.c_w[k].reserve(sizeof(double) * (size_t)std::max<int64_t>(m, 1));
+        if (k == 2) {
+            c.c_eps.reserve(sizeof(double) * (size_t)std::max<int64_t>(m, 1));
+            c.c_dtype.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1));
+        }
+        c.ncoll[k] = 0;
+        if (m == 0) continue;
+        const unsigned g = grid_for(m, 256, INT32_MAX);
+        if (k == 2) {
+            hipLaunchKernelGGL(k_ee_keys, dim3(g), dim3(256), 0, s, S.ee, m, keys, vals);
+            radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
+                             vbits, c.np_tmp.p, s);
+            hipLaunchKernelGGL(k_ee_gather, dim3(g), dim3(256), 0, s, S.ee, vals, m,
+                               c.c_ids[2].as<int2>(), c.c_w[2].as<double>(), c.c_eps.as<double>(),
+                               c.c_dtype.as<int32_t>());
+            c.ncoll[2] = m; // each unordered edge pair is one candidate: no duplicates
+        } else {
+            const uint64_t* src = k == 0 ? S.vv : (k == 1 ? S.ev : S.fv);
+            IPCG_HIP(hipMemcpyAsync(keys, src, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, s, vals, m);
+            radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
+                             k == 3 ? fbits : vbits, c.np_tmp.p, s);
+            if (k == 3) { // each (face, vertex) candidate is unique
+                hipLaunchKernelGGL(k_keys_to_pairs, dim3(g), dim3(256), 0, s, keys, m,
+                                   c.c_ids[3].as<int2>(), c.c_w[3].as<double>());
+                c.ncoll[3] = m;
+            } else {
+                run_length_encode(keys, m, c.np_ukeys.as<uint64_t>(), c.np_start.as<uint32_t>(),
+                                  nu_dev, c.np_tmp.p, s);
+                hipLaunchKernelGGL(k_runs_to_weights, dim3(g), dim3(256), 0, s,
+                                   c.np_ukeys.as<uint64_t>(), c.np_start.as<uint32_t>(), nu_dev,
+                                   c.c_ids[k].as<int2>(), c.c_w[k].as<double>());
+                uint32_t* hu = reinterpret_cast<uint32_t*>(h);
+                IPCG_HIP(hipMemcpyAsync(hu, nu_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                IPCG_HIP(hipStreamSynchronize(s));
+                c.ncoll[k] = hu[0];
+            }
+        }
+        IPCG_CHECK_LAUNCH();
+    }
+    c.stage_end();
+    c.coll_dmin = dmin;
+    c.coll_valid = true;
+    c.hess_valid = false;
+}
+
+} // namespace ipcg

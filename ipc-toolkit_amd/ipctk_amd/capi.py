@@ -1,0 +1,264 @@
+"""ctypes binding of the C ABI in include/ipcg.h (libipcg.so).
+
+This is the thinnest host layer: one `Context` per collision mesh, numpy
+arrays in and out.  It raises on any non-zero status with the context's
+error message.  The library must be built (``make -C ipc-toolkit_amd/csrc``
+or ``__graft_entry__.build()``); importing fails loudly if it is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libipcg.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_NUMERIC, ERR_NO_DEVICE = range(7)
+LAYOUT_AOS, LAYOUT_SOA = 0, 1
+BP_LBVH, BP_SAP = 0, 1
+PSD = {"none": 0, "clamp": 1, "abs": 2}
+
+# every symbol the header declares
+EXPORTS = [
+    "ipcg_ctx_create", "ipcg_ctx_destroy", "ipcg_last_error", "ipcg_ctx_synchronize",
+    "ipcg_ctx_stream", "ipcg_mesh_upload", "ipcg_positions_upload",
+    "ipcg_positions_upload_device", "ipcg_broad_phase", "ipcg_candidates_count",
+    "ipcg_candidates_download", "ipcg_collisions_build",
+    "ipcg_collisions_build_from_candidates", "ipcg_collisions_count",
+    "ipcg_collisions_download", "ipcg_barrier_energy", "ipcg_barrier_gradient",
+    "ipcg_barrier_hessian", "ipcg_hessian_download", "ipcg_device_results",
+    "ipcg_compute_collision_free_stepsize", "ipcg_candidates_stepsize",
+    "ipcg_is_step_collision_free", "ipcg_candidates_toi", "ipcg_stage_times",
+    "ipcg_stage_times_reset",
+]
+
+_lib = None
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+_L = C.POINTER(C.c_int64)
+
+
+class IPCGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ipcg error {code}: {msg}")
+        self.code = code
+
+
+def load(path=LIB_PATH):
+    """Load libipcg.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} not built: run `make -C ipc-toolkit_amd/csrc`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.ipcg_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.ipcg_ctx_destroy.argtypes = [vp]
+    L.ipcg_last_error.argtypes = [vp]
+    L.ipcg_last_error.restype = C.c_char_p
+    L.ipcg_ctx_synchronize.argtypes = [vp]
+    L.ipcg_ctx_stream.argtypes = [vp]
+    L.ipcg_ctx_stream.restype = vp
+    L.ipcg_mesh_upload.argtypes = [vp, C.c_int32, _D, C.c_int, C.c_int32, _I, C.c_int32, _I, _I]
+    L.ipcg_positions_upload.argtypes = [vp, C.c_int, _D, C.c_int]
+    L.ipcg_positions_upload_device.argtypes = [vp, C.c_int, vp, C.c_int]
+    L.ipcg_broad_phase.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    L.ipcg_candidates_count.argtypes = [vp, _L, _L]
+    L.ipcg_candidates_download.argtypes = [vp, _I, _I]
+    L.ipcg_collisions_build.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.ipcg_collisions_build_from_candidates.argtypes = [vp, C.c_double, C.c_double]
+    L.ipcg_collisions_count.argtypes = [vp, _L]
+    L.ipcg_collisions_download.argtypes = [vp, _I, _D, _I, _D, _I, _D, _D, _I, _I, _D]
+    L.ipcg_barrier_energy.argtypes = [vp, C.c_double, C.c_double, _D]
+    L.ipcg_barrier_gradient.argtypes = [vp, C.c_double, C.c_double, _D]
+    L.ipcg_barrier_hessian.argtypes = [vp, C.c_double, C.c_double, C.c_int, _L]
+    L.ipcg_hessian_download.argtypes = [vp, _L, _I, _D]
+    L.ipcg_device_results.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                      C.POINTER(vp)]
+    L.ipcg_compute_collision_free_stepsize.argtypes = [vp, C.c_double, C.c_double, C.c_int64,
+                                                       C.c_double, C.c_int, _D]
+    L.ipcg_candidates_stepsize.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double, _D]
+    L.ipcg_is_step_collision_free.argtypes = [vp, C.c_double, C.c_double, C.c_int64,
+                                              C.c_double, C.c_int, C.POINTER(C.c_int)]
+    L.ipcg_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double, _D, _D]
+    L.ipcg_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), _D, _L]
+    L.ipcg_stage_times_reset.argtypes = [vp, C.c_int]
+    for name in EXPORTS:
+        if name not in ("ipcg_last_error", "ipcg_ctx_stream"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_I)
+
+
+class Context:
+    """One device context: a collision mesh and its device-resident state."""
+
+    def __init__(self, device=0):
+        self._L = load()
+        self._p = C.c_void_p()
+        rc = self._L.ipcg_ctx_create(device, C.byref(self._p))
+        if rc != OK:
+            raise IPCGError(rc, "ipcg_ctx_create failed (no HIP device?)")
+        self.nV = 0
+
+    def close(self):
+        if self._p:
+            self._L.ipcg_ctx_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise IPCGError(rc, self._L.ipcg_last_error(self._p).decode())
+
+    # ---- mesh / positions --------------------------------------------------
+    def upload_mesh(self, rest, E, F, vertex_patch=None):
+        rest, pr = _d(rest)
+        E, pE = _i(np.asarray(E).reshape(-1, 2))
+        F, pF = _i(np.asarray(F).reshape(-1, 3))
+        pp = None
+        if vertex_patch is not None:
+            vertex_patch, pp = _i(vertex_patch)
+        self._check(self._L.ipcg_mesh_upload(self._p, len(rest), pr, LAYOUT_AOS, len(E), pE,
+                                             len(F), pF, pp))
+        self.nV, self.E, self.F = len(rest), E, F
+
+    def upload_positions(self, slot, V):
+        V, pV = _d(V)
+        self._check(self._L.ipcg_positions_upload(self._p, slot, pV, LAYOUT_AOS))
+
+    # ---- broad phase -------------------------------------------------------
+    def broad_phase(self, swept=False, inflation=0.0, method=BP_LBVH):
+        self._check(self._L.ipcg_broad_phase(self._p, int(swept), inflation, method))
+
+    def candidates(self):
+        ne, nf = C.c_int64(), C.c_int64()
+        self._check(self._L.ipcg_candidates_count(self._p, C.byref(ne), C.byref(nf)))
+        ee = np.zeros((ne.value, 2), np.int32)
+        fv = np.zeros((nf.value, 2), np.int32)
+        self._check(self._L.ipcg_candidates_download(self._p, ee.ctypes.data_as(_I),
+                                                     fv.ctypes.data_as(_I)))
+        return ee, fv
+
+    # ---- collisions --------------------------------------------------------
+    def build_collisions(self, dhat, dmin=0.0, method=BP_LBVH):
+        self._check(self._L.ipcg_collisions_build(self._p, dhat, dmin, method))
+
+    def build_collisions_from_candidates(self, dhat, dmin=0.0):
+        self._check(self._L.ipcg_collisions_build_from_candidates(self._p, dhat, dmin))
+
+    def collision_counts(self):
+        c = np.zeros(4, np.int64)
+        self._check(self._L.ipcg_collisions_count(self._p, c.ctypes.data_as(_L)))
+        return c
+
+    def collisions(self):
+        n = self.collision_counts()
+        out = {
+            "vv": np.zeros((n[0], 2), np.int32), "vv_w": np.zeros(n[0]),
+            "ev": np.zeros((n[1], 2), np.int32), "ev_w": np.zeros(n[1]),
+            "ee": np.zeros((n[2], 2), np.int32), "ee_w": np.zeros(n[2]),
+            "ee_eps_x": np.zeros(n[2]), "ee_dtype": np.zeros(n[2], np.int32),
+            "fv": np.zeros((n[3], 2), np.int32), "fv_w": np.zeros(n[3]),
+        }
+        a = out
+        self._check(self._L.ipcg_collisions_download(
+            self._p, a["vv"].ctypes.data_as(_I), a["vv_w"].ctypes.data_as(_D),
+            a["ev"].ctypes.data_as(_I), a["ev_w"].ctypes.data_as(_D),
+            a["ee"].ctypes.data_as(_I), a["ee_w"].ctypes.data_as(_D),
+            a["ee_eps_x"].ctypes.data_as(_D), a["ee_dtype"].ctypes.data_as(_I),
+            a["fv"].ctypes.data_as(_I), a["fv_w"].ctypes.data_as(_D)))
+        return out
+
+    # ---- barrier potential -------------------------------------------------
+    def energy(self, dhat, kappa):
+        e = C.c_double()
+        self._check(self._L.ipcg_barrier_energy(self._p, dhat, kappa, C.byref(e)))
+        return e.value
+
+    def gradient(self, dhat, kappa, download=True):
+        g = np.zeros(3 * self.nV) if download else None
+        self._check(self._L.ipcg_barrier_gradient(self._p, dhat, kappa,
+                                                  g.ctypes.data_as(_D) if download else None))
+        return g
+
+    def hessian(self, dhat, kappa, psd="clamp", download=True):
+        nnz = C.c_int64()
+        self._check(self._L.ipcg_barrier_hessian(self._p, dhat, kappa, PSD[psd], C.byref(nnz)))
+        if not download:
+            return nnz.value
+        import scipy.sparse as sp
+        n = 3 * self.nV
+        rowptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        self._check(self._L.ipcg_hessian_download(self._p, rowptr.ctypes.data_as(_L),
+                                                  col.ctypes.data_as(_I), val.ctypes.data_as(_D)))
+        return sp.csr_matrix((val, col, rowptr), shape=(n, n))
+
+    # ---- CCD ---------------------------------------------------------------
+    def stepsize(self, min_distance=0.0, tolerance=1e-6, max_iterations=10_000_000,
+                 rescale=0.8, method=BP_LBVH):
+        s = C.c_double()
+        self._check(self._L.ipcg_compute_collision_free_stepsize(
+            self._p, min_distance, tolerance, max_iterations, rescale, method, C.byref(s)))
+        return s.value
+
+    def candidates_stepsize(self, min_distance=0.0, tolerance=1e-6, max_iterations=10_000_000,
+                            rescale=0.8):
+        s = C.c_double()
+        self._check(self._L.ipcg_candidates_stepsize(self._p, min_distance, tolerance,
+                                                     max_iterations, rescale, C.byref(s)))
+        return s.value
+
+    def is_step_collision_free(self, min_distance=0.0, tolerance=1e-6,
+                               max_iterations=10_000_000, rescale=0.8, method=BP_LBVH):
+        r = C.c_int()
+        self._check(self._L.ipcg_is_step_collision_free(self._p, min_distance, tolerance,
+                                                        max_iterations, rescale, method,
+                                                        C.byref(r)))
+        return bool(r.value)
+
+    def candidates_toi(self, min_distance=0.0, tolerance=1e-6, max_iterations=10_000_000,
+                       rescale=0.8):
+        ne, nf = C.c_int64(), C.c_int64()
+        self._check(self._L.ipcg_candidates_count(self._p, C.byref(ne), C.byref(nf)))
+        te = np.zeros(ne.value)
+        tf = np.zeros(nf.value)
+        self._check(self._L.ipcg_candidates_toi(self._p, min_distance, tolerance, max_iterations,
+                                                rescale, te.ctypes.data_as(_D),
+                                                tf.ctypes.data_as(_D)))
+        return te, tf
+
+    # ---- timing ------------------------------------------------------------
+    def timing(self, enable=True):
+        self._check(self._L.ipcg_stage_times_reset(self._p, int(enable)))
+
+    def stage_times(self):
+        names = (C.c_char_p * 64)()
+        ms = np.zeros(64)
+        calls = np.zeros(64, np.int64)
+        n = self._L.ipcg_stage_times(self._p, 64, names, ms.ctypes.data_as(_D),
+                                     calls.ctypes.data_as(_L))
+        if n < 0:
+            raise IPCGError(-n, self._L.ipcg_last_error(self._p).decode())
+        return {names[i].decode(): (ms[i], int(calls[i])) for i in range(n)}
+
+    def synchronize(self):
+        self._check(self._L.ipcg_ctx_synchronize(self._p))

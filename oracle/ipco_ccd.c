@@ -13,6 +13,10 @@
 
 #include <limits.h>
 
+/* The no_zero_toi retry loop is capped at 64 halvings here and in the HIP
+ * kernels (Tight-Inclusion allows INT_MAX); a level cap of 1024 keeps every
+ * search finite. */
+
 /* work counters of the last root finder call on this thread (test probes) */
 static __thread long tl_refine, tl_max_queue;
 
@@ -240,6 +244,7 @@ static int ti_bfs(int is_vf, const double *s, const double *e,
     int ret = 0;
     while (h.n > 0) {
         if (h.n > tl_max_queue) tl_max_queue = h.n;
+        if (h.a[0].level >= 1023) break; /* level cap (see top of file) */
         const tbox b = heap_pop(&h);
         if (b.t0 >= toi_skip || b.t0 >= skip_from) continue;
         if (box_in_level != b.level) {
@@ -346,7 +351,7 @@ static int ticcd_skip(int is_vf, const double *x_t0, const double *x_t1,
             }
         }
         iter++;
-    } while (no_zero_toi && iter < INT_MAX && hit && *toi == 0);
+    } while (no_zero_toi && iter < 64 && hit && *toi == 0);
     return hit;
 }
 
