@@ -14,6 +14,7 @@
 // child boxes, child links and the children's rightmost leaf positions in
 // one 64-byte line, so a traversal step is one line load.
 #include "context.hpp"
+#include "shard.hpp"
 #include "geometry.hpp"
 #include "sort.hpp"
 
@@ -24,11 +25,6 @@ namespace {
 constexpr int BP_BLOCK = 256;
 constexpr int TR_BLOCK = 128;
 constexpr int TR_STACK = 64; // lbvh.cpp:378 MAX_STACK_SIZE
-
-__device__ __forceinline__ d3 ldx(const double* X, int nV, int i)
-{
-    return { X[i], X[nV + i], X[2 * (int64_t)nV + i] };
-}
 
 // AABB::from_point (+ swept union), aabb.cpp:21-55
 __device__ __forceinline__ void vbox(const double* X0, const double* X1, int nV, int i, double r,
@@ -161,85 +157,128 @@ __global__ __launch_bounds__(BP_BLOCK) void k_prim_boxes(
     }
 }
 
+// Common-prefix length of the sorted keys i and j extended by their
+// positions, so equal Morton codes still order strictly; -1 outside [0, n).
 __device__ __forceinline__ int delta(const uint64_t* codes, int n, int i, int j)
 {
     if (j < 0 || j >= n) return -1;
     const uint64_t ci = codes[i], cj = codes[j];
-    if (ci == cj) return 32 + __clz((unsigned)(i ^ j));
+    if (ci == cj) return 64 + __clz((unsigned)(i ^ j));
     return __clzll(ci ^ cj);
 }
 
-struct BuildInfo {
-    int32_t visits, lrange, rrange, pad;
-};
+// Hierarchy of the binary radix tree over the sorted keys (Karras 2012):
+// internal node i covers a key range that has i as one end and splits it at
+// the highest differing bit.  Every node is placed independently, so the
+// build needs no inter-thread communication.  The tree shape differs from
+// the reference's Apetrei construction (lbvh.cpp:197-302); the candidate set
+// does not depend on it.
+__global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, int n, Node* nodes,
+                                                     int32_t* parent, int32_t* leaf_parent)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = delta(codes, n, i, i - d);
+    int lmax = 2;
+    while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(codes, n, i, j);
+    int sp = 0;
+    for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
+        if (delta(codes, n, i, i + (sp + t) * d) > dnode) sp += t;
+        if (t == 1) break;
+    }
+    const int g = i + sp * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    Node* N = nodes + i;
+    if (lo == g) {
+        N->c0 = ~g;
+        leaf_parent[g] = i;
+    } else {
+        N->c0 = g;
+        parent[g] = i;
+    }
+    if (hi == g + 1) {
+        N->c1 = ~(g + 1);
+        leaf_parent[g + 1] = i;
+    } else {
+        N->c1 = g + 1;
+        parent[g + 1] = i;
+    }
+    N->r0 = g;
+    N->r1 = hi;
+    if (i == 0) parent[0] = -1;
+}
 
-// Apetrei 2014 bottom-up build (lbvh.cpp:197-302)
-__global__ __launch_bounds__(BP_BLOCK) void k_lbvh_build(const uint64_t* codes,
-                                                         const uint32_t* ids, const float* box32,
-                                                         int n, Node* nodes, BuildInfo* info,
-                                                         int32_t* root)
+// Bottom-up box refit, one level of completed nodes per launch: a node's
+// box goes into its slot of the parent, and the second child to arrive puts
+// the parent on the next work list.  Boxes written in one launch are read
+// in the next, so no inter-workgroup fences are needed.
+__device__ __forceinline__ void refit_push(Node* nodes, int32_t p, int32_t self,
+                                           const float* lo, const float* hi, int32_t* visits,
+                                           int32_t* next, uint32_t* next_n)
+{
+    Node* P = nodes + p;
+    if (P->c0 == self) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            P->lo0[k] = lo[k];
+            P->hi0[k] = hi[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            P->lo1[k] = lo[k];
+            P->hi1[k] = hi[k];
+        }
+    }
+    if (atomicAdd(visits + p, 1) == 1) next[atomicAdd(next_n, 1u)] = p;
+}
+
+__global__ __launch_bounds__(BP_BLOCK) void k_refit_leaves(const uint32_t* ids,
+                                                           const float* box32, int n,
+                                                           const int32_t* leaf_parent,
+                                                           Node* nodes, int32_t* visits,
+                                                           int32_t* next, uint32_t* next_n)
 {
     const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
     if (i >= n) return;
-    float lo[3], hi[3];
     const uint32_t pid = ids[i];
+    float lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         lo[k] = box32[(int64_t)6 * pid + k];
         hi[k] = box32[(int64_t)6 * pid + 3 + k];
     }
-    if (n == 1) {
-        *root = -1;
+    refit_push(nodes, leaf_parent[i], ~i, lo, hi, visits, next, next_n);
+}
+
+__global__ __launch_bounds__(BP_BLOCK) void k_refit_level(const int32_t* cur,
+                                                          const uint32_t* cur_n,
+                                                          const int32_t* parent, Node* nodes,
+                                                          int32_t* visits, int32_t* next,
+                                                          uint32_t* next_n, int32_t* root)
+{
+    const uint32_t t = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (t >= *cur_n) return;
+    const int32_t j = cur[t];
+    const Node* N = nodes + j;
+    float lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(N->lo0[k], N->lo1[k]);
+        hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
+    }
+    const int32_t p = parent[j];
+    if (p < 0) {
+        *root = j; // the tree is complete
         return;
     }
-    int lk = i, rk = i, cur = ~i;
-    for (int step = 0;; step++) {
-        if (step >= n) { // a walk is at most n - 1 steps; a longer one is a broken build
-            atomicOr(&info[0].pad, 1);
-            return;
-        }
-        const bool is_a = (lk == 0) || (rk != n - 1 && delta(codes, n, rk, rk + 1)
-                                                           > delta(codes, n, lk - 1, lk));
-        const int parent = is_a ? rk : lk - 1;
-        Node* P = nodes + parent;
-        if (is_a) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                P->lo0[k] = lo[k];
-                P->hi0[k] = hi[k];
-            }
-            P->c0 = cur;
-            P->r0 = rk;
-            info[parent].lrange = lk;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                P->lo1[k] = lo[k];
-                P->hi1[k] = hi[k];
-            }
-            P->c1 = cur;
-            P->r1 = rk;
-            info[parent].rrange = rk;
-        }
-        __threadfence();
-        if (atomicAdd(&info[parent].visits, 1) == 0) return;
-        __threadfence();
-        const volatile Node* VP = P;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const float sl = is_a ? VP->lo1[k] : VP->lo0[k];
-            const float sh = is_a ? VP->hi1[k] : VP->hi0[k];
-            lo[k] = fminf(lo[k], sl);
-            hi[k] = fmaxf(hi[k], sh);
-        }
-        lk = ((volatile BuildInfo*)info)[parent].lrange;
-        rk = ((volatile BuildInfo*)info)[parent].rrange;
-        cur = parent;
-        if (lk == 0 && rk == n - 1) {
-            *root = parent;
-            return;
-        }
-    }
+    refit_push(nodes, p, j, lo, hi, visits, next, next_n);
 }
 
 __device__ __forceinline__ bool overlap(const float* q, const float* lo, const float* hi)
@@ -284,8 +323,8 @@ __device__ __forceinline__ bool can_fv(const Filter& f, int fi, int vi)
 template <bool TRI>
 __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
-    const uint32_t* tids, const float* tbox, int nleaves, Filter flt, int2* out,
-    unsigned long long* counter, int64_t cap, int* status)
+    const uint32_t* tids, const float* tbox, int nleaves, Filter flt, int2* out, Shards S,
+    int* status)
 {
     __shared__ int stack[TR_STACK][TR_BLOCK];
     const int t = threadIdx.x;
@@ -311,10 +350,12 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
             const float* b = tbox + (int64_t)6 * fp;
             if (overlap(q, b, b + 3) && can_fv(flt, fp, qprim)) hit = fp;
         }
-        const unsigned long long slot = wave_append(counter, hit >= 0 ? 1u : 0u);
-        if (hit >= 0 && (int64_t)slot < cap) out[slot] = make_int2(hit, qprim);
+        const int sh = wave_shard();
+        const unsigned long long slot = shard_append(S, sh, hit >= 0 ? 1u : 0u);
+        if (hit >= 0 && (int64_t)slot < S.scap) out[sh * S.scap + (int64_t)slot] = make_int2(hit, qprim);
         return;
     }
+    const int sh = wave_shard();
     int node = root, sp = 0, steps = 0;
     const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
     while (__any(active)) {
@@ -363,13 +404,13 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
             }
         }
         const unsigned cnt = (h0 >= 0 ? 1u : 0u) + (h1 >= 0 ? 1u : 0u);
-        unsigned long long slot = wave_append(counter, cnt);
+        int64_t slot = (int64_t)shard_append(S, sh, cnt);
+        int2* reg = out + sh * S.scap;
         if (h0 >= 0) {
-            if ((int64_t)slot < cap) out[slot] = TRI ? make_int2(qprim, h0) : make_int2(h0, qprim);
+            if (slot < S.scap) reg[slot] = TRI ? make_int2(qprim, h0) : make_int2(h0, qprim);
             slot++;
         }
-        if (h1 >= 0 && (int64_t)slot < cap)
-            out[slot] = TRI ? make_int2(qprim, h1) : make_int2(h1, qprim);
+        if (h1 >= 0 && slot < S.scap) reg[slot] = TRI ? make_int2(qprim, h1) : make_int2(h1, qprim);
     }
 }
 
@@ -404,14 +445,46 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     c.stage_end();
     if (!with_nodes) return;
     T.nodes.reserve(sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1));
-    T.info.reserve(sizeof(BuildInfo) * (size_t)(n > 1 ? n - 1 : 1));
-    IPCG_HIP(hipMemsetAsync(T.info.p, 0, sizeof(BuildInfo) * (size_t)(n > 1 ? n - 1 : 1), s));
+    if (n == 1) { // single leaf: no internal node (lbvh.cpp:386-396)
+        const int32_t one = -1;
+        IPCG_HIP(hipMemcpyAsync(T.root.p, &one, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    // int32 arrays: parent[n-1], leaf_parent[n], visits[n-1], two work lists
+    // of n, and two counters
+    const size_t ni = (size_t)n - 1;
+    T.info.reserve(sizeof(int32_t) * (2 * ni + 3 * (size_t)n) + 256);
+    int32_t* parent = T.info.as<int32_t>();
+    int32_t* leaf_parent = parent + ni;
+    int32_t* visits = leaf_parent + n;
+    int32_t* work[2] = { visits + ni, visits + ni + n };
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(work[1] + n);
+    IPCG_HIP(hipMemsetAsync(visits, 0, sizeof(int32_t) * ni, s));
+    IPCG_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(uint32_t), s));
     IPCG_HIP(hipMemsetAsync(T.root.p, 0x80, sizeof(int32_t), s)); // "unfinished" sentinel
     c.stage_begin("bp_build");
-    hipLaunchKernelGGL(k_lbvh_build, dim3(g), dim3(BP_BLOCK), 0, s, T.keys.as<uint64_t>(),
-                       T.ids.as<uint32_t>(), T.box.as<float>(), n, T.nodes.as<Node>(),
-                       T.info.as<BuildInfo>(), T.root.as<int32_t>());
+    hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, BP_BLOCK)), dim3(BP_BLOCK), 0, s,
+                       T.keys.as<uint64_t>(), n, T.nodes.as<Node>(), parent, leaf_parent);
+    hipLaunchKernelGGL(k_refit_leaves, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(),
+                       T.box.as<float>(), n, leaf_parent, T.nodes.as<Node>(), visits, work[0],
+                       cnt);
     IPCG_CHECK_LAUNCH();
+    uint32_t* h = static_cast<uint32_t*>(c.pinned);
+    // every level completes at least one node, so n - 1 levels bound the loop
+    for (size_t level = 0;; level++) {
+        IPCG_HIP(hipMemcpyAsync(h, cnt + (level & 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        const uint32_t m = h[0];
+        if (m == 0) break;
+        if (level >= ni) throw Error(IPCG_ERR_NUMERIC, "LBVH refit did not converge");
+        uint32_t* nxt = cnt + ((level + 1) & 1);
+        IPCG_HIP(hipMemsetAsync(nxt, 0, sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_refit_level, dim3(grid_for(m, BP_BLOCK)), dim3(BP_BLOCK), 0, s,
+                           work[level & 1], cnt + (level & 1), parent, T.nodes.as<Node>(),
+                           visits, work[(level + 1) & 1], nxt, T.root.as<int32_t>());
+        IPCG_CHECK_LAUNCH();
+    }
     c.stage_end();
 }
 
@@ -451,51 +524,67 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
 
     c.counters.reserve(sizeof(unsigned long long) * 4);
     Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr };
-    unsigned long long* cnt = c.counters.as<unsigned long long>();
-    int* status = reinterpret_cast<int*>(cnt + 2);
-    unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
-    for (int attempt = 0; attempt < 2; attempt++) {
-        if (c.cap_ee == 0) c.cap_ee = std::max<int64_t>(1024, 16 * (int64_t)c.nE);
-        if (c.cap_fv == 0) c.cap_fv = std::max<int64_t>(1024, 8 * (int64_t)c.nV);
-        c.cand_ee.reserve(sizeof(int2) * (size_t)c.cap_ee);
-        c.cand_fv.reserve(sizeof(int2) * (size_t)c.cap_fv);
-        IPCG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * 4, s));
+    int* status = reinterpret_cast<int*>(c.counters.as<unsigned long long>() + 2);
+    c.shard_cnt.reserve(2 * SHARD_CNT_BYTES);
+    c.shard_off.reserve(sizeof(int64_t) * 2 * (SHARDS + 1));
+    unsigned long long* cnt[2] = { c.shard_cnt.as<unsigned long long>(),
+                                   c.shard_cnt.as<unsigned long long>() + SHARDS * SHARD_PAD };
+    for (int attempt = 0; attempt < 3; attempt++) {
+        // per-shard regions; their sum bounds the list until a retry grows it
+        if (c.scap_ee == 0) c.scap_ee = std::max<int64_t>(1024, 16 * (int64_t)c.nE / SHARDS);
+        if (c.scap_fv == 0) c.scap_fv = std::max<int64_t>(1024, 8 * (int64_t)c.nV / SHARDS);
+        c.cand_stage[0].reserve(sizeof(int2) * (size_t)(SHARDS * c.scap_ee));
+        c.cand_stage[1].reserve(sizeof(int2) * (size_t)(SHARDS * c.scap_fv));
+        IPCG_HIP(hipMemsetAsync(c.shard_cnt.p, 0, 2 * SHARD_CNT_BYTES, s));
+        IPCG_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
         c.stage_begin("bp_traverse");
         if (c.nE >= 2)
             hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0,
                                s, c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
                                c.te.nodes.as<Node>(), c.te.root.as<int32_t>(),
                                c.te.ids.as<uint32_t>(), c.te.box.as<float>(), c.nE, flt,
-                               c.cand_ee.as<int2>(), cnt, c.cap_ee, status);
+                               c.cand_stage[0].as<int2>(), Shards { cnt[0], c.scap_ee }, status);
         if (c.nF >= 1)
             hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(c.nV, TR_BLOCK)), dim3(TR_BLOCK),
                                0, s, c.tv.box.as<float>(), c.tv.ids.as<uint32_t>(), c.nV,
                                c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(),
                                c.tf.ids.as<uint32_t>(), c.tf.box.as<float>(), c.nF, flt,
-                               c.cand_fv.as<int2>(), cnt + 1, c.cap_fv, status);
+                               c.cand_stage[1].as<int2>(), Shards { cnt[1], c.scap_fv }, status);
         IPCG_CHECK_LAUNCH();
         c.stage_end();
-        IPCG_HIP(hipMemcpyAsync(h, cnt, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
+        int* hs = static_cast<int*>(c.pinned);
+        IPCG_HIP(hipMemcpyAsync(hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));
-        if ((int)h[2] & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
-        if ((int)h[2] & 1)
-            throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
-        const int64_t ne = (int64_t)h[0], nf = (int64_t)h[1];
+        const int st = hs[0];
+        if (st & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
+        if (st & 1) throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
+        const ShardCounts ke = shard_counts(cnt[0], c.pinned, s);
+        const ShardCounts kf = shard_counts(cnt[1], c.pinned, s);
         bool again = false;
-        if (ne > c.cap_ee) {
-            c.cap_ee = ne + ne / 8 + 1024;
+        if (ke.max > c.scap_ee) {
+            c.scap_ee = ke.max + ke.max / 4 + 1024;
             again = true;
         }
-        if (nf > c.cap_fv) {
-            c.cap_fv = nf + nf / 8 + 1024;
+        if (kf.max > c.scap_fv) {
+            c.scap_fv = kf.max + kf.max / 4 + 1024;
             again = true;
         }
-        c.n_ee = ne;
-        c.n_fv = nf;
-        if (!again) break;
+        if (again) continue;
+        c.stage_begin("bp_compact");
+        c.cand_ee.reserve(sizeof(int2) * (size_t)std::max<int64_t>(ke.total, 1));
+        c.cand_fv.reserve(sizeof(int2) * (size_t)std::max<int64_t>(kf.total, 1));
+        shard_compact(c.cand_stage[0].as<int2>(), c.scap_ee, cnt[0], ke.max, c.cand_ee.as<int2>(),
+                      c.shard_off.as<int64_t>(), s);
+        shard_compact(c.cand_stage[1].as<int2>(), c.scap_fv, cnt[1], kf.max, c.cand_fv.as<int2>(),
+                      c.shard_off.as<int64_t>() + (SHARDS + 1), s);
+        c.stage_end();
+        c.n_ee = ke.total;
+        c.n_fv = kf.total;
+        c.cand_valid = true;
+        c.cand_swept = swept;
+        return;
     }
-    c.cand_valid = true;
-    c.cand_swept = swept;
+    throw Error(IPCG_ERR_NUMERIC, "broad phase candidate lists kept outgrowing their regions");
 }
 
 } // namespace ipcg

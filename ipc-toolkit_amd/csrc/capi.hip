@@ -2,6 +2,7 @@
 // wrappers around the stage entry points.  Nothing here throws across the
 // ABI: every entry point catches and records the message in the context.
 #include "context.hpp"
+#include "geometry.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -66,14 +67,25 @@ template <class F> int guard(ipcg_ctx* ctx, F&& f)
     }
 }
 
-__global__ void k_aos_to_soa(const double* in, double* out, int n)
+// caller layout (SoA: x[n], y[n], z[n]; AoS: xyz per vertex) -> 32-byte
+// records of the device layout
+__global__ void k_to_records(const double* in, int layout, double* out, int n)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) {
-        out[i] = in[3 * (int64_t)i];
-        out[n + i] = in[3 * (int64_t)i + 1];
-        out[2 * (int64_t)n + i] = in[3 * (int64_t)i + 2];
+    if (i >= n) return;
+    double x, y, z;
+    if (layout == IPCG_LAYOUT_SOA) {
+        x = in[i];
+        y = in[n + i];
+        z = in[2 * (int64_t)n + i];
+    } else {
+        x = in[3 * (int64_t)i];
+        y = in[3 * (int64_t)i + 1];
+        z = in[3 * (int64_t)i + 2];
     }
+    double* o = out + PSTRIDE * (int64_t)i;
+    *reinterpret_cast<double2*>(o) = make_double2(x, y);
+    *reinterpret_cast<double2*>(o + 2) = make_double2(z, 0.0);
 }
 
 __global__ void k_gather_pairs(const int2* in, int64_t n, int32_t* out)
@@ -87,21 +99,20 @@ __global__ void k_gather_pairs(const int2* in, int64_t n, int32_t* out)
 
 void upload_positions(Ctx& c, DevBuf& dst, const double* src, int layout, bool device_src)
 {
-    const size_t bytes = sizeof(double) * 3 * (size_t)c.nV;
-    dst.reserve(bytes > 0 ? bytes : 8);
-    if (c.nV == 0) return;
-    const hipMemcpyKind kind = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    if (layout == IPCG_LAYOUT_SOA) {
-        IPCG_HIP(hipMemcpyAsync(dst.p, src, bytes, kind, c.stream));
-    } else if (layout == IPCG_LAYOUT_AOS) {
-        c.np_tmp.reserve(bytes);
-        IPCG_HIP(hipMemcpyAsync(c.np_tmp.p, src, bytes, kind, c.stream));
-        hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(c.nV, 256)), dim3(256), 0, c.stream,
-                           c.np_tmp.as<double>(), dst.as<double>(), c.nV);
-        IPCG_CHECK_LAUNCH();
-    } else {
+    if (layout != IPCG_LAYOUT_SOA && layout != IPCG_LAYOUT_AOS)
         throw Error(IPCG_ERR_ARG, "unknown position layout");
+    dst.reserve(sizeof(double) * PSTRIDE * (size_t)std::max(c.nV, 1));
+    if (c.nV == 0) return;
+    const double* in = src;
+    if (!device_src) { // stage the caller's array on the device first
+        const size_t bytes = sizeof(double) * 3 * (size_t)c.nV;
+        c.np_tmp.reserve(bytes);
+        IPCG_HIP(hipMemcpyAsync(c.np_tmp.p, src, bytes, hipMemcpyHostToDevice, c.stream));
+        in = c.np_tmp.as<double>();
     }
+    hipLaunchKernelGGL(k_to_records, dim3(grid_for(c.nV, 256)), dim3(256), 0, c.stream, in, layout,
+                       dst.as<double>(), c.nV);
+    IPCG_CHECK_LAUNCH();
     if (!device_src) IPCG_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -124,7 +135,7 @@ int ipcg_ctx_create(int device, ipcg_ctx** out)
         c.device = device;
         IPCG_HIP(hipSetDevice(device));
         IPCG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-        IPCG_HIP(hipHostMalloc(&c.pinned, 4096, hipHostMallocDefault));
+        IPCG_HIP(hipHostMalloc(&c.pinned, PINNED_BYTES, hipHostMallocDefault));
         c.counters.reserve(sizeof(unsigned long long) * 16);
     });
     if (rc != IPCG_OK) {
@@ -151,7 +162,9 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.np_start, &c.np_tmp, &c.np_raw, &c.red_partial, &c.red_out, &c.grad, &c.hq,
                        &c.blk_keys, &c.blk_keys2, &c.blk_vals, &c.blk_vals2, &c.blk_ukeys, &c.blk_start,
                        &c.blk_tmp, &c.blk_mask, &c.row_cnt, &c.rowptr, &c.col, &c.val, &c.ccd_bound,
-                       &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status };
+                       &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status,
+                       &c.hp_mark, &c.hp_bptr, &c.hp_bcol, &c.cand_stage[0], &c.cand_stage[1],
+                       &c.shard_cnt, &c.shard_off, &c.np_eec };
     for (DevBuf* b : bufs) b->release();
     for (auto& p : c.timer.pending) {
         (void)hipEventDestroy(p.a);
@@ -226,7 +239,7 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
         c.has_mesh = true;
         c.has_pos[0] = c.has_pos[1] = false;
         c.cand_valid = c.coll_valid = c.hess_valid = false;
-        c.cap_ee = c.cap_fv = 0;
+        c.scap_ee = c.scap_fv = 0;
     });
 }
 

@@ -33,11 +33,6 @@ constexpr int WF_BATCH = 4096;
 constexpr int64_t WF_MAX_BOXES = 1ll << 26;
 constexpr double SMALL_TOI = 1e-6; // tight_inclusion_ccd.hpp:14
 
-__device__ __forceinline__ d3 ldx(const double* X, int nV, int i)
-{
-    return { X[i], X[nV + i], X[2 * (int64_t)nV + i] };
-}
-
 struct CCDArgs {
     const double* X0;
     const double* X1;

@@ -10,6 +10,8 @@
 
 namespace ipcg {
 
+constexpr size_t PINNED_BYTES = 64 * 1024;
+
 struct StageTimer {
     bool enabled = false;
     struct Rec {
@@ -58,26 +60,31 @@ struct Ctx {
     DevBuf cand_ee, cand_fv;   // int2 pairs
     DevBuf counters;           // uint64 counters (device)
     int64_t n_ee = 0, n_fv = 0;
-    int64_t cap_ee = 0, cap_fv = 0;
+    int64_t scap_ee = 0, scap_fv = 0;  // per-shard regions of the traversal output
+    DevBuf cand_stage[2];              // sharded traversal output (ee, fv)
+    DevBuf shard_cnt, shard_off;       // shard counters / offsets of the append lists
 
     // collision set
     bool coll_valid = false;
     double coll_dmin = 0;
     int64_t ncoll[4] = { 0, 0, 0, 0 };
     DevBuf c_ids[4], c_w[4], c_eps, c_dtype; // ids: int2 per collision
-    DevBuf np_keys, np_keys2, np_vals, np_vals2, np_ukeys, np_start, np_tmp, np_raw;
+    DevBuf np_keys, np_keys2, np_vals, np_vals2, np_ukeys, np_start, np_tmp, np_raw, np_eec;
 
     // potential
     DevBuf red_partial, red_out, grad, hq;
     DevBuf blk_keys, blk_keys2, blk_vals, blk_vals2, blk_ukeys, blk_start, blk_tmp;
     DevBuf blk_mask, row_cnt, rowptr, col, val;
+    DevBuf hp_mark, hp_bptr, hp_bcol; // block pattern of the collision set
+    int64_t nblocks = 0;
+    bool pattern_valid = false;
     int64_t nnz = 0;
     bool hess_valid = false;
 
     // ccd
     DevBuf ccd_bound, ccd_slab, ccd_overflow, ccd_toi, ccd_status;
 
-    // host pinned staging
+    // host pinned staging (PINNED_BYTES)
     void* pinned = nullptr;
 
     StageTimer timer;

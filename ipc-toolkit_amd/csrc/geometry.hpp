@@ -35,6 +35,17 @@ struct d3 {
     double x, y, z;
 };
 
+// Positions live in HBM as one 32-byte record per vertex (x, y, z, unused):
+// a gathered vertex is one aligned access to one cache line.
+constexpr int PSTRIDE = 4;
+
+__device__ __forceinline__ d3 ldx(const double* X, int /*nV*/, int i)
+{
+    const double* p = X + PSTRIDE * (int64_t)i;
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    return { a.x, a.y, p[2] };
+}
+
 IPCG_HD d3 sub(const d3& a, const d3& b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
 IPCG_HD d3 add(const d3& a, const d3& b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
 IPCG_HD d3 scale(const d3& a, double s) { return { a.x * s, a.y * s, a.z * s }; }

@@ -7,6 +7,7 @@
 // (face, vertex).
 #include "context.hpp"
 #include "geometry.hpp"
+#include "shard.hpp"
 #include "sort.hpp"
 
 namespace ipcg {
@@ -15,22 +16,22 @@ namespace {
 
 constexpr int NP_BLOCK = 256;
 
-__device__ __forceinline__ d3 ldx(const double* X, int nV, int i)
-{
-    return { X[i], X[nV + i], X[2 * (int64_t)nV + i] };
-}
-
 struct EERec {
     int32_t ea, eb, dtype, pad;
     double eps_x;
 };
 
+// Sharded staging of the four collision streams: stream k's shard sh owns
+// slots [sh * scap, (sh + 1) * scap).  A wave appends at most 64 entries per
+// stream and shard sh serves every SHARDS-th wave, so scap =
+// 64 * ceil(waves / SHARDS) never overflows.
 struct Streams {
     uint64_t* vv;
     uint64_t* ev;
     EERec* ee;
     uint64_t* fv;
-    unsigned long long* count; // [4]
+    unsigned long long* cnt; // 4 x SHARDS x SHARD_PAD
+    int64_t scap;
 };
 
 __device__ __forceinline__ uint64_t vv_key(int a, int b)
@@ -102,15 +103,18 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
             }
         }
     }
-    // one wave-aggregated append per stream
+    // one wave-aggregated append per stream into this wave's shard
+    const int sh = wave_shard();
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const unsigned long long slot = wave_append(S.count + k, kind == k ? 1u : 0u);
-        if (kind == k) {
-            if (k == 0) S.vv[slot] = key;
-            else if (k == 1) S.ev[slot] = key;
-            else if (k == 2) S.ee[slot] = er;
-            else S.fv[slot] = key;
+        const Shards Sk { S.cnt + (int64_t)k * SHARDS * SHARD_PAD, S.scap };
+        const int64_t slot = (int64_t)shard_append(Sk, sh, kind == k ? 1u : 0u);
+        if (kind == k && slot < S.scap) {
+            const int64_t at = sh * S.scap + slot;
+            if (k == 0) S.vv[at] = key;
+            else if (k == 1) S.ev[at] = key;
+            else if (k == 2) S.ee[at] = er;
+            else S.fv[at] = key;
         }
     }
 }
@@ -180,9 +184,10 @@ void np_build(Ctx& c, double dhat, double dmin)
     hipStream_t s = c.stream;
     c.coll_valid = false;
     const int64_t n = c.n_ee + c.n_fv;
-    const int64_t cap = n > 0 ? n : 1;
-    c.np_raw.reserve(sizeof(uint64_t) * 2 * cap + sizeof(EERec) * (size_t)(c.n_ee > 0 ? c.n_ee : 1)
-                     + sizeof(uint64_t) * cap + 256);
+    const int64_t waves = (n + WAVE - 1) / WAVE;
+    const int64_t scap = std::max<int64_t>(WAVE * ((waves + SHARDS - 1) / SHARDS), WAVE);
+    const int64_t cap = SHARDS * scap; // entries per stream
+    c.np_raw.reserve((3 * sizeof(uint64_t) + sizeof(EERec)) * (size_t)cap + 256);
     char* base = c.np_raw.as<char>();
     Streams S;
     S.vv = reinterpret_cast<uint64_t*>(base);
@@ -190,8 +195,11 @@ void np_build(Ctx& c, double dhat, double dmin)
     S.fv = S.ev + cap;
     S.ee = reinterpret_cast<EERec*>(S.fv + cap);
     c.counters.reserve(sizeof(unsigned long long) * 8);
-    S.count = c.counters.as<unsigned long long>() + 4;
-    IPCG_HIP(hipMemsetAsync(S.count, 0, sizeof(unsigned long long) * 4, s));
+    c.shard_cnt.reserve(4 * SHARD_CNT_BYTES);
+    c.shard_off.reserve(sizeof(int64_t) * 4 * (SHARDS + 1));
+    S.cnt = c.shard_cnt.as<unsigned long long>();
+    S.scap = scap;
+    IPCG_HIP(hipMemsetAsync(S.cnt, 0, 4 * SHARD_CNT_BYTES, s));
     const double off = dmin + dhat;
     c.stage_begin("np_classify");
     if (n > 0)
@@ -202,9 +210,13 @@ void np_build(Ctx& c, double dhat, double dmin)
     IPCG_CHECK_LAUNCH();
     c.stage_end();
     unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
-    IPCG_HIP(hipMemcpyAsync(h, S.count, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, s));
-    IPCG_HIP(hipStreamSynchronize(s));
-    const int64_t nraw[4] = { (int64_t)h[0], (int64_t)h[1], (int64_t)h[2], (int64_t)h[3] };
+    ShardCounts kc[4];
+    int64_t nraw[4];
+    for (int k = 0; k < 4; k++) {
+        kc[k] = shard_counts(S.cnt + (int64_t)k * SHARDS * SHARD_PAD, c.pinned, s);
+        if (kc[k].max > scap) throw Error(IPCG_ERR_NUMERIC, "collision stream outgrew its shard");
+        nraw[k] = kc[k].total;
+    }
 
     c.stage_begin("np_merge");
     int64_t mx = 1;
@@ -232,17 +244,22 @@ void np_build(Ctx& c, double dhat, double dmin)
         c.ncoll[k] = 0;
         if (m == 0) continue;
         const unsigned g = grid_for(m, 256, INT32_MAX);
+        const unsigned long long* kcnt = S.cnt + (int64_t)k * SHARDS * SHARD_PAD;
+        int64_t* koff = c.shard_off.as<int64_t>() + k * (SHARDS + 1);
         if (k == 2) {
-            hipLaunchKernelGGL(k_ee_keys, dim3(g), dim3(256), 0, s, S.ee, m, keys, vals);
+            c.np_eec.reserve(sizeof(EERec) * (size_t)m);
+            EERec* ee = c.np_eec.as<EERec>();
+            shard_compact(S.ee, scap, kcnt, kc[k].max, ee, koff, s);
+            hipLaunchKernelGGL(k_ee_keys, dim3(g), dim3(256), 0, s, ee, m, keys, vals);
             radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
                              vbits, c.np_tmp.p, s);
-            hipLaunchKernelGGL(k_ee_gather, dim3(g), dim3(256), 0, s, S.ee, vals, m,
+            hipLaunchKernelGGL(k_ee_gather, dim3(g), dim3(256), 0, s, ee, vals, m,
                                c.c_ids[2].as<int2>(), c.c_w[2].as<double>(), c.c_eps.as<double>(),
                                c.c_dtype.as<int32_t>());
             c.ncoll[2] = m; // each unordered edge pair is one candidate: no duplicates
         } else {
             const uint64_t* src = k == 0 ? S.vv : (k == 1 ? S.ev : S.fv);
-            IPCG_HIP(hipMemcpyAsync(keys, src, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, s));
+            shard_compact(src, scap, kcnt, kc[k].max, keys, koff, s);
             hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, s, vals, m);
             radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
                              k == 3 ? fbits : vbits, c.np_tmp.p, s);
@@ -268,6 +285,7 @@ void np_build(Ctx& c, double dhat, double dmin)
     c.coll_dmin = dmin;
     c.coll_valid = true;
     c.hess_valid = false;
+    c.pattern_valid = false;
 }
 
 } // namespace ipcg

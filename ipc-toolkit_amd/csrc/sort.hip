@@ -8,6 +8,7 @@
 //      running per-digit count in LDS; wave counts are prefix-summed across
 //      the block, then every key is written to base + wave prefix + rank.
 // Passes whose digit is constant over all keys are skipped.
+#include "shard.hpp"
 #include "sort.hpp"
 
 namespace ipcg {
@@ -330,6 +331,44 @@ void run_length_encode(const uint64_t* keys, int64_t n, uint64_t* unique_keys,
     exclusive_scan_u32(flags, pos, n, n_unique_dev, stmp, s);
     hipLaunchKernelGGL(k_rle_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, flags,
                        pos, unique_keys, run_start, n_unique_dev);
+    IPCG_CHECK_LAUNCH();
+}
+
+__global__ void k_shard_offsets(const unsigned long long* cnt, int64_t scap, int64_t* off)
+{
+    __shared__ int64_t v[SHARDS];
+    const int t = threadIdx.x;
+    const int64_t c = (int64_t)cnt[(int64_t)t * SHARD_PAD];
+    v[t] = c < scap ? c : scap;
+    __syncthreads();
+    if (t == 0) {
+        int64_t acc = 0;
+        for (int i = 0; i < SHARDS; i++) {
+            off[i] = acc;
+            acc += v[i];
+        }
+        off[SHARDS] = acc;
+    }
+}
+
+ShardCounts shard_counts(const unsigned long long* cnt_dev, void* pinned, hipStream_t s)
+{
+    IPCG_HIP(hipMemcpyAsync(pinned, cnt_dev, SHARD_CNT_BYTES, hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    const unsigned long long* h = static_cast<const unsigned long long*>(pinned);
+    ShardCounts r;
+    for (int i = 0; i < SHARDS; i++) {
+        const int64_t v = (int64_t)h[(int64_t)i * SHARD_PAD];
+        r.total += v;
+        r.max = std::max(r.max, v);
+    }
+    return r;
+}
+
+void shard_offsets(const unsigned long long* cnt_dev, int64_t scap, int64_t* off_dev,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_shard_offsets, dim3(1), dim3(SHARDS), 0, s, cnt_dev, scap, off_dev);
     IPCG_CHECK_LAUNCH();
 }
 

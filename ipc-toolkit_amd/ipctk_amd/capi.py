@@ -139,9 +139,21 @@ class Context:
                                              len(F), pF, pp))
         self.nV, self.E, self.F = len(rest), E, F
 
-    def upload_positions(self, slot, V):
+    def upload_positions(self, slot, V, layout="aos"):
+        """V: (nV, 3) positions; layout "soa" sends them as x[], y[], z[]."""
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        if layout == "soa":
+            V = np.ascontiguousarray(V.T)
         V, pV = _d(V)
-        self._check(self._L.ipcg_positions_upload(self._p, slot, pV, LAYOUT_AOS))
+        self._check(self._L.ipcg_positions_upload(self._p, slot, pV,
+                                                  LAYOUT_SOA if layout == "soa" else LAYOUT_AOS))
+
+    def upload_positions_device(self, slot, ptr, layout="aos"):
+        """ptr: device address of nV x 3 float64 positions (e.g. a torch
+        tensor's data_ptr()), AoS or SoA."""
+        self._check(self._L.ipcg_positions_upload_device(
+            self._p, slot, C.c_void_p(ptr),
+            LAYOUT_SOA if layout == "soa" else LAYOUT_AOS))
 
     # ---- broad phase -------------------------------------------------------
     def broad_phase(self, swept=False, inflation=0.0, method=BP_LBVH):

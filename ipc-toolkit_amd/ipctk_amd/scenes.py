@@ -34,10 +34,10 @@ def grid_sheet(n, z=0.0, offset=(0.0, 0.0), size=1.0):
 
 def edges_from_faces(F):
     F = np.asarray(F, dtype=np.int64)
-    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
-    e.sort(axis=1)
-    e = np.unique(e, axis=0)
-    return e.astype(np.int32)
+    a = np.concatenate([F[:, 0], F[:, 1], F[:, 2]])
+    b = np.concatenate([F[:, 1], F[:, 2], F[:, 0]])
+    key = np.unique((np.minimum(a, b) << 32) | np.maximum(a, b))
+    return np.stack([key >> 32, key & 0xFFFFFFFF], 1).astype(np.int32)
 
 
 def merge(meshes):
@@ -71,13 +71,19 @@ class Scene:
         return len(self.V), len(self.E), len(self.F)
 
 
-def two_sheets(n=64, seed=0, dhat=1e-3, kappa=1e3):
+def two_sheets(n=64, seed=0, dhat=1e-3, kappa=1e3, twist=0.0):
     """Two stacked n x n sheets: bottom at z = 0, top at
     z = 0.5 dhat + U(-0.25, 0.25) dhat per vertex.  Step: the top sheet
-    moves by (0, 0, -1e-3) + U(-1e-4, 1e-4)^3 per vertex."""
+    moves by (0, 0, -1e-3) + U(-1e-4, 1e-4)^3 per vertex.  `twist` rotates
+    the top sheet about the vertical axis through the square's centre (radians;
+    0 keeps its edges parallel to the bottom sheet's)."""
     rng = np.random.default_rng(seed)
     Vb, Fb = grid_sheet(n, 0.0)
     Vt, Ft = grid_sheet(n, 0.0)
+    if twist:
+        c, s_ = math.cos(twist), math.sin(twist)
+        x, y = Vt[:, 0] - 0.5, Vt[:, 1] - 0.5
+        Vt[:, 0], Vt[:, 1] = 0.5 + c * x - s_ * y, 0.5 + s_ * x + c * y
     Vt[:, 2] = 0.5 * dhat + rng.uniform(-0.25, 0.25, len(Vt)) * dhat
     V, E, F = merge([(Vb, Fb), (Vt, Ft)])
     V1 = V.copy()
@@ -86,22 +92,25 @@ def two_sheets(n=64, seed=0, dhat=1e-3, kappa=1e3):
     disp[:, 2] = -1e-3
     disp += rng.uniform(-1e-4, 1e-4, (len(Vt), 3))
     V1[nb:] += disp
-    return Scene(f"two_sheets_{n}", V, E, F, V1, dhat, kappa, seed=seed)
+    name = f"two_sheets_{n}" + (f"_twist{twist:g}" if twist else "")
+    return Scene(name, V, E, F, V1, dhat, kappa, seed=seed)
 
 
 def stacked_sheets(n=512, sheets=8, seed=0, dhat=1e-3, kappa=1e4, spacing=0.6,
-                   jitter=0.2, inplane_sigma=0.5, dz=-0.8):
+                   jitter=0.2, inplane_sigma=0.5, dz=-0.8, h=None):
     """`sheets` n x n sheets stacked in z, sheet k at k * spacing * dhat plus a
     per-vertex z-jitter U(-jitter, jitter) dhat.  "Interleaved": odd sheets
     are shifted by half a grid cell in x and y, so no vertex sits exactly
     above a vertex of the neighbouring sheets.  Step: per-vertex in-plane
-    N(0, (inplane_sigma dhat)^2) and a uniform dz * dhat in z."""
+    N(0, (inplane_sigma dhat)^2) and a uniform dz * dhat in z.  `h` is the
+    grid spacing (default 1/(n-1): unit-square sheets); a smaller n with the
+    default spacing of a larger scene is a tile of that scene's workload."""
     rng = np.random.default_rng(seed)
-    h = 1.0 / (n - 1)
+    h = 1.0 / (n - 1) if h is None else float(h)
     meshes = []
     for k in range(sheets):
         off = (0.5 * h, 0.5 * h) if (k % 2) else (0.0, 0.0)
-        Vk, Fk = grid_sheet(n, 0.0, offset=off)
+        Vk, Fk = grid_sheet(n, 0.0, offset=off, size=h * (n - 1))
         Vk[:, 2] = k * spacing * dhat + rng.uniform(-jitter, jitter, len(Vk)) * dhat
         meshes.append((Vk, Fk))
     V, E, F = merge(meshes)
@@ -139,3 +148,42 @@ def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
         F = newF
     V = np.array(V) * radius + np.asarray(center, dtype=np.float64)
     return V, np.array(F, dtype=np.int32)
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q *= np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def sphere_chain(count=2, subdiv=3, radius=0.1, seed=0, dhat=1e-3, kappa=1e3,
+                 approach=1.5, noise=0.1):
+    """`count` randomly rotated icospheres of one subdivision level in a row
+    along x, neighbouring surfaces U(0.2, 1.5) dhat apart.  Step: sphere k
+    moves by (c - k) * approach * dhat in x towards the chain centre c, plus a
+    per-vertex N(0, (noise dhat)^2) perturbation, so neighbours close their
+    gap and pass into each other."""
+    rng = np.random.default_rng(seed)
+    Vs, Fs = icosphere(subdiv, radius)
+    # the surface gap uses the polyhedron's extent along x, not the sphere's
+    meshes, x = [], 0.0
+    shifts = []
+    for k in range(count):
+        R = _rotation(rng)
+        Vk = Vs @ R.T
+        lo, hi = Vk[:, 0].min(), Vk[:, 0].max()
+        if k:
+            x += rng.uniform(0.2, 1.5) * dhat
+        Vk = Vk + np.array([x - lo, 0.0, 0.0])
+        x += hi - lo
+        meshes.append((Vk, Fs))
+        shifts.append(((count - 1) / 2.0 - k) * approach * dhat)
+    V, E, F = merge(meshes)
+    V1 = V.copy()
+    nv = len(Vs)
+    for k in range(count):
+        V1[k * nv:(k + 1) * nv, 0] += shifts[k]
+    V1 += rng.normal(0.0, noise * dhat, V1.shape)
+    return Scene(f"sphere_chain_{count}x{subdiv}", V, E, F, V1, dhat, kappa, seed=seed)

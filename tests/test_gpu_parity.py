@@ -1,0 +1,215 @@
+"""Device path (libipcg.so through its C ABI) against the CPU oracle.
+
+Integer and index results (candidate sets, collision sets, dtypes, weights,
+mollifier thresholds, per-candidate TOIs, step sizes) must be bit-identical.
+Floating-point reductions may differ in summation order only: energy and
+gradient within 1e-12 relative, Hessians within 1e-10 relative (Frobenius
+norm over the union of both sparsity patterns), the contract's tolerance."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import pair_set, rel_err, sparse_rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_SUM = 1e-12
+TOL_HESS = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def tet():
+    V0 = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    V1 = V0.copy()
+    V1[3] += [0.1, -0.1, 0]
+    V1[3, 2] = -0.5e-3
+    E = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]], dtype=np.int32)
+    F = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], dtype=np.int32)
+    from ipctk_amd.scenes import Scene
+    return Scene("tet", V0, E, F, V1, 1e-3, 1e3)
+
+
+def make(scenes, name):
+    if name == "tet":
+        return tet()
+    kind, *args = name.split(":")
+    if kind == "sheets":
+        return scenes.two_sheets(int(args[0]), seed=int(args[1]))
+    if kind == "twisted":
+        return scenes.two_sheets(int(args[0]), seed=int(args[1]), twist=0.3)
+    if kind == "spheres":
+        return scenes.sphere_chain(int(args[0]), int(args[1]), seed=int(args[2]))
+    if kind == "stack":
+        return scenes.stacked_sheets(int(args[0]), 8, seed=int(args[1]), h=1.0 / 511)
+    raise ValueError(name)
+
+
+SCENES = ["tet", "sheets:16:0", "twisted:12:1", "spheres:4:3:0", "stack:32:0"]
+
+
+def upload(ctx, s):
+    ctx.upload_mesh(s.V, s.E, s.F)
+    ctx.upload_positions(0, s.V)
+    ctx.upload_positions(1, s.V1)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_candidates(ctx, O, scenes, name):
+    s = make(scenes, name)
+    say("case", name, s.sizes)
+    upload(ctx, s)
+    for swept in (False, True):
+        r = 0.0 if swept else 0.5 * s.dhat
+        ctx.broad_phase(swept, r)
+        ee_g, fv_g = ctx.candidates()
+        ee_o, fv_o = O.candidates(s.V, s.E, s.F, r, V1=s.V1 if swept else None)
+        # EE pairs are oriented (lower Morton position first): compare oriented
+        assert pair_set(ee_g) == pair_set(ee_o)
+        assert len(ee_g) == len(ee_o)
+        assert pair_set(fv_g) == pair_set(fv_o) and len(fv_g) == len(fv_o)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_collisions_and_potentials(ctx, O, scenes, name):
+    s = make(scenes, name)
+    say("case", name, s.sizes)
+    upload(ctx, s)
+    ctx.build_collisions(s.dhat)
+    cg = ctx.collisions()
+    ee, fv = O.candidates(s.V, s.E, s.F, 0.5 * s.dhat)
+    co = O.build_collisions(s.V, s.E, s.F, ee, fv, s.dhat)
+    for k in ("vv", "ev", "ee", "fv"):
+        assert np.array_equal(getattr(co, k), cg[k]), k
+        assert np.array_equal(getattr(co, k + "_w"), cg[k + "_w"]), k
+    assert np.array_equal(co.ee_eps_x, cg["ee_eps_x"])
+    assert np.array_equal(co.ee_dtype, cg["ee_dtype"])
+    Eo = O.barrier_energy(co, s.V, s.E, s.F, s.dhat, s.kappa)
+    Eg = ctx.energy(s.dhat, s.kappa)
+    assert abs(Eg - Eo) <= TOL_SUM * abs(Eo) + 1e-300
+    go = O.barrier_gradient(co, s.V, s.E, s.F, s.dhat, s.kappa)
+    gg = ctx.gradient(s.dhat, s.kappa)
+    assert rel_err(gg, go) <= TOL_SUM
+    for psd in ("none", "clamp", "abs"):
+        Ho = O.barrier_hessian(co, s.V, s.E, s.F, s.dhat, s.kappa, psd)
+        Hg = ctx.hessian(s.dhat, s.kappa, psd)
+        assert Hg.shape == Ho.shape
+        assert sparse_rel_err(Hg, Ho) <= TOL_HESS, psd
+        # CSR well formed: sorted unique columns per row
+        for r in range(0, Hg.shape[0], max(1, Hg.shape[0] // 97)):
+            cols = Hg.indices[Hg.indptr[r]:Hg.indptr[r + 1]]
+            assert np.all(np.diff(cols) > 0)
+
+
+@pytest.mark.parametrize("name", ["tet", "spheres:4:3:0", "spheres:2:3:5", "stack:24:1"])
+def test_per_candidate_toi(ctx, O, scenes, name):
+    s = make(scenes, name)
+    say("case", name, s.sizes)
+    upload(ctx, s)
+    ctx.broad_phase(True, 0.0)
+    ee, fv = ctx.candidates()
+    te, tf = ctx.candidates_toi()
+    oe = O.candidates_toi(s.V, s.V1, s.E, s.F, ee, True)
+    of = O.candidates_toi(s.V, s.V1, s.E, s.F, fv, False)
+    assert np.array_equal(te, oe) and np.array_equal(tf, of)
+
+
+@pytest.mark.parametrize("name", ["tet", "sheets:64:0", "twisted:12:1", "spheres:4:3:0",
+                                  "stack:32:0"])
+def test_stepsize(ctx, O, scenes, name):
+    s = make(scenes, name)
+    say("case", name, s.sizes)
+    upload(ctx, s)
+    t = ctx.stepsize()
+    assert t == O.compute_collision_free_stepsize(s.V, s.V1, s.E, s.F)
+    assert O.is_step_collision_free(s.V, s.V + t * (s.V1 - s.V), s.E, s.F)
+    assert ctx.is_step_collision_free() == O.is_step_collision_free(s.V, s.V1, s.E, s.F)
+
+
+def test_min_distance_stepsize(ctx, O, scenes):
+    s = make(scenes, "spheres:4:3:0")
+    say("case min_distance", s.sizes)
+    upload(ctx, s)
+    for md in (1e-5, 2e-4):
+        assert ctx.stepsize(min_distance=md) == O.compute_collision_free_stepsize(
+            s.V, s.V1, s.E, s.F, min_distance=md)
+
+
+def test_layouts_and_device_upload(ctx, scenes):
+    """SoA, AoS and device-resident uploads give identical results"""
+    s = make(scenes, "twisted:12:1")
+    say("case layouts", s.sizes)
+    out = []
+    for mode in ("aos", "soa", "device"):
+        ctx.upload_mesh(s.V, s.E, s.F)
+        if mode == "device":
+            import torch
+            t0 = torch.tensor(s.V, device="cuda:0")
+            t1 = torch.tensor(np.ascontiguousarray(s.V1.T), device="cuda:0")
+            torch.cuda.synchronize()
+            ctx.upload_positions_device(0, t0.data_ptr(), "aos")
+            ctx.upload_positions_device(1, t1.data_ptr(), "soa")
+            ctx.synchronize()
+        else:
+            ctx.upload_positions(0, s.V, mode)
+            ctx.upload_positions(1, s.V1, mode)
+        ctx.build_collisions(s.dhat)
+        out.append((ctx.collision_counts().tolist(), ctx.energy(s.dhat, s.kappa),
+                    ctx.gradient(s.dhat, s.kappa).tolist(), ctx.stepsize()))
+    assert out[0] == out[1] == out[2]
+
+
+def test_vertex_patch_filter(capi, O, scenes):
+    s = make(scenes, "sheets:16:0")
+    group = (np.arange(len(s.V)) >= len(s.V) // 2).astype(np.int32)
+    say("case patch filter", s.sizes)
+    ctx = capi.Context(0)
+    try:
+        ctx.upload_mesh(s.V, s.E, s.F, vertex_patch=group)
+        ctx.upload_positions(0, s.V)
+        ctx.broad_phase(False, 0.5 * s.dhat)
+        ee_g, fv_g = ctx.candidates()
+        ee_o, fv_o = O.candidates(s.V, s.E, s.F, 0.5 * s.dhat, vertex_group=group)
+        assert pair_set(ee_g) == pair_set(ee_o) and pair_set(fv_g) == pair_set(fv_o)
+    finally:
+        ctx.close()
+
+
+def test_empty_and_small_meshes(ctx, O):
+    """no candidates, a lone triangle, an edge soup without faces"""
+    T = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    F = np.array([[0, 1, 2]], dtype=np.int32)
+    E = np.array([[0, 1], [0, 2], [1, 2]], dtype=np.int32)
+    for V, EE, FF in ((T, E, F), (np.vstack([T, T + [0, 0, 5]]),
+                                 np.vstack([E, E + 3]), np.vstack([F, F + 3])),
+                      (np.array([[0.0, 0, 0], [1, 0, 0], [0.5, -0.5, 1e-4], [0.5, 0.5, 1e-4]]),
+                       np.array([[0, 1], [2, 3]], dtype=np.int32), np.zeros((0, 3), np.int32))):
+        say("case small", len(V), len(EE), len(FF))
+        ctx.upload_mesh(V, EE, FF)
+        ctx.upload_positions(0, V)
+        ctx.upload_positions(1, V + [0, 0, -2e-4])
+        ctx.build_collisions(1e-3)
+        ee, fv = O.candidates(V, EE, FF, 0.5e-3)
+        co = O.build_collisions(V, EE, FF, ee, fv, 1e-3)
+        assert int(ctx.collision_counts().sum()) == len(co)
+        assert ctx.energy(1e-3, 1.0) == pytest.approx(
+            O.barrier_energy(co, V, EE, FF, 1e-3, 1.0), rel=1e-12, abs=0)
+        H = ctx.hessian(1e-3, 1.0, "clamp")
+        assert H.shape == (3 * len(V), 3 * len(V))
+        assert ctx.stepsize() == O.compute_collision_free_stepsize(V, V + [0, 0, -2e-4], EE, FF)
+
+
+def test_errors_are_reported(ctx, capi):
+    with pytest.raises(capi.IPCGError):
+        ctx.energy(1e-3, 1.0)  # no collision set yet
+    with pytest.raises(capi.IPCGError):
+        ctx.upload_positions(0, np.zeros((3, 3)))  # no mesh
+    ctx.upload_mesh(np.zeros((3, 3)), np.array([[0, 1]]), np.zeros((0, 3), np.int32))
+    with pytest.raises(capi.IPCGError):
+        ctx.broad_phase(False, -1.0)
+    with pytest.raises(capi.IPCGError):
+        ctx.stepsize(rescale=0.0)
