@@ -414,6 +414,171 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sweep and prune (sweep_and_prune.cpp:22-249 over scalable-ccd's
+// sort_and_sweep): FP64 boxes sorted by (min along the sweep axis, id); a
+// lane per sorted box scans forward while the next box starts within its
+// extent along the axis and reports full 3D overlaps that pass the filters.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ordered_bits(double x)
+{
+    x = x + 0.0; // -0 -> +0: equal keys stay equal
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+template <int ARITY>
+__global__ __launch_bounds__(BP_BLOCK) void k_sap_boxes(const double* X0, const double* X1,
+                                                        int nV, const int32_t* I, int n, double r,
+                                                        int axis, double* box64, uint64_t* keys,
+                                                        uint32_t* ids)
+{
+    const int p = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (p >= n) return;
+    double lo[3], hi[3];
+    vbox(X0, X1, nV, ARITY == 1 ? p : I[(int64_t)ARITY * p], r, lo, hi);
+#pragma unroll
+    for (int j = 1; j < ARITY; j++) {
+        double l[3], h[3];
+        vbox(X0, X1, nV, I[(int64_t)ARITY * p + j], r, l, h);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = l[k] < lo[k] ? l[k] : lo[k];
+            hi[k] = hi[k] < h[k] ? h[k] : hi[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        box64[(int64_t)6 * p + k] = lo[k];
+        box64[(int64_t)6 * p + 3 + k] = hi[k];
+    }
+    keys[p] = ordered_bits(lo[axis]);
+    ids[p] = (uint32_t)p;
+}
+
+// boxes in sorted order (coalesced sweeps)
+__global__ __launch_bounds__(BP_BLOCK) void k_sap_gather(const double* box64, const uint32_t* ids,
+                                                         int n, double* sorted)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = ids[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) sorted[(int64_t)6 * i + k] = box64[(int64_t)6 * p + k];
+}
+
+__device__ __forceinline__ bool overlap64(const double* a, const double* b)
+{
+    return a[0] <= b[3] && a[1] <= b[4] && a[2] <= b[5] && b[0] <= a[3] && b[1] <= a[4]
+        && b[2] <= a[5];
+}
+
+// Edge-edge: pairs (earlier, later) in sorted order.
+__global__ __launch_bounds__(TR_BLOCK) void k_sap_self(const double* sb, const uint32_t* ids, int n,
+                                                       int axis, Filter flt, int2* out, Shards S)
+{
+    const int i = blockIdx.x * TR_BLOCK + threadIdx.x;
+    bool active = i < n;
+    double a[6] = { 0, 0, 0, 0, 0, 0 };
+    int ai = 0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) a[k] = sb[(int64_t)6 * i + k];
+        ai = (int)ids[i];
+    }
+    const int sh = wave_shard();
+    int j = i + 1;
+    while (__any(active)) {
+        int hit = -1;
+        if (active) {
+            if (j >= n || sb[(int64_t)6 * j + axis] > a[3 + axis]) {
+                active = false;
+            } else {
+                const int bj = (int)ids[j];
+                if (overlap64(a, sb + (int64_t)6 * j) && can_ee(flt, ai, bj)) hit = bj;
+                j++;
+            }
+        }
+        const int64_t slot = (int64_t)shard_append(S, sh, hit >= 0 ? 1u : 0u);
+        if (hit >= 0 && slot < S.scap) out[sh * S.scap + slot] = make_int2(ai, hit);
+    }
+}
+
+// Face-vertex from one side: each query box (sorted, `qa`) scans the other
+// list (`sb`) from the first box starting at (STRICT: after) its own start.
+// FACES: the queries are faces; otherwise vertices.
+template <bool FACES, bool STRICT>
+__global__ __launch_bounds__(TR_BLOCK) void k_sap_cross(const double* qa, const uint32_t* qids,
+                                                        int nq, const double* sb,
+                                                        const uint32_t* sids, int ns, int axis,
+                                                        Filter flt, int2* out, Shards S)
+{
+    const int i = blockIdx.x * TR_BLOCK + threadIdx.x;
+    bool active = i < nq;
+    double a[6] = { 0, 0, 0, 0, 0, 0 };
+    int ai = 0, j = 0;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) a[k] = qa[(int64_t)6 * i + k];
+        ai = (int)qids[i];
+        int lo = 0, hi = ns;
+        while (lo < hi) {
+            const int m = (lo + hi) >> 1;
+            const double km = sb[(int64_t)6 * m + axis];
+            if (STRICT ? km <= a[axis] : km < a[axis]) lo = m + 1;
+            else hi = m;
+        }
+        j = lo;
+    }
+    const int sh = wave_shard();
+    while (__any(active)) {
+        int2 hit = make_int2(-1, -1);
+        if (active) {
+            if (j >= ns || sb[(int64_t)6 * j + axis] > a[3 + axis]) {
+                active = false;
+            } else {
+                const int bj = (int)sids[j];
+                const int f = FACES ? ai : bj, v = FACES ? bj : ai;
+                if (overlap64(a, sb + (int64_t)6 * j) && can_fv(flt, f, v)) hit = make_int2(f, v);
+                j++;
+            }
+        }
+        const int64_t slot = (int64_t)shard_append(S, sh, hit.x >= 0 ? 1u : 0u);
+        if (hit.x >= 0 && slot < S.scap) out[sh * S.scap + slot] = hit;
+    }
+}
+
+void sap_list(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const double* X0,
+              const double* X1, double r, int axis)
+{
+    hipStream_t s = c.stream;
+    T.n = n;
+    if (n <= 0) return;
+    T.box.reserve(sizeof(double) * 6 * (size_t)n);
+    T.sbox.reserve(sizeof(double) * 6 * (size_t)n);
+    T.keys.reserve(sizeof(uint64_t) * (size_t)n);
+    T.keys2.reserve(sizeof(uint64_t) * (size_t)n);
+    T.ids.reserve(sizeof(uint32_t) * (size_t)n);
+    T.ids2.reserve(sizeof(uint32_t) * (size_t)n);
+    const unsigned g = grid_for(n, BP_BLOCK);
+    if (arity == 1)
+        hipLaunchKernelGGL(k_sap_boxes<1>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
+                           axis, T.box.as<double>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+    else if (arity == 2)
+        hipLaunchKernelGGL(k_sap_boxes<2>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
+                           axis, T.box.as<double>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_sap_boxes<3>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
+                           axis, T.box.as<double>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+    IPCG_CHECK_LAUNCH();
+    c.sort_tmp.reserve(radix_tmp_bytes(n));
+    radix_sort_pairs(T.keys.as<uint64_t>(), T.ids.as<uint32_t>(), T.keys2.as<uint64_t>(),
+                     T.ids2.as<uint32_t>(), n, 64, c.sort_tmp.p, s);
+    hipLaunchKernelGGL(k_sap_gather, dim3(g), dim3(BP_BLOCK), 0, s, T.box.as<double>(),
+                       T.ids.as<uint32_t>(), n, T.sbox.as<double>());
+    IPCG_CHECK_LAUNCH();
+}
+
 void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const double* X0,
                 const double* X1, double r, bool with_nodes)
 {
@@ -495,7 +660,9 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     c.require_mesh();
     c.require_pos(0);
     if (swept) c.require_pos(1);
-    if (method != IPCG_BP_LBVH) throw Error(IPCG_ERR_ARG, "unsupported broad phase method");
+    if (method != IPCG_BP_LBVH && method != IPCG_BP_SAP)
+        throw Error(IPCG_ERR_ARG, "unsupported broad phase method");
+    const bool sap = method == IPCG_BP_SAP;
     hipStream_t s = c.stream;
     const double* X0 = c.X[0].as<double>();
     const double* X1 = swept ? c.X[1].as<double>() : nullptr;
@@ -517,10 +684,23 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                        nb, c.aabb.as<double>());
     IPCG_CHECK_LAUNCH();
     c.stage_end();
-    // vertex order (queries of the face tree), edge tree, face tree
-    build_tree(c, c.tv, nullptr, 1, c.nV, X0, X1, inflation, false);
-    build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, true);
-    build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true);
+    int axis = 0;
+    if (sap) {
+        // sweep axis: largest extent of the mesh box
+        double* hb = static_cast<double*>(c.pinned);
+        IPCG_HIP(hipMemcpyAsync(hb, c.aabb.p, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        for (int k = 1; k < 3; k++)
+            if (hb[3 + k] - hb[k] > hb[3 + axis] - hb[axis]) axis = k;
+        sap_list(c, c.tv, nullptr, 1, c.nV, X0, X1, inflation, axis);
+        sap_list(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, axis);
+        sap_list(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, axis);
+    } else {
+        // vertex order (queries of the face tree), edge tree, face tree
+        build_tree(c, c.tv, nullptr, 1, c.nV, X0, X1, inflation, false);
+        build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, true);
+        build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true);
+    }
 
     c.counters.reserve(sizeof(unsigned long long) * 4);
     Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr };
@@ -538,6 +718,25 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         IPCG_HIP(hipMemsetAsync(c.shard_cnt.p, 0, 2 * SHARD_CNT_BYTES, s));
         IPCG_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
         c.stage_begin("bp_traverse");
+        if (sap) {
+            const Shards S0 { cnt[0], c.scap_ee }, S1 { cnt[1], c.scap_fv };
+            if (c.nE >= 2)
+                hipLaunchKernelGGL(k_sap_self, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0, s,
+                                   c.te.sbox.as<double>(), c.te.ids.as<uint32_t>(), c.nE, axis,
+                                   flt, c.cand_stage[0].as<int2>(), S0);
+            if (c.nF >= 1) {
+                hipLaunchKernelGGL((k_sap_cross<true, false>), dim3(grid_for(c.nF, TR_BLOCK)),
+                                   dim3(TR_BLOCK), 0, s, c.tf.sbox.as<double>(),
+                                   c.tf.ids.as<uint32_t>(), c.nF, c.tv.sbox.as<double>(),
+                                   c.tv.ids.as<uint32_t>(), c.nV, axis, flt,
+                                   c.cand_stage[1].as<int2>(), S1);
+                hipLaunchKernelGGL((k_sap_cross<false, true>), dim3(grid_for(c.nV, TR_BLOCK)),
+                                   dim3(TR_BLOCK), 0, s, c.tv.sbox.as<double>(),
+                                   c.tv.ids.as<uint32_t>(), c.nV, c.tf.sbox.as<double>(),
+                                   c.tf.ids.as<uint32_t>(), c.nF, axis, flt,
+                                   c.cand_stage[1].as<int2>(), S1);
+            }
+        } else {
         if (c.nE >= 2)
             hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0,
                                s, c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
@@ -550,6 +749,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(),
                                c.tf.ids.as<uint32_t>(), c.tf.box.as<float>(), c.nF, flt,
                                c.cand_stage[1].as<int2>(), Shards { cnt[1], c.scap_fv }, status);
+        }
         IPCG_CHECK_LAUNCH();
         c.stage_end();
         int* hs = static_cast<int*>(c.pinned);

@@ -164,7 +164,8 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.blk_tmp, &c.blk_mask, &c.row_cnt, &c.rowptr, &c.col, &c.val, &c.ccd_bound,
                        &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status,
                        &c.hp_mark, &c.hp_bptr, &c.hp_bcol, &c.cand_stage[0], &c.cand_stage[1],
-                       &c.shard_cnt, &c.shard_off, &c.np_eec };
+                       &c.shard_cnt, &c.shard_off, &c.np_eec, &c.ccd_stage, &c.ccd_todo,
+                       &c.tv.sbox, &c.te.sbox, &c.tf.sbox };
     for (DevBuf* b : bufs) b->release();
     for (auto& p : c.timer.pending) {
         (void)hipEventDestroy(p.a);

@@ -17,6 +17,7 @@
 // non-negative double (unsigned atomic min); every search hides outcomes at
 // or above it, which leaves the final minimum independent of scheduling.
 #include "context.hpp"
+#include "shard.hpp"
 #include "sort.hpp"
 #include "ti.hpp"
 
@@ -115,11 +116,39 @@ __device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double&
 }
 
 // ---------------------------------------------------------------------------
+// root pass: one lane per candidate settles the strategy prologue and the
+// first level of the search (the inclusion test of the whole box [0,1]^3);
+// only candidates whose root box includes the origin go on to a search.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t* stage)
+{
+    const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool search = false;
+    if (ci < A.n_ee + A.n_fv) {
+        TIQuery q;
+        double eff = 0, adj = 0;
+        const int pre = candidate_setup(A, ci, q, eff, adj);
+        if (pre >= 0) {
+            report(A, ci, pre == 1, 0.0);
+        } else {
+            ti_filter(q, eff > 0);
+            bool box_in;
+            double w[3];
+            search = ti_inclusion(q, TIBox { 0.0, 0.0, 0.0 }, 1.0, 1.0, 1.0, eff, box_in, w);
+            if (!search) report(A, ci, false, INFINITY);
+        }
+    }
+    const int sh = wave_shard();
+    const int64_t slot = (int64_t)shard_append(S, sh, search ? 1u : 0u);
+    if (search && slot < S.scap) stage[sh * S.scap + slot] = (uint32_t)ci;
+}
+
+// ---------------------------------------------------------------------------
 // lane path
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, TIBox* slab,
-                                                  unsigned long long* work, uint32_t* heavy,
-                                                  unsigned long long* n_heavy)
+__global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, const uint32_t* todo, int64_t n_todo,
+                                                  TIBox* slab, unsigned long long* work,
+                                                  uint32_t* heavy, unsigned long long* n_heavy)
 {
     const int64_t lane = (int64_t)blockIdx.x * CB + threadIdx.x;
     TISlab S;
@@ -127,12 +156,12 @@ __global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, TIBox* slab,
     S.cur = slab + lane * 3 * (int64_t)LANE_CAP;
     S.lo = S.cur + LANE_CAP;
     S.hi = S.lo + LANE_CAP;
-    const int64_t n = A.n_ee + A.n_fv;
     for (;;) {
         if (A.any_hit && __hip_atomic_load(A.any_hit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             break;
-        const int64_t ci = (int64_t)atomicAdd(work, 1ull);
-        if (ci >= n) break;
+        const int64_t k = (int64_t)atomicAdd(work, 1ull);
+        if (k >= n_todo) break;
+        const int64_t ci = todo[k];
         TIQuery q;
         double eff = 0, adj = 0, toi = INFINITY;
         const int pre = candidate_setup(A, ci, q, eff, adj);
@@ -625,11 +654,34 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     A.any_hit = early_exit ? any_hit : nullptr;
     unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
 
-    c.stage_begin("ccd_lanes");
-    const int64_t ln = std::min<int64_t>(LANES, ((n + CB - 1) / CB) * CB);
-    hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A,
-                       c.ccd_slab.as<TIBox>(), work, heavy, n_heavy);
+    // root pass -> candidates that need a search (sharded, then packed)
+    c.stage_begin("ccd_root");
+    const int64_t waves = (n + WAVE - 1) / WAVE;
+    const int64_t scap = std::max<int64_t>(WAVE * ((waves + SHARDS - 1) / SHARDS), WAVE);
+    c.ccd_stage.reserve(sizeof(uint32_t) * (size_t)(SHARDS * scap));
+    c.shard_cnt.reserve(SHARD_CNT_BYTES);
+    c.shard_off.reserve(sizeof(int64_t) * (SHARDS + 1));
+    IPCG_HIP(hipMemsetAsync(c.shard_cnt.p, 0, SHARD_CNT_BYTES, s));
+    const Shards RS { c.shard_cnt.as<unsigned long long>(), scap };
+    hipLaunchKernelGGL(k_ccd_root, dim3(grid_for(n, 256, INT32_MAX)), dim3(256), 0, s, A, RS,
+                       c.ccd_stage.as<uint32_t>());
     IPCG_CHECK_LAUNCH();
+    const ShardCounts rc = shard_counts(RS.cnt, c.pinned, s);
+    if (rc.max > scap) throw Error(IPCG_ERR_NUMERIC, "CCD root pass outgrew its shard");
+    c.ccd_todo.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(rc.total, 1));
+    shard_compact(c.ccd_stage.as<uint32_t>(), scap, RS.cnt, rc.max, c.ccd_todo.as<uint32_t>(),
+                  c.shard_off.as<int64_t>(), s);
+    c.stage_end();
+    const int64_t n_todo = rc.total;
+
+    c.stage_begin("ccd_lanes");
+    if (n_todo > 0) {
+        const int64_t ln = std::min<int64_t>(LANES, ((n_todo + CB - 1) / CB) * CB);
+        hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A,
+                           c.ccd_todo.as<uint32_t>(), n_todo, c.ccd_slab.as<TIBox>(), work, heavy,
+                           n_heavy);
+        IPCG_CHECK_LAUNCH();
+    }
     c.stage_end();
     IPCG_HIP(hipMemcpyAsync(h, n_heavy, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     IPCG_HIP(hipStreamSynchronize(s));

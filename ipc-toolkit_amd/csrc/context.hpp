@@ -32,12 +32,13 @@ static_assert(sizeof(Node) == 64, "Node is one 64-byte line");
 
 struct Tree {
     int32_t n = 0;       // leaves
-    DevBuf box;          // float[6] per primitive (original order)
+    DevBuf box;          // float[6] (LBVH) or double[6] (sweep) per primitive
     DevBuf keys, keys2;  // uint64 Morton codes (sorted in place)
     DevBuf ids, ids2;    // uint32 primitive id per sorted position
     DevBuf nodes;        // Node[n-1]
     DevBuf info;         // construction counters / ranges
     DevBuf root;         // int32 root node (-1: single leaf)
+    DevBuf sbox;         // sweep and prune: FP64 boxes in sorted order
 };
 
 struct Ctx {
@@ -83,6 +84,7 @@ struct Ctx {
 
     // ccd
     DevBuf ccd_bound, ccd_slab, ccd_overflow, ccd_toi, ccd_status;
+    DevBuf ccd_stage, ccd_todo; // root-pass survivors (sharded, packed)
 
     // host pinned staging (PINNED_BYTES)
     void* pinned = nullptr;

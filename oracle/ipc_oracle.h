@@ -149,6 +149,15 @@ int ipco_candidates(const double *V0, const double *V1, int nV,
                     const int32_t *E, int nE, const int32_t *F, int nF,
                     double inflation, const int32_t *vertex_group,
                     ipco_pairs *ee, ipco_pairs *fv);
+/* Sweep and prune over FP64 boxes (sweep_and_prune.cpp:22-249; sort and
+ * sweep restated from scalable-ccd): same filters, edge-edge pairs as
+ * (earlier, later) in (min along the sweep axis, id) order.  The sweep axis
+ * is the largest extent of the mesh box (ipco_sap_axis). */
+int ipco_candidates_sap(const double *V0, const double *V1, int nV,
+                        const int32_t *E, int nE, const int32_t *F, int nF,
+                        double inflation, const int32_t *vertex_group,
+                        ipco_pairs *ee, ipco_pairs *fv);
+int ipco_sap_axis(const double *V0, const double *V1, int nV, double inflation);
 void ipco_pairs_free(ipco_pairs *p);
 
 /* ---- narrow phase -------------------------------------------------------- */

@@ -451,6 +451,158 @@ int ipco_candidates(const double *V0, const double *V1, int nV,
     return 0;
 }
 
+/* ---- sweep and prune (sweep_and_prune.cpp:22-249) ------------------------
+ * The reference delegates to scalable-ccd's sort_and_sweep (c80af01, not in
+ * the reference tree).  Restated behaviour: FP64 boxes (the conservative
+ * boxes above), sorted by (min along the sweep axis, id); a pair is
+ * reported when its boxes overlap on all three axes and it passes the
+ * adjacency and patch filters (sweep_and_prune.cpp:186-249).  The sweep axis
+ * is the axis of largest mesh-box extent (first on ties).  Edge-edge pairs
+ * are reported as (earlier, later) in the sorted order; face-vertex pairs
+ * as (face, vertex). */
+typedef struct skey {
+    double key;
+    int32_t id;
+} skey;
+
+static int cmp_skey(const void *a, const void *b)
+{
+    const skey *x = (const skey *)a, *y = (const skey *)b;
+    if (x->key < y->key) return -1;
+    if (x->key > y->key) return 1;
+    return (x->id > y->id) - (x->id < y->id);
+}
+
+static inline int overlap64(const box64 *a, const box64 *b)
+{
+    return a->lo[0] <= b->hi[0] && a->lo[1] <= b->hi[1] && a->lo[2] <= b->hi[2]
+        && b->lo[0] <= a->hi[0] && b->lo[1] <= a->hi[1] && b->lo[2] <= a->hi[2];
+}
+
+static skey *sorted_keys(const box64 *b, int n, int axis)
+{
+    skey *k = (skey *)malloc(sizeof(skey) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) {
+        k[i].key = b[i].lo[axis];
+        k[i].id = i;
+    }
+    qsort(k, (size_t)n, sizeof(skey), cmp_skey);
+    return k;
+}
+
+static void gather_pairs(pairvec *loc, int nt, ipco_pairs *out)
+{
+    int64_t tot = 0;
+    for (int k = 0; k < nt; k++) tot += loc[k].n;
+    out->n = tot;
+    out->ij = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(tot > 0 ? tot : 1));
+    int64_t o = 0;
+    for (int k = 0; k < nt; k++) {
+        if (loc[k].n)
+            memcpy(out->ij + 2 * o, loc[k].ij, sizeof(int32_t) * 2 * (size_t)loc[k].n);
+        o += loc[k].n;
+        free(loc[k].ij);
+    }
+    free(loc);
+}
+
+int ipco_sap_axis(const double *V0, const double *V1, int nV, double inflation)
+{
+    box64 *vb = vertex_boxes64(V0, V1, nV, inflation);
+    double mn[3] = { INFINITY, INFINITY, INFINITY };
+    double mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = 0; i < nV; i++)
+        for (int k = 0; k < 3; k++) {
+            if (vb[i].lo[k] < mn[k]) mn[k] = vb[i].lo[k];
+            if (mx[k] < vb[i].hi[k]) mx[k] = vb[i].hi[k];
+        }
+    free(vb);
+    int axis = 0;
+    for (int k = 1; k < 3; k++)
+        if (mx[k] - mn[k] > mx[axis] - mn[axis]) axis = k;
+    return axis;
+}
+
+int ipco_candidates_sap(const double *V0, const double *V1, int nV,
+                        const int32_t *E, int nE, const int32_t *F, int nF,
+                        double inflation, const int32_t *vertex_group,
+                        ipco_pairs *ee, ipco_pairs *fv)
+{
+    ee->n = fv->n = 0;
+    ee->ij = fv->ij = NULL;
+    if (nV <= 0) return 0;
+    const int axis = ipco_sap_axis(V0, V1, nV, inflation);
+    box64 *vb = vertex_boxes64(V0, V1, nV, inflation);
+    box64 *eb = element_boxes64(vb, E, nE, 2);
+    box64 *fb = element_boxes64(vb, F, nF, 3);
+    filt f = { E, F, vertex_group };
+    const int nt = ipco_get_max_threads();
+    /* edge-edge: one list */
+    skey *ek = sorted_keys(eb, nE, axis);
+    pairvec *loc = (pairvec *)calloc((size_t)nt, sizeof(pairvec));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < nE; i++) {
+            const box64 *a = &eb[ek[i].id];
+            for (int j = i + 1; j < nE && ek[j].key <= a->hi[axis]; j++)
+                if (overlap64(a, &eb[ek[j].id]) && can_ee(&f, ek[i].id, ek[j].id))
+                    pv_push(&loc[tid], ek[i].id, ek[j].id);
+        }
+    }
+    gather_pairs(loc, nt, ee);
+    /* face-vertex: two lists; a pair is found from the box that starts
+     * first along the axis (ties: from the face) */
+    skey *fk = sorted_keys(fb, nF, axis);
+    skey *vk = sorted_keys(vb, nV, axis);
+    loc = (pairvec *)calloc((size_t)nt, sizeof(pairvec));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < nF; i++) {
+            const box64 *a = &fb[fk[i].id];
+            int lo = 0, hi = nV; /* first vertex with key >= face min */
+            while (lo < hi) {
+                const int m = (lo + hi) / 2;
+                if (vk[m].key < a->lo[axis]) lo = m + 1;
+                else hi = m;
+            }
+            for (int j = lo; j < nV && vk[j].key <= a->hi[axis]; j++)
+                if (overlap64(a, &vb[vk[j].id]) && can_fv(&f, fk[i].id, vk[j].id))
+                    pv_push(&loc[tid], fk[i].id, vk[j].id);
+        }
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < nV; i++) {
+            const box64 *a = &vb[vk[i].id];
+            int lo = 0, hi = nF; /* first face with key > vertex min */
+            while (lo < hi) {
+                const int m = (lo + hi) / 2;
+                if (fk[m].key <= a->lo[axis]) lo = m + 1;
+                else hi = m;
+            }
+            for (int j = lo; j < nF && fk[j].key <= a->hi[axis]; j++)
+                if (overlap64(a, &fb[fk[j].id]) && can_fv(&f, fk[j].id, vk[i].id))
+                    pv_push(&loc[tid], fk[j].id, vk[i].id);
+        }
+    }
+    gather_pairs(loc, nt, fv);
+    free(ek);
+    free(fk);
+    free(vk);
+    free(vb);
+    free(eb);
+    free(fb);
+    return 0;
+}
+
 void ipco_pairs_free(ipco_pairs *p)
 {
     free(p->ij);

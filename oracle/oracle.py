@@ -102,6 +102,9 @@ def lib():
                                            C.c_double, _F, _F, _F]
         L.ipco_candidates.argtypes = [_D, _D, C.c_int, _I, C.c_int, _I, C.c_int,
                                       C.c_double, _I, C.POINTER(_Pairs), C.POINTER(_Pairs)]
+        L.ipco_candidates_sap.argtypes = L.ipco_candidates.argtypes
+        L.ipco_sap_axis.argtypes = [_D, _D, C.c_int, C.c_double]
+        L.ipco_sap_axis.restype = C.c_int
         L.ipco_pairs_free.argtypes = [C.POINTER(_Pairs)]
         L.ipco_build_collisions.argtypes = [_D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I,
                                             C.POINTER(_Pairs), C.POINTER(_Pairs),
@@ -304,8 +307,10 @@ def _pairs_struct(a):
     return s, a
 
 
-def candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None):
-    """(ee, fv) candidate arrays, LBVH semantics: ee rows (query, other)."""
+def candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None, method="lbvh"):
+    """(ee, fv) candidate arrays.  method "lbvh": FP32 boxes, ee rows
+    (query, other) in Morton order; "sap": FP64 boxes, ee rows (earlier,
+    later) in sweep order."""
     V0, p0 = _d(V0)
     p1 = None
     if V1 is not None:
@@ -316,9 +321,17 @@ def candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None):
     if vertex_group is not None:
         vertex_group, pg = _i(vertex_group)
     ee, fv = _Pairs(), _Pairs()
-    lib().ipco_candidates(p0, p1, len(V0), pE, len(E), pF, len(F), inflation, pg,
-                          C.byref(ee), C.byref(fv))
+    fn = lib().ipco_candidates_sap if method == "sap" else lib().ipco_candidates
+    fn(p0, p1, len(V0), pE, len(E), pF, len(F), inflation, pg, C.byref(ee), C.byref(fv))
     return _take_pairs(ee), _take_pairs(fv)
+
+
+def sap_axis(V0, inflation=0.0, V1=None):
+    V0, p0 = _d(V0)
+    p1 = None
+    if V1 is not None:
+        V1, p1 = _d(V1)
+    return lib().ipco_sap_axis(p0, p1, len(V0), inflation)
 
 
 # ---- narrow phase + potentials ---------------------------------------------

@@ -213,3 +213,30 @@ def test_errors_are_reported(ctx, capi):
         ctx.broad_phase(False, -1.0)
     with pytest.raises(capi.IPCGError):
         ctx.stepsize(rescale=0.0)
+
+
+@pytest.mark.parametrize("name", ["tet", "sheets:12:0", "spheres:4:3:0", "stack:16:1"])
+def test_sweep_and_prune(ctx, capi, O, scenes, name):
+    """IPCG_BP_SAP: the FP64-box overlap sets with sweep-order edge-edge
+    orientation, equal to the oracle's sweep; collisions and step sizes built
+    on them equal the LBVH ones"""
+    s = make(scenes, name)
+    say("case sap", name, s.sizes)
+    upload(ctx, s)
+    for swept in (False, True):
+        r = 0.0 if swept else 0.5 * s.dhat
+        ctx.broad_phase(swept, r, method=capi.BP_SAP)
+        ee_g, fv_g = ctx.candidates()
+        ee_o, fv_o = O.candidates(s.V, s.E, s.F, r, V1=s.V1 if swept else None, method="sap")
+        assert pair_set(ee_g) == pair_set(ee_o) and len(ee_g) == len(ee_o)
+        assert pair_set(fv_g) == pair_set(fv_o) and len(fv_g) == len(fv_o)
+    ctx.build_collisions(s.dhat, method=capi.BP_SAP)
+    n_sap = ctx.collision_counts().tolist()
+    ctx.build_collisions(s.dhat)
+    assert n_sap == ctx.collision_counts().tolist()
+    # the sweep's candidate set is a subset of the LBVH's (FP64 vs FP32
+    # boxes); it equals the oracle's sweep-based step
+    t_sap = ctx.stepsize(method=capi.BP_SAP)
+    assert t_sap >= ctx.stepsize()
+    ee, fv = O.candidates(s.V, s.E, s.F, 0.0, V1=s.V1, method="sap")
+    assert t_sap == O.candidates_stepsize(s.V, s.V1, s.E, s.F, ee, fv)

@@ -128,3 +128,47 @@ def test_ee_orientation_is_lower_morton_first(O, scenes):
     ee2, _ = O.candidates(s.V, s.E, s.F, 0.5 * s.dhat)
     assert pair_set(ee1) == pair_set(ee2)
     assert not (pair_set(ee1) & pair_set(ee1[:, ::-1]))
+
+
+def brute64(V0, E, F, r, V1=None):
+    """FP64-box overlap sets (no FP32 rounding)"""
+    lo = V0.copy() if V1 is None else np.minimum(V0, V1)
+    hi = V0.copy() if V1 is None else np.maximum(V0, V1)
+    lo = np.nextafter(lo - r, -np.inf)
+    hi = np.nextafter(hi + r, np.inf)
+    vb = (lo, hi)
+    eb = (lo[E].min(axis=1), hi[E].max(axis=1))
+    fb = (lo[F].min(axis=1), hi[F].max(axis=1))
+    M = overlap(eb, eb)
+    share = (E[:, None, :, None] == E[None, :, None, :]).any(axis=(2, 3))
+    ii, jj = np.nonzero(np.triu(M & ~share, 1))
+    Mf = overlap(fb, vb)
+    inc = (F[:, :, None] == np.arange(len(V0))[None, None, :]).any(axis=1)
+    fi, vi = np.nonzero(Mf & ~inc)
+    return set(zip(ii.tolist(), jj.tolist())), set(zip(fi.tolist(), vi.tolist())), eb
+
+
+@pytest.mark.parametrize("swept", [False, True])
+def test_sweep_and_prune_equals_fp64_brute_force(O, scenes, swept):
+    """sweep and prune reports exactly the FP64 box overlaps; edge-edge pairs
+    are oriented by (min along the sweep axis, id)"""
+    for s in (scenes.two_sheets(8, seed=4), scenes.sphere_chain(3, 2, seed=1),
+              scenes.stacked_sheets(10, 4, seed=0, h=1.0 / 511)):
+        r = 0.0 if swept else 0.5 * s.dhat
+        V1 = s.V1 if swept else None
+        ee, fv = O.candidates(s.V, s.E, s.F, r, V1=V1, method="sap")
+        bee, bfv, eb = brute64(s.V, s.E, s.F, r, V1)
+        assert unordered_set(ee) == bee and len(ee) == len(bee)
+        assert pair_set(fv) == bfv and len(fv) == len(bfv)
+        ax = O.sap_axis(s.V, r, V1=V1)
+        key = lambda e: (eb[0][e, ax], e)
+        assert all(key(a) < key(b) for a, b in ee.tolist())
+        # FP64 overlaps are a subset of the FP32 (LBVH) overlaps
+        lee, lfv = O.candidates(s.V, s.E, s.F, r, V1=V1)
+        assert unordered_set(ee) <= unordered_set(lee) and pair_set(fv) <= pair_set(lfv)
+
+
+def test_sweep_axis_is_largest_extent(O):
+    V = np.array([[0.0, 0, 0], [1, 5, 2], [0.5, 1, 1]])
+    assert O.sap_axis(V) == 1
+    assert O.sap_axis(V * [9, 1, 1]) == 0
