@@ -91,6 +91,18 @@ int ipcg_candidates_count(ipcg_ctx* ctx, int64_t* n_ee, int64_t* n_fv);
 /* ee: 2*n_ee int32, fv: 2*n_fv int32 (either may be NULL) */
 int ipcg_candidates_download(ipcg_ctx* ctx, int32_t* ee, int32_t* fv);
 
+/* ---- codim candidates (candidates.cpp:65-118) ----------------------------
+ * Meshes with free vertices (in no edge) or free edges (in no face,
+ * collision_mesh.cpp:145-182) get two more lists from every broad phase
+ * call: codim vertex pairs (each unordered pair once) and (codim edge, codim
+ * vertex) pairs, FP32-box (LBVH) semantics with either method.  The
+ * collision build and the CCD entry points use them together with the
+ * edge-edge and face-vertex lists. */
+int ipcg_mesh_codim(ipcg_ctx* ctx, int32_t* n_codim_vertices, int32_t* n_codim_edges);
+int ipcg_codim_candidates_count(ipcg_ctx* ctx, int64_t* n_vv, int64_t* n_ev);
+/* vv: 2*n_vv int32, ev: 2*n_ev int32 (either may be NULL) */
+int ipcg_codim_candidates_download(ipcg_ctx* ctx, int32_t* vv, int32_t* ev);
+
 /* ---- collision set (normal_collisions.cpp:20-158) ------------------------
  * build: static broad phase with inflation 0.5*(dhat+dmin) followed by the
  * narrow phase (NormalCollisions::build(mesh, V, dhat, dmin, broad_phase)).
@@ -145,6 +157,12 @@ int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double toler
 int ipcg_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
                         int64_t max_iterations, double conservative_rescaling,
                         double* toi_ee, double* toi_fv);
+
+/* the same for the codim lists (point-point and point-edge CCD,
+ * tight_inclusion_ccd.cpp:77-200); toi_vv: n_vv, toi_ev: n_ev */
+int ipcg_codim_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
+                              int64_t max_iterations, double conservative_rescaling,
+                              double* toi_vv, double* toi_ev);
 
 /* ---- timing of the last call's kernels (for benchmarks) ----------------- */
 /* Kernel time of named stages recorded with HIP events on the context

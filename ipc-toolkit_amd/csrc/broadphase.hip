@@ -131,7 +131,8 @@ __global__ __launch_bounds__(BP_BLOCK) void k_prim_boxes(
     const int p = blockIdx.x * BP_BLOCK + threadIdx.x;
     if (p >= n) return;
     double lo[3], hi[3];
-    vbox(X0, X1, nV, ARITY == 1 ? p : I[(int64_t)ARITY * p], r, lo, hi);
+    // I: vertex ids per primitive (arity 1: a vertex subset, or null for all)
+    vbox(X0, X1, nV, (ARITY == 1 && !I) ? p : I[(int64_t)ARITY * p], r, lo, hi);
 #pragma unroll
     for (int j = 1; j < ARITY; j++) {
         double l[3], h[3];
@@ -307,6 +308,14 @@ __device__ __forceinline__ bool can_ee(const Filter& f, int ea, int eb)
     return can_vv(f, a0, b0) || can_vv(f, a0, b1) || can_vv(f, a1, b0) || can_vv(f, a1, b1);
 }
 
+// lbvh.cpp:800-807 can_edge_vertex_collide
+__device__ __forceinline__ bool can_ev(const Filter& f, int ei, int vi)
+{
+    const int e0 = f.E[2 * (int64_t)ei], e1 = f.E[2 * (int64_t)ei + 1];
+    if (vi == e0 || vi == e1) return false;
+    return can_vv(f, vi, e0) || can_vv(f, vi, e1);
+}
+
 // lbvh.cpp:851-858 can_face_vertex_collide
 __device__ __forceinline__ bool can_fv(const Filter& f, int fi, int vi)
 {
@@ -316,16 +325,39 @@ __device__ __forceinline__ bool can_fv(const Filter& f, int fi, int vi)
     return can_vv(f, vi, f0) || can_vv(f, vi, f1) || can_vv(f, vi, f2);
 }
 
-// One lane per query leaf (in the query set's Morton order).  TRI: edge-edge
-// self traversal reporting (query, other) for other at a later sorted
-// position; otherwise vertex queries against the face tree reporting
-// (face, vertex).
-template <bool TRI>
+// One lane per query leaf (in the query set's Morton order).  MODE:
+//  TR_EE  edge-edge self traversal: (query, other) for other at a later
+//         sorted position;
+//  TR_FV  vertex queries against the face tree: (face, vertex);
+//  TR_VV  codim vertices, self traversal: (query, other) in mesh ids;
+//  TR_EV  codim vertex queries against the codim-edge tree: (edge, vertex)
+//         in mesh ids (candidates.cpp:65-118).
+enum { TR_EE = 0, TR_FV = 1, TR_VV = 2, TR_EV = 3 };
+
+template <int MODE>
+__device__ __forceinline__ int2 leaf_pair(const Filter& flt, const int32_t* vmap,
+                                          const int32_t* emap, int q, int j)
+{
+    if constexpr (MODE == TR_EE) {
+        return can_ee(flt, q, j) ? make_int2(q, j) : make_int2(-1, -1);
+    } else if constexpr (MODE == TR_FV) {
+        return can_fv(flt, j, q) ? make_int2(j, q) : make_int2(-1, -1);
+    } else if constexpr (MODE == TR_VV) {
+        const int a = vmap[q], b = vmap[j];
+        return can_vv(flt, a, b) ? make_int2(a, b) : make_int2(-1, -1);
+    } else {
+        const int e = emap[j], v = vmap[q];
+        return can_ev(flt, e, v) ? make_int2(e, v) : make_int2(-1, -1);
+    }
+}
+
+template <int MODE>
 __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
-    const uint32_t* tids, const float* tbox, int nleaves, Filter flt, int2* out, Shards S,
-    int* status)
+    const uint32_t* tids, const float* tbox, int nleaves, Filter flt, const int32_t* vmap,
+    const int32_t* emap, int2* out, Shards S, int* status)
 {
+    constexpr bool TRI = MODE == TR_EE || MODE == TR_VV;
     __shared__ int stack[TR_STACK][TR_BLOCK];
     const int t = threadIdx.x;
     const int qpos = blockIdx.x * TR_BLOCK + t;
@@ -343,19 +375,18 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
         if (t == 0) atomicOr(status, 2);
         return;
     }
+    const int sh = wave_shard();
     if (root == -1) { // single target leaf (lbvh.cpp:386-396)
-        int hit = -1;
+        int2 hit = make_int2(-1, -1);
         if (!TRI && active) {
             const int fp = (int)tids[0];
             const float* b = tbox + (int64_t)6 * fp;
-            if (overlap(q, b, b + 3) && can_fv(flt, fp, qprim)) hit = fp;
+            if (overlap(q, b, b + 3)) hit = leaf_pair<MODE>(flt, vmap, emap, qprim, fp);
         }
-        const int sh = wave_shard();
-        const unsigned long long slot = shard_append(S, sh, hit >= 0 ? 1u : 0u);
-        if (hit >= 0 && (int64_t)slot < S.scap) out[sh * S.scap + (int64_t)slot] = make_int2(hit, qprim);
+        const unsigned long long slot = shard_append(S, sh, hit.x >= 0 ? 1u : 0u);
+        if (hit.x >= 0 && (int64_t)slot < S.scap) out[sh * S.scap + (int64_t)slot] = hit;
         return;
     }
-    const int sh = wave_shard();
     int node = root, sp = 0, steps = 0;
     const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
     while (__any(active)) {
@@ -363,7 +394,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
             atomicOr(status, 2);
             active = false;
         }
-        int h0 = -1, h1 = -1;
+        int2 h0 = make_int2(-1, -1), h1 = make_int2(-1, -1);
         bool nd_ok = true;
         if (active) {
             const Node nd = nodes[node];
@@ -379,14 +410,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
                 if (o1 && nd.r1 <= qpos) o1 = false;
             }
             const bool l0 = nd.c0 < 0, l1 = nd.c1 < 0;
-            if (o0 && l0) {
-                const int j = (int)tids[~nd.c0];
-                if (TRI ? can_ee(flt, qprim, j) : can_fv(flt, j, qprim)) h0 = j;
-            }
-            if (o1 && l1) {
-                const int j = (int)tids[~nd.c1];
-                if (TRI ? can_ee(flt, qprim, j) : can_fv(flt, j, qprim)) h1 = j;
-            }
+            if (o0 && l0) h0 = leaf_pair<MODE>(flt, vmap, emap, qprim, (int)tids[~nd.c0]);
+            if (o1 && l1) h1 = leaf_pair<MODE>(flt, vmap, emap, qprim, (int)tids[~nd.c1]);
             const bool t0 = o0 && !l0, t1 = o1 && !l1;
             if (!nd_ok) {
             } else if (!t0 && !t1) {
@@ -403,14 +428,14 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
                 }
             }
         }
-        const unsigned cnt = (h0 >= 0 ? 1u : 0u) + (h1 >= 0 ? 1u : 0u);
+        const unsigned cnt = (h0.x >= 0 ? 1u : 0u) + (h1.x >= 0 ? 1u : 0u);
         int64_t slot = (int64_t)shard_append(S, sh, cnt);
         int2* reg = out + sh * S.scap;
-        if (h0 >= 0) {
-            if (slot < S.scap) reg[slot] = TRI ? make_int2(qprim, h0) : make_int2(h0, qprim);
+        if (h0.x >= 0) {
+            if (slot < S.scap) reg[slot] = h0;
             slot++;
         }
-        if (h1 >= 0 && slot < S.scap) reg[slot] = TRI ? make_int2(qprim, h1) : make_int2(h1, qprim);
+        if (h1.x >= 0 && slot < S.scap) reg[slot] = h1;
     }
 }
 
@@ -667,7 +692,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     const double* X0 = c.X[0].as<double>();
     const double* X1 = swept ? c.X[1].as<double>() : nullptr;
     c.cand_valid = false;
-    c.n_ee = c.n_fv = 0;
+    c.n_ee = c.n_fv = c.n_vv = c.n_ev = 0;
     if (c.nV == 0) {
         c.cand_valid = true;
         c.cand_swept = swept;
@@ -701,25 +726,39 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, true);
         build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true);
     }
+    // codim lists (candidates.cpp:65-118): a tree over the codim vertices
+    // (its own queries) and one over the codim edges; LBVH semantics with
+    // either method
+    const bool codim = c.n_cv > 0;
+    if (codim) {
+        build_tree(c, c.tcv, c.cv.as<int32_t>(), 1, c.n_cv, X0, X1, inflation, true);
+        build_tree(c, c.tce, c.cev.as<int32_t>(), 2, c.n_ce, X0, X1, inflation, true);
+    }
 
     c.counters.reserve(sizeof(unsigned long long) * 4);
     Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr };
     int* status = reinterpret_cast<int*>(c.counters.as<unsigned long long>() + 2);
-    c.shard_cnt.reserve(2 * SHARD_CNT_BYTES);
-    c.shard_off.reserve(sizeof(int64_t) * 2 * (SHARDS + 1));
-    unsigned long long* cnt[2] = { c.shard_cnt.as<unsigned long long>(),
-                                   c.shard_cnt.as<unsigned long long>() + SHARDS * SHARD_PAD };
+    c.shard_cnt.reserve(4 * SHARD_CNT_BYTES);
+    c.shard_off.reserve(sizeof(int64_t) * 4 * (SHARDS + 1));
+    unsigned long long* cnt[4];
+    for (int k = 0; k < 4; k++) cnt[k] = c.shard_cnt.as<unsigned long long>() + k * SHARDS * SHARD_PAD;
+    int64_t* scap[4] = { &c.scap_ee, &c.scap_fv, &c.scap_vv, &c.scap_ev };
+    const int64_t guess[4] = { 16 * (int64_t)c.nE, 8 * (int64_t)c.nV, 4 * (int64_t)c.n_cv,
+                               4 * (int64_t)c.n_cv };
+    DevBuf* final_list[4] = { &c.cand_ee, &c.cand_fv, &c.cand_vv, &c.cand_ev };
+    int64_t* final_n[4] = { &c.n_ee, &c.n_fv, &c.n_vv, &c.n_ev };
+    const int lists = codim ? 4 : 2;
     for (int attempt = 0; attempt < 3; attempt++) {
-        // per-shard regions; their sum bounds the list until a retry grows it
-        if (c.scap_ee == 0) c.scap_ee = std::max<int64_t>(1024, 16 * (int64_t)c.nE / SHARDS);
-        if (c.scap_fv == 0) c.scap_fv = std::max<int64_t>(1024, 8 * (int64_t)c.nV / SHARDS);
-        c.cand_stage[0].reserve(sizeof(int2) * (size_t)(SHARDS * c.scap_ee));
-        c.cand_stage[1].reserve(sizeof(int2) * (size_t)(SHARDS * c.scap_fv));
-        IPCG_HIP(hipMemsetAsync(c.shard_cnt.p, 0, 2 * SHARD_CNT_BYTES, s));
+        // per-shard regions; their sum bounds a list until a retry grows it
+        for (int k = 0; k < lists; k++) {
+            if (*scap[k] == 0) *scap[k] = std::max<int64_t>(1024, guess[k] / SHARDS);
+            c.cand_stage[k].reserve(sizeof(int2) * (size_t)(SHARDS * *scap[k]));
+        }
+        IPCG_HIP(hipMemsetAsync(c.shard_cnt.p, 0, 4 * SHARD_CNT_BYTES, s));
         IPCG_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
+        const Shards S0 { cnt[0], c.scap_ee }, S1 { cnt[1], c.scap_fv };
         c.stage_begin("bp_traverse");
         if (sap) {
-            const Shards S0 { cnt[0], c.scap_ee }, S1 { cnt[1], c.scap_fv };
             if (c.nE >= 2)
                 hipLaunchKernelGGL(k_sap_self, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0, s,
                                    c.te.sbox.as<double>(), c.te.ids.as<uint32_t>(), c.nE, axis,
@@ -737,18 +776,37 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.cand_stage[1].as<int2>(), S1);
             }
         } else {
-        if (c.nE >= 2)
-            hipLaunchKernelGGL(k_traverse<true>, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0,
-                               s, c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
-                               c.te.nodes.as<Node>(), c.te.root.as<int32_t>(),
-                               c.te.ids.as<uint32_t>(), c.te.box.as<float>(), c.nE, flt,
-                               c.cand_stage[0].as<int2>(), Shards { cnt[0], c.scap_ee }, status);
-        if (c.nF >= 1)
-            hipLaunchKernelGGL(k_traverse<false>, dim3(grid_for(c.nV, TR_BLOCK)), dim3(TR_BLOCK),
-                               0, s, c.tv.box.as<float>(), c.tv.ids.as<uint32_t>(), c.nV,
-                               c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(),
-                               c.tf.ids.as<uint32_t>(), c.tf.box.as<float>(), c.nF, flt,
-                               c.cand_stage[1].as<int2>(), Shards { cnt[1], c.scap_fv }, status);
+            if (c.nE >= 2)
+                hipLaunchKernelGGL(k_traverse<TR_EE>, dim3(grid_for(c.nE, TR_BLOCK)),
+                                   dim3(TR_BLOCK), 0, s, c.te.box.as<float>(),
+                                   c.te.ids.as<uint32_t>(), c.nE, c.te.nodes.as<Node>(),
+                                   c.te.root.as<int32_t>(), c.te.ids.as<uint32_t>(),
+                                   c.te.box.as<float>(), c.nE, flt, nullptr, nullptr,
+                                   c.cand_stage[0].as<int2>(), S0, status);
+            if (c.nF >= 1)
+                hipLaunchKernelGGL(k_traverse<TR_FV>, dim3(grid_for(c.nV, TR_BLOCK)),
+                                   dim3(TR_BLOCK), 0, s, c.tv.box.as<float>(),
+                                   c.tv.ids.as<uint32_t>(), c.nV, c.tf.nodes.as<Node>(),
+                                   c.tf.root.as<int32_t>(), c.tf.ids.as<uint32_t>(),
+                                   c.tf.box.as<float>(), c.nF, flt, nullptr, nullptr,
+                                   c.cand_stage[1].as<int2>(), S1, status);
+        }
+        if (codim) {
+            const Shards S2 { cnt[2], c.scap_vv }, S3 { cnt[3], c.scap_ev };
+            if (c.n_cv >= 2)
+                hipLaunchKernelGGL(k_traverse<TR_VV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
+                                   dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
+                                   c.tcv.ids.as<uint32_t>(), c.n_cv, c.tcv.nodes.as<Node>(),
+                                   c.tcv.root.as<int32_t>(), c.tcv.ids.as<uint32_t>(),
+                                   c.tcv.box.as<float>(), c.n_cv, flt, c.cv.as<int32_t>(),
+                                   c.ce.as<int32_t>(), c.cand_stage[2].as<int2>(), S2, status);
+            if (c.n_ce >= 1)
+                hipLaunchKernelGGL(k_traverse<TR_EV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
+                                   dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
+                                   c.tcv.ids.as<uint32_t>(), c.n_cv, c.tce.nodes.as<Node>(),
+                                   c.tce.root.as<int32_t>(), c.tce.ids.as<uint32_t>(),
+                                   c.tce.box.as<float>(), c.n_ce, flt, c.cv.as<int32_t>(),
+                                   c.ce.as<int32_t>(), c.cand_stage[3].as<int2>(), S3, status);
         }
         IPCG_CHECK_LAUNCH();
         c.stage_end();
@@ -758,28 +816,27 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         const int st = hs[0];
         if (st & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
         if (st & 1) throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
-        const ShardCounts ke = shard_counts(cnt[0], c.pinned, s);
-        const ShardCounts kf = shard_counts(cnt[1], c.pinned, s);
+        ShardCounts kc[4];
         bool again = false;
-        if (ke.max > c.scap_ee) {
-            c.scap_ee = ke.max + ke.max / 4 + 1024;
-            again = true;
-        }
-        if (kf.max > c.scap_fv) {
-            c.scap_fv = kf.max + kf.max / 4 + 1024;
-            again = true;
+        for (int k = 0; k < lists; k++) {
+            kc[k] = shard_counts(cnt[k], c.pinned, s);
+            if (kc[k].max > *scap[k]) {
+                *scap[k] = kc[k].max + kc[k].max / 4 + 1024;
+                again = true;
+            }
         }
         if (again) continue;
         c.stage_begin("bp_compact");
-        c.cand_ee.reserve(sizeof(int2) * (size_t)std::max<int64_t>(ke.total, 1));
-        c.cand_fv.reserve(sizeof(int2) * (size_t)std::max<int64_t>(kf.total, 1));
-        shard_compact(c.cand_stage[0].as<int2>(), c.scap_ee, cnt[0], ke.max, c.cand_ee.as<int2>(),
-                      c.shard_off.as<int64_t>(), s);
-        shard_compact(c.cand_stage[1].as<int2>(), c.scap_fv, cnt[1], kf.max, c.cand_fv.as<int2>(),
-                      c.shard_off.as<int64_t>() + (SHARDS + 1), s);
+        for (int k = 0; k < 4; k++) {
+            const int64_t m = k < lists ? kc[k].total : 0;
+            final_list[k]->reserve(sizeof(int2) * (size_t)std::max<int64_t>(m, 1));
+            if (m > 0)
+                shard_compact(c.cand_stage[k].as<int2>(), *scap[k], cnt[k], kc[k].max,
+                              final_list[k]->as<int2>(), c.shard_off.as<int64_t>() + k * (SHARDS + 1),
+                              s);
+            *final_n[k] = m;
+        }
         c.stage_end();
-        c.n_ee = ke.total;
-        c.n_fv = kf.total;
         c.cand_valid = true;
         c.cand_swept = swept;
         return;

@@ -165,7 +165,12 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status,
                        &c.hp_mark, &c.hp_bptr, &c.hp_bcol, &c.cand_stage[0], &c.cand_stage[1],
                        &c.shard_cnt, &c.shard_off, &c.np_eec, &c.ccd_stage, &c.ccd_todo,
-                       &c.tv.sbox, &c.te.sbox, &c.tf.sbox };
+                       &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
+                       &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3] };
+    for (Tree* T : { &c.tcv, &c.tce })
+        for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
+                           &T->root, &T->sbox })
+            b->release();
     for (DevBuf* b : bufs) b->release();
     for (auto& p : c.timer.pending) {
         (void)hipEventDestroy(p.a);
@@ -230,6 +235,28 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
         if (nE) IPCG_HIP(hipMemcpyAsync(c.E.p, edges, sizeof(int32_t) * 2 * nE, hipMemcpyHostToDevice, c.stream));
         if (nF) IPCG_HIP(hipMemcpyAsync(c.F.p, faces, sizeof(int32_t) * 3 * nF, hipMemcpyHostToDevice, c.stream));
         if (nF) IPCG_HIP(hipMemcpyAsync(c.F2E.p, f2e.data(), sizeof(int32_t) * 3 * nF, hipMemcpyHostToDevice, c.stream));
+        // codim vertices (in no edge) and codim edges (in no face)
+        // (collision_mesh.cpp:145-182)
+        std::vector<char> in_edge((size_t)std::max(nV, 1), 0), in_face((size_t)std::max(nE, 1), 0);
+        for (int64_t i = 0; i < 2 * (int64_t)nE; i++) in_edge[edges[i]] = 1;
+        for (int32_t x : f2e) in_face[x] = 1;
+        std::vector<int32_t> cv, ce, cev;
+        for (int32_t v = 0; v < nV; v++)
+            if (!in_edge[v]) cv.push_back(v);
+        for (int32_t e = 0; e < nE; e++)
+            if (!in_face[e]) {
+                ce.push_back(e);
+                cev.push_back(edges[2 * e]);
+                cev.push_back(edges[2 * e + 1]);
+            }
+        c.n_cv = (int32_t)cv.size();
+        c.n_ce = (int32_t)ce.size();
+        c.cv.reserve(sizeof(int32_t) * std::max<size_t>(cv.size(), 1));
+        c.ce.reserve(sizeof(int32_t) * std::max<size_t>(ce.size(), 1));
+        c.cev.reserve(sizeof(int32_t) * std::max<size_t>(cev.size(), 1));
+        if (!cv.empty()) IPCG_HIP(hipMemcpyAsync(c.cv.p, cv.data(), sizeof(int32_t) * cv.size(), hipMemcpyHostToDevice, c.stream));
+        if (!ce.empty()) IPCG_HIP(hipMemcpyAsync(c.ce.p, ce.data(), sizeof(int32_t) * ce.size(), hipMemcpyHostToDevice, c.stream));
+        if (!cev.empty()) IPCG_HIP(hipMemcpyAsync(c.cev.p, cev.data(), sizeof(int32_t) * cev.size(), hipMemcpyHostToDevice, c.stream));
         c.has_patch = vertex_patch != nullptr;
         if (c.has_patch) {
             c.patch.reserve(sizeof(int32_t) * (size_t)std::max(nV, 1));
@@ -240,7 +267,7 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
         c.has_mesh = true;
         c.has_pos[0] = c.has_pos[1] = false;
         c.cand_valid = c.coll_valid = c.hess_valid = false;
-        c.scap_ee = c.scap_fv = 0;
+        c.scap_ee = c.scap_fv = c.scap_vv = c.scap_ev = 0;
     });
 }
 
@@ -289,6 +316,36 @@ int ipcg_candidates_download(ipcg_ctx* ctx, int32_t* ee, int32_t* fv)
             IPCG_HIP(hipMemcpyAsync(ee, c.cand_ee.p, sizeof(int2) * c.n_ee, hipMemcpyDeviceToHost, c.stream));
         if (fv && c.n_fv)
             IPCG_HIP(hipMemcpyAsync(fv, c.cand_fv.p, sizeof(int2) * c.n_fv, hipMemcpyDeviceToHost, c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_mesh_codim(ipcg_ctx* ctx, int32_t* n_codim_vertices, int32_t* n_codim_edges)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.require_mesh();
+        if (n_codim_vertices) *n_codim_vertices = c.n_cv;
+        if (n_codim_edges) *n_codim_edges = c.n_ce;
+    });
+}
+
+int ipcg_codim_candidates_count(ipcg_ctx* ctx, int64_t* n_vv, int64_t* n_ev)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.cand_valid) throw Error(IPCG_ERR_STATE, "no candidate set");
+        if (n_vv) *n_vv = c.n_vv;
+        if (n_ev) *n_ev = c.n_ev;
+    });
+}
+
+int ipcg_codim_candidates_download(ipcg_ctx* ctx, int32_t* vv, int32_t* ev)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.cand_valid) throw Error(IPCG_ERR_STATE, "no candidate set");
+        if (vv && c.n_vv)
+            IPCG_HIP(hipMemcpyAsync(vv, c.cand_vv.p, sizeof(int2) * c.n_vv, hipMemcpyDeviceToHost, c.stream));
+        if (ev && c.n_ev)
+            IPCG_HIP(hipMemcpyAsync(ev, c.cand_ev.p, sizeof(int2) * c.n_ev, hipMemcpyDeviceToHost, c.stream));
         IPCG_HIP(hipStreamSynchronize(c.stream));
     });
 }
@@ -403,7 +460,7 @@ int ipcg_compute_collision_free_stepsize(ipcg_ctx* ctx, double min_distance, dou
         check_ccd_args(min_distance, tolerance, rescale);
         if (!step) throw Error(IPCG_ERR_ARG, "null output");
         bp_build(c, true, 0.5 * min_distance, method);
-        *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr, nullptr);
+        *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr);
     });
 }
 
@@ -413,7 +470,7 @@ int ipcg_candidates_stepsize(ipcg_ctx* ctx, double min_distance, double toleranc
     return guard(ctx, [&](Ctx& c) {
         check_ccd_args(min_distance, tolerance, rescale);
         if (!step) throw Error(IPCG_ERR_ARG, "null output");
-        *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr, nullptr);
+        *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr);
     });
 }
 
@@ -425,7 +482,7 @@ int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double toler
         check_ccd_args(min_distance, tolerance, rescale);
         if (!collision_free) throw Error(IPCG_ERR_ARG, "null output");
         bp_build(c, true, 0.5 * min_distance, method);
-        const double r = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, true, nullptr, nullptr);
+        const double r = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, true, nullptr);
         *collision_free = r == 1.0 ? 1 : 0;
     });
 }
@@ -436,7 +493,20 @@ int ipcg_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
     return guard(ctx, [&](Ctx& c) {
         check_ccd_args(min_distance, tolerance, rescale);
         if (!toi_ee && !toi_fv) throw Error(IPCG_ERR_ARG, "null outputs");
-        ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, toi_ee, toi_fv);
+        double* const out[4] = { toi_ee, toi_fv, nullptr, nullptr };
+        ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, out);
+    });
+}
+
+int ipcg_codim_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
+                              int64_t max_iterations, double rescale, double* toi_vv,
+                              double* toi_ev)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!toi_vv && !toi_ev) throw Error(IPCG_ERR_ARG, "null outputs");
+        double* const out[4] = { nullptr, nullptr, toi_vv, toi_ev };
+        ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, out);
     });
 }
 

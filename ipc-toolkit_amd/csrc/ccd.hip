@@ -44,6 +44,10 @@ struct CCDArgs {
     int64_t n_ee;
     const int2* fv;
     int64_t n_fv;
+    const int2* vv; // codim vertex pairs
+    int64_t n_vv;
+    const int2* ev; // (codim edge, codim vertex)
+    int64_t n_ev;
     double min_distance, tolerance, rescale;
     long max_iter;
     bool bounded;          // hide outcomes >= running minimum
@@ -71,21 +75,39 @@ __device__ __forceinline__ void report(const CCDArgs& A, int64_t ci, bool hit, d
 // needed, q / eff / adj set).
 __device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double& eff, double& adj)
 {
+    // candidates [0, n_ee) edge-edge, then face-vertex, codim vertex-vertex
+    // and codim edge-vertex; point-point and point-edge run as degenerate
+    // edge-edge queries (p0, p0, p1, p1) and (p, p, e0, e1)
+    // (tight_inclusion_ccd.cpp:77-200)
     int ids[4];
-    const bool is_ee = ci < A.n_ee;
-    if (is_ee) {
+    int kind; // 0 ee, 1 fv, 2 vv, 3 ev
+    if (ci < A.n_ee) {
+        kind = 0;
         const int2 c = A.ee[ci];
         ids[0] = A.E[2 * (int64_t)c.x];
         ids[1] = A.E[2 * (int64_t)c.x + 1];
         ids[2] = A.E[2 * (int64_t)c.y];
         ids[3] = A.E[2 * (int64_t)c.y + 1];
-    } else {
-        const int2 c = A.fv[ci - A.n_ee];
+    } else if ((ci -= A.n_ee) < A.n_fv) {
+        kind = 1;
+        const int2 c = A.fv[ci];
         ids[0] = c.y;
         ids[1] = A.F[3 * (int64_t)c.x];
         ids[2] = A.F[3 * (int64_t)c.x + 1];
         ids[3] = A.F[3 * (int64_t)c.x + 2];
+    } else if ((ci -= A.n_fv) < A.n_vv) {
+        kind = 2;
+        const int2 c = A.vv[ci];
+        ids[0] = ids[1] = c.x;
+        ids[2] = ids[3] = c.y;
+    } else {
+        kind = 3;
+        const int2 c = A.ev[ci - A.n_vv];
+        ids[0] = ids[1] = c.y;
+        ids[2] = A.E[2 * (int64_t)c.x];
+        ids[3] = A.E[2 * (int64_t)c.x + 1];
     }
+    const bool is_ee = kind != 1;
     d3 p0[4];
     bool moving = false;
 #pragma unroll
@@ -101,9 +123,15 @@ __device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double&
         moving |= p1.x != p0[j].x || p1.y != p0[j].y || p1.z != p0[j].z;
     }
     q.vf = !is_ee;
-    const double d2 = is_ee
-        ? ee_dist2(p0[0], p0[1], p0[2], p0[3], edge_edge_type(p0[0], p0[1], p0[2], p0[3]))
-        : pt_dist2(p0[0], p0[1], p0[2], p0[3], point_triangle_type(p0[0], p0[1], p0[2], p0[3]));
+    double d2;
+    if (kind == 0)
+        d2 = ee_dist2(p0[0], p0[1], p0[2], p0[3], edge_edge_type(p0[0], p0[1], p0[2], p0[3]));
+    else if (kind == 1)
+        d2 = pt_dist2(p0[0], p0[1], p0[2], p0[3], point_triangle_type(p0[0], p0[1], p0[2], p0[3]));
+    else if (kind == 2)
+        d2 = sqn(sub(p0[0], p0[2]));
+    else
+        d2 = pe_dist2(p0[0], p0[2], p0[3], point_edge_type(p0[0], p0[2], p0[3]));
     const double d0 = sqrt(d2);
     const double dmin = A.min_distance;
     if (!(d0 > dmin)) return 1; // check_initial_distance.hpp:7-21 (also without motion)
@@ -604,7 +632,7 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
 } // namespace
 
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, double rescale,
-                    bool early_exit, double* toi_ee, double* toi_fv)
+                    bool early_exit, double* const* toi_out)
 {
     c.require_mesh();
     c.require_pos(0);
@@ -612,8 +640,8 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     if (!c.cand_valid || !c.cand_swept)
         throw Error(IPCG_ERR_STATE, "no swept candidate set (run the swept broad phase)");
     hipStream_t s = c.stream;
-    const int64_t n = c.n_ee + c.n_fv;
-    const bool per_cand = toi_ee || toi_fv;
+    const int64_t n = c.n_ee + c.n_fv + c.n_vv + c.n_ev;
+    const bool per_cand = toi_out != nullptr;
     if (n == 0) return 1.0;
     if (n >= (1ll << 32)) throw Error(IPCG_ERR_ARG, "too many CCD candidates");
     // device scalars: bound, work counter, heavy count, hit flag, error, roots
@@ -644,6 +672,10 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     A.n_ee = c.n_ee;
     A.fv = c.cand_fv.as<int2>();
     A.n_fv = c.n_fv;
+    A.vv = c.cand_vv.as<int2>();
+    A.n_vv = c.n_vv;
+    A.ev = c.cand_ev.as<int2>();
+    A.n_ev = c.n_ev;
     A.min_distance = min_distance;
     A.tolerance = tol;
     A.rescale = rescale;
@@ -716,13 +748,15 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
         if (*reinterpret_cast<int*>(h))
             throw Error(IPCG_ERR_NUMERIC, "CCD root search exceeded the level limit");
     }
-    if (per_cand) {
-        if (toi_ee && c.n_ee)
-            IPCG_HIP(hipMemcpyAsync(toi_ee, c.ccd_toi.p, sizeof(double) * c.n_ee,
-                                    hipMemcpyDeviceToHost, s));
-        if (toi_fv && c.n_fv)
-            IPCG_HIP(hipMemcpyAsync(toi_fv, c.ccd_toi.as<double>() + c.n_ee,
-                                    sizeof(double) * c.n_fv, hipMemcpyDeviceToHost, s));
+    if (per_cand) { // toi_out: ee, fv, vv, ev (any null)
+        const int64_t cnt[4] = { c.n_ee, c.n_fv, c.n_vv, c.n_ev };
+        int64_t off = 0;
+        for (int k = 0; k < 4; k++) {
+            if (toi_out[k] && cnt[k])
+                IPCG_HIP(hipMemcpyAsync(toi_out[k], c.ccd_toi.as<double>() + off,
+                                        sizeof(double) * cnt[k], hipMemcpyDeviceToHost, s));
+            off += cnt[k];
+        }
         IPCG_HIP(hipStreamSynchronize(s));
         return 1.0;
     }

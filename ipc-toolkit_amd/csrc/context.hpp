@@ -53,16 +53,22 @@ struct Ctx {
     DevBuf X[2];
     bool has_pos[2] = { false, false };
 
+    // codim vertices (in no edge) and codim edges (in no face), their
+    // endpoint pairs (collision_mesh.cpp:145-182)
+    int32_t n_cv = 0, n_ce = 0;
+    DevBuf cv, ce, cev;
+
     // broad phase
-    Tree tv, te, tf;
+    Tree tv, te, tf, tcv, tce;
     DevBuf aabb_partial, aabb; // mesh bounds
     DevBuf sort_tmp;
     bool cand_valid = false, cand_swept = false;
     DevBuf cand_ee, cand_fv;   // int2 pairs
     DevBuf counters;           // uint64 counters (device)
-    int64_t n_ee = 0, n_fv = 0;
-    int64_t scap_ee = 0, scap_fv = 0;  // per-shard regions of the traversal output
-    DevBuf cand_stage[2];              // sharded traversal output (ee, fv)
+    int64_t n_ee = 0, n_fv = 0, n_vv = 0, n_ev = 0;
+    DevBuf cand_vv, cand_ev;           // codim lists: (v, v) and (edge, vertex)
+    int64_t scap_ee = 0, scap_fv = 0, scap_vv = 0, scap_ev = 0; // shard regions
+    DevBuf cand_stage[4];              // sharded traversal output (ee, fv, vv, ev)
     DevBuf shard_cnt, shard_off;       // shard counters / offsets of the append lists
 
     // collision set
@@ -104,7 +110,9 @@ void np_build(Ctx& c, double dhat, double dmin);
 double pot_energy(Ctx& c, double dhat, double kappa);
 void pot_gradient(Ctx& c, double dhat, double kappa);
 void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
+// toi_out: null (step size) or 4 host arrays (ee, fv, vv, ev; any null)
+// receiving per-candidate times of impact
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
-                    double rescale, bool early_exit, double* toi_ee, double* toi_fv);
+                    double rescale, bool early_exit, double* const* toi_out);
 
 } // namespace ipcg

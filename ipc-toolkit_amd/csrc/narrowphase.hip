@@ -45,11 +45,12 @@ __device__ __forceinline__ uint64_t pair_key(int a, int b)
     return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
 }
 
-// candidates: [0, n_ee) edge-edge, [n_ee, n_ee + n_fv) face-vertex
+// candidates: [0, n_ee) edge-edge, then face-vertex, codim vertex-vertex,
+// codim edge-vertex
 __global__ __launch_bounds__(NP_BLOCK) void k_classify(
     const double* X, const double* R, int nV, const int32_t* E, const int32_t* F,
     const int32_t* F2E, const int2* ee, int64_t n_ee, const int2* fv, int64_t n_fv,
-    double active2, Streams S)
+    const int2* vv, int64_t n_vv, const int2* ev, int64_t n_ev, double active2, Streams S)
 {
     const int64_t i = (int64_t)blockIdx.x * NP_BLOCK + threadIdx.x;
     int kind = -1; // 0 vv, 1 ev, 2 ee, 3 fv
@@ -100,6 +101,25 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
             case PT_P_E1: kind = 1; key = pair_key(F2E[3 * (int64_t)fi + 1], vi); break;
             case PT_P_E2: kind = 1; key = pair_key(F2E[3 * (int64_t)fi + 2], vi); break;
             default: kind = 3; key = pair_key(fi, vi);
+            }
+        }
+    } else if (i < n_ee + n_fv + n_vv) { // normal_collisions_builder.cpp:26-54
+        const int2 c = vv[i - n_ee - n_fv];
+        if (sqn(sub(ldx(X, nV, c.x), ldx(X, nV, c.y))) < active2) {
+            kind = 0;
+            key = vv_key(c.x, c.y);
+        }
+    } else if (i < n_ee + n_fv + n_vv + n_ev) { // normal_collisions_builder.cpp:56-138
+        const int2 c = ev[i - n_ee - n_fv - n_vv];
+        const int ei = c.x, vi = c.y;
+        const int e0 = E[2 * (int64_t)ei], e1 = E[2 * (int64_t)ei + 1];
+        const d3 p = ldx(X, nV, vi), q0 = ldx(X, nV, e0), q1 = ldx(X, nV, e1);
+        const int dt = point_edge_type(p, q0, q1);
+        if (pe_dist2(p, q0, q1, dt) < active2) {
+            switch (dt) {
+            case PE_P_E0: kind = 0; key = vv_key(vi, e0); break;
+            case PE_P_E1: kind = 0; key = vv_key(vi, e1); break;
+            default: kind = 1; key = pair_key(ei, vi);
             }
         }
     }
@@ -183,7 +203,7 @@ void np_build(Ctx& c, double dhat, double dmin)
         throw Error(IPCG_ERR_STATE, "no static candidate set (run the static broad phase)");
     hipStream_t s = c.stream;
     c.coll_valid = false;
-    const int64_t n = c.n_ee + c.n_fv;
+    const int64_t n = c.n_ee + c.n_fv + c.n_vv + c.n_ev;
     const int64_t waves = (n + WAVE - 1) / WAVE;
     const int64_t scap = std::max<int64_t>(WAVE * ((waves + SHARDS - 1) / SHARDS), WAVE);
     const int64_t cap = SHARDS * scap; // entries per stream
@@ -206,7 +226,8 @@ void np_build(Ctx& c, double dhat, double dmin)
         hipLaunchKernelGGL(k_classify, dim3(grid_for(n, NP_BLOCK, INT32_MAX)), dim3(NP_BLOCK), 0, s,
                            c.X[0].as<double>(), c.rest.as<double>(), c.nV, c.E.as<int32_t>(),
                            c.F.as<int32_t>(), c.F2E.as<int32_t>(), c.cand_ee.as<int2>(), c.n_ee,
-                           c.cand_fv.as<int2>(), c.n_fv, off * off, S);
+                           c.cand_fv.as<int2>(), c.n_fv, c.cand_vv.as<int2>(), c.n_vv,
+                           c.cand_ev.as<int2>(), c.n_ev, off * off, S);
     IPCG_CHECK_LAUNCH();
     c.stage_end();
     unsigned long long* h = static_cast<unsigned long long*>(c.pinned);

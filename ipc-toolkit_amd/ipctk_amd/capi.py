@@ -29,7 +29,8 @@ EXPORTS = [
     "ipcg_barrier_hessian", "ipcg_hessian_download", "ipcg_device_results",
     "ipcg_compute_collision_free_stepsize", "ipcg_candidates_stepsize",
     "ipcg_is_step_collision_free", "ipcg_candidates_toi", "ipcg_stage_times",
-    "ipcg_stage_times_reset",
+    "ipcg_stage_times_reset", "ipcg_mesh_codim", "ipcg_codim_candidates_count",
+    "ipcg_codim_candidates_download", "ipcg_codim_candidates_toi",
 ]
 
 _lib = None
@@ -82,6 +83,11 @@ def load(path=LIB_PATH):
     L.ipcg_is_step_collision_free.argtypes = [vp, C.c_double, C.c_double, C.c_int64,
                                               C.c_double, C.c_int, C.POINTER(C.c_int)]
     L.ipcg_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double, _D, _D]
+    L.ipcg_mesh_codim.argtypes = [vp, _I, _I]
+    L.ipcg_codim_candidates_count.argtypes = [vp, _L, _L]
+    L.ipcg_codim_candidates_download.argtypes = [vp, _I, _I]
+    L.ipcg_codim_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double,
+                                            _D, _D]
     L.ipcg_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), _D, _L]
     L.ipcg_stage_times_reset.argtypes = [vp, C.c_int]
     for name in EXPORTS:
@@ -167,6 +173,22 @@ class Context:
         self._check(self._L.ipcg_candidates_download(self._p, ee.ctypes.data_as(_I),
                                                      fv.ctypes.data_as(_I)))
         return ee, fv
+
+    def codim_counts(self):
+        """(codim vertices, codim edges) of the uploaded mesh"""
+        a, b = C.c_int32(), C.c_int32()
+        self._check(self._L.ipcg_mesh_codim(self._p, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def codim_candidates(self):
+        """(vv, ev) codim candidate lists of the last broad phase"""
+        nv, ne = C.c_int64(), C.c_int64()
+        self._check(self._L.ipcg_codim_candidates_count(self._p, C.byref(nv), C.byref(ne)))
+        vv = np.zeros((nv.value, 2), np.int32)
+        ev = np.zeros((ne.value, 2), np.int32)
+        self._check(self._L.ipcg_codim_candidates_download(self._p, vv.ctypes.data_as(_I),
+                                                           ev.ctypes.data_as(_I)))
+        return vv, ev
 
     # ---- collisions --------------------------------------------------------
     def build_collisions(self, dhat, dmin=0.0, method=BP_LBVH):
@@ -257,6 +279,18 @@ class Context:
                                                 rescale, te.ctypes.data_as(_D),
                                                 tf.ctypes.data_as(_D)))
         return te, tf
+
+    def codim_candidates_toi(self, min_distance=0.0, tolerance=1e-6,
+                             max_iterations=10_000_000, rescale=0.8):
+        nv, ne = C.c_int64(), C.c_int64()
+        self._check(self._L.ipcg_codim_candidates_count(self._p, C.byref(nv), C.byref(ne)))
+        tv = np.zeros(nv.value)
+        te = np.zeros(ne.value)
+        self._check(self._L.ipcg_codim_candidates_toi(self._p, min_distance, tolerance,
+                                                      max_iterations, rescale,
+                                                      tv.ctypes.data_as(_D),
+                                                      te.ctypes.data_as(_D)))
+        return tv, te
 
     # ---- timing ------------------------------------------------------------
     def timing(self, enable=True):

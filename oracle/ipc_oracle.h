@@ -149,6 +149,18 @@ int ipco_candidates(const double *V0, const double *V1, int nV,
                     const int32_t *E, int nE, const int32_t *F, int nF,
                     double inflation, const int32_t *vertex_group,
                     ipco_pairs *ee, ipco_pairs *fv);
+/* candidates.cpp:65-118: codim vertex-vertex pairs (each unordered pair
+ * once) and (codim edge, codim vertex) pairs, LBVH semantics (FP32 boxes).
+ * F2E: faces_to_edges (ipco_faces_to_edges).  ipco_codim_lists fills the
+ * codim vertex / edge ids (either may be NULL) and counts[2]. */
+int ipco_codim_lists(int nV, const int32_t *E, int nE, const int32_t *F2E,
+                     int nF, int32_t *cv, int32_t *ce, int32_t *counts);
+int ipco_codim_candidates(const double *V0, const double *V1, int nV,
+                          const int32_t *E, int nE, const int32_t *F, int nF,
+                          const int32_t *F2E, double inflation,
+                          const int32_t *vertex_group, ipco_pairs *vv,
+                          ipco_pairs *ev);
+
 /* Sweep and prune over FP64 boxes (sweep_and_prune.cpp:22-249; sort and
  * sweep restated from scalable-ccd): same filters, edge-edge pairs as
  * (earlier, later) in (min along the sweep axis, id) order.  The sweep axis
@@ -177,12 +189,14 @@ typedef struct ipco_collisions {
 } ipco_collisions;
 
 /* normal_collisions.cpp:38-158 + normal_collisions_builder.cpp:26-336,
- * 547-689 (IPC collision-set type, no area weighting).  Output is sorted
+ * 547-689 (IPC collision-set type, no area weighting); any candidate list
+ * may be NULL.  Output is sorted
  * canonically: vv by (min,max), ev by (edge,vertex), ee by
  * ((min,max),dtype), fv by (face,vertex). */
 int ipco_build_collisions(const double *V, int nV, const double *V_rest,
                           const int32_t *E, int nE, const int32_t *F, int nF,
-                          const int32_t *F2E, const ipco_pairs *ee_cand,
+                          const int32_t *F2E, const ipco_pairs *vv_cand,
+                          const ipco_pairs *ev_cand, const ipco_pairs *ee_cand,
                           const ipco_pairs *fv_cand, double dhat, double dmin,
                           ipco_collisions *out);
 void ipco_collisions_free(ipco_collisions *c);
@@ -236,17 +250,19 @@ int ipco_stencil_ccd(int kind, const double *x_t0, const double *x_t1,
                      double min_distance, double tmax,
                      const ipco_ccd_params *prm, double *toi);
 
-/* candidates.cpp:252-292.  running_tmax != 0 follows the reference's
- * running earliest-toi (order dependent); 0 gives every candidate tmax=1
- * (one valid schedule of the reference, deterministic).  Returns the step
- * in [0,1]. */
+/* candidates.cpp:252-292 over the ee, fv, vv and ev lists (vv / ev may be
+ * NULL).  running_tmax != 0 follows the reference's running earliest-toi
+ * (order dependent); 0 searches every candidate on [0,1] hiding outcomes at
+ * or above the running minimum (deterministic: the minimum of the
+ * per-candidate tois).  Returns the step in [0,1]. */
 double ipco_candidates_stepsize(const double *V0, const double *V1, int nV,
                                 const int32_t *E, const int32_t *F,
                                 const ipco_pairs *ee, const ipco_pairs *fv,
+                                const ipco_pairs *vv, const ipco_pairs *ev,
                                 double min_distance,
                                 const ipco_ccd_params *prm, int running_tmax);
-/* ipc.cpp:45-101 default (LBVH) path: swept candidates with inflation
- * 0.5*min_distance then the above. */
+/* ipc.cpp:45-101 default (LBVH) path: swept candidates (surface and codim
+ * lists) with inflation 0.5*min_distance then the above. */
 double ipco_compute_collision_free_stepsize(
     const double *V0, const double *V1, int nV, const int32_t *E, int nE,
     const int32_t *F, int nF, double min_distance,
@@ -257,10 +273,10 @@ int ipco_is_step_collision_free(const double *V0, const double *V1, int nV,
                                 int nF, double min_distance,
                                 const ipco_ccd_params *prm);
 
-/* per-candidate toi for a candidate list (tmax = 1 each); kind 0 = fv list,
- * 1 = ee list; toi[i] = +inf when no hit. */
+/* per-candidate toi for a candidate list (tmax = 1 each); kind 0 = fv
+ * list, 1 = ee, 2 = ev (edge, vertex), 3 = vv; toi[i] = +inf when no hit. */
 void ipco_candidates_toi(const double *V0, const double *V1, const int32_t *E,
-                         const int32_t *F, const ipco_pairs *cand, int is_ee,
+                         const int32_t *F, const ipco_pairs *cand, int kind,
                          double min_distance, const ipco_ccd_params *prm,
                          double *toi);
 

@@ -290,6 +290,7 @@ static inline int box_overlap(const float *a, const float *b)
 /* ---- candidate filters (lbvh.cpp:810-873, collision_filter.hpp) ---------- */
 typedef struct filt {
     const int32_t *E, *F, *group;
+    const int32_t *vmap, *emap; /* subset -> mesh ids (codim lists) */
 } filt;
 
 static inline int can_vv(const filt *f, int a, int b)
@@ -304,6 +305,14 @@ static int can_ee(const filt *f, int ea, int eb)
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return 0;
     return can_vv(f, a0, b0) || can_vv(f, a0, b1) || can_vv(f, a1, b0)
         || can_vv(f, a1, b1);
+}
+
+/* lbvh.cpp:800-807 can_edge_vertex_collide */
+static int can_ev(const filt *f, int ei, int vi)
+{
+    const int e0 = f->E[2 * (size_t)ei], e1 = f->E[2 * (size_t)ei + 1];
+    if (vi == e0 || vi == e1) return 0;
+    return can_vv(f, vi, e0) || can_vv(f, vi, e1);
 }
 
 static int can_fv(const filt *f, int fi, int vi)
@@ -330,16 +339,35 @@ static void pv_push(pairvec *v, int32_t a, int32_t b)
     v->n++;
 }
 
-/* traverse_lbvh (lbvh.cpp:368-455): kind 0 = EE (triangular),
- * 1 = FV (query = vertex leaf, swap order -> (face, vertex)) */
+/* the pair a query leaf `q` and a target leaf `j` report, if any */
+static void emit(const filt *f, int kind, int q, int j, pairvec *out)
+{
+    switch (kind) {
+    case 0: /* EE (query, other) */
+        if (can_ee(f, q, j)) pv_push(out, q, j);
+        break;
+    case 1: /* FV (face, vertex) */
+        if (can_fv(f, j, q)) pv_push(out, j, q);
+        break;
+    case 2: /* codim VV (query, other) in mesh ids */
+        if (can_vv(f, f->vmap[q], f->vmap[j])) pv_push(out, f->vmap[q], f->vmap[j]);
+        break;
+    default: /* codim EV (edge, vertex) in mesh ids */
+        if (can_ev(f, f->emap[j], f->vmap[q])) pv_push(out, f->emap[j], f->vmap[q]);
+    }
+}
+
+/* traverse_lbvh (lbvh.cpp:368-455): kinds 0 (EE) and 2 (codim VV) are
+ * triangular self traversals; 1 (FV) and 3 (codim EV) query vertex leaves
+ * against the face / codim-edge tree (swap order). */
 static void traverse(const float *qbox, int qprim, int qpos, const lbvh *t,
                      int kind, const filt *f, pairvec *out)
 {
     const int nn = 2 * t->n - 1;
+    const int tri = kind == 0 || kind == 2;
     if (nn == 1) {
-        if (kind == 0) return;
-        if (box_overlap(t->box, qbox) && can_fv(f, t->left[0], qprim))
-            pv_push(out, t->left[0], qprim);
+        if (tri) return;
+        if (box_overlap(t->box, qbox)) emit(f, kind, qprim, t->left[0], out);
         return;
     }
     int stack[128];
@@ -350,21 +378,13 @@ static void traverse(const float *qbox, int qprim, int qpos, const lbvh *t,
         const int l = t->left[node], r = t->right[node];
         int il = box_overlap(t->box + 6 * (size_t)l, qbox);
         int ir = box_overlap(t->box + 6 * (size_t)r, qbox);
-        if (kind == 0) {
+        if (tri) {
             if (il && t->rml[l] <= qpos) il = 0;
             if (ir && t->rml[r] <= qpos) ir = 0;
         }
         const int leaf_l = t->right[l] == 0, leaf_r = t->right[r] == 0;
-        if (il && leaf_l) {
-            const int j = t->left[l];
-            if (kind == 0 ? can_ee(f, qprim, j) : can_fv(f, j, qprim))
-                kind == 0 ? pv_push(out, qprim, j) : pv_push(out, j, qprim);
-        }
-        if (ir && leaf_r) {
-            const int j = t->left[r];
-            if (kind == 0 ? can_ee(f, qprim, j) : can_fv(f, j, qprim))
-                kind == 0 ? pv_push(out, qprim, j) : pv_push(out, j, qprim);
-        }
+        if (il && leaf_l) emit(f, kind, qprim, t->left[l], out);
+        if (ir && leaf_r) emit(f, kind, qprim, t->left[r], out);
         const int tl = il && !leaf_l, tr = ir && !leaf_r;
         if (!tl && !tr) {
             node = stack[--sp];
@@ -387,7 +407,7 @@ static void detect(const lbvh *src, const lbvh *dst, int kind, const filt *f,
     out->n = 0;
     out->ij = NULL;
     if (src->n == 0 || dst->n == 0) return;
-    if (kind == 0 && dst->n <= 1) return;
+    if ((kind == 0 || kind == 2) && dst->n <= 1) return;
     const int LEAF = src->n - 1;
     int nt = ipco_get_max_threads();
     pairvec *loc = (pairvec *)calloc((size_t)nt, sizeof(pairvec));
@@ -439,7 +459,7 @@ int ipco_candidates(const double *V0, const double *V1, int nV,
     lbvh_build(vb, nV, mn, mx, &tv);
     lbvh_build(eb, nE, mn, mx, &te);
     lbvh_build(fb, nF, mn, mx, &tf);
-    filt f = { E, F, vertex_group };
+    filt f = { E, F, vertex_group, NULL, NULL };
     detect(&te, &te, 0, &f, ee); /* lbvh.cpp:742-754 */
     detect(&tv, &tf, 1, &f, fv); /* lbvh.cpp:756-769 */
     lbvh_free(&tv);
@@ -448,6 +468,92 @@ int ipco_candidates(const double *V0, const double *V1, int nV,
     free(vb);
     free(eb);
     free(fb);
+    return 0;
+}
+
+/* ---- codim candidates (candidates.cpp:65-118) ----------------------------
+ * Codim vertices: referenced by no edge; codim edges: in no face
+ * (collision_mesh.cpp:145-182).  Vertex-vertex pairs among codim vertices
+ * (an LBVH over their boxes alone), and (codim edge, codim vertex) pairs.
+ * The reference applies mesh.can_collide to indices of its sub-arrays
+ * there; this restatement applies it to mesh ids. */
+int ipco_codim_lists(int nV, const int32_t *E, int nE, const int32_t *F2E,
+                     int nF, int32_t *cv, int32_t *ce, int32_t *counts)
+{
+    char *used = (char *)calloc((size_t)(nV > 0 ? nV : 1), 1);
+    for (int64_t i = 0; i < 2 * (int64_t)nE; i++) used[E[i]] = 1;
+    int n = 0;
+    for (int i = 0; i < nV; i++)
+        if (!used[i]) {
+            if (cv) cv[n] = i;
+            n++;
+        }
+    counts[0] = n;
+    free(used);
+    char *fe = (char *)calloc((size_t)(nE > 0 ? nE : 1), 1);
+    for (int64_t i = 0; i < 3 * (int64_t)nF; i++) fe[F2E[i]] = 1;
+    n = 0;
+    for (int i = 0; i < nE; i++)
+        if (!fe[i]) {
+            if (ce) ce[n] = i;
+            n++;
+        }
+    counts[1] = n;
+    free(fe);
+    return 0;
+}
+
+int ipco_codim_candidates(const double *V0, const double *V1, int nV,
+                          const int32_t *E, int nE, const int32_t *F, int nF,
+                          const int32_t *F2E, double inflation,
+                          const int32_t *vertex_group, ipco_pairs *vv,
+                          ipco_pairs *ev)
+{
+    vv->n = ev->n = 0;
+    vv->ij = ev->ij = NULL;
+    int32_t counts[2];
+    ipco_codim_lists(nV, E, nE, F2E, nF, NULL, NULL, counts);
+    const int ncv = counts[0], nce = counts[1];
+    if (ncv == 0) return 0;
+    int32_t *cv = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncv);
+    int32_t *ce = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nce > 0 ? nce : 1));
+    ipco_codim_lists(nV, E, nE, F2E, nF, cv, ce, counts);
+    box64 *vb = vertex_boxes64(V0, V1, nV, inflation);
+    box64 *sub = (box64 *)malloc(sizeof(box64) * (size_t)ncv);
+    double mn[3] = { INFINITY, INFINITY, INFINITY };
+    double mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = 0; i < ncv; i++) {
+        sub[i] = vb[cv[i]];
+        for (int k = 0; k < 3; k++) {
+            if (sub[i].lo[k] < mn[k]) mn[k] = sub[i].lo[k];
+            if (mx[k] < sub[i].hi[k]) mx[k] = sub[i].hi[k];
+        }
+    }
+    filt f = { E, F, vertex_group, cv, ce };
+    lbvh tv;
+    lbvh_build(sub, ncv, mn, mx, &tv);
+    detect(&tv, &tv, 2, &f, vv);
+    if (nce > 0) {
+        box64 *eb = (box64 *)malloc(sizeof(box64) * (size_t)nce);
+        for (int i = 0; i < nce; i++) {
+            const int e = ce[i];
+            box_union(&vb[E[2 * (size_t)e]], &vb[E[2 * (size_t)e + 1]], &eb[i]);
+            for (int k = 0; k < 3; k++) {
+                if (eb[i].lo[k] < mn[k]) mn[k] = eb[i].lo[k];
+                if (mx[k] < eb[i].hi[k]) mx[k] = eb[i].hi[k];
+            }
+        }
+        lbvh te;
+        lbvh_build(eb, nce, mn, mx, &te);
+        detect(&tv, &te, 3, &f, ev);
+        lbvh_free(&te);
+        free(eb);
+    }
+    lbvh_free(&tv);
+    free(sub);
+    free(vb);
+    free(cv);
+    free(ce);
     return 0;
 }
 
@@ -535,7 +641,7 @@ int ipco_candidates_sap(const double *V0, const double *V1, int nV,
     box64 *vb = vertex_boxes64(V0, V1, nV, inflation);
     box64 *eb = element_boxes64(vb, E, nE, 2);
     box64 *fb = element_boxes64(vb, F, nF, 3);
-    filt f = { E, F, vertex_group };
+    filt f = { E, F, vertex_group, NULL, NULL };
     const int nt = ipco_get_max_threads();
     /* edge-edge: one list */
     skey *ek = sorted_keys(eb, nE, axis);

@@ -12,6 +12,7 @@
 #include "ipco_internal.h"
 
 #include <limits.h>
+#include <stdio.h>
 
 /* The no_zero_toi retry loop is capped at 64 halvings here and in the HIP
  * kernels (Tight-Inclusion allows INT_MAX); a level cap of 1024 keeps every
@@ -469,67 +470,114 @@ static int stencil_ccd_bounded(int kind, const double *x0, const double *x1,
     return ccd_strategy(is_vf, s, e, min_distance, d0, adj, tmax, prm, bound, toi);
 }
 
+/* candidate kinds: 0 face-vertex (face, vertex), 1 edge-edge, 2 edge-vertex
+ * (edge, vertex), 3 vertex-vertex; dofs in the order of ipco_stencil_ccd */
 static void cand_dofs(const double *V, const int32_t *E, const int32_t *F,
-                      int is_ee, int32_t a, int32_t b, double *x)
+                      int kind, int32_t a, int32_t b, double *x)
 {
     int32_t ids[4];
-    if (is_ee) {
+    int np = 4;
+    switch (kind) {
+    case 1:
         ids[0] = E[2 * (size_t)a];
         ids[1] = E[2 * (size_t)a + 1];
         ids[2] = E[2 * (size_t)b];
         ids[3] = E[2 * (size_t)b + 1];
-    } else { /* (face, vertex) -> (v, f0, f1, f2) */
+        break;
+    case 0: /* (face, vertex) -> (v, f0, f1, f2) */
         ids[0] = b;
         ids[1] = F[3 * (size_t)a];
         ids[2] = F[3 * (size_t)a + 1];
         ids[3] = F[3 * (size_t)a + 2];
+        break;
+    case 2: /* (edge, vertex) -> (v, e0, e1) */
+        ids[0] = b;
+        ids[1] = E[2 * (size_t)a];
+        ids[2] = E[2 * (size_t)a + 1];
+        np = 3;
+        break;
+    default: /* (v0, v1) */
+        ids[0] = a;
+        ids[1] = b;
+        np = 2;
     }
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < np; i++)
         for (int k = 0; k < 3; k++) x[3 * i + k] = V[3 * (size_t)ids[i] + k];
 }
 
 void ipco_candidates_toi(const double *V0, const double *V1, const int32_t *E,
-                         const int32_t *F, const ipco_pairs *cand, int is_ee,
+                         const int32_t *F, const ipco_pairs *cand, int kind,
                          double min_distance, const ipco_ccd_params *prm,
                          double *toi)
 {
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < cand->n; i++) {
         double x0[12], x1[12], t = INFINITY;
-        cand_dofs(V0, E, F, is_ee, cand->ij[2 * i], cand->ij[2 * i + 1], x0);
-        cand_dofs(V1, E, F, is_ee, cand->ij[2 * i], cand->ij[2 * i + 1], x1);
-        if (!ipco_stencil_ccd(is_ee ? 1 : 0, x0, x1, min_distance, 1.0, prm, &t))
-            t = INFINITY;
+        cand_dofs(V0, E, F, kind, cand->ij[2 * i], cand->ij[2 * i + 1], x0);
+        cand_dofs(V1, E, F, kind, cand->ij[2 * i], cand->ij[2 * i + 1], x1);
+        if (!ipco_stencil_ccd(kind, x0, x1, min_distance, 1.0, prm, &t)) t = INFINITY;
         toi[i] = t;
     }
+}
+
+/* the four candidate lists as one index range: ee, fv, vv, ev (any NULL) */
+typedef struct clist {
+    const ipco_pairs *p[4];
+    int kind[4];
+    int64_t off[5];
+} clist;
+
+static void clist_init(clist *c, const ipco_pairs *ee, const ipco_pairs *fv,
+                       const ipco_pairs *vv, const ipco_pairs *ev)
+{
+    const ipco_pairs *p[4] = { ee, fv, vv, ev };
+    const int kind[4] = { 1, 0, 3, 2 };
+    c->off[0] = 0;
+    for (int k = 0; k < 4; k++) {
+        c->p[k] = p[k];
+        c->kind[k] = kind[k];
+        c->off[k + 1] = c->off[k] + (p[k] ? p[k]->n : 0);
+    }
+}
+
+static void clist_get(const clist *c, int64_t i, int *kind, int32_t *a, int32_t *b)
+{
+    int k = 0;
+    while (i >= c->off[k + 1]) k++;
+    const int64_t j = i - c->off[k];
+    *kind = c->kind[k];
+    *a = c->p[k]->ij[2 * j];
+    *b = c->p[k]->ij[2 * j + 1];
 }
 
 double ipco_candidates_stepsize(const double *V0, const double *V1, int nV,
                                 const int32_t *E, const int32_t *F,
                                 const ipco_pairs *ee, const ipco_pairs *fv,
+                                const ipco_pairs *vv, const ipco_pairs *ev,
                                 double min_distance,
                                 const ipco_ccd_params *prm, int running_tmax)
 {
     (void)nV;
-    const int64_t n = ee->n + fv->n;
+    clist L;
+    clist_init(&L, ee, fv, vv, ev);
+    const int64_t n = L.off[4];
     if (n == 0) return 1.0;
     double earliest = 1.0;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < n; i++) {
-        const int is_ee = i < ee->n;
-        const ipco_pairs *c = is_ee ? ee : fv;
-        const int64_t j = is_ee ? i : i - ee->n;
+        int kind;
+        int32_t a, b;
+        clist_get(&L, i, &kind, &a, &b);
         double x0[12], x1[12], t = INFINITY, cur;
 #pragma omp atomic read
         cur = earliest;
-        cand_dofs(V0, E, F, is_ee, c->ij[2 * j], c->ij[2 * j + 1], x0);
-        cand_dofs(V1, E, F, is_ee, c->ij[2 * j], c->ij[2 * j + 1], x1);
+        cand_dofs(V0, E, F, kind, a, b, x0);
+        cand_dofs(V1, E, F, kind, a, b, x1);
         int hit;
         if (running_tmax) /* candidates.cpp:269-287: tmax = running min */
-            hit = ipco_stencil_ccd(is_ee ? 1 : 0, x0, x1, min_distance, cur, prm, &t);
+            hit = ipco_stencil_ccd(kind, x0, x1, min_distance, cur, prm, &t);
         else /* every candidate on [0,1]; outcomes >= the running min hidden */
-            hit = stencil_ccd_bounded(is_ee ? 1 : 0, x0, x1, min_distance, 1.0, prm,
-                                      cur, &t);
+            hit = stencil_ccd_bounded(kind, x0, x1, min_distance, 1.0, prm, cur, &t);
         if (hit) {
 #pragma omp critical(ipco_toi)
             {
@@ -540,17 +588,35 @@ double ipco_candidates_stepsize(const double *V0, const double *V1, int nV,
     return earliest;
 }
 
+/* Candidates::build (dynamic) of candidates.cpp:120-222: surface lists and
+ * the codim lists */
+static void all_candidates(const double *V0, const double *V1, int nV, const int32_t *E,
+                           int nE, const int32_t *F, int nF, double inflation,
+                           ipco_pairs *ee, ipco_pairs *fv, ipco_pairs *vv, ipco_pairs *ev)
+{
+    ipco_candidates(V0, V1, nV, E, nE, F, nF, inflation, NULL, ee, fv);
+    int32_t *F2E = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)(nF > 0 ? nF : 1));
+    if (nF > 0 && ipco_faces_to_edges(E, nE, F, nF, F2E) != 0) {
+        fprintf(stderr, "ipco: a face edge is missing from the edge list\n");
+        abort();
+    }
+    ipco_codim_candidates(V0, V1, nV, E, nE, F, nF, F2E, inflation, NULL, vv, ev);
+    free(F2E);
+}
+
 double ipco_compute_collision_free_stepsize(
     const double *V0, const double *V1, int nV, const int32_t *E, int nE,
     const int32_t *F, int nF, double min_distance,
     const ipco_ccd_params *prm, int running_tmax)
 {
-    ipco_pairs ee, fv;
-    ipco_candidates(V0, V1, nV, E, nE, F, nF, 0.5 * min_distance, NULL, &ee, &fv);
-    const double r = ipco_candidates_stepsize(V0, V1, nV, E, F, &ee, &fv,
+    ipco_pairs ee, fv, vv, ev;
+    all_candidates(V0, V1, nV, E, nE, F, nF, 0.5 * min_distance, &ee, &fv, &vv, &ev);
+    const double r = ipco_candidates_stepsize(V0, V1, nV, E, F, &ee, &fv, &vv, &ev,
                                               min_distance, prm, running_tmax);
     ipco_pairs_free(&ee);
     ipco_pairs_free(&fv);
+    ipco_pairs_free(&vv);
+    ipco_pairs_free(&ev);
     return r;
 }
 
@@ -559,29 +625,33 @@ int ipco_is_step_collision_free(const double *V0, const double *V1, int nV,
                                 int nF, double min_distance,
                                 const ipco_ccd_params *prm)
 {
-    ipco_pairs ee, fv;
-    ipco_candidates(V0, V1, nV, E, nE, F, nF, 0.5 * min_distance, NULL, &ee, &fv);
+    ipco_pairs ee, fv, vv, ev;
+    all_candidates(V0, V1, nV, E, nE, F, nF, 0.5 * min_distance, &ee, &fv, &vv, &ev);
+    clist L;
+    clist_init(&L, &ee, &fv, &vv, &ev);
     int free_ = 1;
-    const int64_t n = ee.n + fv.n;
+    const int64_t n = L.off[4];
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < n; i++) {
         int done;
 #pragma omp atomic read
         done = free_;
         if (!done) continue;
-        const int is_ee = i < ee.n;
-        const ipco_pairs *c = is_ee ? &ee : &fv;
-        const int64_t j = is_ee ? i : i - ee.n;
+        int kind;
+        int32_t a, b;
+        clist_get(&L, i, &kind, &a, &b);
         double x0[12], x1[12], t;
-        cand_dofs(V0, E, F, is_ee, c->ij[2 * j], c->ij[2 * j + 1], x0);
-        cand_dofs(V1, E, F, is_ee, c->ij[2 * j], c->ij[2 * j + 1], x1);
-        if (ipco_stencil_ccd(is_ee ? 1 : 0, x0, x1, min_distance, 1.0, prm, &t)) {
+        cand_dofs(V0, E, F, kind, a, b, x0);
+        cand_dofs(V1, E, F, kind, a, b, x1);
+        if (ipco_stencil_ccd(kind, x0, x1, min_distance, 1.0, prm, &t)) {
 #pragma omp atomic write
             free_ = 0;
         }
     }
     ipco_pairs_free(&ee);
     ipco_pairs_free(&fv);
+    ipco_pairs_free(&vv);
+    ipco_pairs_free(&ev);
     return free_;
 }
 

@@ -30,6 +30,31 @@ static inline const double *vx(const double *V, int i)
     return V + 3 * (size_t)i;
 }
 
+/* normal_collisions_builder.cpp:26-54 add_vertex_vertex_collision */
+static void vv_candidate(const double *V, int32_t vi, int32_t vj, double active2, rec *r)
+{
+    r->type = -1;
+    const double d = ipco_point_point_distance(vx(V, vi), vx(V, vj));
+    if (d < active2) vv_rec(r, vi, vj);
+}
+
+/* normal_collisions_builder.cpp:56-138 add_edge_vertex_collision */
+static void ev_candidate(const double *V, const int32_t *E, int32_t ei, int32_t vi,
+                         double active2, rec *r)
+{
+    r->type = -1;
+    const int32_t e0 = E[2 * (size_t)ei], e1 = E[2 * (size_t)ei + 1];
+    const double *p = vx(V, vi), *q0 = vx(V, e0), *q1 = vx(V, e1);
+    const int dt = ipco_point_edge_distance_type(p, q0, q1);
+    const double d = ipco_point_edge_distance(p, q0, q1, dt);
+    if (!(d < active2)) return;
+    switch (dt) {
+    case IPCO_PE_P_E0: vv_rec(r, vi, e0); break;
+    case IPCO_PE_P_E1: vv_rec(r, vi, e1); break;
+    default: ev_rec(r, ei, vi);
+    }
+}
+
 /* normal_collisions_builder.cpp:239-336 add_face_vertex_collision */
 static void fv_candidate(const double *V, const int32_t *F, const int32_t *F2E,
                          int32_t fi, int32_t vi, double active2, rec *r)
@@ -103,7 +128,8 @@ static uint64_t rec_key(const rec *r)
 
 int ipco_build_collisions(const double *V, int nV, const double *V_rest,
                           const int32_t *E, int nE, const int32_t *F, int nF,
-                          const int32_t *F2E, const ipco_pairs *ee_cand,
+                          const int32_t *F2E, const ipco_pairs *vv_cand,
+                          const ipco_pairs *ev_cand, const ipco_pairs *ee_cand,
                           const ipco_pairs *fv_cand, double dhat, double dmin,
                           ipco_collisions *out)
 {
@@ -116,8 +142,18 @@ int ipco_build_collisions(const double *V, int nV, const double *V_rest,
     const double active2 = off * off; /* normal_collisions.cpp:52-54 */
     const int64_t nfv = fv_cand ? fv_cand->n : 0;
     const int64_t nee = ee_cand ? ee_cand->n : 0;
-    const int64_t n = nfv + nee;
+    const int64_t nvv = vv_cand ? vv_cand->n : 0;
+    const int64_t nev = ev_cand ? ev_cand->n : 0;
+    const int64_t n = nfv + nee + nvv + nev;
     rec *R = (rec *)malloc(sizeof(rec) * (size_t)(n > 0 ? n : 1));
+    rec *Rc = R + nee + nfv; /* codim candidates after the surface ones */
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nvv; i++)
+        vv_candidate(V, vv_cand->ij[2 * i], vv_cand->ij[2 * i + 1], active2, &Rc[i]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nev; i++)
+        ev_candidate(V, E, ev_cand->ij[2 * i], ev_cand->ij[2 * i + 1], active2,
+                     &Rc[nvv + i]);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nee; i++)
         ee_candidate(V, V_rest, E, ee_cand->ij[2 * i], ee_cand->ij[2 * i + 1],

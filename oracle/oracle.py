@@ -108,7 +108,12 @@ def lib():
         L.ipco_pairs_free.argtypes = [C.POINTER(_Pairs)]
         L.ipco_build_collisions.argtypes = [_D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I,
                                             C.POINTER(_Pairs), C.POINTER(_Pairs),
+                                            C.POINTER(_Pairs), C.POINTER(_Pairs),
                                             C.c_double, C.c_double, C.POINTER(_Collisions)]
+        L.ipco_codim_candidates.argtypes = [_D, _D, C.c_int, _I, C.c_int, _I, C.c_int, _I,
+                                            C.c_double, _I, C.POINTER(_Pairs),
+                                            C.POINTER(_Pairs)]
+        L.ipco_codim_lists.argtypes = [C.c_int, _I, C.c_int, _I, C.c_int, _I, _I, _I]
         L.ipco_collisions_free.argtypes = [C.POINTER(_Collisions)]
         L.ipco_barrier_energy.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
                                           C.c_double, C.c_double]
@@ -123,6 +128,7 @@ def lib():
         L.ipco_stencil_ccd.argtypes = [C.c_int, _D, _D, C.c_double, C.c_double,
                                        C.POINTER(CCDParams), _D]
         L.ipco_candidates_stepsize.argtypes = [_D, _D, C.c_int, _I, _I, C.POINTER(_Pairs),
+                                               C.POINTER(_Pairs), C.POINTER(_Pairs),
                                                C.POINTER(_Pairs), C.c_double,
                                                C.POINTER(CCDParams), C.c_int]
         L.ipco_candidates_stepsize.restype = C.c_double
@@ -326,6 +332,40 @@ def candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None, method="lbvh
     return _take_pairs(ee), _take_pairs(fv)
 
 
+def codim_lists(nV, E, F):
+    """(codim vertex ids, codim edge ids): vertices in no edge, edges in no
+    face (collision_mesh.cpp:145-182)."""
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F2E, pF2E = _i(faces_to_edges(E, F) if len(F) else np.zeros((0, 3), np.int32))
+    cnt = np.zeros(2, np.int32)
+    lib().ipco_codim_lists(nV, pE, len(E), pF2E, len(F2E), None, None,
+                           cnt.ctypes.data_as(_I))
+    cv = np.zeros(max(cnt[0], 1), np.int32)
+    ce = np.zeros(max(cnt[1], 1), np.int32)
+    lib().ipco_codim_lists(nV, pE, len(E), pF2E, len(F2E), cv.ctypes.data_as(_I),
+                           ce.ctypes.data_as(_I), cnt.ctypes.data_as(_I))
+    return cv[:cnt[0]], ce[:cnt[1]]
+
+
+def codim_candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None):
+    """(vv, ev) codim candidate arrays (candidates.cpp:65-118): vv pairs of
+    codim vertices, ev rows (codim edge, codim vertex)."""
+    V0, p0 = _d(V0)
+    p1 = None
+    if V1 is not None:
+        V1, p1 = _d(V1)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    F2E, pF2E = _i(faces_to_edges(E, F) if len(F) else np.zeros((0, 3), np.int32))
+    pg = None
+    if vertex_group is not None:
+        vertex_group, pg = _i(vertex_group)
+    vv, ev = _Pairs(), _Pairs()
+    lib().ipco_codim_candidates(p0, p1, len(V0), pE, len(E), pF, len(F), pF2E, inflation,
+                                pg, C.byref(vv), C.byref(ev))
+    return _take_pairs(vv), _take_pairs(ev)
+
+
 def sap_axis(V0, inflation=0.0, V1=None):
     V0, p0 = _d(V0)
     p1 = None
@@ -370,20 +410,24 @@ class Collisions:
             pass
 
 
-def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None):
+def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_cand=None,
+                     ev_cand=None):
     V, pV = _d(V)
     if V_rest is None:
         V_rest = V
     V_rest, pR = _d(V_rest)
-    E, pE = _i(E)
-    F, pF = _i(F)
-    F2E = faces_to_edges(E, F)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    F2E = faces_to_edges(E, F) if len(F) else np.zeros((0, 3), np.int32)
     F2E, pF2E = _i(F2E)
     ee_s, ee_a = _pairs_struct(ee_cand)
     fv_s, fv_a = _pairs_struct(fv_cand)
+    vv_s, vv_a = _pairs_struct(np.zeros((0, 2)) if vv_cand is None else vv_cand)
+    ev_s, ev_a = _pairs_struct(np.zeros((0, 2)) if ev_cand is None else ev_cand)
     out = _Collisions()
     lib().ipco_build_collisions(pV, len(V), pR, pE, len(E), pF, len(F), pF2E,
-                                C.byref(ee_s), C.byref(fv_s), dhat, dmin, C.byref(out))
+                                C.byref(vv_s), C.byref(ev_s), C.byref(ee_s), C.byref(fv_s),
+                                dhat, dmin, C.byref(out))
     return Collisions(out, (E, F))
 
 
@@ -447,29 +491,37 @@ def stencil_ccd(kind, x_t0, x_t1, min_distance=0.0, tmax=1.0, params=DEFAULT_CCD
     return bool(hit), toi.value
 
 
+CAND_KIND = {"fv": 0, "ee": 1, "ev": 2, "vv": 3}
+
+
 def candidates_toi(V0, V1, E, F, cand, is_ee, min_distance=0.0, params=DEFAULT_CCD):
+    """is_ee: True (ee list), False (fv list) or a kind name ("ee", "fv",
+    "ev", "vv")."""
     V0, p0 = _d(V0)
     V1, p1 = _d(V1)
     E, pE = _i(E)
     F, pF = _i(F)
     s, a = _pairs_struct(cand)
     out = np.zeros(len(a))
-    lib().ipco_candidates_toi(p0, p1, pE, pF, C.byref(s), int(is_ee), min_distance,
+    kind = CAND_KIND[is_ee] if isinstance(is_ee, str) else int(bool(is_ee))
+    lib().ipco_candidates_toi(p0, p1, pE, pF, C.byref(s), kind, min_distance,
                               C.byref(params), out.ctypes.data_as(_D))
     return out
 
 
 def candidates_stepsize(V0, V1, E, F, ee, fv, min_distance=0.0, params=DEFAULT_CCD,
-                        running_tmax=False):
+                        running_tmax=False, vv=None, ev=None):
     V0, p0 = _d(V0)
     V1, p1 = _d(V1)
-    E, pE = _i(E)
-    F, pF = _i(F)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
     es, ea = _pairs_struct(ee)
     fs, fa = _pairs_struct(fv)
+    vs, va = _pairs_struct(np.zeros((0, 2)) if vv is None else vv)
+    ws, wa = _pairs_struct(np.zeros((0, 2)) if ev is None else ev)
     return lib().ipco_candidates_stepsize(p0, p1, len(V0), pE, pF, C.byref(es),
-                                          C.byref(fs), min_distance, C.byref(params),
-                                          int(running_tmax))
+                                          C.byref(fs), C.byref(vs), C.byref(ws),
+                                          min_distance, C.byref(params), int(running_tmax))
 
 
 def compute_collision_free_stepsize(V0, V1, E, F, min_distance=0.0, params=DEFAULT_CCD,

@@ -240,3 +240,89 @@ def test_sweep_and_prune(ctx, capi, O, scenes, name):
     assert t_sap >= ctx.stepsize()
     ee, fv = O.candidates(s.V, s.E, s.F, 0.0, V1=s.V1, method="sap")
     assert t_sap == O.candidates_stepsize(s.V, s.V1, s.E, s.F, ee, fv)
+
+
+def codim_soup(seed=3):
+    """free points, free segments and a few triangles in the unit cube"""
+    rng = np.random.default_rng(seed)
+    V = rng.uniform(0, 1, (120, 3))
+    F = np.array([[0, 1, 2], [3, 4, 5], [3, 5, 6]], dtype=np.int32)
+    from ipctk_amd.scenes import edges_from_faces
+    seg = np.array([[7 + 2 * i, 8 + 2 * i] for i in range(30)], dtype=np.int32)
+    E = np.vstack([edges_from_faces(F), seg]).astype(np.int32)
+    V1 = V + rng.normal(0, 0.03, V.shape)
+    return V, E, F, V1
+
+
+def test_codim_candidates_collisions_ccd(ctx, O):
+    V, E, F, V1 = codim_soup()
+    say("case codim soup", len(V), len(E), len(F))
+    ctx.upload_mesh(V, E, F)
+    ctx.upload_positions(0, V)
+    ctx.upload_positions(1, V1)
+    cv, ce = O.codim_lists(len(V), E, F)
+    assert ctx.codim_counts() == (len(cv), len(ce))
+    for swept, r in ((False, 0.02), (True, 0.0)):
+        ctx.broad_phase(swept, r)
+        vv_g, ev_g = ctx.codim_candidates()
+        vv_o, ev_o = O.codim_candidates(V, E, F, r, V1=V1 if swept else None)
+        assert {tuple(sorted(p)) for p in vv_g.tolist()} == {tuple(sorted(p)) for p in vv_o.tolist()}
+        assert len(vv_g) == len(vv_o)
+        assert pair_set(ev_g) == pair_set(ev_o) and len(ev_g) == len(ev_o)
+        if swept:
+            tv, te = ctx.codim_candidates_toi()
+            assert np.array_equal(tv, O.candidates_toi(V, V1, E, F, vv_g, "vv"))
+            assert np.array_equal(te, O.candidates_toi(V, V1, E, F, ev_g, "ev"))
+    dhat = 0.05
+    ctx.build_collisions(dhat)
+    cg = ctx.collisions()
+    ee, fv = O.candidates(V, E, F, 0.5 * dhat)
+    vv, ev = O.codim_candidates(V, E, F, 0.5 * dhat)
+    co = O.build_collisions(V, E, F, ee, fv, dhat, vv_cand=vv, ev_cand=ev)
+    for k in ("vv", "ev", "ee", "fv"):
+        assert np.array_equal(getattr(co, k), cg[k]) and np.array_equal(getattr(co, k + "_w"), cg[k + "_w"])
+    assert len(co.vv) + len(co.ev) > 0
+    assert rel_err(ctx.gradient(dhat, 1.0), O.barrier_gradient(co, V, E, F, dhat, 1.0)) <= TOL_SUM
+    assert sparse_rel_err(ctx.hessian(dhat, 1.0, "clamp"),
+                          O.barrier_hessian(co, V, E, F, dhat, 1.0, "clamp")) <= TOL_HESS
+    assert ctx.stepsize() == O.compute_collision_free_stepsize(V, V1, E, F)
+
+
+def test_codim_kats(ctx, O):
+    """test_normal_collisions.cpp:14-206 through the device path"""
+    E0, F0 = np.zeros((0, 2), np.int32), np.zeros((0, 3), np.int32)
+    V = np.array([[0, 0, 0], [0, 0, 1], [0, 1, 0], [0, 1, 1], [1, 0, 0], [1, 0, 1],
+                  [1, 1, 0], [1, 1, 1]], dtype=np.float64)
+    V -= V.mean(axis=0)
+    say("case codim cube")
+    ctx.upload_mesh(V, E0, F0)
+    ctx.upload_positions(0, V)
+    ctx.build_collisions(0.25, 0.8)
+    assert ctx.collision_counts().tolist() == [12, 0, 0, 0]
+    g = ctx.gradient(0.25, 1.0).reshape(-1, 3)
+    f = -g / np.linalg.norm(g, axis=1, keepdims=True)
+    assert np.allclose(f, V / np.linalg.norm(V, axis=1, keepdims=True), atol=1e-12)
+    V1 = V.copy()
+    V1[:, 1] *= 0.5
+    ctx.upload_positions(1, V1)
+    assert not ctx.is_step_collision_free(min_distance=0.8)
+    assert ctx.stepsize(min_distance=0.8) == O.compute_collision_free_stepsize(
+        V, V1, E0, F0, min_distance=0.8)
+    # edge-vertex KAT
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 0, -1], [-1, 0, 0], [0, 0, 1], [0, 1, 0],
+                  [0, 2, 0], [0, 3, 0]], dtype=np.float64)
+    E = np.array([[0, 1], [0, 2], [0, 3], [0, 4]], dtype=np.int32)
+    V1 = V.copy()
+    V1[-3:, 1] -= 4
+    say("case codim edge-vertex")
+    ctx.upload_mesh(V, E, F0)
+    ctx.upload_positions(0, V)
+    ctx.upload_positions(1, V1)
+    ctx.broad_phase(True, 1e-3)
+    vv, ev = ctx.codim_candidates()
+    ee, fv = ctx.candidates()
+    assert (len(vv), len(ev), len(ee), len(fv)) == (3, 12, 0, 0)
+    t = ctx.stepsize(min_distance=2e-3)
+    assert t == pytest.approx((1 - (2e-3 + 1e-4)) / 4, rel=1.2e-5)
+    ctx.build_collisions(0.25, 0.8)
+    assert ctx.collision_counts().tolist() == [2, 4, 0, 0]
