@@ -139,28 +139,33 @@ def main():
     ms_step = 1e3 * total / args.steps
     value = pairs * args.steps / total
 
-    # roofline of the dominant stage (HIP events on the context stream)
-    per_stage = {k: v[0] / v[1] for k, v in stages.items()}
-    dom = max(per_stage, key=per_stage.get)
+    # Roofline of the dominant kernel.  Each stage below is one kernel
+    # launch per call (HIP events on the context stream); its algorithmic
+    # bytes per launch follow DESIGN.md section 3.
+    per_launch = {k: v[0] / v[1] for k, v in stages.items()}
+    per_step = {k: v[0] / args.steps for k, v in stages.items()}
+    n_vv, n_ev, n_ee, n_fv = [int(x) for x in counts]
     stage_bytes = {
-        "bp_boxes": 24 * nV,
-        "bp_sort": 24 * (nV + nE + nF),
-        "bp_build": 64 * (nE + nF),
-        "bp_traverse": 8 * (ne_s + nf_s),
+        "bp_traverse_ee.static": 24 * nE + 8 * ne_s,
+        "bp_traverse_fv.static": 24 * nV + 8 * nf_s,
+        "bp_traverse_ee.swept": 24 * nE + 8 * ne_d,
+        "bp_traverse_fv.swept": 24 * nV + 8 * nf_d,
         "np_classify": 8 * (ne_s + nf_s) + 16 * pairs,
-        "np_merge": 16 * pairs,
-        "energy": 16 * pairs,
+        "energy": 16 * pairs + 8,
         "gradient": 16 * pairs + 24 * nV,
-        "hessian_local": 16 * pairs,
+        "hessian_scatter_ee": 16 * n_ee,
+        "hessian_scatter_fv": 16 * n_fv,
+        "hessian_scatter_vv_ev": 16 * (n_vv + n_ev),
         "hessian_assemble": 12 * nnz + 8 * (3 * nV + 1),
-        "ccd_lanes": 8 * (ne_d + nf_d) + 48 * nV,
-        "ccd_wavefront": 8 * (ne_d + nf_d),
+        "ccd_root": (8 + 4 * 2 * 32) * (ne_d + nf_d),
     }
-    dom_bytes = stage_bytes.get(dom, 0) / max(1, stages[dom][1] // args.steps)
-    achieved = dom_bytes / (per_stage[dom] * 1e-3) / 1e9
+    known = {k: v for k, v in per_step.items() if k in stage_bytes}
+    dom = max(known, key=known.get)
+    achieved = stage_bytes[dom] / (per_launch[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": dom, "ms_per_launch": round(per_stage[dom], 4)}
+            "kernel": dom, "ms_per_launch": round(per_launch[dom], 4),
+            "bytes_per_launch": int(stage_bytes[dom])}
 
     cpu = None
     if args.cpu_tile > 0:
@@ -189,7 +194,7 @@ def main():
                    "collisions": pairs, "collisions_by_type": [int(x) for x in counts],
                    "candidates_static": ne_s + nf_s, "candidates_swept": ne_d + nf_d,
                    "hessian_nnz": int(nnz), "step_size": step,
-                   "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+                   "stage_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
                    "bytes_algorithmic": algorithmic_bytes(sc, counts, ne_s + nf_s,
                                                           ne_d + nf_d, nnz)},
         "roofline": roof,

@@ -1,0 +1,450 @@
+// ipc_amd.hpp -- C++ host API of the MI355X contact path, mirroring the
+// reference's interface for it (IPC Toolkit v1.6.0):
+//
+//   ipc::CollisionMesh          collision_mesh.hpp:14-126
+//   ipc::Candidates             candidates/candidates.hpp (build static /
+//                               swept, compute_collision_free_stepsize,
+//                               is_step_collision_free)
+//   ipc::NormalCollisions       collisions/normal/normal_collisions.hpp:45-63
+//   ipc::BarrierPotential       potentials/barrier_potential.hpp,
+//                               potentials/potential.hpp:34-60
+//   ipc::compute_collision_free_stepsize, ipc::is_step_collision_free
+//                               ipc.hpp (ipc.cpp:20-101)
+//   ipc::TightInclusionCCD      ccd/tight_inclusion_ccd.hpp:11-28 (parameters)
+//   ipc::PSDProjectionMethod    utils/eigen_ext.hpp:202-206
+//
+// Same names, argument meanings and defaults; errors are thrown as
+// std::runtime_error with the library's message, as the reference's
+// log_and_throw_error does (logger.cpp:41-45).  Everything runs through the
+// C ABI of ipcg.h on one HIP device; a CollisionMesh owns one device
+// context and keeps topology and positions resident.
+//
+// Matrices are small dense row-major containers (no Eigen dependency):
+// MatrixXd V{{x0, y0, z0}, {x1, y1, z1}, ...}, MatrixXi E{{0, 1}, ...}.
+// Gradients are RowMajor [x0, y0, z0, x1, ...] (config.hpp.in:45) and
+// Hessians are CSR (the matrix is symmetric, so CSR equals the reference's
+// column-major CSC).
+#pragma once
+
+#include "ipcg.h"
+
+#include <cstdint>
+#include <initializer_list>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ipc {
+
+template <class T> class Matrix {
+public:
+    Matrix() = default;
+    Matrix(int rows, int cols) : r(rows), c(cols), v((size_t)rows * cols) {}
+    Matrix(std::initializer_list<std::initializer_list<T>> rows_)
+    {
+        r = (int)rows_.size();
+        c = r ? (int)rows_.begin()->size() : 0;
+        v.reserve((size_t)r * c);
+        for (const auto& row : rows_) {
+            if ((int)row.size() != c) throw std::invalid_argument("ragged matrix literal");
+            v.insert(v.end(), row.begin(), row.end());
+        }
+    }
+    int rows() const { return r; }
+    int cols() const { return c; }
+    T& operator()(int i, int j) { return v[(size_t)i * c + j]; }
+    const T& operator()(int i, int j) const { return v[(size_t)i * c + j]; }
+    T* data() { return v.data(); }
+    const T* data() const { return v.data(); }
+
+private:
+    int r = 0, c = 0;
+    std::vector<T> v;
+};
+using MatrixXd = Matrix<double>;
+using MatrixXi = Matrix<int32_t>;
+
+enum class PSDProjectionMethod : int { NONE = IPCG_PSD_NONE, CLAMP = IPCG_PSD_CLAMP,
+                                       ABS = IPCG_PSD_ABS };
+
+// broad_phase/broad_phase.hpp: the device broad phases
+enum class BroadPhaseMethod : int { LBVH = IPCG_BP_LBVH, SWEEP_AND_PRUNE = IPCG_BP_SAP };
+
+namespace detail {
+inline void check(ipcg_ctx* c, int rc)
+{
+    if (rc != IPCG_OK) throw std::runtime_error(std::string(c ? ipcg_last_error(c) : "ipcg"));
+}
+} // namespace detail
+
+// ccd/tight_inclusion_ccd.hpp:11-28
+struct TightInclusionCCD {
+    static constexpr double DEFAULT_TOLERANCE = 1e-6;
+    static constexpr long DEFAULT_MAX_ITERATIONS = 10'000'000;
+    static constexpr double DEFAULT_CONSERVATIVE_RESCALING = 0.8;
+    double tolerance = DEFAULT_TOLERANCE;
+    long max_iterations = DEFAULT_MAX_ITERATIONS;
+    double conservative_rescaling = DEFAULT_CONSERVATIVE_RESCALING;
+    TightInclusionCCD(double tol = DEFAULT_TOLERANCE, long max_iter = DEFAULT_MAX_ITERATIONS,
+                      double rescaling = DEFAULT_CONSERVATIVE_RESCALING)
+        : tolerance(tol), max_iterations(max_iter), conservative_rescaling(rescaling)
+    {
+    }
+};
+
+// collision_mesh.hpp:14-126 (full-mesh form; vertex_patches gives the
+// patch filter of collision_filter.hpp:113-118)
+class CollisionMesh {
+public:
+    CollisionMesh(const MatrixXd& rest_positions, const MatrixXi& edges = MatrixXi(0, 2),
+                  const MatrixXi& faces = MatrixXi(0, 3),
+                  const std::vector<int32_t>& vertex_patches = {}, int device = 0)
+        : nV(rest_positions.rows()), nE(edges.rows()), nF(faces.rows()), E(edges), F(faces)
+    {
+        if (rest_positions.rows() && rest_positions.cols() != 3)
+            throw std::invalid_argument("positions must be #V x 3");
+        if ((edges.rows() && edges.cols() != 2) || (faces.rows() && faces.cols() != 3))
+            throw std::invalid_argument("edges must be #E x 2 and faces #F x 3");
+        ipcg_ctx* c = nullptr;
+        const int rc = ipcg_ctx_create(device, &c);
+        if (rc != IPCG_OK) throw std::runtime_error("no HIP device for the collision mesh");
+        ctx.reset(c);
+        if (!vertex_patches.empty() && (int)vertex_patches.size() != nV)
+            throw std::invalid_argument("one patch id per vertex");
+        detail::check(c, ipcg_mesh_upload(c, nV, rest_positions.data(), IPCG_LAYOUT_AOS, nE,
+                                          edges.data(), nF, faces.data(),
+                                          vertex_patches.empty() ? nullptr
+                                                                 : vertex_patches.data()));
+        detail::check(c, ipcg_mesh_codim(c, &ncv, &nce));
+    }
+    int num_vertices() const { return nV; }
+    int num_edges() const { return nE; }
+    int num_faces() const { return nF; }
+    int num_codim_vertices() const { return ncv; }
+    int num_codim_edges() const { return nce; }
+    const MatrixXi& edges() const { return E; }
+    const MatrixXi& faces() const { return F; }
+    ipcg_ctx* context() const { return ctx.get(); }
+    // counts the device broad-phase calls, so a Candidates object can tell
+    // whether the device still holds its lists
+    uint64_t broad_phase_generation() const { return generation; }
+    void touch_broad_phase() const { generation++; }
+    // counts the device collision-set builds
+    uint64_t collisions_generation() const { return cgeneration; }
+    void touch_collisions() const { cgeneration++; }
+
+    // slot 0: x / x_t0, slot 1: x_t1
+    void set_positions(int slot, const MatrixXd& V) const
+    {
+        if (V.rows() != nV || (nV && V.cols() != 3))
+            throw std::invalid_argument("positions must be #V x 3");
+        detail::check(ctx.get(), ipcg_positions_upload(ctx.get(), slot, V.data(), IPCG_LAYOUT_AOS));
+    }
+
+private:
+    struct Del {
+        void operator()(ipcg_ctx* c) const { ipcg_ctx_destroy(c); }
+    };
+    int nV, nE, nF;
+    int32_t ncv = 0, nce = 0;
+    mutable uint64_t generation = 0, cgeneration = 0;
+    MatrixXi E, F;
+    std::unique_ptr<ipcg_ctx, Del> ctx;
+};
+
+struct VertexVertexCandidate { int32_t vertex0_id, vertex1_id; };
+struct EdgeVertexCandidate { int32_t edge_id, vertex_id; };
+struct EdgeEdgeCandidate { int32_t edge0_id, edge1_id; };
+struct FaceVertexCandidate { int32_t face_id, vertex_id; };
+
+// candidates/candidates.hpp
+class Candidates {
+public:
+    std::vector<VertexVertexCandidate> vv_candidates;
+    std::vector<EdgeVertexCandidate> ev_candidates;
+    std::vector<EdgeEdgeCandidate> ee_candidates;
+    std::vector<FaceVertexCandidate> fv_candidates;
+
+    // candidates.cpp:43-118 (static)
+    void build(const CollisionMesh& mesh, const MatrixXd& vertices, double inflation_radius = 0,
+               BroadPhaseMethod bp = BroadPhaseMethod::LBVH)
+    {
+        mesh.set_positions(0, vertices);
+        mesh.touch_broad_phase();
+        detail::check(mesh.context(), ipcg_broad_phase(mesh.context(), 0, inflation_radius, int(bp)));
+        download(mesh);
+        swept_for = nullptr;
+    }
+    // candidates.cpp:120-222 (swept)
+    void build(const CollisionMesh& mesh, const MatrixXd& vertices_t0, const MatrixXd& vertices_t1,
+               double inflation_radius = 0, BroadPhaseMethod bp = BroadPhaseMethod::LBVH)
+    {
+        mesh.set_positions(0, vertices_t0);
+        mesh.set_positions(1, vertices_t1);
+        mesh.touch_broad_phase();
+        detail::check(mesh.context(), ipcg_broad_phase(mesh.context(), 1, inflation_radius, int(bp)));
+        download(mesh);
+        swept_for = &mesh;
+        generation = mesh.broad_phase_generation();
+        inflation = inflation_radius;
+        method = bp;
+    }
+    size_t size() const
+    {
+        return vv_candidates.size() + ev_candidates.size() + ee_candidates.size()
+            + fv_candidates.size();
+    }
+    bool empty() const { return size() == 0; }
+
+    // candidates.cpp:252-292 over the device lists of the last swept build
+    double compute_collision_free_stepsize(const CollisionMesh& mesh, const MatrixXd& vertices_t0,
+                                           const MatrixXd& vertices_t1, double min_distance = 0,
+                                           const TightInclusionCCD& ccd = {}) const
+    {
+        require_swept(mesh, vertices_t0, vertices_t1);
+        double step = 1;
+        detail::check(mesh.context(),
+                      ipcg_candidates_stepsize(mesh.context(), min_distance, ccd.tolerance,
+                                               ccd.max_iterations, ccd.conservative_rescaling,
+                                               &step));
+        return step;
+    }
+    // candidates.cpp:224-250
+    bool is_step_collision_free(const CollisionMesh& mesh, const MatrixXd& vertices_t0,
+                                const MatrixXd& vertices_t1, double min_distance = 0,
+                                const TightInclusionCCD& ccd = {}) const
+    {
+        require_swept(mesh, vertices_t0, vertices_t1);
+        int free_ = 0;
+        detail::check(mesh.context(), ipcg_candidates_is_step_collision_free(
+                                          mesh.context(), min_distance, ccd.tolerance,
+                                          ccd.max_iterations, ccd.conservative_rescaling, &free_));
+        return free_ != 0;
+    }
+
+private:
+    const CollisionMesh* swept_for = nullptr;
+    uint64_t generation = 0;
+    double inflation = 0;
+    BroadPhaseMethod method = BroadPhaseMethod::LBVH;
+    // The device evaluates its own copy of the lists; when another call
+    // replaced them since build(), the swept broad phase runs again with the
+    // same trajectory, inflation and method, which reproduces the same lists.
+    void require_swept(const CollisionMesh& mesh, const MatrixXd& V0, const MatrixXd& V1) const
+    {
+        if (swept_for != &mesh)
+            throw std::runtime_error("candidates were not built from this mesh's trajectory");
+        mesh.set_positions(0, V0);
+        mesh.set_positions(1, V1);
+        if (mesh.broad_phase_generation() != generation) {
+            mesh.touch_broad_phase();
+            detail::check(mesh.context(),
+                          ipcg_broad_phase(mesh.context(), 1, inflation, int(method)));
+            const_cast<Candidates*>(this)->generation = mesh.broad_phase_generation();
+        }
+    }
+    void download(const CollisionMesh& mesh)
+    {
+        ipcg_ctx* c = mesh.context();
+        int64_t ne = 0, nf = 0, nv = 0, nev = 0;
+        detail::check(c, ipcg_candidates_count(c, &ne, &nf));
+        detail::check(c, ipcg_codim_candidates_count(c, &nv, &nev));
+        std::vector<int32_t> ee(2 * ne), fv(2 * nf), vv(2 * nv), ev(2 * nev);
+        detail::check(c, ipcg_candidates_download(c, ee.data(), fv.data()));
+        detail::check(c, ipcg_codim_candidates_download(c, vv.data(), ev.data()));
+        ee_candidates.resize(ne);
+        fv_candidates.resize(nf);
+        vv_candidates.resize(nv);
+        ev_candidates.resize(nev);
+        for (int64_t i = 0; i < ne; i++) ee_candidates[i] = { ee[2 * i], ee[2 * i + 1] };
+        for (int64_t i = 0; i < nf; i++) fv_candidates[i] = { fv[2 * i], fv[2 * i + 1] };
+        for (int64_t i = 0; i < nv; i++) vv_candidates[i] = { vv[2 * i], vv[2 * i + 1] };
+        for (int64_t i = 0; i < nev; i++) ev_candidates[i] = { ev[2 * i], ev[2 * i + 1] };
+    }
+};
+
+struct VertexVertexNormalCollision { int32_t vertex0_id, vertex1_id; double weight; };
+struct EdgeVertexNormalCollision { int32_t edge_id, vertex_id; double weight; };
+struct EdgeEdgeNormalCollision {
+    int32_t edge0_id, edge1_id;
+    double weight, eps_x;
+    int32_t dtype; // EdgeEdgeDistanceType stored at build (edge_edge.hpp:96)
+};
+struct FaceVertexNormalCollision { int32_t face_id, vertex_id; double weight; };
+
+// collisions/normal/normal_collisions.hpp:45-63
+class NormalCollisions {
+public:
+    std::vector<VertexVertexNormalCollision> vv_collisions;
+    std::vector<EdgeVertexNormalCollision> ev_collisions;
+    std::vector<EdgeEdgeNormalCollision> ee_collisions;
+    std::vector<FaceVertexNormalCollision> fv_collisions;
+
+    // normal_collisions.cpp:20-36
+    void build(const CollisionMesh& mesh, const MatrixXd& vertices, double dhat, double dmin = 0,
+               BroadPhaseMethod bp = BroadPhaseMethod::LBVH)
+    {
+        mesh.set_positions(0, vertices);
+        mesh.touch_broad_phase();
+        mesh.touch_collisions();
+        detail::check(mesh.context(), ipcg_collisions_build(mesh.context(), dhat, dmin, int(bp)));
+        download(mesh);
+        built_for = &mesh;
+        m_generation = mesh.collisions_generation();
+        m_dmin = dmin;
+    }
+    size_t size() const
+    {
+        return vv_collisions.size() + ev_collisions.size() + ee_collisions.size()
+            + fv_collisions.size();
+    }
+    bool empty() const { return size() == 0; }
+    double dmin() const { return m_dmin; }
+    const CollisionMesh* mesh() const { return built_for; }
+    // the device holds this set (no later build on the same mesh)
+    bool resident() const
+    {
+        return built_for && built_for->collisions_generation() == m_generation;
+    }
+
+private:
+    const CollisionMesh* built_for = nullptr;
+    uint64_t m_generation = 0;
+    double m_dmin = 0;
+    void download(const CollisionMesh& mesh)
+    {
+        ipcg_ctx* c = mesh.context();
+        int64_t n[4];
+        detail::check(c, ipcg_collisions_count(c, n));
+        std::vector<int32_t> vv(2 * n[0]), ev(2 * n[1]), ee(2 * n[2]), fv(2 * n[3]), dt(n[2]);
+        std::vector<double> wvv(n[0]), wev(n[1]), wee(n[2]), eps(n[2]), wfv(n[3]);
+        detail::check(c, ipcg_collisions_download(c, vv.data(), wvv.data(), ev.data(), wev.data(),
+                                                  ee.data(), wee.data(), eps.data(), dt.data(),
+                                                  fv.data(), wfv.data()));
+        vv_collisions.resize(n[0]);
+        ev_collisions.resize(n[1]);
+        ee_collisions.resize(n[2]);
+        fv_collisions.resize(n[3]);
+        for (int64_t i = 0; i < n[0]; i++) vv_collisions[i] = { vv[2 * i], vv[2 * i + 1], wvv[i] };
+        for (int64_t i = 0; i < n[1]; i++) ev_collisions[i] = { ev[2 * i], ev[2 * i + 1], wev[i] };
+        for (int64_t i = 0; i < n[2]; i++)
+            ee_collisions[i] = { ee[2 * i], ee[2 * i + 1], wee[i], eps[i], dt[i] };
+        for (int64_t i = 0; i < n[3]; i++) fv_collisions[i] = { fv[2 * i], fv[2 * i + 1], wfv[i] };
+    }
+};
+
+// CSR of a symmetric 3V x 3V matrix (int64 row pointers)
+struct SparseMatrix {
+    int64_t rows = 0, cols = 0;
+    std::vector<int64_t> outer;
+    std::vector<int32_t> inner;
+    std::vector<double> values;
+    int64_t nonZeros() const { return (int64_t)values.size(); }
+    double coeff(int64_t r, int64_t c) const
+    {
+        for (int64_t k = outer[r]; k < outer[r + 1]; k++)
+            if (inner[k] == c) return values[k];
+        return 0;
+    }
+};
+
+// potentials/barrier_potential.hpp (use_physical_barrier = false)
+class BarrierPotential {
+public:
+    BarrierPotential(double dhat, double stiffness = 1.0) : m_dhat(dhat), m_stiffness(stiffness)
+    {
+        if (!(dhat > 0)) throw std::invalid_argument("dhat must be positive");
+    }
+    double dhat() const { return m_dhat; }
+    double stiffness() const { return m_stiffness; }
+
+    // potential.cpp:36-57 (over the collision set at `vertices`)
+    double operator()(const NormalCollisions& collisions, const CollisionMesh& mesh,
+                      const MatrixXd& vertices) const
+    {
+        prepare(collisions, mesh, vertices);
+        double e = 0;
+        detail::check(mesh.context(), ipcg_barrier_energy(mesh.context(), m_dhat, m_stiffness, &e));
+        return e;
+    }
+    // potential.cpp:59-95
+    std::vector<double> gradient(const NormalCollisions& collisions, const CollisionMesh& mesh,
+                                 const MatrixXd& vertices) const
+    {
+        prepare(collisions, mesh, vertices);
+        std::vector<double> g(3 * (size_t)mesh.num_vertices());
+        detail::check(mesh.context(),
+                      ipcg_barrier_gradient(mesh.context(), m_dhat, m_stiffness, g.data()));
+        return g;
+    }
+    // potential.cpp:97-222
+    SparseMatrix hessian(const NormalCollisions& collisions, const CollisionMesh& mesh,
+                         const MatrixXd& vertices,
+                         PSDProjectionMethod project_hessian_to_psd = PSDProjectionMethod::NONE) const
+    {
+        prepare(collisions, mesh, vertices);
+        ipcg_ctx* c = mesh.context();
+        int64_t nnz = 0;
+        detail::check(c, ipcg_barrier_hessian(c, m_dhat, m_stiffness, int(project_hessian_to_psd),
+                                              &nnz));
+        SparseMatrix H;
+        H.rows = H.cols = 3 * (int64_t)mesh.num_vertices();
+        H.outer.resize(H.rows + 1);
+        H.inner.resize(nnz);
+        H.values.resize(nnz);
+        detail::check(c, ipcg_hessian_download(c, H.outer.data(), H.inner.data(), H.values.data()));
+        return H;
+    }
+
+private:
+    double m_dhat, m_stiffness;
+    // the device evaluates the collision set it holds at slot-0 positions
+    static void prepare(const NormalCollisions& collisions, const CollisionMesh& mesh,
+                        const MatrixXd& vertices)
+    {
+        if (collisions.mesh() != &mesh)
+            throw std::runtime_error("collisions were not built for this mesh");
+        if (!collisions.resident())
+            throw std::runtime_error("a later collision build replaced this set on the device");
+        mesh.set_positions(0, vertices);
+    }
+};
+
+// ipc.cpp:45-101
+inline double compute_collision_free_stepsize(const CollisionMesh& mesh,
+                                              const MatrixXd& vertices_t0,
+                                              const MatrixXd& vertices_t1,
+                                              double min_distance = 0,
+                                              BroadPhaseMethod bp = BroadPhaseMethod::LBVH,
+                                              const TightInclusionCCD& ccd = {})
+{
+    mesh.set_positions(0, vertices_t0);
+    mesh.set_positions(1, vertices_t1);
+    mesh.touch_broad_phase();
+    double step = 1;
+    detail::check(mesh.context(),
+                  ipcg_compute_collision_free_stepsize(mesh.context(), min_distance, ccd.tolerance,
+                                                       ccd.max_iterations,
+                                                       ccd.conservative_rescaling, int(bp), &step));
+    return step;
+}
+
+// ipc.cpp:20-41
+inline bool is_step_collision_free(const CollisionMesh& mesh, const MatrixXd& vertices_t0,
+                                   const MatrixXd& vertices_t1, double min_distance = 0,
+                                   BroadPhaseMethod bp = BroadPhaseMethod::LBVH,
+                                   const TightInclusionCCD& ccd = {})
+{
+    mesh.set_positions(0, vertices_t0);
+    mesh.set_positions(1, vertices_t1);
+    mesh.touch_broad_phase();
+    int free_ = 0;
+    detail::check(mesh.context(),
+                  ipcg_is_step_collision_free(mesh.context(), min_distance, ccd.tolerance,
+                                              ccd.max_iterations, ccd.conservative_rescaling,
+                                              int(bp), &free_));
+    return free_ != 0;
+}
+
+} // namespace ipc

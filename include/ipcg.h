@@ -148,6 +148,12 @@ int ipcg_compute_collision_free_stepsize(ipcg_ctx* ctx, double min_distance,
 int ipcg_candidates_stepsize(ipcg_ctx* ctx, double min_distance, double tolerance,
                              int64_t max_iterations, double conservative_rescaling,
                              double* step);
+/* Candidates::is_step_collision_free (candidates.cpp:224-250) over the
+ * current swept lists: 1 if no candidate reports a hit */
+int ipcg_candidates_is_step_collision_free(ipcg_ctx* ctx, double min_distance,
+                                           double tolerance, int64_t max_iterations,
+                                           double conservative_rescaling,
+                                           int* collision_free);
 /* is_step_collision_free (ipc.cpp:20-41): 1 if no candidate reports a hit */
 int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double tolerance,
                                 int64_t max_iterations, double conservative_rescaling,

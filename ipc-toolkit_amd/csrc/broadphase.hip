@@ -693,6 +693,11 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     const double* X1 = swept ? c.X[1].as<double>() : nullptr;
     c.cand_valid = false;
     c.n_ee = c.n_fv = c.n_vv = c.n_ev = 0;
+    struct Tag { // stage names carry the query kind while this call runs
+        Ctx& c;
+        Tag(Ctx& c, bool swept) : c(c) { c.stage_tag = swept ? ".swept" : ".static"; }
+        ~Tag() { c.stage_tag.clear(); }
+    } tag(c, swept);
     if (c.nV == 0) {
         c.cand_valid = true;
         c.cand_swept = swept;
@@ -757,8 +762,8 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         IPCG_HIP(hipMemsetAsync(c.shard_cnt.p, 0, 4 * SHARD_CNT_BYTES, s));
         IPCG_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
         const Shards S0 { cnt[0], c.scap_ee }, S1 { cnt[1], c.scap_fv };
-        c.stage_begin("bp_traverse");
         if (sap) {
+            c.stage_begin("bp_sweep");
             if (c.nE >= 2)
                 hipLaunchKernelGGL(k_sap_self, dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0, s,
                                    c.te.sbox.as<double>(), c.te.ids.as<uint32_t>(), c.nE, axis,
@@ -775,7 +780,9 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.tf.ids.as<uint32_t>(), c.nF, axis, flt,
                                    c.cand_stage[1].as<int2>(), S1);
             }
+            c.stage_end();
         } else {
+            c.stage_begin("bp_traverse_ee");
             if (c.nE >= 2)
                 hipLaunchKernelGGL(k_traverse<TR_EE>, dim3(grid_for(c.nE, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.te.box.as<float>(),
@@ -783,6 +790,8 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.te.root.as<int32_t>(), c.te.ids.as<uint32_t>(),
                                    c.te.box.as<float>(), c.nE, flt, nullptr, nullptr,
                                    c.cand_stage[0].as<int2>(), S0, status);
+            c.stage_end();
+            c.stage_begin("bp_traverse_fv");
             if (c.nF >= 1)
                 hipLaunchKernelGGL(k_traverse<TR_FV>, dim3(grid_for(c.nV, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tv.box.as<float>(),
@@ -790,8 +799,10 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.tf.root.as<int32_t>(), c.tf.ids.as<uint32_t>(),
                                    c.tf.box.as<float>(), c.nF, flt, nullptr, nullptr,
                                    c.cand_stage[1].as<int2>(), S1, status);
+            c.stage_end();
         }
         if (codim) {
+            c.stage_begin("bp_traverse_codim");
             const Shards S2 { cnt[2], c.scap_vv }, S3 { cnt[3], c.scap_ev };
             if (c.n_cv >= 2)
                 hipLaunchKernelGGL(k_traverse<TR_VV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
@@ -807,9 +818,9 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.tce.root.as<int32_t>(), c.tce.ids.as<uint32_t>(),
                                    c.tce.box.as<float>(), c.n_ce, flt, c.cv.as<int32_t>(),
                                    c.ce.as<int32_t>(), c.cand_stage[3].as<int2>(), S3, status);
+            c.stage_end();
         }
         IPCG_CHECK_LAUNCH();
-        c.stage_end();
         int* hs = static_cast<int*>(c.pinned);
         IPCG_HIP(hipMemcpyAsync(hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));

@@ -33,7 +33,7 @@ void Ctx::stage_begin(const char* name)
     hipEvent_t e;
     IPCG_HIP(hipEventCreate(&e));
     IPCG_HIP(hipEventRecord(e, stream));
-    open_stage.push_back({ e, name });
+    open_stage.push_back({ e, std::string(name) + stage_tag });
 }
 
 void Ctx::stage_end()
@@ -484,6 +484,18 @@ int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double toler
         bp_build(c, true, 0.5 * min_distance, method);
         const double r = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, true, nullptr);
         *collision_free = r == 1.0 ? 1 : 0;
+    });
+}
+
+int ipcg_candidates_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double tolerance,
+                                           int64_t max_iterations, double rescale,
+                                           int* collision_free)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!collision_free) throw Error(IPCG_ERR_ARG, "null output");
+        const double r = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, true, nullptr);
+        *collision_free = r >= 1.0 ? 1 : 0;
     });
 }
 

@@ -99,6 +99,7 @@ struct Ctx {
     void stage_begin(const char* name);
     void stage_end();
     std::vector<std::pair<hipEvent_t, std::string>> open_stage;
+    std::string stage_tag; // appended to stage names (e.g. ".static")
 
     void require_mesh() const;
     void require_pos(int slot) const;

@@ -516,13 +516,19 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     double* bval = c.hq.as<double>();
     const uint32_t* bptr = c.hp_bptr.as<uint32_t>();
     const int32_t* bcol = c.hp_bcol.as<int32_t>();
-    c.stage_begin("hessian_local");
     IPCG_HIP(hipMemsetAsync(bmask, 0, sizeof(uint32_t) * nb, s));
     IPCG_HIP(hipMemsetAsync(bval, 0, sizeof(double) * 9 * nb, s));
+    c.stage_begin("hessian_scatter_vv_ev");
     if (C.n[0]) hipLaunchKernelGGL(k_hessian_scatter<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
     if (C.n[1]) hipLaunchKernelGGL(k_hessian_scatter<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
+    c.stage_end();
+    c.stage_begin("hessian_scatter_ee");
     if (C.n[2]) hipLaunchKernelGGL(k_hessian_scatter<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
+    c.stage_end();
+    c.stage_begin("hessian_scatter_fv");
     if (C.n[3]) hipLaunchKernelGGL(k_hessian_scatter<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
+    c.stage_end();
+    c.stage_begin("hessian_mirror");
     hipLaunchKernelGGL(k_mirror_blocks, dim3(grid_for(c.nV, 256)), dim3(256), 0, s, bptr, bcol,
                        c.nV, bval, bmask);
     IPCG_CHECK_LAUNCH();

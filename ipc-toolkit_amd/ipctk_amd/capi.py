@@ -31,6 +31,7 @@ EXPORTS = [
     "ipcg_is_step_collision_free", "ipcg_candidates_toi", "ipcg_stage_times",
     "ipcg_stage_times_reset", "ipcg_mesh_codim", "ipcg_codim_candidates_count",
     "ipcg_codim_candidates_download", "ipcg_codim_candidates_toi",
+    "ipcg_candidates_is_step_collision_free",
 ]
 
 _lib = None
@@ -84,6 +85,8 @@ def load(path=LIB_PATH):
                                               C.c_double, C.c_int, C.POINTER(C.c_int)]
     L.ipcg_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double, _D, _D]
     L.ipcg_mesh_codim.argtypes = [vp, _I, _I]
+    L.ipcg_candidates_is_step_collision_free.argtypes = [vp, C.c_double, C.c_double, C.c_int64,
+                                                         C.c_double, C.POINTER(C.c_int)]
     L.ipcg_codim_candidates_count.argtypes = [vp, _L, _L]
     L.ipcg_codim_candidates_download.argtypes = [vp, _I, _I]
     L.ipcg_codim_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double,

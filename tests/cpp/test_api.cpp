@@ -1,0 +1,170 @@
+// C++ host API (include/ipc_amd.hpp) on a device: the reference's codim
+// known-answer tests (tests/src/tests/collisions/test_normal_collisions.cpp:
+// 14-206) and the squash-tet / vertex-on-plane cases, written against the
+// mirrored interface.  Prints one line per check; exit status = failures.
+#include "ipc_amd.hpp"
+
+#include <cmath>
+#include <cstdio>
+
+static int failures = 0;
+#define CHECK(cond)                                                                     \
+    do {                                                                                \
+        const bool ok_ = (cond);                                                        \
+        std::printf("%s %s:%d %s\n", ok_ ? "ok  " : "FAIL", __FILE__, __LINE__, #cond); \
+        if (!ok_) failures++;                                                           \
+    } while (0)
+
+static bool approx(double a, double b, double rel = 1.1920928955078125e-05)
+{
+    return std::fabs(a - b) <= rel * std::fmax(std::fabs(a), std::fabs(b));
+}
+
+static void codim_vertex_vertex()
+{
+    using namespace ipc;
+    constexpr double thickness = 0.4, min_distance = 2 * thickness;
+    MatrixXd V { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 1, 0 }, { 0, 1, 1 },
+                 { 1, 0, 0 }, { 1, 0, 1 }, { 1, 1, 0 }, { 1, 1, 1 } };
+    for (int i = 0; i < 8; i++)
+        for (int k = 0; k < 3; k++) V(i, k) -= 0.5;
+    CollisionMesh mesh(V);
+    CHECK(mesh.num_vertices() == 8);
+    CHECK(mesh.num_codim_vertices() == 8);
+    CHECK(mesh.num_edges() == 0 && mesh.num_faces() == 0);
+
+    MatrixXd V1 = V;
+    for (int i = 0; i < 8; i++) V1(i, 1) *= 0.5;
+    Candidates candidates;
+    candidates.build(mesh, V, V1, thickness);
+    CHECK(!candidates.empty());
+    CHECK(candidates.vv_candidates.size() == candidates.size());
+    CHECK(!candidates.is_step_collision_free(mesh, V, V1, min_distance));
+    const double expected_toi = (1 - (min_distance + 1e-4)) / 2.0 / 0.25;
+    CHECK(approx(candidates.compute_collision_free_stepsize(mesh, V, V1, min_distance),
+                 expected_toi));
+
+    const double dhat = 0.25;
+    NormalCollisions collisions;
+    collisions.build(mesh, V, dhat, min_distance);
+    CHECK(collisions.size() == 12);
+    CHECK(collisions.vv_collisions.size() == 12);
+    BarrierPotential barrier_potential(dhat, 1.0);
+    CHECK(barrier_potential(collisions, mesh, V) > 0.0);
+    const std::vector<double> grad = barrier_potential.gradient(collisions, mesh, V);
+    bool radial = true;
+    for (int i = 0; i < 8; i++) {
+        double f[3], n = 0, m = 0, d = 0;
+        for (int k = 0; k < 3; k++) {
+            f[k] = -grad[3 * i + k];
+            n += f[k] * f[k];
+            m += V(i, k) * V(i, k);
+        }
+        for (int k = 0; k < 3; k++) d += f[k] * V(i, k);
+        radial &= std::fabs(d / std::sqrt(n * m) - 1) < 1e-12;
+    }
+    CHECK(radial);
+}
+
+static void codim_edge_vertex()
+{
+    using namespace ipc;
+    constexpr double thickness = 1e-3, min_distance = 2 * thickness;
+    const MatrixXd V { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, -1 }, { -1, 0, 0 },
+                       { 0, 0, 1 }, { 0, 1, 0 }, { 0, 2, 0 }, { 0, 3, 0 } };
+    const MatrixXi E { { 0, 1 }, { 0, 2 }, { 0, 3 }, { 0, 4 } };
+    CollisionMesh mesh(V, E);
+    CHECK(mesh.num_vertices() == 8);
+    CHECK(mesh.num_codim_vertices() == 3);
+    CHECK(mesh.num_codim_edges() == 4);
+
+    MatrixXd V1 = V;
+    for (int i = 5; i < 8; i++) V1(i, 1) -= 4;
+    Candidates candidates;
+    candidates.build(mesh, V, V1, thickness);
+    CHECK(candidates.size() == 15);
+    CHECK(candidates.vv_candidates.size() == 3);
+    CHECK(candidates.ev_candidates.size() == 12);
+    CHECK(candidates.ee_candidates.size() == 0);
+    CHECK(candidates.fv_candidates.size() == 0);
+    CHECK(!candidates.is_step_collision_free(mesh, V, V1, min_distance));
+    const double expected_toi = (1 - (min_distance + 1e-4)) / 4;
+    CHECK(approx(candidates.compute_collision_free_stepsize(mesh, V, V1, min_distance),
+                 expected_toi));
+
+    NormalCollisions collisions;
+    const double dhat = 0.25;
+    collisions.build(mesh, V, dhat, /*min_distance=*/0.8);
+    CHECK(collisions.size() == 6);
+    CHECK(collisions.vv_collisions.size() == 2);
+    CHECK(collisions.ev_collisions.size() == 4);
+    CHECK(collisions.ee_collisions.size() == 0);
+    CHECK(collisions.fv_collisions.size() == 0);
+    CHECK(BarrierPotential(dhat, 1.0)(collisions, mesh, V) > 0.0);
+}
+
+static void vertex_on_plane()
+{
+    using namespace ipc;
+    const double h = 1e-2;
+    const MatrixXd V0 { { -10, -10, 0 }, { 10, -10, 0 }, { 0, 10, 0 }, { 0.1, 0.2, h } };
+    MatrixXd V1 = V0;
+    V1(3, 2) = -h;
+    const MatrixXi F { { 0, 1, 2 } };
+    const MatrixXi E { { 0, 1 }, { 0, 2 }, { 1, 2 } };
+    CollisionMesh mesh(V0, E, F);
+    const double t = compute_collision_free_stepsize(mesh, V0, V1);
+    const double t_c = 0.5 - 1e-4 / (2 * h);
+    CHECK(t <= t_c && t >= t_c - 2e-6 / (2 * h) - 1e-9);
+    MatrixXd Vt = V0;
+    for (int k = 0; k < 3; k++) Vt(3, k) = V0(3, k) + t * (V1(3, k) - V0(3, k));
+    CHECK(is_step_collision_free(mesh, V0, Vt));
+    CHECK(!is_step_collision_free(mesh, V0, V1));
+    // the same through the sweep
+    CHECK(compute_collision_free_stepsize(mesh, V0, V1, 0, BroadPhaseMethod::SWEEP_AND_PRUNE) == t);
+    // a vertex hovering at 0.5 dhat over the face: one face-vertex collision
+    NormalCollisions collisions;
+    MatrixXd Vh = V0;
+    Vh(3, 2) = 5e-4;
+    collisions.build(mesh, Vh, 1e-3);
+    CHECK(collisions.fv_collisions.size() == 1 && collisions.size() == 1);
+    BarrierPotential B(1e-3, 1e3);
+    const SparseMatrix H = B.hessian(collisions, mesh, Vh, PSDProjectionMethod::CLAMP);
+    CHECK(H.rows == 12 && H.nonZeros() > 0);
+    CHECK(H.coeff(11, 11) > 0);
+}
+
+static void errors_throw()
+{
+    using namespace ipc;
+    bool threw = false;
+    try {
+        CollisionMesh mesh(MatrixXd { { 0, 0, 0 }, { 1, 0, 0 } }, MatrixXi { { 0, 5 } });
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        CollisionMesh mesh(MatrixXd { { 0, 0, 0 }, { 1, 0, 0 } });
+        BarrierPotential(1e-3)(NormalCollisions(), mesh, MatrixXd { { 0, 0, 0 }, { 1, 0, 0 } });
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+int main()
+{
+    try {
+        codim_vertex_vertex();
+        codim_edge_vertex();
+        vertex_on_plane();
+        errors_throw();
+    } catch (const std::exception& e) {
+        std::printf("FAIL exception: %s\n", e.what());
+        failures++;
+    }
+    std::printf("%d failure(s)\n", failures);
+    return failures;
+}
