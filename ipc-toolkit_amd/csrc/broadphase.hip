@@ -334,28 +334,49 @@ __device__ __forceinline__ bool can_fv(const Filter& f, int fi, int vi)
 //         in mesh ids (candidates.cpp:65-118).
 enum { TR_EE = 0, TR_FV = 1, TR_VV = 2, TR_EV = 3 };
 
-template <int MODE>
-__device__ __forceinline__ int2 leaf_pair(const Filter& flt, const int32_t* vmap,
-                                          const int32_t* emap, int q, int j)
+// The query primitive: its output id and vertices (-1 padded).
+struct Query {
+    int id;
+    int v[3];
+};
+
+__device__ __forceinline__ bool patch_ok(const Filter& f, const int* a, int na, const int* b,
+                                         int nb)
 {
-    if constexpr (MODE == TR_EE) {
-        return can_ee(flt, q, j) ? make_int2(q, j) : make_int2(-1, -1);
-    } else if constexpr (MODE == TR_FV) {
-        return can_fv(flt, j, q) ? make_int2(j, q) : make_int2(-1, -1);
-    } else if constexpr (MODE == TR_VV) {
-        const int a = vmap[q], b = vmap[j];
-        return can_vv(flt, a, b) ? make_int2(a, b) : make_int2(-1, -1);
-    } else {
-        const int e = emap[j], v = vmap[q];
-        return can_ev(flt, e, v) ? make_int2(e, v) : make_int2(-1, -1);
+    if (!f.patch) return true;
+    for (int i = 0; i < na; i++)
+        for (int j = 0; j < nb; j++)
+            if (f.patch[a[i]] != f.patch[b[j]]) return true;
+    return false;
+}
+
+// lbvh.cpp:800-858 filters and the reported pair for a leaf record
+template <int MODE>
+__device__ __forceinline__ int2 leaf_pair(const Filter& flt, const Query& q, const int4& r)
+{
+    if constexpr (MODE == TR_EE) { // edge vs edge: no shared vertex
+        if (q.v[0] == r.y || q.v[0] == r.z || q.v[1] == r.y || q.v[1] == r.z) return make_int2(-1, -1);
+        const int b[2] = { r.y, r.z };
+        return patch_ok(flt, q.v, 2, b, 2) ? make_int2(q.id, r.x) : make_int2(-1, -1);
+    } else if constexpr (MODE == TR_FV) { // vertex vs face
+        if (q.v[0] == r.y || q.v[0] == r.z || q.v[0] == r.w) return make_int2(-1, -1);
+        const int b[3] = { r.y, r.z, r.w };
+        return patch_ok(flt, q.v, 1, b, 3) ? make_int2(r.x, q.id) : make_int2(-1, -1);
+    } else if constexpr (MODE == TR_VV) { // codim vertex vs codim vertex
+        const int b[1] = { r.y };
+        return patch_ok(flt, q.v, 1, b, 1) ? make_int2(q.id, r.x) : make_int2(-1, -1);
+    } else { // codim vertex vs codim edge
+        if (q.v[0] == r.y || q.v[0] == r.z) return make_int2(-1, -1);
+        const int b[2] = { r.y, r.z };
+        return patch_ok(flt, q.v, 1, b, 2) ? make_int2(r.x, q.id) : make_int2(-1, -1);
     }
 }
 
 template <int MODE>
 __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
-    const uint32_t* tids, const float* tbox, int nleaves, Filter flt, const int32_t* vmap,
-    const int32_t* emap, int2* out, Shards S, int* status)
+    const int4* leaf, const float* tbox, int nleaves, Filter flt, const int32_t* qmap, int2* out,
+    Shards S, int* status)
 {
     constexpr bool TRI = MODE == TR_EE || MODE == TR_VV;
     __shared__ int stack[TR_STACK][TR_BLOCK];
@@ -363,11 +384,19 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     const int qpos = blockIdx.x * TR_BLOCK + t;
     bool active = qpos < nq;
     float q[6] = { 0, 0, 0, 0, 0, 0 };
-    int qprim = 0;
+    Query qr { 0, { -1, -1, -1 } };
     if (active) {
-        qprim = (int)qids[qpos];
+        const int qprim = (int)qids[qpos];
 #pragma unroll
         for (int k = 0; k < 6; k++) q[k] = qbox[(int64_t)6 * qprim + k];
+        if (MODE == TR_EE) { // query edge: id, endpoints
+            qr.id = qprim;
+            qr.v[0] = flt.E[2 * (int64_t)qprim];
+            qr.v[1] = flt.E[2 * (int64_t)qprim + 1];
+        } else { // query vertex (codim queries map their subset index)
+            qr.id = qmap ? qmap[qprim] : qprim;
+            qr.v[0] = qr.id;
+        }
     }
     const int root = *root_p;
     const int nin = nleaves - 1; // internal nodes
@@ -379,9 +408,9 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     if (root == -1) { // single target leaf (lbvh.cpp:386-396)
         int2 hit = make_int2(-1, -1);
         if (!TRI && active) {
-            const int fp = (int)tids[0];
-            const float* b = tbox + (int64_t)6 * fp;
-            if (overlap(q, b, b + 3)) hit = leaf_pair<MODE>(flt, vmap, emap, qprim, fp);
+            const int4 r = leaf[0];
+            const float* b = tbox;
+            if (overlap(q, b, b + 3)) hit = leaf_pair<MODE>(flt, qr, r);
         }
         const unsigned long long slot = shard_append(S, sh, hit.x >= 0 ? 1u : 0u);
         if (hit.x >= 0 && (int64_t)slot < S.scap) out[sh * S.scap + (int64_t)slot] = hit;
@@ -410,8 +439,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
                 if (o1 && nd.r1 <= qpos) o1 = false;
             }
             const bool l0 = nd.c0 < 0, l1 = nd.c1 < 0;
-            if (o0 && l0) h0 = leaf_pair<MODE>(flt, vmap, emap, qprim, (int)tids[~nd.c0]);
-            if (o1 && l1) h1 = leaf_pair<MODE>(flt, vmap, emap, qprim, (int)tids[~nd.c1]);
+            if (o0 && l0) h0 = leaf_pair<MODE>(flt, qr, leaf[~nd.c0]);
+            if (o1 && l1) h1 = leaf_pair<MODE>(flt, qr, leaf[~nd.c1]);
             const bool t0 = o0 && !l0, t1 = o1 && !l1;
             if (!nd_ok) {
             } else if (!t0 && !t1) {
@@ -604,8 +633,26 @@ void sap_list(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const double*
     IPCG_CHECK_LAUNCH();
 }
 
+// Leaf records in sorted order: the traversal tests a leaf with one 16-byte
+// load (mesh id of the primitive and its vertex ids) instead of two
+// dependent gathers (sorted position -> primitive -> vertices).
+template <int ARITY>
+__global__ __launch_bounds__(BP_BLOCK) void k_leaf_records(const uint32_t* ids, const int32_t* I,
+                                                           const int32_t* out_ids, int n,
+                                                           int4* rec)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int p = (int)ids[i];
+    int v[3] = { -1, -1, -1 };
+#pragma unroll
+    for (int j = 0; j < ARITY; j++) v[j] = I ? I[(int64_t)ARITY * p + j] : p;
+    rec[i] = make_int4(out_ids ? out_ids[p] : p, v[0], v[1], v[2]);
+}
+
+// out_ids: mesh id of each primitive (null: its index)
 void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const double* X0,
-                const double* X1, double r, bool with_nodes)
+                const double* X1, double r, bool with_nodes, const int32_t* out_ids = nullptr)
 {
     hipStream_t s = c.stream;
     T.n = n;
@@ -634,6 +681,17 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
                      T.ids2.as<uint32_t>(), n, 63, c.sort_tmp.p, s);
     c.stage_end();
     if (!with_nodes) return;
+    T.leaf.reserve(sizeof(int4) * (size_t)n);
+    if (arity == 1)
+        hipLaunchKernelGGL(k_leaf_records<1>, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(), I,
+                           out_ids, n, T.leaf.as<int4>());
+    else if (arity == 2)
+        hipLaunchKernelGGL(k_leaf_records<2>, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(), I,
+                           out_ids, n, T.leaf.as<int4>());
+    else
+        hipLaunchKernelGGL(k_leaf_records<3>, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(), I,
+                           out_ids, n, T.leaf.as<int4>());
+    IPCG_CHECK_LAUNCH();
     T.nodes.reserve(sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1));
     if (n == 1) { // single leaf: no internal node (lbvh.cpp:386-396)
         const int32_t one = -1;
@@ -736,8 +794,10 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     // either method
     const bool codim = c.n_cv > 0;
     if (codim) {
-        build_tree(c, c.tcv, c.cv.as<int32_t>(), 1, c.n_cv, X0, X1, inflation, true);
-        build_tree(c, c.tce, c.cev.as<int32_t>(), 2, c.n_ce, X0, X1, inflation, true);
+        build_tree(c, c.tcv, c.cv.as<int32_t>(), 1, c.n_cv, X0, X1, inflation, true,
+                   c.cv.as<int32_t>());
+        build_tree(c, c.tce, c.cev.as<int32_t>(), 2, c.n_ce, X0, X1, inflation, true,
+                   c.ce.as<int32_t>());
     }
 
     c.counters.reserve(sizeof(unsigned long long) * 4);
@@ -787,8 +847,8 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                 hipLaunchKernelGGL(k_traverse<TR_EE>, dim3(grid_for(c.nE, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.te.box.as<float>(),
                                    c.te.ids.as<uint32_t>(), c.nE, c.te.nodes.as<Node>(),
-                                   c.te.root.as<int32_t>(), c.te.ids.as<uint32_t>(),
-                                   c.te.box.as<float>(), c.nE, flt, nullptr, nullptr,
+                                   c.te.root.as<int32_t>(), c.te.leaf.as<int4>(),
+                                   c.te.box.as<float>(), c.nE, flt, nullptr,
                                    c.cand_stage[0].as<int2>(), S0, status);
             c.stage_end();
             c.stage_begin("bp_traverse_fv");
@@ -796,8 +856,8 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                 hipLaunchKernelGGL(k_traverse<TR_FV>, dim3(grid_for(c.nV, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tv.box.as<float>(),
                                    c.tv.ids.as<uint32_t>(), c.nV, c.tf.nodes.as<Node>(),
-                                   c.tf.root.as<int32_t>(), c.tf.ids.as<uint32_t>(),
-                                   c.tf.box.as<float>(), c.nF, flt, nullptr, nullptr,
+                                   c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
+                                   c.tf.box.as<float>(), c.nF, flt, nullptr,
                                    c.cand_stage[1].as<int2>(), S1, status);
             c.stage_end();
         }
@@ -808,16 +868,16 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                 hipLaunchKernelGGL(k_traverse<TR_VV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
                                    c.tcv.ids.as<uint32_t>(), c.n_cv, c.tcv.nodes.as<Node>(),
-                                   c.tcv.root.as<int32_t>(), c.tcv.ids.as<uint32_t>(),
+                                   c.tcv.root.as<int32_t>(), c.tcv.leaf.as<int4>(),
                                    c.tcv.box.as<float>(), c.n_cv, flt, c.cv.as<int32_t>(),
-                                   c.ce.as<int32_t>(), c.cand_stage[2].as<int2>(), S2, status);
+                                   c.cand_stage[2].as<int2>(), S2, status);
             if (c.n_ce >= 1)
                 hipLaunchKernelGGL(k_traverse<TR_EV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
                                    c.tcv.ids.as<uint32_t>(), c.n_cv, c.tce.nodes.as<Node>(),
-                                   c.tce.root.as<int32_t>(), c.tce.ids.as<uint32_t>(),
+                                   c.tce.root.as<int32_t>(), c.tce.leaf.as<int4>(),
                                    c.tce.box.as<float>(), c.n_ce, flt, c.cv.as<int32_t>(),
-                                   c.ce.as<int32_t>(), c.cand_stage[3].as<int2>(), S3, status);
+                                   c.cand_stage[3].as<int2>(), S3, status);
             c.stage_end();
         }
         IPCG_CHECK_LAUNCH();

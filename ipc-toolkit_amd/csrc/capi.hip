@@ -165,11 +165,12 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status,
                        &c.hp_mark, &c.hp_bptr, &c.hp_bcol, &c.cand_stage[0], &c.cand_stage[1],
                        &c.shard_cnt, &c.shard_off, &c.np_eec, &c.ccd_stage, &c.ccd_todo,
+                       &c.tv.leaf, &c.te.leaf, &c.tf.leaf,
                        &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
                        &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3] };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
-                           &T->root, &T->sbox })
+                           &T->root, &T->sbox, &T->leaf })
             b->release();
     for (DevBuf* b : bufs) b->release();
     for (auto& p : c.timer.pending) {

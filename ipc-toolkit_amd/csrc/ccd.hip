@@ -29,7 +29,7 @@ namespace {
 
 constexpr int CB = 64;
 constexpr int LANE_CAP = 256;
-constexpr int LANES = 32768;
+constexpr int LANES = 131072; // 8 waves per CU at this kernel's register use
 constexpr int WF_BATCH = 4096;
 constexpr int64_t WF_MAX_BOXES = 1ll << 26;
 constexpr double SMALL_TOI = 1e-6; // tight_inclusion_ccd.hpp:14
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t*
 {
     const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
     bool search = false;
-    if (ci < A.n_ee + A.n_fv) {
+    if (ci < A.n_ee + A.n_fv + A.n_vv + A.n_ev) {
         TIQuery q;
         double eff = 0, adj = 0;
         const int pre = candidate_setup(A, ci, q, eff, adj);

@@ -39,6 +39,7 @@ struct Tree {
     DevBuf info;         // construction counters / ranges
     DevBuf root;         // int32 root node (-1: single leaf)
     DevBuf sbox;         // sweep and prune: FP64 boxes in sorted order
+    DevBuf leaf;         // int4 per sorted leaf: (mesh id, its vertices, -1 padded)
 };
 
 struct Ctx {

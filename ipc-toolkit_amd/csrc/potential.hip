@@ -316,21 +316,39 @@ __global__ __launch_bounds__(256) void k_mirror_blocks(const uint32_t* bptr, con
     }
 }
 
+// One wave per vertex for the two CSR passes: the lanes walk the vertex's
+// block row together, so the reads of the pattern and the writes of its
+// three matrix rows are contiguous across the wave.
+__device__ __forceinline__ unsigned wave_exclusive_sum(unsigned x, unsigned& total)
+{
+    const int l = lane_id();
+    unsigned inc = x;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const unsigned y = __shfl_up(inc, o, WAVE);
+        if (l >= o) inc += y;
+    }
+    total = __shfl(inc, WAVE - 1, WAVE);
+    return inc - x;
+}
+
 __global__ __launch_bounds__(256) void k_row_counts(const uint32_t* bptr, const uint32_t* bmask,
                                                     int nV, uint64_t* cnt)
 {
-    const int v = blockIdx.x * 256 + threadIdx.x;
+    const int64_t v = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE; // uniform per wave
     if (v >= nV) return;
-    uint64_t c[3] = { 0, 0, 0 };
-    for (uint32_t u = bptr[v]; u < bptr[v + 1]; u++) {
+    const int l = lane_id();
+    unsigned c[3] = { 0, 0, 0 };
+    for (uint32_t u = bptr[v] + l; u < bptr[v + 1]; u += WAVE) {
         const uint32_t m = bmask[u];
         c[0] += __popc(m & 7u);
         c[1] += __popc((m >> 3) & 7u);
         c[2] += __popc((m >> 6) & 7u);
     }
-    cnt[3 * (int64_t)v] = c[0];
-    cnt[3 * (int64_t)v + 1] = c[1];
-    cnt[3 * (int64_t)v + 2] = c[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int o = WAVE / 2; o > 0; o >>= 1) c[k] += __shfl_xor(c[k], o, WAVE);
+    if (l < 3) cnt[3 * v + l] = c[l == 0 ? 0 : (l == 1 ? 1 : 2)];
 }
 
 __global__ __launch_bounds__(256) void k_fill_rows(const uint32_t* bptr, const int32_t* bcol,
@@ -338,21 +356,30 @@ __global__ __launch_bounds__(256) void k_fill_rows(const uint32_t* bptr, const i
                                                    int nV, const int64_t* rowptr, int32_t* col,
                                                    double* val)
 {
-    const int v = blockIdx.x * 256 + threadIdx.x;
+    const int64_t v = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE; // uniform per wave
     if (v >= nV) return;
+    const int l = lane_id();
+    const uint32_t s = bptr[v], e = bptr[v + 1];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        int64_t pos = rowptr[3 * (int64_t)v + k];
-        for (uint32_t u = bptr[v]; u < bptr[v + 1]; u++) {
-            const uint32_t m = bmask[u];
-            const int cb = bcol[u];
+        int64_t pos = rowptr[3 * v + k];
+        for (uint32_t base = s; base < e; base += WAVE) {
+            const uint32_t u = base + l;
+            const uint32_t m = u < e ? (bmask[u] >> (3 * k)) & 7u : 0u;
+            unsigned total;
+            const unsigned off = wave_exclusive_sum(__popc(m), total);
+            if (m) {
+                const int cb = bcol[u];
+                int64_t at = pos + off;
 #pragma unroll
-            for (int l = 0; l < 3; l++)
-                if (m & (1u << (3 * k + l))) {
-                    col[pos] = 3 * cb + l;
-                    val[pos] = bval[9 * (int64_t)u + 3 * k + l];
-                    pos++;
-                }
+                for (int j = 0; j < 3; j++)
+                    if (m & (1u << j)) {
+                        col[at] = 3 * cb + j;
+                        val[at] = bval[9 * (int64_t)u + 3 * k + j];
+                        at++;
+                    }
+            }
+            pos += total;
         }
     }
 }
@@ -538,7 +565,8 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     const int64_t nrows = 3 * (int64_t)c.nV;
     c.row_cnt.reserve(sizeof(uint64_t) * (nrows + 1));
     c.rowptr.reserve(sizeof(int64_t) * (nrows + 1));
-    hipLaunchKernelGGL(k_row_counts, dim3(grid_for(c.nV, 256)), dim3(256), 0, s, bptr, bmask, c.nV,
+    const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
+    hipLaunchKernelGGL(k_row_counts, dim3(gw), dim3(256), 0, s, bptr, bmask, c.nV,
                        c.row_cnt.as<uint64_t>());
     uint64_t* rp = reinterpret_cast<uint64_t*>(c.rowptr.p);
     exclusive_scan_u64(c.row_cnt.as<uint64_t>(), rp, nrows, rp + nrows, c.blk_tmp.p, s);
@@ -549,8 +577,8 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     c.nnz = h[0];
     c.col.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(c.nnz, 1));
     c.val.reserve(sizeof(double) * (size_t)std::max<int64_t>(c.nnz, 1));
-    hipLaunchKernelGGL(k_fill_rows, dim3(grid_for(c.nV, 256)), dim3(256), 0, s, bptr, bcol, bmask,
-                       bval, c.nV, c.rowptr.as<int64_t>(), c.col.as<int32_t>(), c.val.as<double>());
+    hipLaunchKernelGGL(k_fill_rows, dim3(gw), dim3(256), 0, s, bptr, bcol, bmask, bval, c.nV,
+                       c.rowptr.as<int64_t>(), c.col.as<int32_t>(), c.val.as<double>());
     IPCG_CHECK_LAUNCH();
     c.stage_end();
     c.hess_valid = true;
