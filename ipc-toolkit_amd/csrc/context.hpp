@@ -84,6 +84,9 @@ struct Ctx {
     DevBuf blk_keys, blk_keys2, blk_vals, blk_vals2, blk_ukeys, blk_start, blk_tmp;
     DevBuf blk_mask, row_cnt, rowptr, col, val;
     DevBuf hp_mark, hp_bptr, hp_bcol; // block pattern of the collision set
+    DevBuf inc_keys, inc_keys2, inc_vals, inc_vals2, inc_ptr; // vertex -> (collision, point)
+    DevBuf hrec;                      // projected Hessian per collision (Q basis)
+    bool inc_valid = false;
     int64_t nblocks = 0;
     bool pattern_valid = false;
     int64_t nnz = 0;

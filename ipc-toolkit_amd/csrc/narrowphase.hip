@@ -307,6 +307,7 @@ void np_build(Ctx& c, double dhat, double dmin)
     c.coll_valid = true;
     c.hess_valid = false;
     c.pattern_valid = false;
+    c.inc_valid = false;
 }
 
 } // namespace ipcg

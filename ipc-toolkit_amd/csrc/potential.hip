@@ -2,17 +2,18 @@
 // reduction), gradient (FP64 atomic scatter, RowMajor), and the
 // PSD-projected Hessian assembled to CSR (potential.cpp:36-222).
 //
-// Hessian assembly in two phases.  Symbolic (once per collision set): the
-// off-diagonal vertex pairs of all stencils are radix-sorted and run-length
-// encoded; with one diagonal block per touched vertex they give a block-CSR
-// pattern (bptr, sorted bcol).  Numeric: one lane per collision projects its
-// Hessian to PSD in the translation-free basis Q (3K x 3K, K = points - 1),
-// expands the point-space 3x3 blocks and adds them with FP64 atomics into the
-// pattern slot found by binary search in its block row, OR-ing a per-entry
-// "some contribution is nonzero" mask that reproduces the reference's rule of
-// dropping exact-zero triplets before setFromTriplets
-// (local_to_global.hpp:289-301).  One lane per vertex then emits its three
-// CSR rows in column order.
+// Hessian assembly (no global atomics).  Symbolic, once per collision set:
+// the off-diagonal vertex pairs of all stencils are radix-sorted and
+// run-length encoded; with one diagonal block per touched vertex they give a
+// block-CSR pattern (bptr, sorted bcol); the (vertex, collision, point)
+// incidences are radix-sorted by vertex.  Numeric: every collision writes
+// its Hessian projected to PSD in the translation-free basis Q (3K x 3K,
+// K = points - 1) to a record; one wave per vertex then expands the records
+// of its incident collisions into its block row, summing in LDS with a
+// per-entry "some contribution is nonzero" mask that reproduces the
+// reference's rule of dropping exact-zero triplets before setFromTriplets
+// (local_to_global.hpp:289-301), and writes the row; one wave per vertex
+// finally emits its three CSR rows in column order.
 #include "context.hpp"
 #include "sort.hpp"
 #include "stencil.hpp"
@@ -202,27 +203,18 @@ __global__ void k_sort_rows(const uint32_t* bptr, int nV, int32_t* bcol)
     }
 }
 
-__device__ __forceinline__ uint32_t find_block(const uint32_t* bptr, const int32_t* bcol, int r,
-                                               int c)
-{
-    uint32_t lo = bptr[r], hi = bptr[r + 1];
-    while (lo < hi) {
-        const uint32_t m = (lo + hi) >> 1;
-        if (bcol[m] < c) lo = m + 1;
-        else hi = m;
-    }
-    return lo;
-}
-
-// ---- numeric phase: projected stencil Hessians scattered into blocks ---------
+// ---- numeric phase without global atomics ------------------------------------
+// (1) every collision writes its PSD-projected Hessian in the Q basis
+//     (packed 3K x 3K) to a record;
+// (2) the (vertex, collision, point) incidences are sorted by vertex;
+// (3) one wave per vertex owns its block row: the lanes walk the vertex's
+//     incidences, expand each record to the row's 3x3 blocks and add them
+//     into the row's accumulators in LDS, then the wave writes the row.
 template <int KIND>
-__global__ __launch_bounds__(64) void k_hessian_scatter(CollView C, const double* X, int nV,
-                                                        BarrierParams bp, const uint32_t* bptr,
-                                                        const int32_t* bcol, double* bval,
-                                                        uint32_t* bmask)
+__global__ __launch_bounds__(64) void k_hessian_records(CollView C, const double* X, int nV,
+                                                        BarrierParams bp, double* rec)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
-    constexpr int K = NP - 1;
     constexpr int LEN = HqTraits<KIND>::LEN;
     const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (i >= C.n[KIND]) return;
@@ -234,85 +226,144 @@ __global__ __launch_bounds__(64) void k_hessian_scatter(CollView C, const double
     double H[LEN];
     stencil_hessian_q<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
                             KIND == K_EE ? C.eps[i] : 0.0, bp, H);
-    // Point-space blocks of Q H Q^T (local_hessian_to_global_triplets).  The
-    // global matrix is symmetric, so only blocks (row <= col) are summed: the
-    // upper triangle of the diagonal blocks and each off-diagonal block once,
-    // stored in the row of its smaller vertex; k_mirror_blocks fills the rest.
+    double* o = rec + (int64_t)LEN * i;
 #pragma unroll
-    for (int a = 0; a < NP; a++)
-#pragma unroll
-        for (int b = a; b < NP; b++) {
-            const bool tr = v[a] > v[b]; // the stored block is the transpose
-            const int row = tr ? v[b] : v[a], col = tr ? v[a] : v[b];
-            const bool diag = row == col;
-            const uint32_t blk = find_block(bptr, bcol, row, col);
-            if (blk >= bptr[row + 1] || bcol[blk] != col) continue; // not in pattern
-            double* dst = bval + 9 * (int64_t)blk;
-            uint32_t mask = 0;
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-#pragma unroll
-                for (int s = 0; s < 3; s++) {
-                    if (diag && s < r) continue;
-                    double x = 0;
-#pragma unroll
-                    for (int k = 0; k < K; k++)
-#pragma unroll
-                        for (int l = 0; l < K; l++)
-                            x += QBasis<NP>::q(a, k) * QBasis<NP>::q(b, l)
-                                * H[sp(3 * k + r, 3 * l + s)];
-                    if (diag && a != b) { // two stencil points on one vertex
-                        double y = 0;
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-#pragma unroll
-                            for (int l = 0; l < K; l++)
-                                y += QBasis<NP>::q(a, k) * QBasis<NP>::q(b, l)
-                                    * H[sp(3 * k + s, 3 * l + r)];
-                        x += y;
-                    }
-                    if (x != 0) { // exact zeros are not triplets (local_to_global.hpp:291)
-                        const int e = tr ? 3 * s + r : 3 * r + s;
-                        atomicAdd(dst + e, x);
-                        mask |= 1u << e;
-                        if (diag) mask |= 1u << (3 * s + r);
-                    }
-                }
-            if (mask) atomicOr(bmask + blk, mask);
-        }
+    for (int e = 0; e < LEN; e++) o[e] = H[e];
 }
 
-// Lower blocks from the summed upper ones: block (v, c) with c < v is the
-// transpose of block (c, v); diagonal blocks copy their upper triangle.
-__global__ __launch_bounds__(256) void k_mirror_blocks(const uint32_t* bptr, const int32_t* bcol,
-                                                       int nV, double* bval, uint32_t* bmask)
+// (vertex, global collision g, point a): key vertex, value g << 2 | a
+template <int KIND>
+__global__ __launch_bounds__(256) void k_incidence(CollView C, int64_t goff, int64_t ioff,
+                                                   uint64_t* keys, uint32_t* vals)
 {
-    const int v = blockIdx.x * 256 + threadIdx.x;
-    if (v >= nV) return;
-    for (uint32_t u = bptr[v]; u < bptr[v + 1]; u++) {
-        const int cb = bcol[u];
-        if (cb > v) break; // columns are sorted
-        double* dst = bval + 9 * (int64_t)u;
-        if (cb == v) {
+    constexpr int NP = StencilTraits<KIND>::NP;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= C.n[KIND]) return;
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
 #pragma unroll
-            for (int r = 0; r < 3; r++)
+    for (int a = 0; a < NP; a++) {
+        keys[ioff + (int64_t)NP * i + a] = (uint64_t)v[a];
+        vals[ioff + (int64_t)NP * i + a] = (uint32_t)(((goff + i) << 2) | a);
+    }
+}
+
+// first incidence of every vertex in the sorted keys (vptr[nV] = M)
+__global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M, int nV,
+                                                 uint32_t* vptr)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i > M) return;
+    const int64_t prev = i > 0 ? (int64_t)keys[i - 1] : -1;
+    const int64_t cur = i < M ? (int64_t)keys[i] : (int64_t)nV;
+    for (int64_t v = prev + 1; v <= cur; v++) vptr[v] = (uint32_t)i;
+}
+
+struct RecView {
+    const double* rec[4];
+    int64_t goff[5]; // global collision index ranges per kind
+};
+
+constexpr int RW = 64; // block columns per accumulation window (one per lane)
+
+// Adds the contributions of collision i (point a is the row vertex) to the
+// window [w0, w0 + RW) of the row's blocks.
+template <int KIND>
+__device__ __forceinline__ void row_accumulate(const CollView& C, const RecView& R, int64_t i,
+                                               int a, uint32_t row_s, uint32_t m, uint32_t w0,
+                                               const int32_t* bcol, double (*acc)[9],
+                                               uint32_t* msk)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    constexpr int K = NP - 1;
+    constexpr int LEN = HqTraits<KIND>::LEN;
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
+    const double* h = R.rec[KIND] + (int64_t)LEN * i;
+    double H[LEN];
 #pragma unroll
-                for (int s = 0; s < r; s++) dst[3 * r + s] = dst[3 * s + r];
-            continue;
-        }
-        const uint32_t w = find_block(bptr, bcol, cb, v);
-        if (w >= bptr[cb + 1] || bcol[w] != v) continue; // symmetric pattern: not reached
-        const double* src = bval + 9 * (int64_t)w;
-        const uint32_t m = bmask[w];
-        uint32_t t = 0;
+    for (int e = 0; e < LEN; e++) H[e] = h[e];
+    // G[l][r][s] = sum_k q(a,k) H[3k+r][3l+s]
+    double G[K][3][3];
+#pragma unroll
+    for (int l = 0; l < K; l++)
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
-            for (int s = 0; s < 3; s++) {
-                dst[3 * r + s] = src[3 * s + r];
-                if (m & (1u << (3 * s + r))) t |= 1u << (3 * r + s);
+            for (int t = 0; t < 3; t++) {
+                double x = 0;
+#pragma unroll
+                for (int k = 0; k < K; k++) x += QBasis<NP>::q(a, k) * H[sp(3 * k + r, 3 * l + t)];
+                G[l][r][t] = x;
             }
-        bmask[u] = t;
+#pragma unroll
+    for (int b = 0; b < NP; b++) {
+        // block column of point b in the row (sorted row, binary search)
+        uint32_t lo = 0, hi = m;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bcol[row_s + mid] < v[b]) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo >= m || bcol[row_s + lo] != v[b] || lo < w0 || lo >= w0 + RW) continue;
+        const uint32_t slot = lo - w0;
+        uint32_t mask = 0;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                double x = 0;
+#pragma unroll
+                for (int l = 0; l < K; l++) x += QBasis<NP>::q(b, l) * G[l][r][t];
+                if (x != 0) { // exact zeros are not triplets (local_to_global.hpp:291)
+                    atomicAdd(&acc[slot][3 * r + t], x);
+                    mask |= 1u << (3 * r + t);
+                }
+            }
+        if (mask) atomicOr(&msk[slot], mask);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, const uint32_t* vptr,
+                                                      const uint32_t* inc, const uint32_t* bptr,
+                                                      const int32_t* bcol, int nV, double* bval,
+                                                      uint32_t* bmask)
+{
+    __shared__ double acc[256 / WAVE][RW][9];
+    __shared__ uint32_t msk[256 / WAVE][RW];
+    const int wv = threadIdx.x / WAVE, l = lane_id();
+    const int64_t v = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE; // uniform per wave
+    if (v >= nV) return;
+    const uint32_t row_s = bptr[v], m = bptr[v + 1] - row_s;
+    const uint32_t i0 = vptr[v], i1 = vptr[v + 1];
+    for (uint32_t w0 = 0; w0 < m; w0 += RW) {
+#pragma unroll
+        for (int e = 0; e < 9; e++) acc[wv][l][e] = 0;
+        msk[wv][l] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t j = i0 + l; j < i1; j += WAVE) {
+            const uint32_t x = inc[j];
+            const int64_t g = x >> 2;
+            const int a = (int)(x & 3u);
+            if (g < R.goff[1]) row_accumulate<K_VV>(C, R, g, a, row_s, m, w0, bcol, acc[wv], msk[wv]);
+            else if (g < R.goff[2]) row_accumulate<K_EV>(C, R, g - R.goff[1], a, row_s, m, w0, bcol, acc[wv], msk[wv]);
+            else if (g < R.goff[3]) row_accumulate<K_EE>(C, R, g - R.goff[2], a, row_s, m, w0, bcol, acc[wv], msk[wv]);
+            else row_accumulate<K_FV>(C, R, g - R.goff[3], a, row_s, m, w0, bcol, acc[wv], msk[wv]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (w0 + l < m) {
+            const int64_t u = (int64_t)row_s + w0 + l;
+#pragma unroll
+            for (int e = 0; e < 9; e++) bval[9 * u + e] = acc[wv][l][e];
+            bmask[u] = msk[wv][l];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -543,21 +594,69 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     double* bval = c.hq.as<double>();
     const uint32_t* bptr = c.hp_bptr.as<uint32_t>();
     const int32_t* bcol = c.hp_bcol.as<int32_t>();
-    IPCG_HIP(hipMemsetAsync(bmask, 0, sizeof(uint32_t) * nb, s));
-    IPCG_HIP(hipMemsetAsync(bval, 0, sizeof(double) * 9 * nb, s));
-    c.stage_begin("hessian_scatter_vv_ev");
-    if (C.n[0]) hipLaunchKernelGGL(k_hessian_scatter<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
-    if (C.n[1]) hipLaunchKernelGGL(k_hessian_scatter<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
+    // incidences of the collision set (sorted by vertex), cached with the
+    // block pattern until the next collision build
+    const int npk[4] = { 2, 3, 4, 4 };
+    const int lenk[4] = { HqTraits<K_VV>::LEN, HqTraits<K_EV>::LEN, HqTraits<K_EE>::LEN,
+                          HqTraits<K_FV>::LEN };
+    RecView R;
+    R.goff[0] = 0;
+    int64_t M = 0, nrec = 0;
+    for (int k = 0; k < 4; k++) {
+        R.goff[k + 1] = R.goff[k] + C.n[k];
+        M += (int64_t)npk[k] * C.n[k];
+        nrec += (int64_t)lenk[k] * C.n[k];
+    }
+    if (R.goff[4] >= (1ll << 30)) throw Error(IPCG_ERR_ARG, "too many collisions for one Hessian");
+    c.stage_begin("hessian_incidence");
+    if (!c.inc_valid) {
+        const int64_t mk = std::max<int64_t>(M, 1);
+        c.inc_keys.reserve(sizeof(uint64_t) * mk);
+        c.inc_keys2.reserve(sizeof(uint64_t) * mk);
+        c.inc_vals.reserve(sizeof(uint32_t) * mk);
+        c.inc_vals2.reserve(sizeof(uint32_t) * mk);
+        c.inc_ptr.reserve(sizeof(uint32_t) * ((size_t)c.nV + 1));
+        c.blk_tmp.reserve(radix_tmp_bytes(mk));
+        uint64_t* keys = c.inc_keys.as<uint64_t>();
+        uint32_t* vals = c.inc_vals.as<uint32_t>();
+        int64_t io = 0;
+        if (C.n[0]) hipLaunchKernelGGL(k_incidence<K_VV>, dim3(grid_for(C.n[0], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[0], io, keys, vals);
+        io += 2 * C.n[0];
+        if (C.n[1]) hipLaunchKernelGGL(k_incidence<K_EV>, dim3(grid_for(C.n[1], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[1], io, keys, vals);
+        io += 3 * C.n[1];
+        if (C.n[2]) hipLaunchKernelGGL(k_incidence<K_EE>, dim3(grid_for(C.n[2], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[2], io, keys, vals);
+        io += 4 * C.n[2];
+        if (C.n[3]) hipLaunchKernelGGL(k_incidence<K_FV>, dim3(grid_for(C.n[3], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[3], io, keys, vals);
+        IPCG_CHECK_LAUNCH();
+        int vb = 1;
+        while ((1ll << vb) <= c.nV) vb++;
+        radix_sort_pairs(keys, vals, c.inc_keys2.as<uint64_t>(), c.inc_vals2.as<uint32_t>(), M, vb,
+                         c.blk_tmp.p, s);
+        hipLaunchKernelGGL(k_run_ptr, dim3(grid_for(M + 1, 256, INT32_MAX)), dim3(256), 0, s, keys,
+                           M, c.nV, c.inc_ptr.as<uint32_t>());
+        IPCG_CHECK_LAUNCH();
+        c.inc_valid = true;
+    }
     c.stage_end();
-    c.stage_begin("hessian_scatter_ee");
-    if (C.n[2]) hipLaunchKernelGGL(k_hessian_scatter<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
+    // projected Hessian records
+    c.hrec.reserve(sizeof(double) * (size_t)std::max<int64_t>(nrec, 1));
+    double* rec = c.hrec.as<double>();
+    int64_t ro = 0;
+    for (int k = 0; k < 4; k++) {
+        R.rec[k] = rec + ro;
+        ro += (int64_t)lenk[k] * C.n[k];
+    }
+    c.stage_begin("hessian_records");
+    if (C.n[0]) hipLaunchKernelGGL(k_hessian_records<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[0]));
+    if (C.n[1]) hipLaunchKernelGGL(k_hessian_records<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[1]));
+    if (C.n[2]) hipLaunchKernelGGL(k_hessian_records<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[2]));
+    if (C.n[3]) hipLaunchKernelGGL(k_hessian_records<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
+    IPCG_CHECK_LAUNCH();
     c.stage_end();
-    c.stage_begin("hessian_scatter_fv");
-    if (C.n[3]) hipLaunchKernelGGL(k_hessian_scatter<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, bptr, bcol, bval, bmask);
-    c.stage_end();
-    c.stage_begin("hessian_mirror");
-    hipLaunchKernelGGL(k_mirror_blocks, dim3(grid_for(c.nV, 256)), dim3(256), 0, s, bptr, bcol,
-                       c.nV, bval, bmask);
+    c.stage_begin("hessian_rows");
+    const unsigned gwr = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
+    hipLaunchKernelGGL(k_hessian_rows, dim3(gwr), dim3(256), 0, s, C, R, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vals.as<uint32_t>(), bptr, bcol, c.nV, bval, bmask);
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 
