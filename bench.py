@@ -129,6 +129,7 @@ def main():
     stages = ctx.stage_times()
     ctx.timing(False)
 
+    ccd_work = {k: ctx.counter(k) for k in ("ccd.candidates", "ccd.searched", "ccd.wavefront")}
     counts = ctx.collision_counts()
     ctx.broad_phase(False, 0.5 * sc.dhat)
     ne_s, nf_s = [len(x) for x in ctx.candidates()]
@@ -193,7 +194,7 @@ def main():
                    "dhat": sc.dhat, "kappa": sc.kappa, "seed": args.seed,
                    "collisions": pairs, "collisions_by_type": [int(x) for x in counts],
                    "candidates_static": ne_s + nf_s, "candidates_swept": ne_d + nf_d,
-                   "hessian_nnz": int(nnz), "step_size": step,
+                   "hessian_nnz": int(nnz), "step_size": step, "ccd_work": ccd_work,
                    "stage_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
                    "bytes_algorithmic": algorithmic_bytes(sc, counts, ne_s + nf_s,
                                                           ne_d + nf_d, nnz)},

@@ -177,6 +177,10 @@ int ipcg_codim_candidates_toi(ipcg_ctx* ctx, double min_distance, double toleran
 int ipcg_stage_times(ipcg_ctx* ctx, int max, const char** names, double* ms,
                      int64_t* calls);
 int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable);
+/* work counts of the last calls, by name: "ccd.candidates" (swept
+ * candidates of the last CCD), "ccd.searched" (those whose root box
+ * includes the origin), "ccd.wavefront" (searches that outgrew a lane) */
+int ipcg_counter(ipcg_ctx* ctx, const char* name, int64_t* value);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,8 @@ namespace {
 
 constexpr int BP_BLOCK = 256;
 constexpr int TR_BLOCK = 128;
-constexpr int TR_STACK = 64; // lbvh.cpp:378 MAX_STACK_SIZE
+constexpr int TR_STACK = 64;  // lbvh.cpp:378 MAX_STACK_SIZE
+constexpr int TR_LSTACK = 16; // entries per lane kept in LDS; the rest in HBM
 
 // AABB::from_point (+ swept union), aabb.cpp:21-55
 __device__ __forceinline__ void vbox(const double* X0, const double* X1, int nV, int i, double r,
@@ -376,12 +377,15 @@ template <int MODE>
 __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
     const int4* leaf, const float* tbox, int nleaves, Filter flt, const int32_t* qmap, int2* out,
-    Shards S, int* status)
+    Shards S, int* deep, int* status)
 {
     constexpr bool TRI = MODE == TR_EE || MODE == TR_VV;
-    __shared__ int stack[TR_STACK][TR_BLOCK];
+    // The traversal stack of postponed right children: the first TR_LSTACK
+    // entries in LDS (small enough for full occupancy), deeper ones in the
+    // query's own HBM slots (deep: TR_STACK - TR_LSTACK ints per query).
+    __shared__ int stack[TR_LSTACK][TR_BLOCK];
     const int t = threadIdx.x;
-    const int qpos = blockIdx.x * TR_BLOCK + t;
+    const int qpos = (int)xcd_block() * TR_BLOCK + t;
     bool active = qpos < nq;
     float q[6] = { 0, 0, 0, 0, 0, 0 };
     Query qr { 0, { -1, -1, -1 } };
@@ -445,21 +449,29 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
             if (!nd_ok) {
             } else if (!t0 && !t1) {
                 if (sp == 0) active = false;
-                else node = stack[--sp][t];
+                else {
+                    --sp;
+                    node = sp < TR_LSTACK ? stack[sp][t]
+                                          : deep[(int64_t)qpos * (TR_STACK - TR_LSTACK) + sp - TR_LSTACK];
+                }
             } else {
                 node = t0 ? nd.c0 : nd.c1;
                 if (t0 && t1) {
-                    if (sp < TR_STACK) stack[sp++][t] = nd.c1;
-                    else {
+                    if (sp < TR_LSTACK) stack[sp++][t] = nd.c1;
+                    else if (sp < TR_STACK) {
+                        deep[(int64_t)qpos * (TR_STACK - TR_LSTACK) + sp - TR_LSTACK] = nd.c1;
+                        sp++;
+                    } else {
                         atomicOr(status, 1);
                         active = false;
                     }
                 }
             }
         }
-        const unsigned cnt = (h0.x >= 0 ? 1u : 0u) + (h1.x >= 0 ? 1u : 0u);
-        int64_t slot = (int64_t)shard_append(S, sh, cnt);
+        unsigned pre;
+        int64_t slot = (int64_t)shard_append_bits(S, sh, h0.x >= 0, h1.x >= 0, pre);
         int2* reg = out + sh * S.scap;
+        slot += pre;
         if (h0.x >= 0) {
             if (slot < S.scap) reg[slot] = h0;
             slot++;
@@ -813,6 +825,10 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     DevBuf* final_list[4] = { &c.cand_ee, &c.cand_fv, &c.cand_vv, &c.cand_ev };
     int64_t* final_n[4] = { &c.n_ee, &c.n_fv, &c.n_vv, &c.n_ev };
     const int lists = codim ? 4 : 2;
+    // deep traversal stack entries, per query of the largest query set
+    const int64_t nq_max = std::max<int64_t>(std::max<int64_t>(c.nE, c.nV), 1);
+    c.tr_deep.reserve(sizeof(int) * (size_t)nq_max * (TR_STACK - TR_LSTACK));
+    int* deep = c.tr_deep.as<int>();
     for (int attempt = 0; attempt < 3; attempt++) {
         // per-shard regions; their sum bounds a list until a retry grows it
         for (int k = 0; k < lists; k++) {
@@ -849,7 +865,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.te.ids.as<uint32_t>(), c.nE, c.te.nodes.as<Node>(),
                                    c.te.root.as<int32_t>(), c.te.leaf.as<int4>(),
                                    c.te.box.as<float>(), c.nE, flt, nullptr,
-                                   c.cand_stage[0].as<int2>(), S0, status);
+                                   c.cand_stage[0].as<int2>(), S0, deep, status);
             c.stage_end();
             c.stage_begin("bp_traverse_fv");
             if (c.nF >= 1)
@@ -858,7 +874,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.tv.ids.as<uint32_t>(), c.nV, c.tf.nodes.as<Node>(),
                                    c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
                                    c.tf.box.as<float>(), c.nF, flt, nullptr,
-                                   c.cand_stage[1].as<int2>(), S1, status);
+                                   c.cand_stage[1].as<int2>(), S1, deep, status);
             c.stage_end();
         }
         if (codim) {
@@ -870,14 +886,14 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
                                    c.tcv.ids.as<uint32_t>(), c.n_cv, c.tcv.nodes.as<Node>(),
                                    c.tcv.root.as<int32_t>(), c.tcv.leaf.as<int4>(),
                                    c.tcv.box.as<float>(), c.n_cv, flt, c.cv.as<int32_t>(),
-                                   c.cand_stage[2].as<int2>(), S2, status);
+                                   c.cand_stage[2].as<int2>(), S2, deep, status);
             if (c.n_ce >= 1)
                 hipLaunchKernelGGL(k_traverse<TR_EV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
                                    c.tcv.ids.as<uint32_t>(), c.n_cv, c.tce.nodes.as<Node>(),
                                    c.tce.root.as<int32_t>(), c.tce.leaf.as<int4>(),
                                    c.tce.box.as<float>(), c.n_ce, flt, c.cv.as<int32_t>(),
-                                   c.cand_stage[3].as<int2>(), S3, status);
+                                   c.cand_stage[3].as<int2>(), S3, deep, status);
             c.stage_end();
         }
         IPCG_CHECK_LAUNCH();

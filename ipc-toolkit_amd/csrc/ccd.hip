@@ -150,7 +150,7 @@ __device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double&
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t* stage)
 {
-    const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t ci = (int64_t)xcd_block() * 256 + threadIdx.x;
     bool search = false;
     if (ci < A.n_ee + A.n_fv + A.n_vv + A.n_ev) {
         TIQuery q;
@@ -705,6 +705,8 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
                   c.shard_off.as<int64_t>(), s);
     c.stage_end();
     const int64_t n_todo = rc.total;
+    c.stat["ccd.candidates"] = n;
+    c.stat["ccd.searched"] = n_todo;
 
     c.stage_begin("ccd_lanes");
     if (n_todo > 0) {
@@ -718,6 +720,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     IPCG_HIP(hipMemcpyAsync(h, n_heavy, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     IPCG_HIP(hipStreamSynchronize(s));
     const int64_t nh = (int64_t)h[0];
+    c.stat["ccd.wavefront"] = nh;
 
     if (nh > 0) {
         c.stage_begin("ccd_wavefront");

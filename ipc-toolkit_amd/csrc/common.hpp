@@ -67,6 +67,19 @@ struct DevBuf {
     }
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8) and every XCD has its own L2.  xcd_block() maps the workgroup to a
+// logical index so that each XCD runs one contiguous range of logical
+// workgroups: spatially ordered work (Morton-sorted queries, sorted
+// collisions) then shares one L2.  A bijection of [0, gridDim.x).
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned q = nb / NUM_XCD, r = nb % NUM_XCD, x = b % NUM_XCD;
+    return x * q + (x < r ? x : r) + b / NUM_XCD;
+}
+
 // ---- wave64 helpers -------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -95,6 +108,28 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* co
     if (l == WAVE - 1 && total) base = atomicAdd(counter, (unsigned long long)total);
     base = __shfl(base, WAVE - 1, WAVE);
     return base + (x - cnt);
+}
+
+// Appends one item for each lane with `a` set and one for each lane with
+// `b` set (all lanes of the wave must call it together): ballots give the
+// lane's offset (items of lower lanes, a-items then b-items per lane) and
+// one atomic per wave reserves the slots.  Returns the wave's base slot;
+// `pre` receives the lane's offset from it.
+__device__ __forceinline__ unsigned long long wave_append_bits(unsigned long long* counter, bool a,
+                                                               bool b, unsigned& pre)
+{
+    const unsigned long long ma = __ballot(a), mb = __ballot(b);
+    const unsigned total = (unsigned)(__popcll(ma) + __popcll(mb));
+    pre = __builtin_amdgcn_mbcnt_hi((unsigned)(ma >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ma, 0u))
+        + __builtin_amdgcn_mbcnt_hi((unsigned)(mb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mb, 0u));
+    unsigned long long base = 0;
+    if (total) {
+        if (lane_id() == 0) base = atomicAdd(counter, (unsigned long long)total);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)base);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+        base = ((unsigned long long)hi << 32) | lo;
+    }
+    return base;
 }
 
 } // namespace ipcg

@@ -70,6 +70,7 @@ struct Ctx {
     DevBuf cand_vv, cand_ev;           // codim lists: (v, v) and (edge, vertex)
     int64_t scap_ee = 0, scap_fv = 0, scap_vv = 0, scap_ev = 0; // shard regions
     DevBuf cand_stage[4];              // sharded traversal output (ee, fv, vv, ev)
+    DevBuf tr_deep;                    // traversal stack entries beyond LDS
     DevBuf shard_cnt, shard_off;       // shard counters / offsets of the append lists
 
     // collision set
@@ -104,6 +105,7 @@ struct Ctx {
     void stage_end();
     std::vector<std::pair<hipEvent_t, std::string>> open_stage;
     std::string stage_tag; // appended to stage names (e.g. ".static")
+    std::map<std::string, int64_t> stat; // work counts of the last calls
 
     void require_mesh() const;
     void require_pos(int slot) const;

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
     const int32_t* F2E, const int2* ee, int64_t n_ee, const int2* fv, int64_t n_fv,
     const int2* vv, int64_t n_vv, const int2* ev, int64_t n_ev, double active2, Streams S)
 {
-    const int64_t i = (int64_t)blockIdx.x * NP_BLOCK + threadIdx.x;
+    const int64_t i = (int64_t)xcd_block() * NP_BLOCK + threadIdx.x;
     int kind = -1; // 0 vv, 1 ev, 2 ee, 3 fv
     uint64_t key = 0;
     EERec er { 0, 0, 0, 0, 0.0 };

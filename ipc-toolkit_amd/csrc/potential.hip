@@ -102,7 +102,7 @@ __global__ __launch_bounds__(PB) void k_gradient(CollView C, const double* X, in
                                                  BarrierParams bp, double* grad)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
-    const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
     if (i >= C.n[KIND]) return;
     int v[4];
     stencil_ids<KIND>(C, i, v);
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(64) void k_hessian_records(CollView C, const double
 {
     constexpr int NP = StencilTraits<KIND>::NP;
     constexpr int LEN = HqTraits<KIND>::LEN;
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t i = (int64_t)xcd_block() * 64 + threadIdx.x;
     if (i >= C.n[KIND]) return;
     int v[4];
     stencil_ids<KIND>(C, i, v);
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, con
     __shared__ double acc[256 / WAVE][RW][9];
     __shared__ uint32_t msk[256 / WAVE][RW];
     const int wv = threadIdx.x / WAVE, l = lane_id();
-    const int64_t v = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE; // uniform per wave
+    const int64_t v = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
     if (v >= nV) return;
     const uint32_t row_s = bptr[v], m = bptr[v + 1] - row_s;
     const uint32_t i0 = vptr[v], i1 = vptr[v + 1];

@@ -36,6 +36,13 @@ __device__ __forceinline__ unsigned long long shard_append(const Shards& S, int 
     return wave_append(S.cnt + (int64_t)sh * SHARD_PAD, cnt);
 }
 
+// Ballot form for lanes appending at most two items (a, then b).
+__device__ __forceinline__ unsigned long long shard_append_bits(const Shards& S, int sh, bool a,
+                                                                bool b, unsigned& pre)
+{
+    return wave_append_bits(S.cnt + (int64_t)sh * SHARD_PAD, a, b, pre);
+}
+
 struct ShardCounts {
     int64_t total = 0, max = 0;
 };

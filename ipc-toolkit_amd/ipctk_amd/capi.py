@@ -31,7 +31,7 @@ EXPORTS = [
     "ipcg_is_step_collision_free", "ipcg_candidates_toi", "ipcg_stage_times",
     "ipcg_stage_times_reset", "ipcg_mesh_codim", "ipcg_codim_candidates_count",
     "ipcg_codim_candidates_download", "ipcg_codim_candidates_toi",
-    "ipcg_candidates_is_step_collision_free",
+    "ipcg_candidates_is_step_collision_free", "ipcg_counter",
 ]
 
 _lib = None
@@ -85,6 +85,7 @@ def load(path=LIB_PATH):
                                               C.c_double, C.c_int, C.POINTER(C.c_int)]
     L.ipcg_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double, _D, _D]
     L.ipcg_mesh_codim.argtypes = [vp, _I, _I]
+    L.ipcg_counter.argtypes = [vp, C.c_char_p, _L]
     L.ipcg_candidates_is_step_collision_free.argtypes = [vp, C.c_double, C.c_double, C.c_int64,
                                                          C.c_double, C.POINTER(C.c_int)]
     L.ipcg_codim_candidates_count.argtypes = [vp, _L, _L]
@@ -308,6 +309,11 @@ class Context:
         if n < 0:
             raise IPCGError(-n, self._L.ipcg_last_error(self._p).decode())
         return {names[i].decode(): (ms[i], int(calls[i])) for i in range(n)}
+
+    def counter(self, name):
+        v = C.c_int64()
+        self._check(self._L.ipcg_counter(self._p, name.encode(), C.byref(v)))
+        return v.value
 
     def synchronize(self):
         self._check(self._L.ipcg_ctx_synchronize(self._p))
