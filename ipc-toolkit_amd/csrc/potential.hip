@@ -267,11 +267,12 @@ struct RecView {
 constexpr int RW = 64; // block columns per accumulation window (one per lane)
 
 // Adds the contributions of collision i (point a is the row vertex) to the
-// window [w0, w0 + RW) of the row's blocks.
+// window of the row's blocks whose columns are cols[0, RW) (sorted, padded
+// with INT32_MAX).  The record stays in memory (read through the cache)
+// so the kernel keeps few registers.
 template <int KIND>
 __device__ __forceinline__ void row_accumulate(const CollView& C, const RecView& R, int64_t i,
-                                               int a, uint32_t row_s, uint32_t m, uint32_t w0,
-                                               const int32_t* bcol, double (*acc)[9],
+                                               int a, const int32_t* cols, double (*acc)[9],
                                                uint32_t* msk)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
@@ -280,9 +281,6 @@ __device__ __forceinline__ void row_accumulate(const CollView& C, const RecView&
     int v[4];
     stencil_ids<KIND>(C, i, v);
     const double* h = R.rec[KIND] + (int64_t)LEN * i;
-    double H[LEN];
-#pragma unroll
-    for (int e = 0; e < LEN; e++) H[e] = h[e];
     // G[l][r][s] = sum_k q(a,k) H[3k+r][3l+s]
     double G[K][3][3];
 #pragma unroll
@@ -293,20 +291,20 @@ __device__ __forceinline__ void row_accumulate(const CollView& C, const RecView&
             for (int t = 0; t < 3; t++) {
                 double x = 0;
 #pragma unroll
-                for (int k = 0; k < K; k++) x += QBasis<NP>::q(a, k) * H[sp(3 * k + r, 3 * l + t)];
+                for (int k = 0; k < K; k++) x += QBasis<NP>::q(a, k) * h[sp(3 * k + r, 3 * l + t)];
                 G[l][r][t] = x;
             }
 #pragma unroll
     for (int b = 0; b < NP; b++) {
-        // block column of point b in the row (sorted row, binary search)
-        uint32_t lo = 0, hi = m;
+        const int col = v[b];
+        if (col < cols[0] || col > cols[RW - 1]) continue; // outside this window
+        int lo = 0, hi = RW;
         while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (bcol[row_s + mid] < v[b]) lo = mid + 1;
+            const int mid = (lo + hi) >> 1;
+            if (cols[mid] < col) lo = mid + 1;
             else hi = mid;
         }
-        if (lo >= m || bcol[row_s + lo] != v[b] || lo < w0 || lo >= w0 + RW) continue;
-        const uint32_t slot = lo - w0;
+        if (lo >= RW || cols[lo] != col) continue;
         uint32_t mask = 0;
 #pragma unroll
         for (int r = 0; r < 3; r++)
@@ -316,11 +314,11 @@ __device__ __forceinline__ void row_accumulate(const CollView& C, const RecView&
 #pragma unroll
                 for (int l = 0; l < K; l++) x += QBasis<NP>::q(b, l) * G[l][r][t];
                 if (x != 0) { // exact zeros are not triplets (local_to_global.hpp:291)
-                    atomicAdd(&acc[slot][3 * r + t], x);
+                    atomicAdd(&acc[lo][3 * r + t], x);
                     mask |= 1u << (3 * r + t);
                 }
             }
-        if (mask) atomicOr(&msk[slot], mask);
+        if (mask) atomicOr(&msk[lo], mask);
     }
 }
 
@@ -331,6 +329,7 @@ __global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, con
 {
     __shared__ double acc[256 / WAVE][RW][9];
     __shared__ uint32_t msk[256 / WAVE][RW];
+    __shared__ int32_t cols[256 / WAVE][RW];
     const int wv = threadIdx.x / WAVE, l = lane_id();
     const int64_t v = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
     if (v >= nV) return;
@@ -340,6 +339,7 @@ __global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, con
 #pragma unroll
         for (int e = 0; e < 9; e++) acc[wv][l][e] = 0;
         msk[wv][l] = 0;
+        cols[wv][l] = w0 + l < m ? bcol[row_s + w0 + l] : INT32_MAX;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -347,10 +347,10 @@ __global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, con
             const uint32_t x = inc[j];
             const int64_t g = x >> 2;
             const int a = (int)(x & 3u);
-            if (g < R.goff[1]) row_accumulate<K_VV>(C, R, g, a, row_s, m, w0, bcol, acc[wv], msk[wv]);
-            else if (g < R.goff[2]) row_accumulate<K_EV>(C, R, g - R.goff[1], a, row_s, m, w0, bcol, acc[wv], msk[wv]);
-            else if (g < R.goff[3]) row_accumulate<K_EE>(C, R, g - R.goff[2], a, row_s, m, w0, bcol, acc[wv], msk[wv]);
-            else row_accumulate<K_FV>(C, R, g - R.goff[3], a, row_s, m, w0, bcol, acc[wv], msk[wv]);
+            if (g < R.goff[1]) row_accumulate<K_VV>(C, R, g, a, cols[wv], acc[wv], msk[wv]);
+            else if (g < R.goff[2]) row_accumulate<K_EV>(C, R, g - R.goff[1], a, cols[wv], acc[wv], msk[wv]);
+            else if (g < R.goff[3]) row_accumulate<K_EE>(C, R, g - R.goff[2], a, cols[wv], acc[wv], msk[wv]);
+            else row_accumulate<K_FV>(C, R, g - R.goff[3], a, cols[wv], acc[wv], msk[wv]);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
