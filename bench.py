@@ -129,7 +129,8 @@ def main():
     stages = ctx.stage_times()
     ctx.timing(False)
 
-    ccd_work = {k: ctx.counter(k) for k in ("ccd.candidates", "ccd.searched", "ccd.wavefront")}
+    ccd_work = {k: ctx.counter(k) for k in ("ccd.candidates", "ccd.searched", "ccd.spilled",
+                                              "ccd.wavefront")}
     counts = ctx.collision_counts()
     ctx.broad_phase(False, 0.5 * sc.dhat)
     ne_s, nf_s = [len(x) for x in ctx.candidates()]

@@ -179,7 +179,8 @@ int ipcg_stage_times(ipcg_ctx* ctx, int max, const char** names, double* ms,
 int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable);
 /* work counts of the last calls, by name: "ccd.candidates" (swept
  * candidates of the last CCD), "ccd.searched" (those whose root box
- * includes the origin), "ccd.wavefront" (searches that outgrew a lane) */
+ * includes the origin), "ccd.spilled" (searches that outgrew the LDS
+ * lists), "ccd.wavefront" (searches that outgrew a lane's HBM slab) */
 int ipcg_counter(ipcg_ctx* ctx, const char* name, int64_t* value);
 
 #ifdef __cplusplus

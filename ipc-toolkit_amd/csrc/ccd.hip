@@ -3,10 +3,13 @@
 // (tight_inclusion_ccd.cpp:33-336) and the step-size minimum
 // (candidates.cpp:252-292).
 //
-// Two paths share one device restatement of the algorithm (ti.hpp):
-//  * lane path: persistent lanes pull candidates from an atomic counter and
-//    run the whole breadth-first search in a private 3 x CAP box slab.  Most
-//    candidates are rejected at the root box or resolve in a few levels.
+// Three tiers share one device restatement of the algorithm (ti.hpp):
+//  * LDS tier: one lane per candidate that survived the root pass, its level
+//    lists in LDS (LDS_CAP packed boxes per list).  Almost every candidate
+//    resolves here within a few levels.
+//  * slab tier, for searches that outgrew the LDS lists: persistent lanes
+//    pull candidates from an atomic counter and run the whole breadth-first
+//    search in a private 3 x LANE_CAP box slab in HBM.
 //  * wavefront path, for candidates whose levels outgrow the slab: all of
 //    them advance one level per round of short kernels.  Every box of the
 //    level is evaluated by its own lane (the inclusion test is the work),
@@ -28,6 +31,7 @@ namespace ipcg {
 namespace {
 
 constexpr int CB = 64;
+constexpr int LDS_CAP = 8;
 constexpr int LANE_CAP = 256;
 constexpr int LANES = 131072; // 8 waves per CU at this kernel's register use
 constexpr int WF_BATCH = 4096;
@@ -172,7 +176,41 @@ __global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t*
 }
 
 // ---------------------------------------------------------------------------
-// lane path
+// LDS tier
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CB) void k_ccd_lds(CCDArgs A, const uint32_t* todo, int64_t n_todo,
+                                                uint32_t* spill, unsigned long long* n_spill)
+{
+    __shared__ unsigned long long lists[3 * LDS_CAP * CB];
+    const int64_t k = (int64_t)xcd_block() * CB + threadIdx.x;
+    if (k >= n_todo) return;
+    if (A.any_hit && __hip_atomic_load(A.any_hit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return;
+    const TILdsLists L { lists + threadIdx.x, CB, LDS_CAP };
+    const int64_t ci = todo[k];
+    TIQuery q;
+    double eff = 0, adj = 0, toi = INFINITY;
+    const int pre = candidate_setup(A, ci, q, eff, adj);
+    if (pre >= 0) {
+        report(A, ci, pre == 1, 0.0);
+        return;
+    }
+    TIBound B1 { A.bounded ? A.bound : nullptr, 1.0, SMALL_TOI };
+    int r = ti_query_lds(q, eff, adj, A.max_iter, false, B1, L, toi);
+    if (r == TI_HIT && toi < SMALL_TOI) {
+        TIBound B2 { A.bounded ? A.bound : nullptr, (1.0 / A.rescale) * (1 + 1e-12), 0.0 };
+        r = ti_query_lds(q, A.min_distance, adj, -1, true, B2, L, toi);
+        if (r == TI_HIT) toi *= A.rescale;
+    }
+    if (r == TI_OVERFLOW) {
+        spill[atomicAdd(n_spill, 1ull)] = (uint32_t)ci;
+        return;
+    }
+    report(A, ci, r == TI_HIT, toi);
+}
+
+// ---------------------------------------------------------------------------
+// slab tier
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, const uint32_t* todo, int64_t n_todo,
                                                   TIBox* slab, unsigned long long* work,
@@ -654,6 +692,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     int* error = reinterpret_cast<int*>(st + 256);
     unsigned long long* nroot = reinterpret_cast<unsigned long long*>(st + 320);
     unsigned long long* npend = reinterpret_cast<unsigned long long*>(st + 384); // [2]
+    unsigned long long* n_spill = reinterpret_cast<unsigned long long*>(st + 448);
     const double one = 1.0;
     IPCG_HIP(hipMemsetAsync(st, 0, 512, s));
     IPCG_HIP(hipMemcpyAsync(bound, &one, sizeof(double), hipMemcpyHostToDevice, s));
@@ -708,49 +747,66 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     c.stat["ccd.candidates"] = n;
     c.stat["ccd.searched"] = n_todo;
 
-    c.stage_begin("ccd_lanes");
-    if (n_todo > 0) {
-        const int64_t ln = std::min<int64_t>(LANES, ((n_todo + CB - 1) / CB) * CB);
-        hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A,
-                           c.ccd_todo.as<uint32_t>(), n_todo, c.ccd_slab.as<TIBox>(), work, heavy,
-                           n_heavy);
-        IPCG_CHECK_LAUNCH();
-    }
-    c.stage_end();
-    IPCG_HIP(hipMemcpyAsync(h, n_heavy, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    IPCG_HIP(hipStreamSynchronize(s));
-    const int64_t nh = (int64_t)h[0];
-    c.stat["ccd.wavefront"] = nh;
-
-    if (nh > 0) {
-        c.stage_begin("ccd_wavefront");
-        DevBuf& wst = c.ccd_bound; // per-candidate wavefront state
-        wst.reserve(sizeof(WFState) * (size_t)WF_BATCH);
-        WFState* states = wst.as<WFState>();
-        // Batches of heavy candidates advance together.  A batch whose level
-        // outgrows the box budget is abandoned (results are only reported at
-        // the end of a batch, so this has no side effect) and retried with a
-        // quarter of the candidates.
-        int64_t batch = WF_BATCH;
-        for (int64_t first = 0; first < nh;) {
-            const int64_t nb = std::min<int64_t>(batch, nh - first);
-            const bool ok = wf_batch(c, A, heavy, first, nb, states, early_exit, any_hit, error,
-                                     nroot, npend, h);
-            if (!ok) {
-                if (nb == 1)
-                    throw Error(IPCG_ERR_NUMERIC, "CCD wavefront level exceeds its box budget");
-                batch = std::max<int64_t>(1, nb / 4);
-                continue;
-            }
-            first += nb;
-            batch = std::min<int64_t>(WF_BATCH, 2 * batch);
+    // One search of every surviving candidate: the LDS tier, the slab tier
+    // for its overflow, then the wavefront for searches that outgrew a slab.
+    // The root pass's staging array is free again and holds the spill list.
+    auto search = [&]() {
+        uint32_t* spill = c.ccd_stage.as<uint32_t>();
+        c.stage_begin("ccd_lds");
+        if (n_todo > 0) {
+            hipLaunchKernelGGL(k_ccd_lds, dim3((unsigned)((n_todo + CB - 1) / CB)), dim3(CB), 0, s,
+                               A, c.ccd_todo.as<uint32_t>(), n_todo, spill, n_spill);
+            IPCG_CHECK_LAUNCH();
         }
         c.stage_end();
-        IPCG_HIP(hipMemcpyAsync(h, error, sizeof(int), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipMemcpyAsync(h, n_spill, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));
-        if (*reinterpret_cast<int*>(h))
-            throw Error(IPCG_ERR_NUMERIC, "CCD root search exceeded the level limit");
-    }
+        const int64_t n2 = (int64_t)h[0];
+        c.stat["ccd.spilled"] = n2;
+        c.stage_begin("ccd_lanes");
+        if (n2 > 0) {
+            const int64_t ln = std::min<int64_t>(LANES, ((n2 + CB - 1) / CB) * CB);
+            hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A, spill,
+                               n2, c.ccd_slab.as<TIBox>(), work, heavy, n_heavy);
+            IPCG_CHECK_LAUNCH();
+        }
+        c.stage_end();
+        IPCG_HIP(hipMemcpyAsync(h, n_heavy, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        const int64_t nh = (int64_t)h[0];
+        c.stat["ccd.wavefront"] += nh;
+        if (nh > 0) {
+            c.stage_begin("ccd_wavefront");
+            DevBuf& wst = c.ccd_bound; // per-candidate wavefront state
+            wst.reserve(sizeof(WFState) * (size_t)WF_BATCH);
+            WFState* states = wst.as<WFState>();
+            // Batches of heavy candidates advance together.  A batch whose
+            // level outgrows the box budget is abandoned (results are only
+            // reported at the end of a batch, so this has no side effect)
+            // and retried with a quarter of the candidates.
+            int64_t batch = WF_BATCH;
+            for (int64_t first = 0; first < nh;) {
+                const int64_t nb = std::min<int64_t>(batch, nh - first);
+                const bool ok = wf_batch(c, A, heavy, first, nb, states, early_exit, any_hit,
+                                         error, nroot, npend, h);
+                if (!ok) {
+                    if (nb == 1)
+                        throw Error(IPCG_ERR_NUMERIC, "CCD wavefront level exceeds its box budget");
+                    batch = std::max<int64_t>(1, nb / 4);
+                    continue;
+                }
+                first += nb;
+                batch = std::min<int64_t>(WF_BATCH, 2 * batch);
+            }
+            c.stage_end();
+            IPCG_HIP(hipMemcpyAsync(h, error, sizeof(int), hipMemcpyDeviceToHost, s));
+            IPCG_HIP(hipStreamSynchronize(s));
+            if (*reinterpret_cast<int*>(h))
+                throw Error(IPCG_ERR_NUMERIC, "CCD root search exceeded the level limit");
+        }
+    };
+    c.stat["ccd.wavefront"] = 0;
+    search();
     if (per_cand) { // toi_out: ee, fv, vv, ev (any null)
         const int64_t cnt[4] = { c.n_ee, c.n_fv, c.n_vv, c.n_ev };
         int64_t off = 0;

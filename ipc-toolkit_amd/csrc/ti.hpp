@@ -296,17 +296,172 @@ __device__ inline int ti_bfs(const TIQuery& q, double co_tol, double ms, long ma
     return TI_MISS;
 }
 
+// ---------------------------------------------------------------------------
+// Level lists in LDS.  Every box of a level is its lower corner counted in
+// the level widths 2^-kt, 2^-ku, 2^-kv, so TI_LDS_BITS bits per axis hold
+// it exactly while no axis has been halved more often than that; a search
+// that needs more halvings, or more than `cap` boxes in a list, returns
+// TI_OVERFLOW (the caller runs it again on the HBM slab).  The decoded
+// corners are the same doubles the slab search visits, so both searches
+// give the same outcome.
+// ---------------------------------------------------------------------------
+constexpr int TI_LDS_BITS = 21;
+constexpr unsigned TI_LDS_MASK = (1u << TI_LDS_BITS) - 1;
+
+struct TILdsLists {
+    unsigned long long* base; // this lane's slot 0 of list 0
+    int stride;               // elements between a lane's consecutive slots
+    int cap;                  // boxes per list
+    __device__ unsigned long long& at(int list, int j) const
+    {
+        return base[(list * cap + j) * stride];
+    }
+};
+
+__device__ __forceinline__ unsigned long long ti_pack(unsigned it, unsigned iu, unsigned iv)
+{
+    return ((unsigned long long)it << (2 * TI_LDS_BITS)) |
+           ((unsigned long long)iu << TI_LDS_BITS) | iv;
+}
+
+__device__ inline int ti_bfs_lds(const TIQuery& q, double co_tol, double ms, long max_itr,
+                                 const TIBound& B, const TILdsLists& L, double& toi,
+                                 double& out_tol)
+{
+    out_tol = co_tol;
+    double temp_out_tol = co_tol, temp_toi = INFINITY;
+    double toi_skip = 4;
+    bool use_skip = false;
+    long refine = 0;
+    int kt = 0, ku = 0, kv = 0; // the level's widths are 2^-k
+    int ncur = 1;
+    L.at(0, 0) = 0;
+    toi = INFINITY;
+    // The running minimum only decreases; a value read once per search
+    // hides a subset of what a fresher read hides, all at or above the final
+    // minimum.
+    const double bound = B.get();
+    while (ncur > 0) {
+        const double wt = ldexp(1.0, -kt), wu = ldexp(1.0, -ku), wv = ldexp(1.0, -kv);
+        bool level_less_tol = true, level_root = false;
+        const bool c1 = wt <= q.tol[0] && wu <= q.tol[1] && wv <= q.tol[2];
+        int axis = 0;
+        {
+            double best = -INFINITY;
+            const double w3[3] = { wt, wu, wv };
+            for (int i = 0; i < 3; i++) {
+                const double r = w3[i] > q.tol[i] ? w3[i] / q.tol[i] : -INFINITY;
+                if (r > best) {
+                    best = r;
+                    axis = i;
+                }
+            }
+        }
+        const int nk = (axis == 0 ? kt : axis == 1 ? ku : kv) + 1;
+        int nlo = 0, nhi = 0;
+        for (int i = 0; i < ncur; i++) {
+            const unsigned long long pb = L.at(0, i);
+            const unsigned it = (unsigned)(pb >> (2 * TI_LDS_BITS));
+            const unsigned iu = (unsigned)(pb >> TI_LDS_BITS) & TI_LDS_MASK;
+            const unsigned iv = (unsigned)pb & TI_LDS_MASK;
+            const TIBox b { ldexp((double)it, -kt), ldexp((double)iu, -ku),
+                            ldexp((double)iv, -kv) };
+            if (b.t >= toi_skip || b.t >= bound) continue;
+            refine++;
+            bool box_in;
+            double tw[3];
+            if (!ti_inclusion(q, b, wt, wu, wv, ms, box_in, tw)) continue;
+            const bool tol_cond = tw[0] <= co_tol && tw[1] <= co_tol && tw[2] <= co_tol;
+            const bool c2 = box_in && level_less_tol;
+            if (!tol_cond) level_less_tol = false;
+            if (c1 || c2 || level_less_tol) {
+                toi = b.t;
+                return TI_HIT;
+            }
+            if (max_itr > 0) {
+                if (!level_root) {
+                    temp_toi = b.t;
+                    double m = co_tol;
+                    for (int k = 0; k < 3; k++) m = tw[k] > m ? tw[k] : m;
+                    temp_out_tol = m;
+                    level_root = true;
+                }
+                if (refine > max_itr) {
+                    toi = temp_toi;
+                    out_tol = temp_out_tol;
+                    return TI_HIT;
+                }
+            }
+            if (tol_cond || box_in) {
+                toi_skip = b.t < toi_skip ? b.t : toi_skip;
+                use_skip = true;
+                continue;
+            }
+            if (nk > TI_LDS_BITS) return TI_OVERFLOW;
+            if (axis == 0) {
+                if (nlo >= L.cap || nhi >= L.cap) return TI_OVERFLOW;
+                L.at(2, nhi++) = ti_pack(2 * it + 1, iu, iv);
+                L.at(1, nlo++) = ti_pack(2 * it, iu, iv);
+            } else {
+                // upper half first, each kept when it can hold a point of the
+                // triangle (vertex-face)
+                const unsigned hu = axis == 1 ? 2 * iu + 1 : iu, hv = axis == 1 ? iv : 2 * iv + 1;
+                const unsigned lu = axis == 1 ? 2 * iu : iu, lv = axis == 1 ? iv : 2 * iv;
+                const int nku = axis == 1 ? nk : ku, nkv = axis == 1 ? kv : nk;
+                if (!q.vf || ldexp((double)hu, -nku) + ldexp((double)hv, -nkv) <= 1) {
+                    if (nlo >= L.cap) return TI_OVERFLOW;
+                    L.at(1, nlo++) = ti_pack(it, hu, hv);
+                }
+                if (!q.vf || b.u + b.v <= 1) {
+                    if (nlo >= L.cap) return TI_OVERFLOW;
+                    L.at(1, nlo++) = ti_pack(it, lu, lv);
+                }
+            }
+        }
+        // next level
+        if (axis == 0) {
+            kt = nk;
+            if (nlo + nhi > L.cap) return TI_OVERFLOW;
+            int i = 0, j = 0, k = 0;
+            while (i < nlo && j < nhi) {
+                const unsigned long long a = L.at(1, i), h = L.at(2, j);
+                if ((h >> (2 * TI_LDS_BITS)) <= (a >> (2 * TI_LDS_BITS))) {
+                    L.at(0, k++) = h;
+                    j++;
+                } else {
+                    L.at(0, k++) = a;
+                    i++;
+                }
+            }
+            while (i < nlo) L.at(0, k++) = L.at(1, i++);
+            while (j < nhi) L.at(0, k++) = L.at(2, j++);
+            ncur = k;
+        } else {
+            if (axis == 1) ku = nk;
+            else kv = nk;
+            for (int i = 0; i < nlo; i++) L.at(0, i) = L.at(1, i);
+            ncur = nlo;
+        }
+    }
+    if (use_skip) {
+        toi = toi_skip;
+        return TI_HIT;
+    }
+    return TI_MISS;
+}
+
 // ticcd::{edgeEdge,vertexFace}CCD with the automatic error filter and the
-// no_zero_toi loop.
-__device__ inline int ti_query(TIQuery& q, double ms, double tolerance, long max_itr,
-                               bool no_zero_toi, const TIBound& B, const TISlab& S, double& toi)
+// no_zero_toi loop around one of the searches above.
+template <class Search>
+__device__ inline int ti_query_with(TIQuery& q, double ms, double tolerance, long max_itr,
+                                    bool no_zero_toi, const Search& bfs, double& toi)
 {
     ti_param_tol(q, tolerance);
     ti_filter(q, ms > 0);
     int r;
     double out_tol;
     for (int it = 0;; it++) {
-        r = ti_bfs(q, tolerance, ms, max_itr, B, S, toi, out_tol);
+        r = bfs(q, tolerance, ms, max_itr, toi, out_tol);
         if (it + 1 >= TI_MAX_NO_ZERO_TOI_ITER) break;
         if (!(no_zero_toi && r == TI_HIT && toi == 0)) break;
         if (out_tol > tolerance) {
@@ -319,6 +474,29 @@ __device__ inline int ti_query(TIQuery& q, double ms, double tolerance, long max
         }
     }
     return r;
+}
+
+__device__ inline int ti_query(TIQuery& q, double ms, double tolerance, long max_itr,
+                               bool no_zero_toi, const TIBound& B, const TISlab& S, double& toi)
+{
+    return ti_query_with(
+        q, ms, tolerance, max_itr, no_zero_toi,
+        [&](const TIQuery& qq, double co_tol, double m, long mi, double& t, double& ot) {
+            return ti_bfs(qq, co_tol, m, mi, B, S, t, ot);
+        },
+        toi);
+}
+
+__device__ inline int ti_query_lds(TIQuery& q, double ms, double tolerance, long max_itr,
+                                   bool no_zero_toi, const TIBound& B, const TILdsLists& L,
+                                   double& toi)
+{
+    return ti_query_with(
+        q, ms, tolerance, max_itr, no_zero_toi,
+        [&](const TIQuery& qq, double co_tol, double m, long mi, double& t, double& ot) {
+            return ti_bfs_lds(qq, co_tol, m, mi, B, L, t, ot);
+        },
+        toi);
 }
 
 } // namespace ipcg
