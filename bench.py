@@ -30,6 +30,25 @@ sys.path.insert(0, os.path.join(ROOT, "ipc-toolkit_amd"))
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix), TFLOP/s
 
+# Algorithmic work models (per unit).
+# Hessian record of one collision: ids, weight (+ eps, dtype for EE), the
+# stencil's points, and the packed projected Hessian written out.
+REC_BYTES = {"ee": 8 + 8 + 8 + 4 + 4 * 24 + 45 * 8,
+             "fv": 8 + 8 + 4 * 24 + 45 * 8,
+             "ev": 8 + 8 + 3 * 24 + 21 * 8}
+# Jacobi PSD projection of an n x n matrix: SWEEPS[n] sweeps of n(n-1)/2
+# rotations (12n + 7 flops each), a convergence sum per sweep (n(n+1)
+# flops) and the reconstruction V diag(lambda) V^T (3n flops per packed
+# entry); SWEEPS is the mean sweep count measured on contact stencils.
+SWEEPS = {3: 4, 6: 6, 9: 7}
+PSD_FLOPS = {n: SWEEPS[n] * (n * (n - 1) // 2 * (12 * n + 7) + n * (n + 1))
+             + 3 * n * n * (n + 1) // 2 for n in (3, 6, 9)}
+# Tight-Inclusion: one box = the 8 corners of F along 3 axes (about 70 flops
+# per axis); a candidate that survives the root pass visits 5.0 boxes on
+# average on the sheet-stack workload (counted in-kernel).
+TI_BOX_FLOPS = 210
+TI_BOXES_PER_SEARCH = 5.0
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -42,6 +61,9 @@ def parse():
     p.add_argument("--cpu-tile", type=int, default=96,
                    help="sheet side of the CPU-baseline tile (0: skip)")
     p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only copy a MIB MiB device buffer five times (torch), the known-size "
+                        "reference for scaling the HBM byte counters of a profiler")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -93,8 +115,26 @@ def cpu_baseline(args, scenes):
                       f"{len(times)} steps, {t * 1e3:.1f} ms/step"}
 
 
+def copy_calibration(mib):
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_bytes": n, "launches": 5, "ms_per_copy": round(1e3 * dt, 4),
+                      "GBps": round(2 * n / dt / 1e9, 1)}))
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path would shard scenes or Morton ranges across GPUs, see DESIGN.md)",
@@ -142,32 +182,50 @@ def main():
     value = pairs * args.steps / total
 
     # Roofline of the dominant kernel.  Each stage below is one kernel
-    # launch per call (HIP events on the context stream); its algorithmic
-    # bytes per launch follow DESIGN.md section 3.
+    # launch per call, timed with HIP events on the context stream.  Its
+    # algorithmic bytes and FP64 flops per launch follow DESIGN.md section 3;
+    # the binding roof is the larger of bytes / HBM peak and flops / FP64 peak.
     per_launch = {k: v[0] / v[1] for k, v in stages.items()}
     per_step = {k: v[0] / args.steps for k, v in stages.items()}
     n_vv, n_ev, n_ee, n_fv = [int(x) for x in counts]
-    stage_bytes = {
-        "bp_traverse_ee.static": 24 * nE + 8 * ne_s,
-        "bp_traverse_fv.static": 24 * nV + 8 * nf_s,
-        "bp_traverse_ee.swept": 24 * nE + 8 * ne_d,
-        "bp_traverse_fv.swept": 24 * nV + 8 * nf_d,
-        "np_classify": 8 * (ne_s + nf_s) + 16 * pairs,
-        "energy": 16 * pairs + 8,
-        "gradient": 16 * pairs + 24 * nV,
-        "hessian_scatter_ee": 16 * n_ee,
-        "hessian_scatter_fv": 16 * n_fv,
-        "hessian_scatter_vv_ev": 16 * (n_vv + n_ev),
-        "hessian_assemble": 12 * nnz + 8 * (3 * nV + 1),
-        "ccd_root": (8 + 4 * 2 * 32) * (ne_d + nf_d),
+    searched = int(ccd_work["ccd.searched"])
+    model = {  # stage: (bytes, flops) per launch
+        "bp_traverse_ee.static": (24 * nE + 8 * ne_s, 0),
+        "bp_traverse_fv.static": (24 * nV + 8 * nf_s, 0),
+        "bp_traverse_ee.swept": (24 * nE + 8 * ne_d, 0),
+        "bp_traverse_fv.swept": (24 * nV + 8 * nf_d, 0),
+        "np_classify": (8 * (ne_s + nf_s) + 16 * pairs, 0),
+        "energy": (16 * pairs + 8, 0),
+        "hessian_records.ee": (REC_BYTES["ee"] * n_ee, PSD_FLOPS[9] * n_ee),
+        "hessian_records.fv": (REC_BYTES["fv"] * n_fv, PSD_FLOPS[9] * n_fv),
+        "hessian_records.ev": (REC_BYTES["ev"] * n_ev, PSD_FLOPS[6] * n_ev),
+        "hessian_assemble": (12 * nnz + 8 * (3 * nV + 1), 0),
+        "ccd_root": (200 * (ne_d + nf_d), 0),
+        "ccd_lds": (204 * searched, TI_BOX_FLOPS * TI_BOXES_PER_SEARCH * searched),
     }
-    known = {k: v for k, v in per_step.items() if k in stage_bytes}
-    dom = max(known, key=known.get)
-    achieved = stage_bytes[dom] / (per_launch[dom] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": dom, "ms_per_launch": round(per_launch[dom], 4),
-            "bytes_per_launch": int(stage_bytes[dom])}
+
+    def roof(k):
+        byt, flo = model[k]
+        t = per_launch[k] * 1e-3
+        gbs, tfs = byt / t / 1e9, flo / t / 1e12
+        if flo and tfs / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS:
+            return {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tfs / FP64_PEAK_TFLOPS, 5),
+                    "flops_per_launch": int(flo)}
+        return {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 5), "bytes_per_launch": int(byt)}
+
+    known = [k for k in per_step if k in model]
+    dom = max(known, key=per_step.get)
+    roof_dom = roof(dom)
+    roof_dom.update({"traffic": None, "kernel": dom,
+                     "ms_per_launch": round(per_launch[dom], 4)})
+    if roof_dom["bound"] == "mfma":
+        roof_dom["note"] = ("FP64 on the vector ALUs; the MI355X FP64 vector and "
+                            "matrix peaks are both 78.6 TFLOP/s")
+    roof_dom["others"] = {k: {"bound": r["bound"], "frac": r["frac"]}
+                          for k in sorted(known, key=per_step.get, reverse=True)[1:6]
+                          for r in [roof(k)]}
 
     cpu = None
     if args.cpu_tile > 0:
@@ -199,7 +257,7 @@ def main():
                    "stage_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
                    "bytes_algorithmic": algorithmic_bytes(sc, counts, ne_s + nf_s,
                                                           ne_d + nf_d, nnz)},
-        "roofline": roof,
+        "roofline": roof_dom,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -281,6 +281,23 @@ IPCG_HD void add_sym_outer(double* Hq, double alpha, const d3* a, const d3* b)
 }
 
 // ---------------------------------------------------------------------------
+// 1/sqrt(x) for x in [1, 2]: the hardware estimate refined by two Newton
+// steps (each squares the relative error), a few ulp from 1.0 / sqrt(x) at
+// a fraction of the instructions of a correctly rounded sqrt and divide.
+// ---------------------------------------------------------------------------
+IPCG_HD double rsqrt_nr(double x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    r = r * fma(-h * r, r, 1.5);
+    return r * fma(-h * r, r, 1.5);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
+// ---------------------------------------------------------------------------
 // PSD projection (cyclic Jacobi, eigenvectors accumulated in registers)
 // method: 0 none, 1 clamp, 2 abs (PSDProjectionMethod, eigen_ext.hpp:202-206)
 // ---------------------------------------------------------------------------
@@ -313,7 +330,7 @@ IPCG_HD void project_psd(double* A /* packed N */, int method)
                 const double theta = (aqq - app) / (2 * apq);
                 double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
                 t = apq == 0.0 ? 0.0 : t;
-                const double c = 1.0 / sqrt(t * t + 1), s = t * c;
+                const double c = rsqrt_nr(t * t + 1), s = t * c; // |t| <= 1
 #pragma unroll
                 for (int k = 0; k < N; k++) {
                     if (k == p || k == q) continue;

@@ -97,9 +97,12 @@ __global__ __launch_bounds__(256) void k_sum(const double* partial, int64_t n, d
     if (threadIdx.x == 0) *out = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
+// Atomic-free gradient: every collision writes its 3 NP derivatives to a
+// record (point-major), then one wave per vertex sums the records of the
+// vertex's incidences in a fixed order.
 template <int KIND>
-__global__ __launch_bounds__(PB) void k_gradient(CollView C, const double* X, int nV,
-                                                 BarrierParams bp, double* grad)
+__global__ __launch_bounds__(PB) void k_gradient_records(CollView C, const double* X, int nV,
+                                                         BarrierParams bp, double* rec)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
     const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
@@ -112,11 +115,12 @@ __global__ __launch_bounds__(PB) void k_gradient(CollView C, const double* X, in
     d3 g[NP];
     stencil_gradient<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
                            KIND == K_EE ? C.eps[i] : 0.0, bp, g);
+    double* o = rec + (int64_t)3 * NP * i;
 #pragma unroll
     for (int j = 0; j < NP; j++) {
-        atomicAdd(grad + 3 * (int64_t)v[j], g[j].x);
-        atomicAdd(grad + 3 * (int64_t)v[j] + 1, g[j].y);
-        atomicAdd(grad + 3 * (int64_t)v[j] + 2, g[j].z);
+        o[3 * j] = g[j].x;
+        o[3 * j + 1] = g[j].y;
+        o[3 * j + 2] = g[j].z;
     }
 }
 
@@ -200,6 +204,163 @@ __global__ void k_sort_rows(const uint32_t* bptr, int nV, int32_t* bcol)
             j--;
         }
         bcol[j] = x;
+    }
+}
+
+// ---- symbolic phase from the vertex incidences -------------------------------
+// The block columns of vertex v's row are the vertices of every stencil that
+// contains v.  One wave per vertex walks its incidences, inserts the stencil
+// vertices into a hash set in LDS, sorts the distinct ones and writes them
+// to a staging area with room for four per incidence; a second pass packs
+// them behind the scanned row lengths.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct CollRanges {
+    int64_t goff[5]; // global collision index ranges per kind
+};
+
+// vertices of global collision g (unused slots repeat the first vertex)
+__device__ __forceinline__ int4 incidence_vertices(const CollView& C, const CollRanges& G,
+                                                   int64_t g)
+{
+    int v[4];
+    if (g < G.goff[1]) {
+        stencil_ids<K_VV>(C, g, v);
+        return make_int4(v[0], v[1], v[0], v[0]);
+    }
+    if (g < G.goff[2]) {
+        stencil_ids<K_EV>(C, g - G.goff[1], v);
+        return make_int4(v[0], v[1], v[2], v[0]);
+    }
+    if (g < G.goff[3]) stencil_ids<K_EE>(C, g - G.goff[2], v);
+    else stencil_ids<K_FV>(C, g - G.goff[3], v);
+    return make_int4(v[0], v[1], v[2], v[3]);
+}
+
+constexpr int PH = 1024; // hash slots per wave
+constexpr int PU = 512;  // most distinct columns of a row on this path
+
+__global__ __launch_bounds__(256) void k_pattern_rows(CollView C, CollRanges G,
+                                                      const uint32_t* vptr, const uint32_t* inc,
+                                                      int nV, int32_t* stage, uint32_t* cnt,
+                                                      int* overflow)
+{
+    __shared__ int32_t hs[256 / WAVE][PH];
+    __shared__ int32_t uq[256 / WAVE][PU];
+    __shared__ unsigned nu[256 / WAVE];
+    const int wv = threadIdx.x / WAVE, l = lane_id();
+    const int64_t v = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
+    if (v >= nV) return;
+    const uint32_t i0 = vptr[v], i1 = vptr[v + 1];
+    if (i0 == i1) {
+        if (l == 0) cnt[v] = 0;
+        return;
+    }
+    for (int j = l; j < PH; j += WAVE) hs[wv][j] = -1;
+    if (l == 0) nu[wv] = 0;
+    wave_sync();
+    for (uint32_t j = i0 + l; j < i1; j += WAVE) {
+        const int4 vs = incidence_vertices(C, G, (int64_t)(inc[j] >> 2));
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int key = b == 0 ? vs.x : b == 1 ? vs.y : b == 2 ? vs.z : vs.w;
+            unsigned h = ((unsigned)key * 2654435761u) >> (32 - 10);
+            for (int probe = 0; probe < PH; probe++) {
+                const int old = atomicCAS(&hs[wv][h], -1, key);
+                if (old == -1) {
+                    const unsigned k = atomicAdd(&nu[wv], 1u);
+                    if (k < PU) uq[wv][k] = key;
+                    break;
+                }
+                if (old == key) break;
+                h = (h + 1) & (PH - 1);
+            }
+        }
+    }
+    wave_sync();
+    const unsigned n = nu[wv];
+    if (n > PU) {
+        if (l == 0) {
+            cnt[v] = 0;
+            atomicOr(overflow, 1);
+        }
+        return;
+    }
+    // bitonic sort of the n distinct columns (padded to a power of two)
+    unsigned P = 1;
+    while (P < n) P <<= 1;
+    for (unsigned t = n + l; t < P; t += WAVE) uq[wv][t] = INT32_MAX;
+    wave_sync();
+    for (unsigned k = 2; k <= P; k <<= 1)
+        for (unsigned jj = k >> 1; jj > 0; jj >>= 1) {
+            for (unsigned t = l; t < P; t += WAVE) {
+                const unsigned q = t ^ jj;
+                if (q > t) {
+                    const int a = uq[wv][t], b = uq[wv][q];
+                    if ((a > b) == ((t & k) == 0)) {
+                        uq[wv][t] = b;
+                        uq[wv][q] = a;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    int32_t* out = stage + 4 * (int64_t)i0;
+    for (unsigned t = l; t < n; t += WAVE) out[t] = uq[wv][t];
+    if (l == 0) cnt[v] = n;
+}
+
+__global__ __launch_bounds__(256) void k_pattern_pack(const uint32_t* vptr, const int32_t* stage,
+                                                      const uint32_t* bptr, int nV, int32_t* bcol)
+{
+    const int64_t v = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE; // uniform per wave
+    if (v >= nV) return;
+    const uint32_t b0 = bptr[v], n = bptr[v + 1] - b0;
+    const int32_t* in = stage + 4 * (int64_t)vptr[v];
+    for (uint32_t t = lane_id(); t < n; t += WAVE) bcol[b0 + t] = in[t];
+}
+
+struct GradRecs {
+    const double* rec[4];
+};
+
+__global__ __launch_bounds__(256) void k_gradient_rows(CollRanges G, GradRecs R,
+                                                       const uint32_t* vptr, const uint32_t* inc,
+                                                       int nV, double* grad)
+{
+    const int64_t v = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
+    if (v >= nV) return;
+    const int l = lane_id();
+    double gx = 0, gy = 0, gz = 0;
+    for (uint32_t j = vptr[v] + l; j < vptr[v + 1]; j += WAVE) {
+        const uint32_t x = inc[j];
+        const int64_t g = x >> 2;
+        const int a = (int)(x & 3u);
+        const double* r;
+        if (g < G.goff[1]) r = R.rec[0] + 6 * g;
+        else if (g < G.goff[2]) r = R.rec[1] + 9 * (g - G.goff[1]);
+        else if (g < G.goff[3]) r = R.rec[2] + 12 * (g - G.goff[2]);
+        else r = R.rec[3] + 12 * (g - G.goff[3]);
+        r += 3 * a;
+        gx += r[0];
+        gy += r[1];
+        gz += r[2];
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        gx += __shfl_xor(gx, o, WAVE);
+        gy += __shfl_xor(gy, o, WAVE);
+        gz += __shfl_xor(gz, o, WAVE);
+    }
+    if (l == 0) {
+        grad[3 * v] = gx;
+        grad[3 * v + 1] = gy;
+        grad[3 * v + 2] = gz;
     }
 }
 
@@ -492,21 +653,90 @@ double pot_energy(Ctx& c, double dhat, double kappa)
     return h[0];
 }
 
+// Global collision ranges per kind and the number of (vertex, collision)
+// incidences of the current collision set.
+CollRanges ranges(const CollView& C, int64_t& M)
+{
+    const int npk[4] = { 2, 3, 4, 4 };
+    CollRanges G;
+    G.goff[0] = 0;
+    M = 0;
+    for (int k = 0; k < 4; k++) {
+        G.goff[k + 1] = G.goff[k] + C.n[k];
+        M += (int64_t)npk[k] * C.n[k];
+    }
+    if (G.goff[4] >= (1ll << 30)) throw Error(IPCG_ERR_ARG, "too many collisions for one context");
+    return G;
+}
+
+// (vertex, global collision g, point a) incidences sorted by vertex (stable,
+// so each vertex's incidences keep the collision order) and the first
+// incidence of every vertex; kept until the next collision build.
+void require_incidences(Ctx& c, const CollView& C, const CollRanges& G, int64_t M)
+{
+    if (c.inc_valid) return;
+    hipStream_t s = c.stream;
+    c.stage_begin("incidence");
+    const int64_t mk = std::max<int64_t>(M, 1);
+    c.inc_keys.reserve(sizeof(uint64_t) * mk);
+    c.inc_keys2.reserve(sizeof(uint64_t) * mk);
+    c.inc_vals.reserve(sizeof(uint32_t) * mk);
+    c.inc_vals2.reserve(sizeof(uint32_t) * mk);
+    c.inc_ptr.reserve(sizeof(uint32_t) * ((size_t)c.nV + 1));
+    c.blk_tmp.reserve(radix_tmp_bytes(mk));
+    uint64_t* keys = c.inc_keys.as<uint64_t>();
+    uint32_t* vals = c.inc_vals.as<uint32_t>();
+    int64_t io = 0;
+    if (C.n[0]) hipLaunchKernelGGL(k_incidence<K_VV>, dim3(grid_for(C.n[0], 256, INT32_MAX)), dim3(256), 0, s, C, G.goff[0], io, keys, vals);
+    io += 2 * C.n[0];
+    if (C.n[1]) hipLaunchKernelGGL(k_incidence<K_EV>, dim3(grid_for(C.n[1], 256, INT32_MAX)), dim3(256), 0, s, C, G.goff[1], io, keys, vals);
+    io += 3 * C.n[1];
+    if (C.n[2]) hipLaunchKernelGGL(k_incidence<K_EE>, dim3(grid_for(C.n[2], 256, INT32_MAX)), dim3(256), 0, s, C, G.goff[2], io, keys, vals);
+    io += 4 * C.n[2];
+    if (C.n[3]) hipLaunchKernelGGL(k_incidence<K_FV>, dim3(grid_for(C.n[3], 256, INT32_MAX)), dim3(256), 0, s, C, G.goff[3], io, keys, vals);
+    IPCG_CHECK_LAUNCH();
+    int vb = 1;
+    while ((1ll << vb) <= c.nV) vb++;
+    radix_sort_pairs(keys, vals, c.inc_keys2.as<uint64_t>(), c.inc_vals2.as<uint32_t>(), M, vb,
+                     c.blk_tmp.p, s);
+    hipLaunchKernelGGL(k_run_ptr, dim3(grid_for(M + 1, 256, INT32_MAX)), dim3(256), 0, s, keys, M,
+                       c.nV, c.inc_ptr.as<uint32_t>());
+    IPCG_CHECK_LAUNCH();
+    c.inc_valid = true;
+    c.stage_end();
+}
+
 void pot_gradient(Ctx& c, double dhat, double kappa)
 {
     require_collisions(c);
     hipStream_t s = c.stream;
     CollView C = view(c);
     BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    require_incidences(c, C, G, M);
     c.grad.reserve(sizeof(double) * 3 * (size_t)std::max(c.nV, 1));
     double* g = c.grad.as<double>();
     const double* X = c.X[0].as<double>();
-    IPCG_HIP(hipMemsetAsync(g, 0, sizeof(double) * 3 * (size_t)c.nV, s));
+    // records share the Hessian record buffer (rewritten by every Hessian)
+    const int npk[4] = { 2, 3, 4, 4 };
+    GradRecs R;
+    int64_t nrec = 0;
+    for (int k = 0; k < 4; k++) nrec += 3 * (int64_t)npk[k] * C.n[k];
+    c.hrec.reserve(sizeof(double) * (size_t)std::max<int64_t>(nrec, 1));
+    int64_t ro = 0;
+    for (int k = 0; k < 4; k++) {
+        R.rec[k] = c.hrec.as<double>() + ro;
+        ro += 3 * (int64_t)npk[k] * C.n[k];
+    }
     c.stage_begin("gradient");
-    if (C.n[0]) hipLaunchKernelGGL(k_gradient<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, g);
-    if (C.n[1]) hipLaunchKernelGGL(k_gradient<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, g);
-    if (C.n[2]) hipLaunchKernelGGL(k_gradient<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, g);
-    if (C.n[3]) hipLaunchKernelGGL(k_gradient<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, g);
+    if (C.n[0]) hipLaunchKernelGGL(k_gradient_records<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[0]));
+    if (C.n[1]) hipLaunchKernelGGL(k_gradient_records<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[1]));
+    if (C.n[2]) hipLaunchKernelGGL(k_gradient_records<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[2]));
+    if (C.n[3]) hipLaunchKernelGGL(k_gradient_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
+    const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
+    hipLaunchKernelGGL(k_gradient_rows, dim3(gw), dim3(256), 0, s, G, R, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vals.as<uint32_t>(), c.nV, g);
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 }
@@ -576,17 +806,64 @@ void hess_pattern(Ctx& c)
     c.pattern_valid = true;
 }
 
+// Block pattern from the vertex incidences (inc_ptr / inc_vals valid).
+// Returns false, leaving the pattern invalid, when a row has more distinct
+// columns than the LDS path holds.
+bool hess_pattern_incidences(Ctx& c, const CollRanges& G, int64_t M)
+{
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    const size_t nv1 = (size_t)c.nV + 1;
+    c.blk_keys.reserve(sizeof(int32_t) * 4 * (size_t)std::max<int64_t>(M, 1));
+    c.hp_mark.reserve(sizeof(uint32_t) * nv1 * 3);
+    c.hp_bptr.reserve(sizeof(uint32_t) * nv1);
+    c.blk_tmp.reserve(scan_tmp_bytes(std::max<int64_t>(c.nV, 1)));
+    uint32_t* cnt = c.hp_mark.as<uint32_t>();
+    uint32_t* bptr = c.hp_bptr.as<uint32_t>();
+    int32_t* stage = c.blk_keys.as<int32_t>();
+    int* ovf = reinterpret_cast<int*>(c.counters.as<unsigned long long>() + 10);
+    IPCG_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
+    const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
+    hipLaunchKernelGGL(k_pattern_rows, dim3(gw), dim3(256), 0, s, C, G, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vals.as<uint32_t>(), c.nV, stage, cnt, ovf);
+    exclusive_scan_u32(cnt, bptr, c.nV, bptr + c.nV, c.blk_tmp.p, s);
+    IPCG_CHECK_LAUNCH();
+    uint32_t* h = static_cast<uint32_t*>(c.pinned);
+    IPCG_HIP(hipMemcpyAsync(h, bptr + c.nV, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipMemcpyAsync(h + 1, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    if (h[1]) return false;
+    c.nblocks = h[0];
+    c.hp_bcol.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(c.nblocks, 1));
+    hipLaunchKernelGGL(k_pattern_pack, dim3(gw), dim3(256), 0, s, c.inc_ptr.as<uint32_t>(), stage,
+                       bptr, c.nV, c.hp_bcol.as<int32_t>());
+    IPCG_CHECK_LAUNCH();
+    c.pattern_valid = true;
+    return true;
+}
+
 void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
 {
     require_collisions(c);
     if (psd < 0 || psd > 2) throw Error(IPCG_ERR_ARG, "invalid PSD projection method");
     hipStream_t s = c.stream;
-    c.stage_begin("hessian_pattern");
-    if (!c.pattern_valid) hess_pattern(c);
-    c.stage_end();
     CollView C = view(c);
     BarrierParams bp { dhat, kappa, c.coll_dmin, psd };
     const double* X = c.X[0].as<double>();
+    const int lenk[4] = { HqTraits<K_VV>::LEN, HqTraits<K_EV>::LEN, HqTraits<K_EE>::LEN,
+                          HqTraits<K_FV>::LEN };
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    require_incidences(c, C, G, M);
+    RecView R;
+    int64_t nrec = 0;
+    for (int k = 0; k < 5; k++) R.goff[k] = G.goff[k];
+    for (int k = 0; k < 4; k++) nrec += (int64_t)lenk[k] * C.n[k];
+    // block pattern: from the incidences, or by sorting the stencil pairs
+    // when a row outgrows the LDS path
+    c.stage_begin("hessian_pattern");
+    if (!c.pattern_valid && !hess_pattern_incidences(c, G, M)) hess_pattern(c);
+    c.stage_end();
     const int64_t nb = std::max<int64_t>(c.nblocks, 1);
     c.blk_mask.reserve(sizeof(uint32_t) * nb);
     c.hq.reserve(sizeof(double) * 9 * nb);
@@ -594,50 +871,6 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     double* bval = c.hq.as<double>();
     const uint32_t* bptr = c.hp_bptr.as<uint32_t>();
     const int32_t* bcol = c.hp_bcol.as<int32_t>();
-    // incidences of the collision set (sorted by vertex), cached with the
-    // block pattern until the next collision build
-    const int npk[4] = { 2, 3, 4, 4 };
-    const int lenk[4] = { HqTraits<K_VV>::LEN, HqTraits<K_EV>::LEN, HqTraits<K_EE>::LEN,
-                          HqTraits<K_FV>::LEN };
-    RecView R;
-    R.goff[0] = 0;
-    int64_t M = 0, nrec = 0;
-    for (int k = 0; k < 4; k++) {
-        R.goff[k + 1] = R.goff[k] + C.n[k];
-        M += (int64_t)npk[k] * C.n[k];
-        nrec += (int64_t)lenk[k] * C.n[k];
-    }
-    if (R.goff[4] >= (1ll << 30)) throw Error(IPCG_ERR_ARG, "too many collisions for one Hessian");
-    c.stage_begin("hessian_incidence");
-    if (!c.inc_valid) {
-        const int64_t mk = std::max<int64_t>(M, 1);
-        c.inc_keys.reserve(sizeof(uint64_t) * mk);
-        c.inc_keys2.reserve(sizeof(uint64_t) * mk);
-        c.inc_vals.reserve(sizeof(uint32_t) * mk);
-        c.inc_vals2.reserve(sizeof(uint32_t) * mk);
-        c.inc_ptr.reserve(sizeof(uint32_t) * ((size_t)c.nV + 1));
-        c.blk_tmp.reserve(radix_tmp_bytes(mk));
-        uint64_t* keys = c.inc_keys.as<uint64_t>();
-        uint32_t* vals = c.inc_vals.as<uint32_t>();
-        int64_t io = 0;
-        if (C.n[0]) hipLaunchKernelGGL(k_incidence<K_VV>, dim3(grid_for(C.n[0], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[0], io, keys, vals);
-        io += 2 * C.n[0];
-        if (C.n[1]) hipLaunchKernelGGL(k_incidence<K_EV>, dim3(grid_for(C.n[1], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[1], io, keys, vals);
-        io += 3 * C.n[1];
-        if (C.n[2]) hipLaunchKernelGGL(k_incidence<K_EE>, dim3(grid_for(C.n[2], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[2], io, keys, vals);
-        io += 4 * C.n[2];
-        if (C.n[3]) hipLaunchKernelGGL(k_incidence<K_FV>, dim3(grid_for(C.n[3], 256, INT32_MAX)), dim3(256), 0, s, C, R.goff[3], io, keys, vals);
-        IPCG_CHECK_LAUNCH();
-        int vb = 1;
-        while ((1ll << vb) <= c.nV) vb++;
-        radix_sort_pairs(keys, vals, c.inc_keys2.as<uint64_t>(), c.inc_vals2.as<uint32_t>(), M, vb,
-                         c.blk_tmp.p, s);
-        hipLaunchKernelGGL(k_run_ptr, dim3(grid_for(M + 1, 256, INT32_MAX)), dim3(256), 0, s, keys,
-                           M, c.nV, c.inc_ptr.as<uint32_t>());
-        IPCG_CHECK_LAUNCH();
-        c.inc_valid = true;
-    }
-    c.stage_end();
     // projected Hessian records
     c.hrec.reserve(sizeof(double) * (size_t)std::max<int64_t>(nrec, 1));
     double* rec = c.hrec.as<double>();
@@ -646,13 +879,28 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
         R.rec[k] = rec + ro;
         ro += (int64_t)lenk[k] * C.n[k];
     }
-    c.stage_begin("hessian_records");
-    if (C.n[0]) hipLaunchKernelGGL(k_hessian_records<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[0]));
-    if (C.n[1]) hipLaunchKernelGGL(k_hessian_records<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[1]));
-    if (C.n[2]) hipLaunchKernelGGL(k_hessian_records<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[2]));
-    if (C.n[3]) hipLaunchKernelGGL(k_hessian_records<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
+    // one launch per stencil kind, each its own timed stage
+    if (C.n[0]) {
+        c.stage_begin("hessian_records.vv");
+        hipLaunchKernelGGL(k_hessian_records<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[0]));
+        c.stage_end();
+    }
+    if (C.n[1]) {
+        c.stage_begin("hessian_records.ev");
+        hipLaunchKernelGGL(k_hessian_records<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[1]));
+        c.stage_end();
+    }
+    if (C.n[2]) {
+        c.stage_begin("hessian_records.ee");
+        hipLaunchKernelGGL(k_hessian_records<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[2]));
+        c.stage_end();
+    }
+    if (C.n[3]) {
+        c.stage_begin("hessian_records.fv");
+        hipLaunchKernelGGL(k_hessian_records<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
+        c.stage_end();
+    }
     IPCG_CHECK_LAUNCH();
-    c.stage_end();
     c.stage_begin("hessian_rows");
     const unsigned gwr = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_hessian_rows, dim3(gwr), dim3(256), 0, s, C, R, c.inc_ptr.as<uint32_t>(),
@@ -664,6 +912,7 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     const int64_t nrows = 3 * (int64_t)c.nV;
     c.row_cnt.reserve(sizeof(uint64_t) * (nrows + 1));
     c.rowptr.reserve(sizeof(int64_t) * (nrows + 1));
+    c.blk_tmp.reserve(scan_tmp_bytes(nrows + 1));
     const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_row_counts, dim3(gw), dim3(256), 0, s, bptr, bmask, c.nV,
                        c.row_cnt.as<uint64_t>());
