@@ -220,6 +220,16 @@ def main():
     roof_dom = roof(dom)
     roof_dom.update({"traffic": None, "kernel": dom,
                      "ms_per_launch": round(per_launch[dom], 4)})
+    # HBM traffic of the dominant kernel: the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes of this workload, scaled by a copy kernel of known size (the
+    # summary and the commands are under profiles/)
+    tpath = os.path.join(ROOT, "profiles", f"traffic__{sc.name}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("kernel") == dom:
+            roof_dom["traffic"] = int(t["traffic_bytes_per_launch"])
+            roof_dom["traffic_source"] = os.path.relpath(tpath, ROOT)
     if roof_dom["bound"] == "mfma":
         roof_dom["note"] = ("FP64 on the vector ALUs; the MI355X FP64 vector and "
                             "matrix peaks are both 78.6 TFLOP/s")
