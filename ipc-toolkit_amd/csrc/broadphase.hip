@@ -219,10 +219,13 @@ __global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, int 
 // Bottom-up box refit, one level of completed nodes per launch: a node's
 // box goes into its slot of the parent, and the second child to arrive puts
 // the parent on the next work list.  Boxes written in one launch are read
-// in the next, so no inter-workgroup fences are needed.
-__device__ __forceinline__ void refit_push(Node* nodes, int32_t p, int32_t self,
-                                           const float* lo, const float* hi, int32_t* visits,
-                                           int32_t* next, uint32_t* next_n)
+// in the next, so no inter-workgroup fences are needed.  Appends to the next
+// list are gathered per workgroup (one global atomic per RF_BLOCK nodes).
+constexpr int RF_BLOCK = 1024;
+
+// Writes the child's box into parent p; returns whether p is now complete.
+__device__ __forceinline__ bool refit_child(Node* nodes, int32_t p, int32_t self,
+                                            const float* lo, const float* hi, int32_t* visits)
 {
     Node* P = nodes + p;
     if (P->c0 == self) {
@@ -238,49 +241,70 @@ __device__ __forceinline__ void refit_push(Node* nodes, int32_t p, int32_t self,
             P->hi1[k] = hi[k];
         }
     }
-    if (atomicAdd(visits + p, 1) == 1) next[atomicAdd(next_n, 1u)] = p;
+    return atomicAdd(visits + p, 1) == 1;
 }
 
-__global__ __launch_bounds__(BP_BLOCK) void k_refit_leaves(const uint32_t* ids,
+// Appends p (when push) to the next work list; every thread of the block
+// must call it.
+__device__ __forceinline__ void refit_append(bool push, int32_t p, int32_t* next,
+                                             uint32_t* next_n)
+{
+    __shared__ uint32_t n_blk, base;
+    if (threadIdx.x == 0) n_blk = 0;
+    __syncthreads();
+    const uint32_t slot = push ? atomicAdd(&n_blk, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0 && n_blk) base = atomicAdd(next_n, n_blk);
+    __syncthreads();
+    if (push) next[base + slot] = p;
+}
+
+__global__ __launch_bounds__(RF_BLOCK) void k_refit_leaves(const uint32_t* ids,
                                                            const float* box32, int n,
                                                            const int32_t* leaf_parent,
                                                            Node* nodes, int32_t* visits,
                                                            int32_t* next, uint32_t* next_n)
 {
-    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t pid = ids[i];
-    float lo[3], hi[3];
+    const int i = blockIdx.x * RF_BLOCK + threadIdx.x;
+    bool push = false;
+    int32_t p = 0;
+    if (i < n) {
+        const uint32_t pid = ids[i];
+        float lo[3], hi[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        lo[k] = box32[(int64_t)6 * pid + k];
-        hi[k] = box32[(int64_t)6 * pid + 3 + k];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = box32[(int64_t)6 * pid + k];
+            hi[k] = box32[(int64_t)6 * pid + 3 + k];
+        }
+        p = leaf_parent[i];
+        push = refit_child(nodes, p, ~i, lo, hi, visits);
     }
-    refit_push(nodes, leaf_parent[i], ~i, lo, hi, visits, next, next_n);
+    refit_append(push, p, next, next_n);
 }
 
-__global__ __launch_bounds__(BP_BLOCK) void k_refit_level(const int32_t* cur,
+__global__ __launch_bounds__(RF_BLOCK) void k_refit_level(const int32_t* cur,
                                                           const uint32_t* cur_n,
                                                           const int32_t* parent, Node* nodes,
                                                           int32_t* visits, int32_t* next,
                                                           uint32_t* next_n, int32_t* root)
 {
-    const uint32_t t = blockIdx.x * BP_BLOCK + threadIdx.x;
-    if (t >= *cur_n) return;
-    const int32_t j = cur[t];
-    const Node* N = nodes + j;
-    float lo[3], hi[3];
+    const uint32_t t = blockIdx.x * RF_BLOCK + threadIdx.x;
+    bool push = false;
+    int32_t p = 0;
+    if (t < *cur_n) {
+        const int32_t j = cur[t];
+        const Node* N = nodes + j;
+        float lo[3], hi[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        lo[k] = fminf(N->lo0[k], N->lo1[k]);
-        hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(N->lo0[k], N->lo1[k]);
+            hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
+        }
+        p = parent[j];
+        if (p < 0) *root = j; // the tree is complete
+        else push = refit_child(nodes, p, j, lo, hi, visits);
     }
-    const int32_t p = parent[j];
-    if (p < 0) {
-        *root = j; // the tree is complete
-        return;
-    }
-    refit_push(nodes, p, j, lo, hi, visits, next, next_n);
+    refit_append(push, p, next, next_n);
 }
 
 __device__ __forceinline__ bool overlap(const float* q, const float* lo, const float* hi)
@@ -726,24 +750,35 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     c.stage_begin("bp_build");
     hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, BP_BLOCK)), dim3(BP_BLOCK), 0, s,
                        T.keys.as<uint64_t>(), n, T.nodes.as<Node>(), parent, leaf_parent);
-    hipLaunchKernelGGL(k_refit_leaves, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(),
-                       T.box.as<float>(), n, leaf_parent, T.nodes.as<Node>(), visits, work[0],
-                       cnt);
+    hipLaunchKernelGGL(k_refit_leaves, dim3(grid_for(n, RF_BLOCK)), dim3(RF_BLOCK), 0, s,
+                       T.ids.as<uint32_t>(), T.box.as<float>(), n, leaf_parent,
+                       T.nodes.as<Node>(), visits, work[0], cnt);
     IPCG_CHECK_LAUNCH();
+    // Round k completes the nodes of height k; their subtrees are disjoint and
+    // hold at least k + 1 leaves each, so round k has at most n / (k + 1)
+    // nodes.
+    // Rounds are launched in batches sized by that bound (a round past the
+    // root finds an empty list) and the host checks for the end once per
+    // batch.  Every round completes at least one node, so n - 1 rounds bound
+    // the loop.
     uint32_t* h = static_cast<uint32_t*>(c.pinned);
-    // every level completes at least one node, so n - 1 levels bound the loop
-    for (size_t level = 0;; level++) {
-        IPCG_HIP(hipMemcpyAsync(h, cnt + (level & 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        IPCG_HIP(hipStreamSynchronize(s));
-        const uint32_t m = h[0];
-        if (m == 0) break;
-        if (level >= ni) throw Error(IPCG_ERR_NUMERIC, "LBVH refit did not converge");
-        uint32_t* nxt = cnt + ((level + 1) & 1);
-        IPCG_HIP(hipMemsetAsync(nxt, 0, sizeof(uint32_t), s));
-        hipLaunchKernelGGL(k_refit_level, dim3(grid_for(m, BP_BLOCK)), dim3(BP_BLOCK), 0, s,
-                           work[level & 1], cnt + (level & 1), parent, T.nodes.as<Node>(),
-                           visits, work[(level + 1) & 1], nxt, T.root.as<int32_t>());
+    constexpr size_t BATCH = 8;
+    for (size_t level = 1;; level += BATCH) {
+        for (size_t k = level; k < level + BATCH; k++) {
+            const int64_t bound = std::max<int64_t>((int64_t)n / (int64_t)(k + 1), 1);
+            uint32_t* cur_n = cnt + ((k - 1) & 1);
+            uint32_t* nxt = cnt + (k & 1);
+            IPCG_HIP(hipMemsetAsync(nxt, 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_refit_level, dim3(grid_for(bound, RF_BLOCK)), dim3(RF_BLOCK), 0,
+                               s, work[(k - 1) & 1], cur_n, parent, T.nodes.as<Node>(), visits,
+                               work[k & 1], nxt, T.root.as<int32_t>());
+        }
         IPCG_CHECK_LAUNCH();
+        IPCG_HIP(hipMemcpyAsync(h, cnt + ((level + BATCH - 1) & 1), sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        if (h[0] == 0) break;
+        if (level >= ni) throw Error(IPCG_ERR_NUMERIC, "LBVH refit did not converge");
     }
     c.stage_end();
 }
