@@ -28,6 +28,7 @@
 
 #include "ipcg.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <initializer_list>
 #include <memory>
@@ -93,6 +94,80 @@ struct TightInclusionCCD {
     }
 };
 
+// CollisionFilter (collision_filter.hpp:30-152) as data: the intersection
+// (operator&) of an optional patch filter, static obstacle filter and codim
+// cross filter; the device evaluates the same predicate.
+struct CollisionFilter {
+    std::vector<int32_t> patches; // make_vertex_patches_filter (empty: none)
+    int32_t n_dynamic = -1;       // make_static_obstacle_filter (< 0: none)
+    int32_t n_codim_vertices = -1; // make_codim_cross_filter (< 0: none)
+
+    bool operator()(size_t vi, size_t vj) const
+    {
+        return (patches.empty() || patches[vi] != patches[vj])
+            && (n_dynamic < 0 || vi < (size_t)n_dynamic || vj < (size_t)n_dynamic)
+            && (n_codim_vertices < 0
+                || ((vi < (size_t)n_codim_vertices) != (vj < (size_t)n_codim_vertices)));
+    }
+    // collision_filter.hpp:74-81; only intersections of distinct filter kinds
+    // are representable here
+    friend CollisionFilter operator&(CollisionFilter a, const CollisionFilter& b)
+    {
+        if ((!a.patches.empty() && !b.patches.empty())
+            || (a.n_dynamic >= 0 && b.n_dynamic >= 0 && a.n_dynamic != b.n_dynamic)
+            || (a.n_codim_vertices >= 0 && b.n_codim_vertices >= 0
+                && a.n_codim_vertices != b.n_codim_vertices))
+            throw std::invalid_argument("unsupported collision filter intersection");
+        if (a.patches.empty()) a.patches = b.patches;
+        if (a.n_dynamic < 0) a.n_dynamic = b.n_dynamic;
+        if (a.n_codim_vertices < 0) a.n_codim_vertices = b.n_codim_vertices;
+        return a;
+    }
+};
+
+// collision_filter.hpp:113-118
+inline CollisionFilter make_vertex_patches_filter(std::vector<int32_t> patch_ids)
+{
+    CollisionFilter f;
+    f.patches = std::move(patch_ids);
+    return f;
+}
+
+// collision_filter.hpp:126-131
+inline CollisionFilter make_static_obstacle_filter(size_t n_dynamic)
+{
+    CollisionFilter f;
+    f.n_dynamic = (int32_t)n_dynamic;
+    return f;
+}
+
+// collision_filter.hpp:138-143
+inline CollisionFilter make_codim_cross_filter(size_t n_codim_vertices)
+{
+    CollisionFilter f;
+    f.n_codim_vertices = (int32_t)n_codim_vertices;
+    return f;
+}
+
+// collision_filter.hpp:152: vertices joined by faces share a label (union
+// find); a vertex in no face is its own component
+inline CollisionFilter make_connected_components_filter(const MatrixXi& faces, int num_vertices)
+{
+    std::vector<int32_t> root((size_t)num_vertices);
+    for (int i = 0; i < num_vertices; i++) root[(size_t)i] = i;
+    auto find = [&](int32_t x) {
+        while (root[(size_t)x] != x) x = root[(size_t)x] = root[(size_t)root[(size_t)x]];
+        return x;
+    };
+    for (int f = 0; f < faces.rows(); f++)
+        for (int k = 1; k < 3; k++) {
+            const int32_t a = find(faces(f, 0)), b = find(faces(f, k));
+            if (a != b) root[(size_t)std::max(a, b)] = std::min(a, b);
+        }
+    for (int i = 0; i < num_vertices; i++) root[(size_t)i] = find(i);
+    return make_vertex_patches_filter(std::move(root));
+}
+
 // collision_mesh.hpp:14-126 (full-mesh form; vertex_patches gives the
 // patch filter of collision_filter.hpp:113-118)
 class CollisionMesh {
@@ -100,7 +175,8 @@ public:
     CollisionMesh(const MatrixXd& rest_positions, const MatrixXi& edges = MatrixXi(0, 2),
                   const MatrixXi& faces = MatrixXi(0, 3),
                   const std::vector<int32_t>& vertex_patches = {}, int device = 0)
-        : nV(rest_positions.rows()), nE(edges.rows()), nF(faces.rows()), E(edges), F(faces)
+        : nV(rest_positions.rows()), nE(edges.rows()), nF(faces.rows()), E(edges), F(faces),
+          patches(vertex_patches)
     {
         if (rest_positions.rows() && rest_positions.cols() != 3)
             throw std::invalid_argument("positions must be #V x 3");
@@ -134,6 +210,20 @@ public:
     uint64_t collisions_generation() const { return cgeneration; }
     void touch_collisions() const { cgeneration++; }
 
+    // CollisionMesh::can_collide (collision_mesh.hpp:338).  The patch labels
+    // are part of the uploaded mesh: a filter's patches must be those given
+    // at construction (or none).
+    void set_can_collide(const CollisionFilter& f)
+    {
+        if (!f.patches.empty() && f.patches != patches)
+            throw std::invalid_argument("patch labels are fixed when the mesh is built");
+        if (f.patches.empty() && !patches.empty())
+            throw std::invalid_argument("the mesh was built with patch labels");
+        detail::check(ctx.get(), ipcg_mesh_filter(ctx.get(), f.n_dynamic, f.n_codim_vertices));
+        touch_broad_phase();
+        touch_collisions();
+    }
+
     // slot 0: x / x_t0, slot 1: x_t1
     void set_positions(int slot, const MatrixXd& V) const
     {
@@ -150,6 +240,7 @@ private:
     int32_t ncv = 0, nce = 0;
     mutable uint64_t generation = 0, cgeneration = 0;
     MatrixXi E, F;
+    std::vector<int32_t> patches;
     std::unique_ptr<ipcg_ctx, Del> ctx;
 };
 

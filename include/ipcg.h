@@ -69,6 +69,18 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest_positions,
                      int layout, int32_t nE, const int32_t* edges, int32_t nF,
                      const int32_t* faces, const int32_t* vertex_patch);
 
+/* Further vertex-pair filters of CollisionMesh::can_collide
+ * (collision_mesh.hpp:338), intersected with the patch filter
+ * (CollisionFilter operator&, collision_filter.hpp:74-81): n_dynamic >= 0
+ * adds make_static_obstacle_filter(n_dynamic) (collision_filter.hpp:126-
+ * 131: i < n_dynamic || j < n_dynamic), n_codim_vertices >= 0 adds
+ * make_codim_cross_filter (collision_filter.hpp:138-143: exactly one of i, j
+ * below n_codim_vertices); -1 turns a filter off.
+ * make_connected_components_filter (collision_filter.hpp:152) is the patch
+ * filter with per-vertex component labels.  A new mesh upload
+ * turns both off; a change invalidates candidates and collisions. */
+int ipcg_mesh_filter(ipcg_ctx* ctx, int32_t n_dynamic, int32_t n_codim_vertices);
+
 /* ---- positions ----------------------------------------------------------
  * slot 0: the configuration x (static queries, potentials, CCD start);
  * slot 1: the CCD end configuration.  _device takes a device pointer

@@ -313,15 +313,27 @@ __device__ __forceinline__ bool overlap(const float* q, const float* lo, const f
         && lo[2] <= q[5];
 }
 
+// CollisionMesh::can_collide as data: the intersection of the vertex
+// patches filter (patch[i] != patch[j]), the static obstacle filter (i or j
+// below n_dynamic) and the codim cross filter (exactly one of i, j below
+// n_codim), each optional (collision_filter.hpp:113-143).
 struct Filter {
     const int32_t* E;
     const int32_t* F;
     const int32_t* patch; // may be null
+    int32_t n_dynamic;    // < 0: no static obstacle filter
+    int32_t n_codim;      // < 0: no codim cross filter
+    __device__ __forceinline__ bool any() const
+    {
+        return patch != nullptr || n_dynamic >= 0 || n_codim >= 0;
+    }
 };
 
 __device__ __forceinline__ bool can_vv(const Filter& f, int a, int b)
 {
-    return f.patch == nullptr || f.patch[a] != f.patch[b];
+    return (f.patch == nullptr || f.patch[a] != f.patch[b])
+        && (f.n_dynamic < 0 || a < f.n_dynamic || b < f.n_dynamic)
+        && (f.n_codim < 0 || ((a < f.n_codim) != (b < f.n_codim)));
 }
 
 // lbvh.cpp:826-849 can_edges_collide
@@ -365,13 +377,14 @@ struct Query {
     int v[3];
 };
 
+// some vertex of a may collide with some vertex of b
 __device__ __forceinline__ bool patch_ok(const Filter& f, const int* a, int na, const int* b,
                                          int nb)
 {
-    if (!f.patch) return true;
+    if (!f.any()) return true;
     for (int i = 0; i < na; i++)
         for (int j = 0; j < nb; j++)
-            if (f.patch[a[i]] != f.patch[b[j]]) return true;
+            if (can_vv(f, a[i], b[j])) return true;
     return false;
 }
 
@@ -848,7 +861,8 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     }
 
     c.counters.reserve(sizeof(unsigned long long) * 4);
-    Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr };
+    Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr,
+                 c.n_dynamic, c.n_codim_filter };
     int* status = reinterpret_cast<int*>(c.counters.as<unsigned long long>() + 2);
     c.shard_cnt.reserve(4 * SHARD_CNT_BYTES);
     c.shard_off.reserve(sizeof(int64_t) * 4 * (SHARDS + 1));

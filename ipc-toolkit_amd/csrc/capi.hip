@@ -264,12 +264,25 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
             c.patch.reserve(sizeof(int32_t) * (size_t)std::max(nV, 1));
             if (nV) IPCG_HIP(hipMemcpyAsync(c.patch.p, vertex_patch, sizeof(int32_t) * nV, hipMemcpyHostToDevice, c.stream));
         }
+        c.n_dynamic = c.n_codim_filter = -1;
         upload_positions(c, c.rest, rest, layout, false);
         IPCG_HIP(hipStreamSynchronize(c.stream));
         c.has_mesh = true;
         c.has_pos[0] = c.has_pos[1] = false;
         c.cand_valid = c.coll_valid = c.hess_valid = false;
         c.scap_ee = c.scap_fv = c.scap_vv = c.scap_ev = 0;
+    });
+}
+
+int ipcg_mesh_filter(ipcg_ctx* ctx, int32_t n_dynamic, int32_t n_codim_vertices)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.require_mesh();
+        if (n_dynamic > c.nV || n_codim_vertices > c.nV)
+            throw Error(IPCG_ERR_ARG, "filter bound beyond the vertex count");
+        c.n_dynamic = n_dynamic < 0 ? -1 : n_dynamic;
+        c.n_codim_filter = n_codim_vertices < 0 ? -1 : n_codim_vertices;
+        c.cand_valid = c.coll_valid = c.hess_valid = false;
     });
 }
 

@@ -50,6 +50,7 @@ struct Ctx {
     // mesh
     int32_t nV = 0, nE = 0, nF = 0;
     bool has_mesh = false, has_patch = false;
+    int32_t n_dynamic = -1, n_codim_filter = -1; // optional vertex-pair filters
     DevBuf E, F, F2E, patch, rest;
     DevBuf X[2];
     bool has_pos[2] = { false, false };

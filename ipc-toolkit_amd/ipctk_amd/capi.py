@@ -31,7 +31,7 @@ EXPORTS = [
     "ipcg_is_step_collision_free", "ipcg_candidates_toi", "ipcg_stage_times",
     "ipcg_stage_times_reset", "ipcg_mesh_codim", "ipcg_codim_candidates_count",
     "ipcg_codim_candidates_download", "ipcg_codim_candidates_toi",
-    "ipcg_candidates_is_step_collision_free", "ipcg_counter",
+    "ipcg_candidates_is_step_collision_free", "ipcg_counter", "ipcg_mesh_filter",
 ]
 
 _lib = None
@@ -85,6 +85,7 @@ def load(path=LIB_PATH):
                                               C.c_double, C.c_int, C.POINTER(C.c_int)]
     L.ipcg_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double, _D, _D]
     L.ipcg_mesh_codim.argtypes = [vp, _I, _I]
+    L.ipcg_mesh_filter.argtypes = [vp, C.c_int32, C.c_int32]
     L.ipcg_counter.argtypes = [vp, C.c_char_p, _L]
     L.ipcg_candidates_is_step_collision_free.argtypes = [vp, C.c_double, C.c_double, C.c_int64,
                                                          C.c_double, C.POINTER(C.c_int)]
@@ -177,6 +178,11 @@ class Context:
         self._check(self._L.ipcg_candidates_download(self._p, ee.ctypes.data_as(_I),
                                                      fv.ctypes.data_as(_I)))
         return ee, fv
+
+    def set_filter(self, n_dynamic=-1, n_codim_vertices=-1):
+        """Static obstacle / codim cross filters, intersected with the
+        vertex patches given to upload_mesh (-1: off)."""
+        self._check(self._L.ipcg_mesh_filter(self._p, int(n_dynamic), int(n_codim_vertices)))
 
     def codim_counts(self):
         """(codim vertices, codim edges) of the uploaded mesh"""

@@ -154,9 +154,47 @@ static void errors_throw()
     CHECK(threw);
 }
 
+// Vertex-pair filters (collision_filter.hpp:113-152) on the codim cube:
+// its 12 vertex-vertex collisions are the cube's edges; vertex i has x = 1
+// iff i >= 4.  Static obstacles {4..7} drop the 4 edges of the x = 1 face;
+// the codim cross filter with the first 4 vertices keeps the 4 edges along
+// x; without faces every vertex is its own connected component.
+static void collision_filters()
+{
+    using namespace ipc;
+    MatrixXd V { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 1, 0 }, { 0, 1, 1 },
+                 { 1, 0, 0 }, { 1, 0, 1 }, { 1, 1, 0 }, { 1, 1, 1 } };
+    const double dhat = 0.25, dmin = 0.8;
+    CollisionMesh mesh(V);
+    auto count = [&](const CollisionFilter& f) {
+        mesh.set_can_collide(f);
+        NormalCollisions c;
+        c.build(mesh, V, dhat, dmin);
+        return c.vv_collisions.size();
+    };
+    CHECK(count(CollisionFilter()) == 12);
+    CHECK(count(make_static_obstacle_filter(4)) == 8);
+    CHECK(count(make_codim_cross_filter(4)) == 4);
+    CHECK(count(make_static_obstacle_filter(4) & make_codim_cross_filter(4)) == 4);
+    CHECK(count(make_static_obstacle_filter(0)) == 0);
+    CHECK(make_connected_components_filter(MatrixXi(0, 3), 8)(0, 1));
+    CHECK(count(CollisionFilter()) == 12);
+    MatrixXi F { { 0, 1, 2 }, { 4, 5, 6 } };
+    const CollisionFilter cc = make_connected_components_filter(F, 8);
+    CHECK(!cc(0, 2) && cc(0, 4) && !cc(4, 6) && cc(3, 7));
+    bool threw = false;
+    try {
+        mesh.set_can_collide(cc); // patch labels belong to the uploaded mesh
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
 int main()
 {
     try {
+        collision_filters();
         codim_vertex_vertex();
         codim_edge_vertex();
         vertex_on_plane();

@@ -212,6 +212,77 @@ def test_vertex_patch_filter(capi, O, scenes):
         ctx.close()
 
 
+def _filtered(E, F, ee, fv, pred):
+    """candidates of an unfiltered set that pass a vertex-pair predicate on
+    some vertex pair (lbvh.cpp:800-858)"""
+    Ee = E[ee[:, 0]], E[ee[:, 1]]
+    keep_ee = np.zeros(len(ee), bool)
+    for i in range(2):
+        for j in range(2):
+            keep_ee |= pred(Ee[0][:, i], Ee[1][:, j])
+    Ff = F[fv[:, 0]]
+    keep_fv = np.zeros(len(fv), bool)
+    for i in range(3):
+        keep_fv |= pred(fv[:, 1], Ff[:, i])
+    return ee[keep_ee], fv[keep_fv]
+
+
+@pytest.mark.parametrize("case", ["static", "codim", "patch+static+codim"])
+def test_static_obstacle_and_codim_cross_filters(capi, O, scenes, case):
+    """make_static_obstacle_filter / make_codim_cross_filter, alone and
+    intersected with the patch filter (collision_filter.hpp:74-143): the
+    device's filtered candidates are the unfiltered oracle set restricted by
+    the predicate, and the collisions follow from them"""
+    s = make(scenes, "sheets:16:0")
+    nV = len(s.V)
+    nd, nc = nV // 2 + 7, nV // 3
+    group = (np.arange(nV) % 5).astype(np.int32)
+    use_patch = "patch" in case
+    n_dyn = nd if "static" in case else -1
+    n_cod = nc if "codim" in case else -1
+
+    def pred(a, b):
+        ok = np.ones(np.shape(a), bool)
+        if use_patch:
+            ok &= group[a] != group[b]
+        if n_dyn >= 0:
+            ok &= (a < n_dyn) | (b < n_dyn)
+        if n_cod >= 0:
+            ok &= (a < n_cod) != (b < n_cod)
+        return ok
+
+    say("case filter", case, s.sizes)
+    ctx = capi.Context(0)
+    try:
+        ctx.upload_mesh(s.V, s.E, s.F, vertex_patch=group if use_patch else None)
+        ctx.set_filter(n_dyn, n_cod)
+        ctx.upload_positions(0, s.V)
+        ctx.upload_positions(1, s.V1)
+        for swept in (False, True):
+            r = 0.0 if swept else 0.5 * s.dhat
+            ctx.broad_phase(swept, r)
+            ee_g, fv_g = ctx.candidates()
+            ee_o, fv_o = O.candidates(s.V, s.E, s.F, r, V1=s.V1 if swept else None)
+            ee_f, fv_f = _filtered(s.E, s.F, ee_o, fv_o, pred)
+            assert 0 < len(ee_f) < len(ee_o) or len(ee_o) == 0
+            assert pair_set(ee_g) == pair_set(ee_f), (case, swept)
+            assert pair_set(fv_g) == pair_set(fv_f), (case, swept)
+        ee_o, fv_o = O.candidates(s.V, s.E, s.F, 0.5 * s.dhat)
+        ee_f, fv_f = _filtered(s.E, s.F, ee_o, fv_o, pred)
+        co = O.build_collisions(s.V, s.E, s.F, ee_f, fv_f, s.dhat)
+        ctx.build_collisions(s.dhat)
+        cg = ctx.collisions()
+        for k in ("vv", "ev", "ee", "fv"):
+            assert np.array_equal(cg[k], getattr(co, k)), (case, k)
+            assert np.array_equal(cg[k + "_w"], getattr(co, k + "_w")), (case, k)
+        ctx.set_filter(-1, -1)  # off again: the unfiltered set
+        ctx.broad_phase(False, 0.5 * s.dhat)
+        if not use_patch:
+            assert pair_set(ctx.candidates()[0]) == pair_set(ee_o)
+    finally:
+        ctx.close()
+
+
 def test_empty_and_small_meshes(ctx, O):
     """no candidates, a lone triangle, an edge soup without faces"""
     T = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
