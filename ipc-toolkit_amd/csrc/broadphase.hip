@@ -459,6 +459,12 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     }
     int node = root, sp = 0, steps = 0;
     const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
+    // Hits wait in registers (up to four per lane) and the wave appends them
+    // together when a lane could overflow or the walk ends: one atomic per
+    // several iterations instead of one per iteration with a hit.
+    int2 b0 = make_int2(-1, -1), b1 = b0, b2 = b0, b3 = b0;
+    unsigned nb = 0;
+    int2* const reg = out + sh * S.scap;
     while (__any(active)) {
         if (active && ++steps > max_steps) {
             atomicOr(status, 2);
@@ -505,15 +511,34 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
                 }
             }
         }
-        unsigned pre;
-        int64_t slot = (int64_t)shard_append_bits(S, sh, h0.x >= 0, h1.x >= 0, pre);
-        int2* reg = out + sh * S.scap;
-        slot += pre;
+        // (nb <= 2 here) queue h0 then h1
         if (h0.x >= 0) {
-            if (slot < S.scap) reg[slot] = h0;
-            slot++;
+            b2 = nb == 2 ? h0 : b2;
+            b1 = nb == 1 ? h0 : b1;
+            b0 = nb == 0 ? h0 : b0;
+            nb++;
         }
-        if (h1.x >= 0 && slot < S.scap) reg[slot] = h1;
+        if (h1.x >= 0) {
+            b3 = nb == 3 ? h1 : b3;
+            b2 = nb == 2 ? h1 : b2;
+            b1 = nb == 1 ? h1 : b1;
+            b0 = nb == 0 ? h1 : b0;
+            nb++;
+        }
+        if (__any(nb > 2)) { // all lanes together
+            const int64_t first = (int64_t)shard_append(S, sh, nb);
+            if (nb > 0 && first < S.scap) reg[first] = b0;
+            if (nb > 1 && first + 1 < S.scap) reg[first + 1] = b1;
+            if (nb > 2 && first + 2 < S.scap) reg[first + 2] = b2;
+            if (nb > 3 && first + 3 < S.scap) reg[first + 3] = b3;
+            nb = 0;
+        }
+    }
+    if (__any(nb > 0)) {
+        const int64_t first = (int64_t)shard_append(S, sh, nb);
+        if (nb > 0 && first < S.scap) reg[first] = b0;
+        if (nb > 1 && first + 1 < S.scap) reg[first + 1] = b1;
+        if (nb > 2 && first + 2 < S.scap) reg[first + 2] = b2;
     }
 }
 
