@@ -26,6 +26,7 @@ constexpr int BP_BLOCK = 256;
 constexpr int TR_BLOCK = 128;
 constexpr int TR_STACK = 64;  // lbvh.cpp:378 MAX_STACK_SIZE
 constexpr int TR_LSTACK = 16; // entries per lane kept in LDS; the rest in HBM
+constexpr int TR_BUF = 4;     // hits a lane holds before the wave appends
 
 // AABB::from_point (+ swept union), aabb.cpp:21-55
 __device__ __forceinline__ void vbox(const double* X0, const double* X1, int nV, int i, double r,
@@ -459,10 +460,12 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     }
     int node = root, sp = 0, steps = 0;
     const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
-    // Hits wait in registers (up to four per lane) and the wave appends them
+    // Hits wait in registers (up to TR_BUF per lane) and the wave appends them
     // together when a lane could overflow or the walk ends: one atomic per
     // several iterations instead of one per iteration with a hit.
-    int2 b0 = make_int2(-1, -1), b1 = b0, b2 = b0, b3 = b0;
+    int2 buf[TR_BUF];
+#pragma unroll
+    for (int k = 0; k < TR_BUF; k++) buf[k] = make_int2(-1, -1);
     unsigned nb = 0;
     int2* const reg = out + sh * S.scap;
     while (__any(active)) {
@@ -511,34 +514,31 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
                 }
             }
         }
-        // (nb <= 2 here) queue h0 then h1
+        // (nb <= TR_BUF - 2 here) queue h0 then h1; constant indices keep
+        // the buffer in registers
         if (h0.x >= 0) {
-            b2 = nb == 2 ? h0 : b2;
-            b1 = nb == 1 ? h0 : b1;
-            b0 = nb == 0 ? h0 : b0;
+#pragma unroll
+            for (int k = 0; k < TR_BUF; k++) buf[k] = nb == (unsigned)k ? h0 : buf[k];
             nb++;
         }
         if (h1.x >= 0) {
-            b3 = nb == 3 ? h1 : b3;
-            b2 = nb == 2 ? h1 : b2;
-            b1 = nb == 1 ? h1 : b1;
-            b0 = nb == 0 ? h1 : b0;
+#pragma unroll
+            for (int k = 0; k < TR_BUF; k++) buf[k] = nb == (unsigned)k ? h1 : buf[k];
             nb++;
         }
-        if (__any(nb > 2)) { // all lanes together
+        if (__any(nb > TR_BUF - 2)) { // all lanes together
             const int64_t first = (int64_t)shard_append(S, sh, nb);
-            if (nb > 0 && first < S.scap) reg[first] = b0;
-            if (nb > 1 && first + 1 < S.scap) reg[first + 1] = b1;
-            if (nb > 2 && first + 2 < S.scap) reg[first + 2] = b2;
-            if (nb > 3 && first + 3 < S.scap) reg[first + 3] = b3;
+#pragma unroll
+            for (int k = 0; k < TR_BUF; k++)
+                if (nb > (unsigned)k && first + k < S.scap) reg[first + k] = buf[k];
             nb = 0;
         }
     }
     if (__any(nb > 0)) {
         const int64_t first = (int64_t)shard_append(S, sh, nb);
-        if (nb > 0 && first < S.scap) reg[first] = b0;
-        if (nb > 1 && first + 1 < S.scap) reg[first + 1] = b1;
-        if (nb > 2 && first + 2 < S.scap) reg[first + 2] = b2;
+#pragma unroll
+        for (int k = 0; k < TR_BUF; k++)
+            if (nb > (unsigned)k && first + k < S.scap) reg[first + k] = buf[k];
     }
 }
 
@@ -627,7 +627,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_sap_self(const double* sb, const u
                 j++;
             }
         }
-        const int64_t slot = (int64_t)shard_append(S, sh, hit >= 0 ? 1u : 0u);
+        unsigned pre;
+        const int64_t slot = (int64_t)shard_append_bits(S, sh, hit >= 0, false, pre) + pre;
         if (hit >= 0 && slot < S.scap) out[sh * S.scap + slot] = make_int2(ai, hit);
     }
 }
@@ -671,7 +672,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_sap_cross(const double* qa, const 
                 j++;
             }
         }
-        const int64_t slot = (int64_t)shard_append(S, sh, hit.x >= 0 ? 1u : 0u);
+        unsigned pre;
+        const int64_t slot = (int64_t)shard_append_bits(S, sh, hit.x >= 0, false, pre) + pre;
         if (hit.x >= 0 && slot < S.scap) out[sh * S.scap + slot] = hit;
     }
 }

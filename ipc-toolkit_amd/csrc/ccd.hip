@@ -171,7 +171,8 @@ __global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t*
         }
     }
     const int sh = wave_shard();
-    const int64_t slot = (int64_t)shard_append(S, sh, search ? 1u : 0u);
+    unsigned pre;
+    const int64_t slot = (int64_t)shard_append_bits(S, sh, search, false, pre) + pre;
     if (search && slot < S.scap) stage[sh * S.scap + slot] = (uint32_t)ci;
 }
 

@@ -128,7 +128,8 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const Shards Sk { S.cnt + (int64_t)k * SHARDS * SHARD_PAD, S.scap };
-        const int64_t slot = (int64_t)shard_append(Sk, sh, kind == k ? 1u : 0u);
+        unsigned pre;
+        const int64_t slot = (int64_t)shard_append_bits(Sk, sh, kind == k, false, pre) + pre;
         if (kind == k && slot < S.scap) {
             const int64_t at = sh * S.scap + slot;
             if (k == 0) S.vv[at] = key;
