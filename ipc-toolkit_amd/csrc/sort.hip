@@ -148,17 +148,26 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_hist(const uint64_t* keys, i
     counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one 8-bit digit.  Each wave ranks its keys per digit
+// with ballots; the block turns the per-wave counts into offsets, places
+// the tile's keys in LDS in digit order, and writes each digit's run to its
+// global range so that neighbouring lanes write neighbouring addresses.
 __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
     const uint64_t* keys, const uint32_t* vals, uint64_t* okeys, uint32_t* ovals, int64_t n,
     int shift, const uint32_t* offsets, int64_t ntiles)
 {
     __shared__ uint32_t wcnt[RS_WAVES][256];
     __shared__ uint32_t tile_off[256];
+    __shared__ uint32_t dstart[256]; // first slot of each digit in the tile
+    __shared__ uint32_t wtot[RS_WAVES];
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_BLOCK) (&wcnt[0][0])[i] = 0;
     tile_off[threadIdx.x] = offsets[(int64_t)threadIdx.x * ntiles + blockIdx.x];
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (RS_CHUNKS * WAVE);
+    const int64_t tile0 = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = tile0 + (int64_t)w * (RS_CHUNKS * WAVE);
     const unsigned long long lt = (1ull << lane) - 1ull;
     uint64_t k[RS_CHUNKS];
     uint32_t v[RS_CHUNKS];
@@ -184,6 +193,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
         rank[c] = before + r;
     }
     __syncthreads();
+    // per digit: wave counts -> offsets within the digit, digit total
+    uint32_t total;
     {
         const int d = threadIdx.x; // RS_BLOCK == 256 digits
         uint32_t run = 0;
@@ -193,19 +204,42 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
             wcnt[ww][d] = run;
             run += c;
         }
+        total = run;
     }
+    // exclusive scan of the digit totals: within each wave, then across
+    uint32_t x = total;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) wtot[w] = x;
     __syncthreads();
+    uint32_t wpre = 0;
+#pragma unroll
+    for (int ww = 0; ww < RS_WAVES; ww++)
+        if (ww < w) wpre += wtot[ww];
+    dstart[threadIdx.x] = wpre + x - total;
+    __syncthreads();
+    // keys in digit order in LDS
 #pragma unroll
     for (int c = 0; c < RS_CHUNKS; c++) {
         const int64_t i = base + c * WAVE + lane;
         if (i < n) {
             const uint32_t d = (uint32_t)(k[c] >> shift) & 0xFF;
-            const uint32_t pos = tile_off[d] + wcnt[w][d] + rank[c];
-            if ((int64_t)pos < n) { // always true for a consistent histogram
-                okeys[pos] = k[c];
-                ovals[pos] = v[c];
-            }
+            const uint32_t lp = dstart[d] + wcnt[w][d] + rank[c];
+            sk[lp] = k[c];
+            sv[lp] = v[c];
         }
+    }
+    __syncthreads();
+    const int tn = (int)(n - tile0 < RS_TILE ? n - tile0 : RS_TILE);
+    for (int j = threadIdx.x; j < tn; j += RS_BLOCK) {
+        const uint64_t key = sk[j];
+        const uint32_t d = (uint32_t)(key >> shift) & 0xFF;
+        const int64_t pos = (int64_t)tile_off[d] + (j - (int)dstart[d]);
+        okeys[pos] = key;
+        ovals[pos] = sv[j];
     }
 }
 
