@@ -224,29 +224,10 @@ struct CollRanges {
     int64_t goff[5]; // global collision index ranges per kind
 };
 
-// vertices of global collision g (unused slots repeat the first vertex)
-__device__ __forceinline__ int4 incidence_vertices(const CollView& C, const CollRanges& G,
-                                                   int64_t g)
-{
-    int v[4];
-    if (g < G.goff[1]) {
-        stencil_ids<K_VV>(C, g, v);
-        return make_int4(v[0], v[1], v[0], v[0]);
-    }
-    if (g < G.goff[2]) {
-        stencil_ids<K_EV>(C, g - G.goff[1], v);
-        return make_int4(v[0], v[1], v[2], v[0]);
-    }
-    if (g < G.goff[3]) stencil_ids<K_EE>(C, g - G.goff[2], v);
-    else stencil_ids<K_FV>(C, g - G.goff[3], v);
-    return make_int4(v[0], v[1], v[2], v[3]);
-}
-
 constexpr int PH = 1024; // hash slots per wave
 constexpr int PU = 512;  // most distinct columns of a row on this path
 
-__global__ __launch_bounds__(256) void k_pattern_rows(CollView C, CollRanges G,
-                                                      const uint32_t* vptr, const uint32_t* inc,
+__global__ __launch_bounds__(256) void k_pattern_rows(const uint32_t* vptr, const int4* inc_vtx,
                                                       int nV, int32_t* stage, uint32_t* cnt,
                                                       int* overflow)
 {
@@ -265,10 +246,11 @@ __global__ __launch_bounds__(256) void k_pattern_rows(CollView C, CollRanges G,
     if (l == 0) nu[wv] = 0;
     wave_sync();
     for (uint32_t j = i0 + l; j < i1; j += WAVE) {
-        const int4 vs = incidence_vertices(C, G, (int64_t)(inc[j] >> 2));
+        const int4 vs = inc_vtx[j];
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int key = b == 0 ? vs.x : b == 1 ? vs.y : b == 2 ? vs.z : vs.w;
+            if (key < 0) continue;
             unsigned h = ((unsigned)key * 2654435761u) >> (32 - 10);
             for (int probe = 0; probe < PH; probe++) {
                 const int old = atomicCAS(&hs[wv][h], -1, key);
@@ -371,11 +353,20 @@ __global__ __launch_bounds__(256) void k_gradient_rows(CollRanges G, GradRecs R,
 // (3) one wave per vertex owns its block row: the lanes walk the vertex's
 //     incidences, expand each record to the row's 3x3 blocks and add them
 //     into the row's accumulators in LDS, then the wave writes the row.
+// Every record holds the projected Hessian in the 4-point Q basis (packed
+// 9 x 9, REC_LEN values), so one code path expands all kinds.  A stencil of
+// NP < 4 points is projected in its own NP-point basis and mapped exactly:
+// with T[k][m] = sum_{x<NP} q4(x,k) qNP(x,m), the 4-point matrix
+// T B T^T expands to the same point blocks (Q4 Q4^T = I - 11^T/4 and the
+// columns of Q_NP sum to zero) and to zero blocks for the unused points.
+constexpr int REC_LEN = 45;
+
 template <int KIND>
 __global__ __launch_bounds__(64) void k_hessian_records(CollView C, const double* X, int nV,
                                                         BarrierParams bp, double* rec)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
+    constexpr int K = NP - 1;
     constexpr int LEN = HqTraits<KIND>::LEN;
     const int64_t i = (int64_t)xcd_block() * 64 + threadIdx.x;
     if (i >= C.n[KIND]) return;
@@ -387,9 +378,35 @@ __global__ __launch_bounds__(64) void k_hessian_records(CollView C, const double
     double H[LEN];
     stencil_hessian_q<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
                             KIND == K_EE ? C.eps[i] : 0.0, bp, H);
-    double* o = rec + (int64_t)LEN * i;
+    double* o = rec + (int64_t)REC_LEN * i;
+    if constexpr (NP == 4) {
 #pragma unroll
-    for (int e = 0; e < LEN; e++) o[e] = H[e];
+        for (int e = 0; e < LEN; e++) o[e] = H[e];
+    } else {
+        double T[3][K];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int m = 0; m < K; m++) {
+                double t = 0;
+#pragma unroll
+                for (int x = 0; x < NP; x++) t += QBasis<4>::q(x, k) * QBasis<NP>::q(x, m);
+                T[k][m] = t;
+            }
+#pragma unroll
+        for (int p = 0; p < 9; p++)
+#pragma unroll
+            for (int q = 0; q <= p; q++) {
+                const int k = p / 3, r = p % 3, l = q / 3, t = q % 3;
+                double acc = 0;
+#pragma unroll
+                for (int m = 0; m < K; m++)
+#pragma unroll
+                    for (int n = 0; n < K; n++)
+                        acc += T[k][m] * T[l][n] * H[sp(3 * m + r, 3 * n + t)];
+                o[sp(p, q)] = acc;
+            }
+    }
 }
 
 // (vertex, global collision g, point a): key vertex, value g << 2 | a
@@ -420,71 +437,86 @@ __global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M
     for (int64_t v = prev + 1; v <= cur; v++) vptr[v] = (uint32_t)i;
 }
 
-struct RecView {
-    const double* rec[4];
-    int64_t goff[5]; // global collision index ranges per kind
-};
+// stencil vertices of every sorted incidence (-1 for unused points)
+__global__ __launch_bounds__(256) void k_inc_vertices(CollView C, CollRanges G,
+                                                      const uint32_t* inc, int64_t M, int4* out)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= M) return;
+    const int64_t g = inc[j] >> 2;
+    int v[4] = { -1, -1, -1, -1 };
+    if (g < G.goff[1]) stencil_ids<K_VV>(C, g, v);
+    else if (g < G.goff[2]) stencil_ids<K_EV>(C, g - G.goff[1], v);
+    else if (g < G.goff[3]) stencil_ids<K_EE>(C, g - G.goff[2], v);
+    else stencil_ids<K_FV>(C, g - G.goff[3], v);
+    out[j] = make_int4(v[0], v[1], v[2], v[3]);
+}
 
 constexpr int RW = 64; // block columns per accumulation window (one per lane)
 
-// Adds the contributions of collision i (point a is the row vertex) to the
-// window of the row's blocks whose columns are cols[0, RW) (sorted, padded
-// with INT32_MAX).  The record stays in memory (read through the cache)
-// so the kernel keeps few registers.
-template <int KIND>
-__device__ __forceinline__ void row_accumulate(const CollView& C, const RecView& R, int64_t i,
-                                               int a, const int32_t* cols, double (*acc)[9],
+// Adds the contributions of one collision (record h in the 4-point basis,
+// stencil vertices vs, the row vertex is point a) to the window of the row's
+// blocks whose columns are cols[0, RW) (sorted, padded with INT32_MAX).  The
+// record stays in memory (read through the cache) and is expanded one block
+// row r at a time, so few values are live at once.
+__device__ __forceinline__ void row_accumulate(const double* h, int a, int4 vs,
+                                               const int32_t* cols, double (*acc)[9],
                                                uint32_t* msk)
 {
-    constexpr int NP = StencilTraits<KIND>::NP;
-    constexpr int K = NP - 1;
-    constexpr int LEN = HqTraits<KIND>::LEN;
-    int v[4];
-    stencil_ids<KIND>(C, i, v);
-    const double* h = R.rec[KIND] + (int64_t)LEN * i;
-    // G[l][r][s] = sum_k q(a,k) H[3k+r][3l+s]
-    double G[K][3][3];
+    constexpr int K = 3;
+    // window slot of each stencil point's column (-1: unused point or outside
+    // this window)
+    int slot[4];
 #pragma unroll
-    for (int l = 0; l < K; l++)
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                double x = 0;
-#pragma unroll
-                for (int k = 0; k < K; k++) x += QBasis<NP>::q(a, k) * h[sp(3 * k + r, 3 * l + t)];
-                G[l][r][t] = x;
-            }
-#pragma unroll
-    for (int b = 0; b < NP; b++) {
-        const int col = v[b];
-        if (col < cols[0] || col > cols[RW - 1]) continue; // outside this window
+    for (int b = 0; b < 4; b++) {
+        const int col = b == 0 ? vs.x : b == 1 ? vs.y : b == 2 ? vs.z : vs.w;
+        slot[b] = -1;
+        if (col < 0 || col < cols[0] || col > cols[RW - 1]) continue;
         int lo = 0, hi = RW;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (cols[mid] < col) lo = mid + 1;
             else hi = mid;
         }
-        if (lo >= RW || cols[lo] != col) continue;
-        uint32_t mask = 0;
+        if (lo < RW && cols[lo] == col) slot[b] = lo;
+    }
+    uint32_t mask[4] = { 0, 0, 0, 0 };
 #pragma unroll
-        for (int r = 0; r < 3; r++)
+    for (int r = 0; r < 3; r++) {
+        // G[l][t] = sum_k q(a,k) H[3k+r][3l+t]
+        double G[K][3];
+#pragma unroll
+        for (int l = 0; l < K; l++)
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 double x = 0;
 #pragma unroll
-                for (int l = 0; l < K; l++) x += QBasis<NP>::q(b, l) * G[l][r][t];
+                for (int k = 0; k < K; k++) x += QBasis<4>::q(a, k) * h[sp(3 * k + r, 3 * l + t)];
+                G[l][t] = x;
+            }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            if (slot[b] < 0) continue;
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                double x = 0;
+#pragma unroll
+                for (int l = 0; l < K; l++) x += QBasis<4>::q(b, l) * G[l][t];
                 if (x != 0) { // exact zeros are not triplets (local_to_global.hpp:291)
-                    atomicAdd(&acc[lo][3 * r + t], x);
-                    mask |= 1u << (3 * r + t);
+                    atomicAdd(&acc[slot[b]][3 * r + t], x);
+                    mask[b] |= 1u << (3 * r + t);
                 }
             }
-        if (mask) atomicOr(&msk[lo], mask);
+        }
     }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        if (mask[b]) atomicOr(&msk[slot[b]], mask[b]);
 }
 
-__global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, const uint32_t* vptr,
-                                                      const uint32_t* inc, const uint32_t* bptr,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_hessian_rows(const double* rec, const uint32_t* vptr,
+                                                      const uint32_t* inc, const int4* inc_vtx,
+                                                      const uint32_t* bptr,
                                                       const int32_t* bcol, int nV, double* bval,
                                                       uint32_t* bmask)
 {
@@ -506,12 +538,8 @@ __global__ __launch_bounds__(256) void k_hessian_rows(CollView C, RecView R, con
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (uint32_t j = i0 + l; j < i1; j += WAVE) {
             const uint32_t x = inc[j];
-            const int64_t g = x >> 2;
-            const int a = (int)(x & 3u);
-            if (g < R.goff[1]) row_accumulate<K_VV>(C, R, g, a, cols[wv], acc[wv], msk[wv]);
-            else if (g < R.goff[2]) row_accumulate<K_EV>(C, R, g - R.goff[1], a, cols[wv], acc[wv], msk[wv]);
-            else if (g < R.goff[3]) row_accumulate<K_EE>(C, R, g - R.goff[2], a, cols[wv], acc[wv], msk[wv]);
-            else row_accumulate<K_FV>(C, R, g - R.goff[3], a, cols[wv], acc[wv], msk[wv]);
+            row_accumulate(rec + (int64_t)REC_LEN * (x >> 2), (int)(x & 3u), inc_vtx[j], cols[wv],
+                           acc[wv], msk[wv]);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -701,6 +729,9 @@ void require_incidences(Ctx& c, const CollView& C, const CollRanges& G, int64_t 
                      c.blk_tmp.p, s);
     hipLaunchKernelGGL(k_run_ptr, dim3(grid_for(M + 1, 256, INT32_MAX)), dim3(256), 0, s, keys, M,
                        c.nV, c.inc_ptr.as<uint32_t>());
+    c.inc_vtx.reserve(sizeof(int4) * (size_t)mk);
+    hipLaunchKernelGGL(k_inc_vertices, dim3(grid_for(M, 256, INT32_MAX)), dim3(256), 0, s, C, G,
+                       vals, M, c.inc_vtx.as<int4>());
     IPCG_CHECK_LAUNCH();
     c.inc_valid = true;
     c.stage_end();
@@ -806,13 +837,12 @@ void hess_pattern(Ctx& c)
     c.pattern_valid = true;
 }
 
-// Block pattern from the vertex incidences (inc_ptr / inc_vals valid).
+// Block pattern from the vertex incidences (inc_ptr / inc_vtx valid).
 // Returns false, leaving the pattern invalid, when a row has more distinct
 // columns than the LDS path holds.
-bool hess_pattern_incidences(Ctx& c, const CollRanges& G, int64_t M)
+bool hess_pattern_incidences(Ctx& c, int64_t M)
 {
     hipStream_t s = c.stream;
-    CollView C = view(c);
     const size_t nv1 = (size_t)c.nV + 1;
     c.blk_keys.reserve(sizeof(int32_t) * 4 * (size_t)std::max<int64_t>(M, 1));
     c.hp_mark.reserve(sizeof(uint32_t) * nv1 * 3);
@@ -824,8 +854,8 @@ bool hess_pattern_incidences(Ctx& c, const CollRanges& G, int64_t M)
     int* ovf = reinterpret_cast<int*>(c.counters.as<unsigned long long>() + 10);
     IPCG_HIP(hipMemsetAsync(ovf, 0, sizeof(int), s));
     const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
-    hipLaunchKernelGGL(k_pattern_rows, dim3(gw), dim3(256), 0, s, C, G, c.inc_ptr.as<uint32_t>(),
-                       c.inc_vals.as<uint32_t>(), c.nV, stage, cnt, ovf);
+    hipLaunchKernelGGL(k_pattern_rows, dim3(gw), dim3(256), 0, s, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vtx.as<int4>(), c.nV, stage, cnt, ovf);
     exclusive_scan_u32(cnt, bptr, c.nV, bptr + c.nV, c.blk_tmp.p, s);
     IPCG_CHECK_LAUNCH();
     uint32_t* h = static_cast<uint32_t*>(c.pinned);
@@ -850,19 +880,13 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     CollView C = view(c);
     BarrierParams bp { dhat, kappa, c.coll_dmin, psd };
     const double* X = c.X[0].as<double>();
-    const int lenk[4] = { HqTraits<K_VV>::LEN, HqTraits<K_EV>::LEN, HqTraits<K_EE>::LEN,
-                          HqTraits<K_FV>::LEN };
     int64_t M;
     const CollRanges G = ranges(C, M);
     require_incidences(c, C, G, M);
-    RecView R;
-    int64_t nrec = 0;
-    for (int k = 0; k < 5; k++) R.goff[k] = G.goff[k];
-    for (int k = 0; k < 4; k++) nrec += (int64_t)lenk[k] * C.n[k];
     // block pattern: from the incidences, or by sorting the stencil pairs
     // when a row outgrows the LDS path
     c.stage_begin("hessian_pattern");
-    if (!c.pattern_valid && !hess_pattern_incidences(c, G, M)) hess_pattern(c);
+    if (!c.pattern_valid && !hess_pattern_incidences(c, M)) hess_pattern(c);
     c.stage_end();
     const int64_t nb = std::max<int64_t>(c.nblocks, 1);
     c.blk_mask.reserve(sizeof(uint32_t) * nb);
@@ -871,40 +895,37 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     double* bval = c.hq.as<double>();
     const uint32_t* bptr = c.hp_bptr.as<uint32_t>();
     const int32_t* bcol = c.hp_bcol.as<int32_t>();
-    // projected Hessian records
-    c.hrec.reserve(sizeof(double) * (size_t)std::max<int64_t>(nrec, 1));
+    // projected Hessian records, REC_LEN values per collision in global
+    // collision order
+    c.hrec.reserve(sizeof(double) * REC_LEN * (size_t)std::max<int64_t>(G.goff[4], 1));
     double* rec = c.hrec.as<double>();
-    int64_t ro = 0;
-    for (int k = 0; k < 4; k++) {
-        R.rec[k] = rec + ro;
-        ro += (int64_t)lenk[k] * C.n[k];
-    }
     // one launch per stencil kind, each its own timed stage
     if (C.n[0]) {
         c.stage_begin("hessian_records.vv");
-        hipLaunchKernelGGL(k_hessian_records<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[0]));
+        hipLaunchKernelGGL(k_hessian_records<K_VV>, dim3(grid_for(C.n[0], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[0]);
         c.stage_end();
     }
     if (C.n[1]) {
         c.stage_begin("hessian_records.ev");
-        hipLaunchKernelGGL(k_hessian_records<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[1]));
+        hipLaunchKernelGGL(k_hessian_records<K_EV>, dim3(grid_for(C.n[1], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[1]);
         c.stage_end();
     }
     if (C.n[2]) {
         c.stage_begin("hessian_records.ee");
-        hipLaunchKernelGGL(k_hessian_records<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[2]));
+        hipLaunchKernelGGL(k_hessian_records<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[2]);
         c.stage_end();
     }
     if (C.n[3]) {
         c.stage_begin("hessian_records.fv");
-        hipLaunchKernelGGL(k_hessian_records<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
+        hipLaunchKernelGGL(k_hessian_records<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[3]);
         c.stage_end();
     }
     IPCG_CHECK_LAUNCH();
     c.stage_begin("hessian_rows");
     const unsigned gwr = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
-    hipLaunchKernelGGL(k_hessian_rows, dim3(gwr), dim3(256), 0, s, C, R, c.inc_ptr.as<uint32_t>(),
-                       c.inc_vals.as<uint32_t>(), bptr, bcol, c.nV, bval, bmask);
+    hipLaunchKernelGGL(k_hessian_rows, dim3(gwr), dim3(256), 0, s, rec, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vals.as<uint32_t>(), c.inc_vtx.as<int4>(), bptr, bcol, c.nV, bval,
+                       bmask);
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 
