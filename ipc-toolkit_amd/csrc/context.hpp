@@ -87,6 +87,7 @@ struct Ctx {
     DevBuf blk_mask, row_cnt, rowptr, col, val;
     DevBuf hp_mark, hp_bptr, hp_bcol; // block pattern of the collision set
     DevBuf inc_keys, inc_keys2, inc_vals, inc_vals2, inc_ptr; // vertex -> (collision, point)
+    DevBuf ee_other;                  // edge-edge collisions of non-interior types (list)
     DevBuf inc_vtx;                   // stencil vertices per sorted incidence (int4)
     DevBuf hrec;                      // projected Hessian per collision (Q basis)
     bool inc_valid = false;

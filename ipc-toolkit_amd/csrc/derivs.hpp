@@ -366,4 +366,197 @@ IPCG_HD void project_psd(double* A /* packed N */, int method)
         }
 }
 
+// ---------------------------------------------------------------------------
+// PSD projection of a 9 x 9 matrix of rank at most KR (contact stencils of
+// the point-triangle and edge-edge interior types have rank 5 in the
+// translation-free basis: 3 positive and 2 negative eigenvalues).
+//
+// Lanczos from b = A w (w fixed, b in the range of A) with full
+// reorthogonalization builds an orthonormal basis Q of the Krylov space
+// until the next vector vanishes; the eigen-pairs of T = Q^T A Q (at most
+// KR x KR, cyclic Jacobi) give A = (Q V) diag(lambda) (Q V)^T.  The result
+// is accepted only if the residual |A - Q T Q^T|_F is at most 1e-12 |A|_F
+// (the Krylov space holds the whole range of A); otherwise A is left
+// untouched and false is returned (the caller runs project_psd<9>).
+// ---------------------------------------------------------------------------
+template <int KR>
+IPCG_HD bool project_psd_krylov9(double* A /* packed 9 */, int method)
+{
+#pragma clang fp contract(fast)
+    constexpr int N = 9;
+    if (method == 0) return true;
+    double nA2 = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) nA2 += (i == j ? 1.0 : 2.0) * A[sp(i, j)] * A[sp(i, j)];
+    if (!(nA2 > 0)) return true; // the zero matrix is its own projection
+    if (!(nA2 < 1e300)) return false;
+    auto matvec = [&](const double* x, double* y) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double acc = 0;
+#pragma unroll
+            for (int j = 0; j < N; j++) acc += A[sp(i, j)] * x[j];
+            y[i] = acc;
+        }
+    };
+    constexpr double w[N] = { 0.3719, -0.6541, 0.2333, 0.8123, -0.1427,
+                              0.5021, -0.7433, 0.0679, 0.4517 };
+    double Q[KR][N], al[KR], be[KR];
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+        al[k] = be[k] = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) Q[k][i] = 0;
+    }
+    double u[N];
+    matvec(w, u);
+    const double tiny = 1e-13 * sqrt(nA2);
+    double nb = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) nb += u[i] * u[i];
+    nb = sqrt(nb);
+    if (!(nb > tiny)) return false;
+#pragma unroll
+    for (int i = 0; i < N; i++) Q[0][i] = u[i] / nb;
+    int m = 0; // Krylov dimension
+    bool closed = false;
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+        if (!closed) {
+            matvec(Q[k], u);
+            double a = 0;
+#pragma unroll
+            for (int i = 0; i < N; i++) a += Q[k][i] * u[i];
+            al[k] = a;
+            // full reorthogonalization, twice
+#pragma unroll
+            for (int pass = 0; pass < 2; pass++)
+#pragma unroll
+                for (int j = 0; j <= k; j++) {
+                    double d = 0;
+#pragma unroll
+                    for (int i = 0; i < N; i++) d += Q[j][i] * u[i];
+#pragma unroll
+                    for (int i = 0; i < N; i++) u[i] -= d * Q[j][i];
+                }
+            double b = 0;
+#pragma unroll
+            for (int i = 0; i < N; i++) b += u[i] * u[i];
+            b = sqrt(b);
+            m = k + 1;
+            if (b <= tiny) {
+                closed = true;
+            } else if (k + 1 < KR) {
+                be[k] = b;
+#pragma unroll
+                for (int i = 0; i < N; i++) Q[k + 1][i] = u[i] / b;
+            }
+        }
+    }
+    if (!closed) return false;
+    // T = Q^T A Q in the basis (tridiagonal up to rounding: use the exact
+    // projection so the residual test sees the same matrix)
+    double T[KR * (KR + 1) / 2];
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+        matvec(Q[k], u);
+#pragma unroll
+        for (int l = 0; l <= k; l++) {
+            double d = 0;
+#pragma unroll
+            for (int i = 0; i < N; i++) d += Q[l][i] * u[i];
+            T[sp(k, l)] = d;
+        }
+    }
+    (void)m;
+    // residual |A - Q T Q^T|_F
+    double r2 = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) {
+            double x = A[sp(i, j)];
+#pragma unroll
+            for (int k = 0; k < KR; k++)
+#pragma unroll
+                for (int l = 0; l < KR; l++) x -= Q[k][i] * T[sp(k, l)] * Q[l][j];
+            r2 += (i == j ? 1.0 : 2.0) * x * x;
+        }
+    if (!(r2 <= 1e-24 * nA2)) return false;
+    // eigen-pairs of T (unused dimensions are zero rows: zero eigenvalues)
+    double V[KR][KR];
+#pragma unroll
+    for (int i = 0; i < KR; i++)
+#pragma unroll
+        for (int j = 0; j < KR; j++) V[i][j] = i == j ? 1.0 : 0.0;
+#pragma clang loop unroll(disable)
+    for (int sweep = 0; sweep < 16; sweep++) {
+        double off = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < KR; i++) {
+            tot += T[sp(i, i)] * T[sp(i, i)];
+#pragma unroll
+            for (int j = 0; j < i; j++) off += T[sp(i, j)] * T[sp(i, j)];
+        }
+        tot += 2 * off;
+        if (!(off > 1e-30 * tot)) break;
+#pragma unroll
+        for (int p = 0; p < KR; p++)
+#pragma unroll
+            for (int q = p + 1; q < KR; q++) {
+                const double apq = T[sp(p, q)];
+                const double app = T[sp(p, p)], aqq = T[sp(q, q)];
+                const double theta = (aqq - app) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                t = apq == 0.0 ? 0.0 : t;
+                const double c = rsqrt_nr(t * t + 1), s = t * c;
+#pragma unroll
+                for (int k = 0; k < KR; k++) {
+                    if (k == p || k == q) continue;
+                    const double akp = T[sp(k, p)], akq = T[sp(k, q)];
+                    T[sp(k, p)] = c * akp - s * akq;
+                    T[sp(k, q)] = s * akp + c * akq;
+                }
+                T[sp(p, p)] = app - t * apq;
+                T[sp(q, q)] = aqq + t * apq;
+                T[sp(p, q)] = 0.0;
+#pragma unroll
+                for (int k = 0; k < KR; k++) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    // W = Q^T V (N x KR columns), A = W diag(lambda+) W^T
+    double lam[KR];
+#pragma unroll
+    for (int c = 0; c < KR; c++) {
+        const double l = T[sp(c, c)];
+        lam[c] = l >= 0 ? l : (method == 1 ? 0.0 : -l);
+    }
+    double W[N][KR];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int c = 0; c < KR; c++) {
+            double x = 0;
+#pragma unroll
+            for (int k = 0; k < KR; k++) x += Q[k][i] * V[k][c];
+            W[i][c] = x;
+        }
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) {
+            double acc = 0;
+#pragma unroll
+            for (int c = 0; c < KR; c++) acc += W[i][c] * lam[c] * W[j][c];
+            A[sp(i, j)] = acc;
+        }
+    return true;
+}
+
 } // namespace ipcg

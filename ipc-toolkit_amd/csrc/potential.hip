@@ -361,23 +361,46 @@ __global__ __launch_bounds__(256) void k_gradient_rows(CollRanges G, GradRecs R,
 // columns of Q_NP sum to zero) and to zero blocks for the unused points.
 constexpr int REC_LEN = 45;
 
-template <int KIND>
+// PATH (4-point stencils): REC_ALL projects every collision with the 9 x 9
+// Jacobi; REC_LOW projects by the rank-5 Krylov path (falling back to the
+// Jacobi for a matrix it does not capture) and, for edge-edge, takes only
+// the interior (EA_EB) type, whose Hessians have rank 5; REC_LIST takes the
+// collisions listed in `only` (the other edge-edge types) with the Jacobi.
+enum { REC_ALL = 0, REC_LOW = 1, REC_LIST = 2 };
+
+template <int KIND, int PATH = REC_ALL>
 __global__ __launch_bounds__(64) void k_hessian_records(CollView C, const double* X, int nV,
-                                                        BarrierParams bp, double* rec)
+                                                        BarrierParams bp, double* rec,
+                                                        const uint32_t* only = nullptr,
+                                                        int64_t n_only = 0)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
     constexpr int K = NP - 1;
     constexpr int LEN = HqTraits<KIND>::LEN;
-    const int64_t i = (int64_t)xcd_block() * 64 + threadIdx.x;
-    if (i >= C.n[KIND]) return;
+    const int64_t t = (int64_t)xcd_block() * 64 + threadIdx.x;
+    int64_t i = t;
+    if constexpr (PATH == REC_LIST) {
+        if (t >= n_only) return;
+        i = only[t];
+    } else {
+        if (i >= C.n[KIND]) return;
+    }
+    if constexpr (PATH == REC_LOW && KIND == K_EE)
+        if (C.dtype[i] != EE_EA_EB) return;
     int v[4];
     stencil_ids<KIND>(C, i, v);
     d3 P[4];
 #pragma unroll
     for (int j = 0; j < NP; j++) P[j] = ldx(X, nV, v[j]);
     double H[LEN];
-    stencil_hessian_q<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
-                            KIND == K_EE ? C.eps[i] : 0.0, bp, H);
+    if constexpr (PATH == REC_LOW && NP == 4) {
+        const bool psd = stencil_hessian_q<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
+                                                 KIND == K_EE ? C.eps[i] : 0.0, bp, H, true);
+        if (psd && !project_psd_krylov9<6>(H, bp.psd)) project_psd<9>(H, bp.psd);
+    } else {
+        stencil_hessian_q<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
+                                KIND == K_EE ? C.eps[i] : 0.0, bp, H);
+    }
     double* o = rec + (int64_t)REC_LEN * i;
     if constexpr (NP == 4) {
 #pragma unroll
@@ -435,6 +458,22 @@ __global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M
     const int64_t prev = i > 0 ? (int64_t)keys[i - 1] : -1;
     const int64_t cur = i < M ? (int64_t)keys[i] : (int64_t)nV;
     for (int64_t v = prev + 1; v <= cur; v++) vptr[v] = (uint32_t)i;
+}
+
+// edge-edge collisions of the other (non-interior) types, whose Hessians
+// have higher rank: flags, then the list at the scanned positions
+__global__ __launch_bounds__(256) void k_ee_other_flags(const int32_t* dtype, int64_t n,
+                                                        uint32_t* flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) flag[i] = dtype[i] != EE_EA_EB ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_ee_other_list(const uint32_t* flag, const uint32_t* pos,
+                                                       int64_t n, uint32_t* list)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && flag[i]) list[pos[i]] = (uint32_t)i;
 }
 
 // stencil vertices of every sorted incidence (-1 for unused points)
@@ -911,13 +950,35 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
         c.stage_end();
     }
     if (C.n[2]) {
+        // interior edge-edge Hessians by the rank-5 path, the other types
+        // (listed by a scan of their flags) by the 9 x 9 Jacobi
+        const int64_t ne = C.n[2];
+        c.ee_other.reserve(sizeof(uint32_t) * 3 * (size_t)ne);
+        uint32_t* flag = c.ee_other.as<uint32_t>();
+        uint32_t* pos = flag + ne;
+        uint32_t* list = pos + ne;
+        c.blk_tmp.reserve(scan_tmp_bytes(ne));
+        hipLaunchKernelGGL(k_ee_other_flags, dim3(grid_for(ne, 256, INT32_MAX)), dim3(256), 0, s, C.dtype, ne, flag);
+        uint32_t* n_other = reinterpret_cast<uint32_t*>(c.counters.as<unsigned long long>() + 11);
+        exclusive_scan_u32(flag, pos, ne, n_other, c.blk_tmp.p, s);
+        hipLaunchKernelGGL(k_ee_other_list, dim3(grid_for(ne, 256, INT32_MAX)), dim3(256), 0, s, flag, pos, ne, list);
+        IPCG_CHECK_LAUNCH();
+        uint32_t* h = static_cast<uint32_t*>(c.pinned);
+        IPCG_HIP(hipMemcpyAsync(h, n_other, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        const int64_t no = h[0];
         c.stage_begin("hessian_records.ee");
-        hipLaunchKernelGGL(k_hessian_records<K_EE>, dim3(grid_for(C.n[2], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[2]);
+        hipLaunchKernelGGL((k_hessian_records<K_EE, REC_LOW>), dim3(grid_for(ne, 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[2], nullptr, (int64_t)0);
         c.stage_end();
+        if (no > 0) {
+            c.stage_begin("hessian_records.ee_other");
+            hipLaunchKernelGGL((k_hessian_records<K_EE, REC_LIST>), dim3(grid_for(no, 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[2], list, no);
+            c.stage_end();
+        }
     }
     if (C.n[3]) {
         c.stage_begin("hessian_records.fv");
-        hipLaunchKernelGGL(k_hessian_records<K_FV>, dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[3]);
+        hipLaunchKernelGGL((k_hessian_records<K_FV, REC_LOW>), dim3(grid_for(C.n[3], 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[3], nullptr, (int64_t)0);
         c.stage_end();
     }
     IPCG_CHECK_LAUNCH();

@@ -170,9 +170,11 @@ IPCG_HD void stencil_gradient(int dtype, const d3* P, double w, double eps_x,
 }
 
 // PSD-projected energy Hessian in the Q basis (packed 3(NP-1) square).
+// defer_psd: leave the projection to the caller; the return value says
+// whether the reference projects this stencil's Hessian at all.
 template <int KIND, int NP = StencilTraits<KIND>::NP>
-IPCG_HD void stencil_hessian_q(int dtype, const d3* P, double w, double eps_x,
-                               const BarrierParams& bp, double* Hq)
+IPCG_HD bool stencil_hessian_q(int dtype, const d3* P, double w, double eps_x,
+                               const BarrierParams& bp, double* Hq, bool defer_psd = false)
 {
     constexpr int kind = KIND;
     constexpr int K = NP - 1, N = 3 * K;
@@ -190,7 +192,7 @@ IPCG_HD void stencil_hessian_q(int dtype, const d3* P, double w, double eps_x,
             to_qspace<NP>(X, mpl, mmi, w * f * M.m1, gx, Hq);
             add_sym_outer<K>(Hq, 0.5 * w * f * M.m2, gx, gx);
         }
-        return;
+        return false;
     }
     const double f1 = bar_f1(d2, bp), f2 = bar_f2(d2, bp);
     d3 gq[K];
@@ -209,7 +211,8 @@ IPCG_HD void stencil_hessian_q(int dtype, const d3* P, double w, double eps_x,
             add_sym_outer<K>(Hq, w * f1 * M.m1, gq, gx);
         }
     }
-    project_psd<N>(Hq, bp.psd);
+    if (!defer_psd) project_psd<N>(Hq, bp.psd);
+    return true;
 }
 
 } // namespace ipcg
