@@ -192,7 +192,9 @@ int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable);
 /* work counts of the last calls, by name: "ccd.candidates" (swept
  * candidates of the last CCD), "ccd.searched" (those whose root box
  * includes the origin), "ccd.spilled" (searches that outgrew the LDS
- * lists), "ccd.wavefront" (searches that outgrew a lane's HBM slab) */
+ * lists), "ccd.wavefront" (searches that outgrew a lane's HBM slab),
+ * "hessian.ee_other" (edge-edge collisions of the last Hessian whose type
+ * is not the interior one; their projection takes the 9 x 9 path) */
 int ipcg_counter(ipcg_ctx* ctx, const char* name, int64_t* value);
 
 #ifdef __cplusplus

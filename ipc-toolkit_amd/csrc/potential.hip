@@ -967,6 +967,7 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
         IPCG_HIP(hipMemcpyAsync(h, n_other, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));
         const int64_t no = h[0];
+        c.stat["hessian.ee_other"] = no;
         c.stage_begin("hessian_records.ee");
         hipLaunchKernelGGL((k_hessian_records<K_EE, REC_LOW>), dim3(grid_for(ne, 64, INT32_MAX)), dim3(64), 0, s, C, X, c.nV, bp, rec + REC_LEN * G.goff[2], nullptr, (int64_t)0);
         c.stage_end();
