@@ -411,7 +411,9 @@ __device__ __forceinline__ int2 leaf_pair(const Filter& flt, const Query& q, con
     }
 }
 
-template <int MODE>
+// SWEPT only names the launches of the swept (CCD) broad phase apart from
+// the static ones in kernel profiles; the code is the same.
+template <int MODE, int SWEPT>
 __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
     const int4* leaf, const float* tbox, int nleaves, Filter flt, const int32_t* qmap, int2* out,
@@ -936,7 +938,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         } else {
             c.stage_begin("bp_traverse_ee");
             if (c.nE >= 2)
-                hipLaunchKernelGGL(k_traverse<TR_EE>, dim3(grid_for(c.nE, TR_BLOCK)),
+                hipLaunchKernelGGL((swept ? k_traverse<TR_EE, 1> : k_traverse<TR_EE, 0>), dim3(grid_for(c.nE, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.te.box.as<float>(),
                                    c.te.ids.as<uint32_t>(), c.nE, c.te.nodes.as<Node>(),
                                    c.te.root.as<int32_t>(), c.te.leaf.as<int4>(),
@@ -945,7 +947,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
             c.stage_end();
             c.stage_begin("bp_traverse_fv");
             if (c.nF >= 1)
-                hipLaunchKernelGGL(k_traverse<TR_FV>, dim3(grid_for(c.nV, TR_BLOCK)),
+                hipLaunchKernelGGL((swept ? k_traverse<TR_FV, 1> : k_traverse<TR_FV, 0>), dim3(grid_for(c.nV, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tv.box.as<float>(),
                                    c.tv.ids.as<uint32_t>(), c.nV, c.tf.nodes.as<Node>(),
                                    c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
@@ -957,14 +959,14 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
             c.stage_begin("bp_traverse_codim");
             const Shards S2 { cnt[2], c.scap_vv }, S3 { cnt[3], c.scap_ev };
             if (c.n_cv >= 2)
-                hipLaunchKernelGGL(k_traverse<TR_VV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
+                hipLaunchKernelGGL((swept ? k_traverse<TR_VV, 1> : k_traverse<TR_VV, 0>), dim3(grid_for(c.n_cv, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
                                    c.tcv.ids.as<uint32_t>(), c.n_cv, c.tcv.nodes.as<Node>(),
                                    c.tcv.root.as<int32_t>(), c.tcv.leaf.as<int4>(),
                                    c.tcv.box.as<float>(), c.n_cv, flt, c.cv.as<int32_t>(),
                                    c.cand_stage[2].as<int2>(), S2, deep, status);
             if (c.n_ce >= 1)
-                hipLaunchKernelGGL(k_traverse<TR_EV>, dim3(grid_for(c.n_cv, TR_BLOCK)),
+                hipLaunchKernelGGL((swept ? k_traverse<TR_EV, 1> : k_traverse<TR_EV, 0>), dim3(grid_for(c.n_cv, TR_BLOCK)),
                                    dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
                                    c.tcv.ids.as<uint32_t>(), c.n_cv, c.tce.nodes.as<Node>(),
                                    c.tce.root.as<int32_t>(), c.tce.leaf.as<int4>(),

@@ -31,7 +31,7 @@ namespace ipcg {
 namespace {
 
 constexpr int CB = 64;
-constexpr int LDS_CAP = 8;
+constexpr int LDS_CAP = 6;
 constexpr int LANE_CAP = 256;
 constexpr int LANES = 131072; // 8 waves per CU at this kernel's register use
 constexpr int WF_BATCH = 4096;
