@@ -246,6 +246,22 @@ def main():
                           for k in sorted(known, key=per_step.get, reverse=True)[1:6]
                           for r in [roof(k)]}
 
+    # host transfers (outside `value`): both position slots up, gradient down
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.upload_positions(0, sc.V)
+        ctx.upload_positions(1, sc.V1)
+    ctx.synchronize()
+    up_ms = (time.perf_counter() - t0) / 3 * 1e3
+    ctx.build_collisions(sc.dhat, sc.dmin)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.gradient(sc.dhat, sc.kappa, download=True)
+    grad_ms = (time.perf_counter() - t0) / 3 * 1e3
+    host_transfer = {"positions_upload_ms": round(up_ms, 3),
+                     "gradient_with_download_ms": round(grad_ms, 3),
+                     "gradient_device_ms": round(per_step.get("gradient", 0.0), 3)}
+
     cpu = None
     if args.cpu_tile > 0:
         try:
@@ -275,7 +291,8 @@ def main():
                    "hessian_nnz": int(nnz), "step_size": step, "ccd_work": ccd_work,
                    "stage_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
                    "bytes_algorithmic": algorithmic_bytes(sc, counts, ne_s + nf_s,
-                                                          ne_d + nf_d, nnz)},
+                                                          ne_d + nf_d, nnz),
+                   "host_transfer": host_transfer},
         "roofline": roof_dom,
         "cpu_baseline": cpu,
     }
