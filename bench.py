@@ -254,6 +254,8 @@ def main():
     ctx.synchronize()
     up_ms = (time.perf_counter() - t0) / 3 * 1e3
     ctx.build_collisions(sc.dhat, sc.dmin)
+    ctx.gradient(sc.dhat, sc.kappa, download=False)  # incidences of the new collision set
+    ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(3):
         ctx.gradient(sc.dhat, sc.kappa, download=True)
