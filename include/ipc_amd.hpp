@@ -538,4 +538,15 @@ inline bool is_step_collision_free(const CollisionMesh& mesh, const MatrixXd& ve
     return free_ != 0;
 }
 
+// ipc.cpp:105-166 (3D): some edge crosses some triangle.  The broad phase
+// method does not change the answer, so none is taken.
+inline bool has_intersections(const CollisionMesh& mesh, const MatrixXd& vertices)
+{
+    mesh.set_positions(0, vertices);
+    mesh.touch_broad_phase();
+    int hit = 0;
+    detail::check(mesh.context(), ipcg_has_intersections(mesh.context(), &hit));
+    return hit != 0;
+}
+
 } // namespace ipc

@@ -115,6 +115,26 @@ int ipcg_codim_candidates_count(ipcg_ctx* ctx, int64_t* n_vv, int64_t* n_ev);
 /* vv: 2*n_vv int32, ev: 2*n_ev int32 (either may be NULL) */
 int ipcg_codim_candidates_download(ipcg_ctx* ctx, int32_t* vv, int32_t* ev);
 
+/* ---- edge-face candidates and intersections ------------------------------
+ * BroadPhase::detect_edge_face_candidates (broad_phase.hpp:91-92,
+ * lbvh.cpp:771-784, filter can_edge_face_collide lbvh.cpp:836-849) after a
+ * static build over positions slot 0 with the given inflation radius:
+ * (edge, face) pairs whose FP32 boxes overlap, sharing no vertex, passing
+ * the mesh filter.  Rebuilds the edge and face trees; the edge-edge and
+ * face-vertex lists of an earlier broad phase call stay valid. */
+int ipcg_edge_face_candidates(ipcg_ctx* ctx, double inflation, int64_t* n_ef);
+/* ef: 2*n_ef int32 */
+int ipcg_edge_face_candidates_download(ipcg_ctx* ctx, int32_t* ef);
+/* has_intersections (ipc.cpp:105-166, 3D) over positions slot 0: edge-face
+ * candidates at 1e-6 x the world bounding-box diagonal
+ * (world_bbox_diagonal_length.hpp:11-15), each tested with
+ * is_edge_intersecting_triangle (intersection.cpp:116-138, the default
+ * non-rational build: exact orient3d, then Eigen's FullPivLU solve).
+ * *intersecting = 1 when some edge crosses some triangle.  The answer
+ * does not depend on the broad phase method (every method reports a
+ * superset of the overlapping pairs), so none is taken. */
+int ipcg_has_intersections(ipcg_ctx* ctx, int* intersecting);
+
 /* ---- collision set (normal_collisions.cpp:20-158) ------------------------
  * build: static broad phase with inflation 0.5*(dhat+dmin) followed by the
  * narrow phase (NormalCollisions::build(mesh, V, dhat, dmin, broad_phase)).
@@ -194,7 +214,10 @@ int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable);
  * includes the origin), "ccd.spilled" (searches that outgrew the LDS
  * lists), "ccd.wavefront" (searches that outgrew a lane's HBM slab),
  * "hessian.ee_other" (edge-edge collisions of the last Hessian whose type
- * is not the interior one; their projection takes the 9 x 9 path) */
+ * is not the interior one; their projection takes the 9 x 9 path),
+ * "intersection.candidates" (edge-face candidates of the last
+ * has_intersections), "intersection.deferred" (their orient3d calls the
+ * floating-point filter left to the exact evaluation) */
 int ipcg_counter(ipcg_ctx* ctx, const char* name, int64_t* value);
 
 #ifdef __cplusplus

@@ -369,8 +369,10 @@ __device__ __forceinline__ bool can_fv(const Filter& f, int fi, int vi)
 //  TR_FV  vertex queries against the face tree: (face, vertex);
 //  TR_VV  codim vertices, self traversal: (query, other) in mesh ids;
 //  TR_EV  codim vertex queries against the codim-edge tree: (edge, vertex)
-//         in mesh ids (candidates.cpp:65-118).
-enum { TR_EE = 0, TR_FV = 1, TR_VV = 2, TR_EV = 3 };
+//         in mesh ids (candidates.cpp:65-118);
+//  TR_EF  edge queries against the face tree: (edge, face)
+//         (lbvh.cpp:771-784 reports the same set from face queries).
+enum { TR_EE = 0, TR_FV = 1, TR_VV = 2, TR_EV = 3, TR_EF = 4 };
 
 // The query primitive: its output id and vertices (-1 padded).
 struct Query {
@@ -404,6 +406,12 @@ __device__ __forceinline__ int2 leaf_pair(const Filter& flt, const Query& q, con
     } else if constexpr (MODE == TR_VV) { // codim vertex vs codim vertex
         const int b[1] = { r.y };
         return patch_ok(flt, q.v, 1, b, 1) ? make_int2(q.id, r.x) : make_int2(-1, -1);
+    } else if constexpr (MODE == TR_EF) { // edge vs face (lbvh.cpp:836-849)
+        if (q.v[0] == r.y || q.v[0] == r.z || q.v[0] == r.w || q.v[1] == r.y || q.v[1] == r.z
+            || q.v[1] == r.w)
+            return make_int2(-1, -1);
+        const int b[3] = { r.y, r.z, r.w };
+        return patch_ok(flt, q.v, 2, b, 3) ? make_int2(q.id, r.x) : make_int2(-1, -1);
     } else { // codim vertex vs codim edge
         if (q.v[0] == r.y || q.v[0] == r.z) return make_int2(-1, -1);
         const int b[2] = { r.y, r.z };
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
         const int qprim = (int)qids[qpos];
 #pragma unroll
         for (int k = 0; k < 6; k++) q[k] = qbox[(int64_t)6 * qprim + k];
-        if (MODE == TR_EE) { // query edge: id, endpoints
+        if (MODE == TR_EE || MODE == TR_EF) { // query edge: id, endpoints
             qr.id = qprim;
             qr.v[0] = flt.E[2 * (int64_t)qprim];
             qr.v[1] = flt.E[2 * (int64_t)qprim + 1];
@@ -825,6 +833,22 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     c.stage_end();
 }
 
+// mesh box of the vertex boxes (broad_phase.cpp:93-125) into c.aabb
+void mesh_bounds(Ctx& c, const double* X0, const double* X1, double inflation)
+{
+    hipStream_t s = c.stream;
+    c.stage_begin("bp_boxes");
+    const int nb = (int)grid_for(c.nV, BP_BLOCK, 1024);
+    c.aabb_partial.reserve(sizeof(double) * 6 * nb);
+    c.aabb.reserve(sizeof(double) * 6);
+    hipLaunchKernelGGL(k_mesh_aabb_partial, dim3(nb), dim3(BP_BLOCK), 0, s, X0, X1, c.nV,
+                       inflation, c.aabb_partial.as<double>());
+    hipLaunchKernelGGL(k_mesh_aabb_final, dim3(1), dim3(64), 0, s, c.aabb_partial.as<double>(),
+                       nb, c.aabb.as<double>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+}
+
 } // namespace
 
 void bp_build(Ctx& c, bool swept, double inflation, int method)
@@ -850,17 +874,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         c.cand_swept = swept;
         return;
     }
-    c.stage_begin("bp_boxes");
-    // mesh box of the vertex boxes (broad_phase.cpp:93-125)
-    const int nb = (int)grid_for(c.nV, BP_BLOCK, 1024);
-    c.aabb_partial.reserve(sizeof(double) * 6 * nb);
-    c.aabb.reserve(sizeof(double) * 6);
-    hipLaunchKernelGGL(k_mesh_aabb_partial, dim3(nb), dim3(BP_BLOCK), 0, s, X0, X1, c.nV,
-                       inflation, c.aabb_partial.as<double>());
-    hipLaunchKernelGGL(k_mesh_aabb_final, dim3(1), dim3(64), 0, s, c.aabb_partial.as<double>(),
-                       nb, c.aabb.as<double>());
-    IPCG_CHECK_LAUNCH();
-    c.stage_end();
+    mesh_bounds(c, X0, X1, inflation);
     int axis = 0;
     if (sap) {
         // sweep axis: largest extent of the mesh box
@@ -1007,6 +1021,68 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         return;
     }
     throw Error(IPCG_ERR_NUMERIC, "broad phase candidate lists kept outgrowing their regions");
+}
+
+// detect_edge_face_candidates (lbvh.cpp:771-784) over static boxes of
+// positions slot 0: edge queries against a face LBVH, (edge, face) pairs
+// into c.cand_ef.  The edge and face trees of an earlier broad phase are
+// rebuilt here; its candidate lists stay valid.
+void bp_edge_face(Ctx& c, double inflation)
+{
+    c.require_mesh();
+    c.require_pos(0);
+    c.n_ef = 0;
+    if (c.nE == 0 || c.nF == 0) return;
+    hipStream_t s = c.stream;
+    const double* X0 = c.X[0].as<double>();
+    struct Tag {
+        Ctx& c;
+        Tag(Ctx& c) : c(c) { c.stage_tag = ".ef"; }
+        ~Tag() { c.stage_tag.clear(); }
+    } tag(c);
+    mesh_bounds(c, X0, nullptr, inflation);
+    build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, nullptr, inflation, false);
+    build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, nullptr, inflation, true);
+    Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr,
+                 c.n_dynamic, c.n_codim_filter };
+    c.counters.reserve(sizeof(unsigned long long) * 4);
+    int* status = reinterpret_cast<int*>(c.counters.as<unsigned long long>() + 2);
+    c.shard_cnt.reserve(4 * SHARD_CNT_BYTES);
+    c.shard_off.reserve(sizeof(int64_t) * 4 * (SHARDS + 1));
+    unsigned long long* cnt = c.shard_cnt.as<unsigned long long>();
+    c.tr_deep.reserve(sizeof(int) * (size_t)c.nE * (TR_STACK - TR_LSTACK));
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (c.scap_ef == 0) c.scap_ef = std::max<int64_t>(1024, 8 * (int64_t)c.nE / SHARDS);
+        c.cand_stage[0].reserve(sizeof(int2) * (size_t)(SHARDS * c.scap_ef));
+        IPCG_HIP(hipMemsetAsync(cnt, 0, SHARD_CNT_BYTES, s));
+        IPCG_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
+        const Shards S0 { cnt, c.scap_ef };
+        c.stage_begin("bp_traverse_ef");
+        hipLaunchKernelGGL((k_traverse<TR_EF, 0>), dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0,
+                           s, c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
+                           c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
+                           c.tf.box.as<float>(), c.nF, flt, nullptr, c.cand_stage[0].as<int2>(), S0,
+                           c.tr_deep.as<int>(), status);
+        IPCG_CHECK_LAUNCH();
+        c.stage_end();
+        int* hs = static_cast<int*>(c.pinned);
+        IPCG_HIP(hipMemcpyAsync(hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        if (hs[0] & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
+        if (hs[0] & 1) throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
+        const ShardCounts kc = shard_counts(cnt, c.pinned, s);
+        if (kc.max > c.scap_ef) {
+            c.scap_ef = kc.max + kc.max / 4 + 1024;
+            continue;
+        }
+        c.cand_ef.reserve(sizeof(int2) * (size_t)std::max<int64_t>(kc.total, 1));
+        if (kc.total > 0)
+            shard_compact(c.cand_stage[0].as<int2>(), c.scap_ef, cnt, kc.max, c.cand_ef.as<int2>(),
+                          c.shard_off.as<int64_t>(), s);
+        c.n_ef = kc.total;
+        return;
+    }
+    throw Error(IPCG_ERR_NUMERIC, "edge-face candidate list kept outgrowing its regions");
 }
 
 } // namespace ipcg

@@ -168,7 +168,8 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.tv.leaf, &c.te.leaf, &c.tf.leaf, &c.inc_keys, &c.inc_keys2,
                        &c.inc_vals, &c.inc_vals2, &c.inc_ptr, &c.inc_vtx, &c.ee_other, &c.hrec, &c.tr_deep,
                        &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
-                       &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3] };
+                       &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3],
+                       &c.cand_ef, &c.isect };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
                            &T->root, &T->sbox, &T->leaf })
@@ -362,6 +363,33 @@ int ipcg_codim_candidates_download(ipcg_ctx* ctx, int32_t* vv, int32_t* ev)
         if (ev && c.n_ev)
             IPCG_HIP(hipMemcpyAsync(ev, c.cand_ev.p, sizeof(int2) * c.n_ev, hipMemcpyDeviceToHost, c.stream));
         IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_edge_face_candidates(ipcg_ctx* ctx, double inflation, int64_t* n_ef)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(inflation >= 0)) throw Error(IPCG_ERR_ARG, "inflation must be >= 0");
+        bp_edge_face(c, inflation);
+        if (n_ef) *n_ef = c.n_ef;
+    });
+}
+
+int ipcg_edge_face_candidates_download(ipcg_ctx* ctx, int32_t* ef)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (ef && c.n_ef)
+            IPCG_HIP(hipMemcpyAsync(ef, c.cand_ef.p, sizeof(int2) * c.n_ef, hipMemcpyDeviceToHost,
+                                    c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_has_intersections(ipcg_ctx* ctx, int* intersecting)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!intersecting) throw Error(IPCG_ERR_ARG, "null output");
+        *intersecting = has_intersections(c) ? 1 : 0;
     });
 }
 

@@ -69,6 +69,9 @@ struct Ctx {
     DevBuf counters;           // uint64 counters (device)
     int64_t n_ee = 0, n_fv = 0, n_vv = 0, n_ev = 0;
     DevBuf cand_vv, cand_ev;           // codim lists: (v, v) and (edge, vertex)
+    DevBuf cand_ef;                    // (edge, face) pairs of the last bp_edge_face
+    int64_t n_ef = 0, scap_ef = 0;
+    DevBuf isect;                      // intersection test: flag, deferred count, list
     int64_t scap_ee = 0, scap_fv = 0, scap_vv = 0, scap_ev = 0; // shard regions
     DevBuf cand_stage[4];              // sharded traversal output (ee, fv, vv, ev)
     DevBuf tr_deep;                    // traversal stack entries beyond LDS
@@ -124,5 +127,7 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
 // receiving per-candidate times of impact
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
                     double rescale, bool early_exit, double* const* toi_out);
+void bp_edge_face(Ctx& c, double inflation);
+bool has_intersections(Ctx& c);
 
 } // namespace ipcg

@@ -32,6 +32,7 @@ EXPORTS = [
     "ipcg_stage_times_reset", "ipcg_mesh_codim", "ipcg_codim_candidates_count",
     "ipcg_codim_candidates_download", "ipcg_codim_candidates_toi",
     "ipcg_candidates_is_step_collision_free", "ipcg_counter", "ipcg_mesh_filter",
+    "ipcg_edge_face_candidates", "ipcg_edge_face_candidates_download", "ipcg_has_intersections",
 ]
 
 _lib = None
@@ -93,6 +94,9 @@ def load(path=LIB_PATH):
     L.ipcg_codim_candidates_download.argtypes = [vp, _I, _I]
     L.ipcg_codim_candidates_toi.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_double,
                                             _D, _D]
+    L.ipcg_edge_face_candidates.argtypes = [vp, C.c_double, _L]
+    L.ipcg_edge_face_candidates_download.argtypes = [vp, _I]
+    L.ipcg_has_intersections.argtypes = [vp, C.POINTER(C.c_int)]
     L.ipcg_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), _D, _L]
     L.ipcg_stage_times_reset.argtypes = [vp, C.c_int]
     for name in EXPORTS:
@@ -199,6 +203,21 @@ class Context:
         self._check(self._L.ipcg_codim_candidates_download(self._p, vv.ctypes.data_as(_I),
                                                            ev.ctypes.data_as(_I)))
         return vv, ev
+
+    def edge_face_candidates(self, inflation=0.0):
+        """(edge, face) pairs over static boxes of positions slot 0
+        (BroadPhase::detect_edge_face_candidates)"""
+        n = C.c_int64()
+        self._check(self._L.ipcg_edge_face_candidates(self._p, inflation, C.byref(n)))
+        ef = np.zeros((n.value, 2), np.int32)
+        self._check(self._L.ipcg_edge_face_candidates_download(self._p, ef.ctypes.data_as(_I)))
+        return ef
+
+    def has_intersections(self):
+        """has_intersections(mesh, V) over positions slot 0"""
+        r = C.c_int()
+        self._check(self._L.ipcg_has_intersections(self._p, C.byref(r)))
+        return bool(r.value)
 
     # ---- collisions --------------------------------------------------------
     def build_collisions(self, dhat, dmin=0.0, method=BP_LBVH):

@@ -158,6 +158,19 @@ def _rotation(rng):
     return q
 
 
+random_rotation = _rotation
+
+
+def cube(size=1.0, center=(0.0, 0.0, 0.0)):
+    """Axis-aligned cube surface: 8 vertices, 12 outward triangles."""
+    V = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], dtype=np.float64)
+    V = (V - 0.5) * size + np.asarray(center, dtype=np.float64)
+    F = np.array([[0, 2, 1], [1, 2, 3], [4, 5, 6], [5, 7, 6], [0, 1, 4], [1, 5, 4],
+                  [2, 6, 3], [3, 6, 7], [0, 4, 2], [2, 4, 6], [1, 3, 5], [3, 7, 5]],
+                 dtype=np.int32)
+    return V, F
+
+
 def sphere_chain(count=2, subdiv=3, radius=0.1, seed=0, dhat=1e-3, kappa=1e3,
                  approach=1.5, noise=0.1):
     """`count` randomly rotated icospheres of one subdivision level in a row

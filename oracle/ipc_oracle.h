@@ -171,6 +171,29 @@ int ipco_candidates_sap(const double *V0, const double *V1, int nV,
                         ipco_pairs *ee, ipco_pairs *fv);
 int ipco_sap_axis(const double *V0, const double *V1, int nV, double inflation);
 void ipco_pairs_free(ipco_pairs *p);
+/* detect_edge_face_candidates (lbvh.cpp:771-784, filter lbvh.cpp:836-849)
+ * over static boxes: (edge, face) pairs, LBVH semantics (FP32 boxes). */
+int ipco_edge_face_candidates(const double *V0, int nV, const int32_t *E, int nE,
+                              const int32_t *F, int nF, double inflation,
+                              const int32_t *vertex_group, ipco_pairs *ef);
+
+/* ---- intersections ------------------------------------------------------- */
+/* exact sign of orient3d (igl::predicates::orient3d over Shewchuk's
+ * predicate): +1, -1 or 0 (coplanar) */
+int ipco_orient3d(const double *pa, const double *pb, const double *pc, const double *pd);
+/* orient3d calls the floating-point filter could not decide (process total) */
+long long ipco_orient3d_exact_calls(void);
+/* Eigen FullPivLU<Matrix3d>(A).solve(b), A column-major */
+void ipco_fullpivlu_solve3(const double *A, const double *b, double *x);
+/* intersection.cpp:116-138 (non-rational build) */
+int ipco_is_edge_intersecting_triangle(const double *e0, const double *e1, const double *t0,
+                                       const double *t1, const double *t2);
+/* world_bbox_diagonal_length.hpp:11-15 */
+double ipco_world_bbox_diagonal(const double *V, int nV);
+/* ipc.cpp:105-166 (3D branch); *result = 1 when some edge crosses a face */
+int ipco_has_intersections(const double *V, int nV, const int32_t *E, int nE,
+                           const int32_t *F, int nF, const int32_t *vertex_group,
+                           int *result);
 
 /* ---- narrow phase -------------------------------------------------------- */
 typedef struct ipco_collisions {

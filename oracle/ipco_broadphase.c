@@ -323,6 +323,18 @@ static int can_fv(const filt *f, int fi, int vi)
     return can_vv(f, vi, f0) || can_vv(f, vi, f1) || can_vv(f, vi, f2);
 }
 
+/* lbvh.cpp:836-849 can_edge_face_collide */
+static int can_ef(const filt *f, int ei, int fi)
+{
+    const int e0 = f->E[2 * (size_t)ei], e1 = f->E[2 * (size_t)ei + 1];
+    const int f0 = f->F[3 * (size_t)fi], f1 = f->F[3 * (size_t)fi + 1],
+              f2 = f->F[3 * (size_t)fi + 2];
+    if (e0 == f0 || e0 == f1 || e0 == f2 || e1 == f0 || e1 == f1 || e1 == f2)
+        return 0;
+    return can_vv(f, e0, f0) || can_vv(f, e0, f1) || can_vv(f, e0, f2)
+        || can_vv(f, e1, f0) || can_vv(f, e1, f1) || can_vv(f, e1, f2);
+}
+
 typedef struct pairvec {
     int64_t n, cap;
     int32_t *ij;
@@ -352,14 +364,18 @@ static void emit(const filt *f, int kind, int q, int j, pairvec *out)
     case 2: /* codim VV (query, other) in mesh ids */
         if (can_vv(f, f->vmap[q], f->vmap[j])) pv_push(out, f->vmap[q], f->vmap[j]);
         break;
-    default: /* codim EV (edge, vertex) in mesh ids */
+    case 3: /* codim EV (edge, vertex) in mesh ids */
         if (can_ev(f, f->emap[j], f->vmap[q])) pv_push(out, f->emap[j], f->vmap[q]);
+        break;
+    default: /* EF: face queries against the edge tree, (edge, face) */
+        if (can_ef(f, j, q)) pv_push(out, j, q);
     }
 }
 
 /* traverse_lbvh (lbvh.cpp:368-455): kinds 0 (EE) and 2 (codim VV) are
  * triangular self traversals; 1 (FV) and 3 (codim EV) query vertex leaves
- * against the face / codim-edge tree (swap order). */
+ * against the face / codim-edge tree (swap order); 4 (EF) queries face
+ * leaves against the edge tree (lbvh.cpp:771-784). */
 static void traverse(const float *qbox, int qprim, int qpos, const lbvh *t,
                      int kind, const filt *f, pairvec *out)
 {
@@ -463,6 +479,38 @@ int ipco_candidates(const double *V0, const double *V1, int nV,
     detect(&te, &te, 0, &f, ee); /* lbvh.cpp:742-754 */
     detect(&tv, &tf, 1, &f, fv); /* lbvh.cpp:756-769 */
     lbvh_free(&tv);
+    lbvh_free(&te);
+    lbvh_free(&tf);
+    free(vb);
+    free(eb);
+    free(fb);
+    return 0;
+}
+
+/* detect_edge_face_candidates (lbvh.cpp:771-784) after a static build
+ * (broad_phase.cpp:59-91 boxes, lbvh.cpp:99-131 trees) */
+int ipco_edge_face_candidates(const double *V0, int nV, const int32_t *E, int nE,
+                              const int32_t *F, int nF, double inflation,
+                              const int32_t *vertex_group, ipco_pairs *ef)
+{
+    ef->n = 0;
+    ef->ij = NULL;
+    if (nV <= 0 || nE <= 0 || nF <= 0) return 0;
+    box64 *vb = vertex_boxes64(V0, NULL, nV, inflation);
+    box64 *eb = element_boxes64(vb, E, nE, 2);
+    box64 *fb = element_boxes64(vb, F, nF, 3);
+    double mn[3] = { INFINITY, INFINITY, INFINITY };
+    double mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = 0; i < nV; i++)
+        for (int k = 0; k < 3; k++) {
+            if (vb[i].lo[k] < mn[k]) mn[k] = vb[i].lo[k];
+            if (mx[k] < vb[i].hi[k]) mx[k] = vb[i].hi[k];
+        }
+    lbvh te, tf;
+    lbvh_build(eb, nE, mn, mx, &te);
+    lbvh_build(fb, nF, mn, mx, &tf);
+    filt f = { E, F, vertex_group, NULL, NULL };
+    detect(&tf, &te, 4, &f, ef);
     lbvh_free(&te);
     lbvh_free(&tf);
     free(vb);

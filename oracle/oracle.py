@@ -140,6 +140,18 @@ def lib():
                                                   C.c_double, C.POINTER(CCDParams)]
         L.ipco_candidates_toi.argtypes = [_D, _D, _I, _I, C.POINTER(_Pairs), C.c_int,
                                           C.c_double, C.POINTER(CCDParams), _D]
+        L.ipco_edge_face_candidates.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int,
+                                                 C.c_double, _I, C.POINTER(_Pairs)]
+        L.ipco_orient3d.argtypes = [_D, _D, _D, _D]
+        L.ipco_orient3d.restype = C.c_int
+        L.ipco_orient3d_exact_calls.restype = C.c_longlong
+        L.ipco_fullpivlu_solve3.argtypes = [_D, _D, _D]
+        L.ipco_is_edge_intersecting_triangle.argtypes = [_D, _D, _D, _D, _D]
+        L.ipco_is_edge_intersecting_triangle.restype = C.c_int
+        L.ipco_world_bbox_diagonal.argtypes = [_D, C.c_int]
+        L.ipco_world_bbox_diagonal.restype = C.c_double
+        L.ipco_has_intersections.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I,
+                                             C.POINTER(C.c_int)]
         L.ipco_set_num_threads.argtypes = [C.c_int]
         L.ipco_get_max_threads.restype = C.c_int
         _lib = L
@@ -364,6 +376,62 @@ def codim_candidates(V0, E, F, inflation=0.0, V1=None, vertex_group=None):
     lib().ipco_codim_candidates(p0, p1, len(V0), pE, len(E), pF, len(F), pF2E, inflation,
                                 pg, C.byref(vv), C.byref(ev))
     return _take_pairs(vv), _take_pairs(ev)
+
+
+def edge_face_candidates(V, E, F, inflation=0.0, vertex_group=None):
+    """(edge, face) candidate array over static boxes (lbvh.cpp:771-784)."""
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    pg = None
+    if vertex_group is not None:
+        vertex_group, pg = _i(vertex_group)
+    ef = _Pairs()
+    lib().ipco_edge_face_candidates(pV, len(V), pE, len(E), pF, len(F), inflation, pg,
+                                    C.byref(ef))
+    return _take_pairs(ef)
+
+
+def orient3d(pa, pb, pc, pd):
+    """exact sign of Shewchuk's orient3d: +1, -1 or 0"""
+    return lib().ipco_orient3d(_d(pa)[1], _d(pb)[1], _d(pc)[1], _d(pd)[1])
+
+
+def orient3d_exact_calls():
+    return int(lib().ipco_orient3d_exact_calls())
+
+
+def fullpivlu_solve3(A, b):
+    """Eigen FullPivLU<Matrix3d>(A).solve(b)"""
+    A = np.asarray(A, dtype=np.float64).reshape(3, 3)
+    Acm, pA = _d(A.T.copy())  # column-major
+    b, pb = _d(b)
+    x = np.zeros(3)
+    lib().ipco_fullpivlu_solve3(pA, pb, x.ctypes.data_as(_D))
+    return x
+
+
+def is_edge_intersecting_triangle(e0, e1, t0, t1, t2):
+    return bool(lib().ipco_is_edge_intersecting_triangle(
+        _d(e0)[1], _d(e1)[1], _d(t0)[1], _d(t1)[1], _d(t2)[1]))
+
+
+def world_bbox_diagonal(V):
+    V, pV = _d(V)
+    return lib().ipco_world_bbox_diagonal(pV, len(V))
+
+
+def has_intersections(V, E, F, vertex_group=None):
+    """ipc.cpp:105-166 (3D)"""
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    pg = None
+    if vertex_group is not None:
+        vertex_group, pg = _i(vertex_group)
+    r = C.c_int(0)
+    lib().ipco_has_intersections(pV, len(V), pE, len(E), pF, len(F), pg, C.byref(r))
+    return bool(r.value)
 
 
 def sap_axis(V0, inflation=0.0, V1=None):

@@ -191,6 +191,24 @@ static void collision_filters()
     CHECK(threw);
 }
 
+// A triangle and an edge through it (ipc.cpp:105-166): the edge's
+// endpoints on either side of the plane and its crossing point inside the
+// triangle; moved aside, the same edge misses.
+static void intersections()
+{
+    using namespace ipc;
+    const MatrixXi F { { 0, 1, 2 } };
+    const MatrixXi E { { 0, 1 }, { 0, 2 }, { 1, 2 }, { 3, 4 } };
+    MatrixXd V { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0.25, 0.25, -1 }, { 0.25, 0.25, 1 } };
+    CollisionMesh mesh(V, E, F);
+    CHECK(has_intersections(mesh, V));
+    V(3, 0) = V(4, 0) = 2.0;
+    CHECK(!has_intersections(mesh, V));
+    V(3, 0) = V(4, 0) = 0.25;
+    V(3, 2) = 0.0; // touching the triangle from above counts (t = 0)
+    CHECK(has_intersections(mesh, V));
+}
+
 int main()
 {
     try {
@@ -199,6 +217,7 @@ int main()
         codim_edge_vertex();
         vertex_on_plane();
         errors_throw();
+        intersections();
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());
         failures++;
