@@ -265,7 +265,7 @@ public:
         mesh.touch_broad_phase();
         detail::check(mesh.context(), ipcg_broad_phase(mesh.context(), 0, inflation_radius, int(bp)));
         download(mesh);
-        swept_for = nullptr;
+        remember(mesh, false, vertices, vertices, inflation_radius, bp);
     }
     // candidates.cpp:120-222 (swept)
     void build(const CollisionMesh& mesh, const MatrixXd& vertices_t0, const MatrixXd& vertices_t1,
@@ -276,10 +276,7 @@ public:
         mesh.touch_broad_phase();
         detail::check(mesh.context(), ipcg_broad_phase(mesh.context(), 1, inflation_radius, int(bp)));
         download(mesh);
-        swept_for = &mesh;
-        generation = mesh.broad_phase_generation();
-        inflation = inflation_radius;
-        method = bp;
+        remember(mesh, true, vertices_t0, vertices_t1, inflation_radius, bp);
     }
     size_t size() const
     {
@@ -288,12 +285,12 @@ public:
     }
     bool empty() const { return size() == 0; }
 
-    // candidates.cpp:252-292 over the device lists of the last swept build
+    // candidates.cpp:252-292 over these candidates (static or swept)
     double compute_collision_free_stepsize(const CollisionMesh& mesh, const MatrixXd& vertices_t0,
                                            const MatrixXd& vertices_t1, double min_distance = 0,
                                            const TightInclusionCCD& ccd = {}) const
     {
-        require_swept(mesh, vertices_t0, vertices_t1);
+        require_lists(mesh, vertices_t0, vertices_t1);
         double step = 1;
         detail::check(mesh.context(),
                       ipcg_candidates_stepsize(mesh.context(), min_distance, ccd.tolerance,
@@ -306,7 +303,7 @@ public:
                                 const MatrixXd& vertices_t1, double min_distance = 0,
                                 const TightInclusionCCD& ccd = {}) const
     {
-        require_swept(mesh, vertices_t0, vertices_t1);
+        require_lists(mesh, vertices_t0, vertices_t1);
         int free_ = 0;
         detail::check(mesh.context(), ipcg_candidates_is_step_collision_free(
                                           mesh.context(), min_distance, ccd.tolerance,
@@ -314,26 +311,88 @@ public:
         return free_ != 0;
     }
 
+    // candidates.cpp:372-408
+    std::vector<double> compute_per_vertex_safe_distances(const CollisionMesh& mesh,
+                                                          const MatrixXd& vertices,
+                                                          double inflation_radius,
+                                                          double min_distance = 0) const
+    {
+        require_lists(mesh);
+        mesh.set_positions(0, vertices);
+        std::vector<double> out((size_t)mesh.num_vertices());
+        detail::check(mesh.context(), ipcg_candidates_per_vertex_safe_distances(
+                                          mesh.context(), inflation_radius, min_distance,
+                                          out.data()));
+        return out;
+    }
+    // candidates.cpp:294-333
+    double compute_noncandidate_conservative_stepsize(const CollisionMesh& mesh,
+                                                      const MatrixXd& displacements,
+                                                      double dhat) const
+    {
+        require_lists(mesh);
+        // slot 1 - slot 0 is the displacement itself
+        mesh.set_positions(0, MatrixXd(displacements.rows(), displacements.cols()));
+        mesh.set_positions(1, displacements);
+        double step = 1;
+        detail::check(mesh.context(), ipcg_candidates_noncandidate_conservative_stepsize(
+                                          mesh.context(), dhat, &step));
+        return step;
+    }
+    // candidates.cpp:335-360; the full CCD fallback replaces the device lists
+    double compute_cfl_stepsize(const CollisionMesh& mesh, const MatrixXd& vertices_t0,
+                                const MatrixXd& vertices_t1, double dhat, double min_distance = 0,
+                                BroadPhaseMethod bp = BroadPhaseMethod::LBVH,
+                                const TightInclusionCCD& ccd = {}) const
+    {
+        require_lists(mesh, vertices_t0, vertices_t1);
+        double step = 1;
+        const int rc = ipcg_candidates_cfl_stepsize(mesh.context(), dhat, min_distance,
+                                                    ccd.tolerance, ccd.max_iterations,
+                                                    ccd.conservative_rescaling, int(bp), &step);
+        mesh.touch_broad_phase();
+        detail::check(mesh.context(), rc);
+        return step;
+    }
+
 private:
-    const CollisionMesh* swept_for = nullptr;
+    const CollisionMesh* built_for = nullptr;
+    bool swept = false;
+    MatrixXd build_v0, build_v1; // inputs of the build, to rebuild the device lists
     uint64_t generation = 0;
     double inflation = 0;
     BroadPhaseMethod method = BroadPhaseMethod::LBVH;
-    // The device evaluates its own copy of the lists; when another call
-    // replaced them since build(), the swept broad phase runs again with the
-    // same trajectory, inflation and method, which reproduces the same lists.
-    void require_swept(const CollisionMesh& mesh, const MatrixXd& V0, const MatrixXd& V1) const
+    void remember(const CollisionMesh& mesh, bool swept_, const MatrixXd& V0, const MatrixXd& V1,
+                  double r, BroadPhaseMethod bp)
     {
-        if (swept_for != &mesh)
-            throw std::runtime_error("candidates were not built from this mesh's trajectory");
-        mesh.set_positions(0, V0);
-        mesh.set_positions(1, V1);
+        built_for = &mesh;
+        swept = swept_;
+        build_v0 = V0;
+        build_v1 = swept_ ? V1 : MatrixXd();
+        generation = mesh.broad_phase_generation();
+        inflation = r;
+        method = bp;
+    }
+    // The device evaluates its own copy of the lists; when another call
+    // replaced them since build(), the same broad phase runs again on the
+    // build's inputs, which reproduces the same lists.
+    void require_lists(const CollisionMesh& mesh) const
+    {
+        if (built_for != &mesh) throw std::runtime_error("candidates were not built for this mesh");
         if (mesh.broad_phase_generation() != generation) {
+            mesh.set_positions(0, build_v0);
+            if (swept) mesh.set_positions(1, build_v1);
             mesh.touch_broad_phase();
             detail::check(mesh.context(),
-                          ipcg_broad_phase(mesh.context(), 1, inflation, int(method)));
+                          ipcg_broad_phase(mesh.context(), swept ? 1 : 0, inflation, int(method)));
             const_cast<Candidates*>(this)->generation = mesh.broad_phase_generation();
         }
+    }
+    void require_lists(const CollisionMesh& mesh, const MatrixXd& V0, const MatrixXd& V1) const
+    {
+        require_lists(mesh);
+        mesh.set_positions(0, V0);
+        mesh.set_positions(1, V1);
     }
     void download(const CollisionMesh& mesh)
     {
@@ -486,6 +545,32 @@ public:
         H.values.resize(nnz);
         detail::check(c, ipcg_hessian_download(c, H.outer.data(), H.inner.data(), H.values.data()));
         return H;
+    }
+
+    // normal_potential.cpp:15-46
+    std::vector<double> gauss_newton_hessian_diagonal(const NormalCollisions& collisions,
+                                                      const CollisionMesh& mesh,
+                                                      const MatrixXd& vertices) const
+    {
+        prepare(collisions, mesh, vertices);
+        std::vector<double> d(3 * (size_t)mesh.num_vertices());
+        detail::check(mesh.context(), ipcg_barrier_gauss_newton_diagonal(
+                                          mesh.context(), m_dhat, m_stiffness, d.data()));
+        return d;
+    }
+    // normal_potential.cpp:48-79; p: 3 #V values, RowMajor
+    double gauss_newton_hessian_quadratic_form(const NormalCollisions& collisions,
+                                               const CollisionMesh& mesh,
+                                               const MatrixXd& vertices,
+                                               const std::vector<double>& p) const
+    {
+        if (p.size() != 3 * (size_t)mesh.num_vertices())
+            throw std::invalid_argument("p must hold 3 #V values");
+        prepare(collisions, mesh, vertices);
+        double v = 0;
+        detail::check(mesh.context(), ipcg_barrier_gauss_newton_quadratic_form(
+                                          mesh.context(), m_dhat, m_stiffness, p.data(), &v));
+        return v;
     }
 
 private:

@@ -165,6 +165,19 @@ int ipcg_hessian_download(ipcg_ctx* ctx, int64_t* rowptr, int32_t* col, double* 
 /* device pointers of the last gradient / CSR (valid until the next call) */
 int ipcg_device_results(ipcg_ctx* ctx, const double** grad, const int64_t** rowptr,
                         const int32_t** col, const double** val);
+/* NormalPotential::gauss_newton_hessian_diagonal (normal_potential.cpp:
+ * 15-46, per collision :236-270): per collision the distance vector
+ * t = sum_i c_i x_i of its stored distance type (closest-point
+ * coefficients), w (4 f''(|t|^2) c_i^2 t_k^2 + 2 f'(|t|^2) c_i^2) summed
+ * per vertex coordinate.  diag: host array of 3*nV doubles (RowMajor). */
+int ipcg_barrier_gauss_newton_diagonal(ipcg_ctx* ctx, double dhat, double kappa,
+                                       double* diag);
+/* NormalPotential::gauss_newton_hessian_quadratic_form (normal_potential.cpp:
+ * 48-79, :272-308): sum over collisions of
+ * w (4 f'' (q.t)^2 + 2 f' |q|^2), q = sum_i c_i p_i.  p: host array of
+ * 3*nV doubles (RowMajor). */
+int ipcg_barrier_gauss_newton_quadratic_form(ipcg_ctx* ctx, double dhat, double kappa,
+                                             const double* p, double* value);
 
 /* ---- CCD (ipc.cpp:45-101, candidates.cpp:252-292, tight_inclusion_ccd) --
  * compute_collision_free_stepsize: swept broad phase (inflation
@@ -176,7 +189,8 @@ int ipcg_compute_collision_free_stepsize(ipcg_ctx* ctx, double min_distance,
                                          double tolerance, int64_t max_iterations,
                                          double conservative_rescaling, int method,
                                          double* step);
-/* same narrow phase over the current (swept) candidate lists */
+/* same narrow phase over the current candidate lists, however they were
+ * built (Candidates::compute_collision_free_stepsize) */
 int ipcg_candidates_stepsize(ipcg_ctx* ctx, double min_distance, double tolerance,
                              int64_t max_iterations, double conservative_rescaling,
                              double* step);
@@ -190,6 +204,26 @@ int ipcg_candidates_is_step_collision_free(ipcg_ctx* ctx, double min_distance,
 int ipcg_is_step_collision_free(ipcg_ctx* ctx, double min_distance, double tolerance,
                                 int64_t max_iterations, double conservative_rescaling,
                                 int method, int* collision_free);
+/* Candidates::compute_per_vertex_safe_distances (candidates.cpp:372-408)
+ * over the current lists at slot-0 positions: per vertex, the smallest
+ * sqrt(distance) - min_distance over the candidates it belongs to (distance
+ * type classified per candidate), at most inflation_radius - min_distance.
+ * out: nV doubles. */
+int ipcg_candidates_per_vertex_safe_distances(ipcg_ctx* ctx, double inflation_radius,
+                                              double min_distance, double* out);
+/* Candidates::compute_noncandidate_conservative_stepsize (candidates.cpp:
+ * 294-333) with displacements slot 1 - slot 0: 0.5 dhat / the largest
+ * displacement norm of a vertex in some current candidate (1 when there is
+ * no candidate). */
+int ipcg_candidates_noncandidate_conservative_stepsize(ipcg_ctx* ctx, double dhat,
+                                                       double* step);
+/* Candidates::compute_cfl_stepsize (candidates.cpp:335-360): the CCD step of
+ * the current lists and the noncandidate step; when the latter is below
+ * half the former, the full compute_collision_free_stepsize (swept broad
+ * phase with `method`, which replaces the current lists). */
+int ipcg_candidates_cfl_stepsize(ipcg_ctx* ctx, double dhat, double min_distance,
+                                 double tolerance, int64_t max_iterations,
+                                 double conservative_rescaling, int method, double* step);
 /* per-candidate times of impact of the current swept lists, each on
  * [0,1] with no bound (+inf when no hit); toi_ee: n_ee, toi_fv: n_fv */
 int ipcg_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,

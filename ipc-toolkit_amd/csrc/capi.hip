@@ -169,7 +169,7 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.inc_vals, &c.inc_vals2, &c.inc_ptr, &c.inc_vtx, &c.ee_other, &c.hrec, &c.tr_deep,
                        &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
                        &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3],
-                       &c.cand_ef, &c.isect };
+                       &c.cand_ef, &c.isect, &c.gn_out, &c.gn_p };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
                            &T->root, &T->sbox, &T->leaf })
@@ -385,6 +385,29 @@ int ipcg_edge_face_candidates_download(ipcg_ctx* ctx, int32_t* ef)
     });
 }
 
+int ipcg_barrier_gauss_newton_diagonal(ipcg_ctx* ctx, double dhat, double kappa, double* diag)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!diag) throw Error(IPCG_ERR_ARG, "null output");
+        if (!(dhat > 0) || !(kappa > 0)) throw Error(IPCG_ERR_ARG, "dhat and kappa must be > 0");
+        pot_gn_diagonal(c, dhat, kappa);
+        if (c.nV)
+            IPCG_HIP(hipMemcpyAsync(diag, c.gn_out.p, sizeof(double) * 3 * (size_t)c.nV,
+                                    hipMemcpyDeviceToHost, c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int ipcg_barrier_gauss_newton_quadratic_form(ipcg_ctx* ctx, double dhat, double kappa,
+                                             const double* p, double* value)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!value || (!p && c.nV)) throw Error(IPCG_ERR_ARG, "null argument");
+        if (!(dhat > 0) || !(kappa > 0)) throw Error(IPCG_ERR_ARG, "dhat and kappa must be > 0");
+        *value = pot_gn_quadratic_form(c, dhat, kappa, p);
+    });
+}
+
 int ipcg_has_intersections(ipcg_ctx* ctx, int* intersecting)
 {
     return guard(ctx, [&](Ctx& c) {
@@ -539,6 +562,49 @@ int ipcg_candidates_is_step_collision_free(ipcg_ctx* ctx, double min_distance, d
         if (!collision_free) throw Error(IPCG_ERR_ARG, "null output");
         const double r = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, true, nullptr);
         *collision_free = r >= 1.0 ? 1 : 0;
+    });
+}
+
+int ipcg_candidates_per_vertex_safe_distances(ipcg_ctx* ctx, double inflation_radius,
+                                              double min_distance, double* out)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!out) throw Error(IPCG_ERR_ARG, "null output");
+        if (!(min_distance >= 0) || !(inflation_radius >= min_distance))
+            throw Error(IPCG_ERR_ARG, "need inflation_radius >= min_distance >= 0");
+        cand_safe_distances(c, inflation_radius, min_distance, out);
+    });
+}
+
+int ipcg_candidates_noncandidate_conservative_stepsize(ipcg_ctx* ctx, double dhat, double* step)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!step) throw Error(IPCG_ERR_ARG, "null output");
+        if (!(dhat > 0)) throw Error(IPCG_ERR_ARG, "dhat must be > 0");
+        *step = cand_noncandidate_stepsize(c, dhat);
+    });
+}
+
+int ipcg_candidates_cfl_stepsize(ipcg_ctx* ctx, double dhat, double min_distance,
+                                 double tolerance, int64_t max_iterations, double rescale,
+                                 int method, double* step)
+{
+    return guard(ctx, [&](Ctx& c) {
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!step) throw Error(IPCG_ERR_ARG, "null output");
+        if (!(dhat > 0)) throw Error(IPCG_ERR_ARG, "dhat must be > 0");
+        if (method != IPCG_BP_LBVH && method != IPCG_BP_SAP)
+            throw Error(IPCG_ERR_ARG, "unsupported broad phase method");
+        const double alpha_c =
+            ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr);
+        const double alpha_f = cand_noncandidate_stepsize(c, dhat);
+        if (alpha_f < 0.5 * alpha_c) { // full CCD (candidates.cpp:352-357)
+            bp_build(c, true, 0.5 * min_distance, method);
+            *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false,
+                                 nullptr);
+        } else {
+            *step = alpha_c < alpha_f ? alpha_c : alpha_f;
+        }
     });
 }
 

@@ -676,8 +676,9 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     c.require_mesh();
     c.require_pos(0);
     c.require_pos(1);
-    if (!c.cand_valid || !c.cand_swept)
-        throw Error(IPCG_ERR_STATE, "no swept candidate set (run the swept broad phase)");
+    // any candidate set (Candidates::compute_collision_free_stepsize takes
+    // the lists as built, static or swept)
+    if (!c.cand_valid) throw Error(IPCG_ERR_STATE, "no candidate set (run a broad phase)");
     hipStream_t s = c.stream;
     const int64_t n = c.n_ee + c.n_fv + c.n_vv + c.n_ev;
     const bool per_cand = toi_out != nullptr;

@@ -93,6 +93,7 @@ struct Ctx {
     DevBuf ee_other;                  // edge-edge collisions of non-interior types (list)
     DevBuf inc_vtx;                   // stencil vertices per sorted incidence (int4)
     DevBuf hrec;                      // projected Hessian per collision (Q basis)
+    DevBuf gn_out, gn_p;              // Gauss-Newton diagonal (3 nV) and direction (3 nV)
     bool inc_valid = false;
     int64_t nblocks = 0;
     bool pattern_valid = false;
@@ -128,6 +129,11 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
                     double rescale, bool early_exit, double* const* toi_out);
 void bp_edge_face(Ctx& c, double inflation);
+void pot_gn_diagonal(Ctx& c, double dhat, double kappa);
+double pot_gn_quadratic_form(Ctx& c, double dhat, double kappa, const double* p_host);
+// candidate-set helpers (candidates.cpp:294-408) over the current lists
+void cand_safe_distances(Ctx& c, double inflation_radius, double min_distance, double* out);
+double cand_noncandidate_stepsize(Ctx& c, double dhat);
 bool has_intersections(Ctx& c);
 
 } // namespace ipcg

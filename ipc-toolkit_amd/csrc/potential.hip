@@ -124,6 +124,151 @@ __global__ __launch_bounds__(PB) void k_gradient_records(CollView C, const doubl
     }
 }
 
+// ---- Gauss-Newton helpers (normal_potential.cpp:236-308) -----------------
+// The distance vector t = sum_i c_i x_i of a collision, with c the
+// closest-point coefficients of its known distance type
+// (candidates/{vertex_vertex,edge_vertex,edge_edge,face_vertex}.cpp
+// compute_coefficients).  Runtime-size vectors (VectorMax3d) reduce left to
+// right; fixed-size dots use dot().
+__device__ __forceinline__ double dot_seq(const d3& a, const d3& b)
+{
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+
+// closest_point.cpp:11-18
+__device__ __forceinline__ double pe_closest(const d3& p, const d3& e0, const d3& e1)
+{
+    const d3 e = sub(e1, e0);
+    return dot_seq(sub(p, e0), e) / dot_seq(e, e);
+}
+
+template <int KIND>
+__device__ __forceinline__ void stencil_coeffs(int dtype, const d3* P, double* c)
+{
+    if constexpr (KIND == K_VV) {
+        c[0] = 1.0;
+        c[1] = -1.0;
+    } else if constexpr (KIND == K_EV) { // P_E
+        const double a = pe_closest(P[0], P[1], P[2]);
+        c[0] = 1.0;
+        c[1] = -1.0 + a;
+        c[2] = -a;
+    } else if constexpr (KIND == K_FV) { // P_T: closest_point.cpp:123-137
+        const d3 b0 = sub(P[2], P[1]), b1 = sub(P[3], P[1]), r = sub(P[0], P[1]);
+        double u, v;
+        solve_sym2_ldlt(dot(b0, b0), dot(b1, b0), dot(b1, b1), dot(b0, r), dot(b1, r), u, v);
+        c[0] = 1.0;
+        c[1] = -1.0 + u + v;
+        c[2] = -u;
+        c[3] = -v;
+    } else {
+        double a;
+        c[0] = c[1] = c[2] = c[3] = 0.0;
+        switch (dtype) {
+        case EE_EA0_EB0: c[0] = 1; c[2] = -1; break;
+        case EE_EA0_EB1: c[0] = 1; c[3] = -1; break;
+        case EE_EA1_EB0: c[1] = 1; c[2] = -1; break;
+        case EE_EA1_EB1: c[1] = 1; c[3] = -1; break;
+        case EE_EA_EB0:
+            a = pe_closest(P[2], P[0], P[1]);
+            c[0] = 1.0 - a; c[1] = a; c[2] = -1.0;
+            break;
+        case EE_EA_EB1:
+            a = pe_closest(P[3], P[0], P[1]);
+            c[0] = 1.0 - a; c[1] = a; c[3] = -1.0;
+            break;
+        case EE_EA0_EB:
+            a = pe_closest(P[0], P[2], P[3]);
+            c[0] = 1.0; c[2] = -1.0 + a; c[3] = -a;
+            break;
+        case EE_EA1_EB:
+            a = pe_closest(P[1], P[2], P[3]);
+            c[1] = 1.0; c[2] = -1.0 + a; c[3] = -a;
+            break;
+        default: { // closest_point.cpp:65-87
+            const d3 d = sub(P[0], P[2]), ea = sub(P[1], P[0]), eb = sub(P[3], P[2]);
+            double x0, x1;
+            solve_sym2_ldlt(sqn(ea), -dot(eb, ea), sqn(eb), -dot(d, ea), dot(d, eb), x0, x1);
+            c[0] = 1.0 - x0; c[1] = x0; c[2] = -1.0 + x1; c[3] = -x1;
+        }
+        }
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ d3 combine(const double* c, const d3* P)
+{
+    d3 t { 0.0, 0.0, 0.0 };
+#pragma unroll
+    for (int i = 0; i < NP; i++) t = add(t, scale(P[i], c[i]));
+    return t;
+}
+
+// normal_potential.cpp:236-270, records in the gradient's layout
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_gn_diag_records(CollView C, const double* X, int nV,
+                                                        BarrierParams bp, double* rec)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
+    if (i >= C.n[KIND]) return;
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
+    d3 P[4];
+#pragma unroll
+    for (int j = 0; j < NP; j++) P[j] = ldx(X, nV, v[j]);
+    double c[4];
+    stencil_coeffs<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, c);
+    const d3 t = combine<NP>(c, P);
+    const double d2 = dot_seq(t, t);
+    const double w = C.w[KIND][i];
+    const double s1 = w * 4.0 * bar_f2(d2, bp), s2 = w * 2.0 * bar_f1(d2, bp);
+    const double t2[3] = { t.x * t.x, t.y * t.y, t.z * t.z };
+    double* o = rec + (int64_t)3 * NP * i;
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        const double c2 = c[j] * c[j];
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[3 * j + k] = s1 * (c2 * t2[k]) + s2 * c2;
+    }
+}
+
+// normal_potential.cpp:272-308; p: 3 nV RowMajor
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_gn_quadratic_form(CollView C, const double* X, int nV,
+                                                          BarrierParams bp, const double* p,
+                                                          double* partial)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    double e = 0;
+    if (i < C.n[KIND]) {
+        int v[4];
+        stencil_ids<KIND>(C, i, v);
+        d3 P[4], Q[4];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            P[j] = ldx(X, nV, v[j]);
+            Q[j] = { p[3 * (int64_t)v[j]], p[3 * (int64_t)v[j] + 1], p[3 * (int64_t)v[j] + 2] };
+        }
+        double c[4];
+        stencil_coeffs<KIND>(KIND == K_EE ? C.dtype[i] : 0, P, c);
+        const d3 t = combine<NP>(c, P), q = combine<NP>(c, Q);
+        const double d2 = dot_seq(t, t);
+        const double qt = dot_seq(q, t);
+        e = C.w[KIND][i] * (4.0 * bar_f2(d2, bp) * (qt * qt) + 2.0 * bar_f1(d2, bp) * dot_seq(q, q));
+    }
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
+    __shared__ double s[PB / WAVE];
+    if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int j = 0; j < PB / WAVE; j++) t += s[j];
+        partial[blockIdx.x] = t;
+    }
+}
+
 template <int KIND> struct HqTraits {
     static constexpr int NP = StencilTraits<KIND>::NP;
     static constexpr int N = 3 * (NP - 1);
@@ -1015,4 +1160,82 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     c.hess_valid = true;
 }
 
+// NormalPotential::gauss_newton_hessian_diagonal (normal_potential.cpp:
+// 15-46) into c.gn_out (3 nV, RowMajor), summed per vertex in incidence
+// order like the gradient.
+void pot_gn_diagonal(Ctx& c, double dhat, double kappa)
+{
+    require_collisions(c);
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    require_incidences(c, C, G, M);
+    c.gn_out.reserve(sizeof(double) * 3 * (size_t)std::max(c.nV, 1));
+    const double* X = c.X[0].as<double>();
+    const int npk[4] = { 2, 3, 4, 4 };
+    GradRecs R;
+    int64_t nrec = 0;
+    for (int k = 0; k < 4; k++) nrec += 3 * (int64_t)npk[k] * C.n[k];
+    c.hrec.reserve(sizeof(double) * (size_t)std::max<int64_t>(nrec, 1));
+    int64_t ro = 0;
+    for (int k = 0; k < 4; k++) {
+        R.rec[k] = c.hrec.as<double>() + ro;
+        ro += 3 * (int64_t)npk[k] * C.n[k];
+    }
+    c.stage_begin("gn_diagonal");
+    if (C.n[0]) hipLaunchKernelGGL(k_gn_diag_records<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[0]));
+    if (C.n[1]) hipLaunchKernelGGL(k_gn_diag_records<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[1]));
+    if (C.n[2]) hipLaunchKernelGGL(k_gn_diag_records<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[2]));
+    if (C.n[3]) hipLaunchKernelGGL(k_gn_diag_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
+    const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
+    hipLaunchKernelGGL(k_gradient_rows, dim3(gw), dim3(256), 0, s, G, R, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vals.as<uint32_t>(), c.nV, c.gn_out.as<double>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+}
+
+// NormalPotential::gauss_newton_hessian_quadratic_form (normal_potential.cpp:
+// 48-79); p_host: 3 nV RowMajor on the host
+double pot_gn_quadratic_form(Ctx& c, double dhat, double kappa, const double* p_host)
+{
+    require_collisions(c);
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const size_t pb = sizeof(double) * 3 * (size_t)std::max(c.nV, 1);
+    c.gn_p.reserve(pb);
+    if (c.nV)
+        IPCG_HIP(hipMemcpyAsync(c.gn_p.p, p_host, sizeof(double) * 3 * (size_t)c.nV,
+                                hipMemcpyHostToDevice, s));
+    int64_t nb[4], tot = 0;
+    for (int k = 0; k < 4; k++) {
+        nb[k] = (C.n[k] + PB - 1) / PB;
+        tot += nb[k];
+    }
+    c.red_partial.reserve(sizeof(double) * (size_t)std::max<int64_t>(tot, 1));
+    c.red_out.reserve(sizeof(double) * 2);
+    double* part = c.red_partial.as<double>();
+    const double* X = c.X[0].as<double>();
+    const double* p = c.gn_p.as<double>();
+    c.stage_begin("gn_quadratic_form");
+    int64_t o = 0;
+    if (nb[0]) hipLaunchKernelGGL(k_gn_quadratic_form<K_VV>, dim3(nb[0]), dim3(PB), 0, s, C, X, c.nV, bp, p, part + o);
+    o += nb[0];
+    if (nb[1]) hipLaunchKernelGGL(k_gn_quadratic_form<K_EV>, dim3(nb[1]), dim3(PB), 0, s, C, X, c.nV, bp, p, part + o);
+    o += nb[1];
+    if (nb[2]) hipLaunchKernelGGL(k_gn_quadratic_form<K_EE>, dim3(nb[2]), dim3(PB), 0, s, C, X, c.nV, bp, p, part + o);
+    o += nb[2];
+    if (nb[3]) hipLaunchKernelGGL(k_gn_quadratic_form<K_FV>, dim3(nb[3]), dim3(PB), 0, s, C, X, c.nV, bp, p, part + o);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, part, tot, c.red_out.as<double>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    double* h = static_cast<double*>(c.pinned);
+    IPCG_HIP(hipMemcpyAsync(h, c.red_out.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    return h[0];
+}
+
 } // namespace ipcg
+

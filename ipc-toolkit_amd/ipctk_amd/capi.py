@@ -33,6 +33,9 @@ EXPORTS = [
     "ipcg_codim_candidates_download", "ipcg_codim_candidates_toi",
     "ipcg_candidates_is_step_collision_free", "ipcg_counter", "ipcg_mesh_filter",
     "ipcg_edge_face_candidates", "ipcg_edge_face_candidates_download", "ipcg_has_intersections",
+    "ipcg_barrier_gauss_newton_diagonal", "ipcg_barrier_gauss_newton_quadratic_form",
+    "ipcg_candidates_per_vertex_safe_distances",
+    "ipcg_candidates_noncandidate_conservative_stepsize", "ipcg_candidates_cfl_stepsize",
 ]
 
 _lib = None
@@ -97,6 +100,12 @@ def load(path=LIB_PATH):
     L.ipcg_edge_face_candidates.argtypes = [vp, C.c_double, _L]
     L.ipcg_edge_face_candidates_download.argtypes = [vp, _I]
     L.ipcg_has_intersections.argtypes = [vp, C.POINTER(C.c_int)]
+    L.ipcg_barrier_gauss_newton_diagonal.argtypes = [vp, C.c_double, C.c_double, _D]
+    L.ipcg_barrier_gauss_newton_quadratic_form.argtypes = [vp, C.c_double, C.c_double, _D, _D]
+    L.ipcg_candidates_per_vertex_safe_distances.argtypes = [vp, C.c_double, C.c_double, _D]
+    L.ipcg_candidates_noncandidate_conservative_stepsize.argtypes = [vp, C.c_double, _D]
+    L.ipcg_candidates_cfl_stepsize.argtypes = [vp, C.c_double, C.c_double, C.c_double,
+                                               C.c_int64, C.c_double, C.c_int, _D]
     L.ipcg_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), _D, _L]
     L.ipcg_stage_times_reset.argtypes = [vp, C.c_int]
     for name in EXPORTS:
@@ -219,6 +228,27 @@ class Context:
         self._check(self._L.ipcg_has_intersections(self._p, C.byref(r)))
         return bool(r.value)
 
+    def per_vertex_safe_distances(self, inflation_radius, min_distance=0.0):
+        """Candidates::compute_per_vertex_safe_distances over the current lists"""
+        out = np.zeros(self.nV)
+        self._check(self._L.ipcg_candidates_per_vertex_safe_distances(
+            self._p, inflation_radius, min_distance, out.ctypes.data_as(_D)))
+        return out
+
+    def noncandidate_conservative_stepsize(self, dhat):
+        r = C.c_double()
+        self._check(self._L.ipcg_candidates_noncandidate_conservative_stepsize(
+            self._p, dhat, C.byref(r)))
+        return r.value
+
+    def cfl_stepsize(self, dhat, min_distance=0.0, tolerance=1e-6, max_iterations=10_000_000,
+                     rescale=0.8, method=BP_LBVH):
+        r = C.c_double()
+        self._check(self._L.ipcg_candidates_cfl_stepsize(self._p, dhat, min_distance, tolerance,
+                                                         max_iterations, rescale, method,
+                                                         C.byref(r)))
+        return r.value
+
     # ---- collisions --------------------------------------------------------
     def build_collisions(self, dhat, dmin=0.0, method=BP_LBVH):
         self._check(self._L.ipcg_collisions_build(self._p, dhat, dmin, method))
@@ -260,6 +290,23 @@ class Context:
         self._check(self._L.ipcg_barrier_gradient(self._p, dhat, kappa,
                                                   g.ctypes.data_as(_D) if download else None))
         return g
+
+    def gauss_newton_diagonal(self, dhat, kappa):
+        """NormalPotential::gauss_newton_hessian_diagonal (3 nV, RowMajor)"""
+        d = np.zeros(3 * self.nV)
+        self._check(self._L.ipcg_barrier_gauss_newton_diagonal(self._p, dhat, kappa,
+                                                               d.ctypes.data_as(_D)))
+        return d
+
+    def gauss_newton_quadratic_form(self, dhat, kappa, p):
+        """NormalPotential::gauss_newton_hessian_quadratic_form; p: 3 nV RowMajor"""
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        if len(p) != 3 * self.nV:
+            raise ValueError("p must hold 3 * nV values")
+        r = C.c_double()
+        self._check(self._L.ipcg_barrier_gauss_newton_quadratic_form(
+            self._p, dhat, kappa, p.ctypes.data_as(_D), C.byref(r)))
+        return r.value
 
     def hessian(self, dhat, kappa, psd="clamp", download=True):
         nnz = C.c_int64()

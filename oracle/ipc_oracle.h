@@ -248,6 +248,33 @@ int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
                          double kappa, int psd, ipco_csr *out);
 void ipco_csr_free(ipco_csr *m);
 
+/* ---- Gauss-Newton helpers (normal_potential.cpp:15-83, 236-308) ---------- */
+/* distance vector t of collision i (collision order vv, ev, ee, fv) and its
+ * coefficients (collision_stencil.cpp:14-25); returns the point count */
+double ipco_distance_vector(const ipco_collisions *c, const double *V, const int32_t *E,
+                            const int32_t *F, int64_t i, double *t, double *coeffs);
+/* diag: 3*nV RowMajor */
+void ipco_gauss_newton_diagonal(const ipco_collisions *c, const double *V, int nV,
+                                const int32_t *E, const int32_t *F, double dhat,
+                                double kappa, double *diag);
+/* p: 3*nV RowMajor */
+double ipco_gauss_newton_quadratic_form(const ipco_collisions *c, const double *V, int nV,
+                                        const int32_t *E, const int32_t *F, double dhat,
+                                        double kappa, const double *p);
+
+/* ---- candidate-set helpers (candidates.cpp:294-408) ----------------------- */
+/* any list may be NULL; out: nV */
+void ipco_per_vertex_safe_distances(const double *V, int nV, const int32_t *E,
+                                    const int32_t *F, const ipco_pairs *ee,
+                                    const ipco_pairs *fv, const ipco_pairs *vv,
+                                    const ipco_pairs *ev, double inflation_radius,
+                                    double min_distance, double *out);
+double ipco_noncandidate_conservative_stepsize(const double *V0, const double *V1, int nV,
+                                               const int32_t *E, const int32_t *F,
+                                               const ipco_pairs *ee, const ipco_pairs *fv,
+                                               const ipco_pairs *vv, const ipco_pairs *ev,
+                                               double dhat);
+
 /* ---- CCD (tight_inclusion_ccd.cpp:23-336 + Tight-Inclusion BFS) --------- */
 typedef struct ipco_ccd_params {
     double tolerance;        /* 1e-6 */

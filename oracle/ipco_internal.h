@@ -55,6 +55,9 @@ void hd_dot3(const hd *a, const hd *b, hd *r);
 void hd_cross3(const hd *a, const hd *b, hd *r); /* a,b,r arrays of 3 */
 void hd_unpack(const hd *a, int n, double *grad, double *hess);
 
+/* Eigen LDLT<Matrix2d> solve (ipco_primitives.c) */
+void ipco_ldlt2(double a00, double a10, double a11, double b0, double b1, double x[2]);
+
 /* ---- misc ---------------------------------------------------------------- */
 void ipco_sort_u64_pairs(uint64_t *key, int64_t *val, int64_t n); /* stable */
 

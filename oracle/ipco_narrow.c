@@ -499,3 +499,271 @@ void ipco_csr_free(ipco_csr *m)
     free(m->val);
     memset(m, 0, sizeof(*m));
 }
+
+/* ---- Gauss-Newton helpers (normal_potential.cpp:15-83, 236-308) ----------
+ * The distance vector t = sum_i c_i x_i of a collision, with c the
+ * closest-point coefficients of its known distance type
+ * (candidates/{vertex_vertex,edge_vertex,edge_edge,face_vertex}.cpp
+ * compute_coefficients, collision_stencil.cpp:14-25).  Vectors of runtime
+ * size (VectorMax3d) reduce left to right; fixed Vector3d dots use the
+ * unrolled order a0 b0 + (a1 b1 + a2 b2). */
+static double dot_seq(const double *a, const double *b)
+{
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+/* closest_point.cpp:11-18 */
+static double pe_closest(const double *p, const double *e0, const double *e1)
+{
+    double e[3], r[3];
+    v3_sub(e1, e0, e);
+    v3_sub(p, e0, r);
+    return dot_seq(r, e) / dot_seq(e, e);
+}
+
+/* closest_point.cpp:65-87 */
+static void ee_closest(const double *ea0, const double *ea1, const double *eb0,
+                       const double *eb1, double x[2])
+{
+    double d[3], ea[3], eb[3];
+    v3_sub(ea0, eb0, d);
+    v3_sub(ea1, ea0, ea);
+    v3_sub(eb1, eb0, eb);
+    const double a01 = -v3_dot(eb, ea);
+    ipco_ldlt2(v3_sqnorm(ea), a01, v3_sqnorm(eb), -v3_dot(d, ea), v3_dot(d, eb), x);
+}
+
+/* closest_point.cpp:123-137 */
+static void pt_closest(const double *p, const double *t0, const double *t1,
+                       const double *t2, double x[2])
+{
+    double b0[3], b1[3], r[3];
+    v3_sub(t1, t0, b0);
+    v3_sub(t2, t0, b1);
+    v3_sub(p, t0, r);
+    ipco_ldlt2(v3_dot(b0, b0), v3_dot(b1, b0), v3_dot(b1, b1), v3_dot(b0, r),
+               v3_dot(b1, r), x);
+}
+
+static void coefficients(const stencil *s, const double *x, double *c)
+{
+    const double *p0 = x, *p1 = x + 3, *p2 = x + 6, *p3 = x + 9;
+    double a, ab[2];
+    switch (s->kind) {
+    case 3: /* VV */
+        c[0] = 1.0; c[1] = -1.0;
+        return;
+    case 2: /* EV, P_E */
+        a = pe_closest(p0, p1, p2);
+        c[0] = 1.0; c[1] = -1.0 + a; c[2] = -a;
+        return;
+    case 0: /* FV, P_T */
+        pt_closest(p0, p1, p2, p3, ab);
+        c[0] = 1.0; c[1] = -1.0 + ab[0] + ab[1]; c[2] = -ab[0]; c[3] = -ab[1];
+        return;
+    default: /* EE by stored type */
+        switch (s->dtype) {
+        case IPCO_EE_EA0_EB0: c[0] = 1; c[1] = 0; c[2] = -1; c[3] = 0; return;
+        case IPCO_EE_EA0_EB1: c[0] = 1; c[1] = 0; c[2] = 0; c[3] = -1; return;
+        case IPCO_EE_EA1_EB0: c[0] = 0; c[1] = 1; c[2] = -1; c[3] = 0; return;
+        case IPCO_EE_EA1_EB1: c[0] = 0; c[1] = 1; c[2] = 0; c[3] = -1; return;
+        case IPCO_EE_EA_EB0:
+            a = pe_closest(p2, p0, p1);
+            c[0] = 1.0 - a; c[1] = a; c[2] = -1.0; c[3] = 0;
+            return;
+        case IPCO_EE_EA_EB1:
+            a = pe_closest(p3, p0, p1);
+            c[0] = 1.0 - a; c[1] = a; c[2] = 0; c[3] = -1.0;
+            return;
+        case IPCO_EE_EA0_EB:
+            a = pe_closest(p0, p2, p3);
+            c[0] = 1.0; c[1] = 0; c[2] = -1.0 + a; c[3] = -a;
+            return;
+        case IPCO_EE_EA1_EB:
+            a = pe_closest(p1, p2, p3);
+            c[0] = 0; c[1] = 1.0; c[2] = -1.0 + a; c[3] = -a;
+            return;
+        default:
+            ee_closest(p0, p1, p2, p3, ab);
+            c[0] = 1.0 - ab[0]; c[1] = ab[0]; c[2] = -1.0 + ab[1]; c[3] = -ab[1];
+        }
+    }
+}
+
+/* sum_i c_i v_i, accumulated in point order */
+static void combine(const double *c, const double *v, int nv, double *t)
+{
+    t[0] = t[1] = t[2] = 0.0;
+    for (int i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) t[k] += c[i] * v[3 * i + k];
+}
+
+double ipco_distance_vector(const ipco_collisions *c, const double *V, const int32_t *E,
+                            const int32_t *F, int64_t i, double *t, double *coeffs)
+{
+    stencil s;
+    double x[12], cc[4];
+    get_stencil(c, E, F, i, &s);
+    gather(V, &s, x);
+    coefficients(&s, x, cc);
+    combine(cc, x, s.nv, t);
+    if (coeffs)
+        for (int j = 0; j < s.nv; j++) coeffs[j] = cc[j];
+    return s.nv;
+}
+
+void ipco_gauss_newton_diagonal(const ipco_collisions *c, const double *V, int nV,
+                                const int32_t *E, const int32_t *F, double dhat,
+                                double kappa, double *diag)
+{
+    const int64_t n = n_collisions(c);
+    memset(diag, 0, sizeof(double) * 3 * (size_t)nV);
+    double *loc = (double *)malloc(sizeof(double) * 12 * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12], cc[4], t[3];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        coefficients(&s, x, cc);
+        combine(cc, x, s.nv, t);
+        const double d2 = dot_seq(t, t);
+        const double gf = f1(d2, c->dmin, dhat, kappa), hf = f2(d2, c->dmin, dhat, kappa);
+        const double s1 = s.w * 4.0 * hf, s2 = s.w * 2.0 * gf;
+        for (int a = 0; a < s.nv; a++)
+            for (int k = 0; k < 3; k++) {
+                const double c2 = cc[a] * cc[a];
+                loc[12 * i + 3 * a + k] = s1 * (c2 * (t[k] * t[k])) + s2 * c2;
+            }
+    }
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        get_stencil(c, E, F, i, &s);
+        for (int a = 0; a < s.nv; a++)
+            for (int k = 0; k < 3; k++)
+                diag[3 * (size_t)s.ids[a] + k] += loc[12 * i + 3 * a + k];
+    }
+    free(loc);
+}
+
+double ipco_gauss_newton_quadratic_form(const ipco_collisions *c, const double *V, int nV,
+                                        const int32_t *E, const int32_t *F, double dhat,
+                                        double kappa, const double *p)
+{
+    (void)nV;
+    const int64_t n = n_collisions(c);
+    double sum = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sum)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12], px[12], cc[4], t[3], q[3];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        gather(p, &s, px);
+        coefficients(&s, x, cc);
+        combine(cc, x, s.nv, t);
+        combine(cc, px, s.nv, q);
+        const double d2 = dot_seq(t, t);
+        const double gf = f1(d2, c->dmin, dhat, kappa), hf = f2(d2, c->dmin, dhat, kappa);
+        const double qt = dot_seq(q, t);
+        sum += s.w * (4.0 * hf * (qt * qt) + 2.0 * gf * dot_seq(q, q));
+    }
+    return sum;
+}
+
+/* ---- candidate-set helpers (candidates.cpp:294-408) --------------------- */
+typedef struct cand_view {
+    const ipco_pairs *l[4]; /* ee, fv, vv, ev */
+} cand_view;
+
+/* vertex ids of candidate j of list k: EE (ea0, ea1, eb0, eb1), FV
+ * (v, f0, f1, f2), VV (a, b), EV (v, e0, e1) */
+static int cand_vertices(const int32_t *E, const int32_t *F, int k, const int32_t *ij, int *v)
+{
+    switch (k) {
+    case 0:
+        v[0] = E[2 * (size_t)ij[0]]; v[1] = E[2 * (size_t)ij[0] + 1];
+        v[2] = E[2 * (size_t)ij[1]]; v[3] = E[2 * (size_t)ij[1] + 1];
+        return 4;
+    case 1:
+        v[0] = ij[1];
+        v[1] = F[3 * (size_t)ij[0]]; v[2] = F[3 * (size_t)ij[0] + 1];
+        v[3] = F[3 * (size_t)ij[0] + 2];
+        return 4;
+    case 2:
+        v[0] = ij[0]; v[1] = ij[1];
+        return 2;
+    default:
+        v[0] = ij[1];
+        v[1] = E[2 * (size_t)ij[0]]; v[2] = E[2 * (size_t)ij[0] + 1];
+        return 3;
+    }
+}
+
+/* CollisionStencil::compute_distance with the type left to be classified */
+static double cand_dist2(const double *V, int k, const int *v)
+{
+    const double *a = V + 3 * (size_t)v[0], *b = V + 3 * (size_t)v[1];
+    switch (k) {
+    case 0:
+        return ipco_edge_edge_distance(a, b, V + 3 * (size_t)v[2], V + 3 * (size_t)v[3], -1);
+    case 1:
+        return ipco_point_triangle_distance(a, b, V + 3 * (size_t)v[2], V + 3 * (size_t)v[3], -1);
+    case 2:
+        return ipco_point_point_distance(a, b);
+    default:
+        return ipco_point_edge_distance(a, b, V + 3 * (size_t)v[2], -1);
+    }
+}
+
+void ipco_per_vertex_safe_distances(const double *V, int nV, const int32_t *E,
+                                    const int32_t *F, const ipco_pairs *ee,
+                                    const ipco_pairs *fv, const ipco_pairs *vv,
+                                    const ipco_pairs *ev, double inflation_radius,
+                                    double min_distance, double *out)
+{
+    const ipco_pairs *L[4] = { ee, fv, vv, ev };
+    for (int i = 0; i < nV; i++) out[i] = inflation_radius - min_distance;
+    for (int k = 0; k < 4; k++) {
+        if (!L[k]) continue;
+        for (int64_t j = 0; j < L[k]->n; j++) {
+            int v[4];
+            const int nv = cand_vertices(E, F, k, L[k]->ij + 2 * j, v);
+            const double d = sqrt(cand_dist2(V, k, v)) - min_distance;
+            for (int a = 0; a < nv; a++)
+                if (d < out[v[a]]) out[v[a]] = d;
+        }
+    }
+}
+
+double ipco_noncandidate_conservative_stepsize(const double *V0, const double *V1, int nV,
+                                               const int32_t *E, const int32_t *F,
+                                               const ipco_pairs *ee, const ipco_pairs *fv,
+                                               const ipco_pairs *vv, const ipco_pairs *ev,
+                                               double dhat)
+{
+    const ipco_pairs *L[4] = { ee, fv, vv, ev };
+    int64_t total = 0;
+    for (int k = 0; k < 4; k++) total += L[k] ? L[k]->n : 0;
+    if (total == 0) return 1.0;
+    unsigned char *mark = (unsigned char *)calloc((size_t)(nV > 0 ? nV : 1), 1);
+    for (int k = 0; k < 4; k++) {
+        if (!L[k]) continue;
+        for (int64_t j = 0; j < L[k]->n; j++) {
+            int v[4];
+            const int nv = cand_vertices(E, F, k, L[k]->ij + 2 * j, v);
+            for (int a = 0; a < nv; a++) mark[v[a]] = 1;
+        }
+    }
+    double mx = 0.0;
+    for (int i = 0; i < nV; i++) {
+        if (!mark[i]) continue;
+        double d[3];
+        v3_sub(V1 + 3 * (size_t)i, V0 + 3 * (size_t)i, d);
+        const double nrm = sqrt(dot_seq(d, d));
+        mx = mx < nrm ? nrm : mx;
+    }
+    free(mark);
+    return 0.5 * dhat / mx;
+}
+

@@ -162,6 +162,11 @@ static void ldlt2_solve(double a00, double a10, double a11, double b0,
     x[1] = swap ? y0 : y1;
 }
 
+void ipco_ldlt2(double a00, double a10, double a11, double b0, double b1, double x[2])
+{
+    ldlt2_solve(a00, a10, a11, b0, b1, x);
+}
+
 /* One edge of point_triangle_distance_type: basis rows (e, e x n),
  * param = (B B^T).ldlt().solve(B (p - a)). */
 static void pt_edge_param(const double p[3], const double a[3],

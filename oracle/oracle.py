@@ -152,6 +152,21 @@ def lib():
         L.ipco_world_bbox_diagonal.restype = C.c_double
         L.ipco_has_intersections.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I,
                                              C.POINTER(C.c_int)]
+        L.ipco_distance_vector.argtypes = [C.POINTER(_Collisions), _D, _I, _I, C.c_int64, _D,
+                                           _D]
+        L.ipco_distance_vector.restype = C.c_double
+        L.ipco_gauss_newton_diagonal.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
+                                                 C.c_double, C.c_double, _D]
+        L.ipco_gauss_newton_quadratic_form.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I,
+                                                       _I, C.c_double, C.c_double, _D]
+        L.ipco_gauss_newton_quadratic_form.restype = C.c_double
+        L.ipco_per_vertex_safe_distances.argtypes = [
+            _D, C.c_int, _I, _I, C.POINTER(_Pairs), C.POINTER(_Pairs), C.POINTER(_Pairs),
+            C.POINTER(_Pairs), C.c_double, C.c_double, _D]
+        L.ipco_noncandidate_conservative_stepsize.argtypes = [
+            _D, _D, C.c_int, _I, _I, C.POINTER(_Pairs), C.POINTER(_Pairs), C.POINTER(_Pairs),
+            C.POINTER(_Pairs), C.c_double]
+        L.ipco_noncandidate_conservative_stepsize.restype = C.c_double
         L.ipco_set_num_threads.argtypes = [C.c_int]
         L.ipco_get_max_threads.restype = C.c_int
         _lib = L
@@ -601,6 +616,78 @@ def compute_collision_free_stepsize(V0, V1, E, F, min_distance=0.0, params=DEFAU
     return lib().ipco_compute_collision_free_stepsize(p0, p1, len(V0), pE, len(E), pF,
                                                       len(F), min_distance,
                                                       C.byref(params), int(running_tmax))
+
+
+def distance_vector(coll, V, E, F, i):
+    """(t, coefficients) of collision i (order vv, ev, ee, fv)"""
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    t = np.zeros(3)
+    c = np.zeros(4)
+    n = int(lib().ipco_distance_vector(C.byref(coll._s), pV, pE, pF, i, t.ctypes.data_as(_D),
+                                       c.ctypes.data_as(_D)))
+    return t, c[:n]
+
+
+def gauss_newton_diagonal(coll, V, E, F, dhat, kappa):
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    g = np.zeros(3 * len(V))
+    lib().ipco_gauss_newton_diagonal(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa,
+                                     g.ctypes.data_as(_D))
+    return g
+
+
+def gauss_newton_quadratic_form(coll, V, E, F, dhat, kappa, p):
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    p, pp = _d(np.asarray(p, dtype=np.float64).reshape(-1))
+    return lib().ipco_gauss_newton_quadratic_form(C.byref(coll._s), pV, len(V), pE, pF, dhat,
+                                                  kappa, pp)
+
+
+def _lists(ee, fv, vv, ev):
+    z = np.zeros((0, 2), np.int32)
+    return [_pairs_struct(z if a is None else a) for a in (ee, fv, vv, ev)]
+
+
+def per_vertex_safe_distances(V, E, F, ee, fv, inflation_radius, min_distance=0.0, vv=None,
+                              ev=None):
+    """Candidates::compute_per_vertex_safe_distances (candidates.cpp:372-408)"""
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    L = _lists(ee, fv, vv, ev)
+    out = np.zeros(len(V))
+    lib().ipco_per_vertex_safe_distances(pV, len(V), pE, pF, *[C.byref(x[0]) for x in L],
+                                         inflation_radius, min_distance,
+                                         out.ctypes.data_as(_D))
+    return out
+
+
+def noncandidate_conservative_stepsize(V0, V1, E, F, ee, fv, dhat, vv=None, ev=None):
+    """Candidates::compute_noncandidate_conservative_stepsize (candidates.cpp:294-333)
+    with displacements V1 - V0"""
+    V0, p0 = _d(V0)
+    V1, p1 = _d(V1)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    L = _lists(ee, fv, vv, ev)
+    return lib().ipco_noncandidate_conservative_stepsize(p0, p1, len(V0), pE, pF,
+                                                         *[C.byref(x[0]) for x in L], dhat)
+
+
+def cfl_stepsize(V0, V1, E, F, ee, fv, dhat, min_distance=0.0, params=DEFAULT_CCD, vv=None,
+                 ev=None):
+    """Candidates::compute_cfl_stepsize (candidates.cpp:335-360)"""
+    a_c = candidates_stepsize(V0, V1, E, F, ee, fv, min_distance, params, vv=vv, ev=ev)
+    a_f = noncandidate_conservative_stepsize(V0, V1, E, F, ee, fv, dhat, vv=vv, ev=ev)
+    if a_f < 0.5 * a_c:
+        return compute_collision_free_stepsize(V0, V1, E, F, min_distance, params)
+    return min(a_c, a_f)
 
 
 def is_step_collision_free(V0, V1, E, F, min_distance=0.0, params=DEFAULT_CCD):

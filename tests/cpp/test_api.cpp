@@ -209,6 +209,34 @@ static void intersections()
     CHECK(has_intersections(mesh, V));
 }
 
+// test_per_vertex_safe_distances.cpp: three points, inflation 10, minimum
+// distance 0.5; then the noncandidate step for a uniform displacement of
+// norm 13/1024 and the CFL step against the full CCD (test_cfl.cpp).
+static void candidate_helpers()
+{
+    using namespace ipc;
+    const MatrixXd V { { -1, 0, 0 }, { 1, 0, 0 }, { 10, 0, 0 } };
+    CollisionMesh mesh(V);
+    Candidates C;
+    C.build(mesh, V, 10.0);
+    CHECK(C.vv_candidates.size() == 3);
+    const std::vector<double> d = C.compute_per_vertex_safe_distances(mesh, V, 10.0, 0.5);
+    CHECK(d.size() == 3 && d[0] == 1.5 && d[1] == 1.5 && d[2] == 8.5);
+    MatrixXd D(3, 3);
+    for (int i = 0; i < 3; i++) {
+        D(i, 0) = 3.0 / 1024;
+        D(i, 1) = 4.0 / 1024;
+        D(i, 2) = 12.0 / 1024;
+    }
+    CHECK(C.compute_noncandidate_conservative_stepsize(mesh, D, 1e-3) == 0.5 * 1e-3 / (13.0 / 1024));
+    MatrixXd V1 = V;
+    V1(0, 0) = 0.5;
+    V1(1, 0) = -0.5; // points 0 and 1 pass through each other
+    const double cfl = C.compute_cfl_stepsize(mesh, V, V1, 1e-3);
+    const double ccd = compute_collision_free_stepsize(mesh, V, V1);
+    CHECK(cfl <= ccd && ccd < 1.0 && cfl > 0.0);
+}
+
 int main()
 {
     try {
@@ -218,6 +246,7 @@ int main()
         vertex_on_plane();
         errors_throw();
         intersections();
+        candidate_helpers();
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());
         failures++;
