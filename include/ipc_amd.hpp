@@ -431,6 +431,10 @@ public:
     std::vector<EdgeEdgeNormalCollision> ee_collisions;
     std::vector<FaceVertexNormalCollision> fv_collisions;
 
+    // normal_collisions.hpp set_use_area_weighting / use_area_weighting
+    void set_use_area_weighting(bool use) { m_area_weighting = use; }
+    bool use_area_weighting() const { return m_area_weighting; }
+
     // normal_collisions.cpp:20-36
     void build(const CollisionMesh& mesh, const MatrixXd& vertices, double dhat, double dmin = 0,
                BroadPhaseMethod bp = BroadPhaseMethod::LBVH)
@@ -438,6 +442,8 @@ public:
         mesh.set_positions(0, vertices);
         mesh.touch_broad_phase();
         mesh.touch_collisions();
+        detail::check(mesh.context(),
+                      ipcg_collisions_set_area_weighting(mesh.context(), m_area_weighting ? 1 : 0));
         detail::check(mesh.context(), ipcg_collisions_build(mesh.context(), dhat, dmin, int(bp)));
         download(mesh);
         built_for = &mesh;
@@ -462,6 +468,7 @@ private:
     const CollisionMesh* built_for = nullptr;
     uint64_t m_generation = 0;
     double m_dmin = 0;
+    bool m_area_weighting = false;
     void download(const CollisionMesh& mesh)
     {
         ipcg_ctx* c = mesh.context();

@@ -142,6 +142,14 @@ int ipcg_has_intersections(ipcg_ctx* ctx, int* intersecting);
  * lists (NormalCollisions::build(candidates, mesh, V, dhat, dmin)). */
 int ipcg_collisions_build(ipcg_ctx* ctx, double dhat, double dmin, int method);
 int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmin);
+/* NormalCollisions::set_use_area_weighting (normal_collisions.hpp,
+ * normal_collisions_builder.cpp:26-336) for the builds that follow: each
+ * candidate's contribution weighs 0.5 (a_i + a_j) (codim vertex pair),
+ * 0.5 a_v (codim edge-vertex), 0.25 (A_ea + A_eb) (edge-edge) or 0.25 a_v
+ * (face-vertex) instead of 1, summed over duplicates, with the mesh's
+ * vertex areas a and edge areas A (CollisionMesh::init_areas,
+ * collision_mesh.cpp:309-376, over the rest positions).  Default off. */
+int ipcg_collisions_set_area_weighting(ipcg_ctx* ctx, int use_area_weighting);
 /* counts[4] = {vv, ev, ee, fv} */
 int ipcg_collisions_count(ipcg_ctx* ctx, int64_t counts[4]);
 /* Canonical order: vv by (min,max); ev by (edge,vertex); ee by

@@ -5,6 +5,7 @@
 #include "geometry.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -169,7 +170,8 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.inc_vals, &c.inc_vals2, &c.inc_ptr, &c.inc_vtx, &c.ee_other, &c.hrec, &c.tr_deep,
                        &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
                        &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3],
-                       &c.cand_ef, &c.isect, &c.gn_out, &c.gn_p };
+                       &c.cand_ef, &c.isect, &c.gn_out, &c.gn_p, &c.v_area, &c.e_area,
+                       &c.np_cw };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
                            &T->root, &T->sbox, &T->leaf })
@@ -266,6 +268,53 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
             if (nV) IPCG_HIP(hipMemcpyAsync(c.patch.p, vertex_patch, sizeof(int32_t) * nV, hipMemcpyHostToDevice, c.stream));
         }
         c.n_dynamic = c.n_codim_filter = -1;
+        // vertex and edge areas over the rest positions (collision_mesh.cpp:
+        // 309-376): a third of the incident faces' areas, else half the
+        // incident edges' lengths (vertices), else 1; a third of the
+        // incident faces' areas, else the edge length (edges)
+        {
+            auto P = [&](int i, int k) {
+                return layout == IPCG_LAYOUT_SOA ? rest[(int64_t)k * nV + i] : rest[3 * (int64_t)i + k];
+            };
+            auto elen = [&](int e) {
+                const int a = edges[2 * e], b = edges[2 * e + 1];
+                const double d0 = P(b, 0) - P(a, 0), d1 = P(b, 1) - P(a, 1), d2 = P(b, 2) - P(a, 2);
+                return std::sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+            };
+            std::vector<double> ve((size_t)nV, -1.0), vf((size_t)nV, -1.0), ea((size_t)nE, -1.0);
+            for (int32_t e = 0; e < nE; e++) {
+                const double l = elen(e);
+                for (int j = 0; j < 2; j++) {
+                    double& x = ve[edges[2 * e + j]];
+                    x = x > 0.0 ? x : 0.0;
+                    x += 0.5 * l;
+                }
+            }
+            for (int32_t f = 0; f < nF; f++) {
+                const int t0 = faces[3 * f], t1 = faces[3 * f + 1], t2 = faces[3 * f + 2];
+                const double a0 = P(t1, 0) - P(t0, 0), a1 = P(t1, 1) - P(t0, 1), a2 = P(t1, 2) - P(t0, 2);
+                const double b0 = P(t2, 0) - P(t0, 0), b1 = P(t2, 1) - P(t0, 1), b2 = P(t2, 2) - P(t0, 2);
+                const double c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
+                const double area = 0.5 * std::sqrt(c0 * c0 + (c1 * c1 + c2 * c2));
+                for (int j = 0; j < 3; j++) {
+                    double& x = vf[faces[3 * f + j]];
+                    x = x > 0.0 ? x : 0.0;
+                    x += area / 3.0;
+                    double& y = ea[f2e[3 * (size_t)f + j]];
+                    y = y > 0.0 ? y : 0.0;
+                    y += area / 3.0;
+                }
+            }
+            std::vector<double> va((size_t)nV);
+            for (int32_t i = 0; i < nV; i++) va[i] = vf[i] < 0 ? (ve[i] < 0 ? 1.0 : ve[i]) : vf[i];
+            for (int32_t e = 0; e < nE; e++)
+                if (ea[e] < 0) ea[e] = elen(e);
+            c.v_area.reserve(sizeof(double) * (size_t)std::max(nV, 1));
+            c.e_area.reserve(sizeof(double) * (size_t)std::max(nE, 1));
+            if (nV) IPCG_HIP(hipMemcpyAsync(c.v_area.p, va.data(), sizeof(double) * nV, hipMemcpyHostToDevice, c.stream));
+            if (nE) IPCG_HIP(hipMemcpyAsync(c.e_area.p, ea.data(), sizeof(double) * nE, hipMemcpyHostToDevice, c.stream));
+            IPCG_HIP(hipStreamSynchronize(c.stream)); // the host vectors go out of scope
+        }
         upload_positions(c, c.rest, rest, layout, false);
         IPCG_HIP(hipStreamSynchronize(c.stream));
         c.has_mesh = true;
@@ -423,6 +472,11 @@ int ipcg_collisions_build(ipcg_ctx* ctx, double dhat, double dmin, int method)
         bp_build(c, false, 0.5 * (dhat + dmin), method);
         np_build(c, dhat, dmin);
     });
+}
+
+int ipcg_collisions_set_area_weighting(ipcg_ctx* ctx, int use_area_weighting)
+{
+    return guard(ctx, [&](Ctx& c) { c.area_weighting = use_area_weighting != 0; });
 }
 
 int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmin)

@@ -52,6 +52,8 @@ struct Ctx {
     bool has_mesh = false, has_patch = false;
     int32_t n_dynamic = -1, n_codim_filter = -1; // optional vertex-pair filters
     DevBuf E, F, F2E, patch, rest;
+    DevBuf v_area, e_area;     // CollisionMesh vertex / edge areas (rest positions)
+    bool area_weighting = false; // NormalCollisions::use_area_weighting
     DevBuf X[2];
     bool has_pos[2] = { false, false };
 
@@ -82,6 +84,7 @@ struct Ctx {
     double coll_dmin = 0;
     int64_t ncoll[4] = { 0, 0, 0, 0 };
     DevBuf c_ids[4], c_w[4], c_eps, c_dtype; // ids: int2 per collision
+    DevBuf np_cw; // contribution weights of the merge (area weighting)
     DevBuf np_keys, np_keys2, np_vals, np_vals2, np_ukeys, np_start, np_tmp, np_raw, np_eec;
 
     // potential

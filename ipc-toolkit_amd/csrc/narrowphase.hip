@@ -30,8 +30,18 @@ struct Streams {
     uint64_t* ev;
     EERec* ee;
     uint64_t* fv;
+    double* vvw; // weights of the vertex-vertex / edge-vertex contributions
+    double* evw; // (area weighting only, else null)
     unsigned long long* cnt; // 4 x SHARDS x SHARD_PAD
     int64_t scap;
+};
+
+// NormalCollisions::use_area_weighting: per-candidate weights
+// (normal_collisions_builder.cpp:42-44, 81, 181-183, 265) from the mesh's
+// vertex areas va and edge areas ea; null va: unit weights.
+struct Areas {
+    const double* va;
+    const double* ea;
 };
 
 __device__ __forceinline__ uint64_t vv_key(int a, int b)
@@ -50,14 +60,17 @@ __device__ __forceinline__ uint64_t pair_key(int a, int b)
 __global__ __launch_bounds__(NP_BLOCK) void k_classify(
     const double* X, const double* R, int nV, const int32_t* E, const int32_t* F,
     const int32_t* F2E, const int2* ee, int64_t n_ee, const int2* fv, int64_t n_fv,
-    const int2* vv, int64_t n_vv, const int2* ev, int64_t n_ev, double active2, Streams S)
+    const int2* vv, int64_t n_vv, const int2* ev, int64_t n_ev, double active2, Streams S,
+    Areas A)
 {
     const int64_t i = (int64_t)xcd_block() * NP_BLOCK + threadIdx.x;
     int kind = -1; // 0 vv, 1 ev, 2 ee, 3 fv
     uint64_t key = 0;
+    double w = 1.0; // weight of this candidate's contribution
     EERec er { 0, 0, 0, 0, 0.0 };
     if (i < n_ee) {
         const int2 c = ee[i];
+        if (A.va) w = 0.25 * (A.ea[c.x] + A.ea[c.y]);
         const int a0 = E[2 * (int64_t)c.x], a1 = E[2 * (int64_t)c.x + 1];
         const int b0 = E[2 * (int64_t)c.y], b1 = E[2 * (int64_t)c.y + 1];
         const d3 pa0 = ldx(X, nV, a0), pa1 = ldx(X, nV, a1), pb0 = ldx(X, nV, b0),
@@ -86,6 +99,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
     } else if (i < n_ee + n_fv) {
         const int2 c = fv[i - n_ee];
         const int fi = c.x, vi = c.y;
+        if (A.va) w = 0.25 * A.va[vi];
         const int f0 = F[3 * (int64_t)fi], f1 = F[3 * (int64_t)fi + 1],
                   f2 = F[3 * (int64_t)fi + 2];
         const d3 p = ldx(X, nV, vi), t0 = ldx(X, nV, f0), t1 = ldx(X, nV, f1),
@@ -105,6 +119,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
         }
     } else if (i < n_ee + n_fv + n_vv) { // normal_collisions_builder.cpp:26-54
         const int2 c = vv[i - n_ee - n_fv];
+        if (A.va) w = 0.5 * (A.va[c.x] + A.va[c.y]);
         if (sqn(sub(ldx(X, nV, c.x), ldx(X, nV, c.y))) < active2) {
             kind = 0;
             key = vv_key(c.x, c.y);
@@ -112,6 +127,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
     } else if (i < n_ee + n_fv + n_vv + n_ev) { // normal_collisions_builder.cpp:56-138
         const int2 c = ev[i - n_ee - n_fv - n_vv];
         const int ei = c.x, vi = c.y;
+        if (A.va) w = 0.5 * A.va[vi];
         const int e0 = E[2 * (int64_t)ei], e1 = E[2 * (int64_t)ei + 1];
         const d3 p = ldx(X, nV, vi), q0 = ldx(X, nV, e0), q1 = ldx(X, nV, e1);
         const int dt = point_edge_type(p, q0, q1);
@@ -132,8 +148,13 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
         const int64_t slot = (int64_t)shard_append_bits(Sk, sh, kind == k, false, pre) + pre;
         if (kind == k && slot < S.scap) {
             const int64_t at = sh * S.scap + slot;
-            if (k == 0) S.vv[at] = key;
-            else if (k == 1) S.ev[at] = key;
+            if (k == 0) {
+                S.vv[at] = key;
+                if (S.vvw) S.vvw[at] = w;
+            } else if (k == 1) {
+                S.ev[at] = key;
+                if (S.evw) S.evw[at] = w;
+            }
             else if (k == 2) S.ee[at] = er;
             else S.fv[at] = key;
         }
@@ -155,25 +176,41 @@ __global__ void k_ee_keys(const EERec* r, int64_t n, uint64_t* keys, uint32_t* v
     }
 }
 
-__global__ void k_ee_gather(const EERec* r, const uint32_t* perm, int64_t n, int2* ids, double* w,
-                            double* eps, int32_t* dtype)
+__global__ void k_ee_gather(const EERec* r, const uint32_t* perm, int64_t n, Areas A, int2* ids,
+                            double* w, double* eps, int32_t* dtype)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) {
         const EERec e = r[perm[i]];
         ids[i] = make_int2(e.ea, e.eb);
-        w[i] = 1.0;
+        w[i] = A.va ? 0.25 * (A.ea[e.ea] + A.ea[e.eb]) : 1.0;
         eps[i] = e.eps_x;
         dtype[i] = e.dtype;
     }
 }
 
-__global__ void k_keys_to_pairs(const uint64_t* keys, int64_t n, int2* ids, double* w)
+__global__ void k_keys_to_pairs(const uint64_t* keys, int64_t n, Areas A, int2* ids, double* w)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) {
-        ids[i] = make_int2((int)(keys[i] >> 32), (int)(keys[i] & 0xFFFFFFFFu));
-        w[i] = 1.0;
+        const int vi = (int)(keys[i] & 0xFFFFFFFFu);
+        ids[i] = make_int2((int)(keys[i] >> 32), vi);
+        w[i] = A.va ? 0.25 * A.va[vi] : 1.0;
+    }
+}
+
+// area weighting: a run's weight is the sum of its contributions' weights
+// in sorted order (the sort is stable over the compacted stream)
+__global__ void k_runs_to_weight_sums(const uint64_t* ukeys, const uint32_t* start,
+                                      const uint32_t* nu, const uint32_t* perm, const double* cw,
+                                      int2* ids, double* w)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < (int64_t)*nu) {
+        ids[i] = make_int2((int)(ukeys[i] >> 32), (int)(ukeys[i] & 0xFFFFFFFFu));
+        double sum = 0.0;
+        for (uint32_t j = start[i]; j < start[i + 1]; j++) sum += cw[perm[j]];
+        w[i] = sum;
     }
 }
 
@@ -208,13 +245,18 @@ void np_build(Ctx& c, double dhat, double dmin)
     const int64_t waves = (n + WAVE - 1) / WAVE;
     const int64_t scap = std::max<int64_t>(WAVE * ((waves + SHARDS - 1) / SHARDS), WAVE);
     const int64_t cap = SHARDS * scap; // entries per stream
-    c.np_raw.reserve((3 * sizeof(uint64_t) + sizeof(EERec)) * (size_t)cap + 256);
+    const bool aw = c.area_weighting;
+    c.np_raw.reserve((3 * sizeof(uint64_t) + sizeof(EERec) + (aw ? 2 * sizeof(double) : 0))
+                         * (size_t)cap + 256);
     char* base = c.np_raw.as<char>();
     Streams S;
     S.vv = reinterpret_cast<uint64_t*>(base);
     S.ev = S.vv + cap;
     S.fv = S.ev + cap;
     S.ee = reinterpret_cast<EERec*>(S.fv + cap);
+    S.vvw = aw ? reinterpret_cast<double*>(S.ee + cap) : nullptr;
+    S.evw = aw ? S.vvw + cap : nullptr;
+    const Areas A { aw ? c.v_area.as<double>() : nullptr, aw ? c.e_area.as<double>() : nullptr };
     c.counters.reserve(sizeof(unsigned long long) * 8);
     c.shard_cnt.reserve(4 * SHARD_CNT_BYTES);
     c.shard_off.reserve(sizeof(int64_t) * 4 * (SHARDS + 1));
@@ -228,7 +270,7 @@ void np_build(Ctx& c, double dhat, double dmin)
                            c.X[0].as<double>(), c.rest.as<double>(), c.nV, c.E.as<int32_t>(),
                            c.F.as<int32_t>(), c.F2E.as<int32_t>(), c.cand_ee.as<int2>(), c.n_ee,
                            c.cand_fv.as<int2>(), c.n_fv, c.cand_vv.as<int2>(), c.n_vv,
-                           c.cand_ev.as<int2>(), c.n_ev, off * off, S);
+                           c.cand_ev.as<int2>(), c.n_ev, off * off, S, A);
     IPCG_CHECK_LAUNCH();
     c.stage_end();
     unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
@@ -275,26 +317,38 @@ void np_build(Ctx& c, double dhat, double dmin)
             hipLaunchKernelGGL(k_ee_keys, dim3(g), dim3(256), 0, s, ee, m, keys, vals);
             radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
                              vbits, c.np_tmp.p, s);
-            hipLaunchKernelGGL(k_ee_gather, dim3(g), dim3(256), 0, s, ee, vals, m,
+            hipLaunchKernelGGL(k_ee_gather, dim3(g), dim3(256), 0, s, ee, vals, m, A,
                                c.c_ids[2].as<int2>(), c.c_w[2].as<double>(), c.c_eps.as<double>(),
                                c.c_dtype.as<int32_t>());
             c.ncoll[2] = m; // each unordered edge pair is one candidate: no duplicates
         } else {
             const uint64_t* src = k == 0 ? S.vv : (k == 1 ? S.ev : S.fv);
             shard_compact(src, scap, kcnt, kc[k].max, keys, koff, s);
+            double* cw = nullptr; // compacted contribution weights (area weighting)
+            if (aw && k < 2) {
+                c.np_cw.reserve(sizeof(double) * (size_t)m);
+                cw = c.np_cw.as<double>();
+                shard_compact(k == 0 ? S.vvw : S.evw, scap, kcnt, kc[k].max, cw, koff, s);
+            }
             hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, s, vals, m);
             radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
                              k == 3 ? fbits : vbits, c.np_tmp.p, s);
             if (k == 3) { // each (face, vertex) candidate is unique
-                hipLaunchKernelGGL(k_keys_to_pairs, dim3(g), dim3(256), 0, s, keys, m,
+                hipLaunchKernelGGL(k_keys_to_pairs, dim3(g), dim3(256), 0, s, keys, m, A,
                                    c.c_ids[3].as<int2>(), c.c_w[3].as<double>());
                 c.ncoll[3] = m;
             } else {
                 run_length_encode(keys, m, c.np_ukeys.as<uint64_t>(), c.np_start.as<uint32_t>(),
                                   nu_dev, c.np_tmp.p, s);
-                hipLaunchKernelGGL(k_runs_to_weights, dim3(g), dim3(256), 0, s,
-                                   c.np_ukeys.as<uint64_t>(), c.np_start.as<uint32_t>(), nu_dev,
-                                   c.c_ids[k].as<int2>(), c.c_w[k].as<double>());
+                if (cw)
+                    hipLaunchKernelGGL(k_runs_to_weight_sums, dim3(g), dim3(256), 0, s,
+                                       c.np_ukeys.as<uint64_t>(), c.np_start.as<uint32_t>(),
+                                       nu_dev, vals, cw, c.c_ids[k].as<int2>(),
+                                       c.c_w[k].as<double>());
+                else
+                    hipLaunchKernelGGL(k_runs_to_weights, dim3(g), dim3(256), 0, s,
+                                       c.np_ukeys.as<uint64_t>(), c.np_start.as<uint32_t>(), nu_dev,
+                                       c.c_ids[k].as<int2>(), c.c_w[k].as<double>());
                 uint32_t* hu = reinterpret_cast<uint32_t*>(h);
                 IPCG_HIP(hipMemcpyAsync(hu, nu_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
                 IPCG_HIP(hipStreamSynchronize(s));

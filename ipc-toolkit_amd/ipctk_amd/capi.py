@@ -36,6 +36,7 @@ EXPORTS = [
     "ipcg_barrier_gauss_newton_diagonal", "ipcg_barrier_gauss_newton_quadratic_form",
     "ipcg_candidates_per_vertex_safe_distances",
     "ipcg_candidates_noncandidate_conservative_stepsize", "ipcg_candidates_cfl_stepsize",
+    "ipcg_collisions_set_area_weighting",
 ]
 
 _lib = None
@@ -100,6 +101,7 @@ def load(path=LIB_PATH):
     L.ipcg_edge_face_candidates.argtypes = [vp, C.c_double, _L]
     L.ipcg_edge_face_candidates_download.argtypes = [vp, _I]
     L.ipcg_has_intersections.argtypes = [vp, C.POINTER(C.c_int)]
+    L.ipcg_collisions_set_area_weighting.argtypes = [vp, C.c_int]
     L.ipcg_barrier_gauss_newton_diagonal.argtypes = [vp, C.c_double, C.c_double, _D]
     L.ipcg_barrier_gauss_newton_quadratic_form.argtypes = [vp, C.c_double, C.c_double, _D, _D]
     L.ipcg_candidates_per_vertex_safe_distances.argtypes = [vp, C.c_double, C.c_double, _D]
@@ -252,6 +254,10 @@ class Context:
     # ---- collisions --------------------------------------------------------
     def build_collisions(self, dhat, dmin=0.0, method=BP_LBVH):
         self._check(self._L.ipcg_collisions_build(self._p, dhat, dmin, method))
+
+    def set_area_weighting(self, use=True):
+        """NormalCollisions::set_use_area_weighting for the builds that follow"""
+        self._check(self._L.ipcg_collisions_set_area_weighting(self._p, int(bool(use))))
 
     def build_collisions_from_candidates(self, dhat, dmin=0.0):
         self._check(self._L.ipcg_collisions_build_from_candidates(self._p, dhat, dmin))

@@ -223,6 +223,24 @@ int ipco_build_collisions(const double *V, int nV, const double *V_rest,
                           const ipco_pairs *fv_cand, double dhat, double dmin,
                           ipco_collisions *out);
 void ipco_collisions_free(ipco_collisions *c);
+/* NormalCollisions::set_use_area_weighting(true): the same build with
+ * weights 0.5 (a_i + a_j) (codim vertex pairs), 0.5 a_v (codim edge-vertex),
+ * 0.25 (A_ea + A_eb) (edge-edge) and 0.25 a_v (face-vertex) per candidate,
+ * summed over duplicates (normal_collisions_builder.cpp:26-336); areas from
+ * ipco_mesh_areas.  NULL areas: unit weights. */
+int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest,
+                                   const int32_t *E, int nE, const int32_t *F, int nF,
+                                   const int32_t *F2E, const ipco_pairs *vv_cand,
+                                   const ipco_pairs *ev_cand, const ipco_pairs *ee_cand,
+                                   const ipco_pairs *fv_cand, double dhat, double dmin,
+                                   const double *vertex_area, const double *edge_area,
+                                   ipco_collisions *out);
+/* CollisionMesh::init_areas (collision_mesh.cpp:309-376): vertex areas (a
+ * third of the incident faces' areas, else half the incident edges'
+ * lengths, else 1) and edge areas (a third of the incident faces' areas,
+ * else the edge length) over rest positions V */
+void ipco_mesh_areas(const double *V, int nV, const int32_t *E, int nE, const int32_t *F,
+                     int nF, const int32_t *F2E, double *vertex_area, double *edge_area);
 
 /* ---- barrier potential (potential.cpp:36-222, normal_potential.cpp:127-232,
  *      barrier_potential.cpp:62-99) --------------------------------------- */

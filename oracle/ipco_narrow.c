@@ -9,7 +9,57 @@ typedef struct rec {
     int32_t a, b; /* VV (min,max); EV (edge,vertex); EE (ea,eb); FV (f,v) */
     int32_t dtype;
     double eps_x;
+    double w; /* weight of this contribution */
 } rec;
+
+/* ---- mesh areas (collision_mesh.cpp:309-376 init_areas) ------------------
+ * edge_length = |e1 - e0| over runtime-size vectors (left-to-right sum);
+ * triangle_area = 0.5 |(t1 - t0) x (t2 - t0)| (fixed-size, unrolled sum). */
+static double edge_len(const double *V, const int32_t *E, int i)
+{
+    const double *a = V + 3 * (size_t)E[2 * (size_t)i], *b = V + 3 * (size_t)E[2 * (size_t)i + 1];
+    const double d0 = b[0] - a[0], d1 = b[1] - a[1], d2 = b[2] - a[2];
+    return sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+}
+
+void ipco_mesh_areas(const double *V, int nV, const int32_t *E, int nE, const int32_t *F,
+                     int nF, const int32_t *F2E, double *vertex_area, double *edge_area)
+{
+    double *ve = (double *)malloc(sizeof(double) * (size_t)(nV > 0 ? nV : 1));
+    double *vf = (double *)malloc(sizeof(double) * (size_t)(nV > 0 ? nV : 1));
+    for (int i = 0; i < nV; i++) ve[i] = vf[i] = -1.0;
+    for (int i = 0; i < nE; i++) {
+        const double l = edge_len(V, E, i);
+        for (int j = 0; j < 2; j++) {
+            const int v = E[2 * (size_t)i + j];
+            ve[v] = ve[v] > 0.0 ? ve[v] : 0.0;
+            ve[v] += 0.5 * l;
+        }
+    }
+    for (int i = 0; i < nE; i++) edge_area[i] = -1.0;
+    for (int i = 0; i < nF; i++) {
+        double e0[3], e1[3], c[3];
+        const double *t0 = V + 3 * (size_t)F[3 * (size_t)i];
+        v3_sub(V + 3 * (size_t)F[3 * (size_t)i + 1], t0, e0);
+        v3_sub(V + 3 * (size_t)F[3 * (size_t)i + 2], t0, e1);
+        v3_cross(e0, e1, c);
+        const double area = 0.5 * sqrt(v3_sqnorm(c));
+        for (int j = 0; j < 3; j++) {
+            const int v = F[3 * (size_t)i + j];
+            vf[v] = vf[v] > 0.0 ? vf[v] : 0.0;
+            vf[v] += area / 3.0;
+            const int e = F2E[3 * (size_t)i + j];
+            edge_area[e] = edge_area[e] > 0.0 ? edge_area[e] : 0.0;
+            edge_area[e] += area / 3.0;
+        }
+    }
+    for (int i = 0; i < nV; i++)
+        vertex_area[i] = vf[i] < 0 ? (ve[i] < 0 ? 1.0 : ve[i]) : vf[i];
+    for (int i = 0; i < nE; i++)
+        if (edge_area[i] < 0) edge_area[i] = edge_len(V, E, i);
+    free(ve);
+    free(vf);
+}
 
 static void vv_rec(rec *r, int32_t u, int32_t v)
 {
@@ -133,6 +183,18 @@ int ipco_build_collisions(const double *V, int nV, const double *V_rest,
                           const ipco_pairs *fv_cand, double dhat, double dmin,
                           ipco_collisions *out)
 {
+    return ipco_build_collisions_weighted(V, nV, V_rest, E, nE, F, nF, F2E, vv_cand, ev_cand,
+                                          ee_cand, fv_cand, dhat, dmin, NULL, NULL, out);
+}
+
+int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest,
+                                   const int32_t *E, int nE, const int32_t *F, int nF,
+                                   const int32_t *F2E, const ipco_pairs *vv_cand,
+                                   const ipco_pairs *ev_cand, const ipco_pairs *ee_cand,
+                                   const ipco_pairs *fv_cand, double dhat, double dmin,
+                                   const double *vertex_area, const double *edge_area,
+                                   ipco_collisions *out)
+{
     (void)nV;
     (void)nE;
     (void)nF;
@@ -147,24 +209,35 @@ int ipco_build_collisions(const double *V, int nV, const double *V_rest,
     const int64_t n = nfv + nee + nvv + nev;
     rec *R = (rec *)malloc(sizeof(rec) * (size_t)(n > 0 ? n : 1));
     rec *Rc = R + nee + nfv; /* codim candidates after the surface ones */
+    const double *va = vertex_area, *ea = edge_area;
+    const int aw = va != NULL;
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < nvv; i++)
-        vv_candidate(V, vv_cand->ij[2 * i], vv_cand->ij[2 * i + 1], active2, &Rc[i]);
+    for (int64_t i = 0; i < nvv; i++) {
+        const int32_t a = vv_cand->ij[2 * i], b = vv_cand->ij[2 * i + 1];
+        vv_candidate(V, a, b, active2, &Rc[i]);
+        Rc[i].w = aw ? 0.5 * (va[a] + va[b]) : 1.0; /* builder.cpp:42-44 */
+    }
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < nev; i++)
+    for (int64_t i = 0; i < nev; i++) {
         ev_candidate(V, E, ev_cand->ij[2 * i], ev_cand->ij[2 * i + 1], active2,
                      &Rc[nvv + i]);
+        Rc[nvv + i].w = aw ? 0.5 * va[ev_cand->ij[2 * i + 1]] : 1.0; /* :81 */
+    }
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < nee; i++)
-        ee_candidate(V, V_rest, E, ee_cand->ij[2 * i], ee_cand->ij[2 * i + 1],
-                     active2, &R[i]);
+    for (int64_t i = 0; i < nee; i++) {
+        const int32_t a = ee_cand->ij[2 * i], b = ee_cand->ij[2 * i + 1];
+        ee_candidate(V, V_rest, E, a, b, active2, &R[i]);
+        R[i].w = aw ? 0.25 * (ea[a] + ea[b]) : 1.0; /* :181-183 */
+    }
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < nfv; i++)
+    for (int64_t i = 0; i < nfv; i++) {
         fv_candidate(V, F, F2E, fv_cand->ij[2 * i], fv_cand->ij[2 * i + 1],
                      active2, &R[nee + i]);
+        R[nee + i].w = aw ? 0.25 * va[fv_cand->ij[2 * i + 1]] : 1.0; /* :265 */
+    }
 
-    /* per-type sort by key, then merge duplicates summing weights
-     * (normal_collisions_builder.cpp:547-600, unit weights) */
+    /* per-type sort by key, then merge duplicates summing weights in
+     * candidate order (normal_collisions_builder.cpp:547-600) */
     for (int t = 0; t < 4; t++) {
         int64_t m = 0;
         for (int64_t i = 0; i < n; i++) m += R[i].type == t;
@@ -195,13 +268,13 @@ int ipco_build_collisions(const double *V, int nV, const double *V_rest,
                 k++;
                 ids[2 * k] = r->a;
                 ids[2 * k + 1] = r->b;
-                w[k] = 1.0;
+                w[k] = r->w;
                 if (t == T_EE) {
                     eps[k] = r->eps_x;
                     dts[k] = r->dtype;
                 }
             } else {
-                w[k] += 1.0;
+                w[k] += r->w;
             }
         }
         free(key);

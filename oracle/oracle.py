@@ -167,6 +167,11 @@ def lib():
             _D, _D, C.c_int, _I, _I, C.POINTER(_Pairs), C.POINTER(_Pairs), C.POINTER(_Pairs),
             C.POINTER(_Pairs), C.c_double]
         L.ipco_noncandidate_conservative_stepsize.restype = C.c_double
+        L.ipco_build_collisions_weighted.argtypes = [
+            _D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I, C.POINTER(_Pairs), C.POINTER(_Pairs),
+            C.POINTER(_Pairs), C.POINTER(_Pairs), C.c_double, C.c_double, _D, _D,
+            C.POINTER(_Collisions)]
+        L.ipco_mesh_areas.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, _D, _D]
         L.ipco_set_num_threads.argtypes = [C.c_int]
         L.ipco_get_max_threads.restype = C.c_int
         _lib = L
@@ -493,8 +498,21 @@ class Collisions:
             pass
 
 
+def mesh_areas(V_rest, E, F):
+    """(vertex areas, edge areas) of CollisionMesh::init_areas"""
+    V, pV = _d(V_rest)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    F2E, pF2E = _i(faces_to_edges(E, F) if len(F) else np.zeros((0, 3), np.int32))
+    va = np.zeros(len(V))
+    ea = np.zeros(max(len(E), 1))
+    lib().ipco_mesh_areas(pV, len(V), pE, len(E), pF, len(F), pF2E, va.ctypes.data_as(_D),
+                          ea.ctypes.data_as(_D))
+    return va, ea[:len(E)]
+
+
 def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_cand=None,
-                     ev_cand=None):
+                     ev_cand=None, area_weighting=False):
     V, pV = _d(V)
     if V_rest is None:
         V_rest = V
@@ -503,14 +521,19 @@ def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_
     F, pF = _i(np.asarray(F).reshape(-1, 3))
     F2E = faces_to_edges(E, F) if len(F) else np.zeros((0, 3), np.int32)
     F2E, pF2E = _i(F2E)
+    pva = pea = None
+    if area_weighting:
+        va, ea = mesh_areas(V_rest, E, F)
+        va, pva = _d(va)
+        ea, pea = _d(ea if len(ea) else np.zeros(1))
     ee_s, ee_a = _pairs_struct(ee_cand)
     fv_s, fv_a = _pairs_struct(fv_cand)
     vv_s, vv_a = _pairs_struct(np.zeros((0, 2)) if vv_cand is None else vv_cand)
     ev_s, ev_a = _pairs_struct(np.zeros((0, 2)) if ev_cand is None else ev_cand)
     out = _Collisions()
-    lib().ipco_build_collisions(pV, len(V), pR, pE, len(E), pF, len(F), pF2E,
-                                C.byref(vv_s), C.byref(ev_s), C.byref(ee_s), C.byref(fv_s),
-                                dhat, dmin, C.byref(out))
+    lib().ipco_build_collisions_weighted(pV, len(V), pR, pE, len(E), pF, len(F), pF2E,
+                                         C.byref(vv_s), C.byref(ev_s), C.byref(ee_s),
+                                         C.byref(fv_s), dhat, dmin, pva, pea, C.byref(out))
     return Collisions(out, (E, F))
 
 

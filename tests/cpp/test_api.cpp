@@ -237,6 +237,27 @@ static void candidate_helpers()
     CHECK(cfl <= ccd && ccd < 1.0 && cfl > 0.0);
 }
 
+// Area weighting (normal_collisions_builder.cpp:26-54): isolated points
+// weigh 1 each, so a point pair weighs 0.5 (1 + 1) = 1; a vertex hovering over
+// a unit right triangle contributes 0.25 of its area (1 for a point with no
+// edge or face) to the face-vertex collision.
+static void area_weighting()
+{
+    using namespace ipc;
+    const MatrixXd V { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0.2, 0.2, 5e-4 } };
+    const MatrixXi F { { 0, 1, 2 } };
+    const MatrixXi E { { 0, 1 }, { 0, 2 }, { 1, 2 } };
+    CollisionMesh mesh(V, E, F);
+    NormalCollisions C;
+    C.set_use_area_weighting(true);
+    C.build(mesh, V, 1e-3);
+    CHECK(C.fv_collisions.size() == 1 && C.size() == 1);
+    CHECK(C.fv_collisions[0].weight == 0.25);
+    C.set_use_area_weighting(false);
+    C.build(mesh, V, 1e-3);
+    CHECK(C.fv_collisions.size() == 1 && C.fv_collisions[0].weight == 1.0);
+}
+
 int main()
 {
     try {
@@ -247,6 +268,7 @@ int main()
         errors_throw();
         intersections();
         candidate_helpers();
+        area_weighting();
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());
         failures++;
