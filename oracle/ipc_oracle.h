@@ -223,6 +223,23 @@ int ipco_build_collisions(const double *V, int nV, const double *V_rest,
                           const ipco_pairs *fv_cand, double dhat, double dmin,
                           ipco_collisions *out);
 void ipco_collisions_free(ipco_collisions *c);
+/* NormalCollisions::build with collision_set_type IMPROVED_MAX_APPROX when
+ * `improved` (normal_collisions.cpp:84-128): the candidate conversions of
+ * candidates.cpp:583-692 and the signed vertex-vertex / edge-vertex /
+ * mollified edge-edge contributions of normal_collisions_builder.cpp:340-543
+ * over the mesh adjacencies of collision_mesh.cpp:247-291, merged with the
+ * ordinary ones.  Duplicate weights are summed in ascending order of value;
+ * an edge-edge collision keeps the orientation and eps_x of the candidate's
+ * contribution, else of the one listing the lower edge id first; collisions
+ * (other than face-vertex) whose weights sum to zero are dropped.  NULL
+ * areas: unit weights.  improved = 0 is ipco_build_collisions_weighted. */
+int ipco_build_collisions_ex(const double *V, int nV, const double *V_rest, const int32_t *E,
+                             int nE, const int32_t *F, int nF, const int32_t *F2E,
+                             const ipco_pairs *vv_cand, const ipco_pairs *ev_cand,
+                             const ipco_pairs *ee_cand, const ipco_pairs *fv_cand, double dhat,
+                             double dmin, const double *vertex_area, const double *edge_area,
+                             int improved, ipco_collisions *out);
+
 /* NormalCollisions::set_use_area_weighting(true): the same build with
  * weights 0.5 (a_i + a_j) (codim vertex pairs), 0.5 a_v (codim edge-vertex),
  * 0.25 (A_ea + A_eb) (edge-edge) and 0.25 a_v (face-vertex) per candidate,

@@ -10,6 +10,7 @@ typedef struct rec {
     int32_t dtype;
     double eps_x;
     double w; /* weight of this contribution */
+    int32_t rank; /* EE representative order: 0 candidate, 1 (min,max), 2 (max,min) */
 } rec;
 
 /* ---- mesh areas (collision_mesh.cpp:309-376 init_areas) ------------------
@@ -195,9 +196,338 @@ int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest
                                    const double *vertex_area, const double *edge_area,
                                    ipco_collisions *out)
 {
-    (void)nV;
-    (void)nE;
-    (void)nF;
+    return ipco_build_collisions_ex(V, nV, V_rest, E, nE, F, nF, F2E, vv_cand, ev_cand, ee_cand,
+                                    fv_cand, dhat, dmin, vertex_area, edge_area, 0, out);
+}
+
+/* ---- mesh adjacencies (collision_mesh.cpp:247-291 init_adjacencies) -------
+ * Sorted, duplicate-free lists in CSR form. */
+typedef struct adjl {
+    int64_t *off;
+    int32_t *id;
+} adjl;
+
+static int cmp_i32(const void *a, const void *b)
+{
+    const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
+}
+
+/* rows[k] -> vals[k] for k < m, n rows */
+static void adj_build(int n, int64_t m, const int32_t *rows, const int32_t *vals, adjl *A)
+{
+    int64_t *cnt = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t k = 0; k < m; k++) cnt[rows[k] + 1]++;
+    for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    int32_t *tmp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) fill[i] = cnt[i];
+    for (int64_t k = 0; k < m; k++) tmp[fill[rows[k]]++] = vals[k];
+    A->off = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    A->id = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    int64_t at = 0;
+    for (int i = 0; i < n; i++) {
+        A->off[i] = at;
+        const int64_t s0 = cnt[i], s1 = cnt[i + 1];
+        qsort(tmp + s0, (size_t)(s1 - s0), sizeof(int32_t), cmp_i32);
+        for (int64_t k = s0; k < s1; k++)
+            if (k == s0 || tmp[k] != tmp[k - 1]) A->id[at++] = tmp[k];
+    }
+    A->off[n] = at;
+    free(cnt);
+    free(tmp);
+    free(fill);
+}
+
+static int64_t adj_deg(const adjl *A, int i)
+{
+    return A->off[i + 1] - A->off[i];
+}
+
+static int adj_has(const adjl *A, int i, int32_t v)
+{
+    int64_t lo = A->off[i], hi = A->off[i + 1];
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (A->id[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < A->off[i + 1] && A->id[lo] == v;
+}
+
+typedef struct mesh_adj {
+    adjl vv;      /* vertex -> vertices sharing an edge */
+    adjl ve;      /* vertex -> incident edges */
+    adjl ev;      /* edge -> vertices opposite it in its faces */
+    uint8_t *bnd; /* is_vertex_on_boundary (3D rule) */
+} mesh_adj;
+
+static void mesh_adj_build(int nV, const int32_t *E, int nE, const int32_t *F, int nF,
+                           const int32_t *F2E, mesh_adj *M)
+{
+    const int64_t m = 2 * (int64_t)nE;
+    int32_t *r = (int32_t *)calloc((size_t)(m > 0 ? m : 1), sizeof(int32_t));
+    int32_t *v = (int32_t *)calloc((size_t)(m > 0 ? m : 1), sizeof(int32_t));
+    for (int i = 0; i < nE; i++) {
+        r[2 * i] = E[2 * i];
+        v[2 * i] = E[2 * i + 1];
+        r[2 * i + 1] = E[2 * i + 1];
+        v[2 * i + 1] = E[2 * i];
+    }
+    adj_build(nV, m, r, v, &M->vv);
+    for (int i = 0; i < nE; i++) v[2 * i] = v[2 * i + 1] = i;
+    adj_build(nV, m, r, v, &M->ve);
+    free(r);
+    free(v);
+    const int64_t mf = 3 * (int64_t)nF;
+    r = (int32_t *)malloc(sizeof(int32_t) * (size_t)(mf > 0 ? mf : 1));
+    v = (int32_t *)malloc(sizeof(int32_t) * (size_t)(mf > 0 ? mf : 1));
+    for (int i = 0; i < nF; i++)
+        for (int j = 0; j < 3; j++) {
+            r[3 * i + j] = F2E[3 * i + j];
+            v[3 * i + j] = F[3 * i + (j + 2) % 3];
+        }
+    adj_build(nE, mf, r, v, &M->ev);
+    free(r);
+    free(v);
+    M->bnd = (uint8_t *)malloc((size_t)(nV > 0 ? nV : 1));
+    memset(M->bnd, 1, (size_t)(nV > 0 ? nV : 1));
+    for (int i = 0; i < nE; i++)
+        if (adj_deg(&M->ev, i) >= 2) M->bnd[E[2 * i]] = M->bnd[E[2 * i + 1]] = 0;
+}
+
+static void mesh_adj_free(mesh_adj *M)
+{
+    free(M->vv.off);
+    free(M->vv.id);
+    free(M->ve.off);
+    free(M->ve.id);
+    free(M->ev.off);
+    free(M->ev.id);
+    free(M->bnd);
+}
+
+/* ---- IMPROVED_MAX_APPROX (normal_collisions.cpp:84-128) ------------------ */
+
+typedef struct recvec {
+    rec *r;
+    int64_t n, cap;
+} recvec;
+
+static rec *recvec_push(recvec *R)
+{
+    if (R->n == R->cap) {
+        R->cap = R->cap ? 2 * R->cap : 64;
+        R->r = (rec *)realloc(R->r, sizeof(rec) * (size_t)R->cap);
+    }
+    rec *x = &R->r[R->n++];
+    memset(x, 0, sizeof(*x));
+    return x;
+}
+
+static void push_vv(recvec *R, int32_t u, int32_t v, double w)
+{
+    rec *x = recvec_push(R);
+    vv_rec(x, u, v);
+    x->w = w;
+}
+
+static void push_ev(recvec *R, int32_t e, int32_t v, double w)
+{
+    rec *x = recvec_push(R);
+    ev_rec(x, e, v);
+    x->w = w;
+}
+
+/* normal_collisions_builder.cpp:116-138: an edge-vertex pair reduced by its
+ * point-edge distance type */
+static void push_ev_typed(recvec *R, const int32_t *E, int32_t ei, int32_t vi, int dt, double w)
+{
+    if (dt == IPCO_PE_P_E0) push_vv(R, vi, E[2 * (size_t)ei], w);
+    else if (dt == IPCO_PE_P_E1) push_vv(R, vi, E[2 * (size_t)ei + 1], w);
+    else push_ev(R, ei, vi, w);
+}
+
+static uint64_t pk(int32_t a, int32_t b)
+{
+    return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+static uint64_t vvk(int32_t a, int32_t b)
+{
+    return a < b ? pk(a, b) : pk(b, a);
+}
+
+/* sort + unique of a key list (candidates.cpp:604-612, 655-657, 686-688) */
+static int64_t unique_keys(uint64_t *k, int64_t n)
+{
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) idx[i] = i;
+    ipco_sort_u64_pairs(k, idx, n);
+    free(idx);
+    int64_t u = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (i == 0 || k[i] != k[i - 1]) k[u++] = k[i];
+    return u;
+}
+
+static void improved_records(const double *V, const double *Vr, const int32_t *E, int nE,
+                             const int32_t *F, int nF, const int32_t *F2E, int nV,
+                             const ipco_pairs *ev_cand, const ipco_pairs *ee_cand,
+                             const ipco_pairs *fv_cand, double active2, const double *va,
+                             const double *ea, recvec *R)
+{
+    mesh_adj M;
+    mesh_adj_build(nV, E, nE, F, nF, F2E, &M);
+    const int aw = va != NULL;
+    const int64_t nev = ev_cand ? ev_cand->n : 0, nee = ee_cand ? ee_cand->n : 0,
+                  nfv = fv_cand ? fv_cand->n : 0;
+    uint64_t *k;
+    int64_t m;
+
+    /* edge-vertex -> vertex-vertex (candidates.cpp:622-629), negative
+     * vertex-vertex weights (normal_collisions_builder.cpp:340-383) */
+    k = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(2 * nev + 1));
+    m = 0;
+    for (int64_t i = 0; i < nev; i++) {
+        const int32_t ei = ev_cand->ij[2 * i], vi = ev_cand->ij[2 * i + 1];
+        for (int j = 0; j < 2; j++) {
+            const int32_t vj = E[2 * (size_t)ei + j];
+            if (ipco_point_point_distance(vx(V, vi), vx(V, vj)) < active2) k[m++] = vvk(vi, vj);
+        }
+    }
+    m = unique_keys(k, m);
+    for (int64_t i = 0; i < m; i++) {
+        const int32_t u[2] = { (int32_t)(k[i] >> 32), (int32_t)(k[i] & 0xFFFFFFFFu) };
+        double w = 0.0;
+        for (int s = 0; s < 2; s++) {
+            const int32_t a = u[s], b = u[1 - s];
+            const int64_t inc = adj_deg(&M.vv, b) - adj_has(&M.vv, b, a);
+            if (inc > 1) w += (double)(1 - inc) * (aw ? 0.5 * va[a] : 1.0);
+        }
+        if (w != 0) push_vv(R, u[0], u[1], w);
+    }
+    free(k);
+
+    /* edge-edge -> edge-vertex (candidates.cpp:665-692), negative
+     * edge-vertex weights with mollified edge-edge corrections
+     * (normal_collisions_builder.cpp:463-543) */
+    k = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(4 * nee + 1));
+    m = 0;
+    for (int64_t i = 0; i < nee; i++) {
+        for (int s = 0; s < 2; s++) {
+            const int32_t ei = ee_cand->ij[2 * i + s], ej = ee_cand->ij[2 * i + 1 - s];
+            const int32_t ei0 = E[2 * (size_t)ei], ei1 = E[2 * (size_t)ei + 1];
+            for (int j = 0; j < 2; j++) {
+                const int32_t vj = E[2 * (size_t)ej + j];
+                const double *p = vx(V, vj), *q0 = vx(V, ei0), *q1 = vx(V, ei1);
+                const int dt = ipco_point_edge_distance_type(p, q0, q1);
+                if (ipco_point_edge_distance(p, q0, q1, dt) < active2) k[m++] = pk(ei, vj);
+            }
+        }
+    }
+    m = unique_keys(k, m);
+    for (int64_t i = 0; i < m; i++) {
+        const int32_t a = (int32_t)(k[i] >> 32), p = (int32_t)(k[i] & 0xFFFFFFFFu);
+        const int32_t a0 = E[2 * (size_t)a], a1 = E[2 * (size_t)a + 1];
+        const double w = aw ? -0.25 * ea[a] : -1.0;
+        const int dt = ipco_point_edge_distance_type(vx(V, p), vx(V, a0), vx(V, a1));
+        int64_t nonmollified = 0;
+        for (int64_t t = M.ve.off[p]; t < M.ve.off[p + 1]; t++) {
+            const int32_t b = M.ve.id[t];
+            const int32_t b0 = E[2 * (size_t)b], b1 = E[2 * (size_t)b + 1];
+            const int32_t q = p == b0 ? b1 : b0;
+            if (q == a0 || q == a1) continue;
+            const double eps_x = ipco_edge_edge_mollifier_threshold(vx(Vr, a0), vx(Vr, a1),
+                                                                    vx(Vr, b0), vx(Vr, b1));
+            const double x = ipco_edge_edge_cross_squarednorm(vx(V, a0), vx(V, a1), vx(V, b0),
+                                                              vx(V, b1));
+            if (x >= eps_x) {
+                nonmollified++;
+                continue;
+            }
+            int edt;
+            if (dt == IPCO_PE_P_E0) edt = p == b0 ? IPCO_EE_EA0_EB0 : IPCO_EE_EA0_EB1;
+            else if (dt == IPCO_PE_P_E1) edt = p == b0 ? IPCO_EE_EA1_EB0 : IPCO_EE_EA1_EB1;
+            else edt = p == b0 ? IPCO_EE_EA_EB0 : IPCO_EE_EA_EB1;
+            rec *x_ = recvec_push(R);
+            x_->type = T_EE;
+            x_->a = a;
+            x_->b = b;
+            x_->dtype = edt;
+            x_->eps_x = eps_x;
+            x_->w = w;
+            x_->rank = a < b ? 1 : 2;
+        }
+        if (nonmollified != 1) push_ev_typed(R, E, a, p, dt, (double)(nonmollified - 1) * w);
+    }
+    free(k);
+
+    /* face-vertex -> edge-vertex (candidates.cpp:640-663), negative
+     * edge-vertex weights (normal_collisions_builder.cpp:427-461) */
+    k = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(3 * nfv + 1));
+    m = 0;
+    for (int64_t i = 0; i < nfv; i++) {
+        const int32_t fi = fv_cand->ij[2 * i], vi = fv_cand->ij[2 * i + 1];
+        for (int j = 0; j < 3; j++) {
+            const int32_t ei = F2E[3 * (size_t)fi + j];
+            const double *p = vx(V, vi), *q0 = vx(V, E[2 * (size_t)ei]),
+                         *q1 = vx(V, E[2 * (size_t)ei + 1]);
+            const int dt = ipco_point_edge_distance_type(p, q0, q1);
+            if (ipco_point_edge_distance(p, q0, q1, dt) < active2) k[m++] = pk(ei, vi);
+        }
+    }
+    m = unique_keys(k, m);
+    for (int64_t i = 0; i < m; i++) {
+        const int32_t ei = (int32_t)(k[i] >> 32), vi = (int32_t)(k[i] & 0xFFFFFFFFu);
+        const int64_t inc = adj_deg(&M.ev, ei) - adj_has(&M.ev, ei, vi);
+        if (inc > 1) {
+            const double w = (double)(1 - inc) * (aw ? 0.25 * va[vi] : 1.0);
+            const int dt = ipco_point_edge_distance_type(vx(V, vi), vx(V, E[2 * (size_t)ei]),
+                                                         vx(V, E[2 * (size_t)ei + 1]));
+            push_ev_typed(R, E, ei, vi, dt, w);
+        }
+    }
+
+    /* face-vertex -> vertex-vertex (candidates.cpp:631-638), positive
+     * vertex-vertex weights (normal_collisions_builder.cpp:385-425) */
+    k = (uint64_t *)realloc(k, sizeof(uint64_t) * (size_t)(3 * nfv + 1));
+    m = 0;
+    for (int64_t i = 0; i < nfv; i++) {
+        const int32_t fi = fv_cand->ij[2 * i], vi = fv_cand->ij[2 * i + 1];
+        for (int j = 0; j < 3; j++) {
+            const int32_t vj = F[3 * (size_t)fi + j];
+            if (ipco_point_point_distance(vx(V, vi), vx(V, vj)) < active2) k[m++] = vvk(vi, vj);
+        }
+    }
+    m = unique_keys(k, m);
+    for (int64_t i = 0; i < m; i++) {
+        const int32_t u[2] = { (int32_t)(k[i] >> 32), (int32_t)(k[i] & 0xFFFFFFFFu) };
+        double w = 0.0;
+        for (int s = 0; s < 2; s++) {
+            const int32_t a = u[s], b = u[1 - s];
+            if (M.bnd[b] || adj_has(&M.vv, b, a)) continue;
+            w += aw ? 0.25 * va[a] : 1.0;
+        }
+        if (w != 0) push_vv(R, u[0], u[1], w);
+    }
+    free(k);
+    mesh_adj_free(&M);
+}
+
+static int cmp_f64(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+int ipco_build_collisions_ex(const double *V, int nV, const double *V_rest, const int32_t *E,
+                             int nE, const int32_t *F, int nF, const int32_t *F2E,
+                             const ipco_pairs *vv_cand, const ipco_pairs *ev_cand,
+                             const ipco_pairs *ee_cand, const ipco_pairs *fv_cand, double dhat,
+                             double dmin, const double *vertex_area, const double *edge_area,
+                             int improved, ipco_collisions *out)
+{
     memset(out, 0, sizeof(*out));
     out->dmin = dmin;
     const double off = dmin + dhat;
@@ -206,8 +536,8 @@ int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest
     const int64_t nee = ee_cand ? ee_cand->n : 0;
     const int64_t nvv = vv_cand ? vv_cand->n : 0;
     const int64_t nev = ev_cand ? ev_cand->n : 0;
-    const int64_t n = nfv + nee + nvv + nev;
-    rec *R = (rec *)malloc(sizeof(rec) * (size_t)(n > 0 ? n : 1));
+    const int64_t n0 = nfv + nee + nvv + nev;
+    rec *R = (rec *)calloc((size_t)(n0 > 0 ? n0 : 1), sizeof(rec));
     rec *Rc = R + nee + nfv; /* codim candidates after the surface ones */
     const double *va = vertex_area, *ea = edge_area;
     const int aw = va != NULL;
@@ -235,9 +565,24 @@ int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest
                      active2, &R[nee + i]);
         R[nee + i].w = aw ? 0.25 * va[fv_cand->ij[2 * i + 1]] : 1.0; /* :265 */
     }
+    int64_t n = n0;
+    if (improved) {
+        recvec X = { NULL, 0, 0 };
+        improved_records(V, V_rest, E, nE, F, nF, F2E, nV, ev_cand, ee_cand, fv_cand, active2,
+                         va, ea, &X);
+        R = (rec *)realloc(R, sizeof(rec) * (size_t)(n0 + X.n > 0 ? n0 + X.n : 1));
+        if (X.n) memcpy(R + n0, X.r, sizeof(rec) * (size_t)X.n);
+        n = n0 + X.n;
+        free(X.r);
+    }
 
-    /* per-type sort by key, then merge duplicates summing weights in
-     * candidate order (normal_collisions_builder.cpp:547-600) */
+    /* Per-type sort by key, then merge duplicates (normal_collisions_builder.cpp:
+     * 547-600) summing their weights in ascending order of value; an
+     * edge-edge collision keeps the ids, eps_x and orientation of its
+     * lowest-rank contribution.  Collisions whose weights sum to zero are
+     * dropped (normal_collisions_builder.cpp:660-684). */
+    double *run = NULL;
+    int64_t run_cap = 0;
     for (int t = 0; t < 4; t++) {
         int64_t m = 0;
         for (int64_t i = 0; i < n; i++) m += R[i].type == t;
@@ -261,22 +606,35 @@ int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest
             eps = (double *)malloc(sizeof(double) * (size_t)(u > 0 ? u : 1));
             dts = (int32_t *)malloc(sizeof(int32_t) * (size_t)(u > 0 ? u : 1));
         }
-        int64_t k = -1;
-        for (int64_t i = 0; i < m; i++) {
-            const rec *r = &R[idx[i]];
-            if (i == 0 || key[i] != key[i - 1] || t == T_FV) {
-                k++;
-                ids[2 * k] = r->a;
-                ids[2 * k + 1] = r->b;
-                w[k] = r->w;
-                if (t == T_EE) {
-                    eps[k] = r->eps_x;
-                    dts[k] = r->dtype;
-                }
-            } else {
-                w[k] += r->w;
+        int64_t k = 0;
+        for (int64_t s = 0; s < m;) {
+            int64_t e = s + 1;
+            if (t != T_FV)
+                while (e < m && key[e] == key[s]) e++;
+            if (e - s > run_cap) {
+                run_cap = 2 * (e - s);
+                run = (double *)realloc(run, sizeof(double) * (size_t)run_cap);
             }
+            const rec *rep = &R[idx[s]];
+            for (int64_t i = s; i < e; i++) {
+                run[i - s] = R[idx[i]].w;
+                if (R[idx[i]].rank < rep->rank) rep = &R[idx[i]];
+            }
+            qsort(run, (size_t)(e - s), sizeof(double), cmp_f64);
+            double sum = 0.0;
+            for (int64_t i = 0; i < e - s; i++) sum += run[i];
+            s = e;
+            if (t != T_FV && sum == 0) continue;
+            ids[2 * k] = rep->a;
+            ids[2 * k + 1] = rep->b;
+            w[k] = sum;
+            if (t == T_EE) {
+                eps[k] = rep->eps_x;
+                dts[k] = rep->dtype;
+            }
+            k++;
         }
+        u = k;
         free(key);
         free(idx);
         switch (t) {
@@ -292,6 +650,7 @@ int ipco_build_collisions_weighted(const double *V, int nV, const double *V_rest
         default: out->nfv = u; out->fv = ids; out->fv_w = w; break;
         }
     }
+    free(run);
     free(R);
     return 0;
 }

@@ -171,6 +171,10 @@ def lib():
             _D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I, C.POINTER(_Pairs), C.POINTER(_Pairs),
             C.POINTER(_Pairs), C.POINTER(_Pairs), C.c_double, C.c_double, _D, _D,
             C.POINTER(_Collisions)]
+        L.ipco_build_collisions_ex.argtypes = [
+            _D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I, C.POINTER(_Pairs), C.POINTER(_Pairs),
+            C.POINTER(_Pairs), C.POINTER(_Pairs), C.c_double, C.c_double, _D, _D, C.c_int,
+            C.POINTER(_Collisions)]
         L.ipco_mesh_areas.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, _D, _D]
         L.ipco_set_num_threads.argtypes = [C.c_int]
         L.ipco_get_max_threads.restype = C.c_int
@@ -512,7 +516,9 @@ def mesh_areas(V_rest, E, F):
 
 
 def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_cand=None,
-                     ev_cand=None, area_weighting=False):
+                     ev_cand=None, area_weighting=False, improved=False):
+    """NormalCollisions::build over the given candidates; area_weighting and
+    improved (CollisionSetType::IMPROVED_MAX_APPROX) select the variants."""
     V, pV = _d(V)
     if V_rest is None:
         V_rest = V
@@ -531,9 +537,9 @@ def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_
     vv_s, vv_a = _pairs_struct(np.zeros((0, 2)) if vv_cand is None else vv_cand)
     ev_s, ev_a = _pairs_struct(np.zeros((0, 2)) if ev_cand is None else ev_cand)
     out = _Collisions()
-    lib().ipco_build_collisions_weighted(pV, len(V), pR, pE, len(E), pF, len(F), pF2E,
-                                         C.byref(vv_s), C.byref(ev_s), C.byref(ee_s),
-                                         C.byref(fv_s), dhat, dmin, pva, pea, C.byref(out))
+    lib().ipco_build_collisions_ex(pV, len(V), pR, pE, len(E), pF, len(F), pF2E, C.byref(vv_s),
+                                   C.byref(ev_s), C.byref(ee_s), C.byref(fv_s), dhat, dmin, pva,
+                                   pea, int(bool(improved)), C.byref(out))
     return Collisions(out, (E, F))
 
 
