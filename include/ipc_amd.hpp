@@ -435,6 +435,12 @@ public:
     void set_use_area_weighting(bool use) { m_area_weighting = use; }
     bool use_area_weighting() const { return m_area_weighting; }
 
+    // normal_collisions.hpp CollisionSetType (OGC is not provided)
+    enum class CollisionSetType { IPC = IPCG_COLLISIONS_IPC,
+                                  IMPROVED_MAX_APPROX = IPCG_COLLISIONS_IMPROVED_MAX_APPROX };
+    void set_collision_set_type(CollisionSetType type) { m_type = type; }
+    CollisionSetType collision_set_type() const { return m_type; }
+
     // normal_collisions.cpp:20-36
     void build(const CollisionMesh& mesh, const MatrixXd& vertices, double dhat, double dmin = 0,
                BroadPhaseMethod bp = BroadPhaseMethod::LBVH)
@@ -444,6 +450,7 @@ public:
         mesh.touch_collisions();
         detail::check(mesh.context(),
                       ipcg_collisions_set_area_weighting(mesh.context(), m_area_weighting ? 1 : 0));
+        detail::check(mesh.context(), ipcg_collisions_set_type(mesh.context(), int(m_type)));
         detail::check(mesh.context(), ipcg_collisions_build(mesh.context(), dhat, dmin, int(bp)));
         download(mesh);
         built_for = &mesh;
@@ -469,6 +476,7 @@ private:
     uint64_t m_generation = 0;
     double m_dmin = 0;
     bool m_area_weighting = false;
+    CollisionSetType m_type = CollisionSetType::IPC;
     void download(const CollisionMesh& mesh)
     {
         ipcg_ctx* c = mesh.context();

@@ -150,10 +150,30 @@ int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmi
  * vertex areas a and edge areas A (CollisionMesh::init_areas,
  * collision_mesh.cpp:309-376, over the rest positions).  Default off. */
 int ipcg_collisions_set_area_weighting(ipcg_ctx* ctx, int use_area_weighting);
+/* NormalCollisions::set_collision_set_type (normal_collisions.cpp:175-191;
+ * CollisionSetType in normal_collisions.hpp) for the builds that follow.
+ * IPCG_COLLISIONS_IMPROVED_MAX_APPROX (normal_collisions.cpp:84-128) adds,
+ * from the candidates converted to sub-element pairs (candidates.cpp:
+ * 583-692), signed vertex-vertex, edge-vertex and mollified edge-edge
+ * contributions (normal_collisions_builder.cpp:340-543) weighted through
+ * the mesh adjacencies (collision_mesh.cpp:247-291).  Duplicates merge by
+ * (vv, ev) pair or (ee) edge pair and distance type, summing weights in
+ * ascending order of value; an edge-edge collision keeps the orientation
+ * and eps_x of its candidate's contribution, else of the contribution that
+ * lists the lower edge id first; vertex-vertex, edge-vertex and edge-edge
+ * collisions whose weights sum to zero are dropped
+ * (normal_collisions_builder.cpp:660-684).  The reference warns that the
+ * approximation is meant to be used with area weighting.  OGC is not
+ * supported (IPCG_ERR_ARG).  Default IPCG_COLLISIONS_IPC. */
+enum {
+    IPCG_COLLISIONS_IPC = 0,
+    IPCG_COLLISIONS_IMPROVED_MAX_APPROX = 1
+};
+int ipcg_collisions_set_type(ipcg_ctx* ctx, int collision_set_type);
 /* counts[4] = {vv, ev, ee, fv} */
 int ipcg_collisions_count(ipcg_ctx* ctx, int64_t counts[4]);
 /* Canonical order: vv by (min,max); ev by (edge,vertex); ee by
- * (min edge, max edge) with ids as oriented by the candidate; fv by
+ * (min edge, max edge, distance type) with ids as oriented by the candidate; fv by
  * (face,vertex).  Any pointer may be NULL. */
 int ipcg_collisions_download(ipcg_ctx* ctx, int32_t* vv, double* vv_w,
                              int32_t* ev, double* ev_w, int32_t* ee,

@@ -171,6 +171,11 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
                        &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3],
                        &c.cand_ef, &c.isect, &c.gn_out, &c.gn_p, &c.v_area, &c.e_area,
+                       &c.adj_vv_off, &c.adj_vv, &c.adj_ve_off, &c.adj_ve, &c.adj_ev_off,
+                       &c.adj_ev, &c.adj_bnd, &c.np_rw, &c.np_flag, &c.np_pos, &c.np_rep,
+                       &c.im_slot, &c.im_slot2, &c.im_vals, &c.im_vals2, &c.im_ukeys,
+                       &c.im_start, &c.im_cnt, &c.im_off, &c.im_vv, &c.im_vvw, &c.im_ev,
+                       &c.im_evw, &c.im_ee,
                        &c.np_cw };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
@@ -198,6 +203,75 @@ int ipcg_ctx_synchronize(ipcg_ctx* ctx)
 }
 
 void* ipcg_ctx_stream(ipcg_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+// CSR of rows[k] -> vals[k], each row sorted and duplicate-free
+// (collision_mesh.cpp:235-243 remove_duplicates)
+static void csr_rows(int32_t n, const std::vector<int32_t>& rows, const std::vector<int32_t>& vals,
+                     std::vector<int32_t>& off, std::vector<int32_t>& ids)
+{
+    std::vector<int32_t> cnt((size_t)n + 1, 0);
+    for (int32_t r : rows) cnt[(size_t)r + 1]++;
+    for (int32_t i = 0; i < n; i++) cnt[(size_t)i + 1] += cnt[i];
+    std::vector<int32_t> tmp(rows.size()), fill(cnt.begin(), cnt.end() - 1);
+    for (size_t k = 0; k < rows.size(); k++) tmp[(size_t)fill[rows[k]]++] = vals[k];
+    off.assign((size_t)n + 1, 0);
+    ids.clear();
+    ids.reserve(tmp.size());
+    for (int32_t i = 0; i < n; i++) {
+        off[i] = (int32_t)ids.size();
+        std::sort(tmp.begin() + cnt[i], tmp.begin() + cnt[(size_t)i + 1]);
+        for (int32_t k = cnt[i]; k < cnt[(size_t)i + 1]; k++)
+            if (k == cnt[i] || tmp[k] != tmp[(size_t)k - 1]) ids.push_back(tmp[k]);
+    }
+    off[n] = (int32_t)ids.size();
+}
+
+static void upload_i32(Ctx& c, DevBuf& b, const std::vector<int32_t>& v)
+{
+    b.reserve(sizeof(int32_t) * std::max<size_t>(v.size(), 1));
+    if (!v.empty())
+        IPCG_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(int32_t) * v.size(), hipMemcpyHostToDevice,
+                                c.stream));
+}
+
+// CollisionMesh::init_adjacencies (collision_mesh.cpp:247-291, 3D rule for
+// the boundary flags): used by the IMPROVED_MAX_APPROX collision set
+static void upload_adjacencies(Ctx& c, int32_t nV, const int32_t* edges, int32_t nE,
+                               const int32_t* faces, int32_t nF, const std::vector<int32_t>& f2e)
+{
+    if (2 * (int64_t)nE >= INT32_MAX || 3 * (int64_t)nF >= INT32_MAX)
+        throw Error(IPCG_ERR_ARG, "mesh too large for 32-bit adjacency offsets");
+    std::vector<int32_t> rows(2 * (size_t)nE), vals(2 * (size_t)nE), off, ids;
+    for (int32_t e = 0; e < nE; e++) {
+        rows[2 * (size_t)e] = edges[2 * e];
+        vals[2 * (size_t)e] = edges[2 * e + 1];
+        rows[2 * (size_t)e + 1] = edges[2 * e + 1];
+        vals[2 * (size_t)e + 1] = edges[2 * e];
+    }
+    csr_rows(nV, rows, vals, off, ids);
+    upload_i32(c, c.adj_vv_off, off);
+    upload_i32(c, c.adj_vv, ids);
+    for (int32_t e = 0; e < nE; e++) vals[2 * (size_t)e] = vals[2 * (size_t)e + 1] = e;
+    csr_rows(nV, rows, vals, off, ids);
+    upload_i32(c, c.adj_ve_off, off);
+    upload_i32(c, c.adj_ve, ids);
+    rows.assign(3 * (size_t)nF, 0);
+    vals.assign(3 * (size_t)nF, 0);
+    for (int32_t f = 0; f < nF; f++)
+        for (int j = 0; j < 3; j++) {
+            rows[3 * (size_t)f + j] = f2e[3 * (size_t)f + j];
+            vals[3 * (size_t)f + j] = faces[3 * f + (j + 2) % 3];
+        }
+    csr_rows(nE, rows, vals, off, ids);
+    std::vector<uint8_t> bnd((size_t)std::max(nV, 1), 1);
+    for (int32_t e = 0; e < nE; e++)
+        if (off[(size_t)e + 1] - off[e] >= 2) bnd[edges[2 * e]] = bnd[edges[2 * e + 1]] = 0;
+    upload_i32(c, c.adj_ev_off, off);
+    upload_i32(c, c.adj_ev, ids);
+    c.adj_bnd.reserve(bnd.size());
+    IPCG_HIP(hipMemcpyAsync(c.adj_bnd.p, bnd.data(), bnd.size(), hipMemcpyHostToDevice, c.stream));
+    IPCG_HIP(hipStreamSynchronize(c.stream)); // the host vectors go out of scope
+}
 
 int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, int32_t nE,
                      const int32_t* edges, int32_t nF, const int32_t* faces,
@@ -315,6 +389,7 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
             if (nE) IPCG_HIP(hipMemcpyAsync(c.e_area.p, ea.data(), sizeof(double) * nE, hipMemcpyHostToDevice, c.stream));
             IPCG_HIP(hipStreamSynchronize(c.stream)); // the host vectors go out of scope
         }
+        upload_adjacencies(c, nV, edges, nE, faces, nF, f2e);
         upload_positions(c, c.rest, rest, layout, false);
         IPCG_HIP(hipStreamSynchronize(c.stream));
         c.has_mesh = true;
@@ -477,6 +552,16 @@ int ipcg_collisions_build(ipcg_ctx* ctx, double dhat, double dmin, int method)
 int ipcg_collisions_set_area_weighting(ipcg_ctx* ctx, int use_area_weighting)
 {
     return guard(ctx, [&](Ctx& c) { c.area_weighting = use_area_weighting != 0; });
+}
+
+int ipcg_collisions_set_type(ipcg_ctx* ctx, int collision_set_type)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (collision_set_type != IPCG_COLLISIONS_IPC
+            && collision_set_type != IPCG_COLLISIONS_IMPROVED_MAX_APPROX)
+            throw Error(IPCG_ERR_ARG, "unsupported collision set type");
+        c.coll_type = collision_set_type;
+    });
 }
 
 int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmin)

@@ -54,6 +54,12 @@ struct Ctx {
     DevBuf E, F, F2E, patch, rest;
     DevBuf v_area, e_area;     // CollisionMesh vertex / edge areas (rest positions)
     bool area_weighting = false; // NormalCollisions::use_area_weighting
+    int coll_type = 0;           // NormalCollisions::CollisionSetType: 0 IPC, 1 IMPROVED_MAX_APPROX
+    // CollisionMesh adjacencies (collision_mesh.cpp:247-291), CSR with int32
+    // offsets, rows sorted and duplicate-free: vertex -> vertices sharing an
+    // edge, vertex -> incident edges, edge -> vertices opposite it in its
+    // faces; is_vertex_on_boundary as bytes
+    DevBuf adj_vv_off, adj_vv, adj_ve_off, adj_ve, adj_ev_off, adj_ev, adj_bnd;
     DevBuf X[2];
     bool has_pos[2] = { false, false };
 
@@ -86,6 +92,12 @@ struct Ctx {
     DevBuf c_ids[4], c_w[4], c_eps, c_dtype; // ids: int2 per collision
     DevBuf np_cw; // contribution weights of the merge (area weighting)
     DevBuf np_keys, np_keys2, np_vals, np_vals2, np_ukeys, np_start, np_tmp, np_raw, np_eec;
+    DevBuf np_rw, np_flag, np_pos, np_rep; // signed merge: run sums, kept flags, positions
+    // IMPROVED_MAX_APPROX: conversion slots, their unique keys, per-key
+    // output counts / offsets, and the signed vertex-vertex, edge-vertex and
+    // edge-edge records they add
+    DevBuf im_slot, im_slot2, im_vals, im_vals2, im_ukeys, im_start, im_cnt, im_off;
+    DevBuf im_vv, im_vvw, im_ev, im_evw, im_ee;
 
     // potential
     DevBuf red_partial, red_out, grad, hq;

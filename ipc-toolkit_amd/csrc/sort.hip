@@ -361,9 +361,11 @@ void run_length_encode(const uint64_t* keys, int64_t n, uint64_t* unique_keys,
     uint32_t* flags = static_cast<uint32_t*>(tmp);
     uint32_t* pos = flags + (n + 1);
     void* stmp = pos + (n + 1);
-    hipLaunchKernelGGL(k_rle_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, flags);
+    // one thread per key: the grid must cover all n keys
+    hipLaunchKernelGGL(k_rle_flags, dim3(grid_for(n, 256, INT32_MAX)), dim3(256), 0, s, keys, n,
+                       flags);
     exclusive_scan_u32(flags, pos, n, n_unique_dev, stmp, s);
-    hipLaunchKernelGGL(k_rle_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, flags,
+    hipLaunchKernelGGL(k_rle_scatter, dim3(grid_for(n, 256, INT32_MAX)), dim3(256), 0, s, keys, n, flags,
                        pos, unique_keys, run_start, n_unique_dev);
     IPCG_CHECK_LAUNCH();
 }

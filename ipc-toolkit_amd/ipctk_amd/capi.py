@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libipcg.so")
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_NUMERIC, ERR_NO_DEVICE = range(7)
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
 BP_LBVH, BP_SAP = 0, 1
+COLLISIONS_IPC, COLLISIONS_IMPROVED_MAX_APPROX = 0, 1
 PSD = {"none": 0, "clamp": 1, "abs": 2}
 
 # every symbol the header declares
@@ -36,7 +37,7 @@ EXPORTS = [
     "ipcg_barrier_gauss_newton_diagonal", "ipcg_barrier_gauss_newton_quadratic_form",
     "ipcg_candidates_per_vertex_safe_distances",
     "ipcg_candidates_noncandidate_conservative_stepsize", "ipcg_candidates_cfl_stepsize",
-    "ipcg_collisions_set_area_weighting",
+    "ipcg_collisions_set_area_weighting", "ipcg_collisions_set_type",
 ]
 
 _lib = None
@@ -102,6 +103,7 @@ def load(path=LIB_PATH):
     L.ipcg_edge_face_candidates_download.argtypes = [vp, _I]
     L.ipcg_has_intersections.argtypes = [vp, C.POINTER(C.c_int)]
     L.ipcg_collisions_set_area_weighting.argtypes = [vp, C.c_int]
+    L.ipcg_collisions_set_type.argtypes = [vp, C.c_int]
     L.ipcg_barrier_gauss_newton_diagonal.argtypes = [vp, C.c_double, C.c_double, _D]
     L.ipcg_barrier_gauss_newton_quadratic_form.argtypes = [vp, C.c_double, C.c_double, _D, _D]
     L.ipcg_candidates_per_vertex_safe_distances.argtypes = [vp, C.c_double, C.c_double, _D]
@@ -258,6 +260,11 @@ class Context:
     def set_area_weighting(self, use=True):
         """NormalCollisions::set_use_area_weighting for the builds that follow"""
         self._check(self._L.ipcg_collisions_set_area_weighting(self._p, int(bool(use))))
+
+    def set_collision_set_type(self, collision_set_type):
+        """NormalCollisions::set_collision_set_type: COLLISIONS_IPC or
+        COLLISIONS_IMPROVED_MAX_APPROX, for the builds that follow"""
+        self._check(self._L.ipcg_collisions_set_type(self._p, int(collision_set_type)))
 
     def build_collisions_from_candidates(self, dhat, dmin=0.0):
         self._check(self._L.ipcg_collisions_build_from_candidates(self._p, dhat, dmin))

@@ -258,6 +258,24 @@ static void area_weighting()
     CHECK(C.fv_collisions.size() == 1 && C.fv_collisions[0].weight == 1.0);
 }
 
+// test_normal_collisions.cpp:120-200: the codim star gains one
+// vertex-vertex collision under the improved max approximation
+static void improved_max_approx()
+{
+    using namespace ipc;
+    const MatrixXd V { { 0, 0, 0 }, { 1, 0, 0 }, { 0, 0, -1 }, { -1, 0, 0 },
+                       { 0, 0, 1 }, { 0, 1, 0 }, { 0, 2, 0 }, { 0, 3, 0 } };
+    const MatrixXi E { { 0, 1 }, { 0, 2 }, { 0, 3 }, { 0, 4 } };
+    CollisionMesh mesh(V, E);
+    NormalCollisions C;
+    C.build(mesh, V, 0.25, 0.8);
+    CHECK(C.size() == 6 && C.vv_collisions.size() == 2 && C.ev_collisions.size() == 4);
+    C.set_collision_set_type(NormalCollisions::CollisionSetType::IMPROVED_MAX_APPROX);
+    CHECK(C.collision_set_type() == NormalCollisions::CollisionSetType::IMPROVED_MAX_APPROX);
+    C.build(mesh, V, 0.25, 0.8);
+    CHECK(C.size() == 7 && C.vv_collisions.size() == 3 && C.ev_collisions.size() == 4);
+}
+
 int main()
 {
     try {
@@ -269,6 +287,7 @@ int main()
         intersections();
         candidate_helpers();
         area_weighting();
+        improved_max_approx();
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());
         failures++;
