@@ -232,13 +232,27 @@ void ipco_collisions_free(ipco_collisions *c);
  * an edge-edge collision keeps the orientation and eps_x of the candidate's
  * contribution, else of the one listing the lower edge id first; collisions
  * (other than face-vertex) whose weights sum to zero are dropped.  NULL
- * areas: unit weights.  improved = 0 is ipco_build_collisions_weighted. */
+ * areas: unit weights.  set_type: 0 IPC (ipco_build_collisions_weighted),
+ * 1 IMPROVED_MAX_APPROX, 2 OGC (the feasible-region filter of
+ * ogc/feasible_region.cpp and the weight division of
+ * normal_collisions_builder.cpp:87-101, 269-289). */
 int ipco_build_collisions_ex(const double *V, int nV, const double *V_rest, const int32_t *E,
                              int nE, const int32_t *F, int nF, const int32_t *F2E,
                              const ipco_pairs *vv_cand, const ipco_pairs *ev_cand,
                              const ipco_pairs *ee_cand, const ipco_pairs *fv_cand, double dhat,
                              double dmin, const double *vertex_area, const double *edge_area,
-                             int improved, ipco_collisions *out);
+                             int set_type, ipco_collisions *out);
+/* ogc/feasible_region.cpp: check_vertex_feasible_region (point x against
+ * vertex vi), check_edge_feasible_region (vertex xi against edge ei), and
+ * is_{edge_vertex,edge_edge,face_vertex}_feasible with the distance type
+ * classified (kind 1: edge a / vertex b; 2: edges a, b; 3: face a / vertex
+ * b).  Return 1 if feasible. */
+int ipco_ogc_vertex_region(const double *V, int nV, const int32_t *E, int nE, const int32_t *F,
+                           int nF, const int32_t *F2E, const double x[3], int vi);
+int ipco_ogc_edge_region(const double *V, int nV, const int32_t *E, int nE, const int32_t *F,
+                         int nF, const int32_t *F2E, int xi, int ei);
+int ipco_ogc_feasible(const double *V, int nV, const int32_t *E, int nE, const int32_t *F, int nF,
+                      const int32_t *F2E, int kind, int a, int b);
 
 /* NormalCollisions::set_use_area_weighting(true): the same build with
  * weights 0.5 (a_i + a_j) (codim vertex pairs), 0.5 a_v (codim edge-vertex),

@@ -259,6 +259,7 @@ typedef struct mesh_adj {
     adjl vv;      /* vertex -> vertices sharing an edge */
     adjl ve;      /* vertex -> incident edges */
     adjl ev;      /* edge -> vertices opposite it in its faces */
+    adjl vf;      /* vertex -> incident faces */
     uint8_t *bnd; /* is_vertex_on_boundary (3D rule) */
 } mesh_adj;
 
@@ -288,6 +289,12 @@ static void mesh_adj_build(int nV, const int32_t *E, int nE, const int32_t *F, i
             v[3 * i + j] = F[3 * i + (j + 2) % 3];
         }
     adj_build(nE, mf, r, v, &M->ev);
+    for (int i = 0; i < nF; i++)
+        for (int j = 0; j < 3; j++) {
+            r[3 * i + j] = F[3 * i + j];
+            v[3 * i + j] = i;
+        }
+    adj_build(nV, mf, r, v, &M->vf);
     free(r);
     free(v);
     M->bnd = (uint8_t *)malloc((size_t)(nV > 0 ? nV : 1));
@@ -304,7 +311,206 @@ static void mesh_adj_free(mesh_adj *M)
     free(M->ve.id);
     free(M->ev.off);
     free(M->ev.id);
+    free(M->vf.off);
+    free(M->vf.id);
     free(M->bnd);
+}
+
+/* ---- OGC feasible regions (ogc/feasible_region.cpp) ----------------------
+ * Dot products over runtime-size vectors sum left to right, over fixed
+ * 3-vectors as a0 b0 + (a1 b1 + a2 b2). */
+static double dot_lr(const double *a, const double *b)
+{
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+static double dot_fx(const double *a, const double *b)
+{
+    return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+
+/* closest_point.cpp:11-18 point_edge_closest_point */
+static double pe_param(const double *p, const double *e0, const double *e1)
+{
+    const double e[3] = { e1[0] - e0[0], e1[1] - e0[1], e1[2] - e0[2] };
+    const double d[3] = { p[0] - e0[0], p[1] - e0[1], p[2] - e0[2] };
+    return dot_lr(d, e) / dot_lr(e, e);
+}
+
+/* feasible_region.cpp:7-24: x behind vertex vi against every neighbour */
+static int ogc_vertex_region(const double *V, const mesh_adj *M, const double *x, int vi)
+{
+    const double *xv = vx(V, vi);
+    const double d[3] = { x[0] - xv[0], x[1] - xv[1], x[2] - xv[2] };
+    for (int64_t t = M->vv.off[vi]; t < M->vv.off[vi + 1]; t++) {
+        const double *xj = vx(V, M->vv.id[t]);
+        const double u[3] = { xv[0] - xj[0], xv[1] - xj[1], xv[2] - xj[2] };
+        if (!(dot_lr(d, u) >= 0)) return 0;
+    }
+    return 1;
+}
+
+/* feasible_region.cpp:26-54: vertex xi over edge ei, beyond the edge from
+ * every opposite vertex */
+static int ogc_edge_region(const double *V, const int32_t *E, const mesh_adj *M, int xi, int ei)
+{
+    const double *x = vx(V, xi), *x0 = vx(V, E[2 * (size_t)ei]), *x1 = vx(V, E[2 * (size_t)ei + 1]);
+    const double alpha = pe_param(x, x0, x1);
+    if (alpha <= 0 || alpha >= 1) return 0;
+    for (int64_t t = M->ev.off[ei]; t < M->ev.off[ei + 1]; t++) {
+        const double *x2 = vx(V, M->ev.id[t]);
+        const double a2 = pe_param(x2, x0, x1);
+        double p[3], u[3], w[3];
+        for (int k = 0; k < 3; k++) {
+            p[k] = (x1[k] - x0[k]) * a2 + x0[k];
+            u[k] = x[k] - p[k];
+            w[k] = p[k] - x2[k];
+        }
+        if (!(dot_fx(u, w) >= 0)) return 0;
+    }
+    return 1;
+}
+
+static int ogc_vv_pair(const double *V, const mesh_adj *M, int a, int b)
+{
+    return ogc_vertex_region(V, M, vx(V, a), b) && ogc_vertex_region(V, M, vx(V, b), a);
+}
+
+/* feasible_region.cpp:56-89 is_edge_vertex_feasible */
+static int ogc_ev_feasible(const double *V, const int32_t *E, const mesh_adj *M, int ei, int vi,
+                           int dt)
+{
+    if (dt == IPCO_PE_P_E0) return ogc_vv_pair(V, M, vi, E[2 * (size_t)ei]);
+    if (dt == IPCO_PE_P_E1) return ogc_vv_pair(V, M, vi, E[2 * (size_t)ei + 1]);
+    return 1;
+}
+
+/* feasible_region.cpp:91-167 is_edge_edge_feasible */
+static int ogc_ee_feasible(const double *V, const int32_t *E, const mesh_adj *M, int ea, int eb,
+                           int dt)
+{
+    const int a[2] = { E[2 * (size_t)ea], E[2 * (size_t)ea + 1] };
+    const int b[2] = { E[2 * (size_t)eb], E[2 * (size_t)eb + 1] };
+    switch (dt) {
+    case IPCO_EE_EA0_EB0: return ogc_vv_pair(V, M, a[0], b[0]);
+    case IPCO_EE_EA0_EB1: return ogc_vv_pair(V, M, a[0], b[1]);
+    case IPCO_EE_EA1_EB0: return ogc_vv_pair(V, M, a[1], b[0]);
+    case IPCO_EE_EA1_EB1: return ogc_vv_pair(V, M, a[1], b[1]);
+    case IPCO_EE_EA_EB: return 1;
+    default: break;
+    }
+    /* EA_EB0 / EA_EB1: the end of eb against its closest point on ea, and
+     * EA0_EB / EA1_EB the other way round */
+    const int on_a = dt == IPCO_EE_EA_EB0 || dt == IPCO_EE_EA_EB1;
+    const int v = on_a ? b[dt == IPCO_EE_EA_EB1] : a[dt == IPCO_EE_EA1_EB];
+    const int *e = on_a ? a : b;
+    const double *q0 = vx(V, e[0]), *q1 = vx(V, e[1]);
+    const double alpha = pe_param(vx(V, v), q0, q1);
+    double xc[3];
+    for (int k = 0; k < 3; k++) xc[k] = (q1[k] - q0[k]) * alpha + q0[k];
+    return ogc_vertex_region(V, M, xc, v);
+}
+
+/* feasible_region.cpp:169-224 is_face_vertex_feasible */
+static int ogc_fv_feasible(const double *V, const int32_t *E, const int32_t *F, const int32_t *F2E,
+                           const mesh_adj *M, int fi, int vi, int dt)
+{
+    if (dt <= IPCO_PT_P_T2) return ogc_vv_pair(V, M, vi, F[3 * (size_t)fi + dt]);
+    if (dt <= IPCO_PT_P_E2) return ogc_edge_region(V, E, M, vi, F2E[3 * (size_t)fi + dt - 3]);
+    return 1;
+}
+
+/* normal_collisions_builder.cpp:56-114 (use_ogc): infeasible edge-vertex
+ * pairs are dropped; a pair reduced to a vertex is divided among the
+ * vertex's incident edges */
+static void ogc_ev(const double *V, const int32_t *E, const mesh_adj *M, int32_t ei, int32_t vi,
+                   rec *r)
+{
+    if (r->type < 0) return;
+    const int32_t e[2] = { E[2 * (size_t)ei], E[2 * (size_t)ei + 1] };
+    const int dt = ipco_point_edge_distance_type(vx(V, vi), vx(V, e[0]), vx(V, e[1]));
+    if (!ogc_ev_feasible(V, E, M, ei, vi, dt)) {
+        r->type = -1;
+        return;
+    }
+    if (dt < IPCO_PE_P_E) {
+        const int64_t n = adj_deg(&M->ve, e[dt]);
+        if (n > 1) r->w /= (double)n;
+    }
+}
+
+/* normal_collisions_builder.cpp:140-182 (use_ogc): the feasibility test
+ * uses the actual distance type, before the mollifier's override */
+static void ogc_ee(const double *V, const int32_t *E, const mesh_adj *M, int32_t ea, int32_t eb,
+                   rec *r)
+{
+    if (r->type < 0) return;
+    const int dt = ipco_edge_edge_distance_type(vx(V, E[2 * (size_t)ea]), vx(V, E[2 * (size_t)ea + 1]),
+                                                vx(V, E[2 * (size_t)eb]), vx(V, E[2 * (size_t)eb + 1]));
+    if (!ogc_ee_feasible(V, E, M, ea, eb, dt)) r->type = -1;
+}
+
+/* normal_collisions_builder.cpp:239-296 (use_ogc): divided among the faces
+ * of the vertex or the opposite vertices of the edge it reduces to */
+static void ogc_fv(const double *V, const int32_t *E, const int32_t *F, const int32_t *F2E,
+                   const mesh_adj *M, int32_t fi, int32_t vi, rec *r)
+{
+    if (r->type < 0) return;
+    const int32_t *f = F + 3 * (size_t)fi;
+    const int dt = ipco_point_triangle_distance_type(vx(V, vi), vx(V, f[0]), vx(V, f[1]), vx(V, f[2]));
+    if (!ogc_fv_feasible(V, E, F, F2E, M, fi, vi, dt)) {
+        r->type = -1;
+        return;
+    }
+    int64_t n = 1;
+    if (dt <= IPCO_PT_P_T2) n = adj_deg(&M->vf, f[dt]);
+    else if (dt <= IPCO_PT_P_E2) n = adj_deg(&M->ev, F2E[3 * (size_t)fi + dt - 3]);
+    if (n > 1) r->w /= (double)n;
+}
+
+int ipco_ogc_vertex_region(const double *V, int nV, const int32_t *E, int nE, const int32_t *F,
+                           int nF, const int32_t *F2E, const double x[3], int vi)
+{
+    mesh_adj M;
+    mesh_adj_build(nV, E, nE, F, nF, F2E, &M);
+    const int r = ogc_vertex_region(V, &M, x, vi);
+    mesh_adj_free(&M);
+    return r;
+}
+
+int ipco_ogc_edge_region(const double *V, int nV, const int32_t *E, int nE, const int32_t *F,
+                         int nF, const int32_t *F2E, int xi, int ei)
+{
+    mesh_adj M;
+    mesh_adj_build(nV, E, nE, F, nF, F2E, &M);
+    const int r = ogc_edge_region(V, E, &M, xi, ei);
+    mesh_adj_free(&M);
+    return r;
+}
+
+int ipco_ogc_feasible(const double *V, int nV, const int32_t *E, int nE, const int32_t *F, int nF,
+                      const int32_t *F2E, int kind, int a, int b)
+{
+    mesh_adj M;
+    mesh_adj_build(nV, E, nE, F, nF, F2E, &M);
+    int r;
+    if (kind == 1) { /* (edge a, vertex b) */
+        const int dt = ipco_point_edge_distance_type(vx(V, b), vx(V, E[2 * (size_t)a]),
+                                                     vx(V, E[2 * (size_t)a + 1]));
+        r = ogc_ev_feasible(V, E, &M, a, b, dt);
+    } else if (kind == 2) { /* (edge a, edge b) */
+        const int dt = ipco_edge_edge_distance_type(
+            vx(V, E[2 * (size_t)a]), vx(V, E[2 * (size_t)a + 1]), vx(V, E[2 * (size_t)b]),
+            vx(V, E[2 * (size_t)b + 1]));
+        r = ogc_ee_feasible(V, E, &M, a, b, dt);
+    } else { /* (face a, vertex b) */
+        const int32_t *f = F + 3 * (size_t)a;
+        const int dt = ipco_point_triangle_distance_type(vx(V, b), vx(V, f[0]), vx(V, f[1]),
+                                                         vx(V, f[2]));
+        r = ogc_fv_feasible(V, E, F, F2E, &M, a, b, dt);
+    }
+    mesh_adj_free(&M);
+    return r;
 }
 
 /* ---- IMPROVED_MAX_APPROX (normal_collisions.cpp:84-128) ------------------ */
@@ -526,8 +732,9 @@ int ipco_build_collisions_ex(const double *V, int nV, const double *V_rest, cons
                              const ipco_pairs *vv_cand, const ipco_pairs *ev_cand,
                              const ipco_pairs *ee_cand, const ipco_pairs *fv_cand, double dhat,
                              double dmin, const double *vertex_area, const double *edge_area,
-                             int improved, ipco_collisions *out)
+                             int set_type, ipco_collisions *out)
 {
+    const int improved = set_type == 1, ogc = set_type == 2;
     memset(out, 0, sizeof(*out));
     out->dmin = dmin;
     const double off = dmin + dhat;
@@ -541,6 +748,8 @@ int ipco_build_collisions_ex(const double *V, int nV, const double *V_rest, cons
     rec *Rc = R + nee + nfv; /* codim candidates after the surface ones */
     const double *va = vertex_area, *ea = edge_area;
     const int aw = va != NULL;
+    mesh_adj M;
+    if (ogc) mesh_adj_build(nV, E, nE, F, nF, F2E, &M);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nvv; i++) {
         const int32_t a = vv_cand->ij[2 * i], b = vv_cand->ij[2 * i + 1];
@@ -552,19 +761,23 @@ int ipco_build_collisions_ex(const double *V, int nV, const double *V_rest, cons
         ev_candidate(V, E, ev_cand->ij[2 * i], ev_cand->ij[2 * i + 1], active2,
                      &Rc[nvv + i]);
         Rc[nvv + i].w = aw ? 0.5 * va[ev_cand->ij[2 * i + 1]] : 1.0; /* :81 */
+        if (ogc) ogc_ev(V, E, &M, ev_cand->ij[2 * i], ev_cand->ij[2 * i + 1], &Rc[nvv + i]);
     }
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nee; i++) {
         const int32_t a = ee_cand->ij[2 * i], b = ee_cand->ij[2 * i + 1];
         ee_candidate(V, V_rest, E, a, b, active2, &R[i]);
         R[i].w = aw ? 0.25 * (ea[a] + ea[b]) : 1.0; /* :181-183 */
+        if (ogc) ogc_ee(V, E, &M, a, b, &R[i]);
     }
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nfv; i++) {
         fv_candidate(V, F, F2E, fv_cand->ij[2 * i], fv_cand->ij[2 * i + 1],
                      active2, &R[nee + i]);
         R[nee + i].w = aw ? 0.25 * va[fv_cand->ij[2 * i + 1]] : 1.0; /* :265 */
+        if (ogc) ogc_fv(V, E, F, F2E, &M, fv_cand->ij[2 * i], fv_cand->ij[2 * i + 1], &R[nee + i]);
     }
+    if (ogc) mesh_adj_free(&M);
     int64_t n = n0;
     if (improved) {
         recvec X = { NULL, 0, 0 };

@@ -175,6 +175,12 @@ def lib():
             _D, C.c_int, _D, _I, C.c_int, _I, C.c_int, _I, C.POINTER(_Pairs), C.POINTER(_Pairs),
             C.POINTER(_Pairs), C.POINTER(_Pairs), C.c_double, C.c_double, _D, _D, C.c_int,
             C.POINTER(_Collisions)]
+        L.ipco_ogc_vertex_region.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, _D,
+                                             C.c_int]
+        L.ipco_ogc_edge_region.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, C.c_int,
+                                           C.c_int]
+        L.ipco_ogc_feasible.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, C.c_int,
+                                        C.c_int, C.c_int]
         L.ipco_mesh_areas.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, _D, _D]
         L.ipco_set_num_threads.argtypes = [C.c_int]
         L.ipco_get_max_threads.restype = C.c_int
@@ -516,9 +522,10 @@ def mesh_areas(V_rest, E, F):
 
 
 def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_cand=None,
-                     ev_cand=None, area_weighting=False, improved=False):
-    """NormalCollisions::build over the given candidates; area_weighting and
-    improved (CollisionSetType::IMPROVED_MAX_APPROX) select the variants."""
+                     ev_cand=None, area_weighting=False, improved=False, ogc=False):
+    """NormalCollisions::build over the given candidates; area_weighting,
+    improved (CollisionSetType::IMPROVED_MAX_APPROX) and ogc
+    (CollisionSetType::OGC) select the variants."""
     V, pV = _d(V)
     if V_rest is None:
         V_rest = V
@@ -539,7 +546,7 @@ def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_
     out = _Collisions()
     lib().ipco_build_collisions_ex(pV, len(V), pR, pE, len(E), pF, len(F), pF2E, C.byref(vv_s),
                                    C.byref(ev_s), C.byref(ee_s), C.byref(fv_s), dhat, dmin, pva,
-                                   pea, int(bool(improved)), C.byref(out))
+                                   pea, 2 if ogc else int(bool(improved)), C.byref(out))
     return Collisions(out, (E, F))
 
 
@@ -726,3 +733,34 @@ def is_step_collision_free(V0, V1, E, F, min_distance=0.0, params=DEFAULT_CCD):
     F, pF = _i(F)
     return bool(lib().ipco_is_step_collision_free(p0, p1, len(V0), pE, len(E), pF, len(F),
                                                   min_distance, C.byref(params)))
+
+
+def _mesh_args(V, E, F):
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    F2E = faces_to_edges(E, F) if len(F) else np.zeros((0, 3), np.int32)
+    F2E, pF2E = _i(F2E)
+    return (V, E, F, F2E), (pV, len(V), pE, len(E), pF, len(F), pF2E)
+
+
+def ogc_vertex_region(V, E, F, x, vi):
+    """ogc::check_vertex_feasible_region(mesh, V, x, vi)"""
+    keep, args = _mesh_args(V, E, F)
+    x, px = _d(np.asarray(x, float).reshape(3))
+    return bool(lib().ipco_ogc_vertex_region(*args, px, int(vi)))
+
+
+def ogc_edge_region(V, E, F, xi, ei):
+    """ogc::check_edge_feasible_region(mesh, V, xi, ei)"""
+    keep, args = _mesh_args(V, E, F)
+    return bool(lib().ipco_ogc_edge_region(*args, int(xi), int(ei)))
+
+
+def ogc_feasible(V, E, F, kind, a, b):
+    """ogc::is_edge_vertex_feasible ("ev": edge a, vertex b),
+    is_edge_edge_feasible ("ee") or is_face_vertex_feasible ("fv": face a,
+    vertex b) with the distance type classified"""
+    keep, args = _mesh_args(V, E, F)
+    k = {"ev": 1, "ee": 2, "fv": 3}[kind]
+    return bool(lib().ipco_ogc_feasible(*args, k, int(a), int(b)))
