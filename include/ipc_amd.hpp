@@ -435,9 +435,10 @@ public:
     void set_use_area_weighting(bool use) { m_area_weighting = use; }
     bool use_area_weighting() const { return m_area_weighting; }
 
-    // normal_collisions.hpp CollisionSetType (OGC is not provided)
+    // normal_collisions.hpp CollisionSetType
     enum class CollisionSetType { IPC = IPCG_COLLISIONS_IPC,
-                                  IMPROVED_MAX_APPROX = IPCG_COLLISIONS_IMPROVED_MAX_APPROX };
+                                  IMPROVED_MAX_APPROX = IPCG_COLLISIONS_IMPROVED_MAX_APPROX,
+                                  OGC = IPCG_COLLISIONS_OGC };
     void set_collision_set_type(CollisionSetType type) { m_type = type; }
     CollisionSetType collision_set_type() const { return m_type; }
 

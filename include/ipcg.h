@@ -163,11 +163,18 @@ int ipcg_collisions_set_area_weighting(ipcg_ctx* ctx, int use_area_weighting);
  * lists the lower edge id first; vertex-vertex, edge-vertex and edge-edge
  * collisions whose weights sum to zero are dropped
  * (normal_collisions_builder.cpp:660-684).  The reference warns that the
- * approximation is meant to be used with area weighting.  OGC is not
- * supported (IPCG_ERR_ARG).  Default IPCG_COLLISIONS_IPC. */
+ * approximation is meant to be used with area weighting.
+ * IPCG_COLLISIONS_OGC drops edge-vertex, edge-edge and face-vertex pairs
+ * whose closest point lies outside the feasible region of the feature it
+ * reduces to (ogc/feasible_region.cpp) and divides a pair reduced to a
+ * vertex or an edge among the vertex's incident edges (codim edge-vertex)
+ * or faces, or the edge's opposite vertices (face-vertex)
+ * (normal_collisions_builder.cpp:56-336 with use_ogc).  Unknown types:
+ * IPCG_ERR_ARG.  Default IPCG_COLLISIONS_IPC. */
 enum {
     IPCG_COLLISIONS_IPC = 0,
-    IPCG_COLLISIONS_IMPROVED_MAX_APPROX = 1
+    IPCG_COLLISIONS_IMPROVED_MAX_APPROX = 1,
+    IPCG_COLLISIONS_OGC = 2
 };
 int ipcg_collisions_set_type(ipcg_ctx* ctx, int collision_set_type);
 /* counts[4] = {vv, ev, ee, fv} */

@@ -172,7 +172,7 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3],
                        &c.cand_ef, &c.isect, &c.gn_out, &c.gn_p, &c.v_area, &c.e_area,
                        &c.adj_vv_off, &c.adj_vv, &c.adj_ve_off, &c.adj_ve, &c.adj_ev_off,
-                       &c.adj_ev, &c.adj_bnd, &c.np_rw, &c.np_flag, &c.np_pos, &c.np_rep,
+                       &c.adj_ev, &c.adj_vf_off, &c.adj_bnd, &c.np_rw, &c.np_flag, &c.np_pos, &c.np_rep,
                        &c.im_slot, &c.im_slot2, &c.im_vals, &c.im_vals2, &c.im_ukeys,
                        &c.im_start, &c.im_cnt, &c.im_off, &c.im_vv, &c.im_vvw, &c.im_ev,
                        &c.im_evw, &c.im_ee,
@@ -263,6 +263,16 @@ static void upload_adjacencies(Ctx& c, int32_t nV, const int32_t* edges, int32_t
             vals[3 * (size_t)f + j] = faces[3 * f + (j + 2) % 3];
         }
     csr_rows(nE, rows, vals, off, ids);
+    {
+        std::vector<int32_t> fr(3 * (size_t)nF), fv(3 * (size_t)nF), foff, fids;
+        for (int32_t f = 0; f < nF; f++)
+            for (int j = 0; j < 3; j++) {
+                fr[3 * (size_t)f + j] = faces[3 * f + j];
+                fv[3 * (size_t)f + j] = f;
+            }
+        csr_rows(nV, fr, fv, foff, fids);
+        upload_i32(c, c.adj_vf_off, foff);
+    }
     std::vector<uint8_t> bnd((size_t)std::max(nV, 1), 1);
     for (int32_t e = 0; e < nE; e++)
         if (off[(size_t)e + 1] - off[e] >= 2) bnd[edges[2 * e]] = bnd[edges[2 * e + 1]] = 0;
@@ -558,8 +568,9 @@ int ipcg_collisions_set_type(ipcg_ctx* ctx, int collision_set_type)
 {
     return guard(ctx, [&](Ctx& c) {
         if (collision_set_type != IPCG_COLLISIONS_IPC
-            && collision_set_type != IPCG_COLLISIONS_IMPROVED_MAX_APPROX)
-            throw Error(IPCG_ERR_ARG, "unsupported collision set type");
+            && collision_set_type != IPCG_COLLISIONS_IMPROVED_MAX_APPROX
+            && collision_set_type != IPCG_COLLISIONS_OGC)
+            throw Error(IPCG_ERR_ARG, "unknown collision set type");
         c.coll_type = collision_set_type;
     });
 }

@@ -54,12 +54,13 @@ struct Ctx {
     DevBuf E, F, F2E, patch, rest;
     DevBuf v_area, e_area;     // CollisionMesh vertex / edge areas (rest positions)
     bool area_weighting = false; // NormalCollisions::use_area_weighting
-    int coll_type = 0;           // NormalCollisions::CollisionSetType: 0 IPC, 1 IMPROVED_MAX_APPROX
+    int coll_type = 0; // NormalCollisions::CollisionSetType: 0 IPC, 1 IMPROVED_MAX_APPROX, 2 OGC
     // CollisionMesh adjacencies (collision_mesh.cpp:247-291), CSR with int32
     // offsets, rows sorted and duplicate-free: vertex -> vertices sharing an
     // edge, vertex -> incident edges, edge -> vertices opposite it in its
-    // faces; is_vertex_on_boundary as bytes
-    DevBuf adj_vv_off, adj_vv, adj_ve_off, adj_ve, adj_ev_off, adj_ev, adj_bnd;
+    // faces, vertex -> incident faces (offsets only); is_vertex_on_boundary
+    // as bytes
+    DevBuf adj_vv_off, adj_vv, adj_ve_off, adj_ve, adj_ev_off, adj_ev, adj_vf_off, adj_bnd;
     DevBuf X[2];
     bool has_pos[2] = { false, false };
 

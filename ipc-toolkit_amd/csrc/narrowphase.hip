@@ -6,8 +6,9 @@
 // vv by (min, max), ev by (edge, vertex), ee by (min edge, max edge, distance
 // type), fv by (face, vertex).
 //
-// With area weighting or the IMPROVED_MAX_APPROX collision set
-// (normal_collisions.cpp:84-128) the weights are real numbers: a run's
+// With area weighting, the IMPROVED_MAX_APPROX collision set
+// (normal_collisions.cpp:84-128) or the OGC set the weights are real
+// numbers: a run's
 // weight is the sum of its contributions in ascending order of value (the
 // reference's own order follows its thread schedule), and vertex-vertex,
 // edge-vertex and edge-edge collisions whose weights sum to zero are dropped.
@@ -68,13 +69,116 @@ __device__ __forceinline__ uint64_t pair_key(int a, int b)
     return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
 }
 
+// CollisionMesh adjacencies (rows sorted, duplicate-free)
+struct Adj {
+    const int32_t *vv_off, *vv; // vertex -> vertices sharing an edge
+    const int32_t *ve_off, *ve; // vertex -> incident edges
+    const int32_t *ev_off, *ev; // edge -> vertices opposite it in its faces
+    const int32_t* vf_off;      // vertex -> incident faces (counts only)
+    const uint8_t* bnd;         // is_vertex_on_boundary
+};
+
+__device__ __forceinline__ bool row_has(const int32_t* off, const int32_t* ids, int row, int v)
+{
+    int lo = off[row], hi = off[row + 1];
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (ids[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < off[row + 1] && ids[lo] == v;
+}
+
+// ---- OGC feasible regions (ogc/feasible_region.cpp) ----------------------
+// Dot products over runtime-size vectors sum left to right, over fixed
+// 3-vectors as a0 b0 + (a1 b1 + a2 b2), as the reference's Eigen types do.
+__device__ __forceinline__ double dot_lr(const d3& a, const d3& b)
+{
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+
+__device__ __forceinline__ double dot_fx(const d3& a, const d3& b)
+{
+    return a.x * b.x + (a.y * b.y + a.z * b.z);
+}
+
+// closest_point.cpp:11-18 point_edge_closest_point
+__device__ __forceinline__ double pe_param(const d3& p, const d3& e0, const d3& e1)
+{
+    const d3 e = sub(e1, e0);
+    return dot_lr(sub(p, e0), e) / dot_lr(e, e);
+}
+
+__device__ __forceinline__ d3 on_edge(const d3& q0, const d3& q1, double a)
+{
+    return { (q1.x - q0.x) * a + q0.x, (q1.y - q0.y) * a + q0.y, (q1.z - q0.z) * a + q0.z };
+}
+
+// feasible_region.cpp:7-24: x behind vertex vi against every neighbour
+__device__ bool ogc_vertex_region(const double* X, int nV, const Adj& J, const d3& x, int vi)
+{
+    const d3 xv = ldx(X, nV, vi);
+    const d3 d = sub(x, xv);
+    for (int t = J.vv_off[vi]; t < J.vv_off[vi + 1]; t++)
+        if (!(dot_lr(d, sub(xv, ldx(X, nV, J.vv[t]))) >= 0)) return false;
+    return true;
+}
+
+__device__ __forceinline__ bool ogc_vv_pair(const double* X, int nV, const Adj& J, int a, int b)
+{
+    return ogc_vertex_region(X, nV, J, ldx(X, nV, a), b)
+        && ogc_vertex_region(X, nV, J, ldx(X, nV, b), a);
+}
+
+// feasible_region.cpp:26-54: vertex xi over edge ei, beyond the edge from
+// every opposite vertex
+__device__ bool ogc_edge_region(const double* X, int nV, const int32_t* E, const Adj& J, int xi,
+                                int ei)
+{
+    const d3 x = ldx(X, nV, xi), x0 = ldx(X, nV, E[2 * (int64_t)ei]),
+             x1 = ldx(X, nV, E[2 * (int64_t)ei + 1]);
+    const double alpha = pe_param(x, x0, x1);
+    if (alpha <= 0 || alpha >= 1) return false;
+    for (int t = J.ev_off[ei]; t < J.ev_off[ei + 1]; t++) {
+        const d3 x2 = ldx(X, nV, J.ev[t]);
+        const d3 p = on_edge(x0, x1, pe_param(x2, x0, x1));
+        if (!(dot_fx(sub(x, p), sub(p, x2)) >= 0)) return false;
+    }
+    return true;
+}
+
+// feasible_region.cpp:91-167 is_edge_edge_feasible (actual distance type)
+__device__ bool ogc_ee_feasible(const double* X, int nV, const Adj& J, int a0, int a1, int b0,
+                                int b1, int dt)
+{
+    switch (dt) {
+    case EE_EA0_EB0: return ogc_vv_pair(X, nV, J, a0, b0);
+    case EE_EA0_EB1: return ogc_vv_pair(X, nV, J, a0, b1);
+    case EE_EA1_EB0: return ogc_vv_pair(X, nV, J, a1, b0);
+    case EE_EA1_EB1: return ogc_vv_pair(X, nV, J, a1, b1);
+    case EE_EA_EB: return true;
+    default: break;
+    }
+    // an end of one edge against its closest point on the other
+    const bool on_a = dt == EE_EA_EB0 || dt == EE_EA_EB1;
+    const int v = on_a ? (dt == EE_EA_EB1 ? b1 : b0) : (dt == EE_EA1_EB ? a1 : a0);
+    const d3 q0 = ldx(X, nV, on_a ? a0 : b0), q1 = ldx(X, nV, on_a ? a1 : b1);
+    const d3 xc = on_edge(q0, q1, pe_param(ldx(X, nV, v), q0, q1));
+    return ogc_vertex_region(X, nV, J, xc, v);
+}
+
 // candidates: [0, n_ee) edge-edge, then face-vertex, codim vertex-vertex,
 // codim edge-vertex
+// OGC: CollisionSetType::OGC (normal_collisions_builder.cpp:56-336 with
+// use_ogc): infeasible edge-vertex / edge-edge / face-vertex pairs are
+// dropped and pairs reduced to a vertex or edge are divided among its
+// incident edges / faces / opposite vertices.
+template <bool OGC>
 __global__ __launch_bounds__(NP_BLOCK) void k_classify(
     const double* X, const double* R, int nV, const int32_t* E, const int32_t* F,
     const int32_t* F2E, const int2* ee, int64_t n_ee, const int2* fv, int64_t n_fv,
     const int2* vv, int64_t n_vv, const int2* ev, int64_t n_ev, double active2, Streams S,
-    Areas A)
+    Areas A, Adj J)
 {
     const int64_t i = (int64_t)xcd_block() * NP_BLOCK + threadIdx.x;
     int kind = -1; // 0 vv, 1 ev, 2 ee, 3 fv
@@ -90,7 +194,7 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
                  pb1 = ldx(X, nV, b1);
         const int actual = edge_edge_type(pa0, pa1, pb0, pb1);
         const double d = ee_dist2(pa0, pa1, pb0, pb1, actual);
-        if (d < active2) {
+        if (d < active2 && (!OGC || ogc_ee_feasible(X, nV, J, a0, a1, b0, b1, actual))) {
             const double eps_x = ee_mollifier_threshold(ldx(R, nV, a0), ldx(R, nV, a1),
                                                         ldx(R, nV, b0), ldx(R, nV, b1));
             const double x = ee_cross2(pa0, pa1, pb0, pb1);
@@ -119,7 +223,19 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
                  t2 = ldx(X, nV, f2);
         const int dt = point_triangle_type(p, t0, t1, t2);
         const double d = pt_dist2(p, t0, t1, t2, dt);
-        if (d < active2) {
+        bool keep = d < active2;
+        if (OGC && keep) { // feasible_region.cpp:169-224; builder.cpp:269-289
+            const int fk = dt <= PT_P_T2 ? F[3 * (int64_t)fi + dt] : 0;
+            const int ek = dt >= PT_P_E0 && dt <= PT_P_E2 ? F2E[3 * (int64_t)fi + dt - 3] : 0;
+            keep = dt <= PT_P_T2 ? ogc_vv_pair(X, nV, J, vi, fk)
+                : dt <= PT_P_E2  ? ogc_edge_region(X, nV, E, J, vi, ek)
+                                 : true;
+            const int n = dt <= PT_P_T2 ? J.vf_off[fk + 1] - J.vf_off[fk]
+                : dt <= PT_P_E2         ? J.ev_off[ek + 1] - J.ev_off[ek]
+                                        : 1;
+            if (n > 1) w /= (double)n;
+        }
+        if (keep) {
             switch (dt) {
             case PT_P_T0: kind = 0; key = vv_key(vi, f0); break;
             case PT_P_T1: kind = 0; key = vv_key(vi, f1); break;
@@ -144,7 +260,14 @@ __global__ __launch_bounds__(NP_BLOCK) void k_classify(
         const int e0 = E[2 * (int64_t)ei], e1 = E[2 * (int64_t)ei + 1];
         const d3 p = ldx(X, nV, vi), q0 = ldx(X, nV, e0), q1 = ldx(X, nV, e1);
         const int dt = point_edge_type(p, q0, q1);
-        if (pe_dist2(p, q0, q1, dt) < active2) {
+        bool keep = pe_dist2(p, q0, q1, dt) < active2;
+        if (OGC && keep && dt != PE_P_E) { // feasible_region.cpp:56-89; builder.cpp:87-101
+            const int vj = dt == PE_P_E0 ? e0 : e1;
+            keep = ogc_vv_pair(X, nV, J, vi, vj);
+            const int n = J.ve_off[vj + 1] - J.ve_off[vj];
+            if (n > 1) w /= (double)n;
+        }
+        if (keep) {
             switch (dt) {
             case PE_P_E0: kind = 0; key = vv_key(vi, e0); break;
             case PE_P_E1: kind = 0; key = vv_key(vi, e1); break;
@@ -319,25 +442,6 @@ __global__ void k_ee_scatter(const EERec* r, int64_t nu, const double* rw, const
 }
 
 // ---- IMPROVED_MAX_APPROX -------------------------------------------------
-
-// CollisionMesh adjacencies (rows sorted, duplicate-free)
-struct Adj {
-    const int32_t *vv_off, *vv; // vertex -> vertices sharing an edge
-    const int32_t *ve_off, *ve; // vertex -> incident edges
-    const int32_t *ev_off, *ev; // edge -> vertices opposite it in its faces
-    const uint8_t* bnd;         // is_vertex_on_boundary
-};
-
-__device__ __forceinline__ bool row_has(const int32_t* off, const int32_t* ids, int row, int v)
-{
-    int lo = off[row], hi = off[row + 1];
-    while (lo < hi) {
-        const int mid = lo + (hi - lo) / 2;
-        if (ids[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo < off[row + 1] && ids[lo] == v;
-}
 
 // conversion slot keys: 2-bit tag above a 62-bit pair key (ids < 2^30)
 constexpr uint64_t CONV_NONE = ~0ull;
@@ -526,6 +630,13 @@ int key_bits_for(int64_t max_id)
 
 } // namespace
 
+static Adj adjacency(const Ctx& c)
+{
+    return { c.adj_vv_off.as<int32_t>(), c.adj_vv.as<int32_t>(), c.adj_ve_off.as<int32_t>(),
+             c.adj_ve.as<int32_t>(),     c.adj_ev_off.as<int32_t>(), c.adj_ev.as<int32_t>(),
+             c.adj_vf_off.as<int32_t>(), c.adj_bnd.as<uint8_t>() };
+}
+
 // read a device u32 through the pinned staging buffer
 static uint32_t read_u32(Ctx& c, const uint32_t* dev)
 {
@@ -583,9 +694,7 @@ static void improved_records(Ctx& c, double active2, Areas A, int64_t x[3])
     c.im_cnt.reserve(sizeof(uint32_t) * 3 * (size_t)nu);
     c.im_off.reserve(sizeof(uint32_t) * 3 * (size_t)nu);
     uint32_t* cnt = c.im_cnt.as<uint32_t>();
-    const Adj J { c.adj_vv_off.as<int32_t>(), c.adj_vv.as<int32_t>(), c.adj_ve_off.as<int32_t>(),
-                  c.adj_ve.as<int32_t>(), c.adj_ev_off.as<int32_t>(), c.adj_ev.as<int32_t>(),
-                  c.adj_bnd.as<uint8_t>() };
+    const Adj J = adjacency(c);
     ImOut O { nullptr, nullptr, nullptr, nullptr, nullptr, c.im_off.as<uint32_t>() };
     const unsigned g = grid_for(nu, NP_BLOCK, INT32_MAX);
     hipLaunchKernelGGL(k_conv_eval<true>, dim3(g), dim3(NP_BLOCK), 0, s, c.im_ukeys.as<uint64_t>(),
@@ -626,7 +735,8 @@ void np_build(Ctx& c, double dhat, double dmin)
     const int64_t cap = SHARDS * scap; // entries per stream
     const bool aw = c.area_weighting;
     const bool improved = c.coll_type == IPCG_COLLISIONS_IMPROVED_MAX_APPROX;
-    const bool real_w = aw || improved; // weights are not plain counts
+    const bool ogc = c.coll_type == IPCG_COLLISIONS_OGC;
+    const bool real_w = aw || improved || ogc; // weights are not plain counts
     if (real_w && c.nE >= (1 << 30))
         throw Error(IPCG_ERR_ARG, "edge ids beyond 2^30 do not fit the edge-edge merge key");
     c.np_raw.reserve((3 * sizeof(uint64_t) + sizeof(EERec) + (real_w ? 2 * sizeof(double) : 0))
@@ -649,11 +759,12 @@ void np_build(Ctx& c, double dhat, double dmin)
     const double off = dmin + dhat;
     c.stage_begin("np_classify");
     if (n > 0)
-        hipLaunchKernelGGL(k_classify, dim3(grid_for(n, NP_BLOCK, INT32_MAX)), dim3(NP_BLOCK), 0, s,
+        hipLaunchKernelGGL(ogc ? k_classify<true> : k_classify<false>,
+                           dim3(grid_for(n, NP_BLOCK, INT32_MAX)), dim3(NP_BLOCK), 0, s,
                            c.X[0].as<double>(), c.rest.as<double>(), c.nV, c.E.as<int32_t>(),
                            c.F.as<int32_t>(), c.F2E.as<int32_t>(), c.cand_ee.as<int2>(), c.n_ee,
                            c.cand_fv.as<int2>(), c.n_fv, c.cand_vv.as<int2>(), c.n_vv,
-                           c.cand_ev.as<int2>(), c.n_ev, off * off, S, A);
+                           c.cand_ev.as<int2>(), c.n_ev, off * off, S, A, adjacency(c));
     IPCG_CHECK_LAUNCH();
     c.stage_end();
     ShardCounts kc[4];

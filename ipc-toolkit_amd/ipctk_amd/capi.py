@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libipcg.so")
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_NUMERIC, ERR_NO_DEVICE = range(7)
 LAYOUT_AOS, LAYOUT_SOA = 0, 1
 BP_LBVH, BP_SAP = 0, 1
-COLLISIONS_IPC, COLLISIONS_IMPROVED_MAX_APPROX = 0, 1
+COLLISIONS_IPC, COLLISIONS_IMPROVED_MAX_APPROX, COLLISIONS_OGC = 0, 1, 2
 PSD = {"none": 0, "clamp": 1, "abs": 2}
 
 # every symbol the header declares
@@ -262,8 +262,9 @@ class Context:
         self._check(self._L.ipcg_collisions_set_area_weighting(self._p, int(bool(use))))
 
     def set_collision_set_type(self, collision_set_type):
-        """NormalCollisions::set_collision_set_type: COLLISIONS_IPC or
-        COLLISIONS_IMPROVED_MAX_APPROX, for the builds that follow"""
+        """NormalCollisions::set_collision_set_type: COLLISIONS_IPC,
+        COLLISIONS_IMPROVED_MAX_APPROX or COLLISIONS_OGC, for the builds that
+        follow"""
         self._check(self._L.ipcg_collisions_set_type(self._p, int(collision_set_type)))
 
     def build_collisions_from_candidates(self, dhat, dmin=0.0):

@@ -259,7 +259,8 @@ static void area_weighting()
 }
 
 // test_normal_collisions.cpp:120-200: the codim star gains one
-// vertex-vertex collision under the improved max approximation
+// vertex-vertex collision under the improved max approximation and keeps
+// its six under OGC
 static void improved_max_approx()
 {
     using namespace ipc;
@@ -274,6 +275,9 @@ static void improved_max_approx()
     CHECK(C.collision_set_type() == NormalCollisions::CollisionSetType::IMPROVED_MAX_APPROX);
     C.build(mesh, V, 0.25, 0.8);
     CHECK(C.size() == 7 && C.vv_collisions.size() == 3 && C.ev_collisions.size() == 4);
+    C.set_collision_set_type(NormalCollisions::CollisionSetType::OGC);
+    C.build(mesh, V, 0.25, 0.8);
+    CHECK(C.size() == 6 && C.vv_collisions.size() == 2 && C.ev_collisions.size() == 4);
 }
 
 int main()

@@ -158,9 +158,9 @@ def test_mollified_star(ctx, capi, O):
     assert abs(ctx.energy(0.05, 1.0) - e_o) <= 1e-12 * abs(e_o)
 
 
-def test_rejects_ogc(ctx, capi):
+def test_rejects_unknown_type(ctx, capi):
     with pytest.raises(capi.IPCGError):
-        ctx.set_collision_set_type(2)
+        ctx.set_collision_set_type(3)
 
 
 def test_benchmark_size_property(ctx, capi, scenes):
