@@ -296,6 +296,40 @@ int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
                          const int32_t *E, const int32_t *F, double dhat,
                          double kappa, int psd, ipco_csr *out);
 void ipco_csr_free(ipco_csr *m);
+/* ---- friction ------------------------------------------------------------
+ * TangentialCollisions::build(mesh, V, collisions, BarrierPotential(dhat,
+ * kappa), mu_s, mu_k) with global coefficients (tangential_collisions.cpp:
+ * 61-186; tangential_collision.cpp:8-29): one record per collision of `c`
+ * (its order); near-parallel edge-edge collisions are not kept.  beta: the
+ * closest-point coordinates (2), basis: the tangent basis (3 x 2 column
+ * major), normal_force: barrier_force_magnitude at V, weight: the
+ * collision's weight.  Energy / gradient / Hessian of FrictionPotential(eps_v)
+ * at velocities Vel (tangential_potential.cpp:156-331), isotropic. */
+typedef struct ipco_friction {
+    int64_t n;
+    uint8_t *keep;
+    double *beta;
+    double *basis;
+    double *normal_force;
+    double *weight;
+    double mu_s, mu_k;
+} ipco_friction;
+int ipco_friction_build(const ipco_collisions *c, const double *V, int nV, const int32_t *E,
+                        const int32_t *F, double dhat, double kappa, double mu_s, double mu_k,
+                        ipco_friction *out);
+void ipco_friction_free(ipco_friction *f);
+double ipco_friction_energy(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
+                            int nV, const int32_t *E, const int32_t *F, double eps_v);
+void ipco_friction_gradient(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
+                            int nV, const int32_t *E, const int32_t *F, double eps_v, double *grad);
+int ipco_friction_hessian(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
+                          int nV, const int32_t *E, const int32_t *F, double eps_v, int psd,
+                          ipco_csr *out);
+/* smooth_mu.cpp / smooth_friction_mollifier.cpp scalar functions: which =
+ * 0 mu, 1 mu', 2 mu f0, 3 mu f1, 4 (mu f1)', 5 mu f1/x, 6 (mu f2 x - mu f1)/x^3,
+ * 7 f0, 8 f1, 9 f2, 10 f1/x, 11 (f2 x - f1)/x^3 */
+double ipco_smooth_mu_fn(int which, double y, double mu_s, double mu_k, double eps_v);
+
 
 /* ---- Gauss-Newton helpers (normal_potential.cpp:15-83, 236-308) ---------- */
 /* distance vector t of collision i (collision order vv, ev, ee, fv) and its

@@ -1065,28 +1065,19 @@ static int local_hessian(const stencil *s, const double *x, double dmin,
     return ipco_project_to_psd(H, nd, psd);
 }
 
-int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
-                         const int32_t *E, const int32_t *F, double dhat,
-                         double kappa, int psd, ipco_csr *out)
+/* local 3nv x 3nv matrices (144 doubles per collision) -> CSR: triplets,
+ * skipping exact zeros (local_to_global.hpp:289-301), summed per entry;
+ * keep (optional) selects the collisions assembled */
+static void csr_from_local(const ipco_collisions *c, const int32_t *E, const int32_t *F, int nV,
+                           const double *loc, const uint8_t *keep, ipco_csr *out)
 {
     const int64_t n = n_collisions(c);
     const int64_t ndof = 3 * (int64_t)nV;
     memset(out, 0, sizeof(*out));
     out->n = (int32_t)ndof;
-    double *loc = (double *)malloc(sizeof(double) * 144 * (size_t)(n > 0 ? n : 1));
-    int rc = 0;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t i = 0; i < n; i++) {
-        stencil s;
-        double x[12];
-        get_stencil(c, E, F, i, &s);
-        gather(V, &s, x);
-        if (local_hessian(&s, x, c->dmin, dhat, kappa, psd, loc + 144 * i) != 0)
-            rc = -1;
-    }
-    /* triplets, skipping exact zeros (local_to_global.hpp:289-301) */
     int64_t nt = 0;
     for (int64_t i = 0; i < n; i++) {
+        if (keep && !keep[i]) continue;
         stencil s;
         get_stencil(c, E, F, i, &s);
         const int nd = 3 * s.nv;
@@ -1096,6 +1087,7 @@ int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
     int64_t *src = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nt > 0 ? nt : 1));
     int64_t t = 0;
     for (int64_t i = 0; i < n; i++) {
+        if (keep && !keep[i]) continue;
         stencil s;
         get_stencil(c, E, F, i, &s);
         const int nd = 3 * s.nv;
@@ -1133,6 +1125,29 @@ int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
     for (int64_t r = 0; r < ndof; r++) out->rowptr[r + 1] += out->rowptr[r];
     free(key);
     free(src);
+}
+
+
+int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
+                         const int32_t *E, const int32_t *F, double dhat,
+                         double kappa, int psd, ipco_csr *out)
+{
+    const int64_t n = n_collisions(c);
+    const int64_t ndof = 3 * (int64_t)nV;
+    memset(out, 0, sizeof(*out));
+    out->n = (int32_t)ndof;
+    double *loc = (double *)malloc(sizeof(double) * 144 * (size_t)(n > 0 ? n : 1));
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        if (local_hessian(&s, x, c->dmin, dhat, kappa, psd, loc + 144 * i) != 0)
+            rc = -1;
+    }
+    csr_from_local(c, E, F, nV, loc, NULL, out);
     free(loc);
     return rc;
 }
@@ -1412,3 +1427,371 @@ double ipco_noncandidate_conservative_stepsize(const double *V0, const double *V
     return 0.5 * dhat / mx;
 }
 
+
+/* ======================================================================
+ * Friction: TangentialCollisions::build over a normal collision set with
+ * global friction coefficients (tangential_collisions.cpp:61-186 with
+ * uniform per-vertex mu, so every blend returns it) and FrictionPotential
+ * energy / gradient / Hessian (tangential_potential.cpp:156-331,
+ * friction_potential.cpp, smooth_mu.cpp, smooth_friction_mollifier.cpp),
+ * isotropic (mu_aniso = 1).
+ * ==================================================================== */
+
+/* smooth_friction_mollifier.cpp:8-53 */
+static double sf_f0(double y, double ev)
+{
+    if (fabs(y) >= ev) return y;
+    return y * y * (1 - y / (3 * ev)) / ev + ev / 3;
+}
+static double sf_f1(double y, double ev)
+{
+    if (fabs(y) >= ev) return 1;
+    const double t = y / ev;
+    return t * (2 - t);
+}
+static double sf_f2(double y, double ev)
+{
+    if (fabs(y) >= ev) return 0;
+    return (2 - 2 * y / ev) / ev;
+}
+static double sf_f1_over_x(double y, double ev)
+{
+    if (fabs(y) >= ev) return 1 / y;
+    return (2 - y / ev) / ev;
+}
+static double sf_f2_x_minus_f1_over_x3(double y, double ev)
+{
+    if (fabs(y) >= ev) return -1 / (y * y * y);
+    return -1 / (y * ev * ev);
+}
+
+/* smooth_mu.cpp:10-112 */
+static double smooth_mu(double y, double ms, double mk, double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return mk;
+    const double z = fabs(y) / ev;
+    if (fabs(y) < 0.5 * ev) return 2 * (mk - ms) * z * z + ms;
+    return -2 * (mk - ms) * (z * (z - 2) + 1) + mk;
+}
+static double smooth_mu_derivative(double y, double ms, double mk, double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return 0;
+    const double z = fabs(y) / ev;
+    if (fabs(y) < 0.5 * ev) return 4 * (mk - ms) * z / ev;
+    return -4 * (mk - ms) * (z - 1) / ev;
+}
+static double smooth_mu_f0(double y, double ms, double mk, double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return mk * sf_f0(y, ev);
+    const double dm = mk - ms;
+    const double z = fabs(y) / ev;
+    if (fabs(y) < 0.5 * ev)
+        return y * z * (z * (z * (1 - 0.4 * z) * dm - ms / 3.0) + ms) + (9.0 / 16.0) * ev * mk
+            - (11.0 / 48.0) * ev * ms;
+    return y * z * (z * (z * (0.4 * z - 2) * dm + (3 * mk - (10.0 / 3.0) * ms)) + (2 * ms - mk))
+        + 0.6 * ev * mk - (4.0 / 15.0) * ev * ms;
+}
+static double smooth_mu_f1(double y, double ms, double mk, double ev)
+{
+    return smooth_mu(y, ms, mk, ev) * sf_f1(y, ev);
+}
+static double smooth_mu_f2(double y, double ms, double mk, double ev)
+{
+    return smooth_mu_derivative(y, ms, mk, ev) * sf_f1(y, ev) + smooth_mu(y, ms, mk, ev) * sf_f2(y, ev);
+}
+static double smooth_mu_f1_over_x(double y, double ms, double mk, double ev)
+{
+    return smooth_mu(y, ms, mk, ev) * sf_f1_over_x(y, ev);
+}
+static double smooth_mu_f2_x_minus_f1_over_x3(double y, double ms, double mk, double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return mk * sf_f2_x_minus_f1_over_x3(y, ev);
+    const double dm = mk - ms;
+    const double z = 1 / ev;
+    if (fabs(y) < 0.5 * ev) return z * z * (z * (8 - 6 * y * z) * dm - ms / y);
+    return z * z * (z * (6 * y * z - 16) * dm + (9 * mk - 10 * ms) / y);
+}
+
+double ipco_smooth_mu_fn(int which, double y, double ms, double mk, double ev)
+{
+    switch (which) {
+    case 0: return smooth_mu(y, ms, mk, ev);
+    case 1: return smooth_mu_derivative(y, ms, mk, ev);
+    case 2: return smooth_mu_f0(y, ms, mk, ev);
+    case 3: return smooth_mu_f1(y, ms, mk, ev);
+    case 4: return smooth_mu_f2(y, ms, mk, ev);
+    case 5: return smooth_mu_f1_over_x(y, ms, mk, ev);
+    case 6: return smooth_mu_f2_x_minus_f1_over_x3(y, ms, mk, ev);
+    case 7: return sf_f0(y, ev);
+    case 8: return sf_f1(y, ev);
+    case 9: return sf_f2(y, ev);
+    case 10: return sf_f1_over_x(y, ev);
+    default: return sf_f2_x_minus_f1_over_x3(y, ev);
+    }
+}
+
+/* Eigen normalized(): divide by the norm when it is positive */
+static void v3_normalize(const double *a, double *o)
+{
+    const double z = v3_sqnorm(a);
+    const double n = z > 0 ? sqrt(z) : 1.0;
+    for (int k = 0; k < 3; k++) o[k] = z > 0 ? a[k] / n : a[k];
+}
+
+/* tangent_basis.cpp:17-175; P column-major 3 x 2 */
+static void tangent_basis(int kind, const double *x, double *P)
+{
+    double a[3], b[3], c[3];
+    if (kind == 3) { /* point-point */
+        const double *p0 = x, *p1 = x + 3;
+        double d[3], cx[3], cy[3];
+        const double ux[3] = { 1, 0, 0 }, uy[3] = { 0, 1, 0 };
+        v3_sub(p1, p0, d);
+        v3_cross(ux, d, cx);
+        v3_cross(uy, d, cy);
+        const double *w = v3_sqnorm(cx) > v3_sqnorm(cy) ? cx : cy;
+        v3_normalize(w, P);
+        v3_cross(d, w, c);
+        v3_normalize(c, P + 3);
+        return;
+    }
+    if (kind == 2) { /* point-edge: p, e0, e1 */
+        v3_sub(x + 6, x + 3, a);
+        v3_sub(x, x + 3, b);
+        v3_normalize(a, P);
+        v3_cross(a, b, c);
+        v3_normalize(c, P + 3);
+        return;
+    }
+    /* edge-edge (ea0, ea1, eb0, eb1) or point-triangle (p, t0, t1, t2):
+     * along the first edge, then normal x edge */
+    if (kind == 1) {
+        v3_sub(x + 3, x, a);
+        v3_sub(x + 9, x + 6, b);
+    } else {
+        v3_sub(x + 6, x + 3, a);
+        v3_sub(x + 9, x + 3, b);
+    }
+    double nrm[3];
+    v3_cross(a, b, nrm);
+    v3_normalize(a, P);
+    v3_cross(nrm, a, c);
+    v3_normalize(c, P + 3);
+}
+
+/* closest-point coordinates (closest_point.cpp:11-137) and the
+ * relative-velocity Jacobian coefficients Gamma (relative_velocity.cpp) */
+static void friction_coords(int kind, const double *x, double *beta, double *coef)
+{
+    beta[0] = beta[1] = 0;
+    coef[0] = coef[1] = coef[2] = coef[3] = 0;
+    switch (kind) {
+    case 3: coef[0] = 1; coef[1] = -1; break;
+    case 2:
+        beta[0] = pe_closest(x, x + 3, x + 6);
+        coef[0] = 1; coef[1] = beta[0] - 1; coef[2] = -beta[0];
+        break;
+    case 1:
+        ee_closest(x, x + 3, x + 6, x + 9, beta);
+        coef[0] = 1 - beta[0]; coef[1] = beta[0]; coef[2] = beta[1] - 1; coef[3] = -beta[1];
+        break;
+    default:
+        pt_closest(x, x + 3, x + 6, x + 9, beta);
+        coef[0] = 1; coef[1] = beta[0] + beta[1] - 1; coef[2] = -beta[0]; coef[3] = -beta[1];
+    }
+}
+
+int ipco_friction_build(const ipco_collisions *c, const double *V, int nV, const int32_t *E,
+                        const int32_t *F, double dhat, double kappa, double mu_s, double mu_k,
+                        ipco_friction *out)
+{
+    (void)nV;
+    const int64_t n = n_collisions(c);
+    memset(out, 0, sizeof(*out));
+    out->n = n;
+    out->mu_s = mu_s;
+    out->mu_k = mu_k;
+    out->keep = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
+    out->beta = (double *)calloc(2 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    out->basis = (double *)calloc(6 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    out->normal_force = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    out->weight = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        stencil s;
+        double x[12], coef[4];
+        get_stencil(c, E, F, i, &s);
+        gather(V, &s, x);
+        double d2;
+        if (s.kind == 1) {
+            /* near-parallel edge pairs are skipped (tangential_collisions.cpp:
+             * 118-121); the rest are line-line (edge_edge.hpp:27-31) */
+            if (ipco_edge_edge_cross_squarednorm(x, x + 3, x + 6, x + 9) < s.eps_x) continue;
+            d2 = ipco_edge_edge_distance(x, x + 3, x + 6, x + 9, IPCO_EE_EA_EB);
+        } else if (s.kind == 3) {
+            d2 = ipco_point_point_distance(x, x + 3);
+        } else if (s.kind == 2) {
+            d2 = ipco_point_edge_distance(x, x + 3, x + 6,
+                                          ipco_point_edge_distance_type(x, x + 3, x + 6));
+        } else {
+            d2 = ipco_point_triangle_distance(
+                x, x + 3, x + 6, x + 9, ipco_point_triangle_distance_type(x, x + 3, x + 6, x + 9));
+        }
+        out->keep[i] = 1;
+        friction_coords(s.kind, x, out->beta + 2 * i, coef);
+        tangent_basis(s.kind, x, out->basis + 6 * i);
+        /* barrier_force_magnitude.cpp:5-15 */
+        out->normal_force[i] = -f1(d2, c->dmin, dhat, kappa) * 2 * sqrt(d2);
+        out->weight[i] = s.w;
+    }
+    return 0;
+}
+
+void ipco_friction_free(ipco_friction *f)
+{
+    free(f->keep);
+    free(f->beta);
+    free(f->basis);
+    free(f->normal_force);
+    free(f->weight);
+    memset(f, 0, sizeof(*f));
+}
+
+/* relative velocity of the closest points (relative_velocity.cpp) */
+static void relative_velocity(int kind, const double *v, const double *beta, double *r)
+{
+    for (int k = 0; k < 3; k++) {
+        switch (kind) {
+        case 3: r[k] = v[k] - v[3 + k]; break;
+        case 2: r[k] = v[k] - ((v[6 + k] - v[3 + k]) * beta[0] + v[3 + k]); break;
+        case 1:
+            r[k] = ((v[3 + k] - v[k]) * beta[0] + v[k]) - ((v[9 + k] - v[6 + k]) * beta[1] + v[6 + k]);
+            break;
+        default:
+            r[k] = v[k] - ((v[3 + k] + beta[0] * (v[6 + k] - v[3 + k])) + beta[1] * (v[9 + k] - v[3 + k]));
+        }
+    }
+}
+
+/* u = P^T r */
+static void tangent_project(const double *P, const double *r, double *u)
+{
+    u[0] = v3_dot(P, r);
+    u[1] = v3_dot(P + 3, r);
+}
+
+static void friction_local(const ipco_friction *fr, const stencil *s, int64_t i, const double *v,
+                           double eps_v, double *energy, double *grad, double *hess, int psd)
+{
+    double r[3], u[2], coef[4], beta2[2];
+    const double *P = fr->basis + 6 * i, *beta = fr->beta + 2 * i;
+    relative_velocity(s->kind, v, beta, r);
+    tangent_project(P, r, u);
+    const double norm = sqrt(u[0] * u[0] + u[1] * u[1]);
+    const double scale = fr->weight[i] * fr->normal_force[i];
+    const double ms = fr->mu_s, mk = fr->mu_k;
+    (void)beta2;
+    /* Gamma coefficients from the stored coordinates */
+    switch (s->kind) {
+    case 3: coef[0] = 1; coef[1] = -1; coef[2] = coef[3] = 0; break;
+    case 2: coef[0] = 1; coef[1] = beta[0] - 1; coef[2] = -beta[0]; coef[3] = 0; break;
+    case 1: coef[0] = 1 - beta[0]; coef[1] = beta[0]; coef[2] = beta[1] - 1; coef[3] = -beta[1]; break;
+    default: coef[0] = 1; coef[1] = beta[0] + beta[1] - 1; coef[2] = -beta[0]; coef[3] = -beta[1];
+    }
+    if (energy) *energy = scale * smooth_mu_f0(norm, ms, mk, eps_v);
+    const int nd = 3 * s->nv;
+    if (grad) {
+        const double f1x = smooth_mu_f1_over_x(norm, ms, mk, eps_v);
+        const double g0 = f1x * u[0] * scale, g1 = f1x * u[1] * scale;
+        for (int a = 0; a < s->nv; a++)
+            for (int k = 0; k < 3; k++)
+                grad[3 * a + k] = (coef[a] * P[k]) * g0 + (coef[a] * P[3 + k]) * g1;
+    }
+    if (hess) {
+        /* inner 2 x 2 matrix M, H = T M T^T with T[3a + k][c] = coef_a P[k][c] */
+        double M[4] = { 0, 0, 0, 0 };
+        const double f1x = smooth_mu_f1_over_x(norm, ms, mk, eps_v);
+        if (norm > eps_v) { /* dynamic: f2 = 0 */
+            if (!(psd != 0 && scale <= 0)) {
+                const double up[2] = { -u[1], u[0] };
+                const double cc = scale * f1x / (norm * norm);
+                M[0] = cc * up[0] * up[0];
+                M[1] = M[2] = cc * up[0] * up[1];
+                M[3] = cc * up[1] * up[1];
+            }
+        } else if (norm == 0) {
+            if (!(psd != 0 && scale <= 0)) M[0] = M[3] = scale * f1x;
+        } else {
+            const double f2 = smooth_mu_f2_x_minus_f1_over_x3(norm, ms, mk, eps_v);
+            M[0] = (f2 * u[0] * u[0] + f1x) * scale;
+            M[1] = M[2] = (f2 * u[0] * u[1]) * scale;
+            M[3] = (f2 * u[1] * u[1] + f1x) * scale;
+            ipco_project_to_psd(M, 2, psd);
+        }
+        for (int a = 0; a < nd; a++)
+            for (int b = 0; b < nd; b++) {
+                const double ta0 = coef[a / 3] * P[a % 3], ta1 = coef[a / 3] * P[3 + a % 3];
+                const double tb0 = coef[b / 3] * P[b % 3], tb1 = coef[b / 3] * P[3 + b % 3];
+                hess[a * nd + b] = ta0 * (M[0] * tb0 + M[1] * tb1) + ta1 * (M[2] * tb0 + M[3] * tb1);
+            }
+    }
+}
+
+static void gather_velocities(const double *Vel, const stencil *s, double *v)
+{
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 3; k++) v[3 * i + k] = i < s->nv ? Vel[3 * (size_t)s->ids[i] + k] : 0.0;
+}
+
+double ipco_friction_energy(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
+                            int nV, const int32_t *E, const int32_t *F, double eps_v)
+{
+    (void)nV;
+    double e = 0;
+    for (int64_t i = 0; i < fr->n; i++) {
+        if (!fr->keep[i]) continue;
+        stencil s;
+        double v[12], ei;
+        get_stencil(c, E, F, i, &s);
+        gather_velocities(Vel, &s, v);
+        friction_local(fr, &s, i, v, eps_v, &ei, NULL, NULL, 0);
+        e += ei;
+    }
+    return e;
+}
+
+void ipco_friction_gradient(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
+                            int nV, const int32_t *E, const int32_t *F, double eps_v, double *grad)
+{
+    memset(grad, 0, sizeof(double) * 3 * (size_t)nV);
+    for (int64_t i = 0; i < fr->n; i++) {
+        if (!fr->keep[i]) continue;
+        stencil s;
+        double v[12], g[12];
+        get_stencil(c, E, F, i, &s);
+        gather_velocities(Vel, &s, v);
+        friction_local(fr, &s, i, v, eps_v, NULL, g, NULL, 0);
+        for (int a = 0; a < s.nv; a++)
+            for (int k = 0; k < 3; k++) grad[3 * (size_t)s.ids[a] + k] += g[3 * a + k];
+    }
+}
+
+int ipco_friction_hessian(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
+                          int nV, const int32_t *E, const int32_t *F, double eps_v, int psd,
+                          ipco_csr *out)
+{
+    const int64_t n = fr->n;
+    double *loc = (double *)calloc(144 * (size_t)(n > 0 ? n : 1), sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        if (!fr->keep[i]) continue;
+        stencil s;
+        double v[12];
+        get_stencil(c, E, F, i, &s);
+        gather_velocities(Vel, &s, v);
+        friction_local(fr, &s, i, v, eps_v, NULL, NULL, loc + 144 * i, psd);
+    }
+    csr_from_local(c, E, F, nV, loc, fr->keep, out);
+    free(loc);
+    return 0;
+}

@@ -48,6 +48,12 @@ class _CSR(C.Structure):
                 ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
 
 
+class _Friction(C.Structure):
+    _fields_ = [("n", C.c_int64), ("keep", C.POINTER(C.c_uint8)), ("beta", C.POINTER(C.c_double)),
+                ("basis", C.POINTER(C.c_double)), ("normal_force", C.POINTER(C.c_double)),
+                ("weight", C.POINTER(C.c_double)), ("mu_s", C.c_double), ("mu_k", C.c_double)]
+
+
 class CCDParams(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_long),
                 ("conservative_rescaling", C.c_double)]
@@ -181,6 +187,18 @@ def lib():
                                            C.c_int]
         L.ipco_ogc_feasible.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, C.c_int,
                                         C.c_int, C.c_int]
+        L.ipco_friction_build.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.POINTER(_Friction)]
+        L.ipco_friction_free.argtypes = [C.POINTER(_Friction)]
+        L.ipco_friction_energy.argtypes = [C.POINTER(_Friction), C.POINTER(_Collisions), _D, C.c_int,
+                                           _I, _I, C.c_double]
+        L.ipco_friction_energy.restype = C.c_double
+        L.ipco_friction_gradient.argtypes = [C.POINTER(_Friction), C.POINTER(_Collisions), _D,
+                                             C.c_int, _I, _I, C.c_double, _D]
+        L.ipco_friction_hessian.argtypes = [C.POINTER(_Friction), C.POINTER(_Collisions), _D,
+                                            C.c_int, _I, _I, C.c_double, C.c_int, C.POINTER(_CSR)]
+        L.ipco_smooth_mu_fn.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.ipco_smooth_mu_fn.restype = C.c_double
         L.ipco_mesh_areas.argtypes = [_D, C.c_int, _I, C.c_int, _I, C.c_int, _I, _D, _D]
         L.ipco_set_num_threads.argtypes = [C.c_int]
         L.ipco_get_max_threads.restype = C.c_int
@@ -764,3 +782,84 @@ def ogc_feasible(V, E, F, kind, a, b):
     keep, args = _mesh_args(V, E, F)
     k = {"ev": 1, "ee": 2, "fv": 3}[kind]
     return bool(lib().ipco_ogc_feasible(*args, k, int(a), int(b)))
+
+
+# ---- friction ----------------------------------------------------------------
+SMOOTH_MU_FN = {"mu": 0, "dmu": 1, "mu_f0": 2, "mu_f1": 3, "mu_f2": 4, "mu_f1_over_x": 5,
+                "mu_f2_x_minus_f1_over_x3": 6, "f0": 7, "f1": 8, "f2": 9, "f1_over_x": 10,
+                "f2_x_minus_f1_over_x3": 11}
+
+
+def smooth_mu_fn(name, y, mu_s, mu_k, eps_v):
+    """smooth_mu.cpp / smooth_friction_mollifier.cpp scalar functions"""
+    return lib().ipco_smooth_mu_fn(SMOOTH_MU_FN[name], float(y), float(mu_s), float(mu_k),
+                                   float(eps_v))
+
+
+class Friction:
+    """TangentialCollisions over a Collisions set (one record per collision,
+    `keep` false for skipped near-parallel edge pairs)."""
+
+    def __init__(self, s, coll, E, F):
+        self._s, self._coll, self._E, self._F = s, coll, E, F
+        n = s.n
+
+        def arr(p, k, dt=np.float64):
+            if n == 0:
+                return np.zeros((0, k) if k > 1 else (0,), dt)
+            a = np.ctypeslib.as_array(p, shape=(n * k,)).copy()
+            return a.reshape(n, k) if k > 1 else a
+
+        self.keep = arr(s.keep, 1, np.uint8).astype(bool)
+        self.beta = arr(s.beta, 2)
+        self.basis = arr(s.basis, 6).reshape(n, 2, 3).transpose(0, 2, 1)  # n x 3 x 2
+        self.normal_force = arr(s.normal_force, 1)
+        self.weight = arr(s.weight, 1)
+
+    def __del__(self):
+        try:
+            lib().ipco_friction_free(C.byref(self._s))
+        except Exception:
+            pass
+
+
+def friction_build(coll, V, E, F, dhat, kappa, mu_s, mu_k=None):
+    V, pV = _d(V)
+    E, pE = _i(np.asarray(E).reshape(-1, 2))
+    F, pF = _i(np.asarray(F).reshape(-1, 3))
+    s = _Friction()
+    lib().ipco_friction_build(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa, mu_s,
+                              mu_s if mu_k is None else mu_k, C.byref(s))
+    return Friction(s, coll, E, F)
+
+
+def _fargs(fr, Vel):
+    Vel, pVel = _d(Vel)
+    _, pE = _i(fr._E)
+    _, pF = _i(fr._F)
+    return Vel, (C.byref(fr._s), C.byref(fr._coll._s), pVel, len(Vel), pE, pF)
+
+
+def friction_energy(fr, Vel, eps_v):
+    keep, a = _fargs(fr, Vel)
+    return lib().ipco_friction_energy(*a, eps_v)
+
+
+def friction_gradient(fr, Vel, eps_v):
+    keep, a = _fargs(fr, Vel)
+    g = np.zeros(3 * len(keep))
+    lib().ipco_friction_gradient(*a, eps_v, g.ctypes.data_as(_D))
+    return g
+
+
+def friction_hessian(fr, Vel, eps_v, psd="clamp"):
+    import scipy.sparse as sp
+    keep, a = _fargs(fr, Vel)
+    m = _CSR()
+    lib().ipco_friction_hessian(*a, eps_v, PSD[psd], C.byref(m))
+    n, nnz = m.n, m.nnz
+    rowptr = np.ctypeslib.as_array(m.rowptr, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(m.col, shape=(max(nnz, 1),))[:nnz].copy()
+    val = np.ctypeslib.as_array(m.val, shape=(max(nnz, 1),))[:nnz].copy()
+    lib().ipco_csr_free(C.byref(m))
+    return sp.csr_matrix((val, col, rowptr), shape=(n, n))
