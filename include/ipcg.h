@@ -214,6 +214,35 @@ int ipcg_barrier_gauss_newton_diagonal(ipcg_ctx* ctx, double dhat, double kappa,
 int ipcg_barrier_gauss_newton_quadratic_form(ipcg_ctx* ctx, double dhat, double kappa,
                                              const double* p, double* value);
 
+/* ---- friction (tangential_collisions.cpp, tangential_potential.cpp) ------
+ * TangentialCollisions::build(mesh, V, collisions, BarrierPotential(dhat,
+ * kappa), mu_s, mu_k) with global coefficients (tangential_collisions.cpp:
+ * 61-186; every per-vertex blend of equal coefficients returns them): over
+ * the current collision set at the slot-0 positions (the lagged
+ * positions), one record per collision: closest-point coordinates
+ * (closest_point.cpp), tangent basis (tangent_basis.cpp), normal force
+ * magnitude -kappa b'(d^2) 2 d (barrier_force_magnitude.cpp:5-15) times the
+ * collision's weight; near-parallel edge-edge collisions (cross product
+ * below eps_x) are not kept (tangential_collisions.cpp:118-121).  A new
+ * collision build or mesh upload discards the friction set. */
+int ipcg_friction_build(ipcg_ctx* ctx, double dhat, double kappa, double mu_s, double mu_k);
+/* per collision in download order (vv, ev, ee, fv): keep flag,
+ * closest-point coordinates (2), tangent basis (3 x 2, column major),
+ * normal force magnitude (0 when not kept); any pointer may be NULL */
+int ipcg_friction_download(ipcg_ctx* ctx, int32_t* keep, double* closest_point,
+                           double* tangent_basis, double* normal_force);
+/* velocities for the friction potential (nV x 3, layout as positions) */
+int ipcg_velocities_upload(ipcg_ctx* ctx, const double* V, int layout);
+/* FrictionPotential(eps_v) (friction_potential.cpp, smooth_mu.cpp,
+ * smooth_friction_mollifier.cpp) at the uploaded velocities
+ * (tangential_potential.cpp:156-331), isotropic: energy, gradient (host
+ * array of 3*nV doubles, or NULL to keep it on device) and Hessian with
+ * the reference's projection of the inner 2 x 2 matrix (psd as for the
+ * barrier; CSR read with ipcg_hessian_download). */
+int ipcg_friction_energy(ipcg_ctx* ctx, double eps_v, double* out);
+int ipcg_friction_gradient(ipcg_ctx* ctx, double eps_v, double* grad);
+int ipcg_friction_hessian(ipcg_ctx* ctx, double eps_v, int psd, int64_t* nnz);
+
 /* ---- CCD (ipc.cpp:45-101, candidates.cpp:252-292, tight_inclusion_ccd) --
  * compute_collision_free_stepsize: swept broad phase (inflation
  * 0.5*min_distance) + Tight-Inclusion CCD of every candidate, slot 0 ->

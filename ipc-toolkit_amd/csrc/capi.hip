@@ -175,7 +175,7 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.adj_ev, &c.adj_vf_off, &c.adj_bnd, &c.np_rw, &c.np_flag, &c.np_pos, &c.np_rep,
                        &c.im_slot, &c.im_slot2, &c.im_vals, &c.im_vals2, &c.im_ukeys,
                        &c.im_start, &c.im_cnt, &c.im_off, &c.im_vv, &c.im_vvw, &c.im_ev,
-                       &c.im_evw, &c.im_ee,
+                       &c.im_evw, &c.im_ee, &c.fric, &c.vel,
                        &c.np_cw };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
@@ -405,6 +405,7 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
         c.has_mesh = true;
         c.has_pos[0] = c.has_pos[1] = false;
         c.cand_valid = c.coll_valid = c.hess_valid = false;
+        c.fric_valid = c.has_vel = false;
         c.scap_ee = c.scap_fv = c.scap_vv = c.scap_ev = 0;
     });
 }
@@ -636,6 +637,61 @@ int ipcg_barrier_hessian(ipcg_ctx* ctx, double dhat, double kappa, int psd, int6
 {
     return guard(ctx, [&](Ctx& c) {
         pot_hessian(c, dhat, kappa, psd);
+        if (nnz) *nnz = c.nnz;
+    });
+}
+
+int ipcg_velocities_upload(ipcg_ctx* ctx, const double* V, int layout)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.require_mesh();
+        if (c.nV > 0 && !V) throw Error(IPCG_ERR_ARG, "null velocities");
+        upload_positions(c, c.vel, V, layout, false);
+        c.has_vel = true;
+    });
+}
+
+int ipcg_friction_build(ipcg_ctx* ctx, double dhat, double kappa, double mu_s, double mu_k)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(dhat > 0) || !(kappa > 0) || !(mu_s >= 0) || !(mu_k >= 0))
+            throw Error(IPCG_ERR_ARG, "dhat and kappa must be > 0, mu_s and mu_k >= 0");
+        fric_build(c, dhat, kappa, mu_s, mu_k);
+    });
+}
+
+int ipcg_friction_download(ipcg_ctx* ctx, int32_t* keep, double* closest_point,
+                           double* tangent_basis, double* normal_force)
+{
+    return guard(ctx, [&](Ctx& c) { fric_download(c, keep, closest_point, tangent_basis, normal_force); });
+}
+
+int ipcg_friction_energy(ipcg_ctx* ctx, double eps_v, double* out)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(eps_v > 0)) throw Error(IPCG_ERR_ARG, "eps_v must be > 0");
+        const double e = fric_energy(c, eps_v);
+        if (out) *out = e;
+    });
+}
+
+int ipcg_friction_gradient(ipcg_ctx* ctx, double eps_v, double* grad)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(eps_v > 0)) throw Error(IPCG_ERR_ARG, "eps_v must be > 0");
+        fric_gradient(c, eps_v);
+        if (grad && c.nV) {
+            IPCG_HIP(hipMemcpyAsync(grad, c.grad.p, sizeof(double) * 3 * c.nV, hipMemcpyDeviceToHost, c.stream));
+            IPCG_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int ipcg_friction_hessian(ipcg_ctx* ctx, double eps_v, int psd, int64_t* nnz)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(eps_v > 0)) throw Error(IPCG_ERR_ARG, "eps_v must be > 0");
+        fric_hessian(c, eps_v, psd);
         if (nnz) *nnz = c.nnz;
     });
 }

@@ -910,6 +910,7 @@ void np_build(Ctx& c, double dhat, double dmin)
     c.coll_dmin = dmin;
     c.coll_valid = true;
     c.hess_valid = false;
+    c.fric_valid = false; // a friction set belongs to the collision set it was built on
     c.pattern_valid = false;
     c.inc_valid = false;
 }

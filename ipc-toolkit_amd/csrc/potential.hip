@@ -18,6 +18,8 @@
 #include "sort.hpp"
 #include "stencil.hpp"
 
+#include <vector>
+
 namespace ipcg {
 
 namespace {
@@ -1056,14 +1058,10 @@ bool hess_pattern_incidences(Ctx& c, int64_t M)
     return true;
 }
 
-void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
+// incidences and block pattern of the current collision set; the record
+// buffer sized for REC_LEN values per collision
+static CollRanges hess_prepare(Ctx& c, const CollView& C)
 {
-    require_collisions(c);
-    if (psd < 0 || psd > 2) throw Error(IPCG_ERR_ARG, "invalid PSD projection method");
-    hipStream_t s = c.stream;
-    CollView C = view(c);
-    BarrierParams bp { dhat, kappa, c.coll_dmin, psd };
-    const double* X = c.X[0].as<double>();
     int64_t M;
     const CollRanges G = ranges(C, M);
     require_incidences(c, C, G, M);
@@ -1072,16 +1070,23 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     c.stage_begin("hessian_pattern");
     if (!c.pattern_valid && !hess_pattern_incidences(c, M)) hess_pattern(c);
     c.stage_end();
-    const int64_t nb = std::max<int64_t>(c.nblocks, 1);
-    c.blk_mask.reserve(sizeof(uint32_t) * nb);
-    c.hq.reserve(sizeof(double) * 9 * nb);
-    uint32_t* bmask = c.blk_mask.as<uint32_t>();
-    double* bval = c.hq.as<double>();
-    const uint32_t* bptr = c.hp_bptr.as<uint32_t>();
-    const int32_t* bcol = c.hp_bcol.as<int32_t>();
+    c.hrec.reserve(sizeof(double) * REC_LEN * (size_t)std::max<int64_t>(G.goff[4], 1));
+    return G;
+}
+
+static void hess_assemble(Ctx& c);
+
+void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
+{
+    require_collisions(c);
+    if (psd < 0 || psd > 2) throw Error(IPCG_ERR_ARG, "invalid PSD projection method");
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    BarrierParams bp { dhat, kappa, c.coll_dmin, psd };
+    const double* X = c.X[0].as<double>();
+    const CollRanges G = hess_prepare(c, C);
     // projected Hessian records, REC_LEN values per collision in global
     // collision order
-    c.hrec.reserve(sizeof(double) * REC_LEN * (size_t)std::max<int64_t>(G.goff[4], 1));
     double* rec = c.hrec.as<double>();
     // one launch per stencil kind, each its own timed stage
     if (C.n[0]) {
@@ -1128,6 +1133,21 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
         c.stage_end();
     }
     IPCG_CHECK_LAUNCH();
+    hess_assemble(c);
+}
+
+// block rows from the records in c.hrec, then the CSR
+static void hess_assemble(Ctx& c)
+{
+    hipStream_t s = c.stream;
+    const double* rec = c.hrec.as<double>();
+    const int64_t nb = std::max<int64_t>(c.nblocks, 1);
+    c.blk_mask.reserve(sizeof(uint32_t) * nb);
+    c.hq.reserve(sizeof(double) * 9 * nb);
+    uint32_t* bmask = c.blk_mask.as<uint32_t>();
+    double* bval = c.hq.as<double>();
+    const uint32_t* bptr = c.hp_bptr.as<uint32_t>();
+    const int32_t* bcol = c.hp_bcol.as<int32_t>();
     c.stage_begin("hessian_rows");
     const unsigned gwr = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_hessian_rows, dim3(gwr), dim3(256), 0, s, rec, c.inc_ptr.as<uint32_t>(),
@@ -1235,6 +1255,530 @@ double pot_gn_quadratic_form(Ctx& c, double dhat, double kappa, const double* p_
     IPCG_HIP(hipMemcpyAsync(h, c.red_out.p, sizeof(double), hipMemcpyDeviceToHost, s));
     IPCG_HIP(hipStreamSynchronize(s));
     return h[0];
+}
+
+
+// ==========================================================================
+// Friction: TangentialCollisions::build over the current collision set with
+// global friction coefficients (tangential_collisions.cpp:61-186 with
+// uniform per-vertex mu) and FrictionPotential energy / gradient / Hessian
+// at velocities (tangential_potential.cpp:156-331, friction_potential.cpp,
+// smooth_mu.cpp, smooth_friction_mollifier.cpp), isotropic.  One record per
+// collision in global collision order, so the gradient rows and the
+// Hessian rows / CSR of the collision set are reused.  The relative
+// velocity is translation invariant, so a collision's Hessian T M T^T
+// (T = Gamma^T P, M 2 x 2) lies in the 4-point Q basis, where it is the
+// Kronecker product (s s^T) (x) (P M P^T) with s_k = sum_x q4(x, k) Gamma_x.
+// ==========================================================================
+namespace {
+
+struct FricRec {
+    double P[6];    // tangent basis, column major 3 x 2
+    double beta[2]; // closest-point coordinates
+    double s;       // weight * normal force magnitude; 0: not kept
+    double keep;    // 1 kept, 0 skipped (near-parallel edge-edge)
+};
+
+struct FricParams {
+    double eps_v, mu_s, mu_k;
+    int psd;
+};
+
+// smooth_friction_mollifier.cpp:8-53
+__device__ __forceinline__ double sf_f0(double y, double ev)
+{
+    if (fabs(y) >= ev) return y;
+    return y * y * (1 - y / (3 * ev)) / ev + ev / 3;
+}
+__device__ __forceinline__ double sf_f1_over_x(double y, double ev)
+{
+    if (fabs(y) >= ev) return 1 / y;
+    return (2 - y / ev) / ev;
+}
+__device__ __forceinline__ double sf_f2_x_minus_f1_over_x3(double y, double ev)
+{
+    if (fabs(y) >= ev) return -1 / (y * y * y);
+    return -1 / (y * ev * ev);
+}
+
+// smooth_mu.cpp:10-112
+__device__ __forceinline__ double smooth_mu(double y, double ms, double mk, double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return mk;
+    const double z = fabs(y) / ev;
+    if (fabs(y) < 0.5 * ev) return 2 * (mk - ms) * z * z + ms;
+    return -2 * (mk - ms) * (z * (z - 2) + 1) + mk;
+}
+__device__ __forceinline__ double smooth_mu_f0(double y, double ms, double mk, double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return mk * sf_f0(y, ev);
+    const double dm = mk - ms;
+    const double z = fabs(y) / ev;
+    if (fabs(y) < 0.5 * ev)
+        return y * z * (z * (z * (1 - 0.4 * z) * dm - ms / 3.0) + ms) + (9.0 / 16.0) * ev * mk
+            - (11.0 / 48.0) * ev * ms;
+    return y * z * (z * (z * (0.4 * z - 2) * dm + (3 * mk - (10.0 / 3.0) * ms)) + (2 * ms - mk))
+        + 0.6 * ev * mk - (4.0 / 15.0) * ev * ms;
+}
+__device__ __forceinline__ double smooth_mu_f1_over_x(double y, double ms, double mk, double ev)
+{
+    return smooth_mu(y, ms, mk, ev) * sf_f1_over_x(y, ev);
+}
+__device__ __forceinline__ double smooth_mu_f2_x_minus_f1_over_x3(double y, double ms, double mk,
+                                                                  double ev)
+{
+    if (ms == mk || fabs(y) >= ev) return mk * sf_f2_x_minus_f1_over_x3(y, ev);
+    const double dm = mk - ms;
+    const double z = 1 / ev;
+    if (fabs(y) < 0.5 * ev) return z * z * (z * (8 - 6 * y * z) * dm - ms / y);
+    return z * z * (z * (6 * y * z - 16) * dm + (9 * mk - 10 * ms) / y);
+}
+
+// Eigen normalized(): divide by the norm when it is positive
+__device__ __forceinline__ d3 normalized(const d3& a)
+{
+    const double z = sqn(a);
+    if (!(z > 0)) return a;
+    const double n = sqrt(z);
+    return { a.x / n, a.y / n, a.z / n };
+}
+
+// tangent_basis.cpp:17-175 (columns c0, c1) and closest-point coordinates
+// (closest_point.cpp:11-137)
+template <int KIND>
+__device__ void fric_geometry(const d3* P, d3& c0, d3& c1, double* beta)
+{
+    beta[0] = beta[1] = 0.0;
+    if constexpr (KIND == K_VV) {
+        const d3 d = sub(P[1], P[0]);
+        const d3 cx = cross(d3 { 1, 0, 0 }, d), cy = cross(d3 { 0, 1, 0 }, d);
+        const d3 w = sqn(cx) > sqn(cy) ? cx : cy;
+        c0 = normalized(w);
+        c1 = normalized(cross(d, w));
+    } else if constexpr (KIND == K_EV) {
+        const d3 e = sub(P[2], P[1]);
+        c0 = normalized(e);
+        c1 = normalized(cross(e, sub(P[0], P[1])));
+        beta[0] = pe_closest(P[0], P[1], P[2]);
+    } else if constexpr (KIND == K_EE) {
+        const d3 ea = sub(P[1], P[0]), eb = sub(P[3], P[2]);
+        const d3 nrm = cross(ea, eb);
+        c0 = normalized(ea);
+        c1 = normalized(cross(nrm, ea));
+        const d3 d = sub(P[0], P[2]);
+        solve_sym2_ldlt(sqn(ea), -dot(eb, ea), sqn(eb), -dot(d, ea), dot(d, eb), beta[0], beta[1]);
+    } else {
+        const d3 e0 = sub(P[2], P[1]), e1 = sub(P[3], P[1]);
+        const d3 nrm = cross(e0, e1);
+        c0 = normalized(e0);
+        c1 = normalized(cross(nrm, e0));
+        const d3 r = sub(P[0], P[1]);
+        solve_sym2_ldlt(dot(e0, e0), dot(e1, e0), dot(e1, e1), dot(e0, r), dot(e1, r), beta[0],
+                        beta[1]);
+    }
+}
+
+// relative-velocity Jacobian coefficients Gamma (relative_velocity.cpp)
+template <int KIND>
+__device__ __forceinline__ void fric_coef(const double* beta, double* g)
+{
+    g[0] = g[1] = g[2] = g[3] = 0.0;
+    if constexpr (KIND == K_VV) {
+        g[0] = 1;
+        g[1] = -1;
+    } else if constexpr (KIND == K_EV) {
+        g[0] = 1;
+        g[1] = beta[0] - 1;
+        g[2] = -beta[0];
+    } else if constexpr (KIND == K_EE) {
+        g[0] = 1 - beta[0];
+        g[1] = beta[0];
+        g[2] = beta[1] - 1;
+        g[3] = -beta[1];
+    } else {
+        g[0] = 1;
+        g[1] = beta[0] + beta[1] - 1;
+        g[2] = -beta[0];
+        g[3] = -beta[1];
+    }
+}
+
+// relative velocity of the closest points (relative_velocity.cpp)
+template <int KIND>
+__device__ __forceinline__ d3 fric_relative_velocity(const d3* v, const double* beta)
+{
+    if constexpr (KIND == K_VV) return sub(v[0], v[1]);
+    auto lerp = [](const d3& a, const d3& b, double t) {
+        return d3 { (b.x - a.x) * t + a.x, (b.y - a.y) * t + a.y, (b.z - a.z) * t + a.z };
+    };
+    if constexpr (KIND == K_EV) return sub(v[0], lerp(v[1], v[2], beta[0]));
+    if constexpr (KIND == K_EE) return sub(lerp(v[0], v[1], beta[0]), lerp(v[2], v[3], beta[1]));
+    const d3 t = { (v[1].x + beta[0] * (v[2].x - v[1].x)) + beta[1] * (v[3].x - v[1].x),
+                   (v[1].y + beta[0] * (v[2].y - v[1].y)) + beta[1] * (v[3].y - v[1].y),
+                   (v[1].z + beta[0] * (v[2].z - v[1].z)) + beta[1] * (v[3].z - v[1].z) };
+    return sub(v[0], t);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_fric_build(CollView C, const double* X, int nV,
+                                                   BarrierParams bp, FricRec* out)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
+    if (i >= C.n[KIND]) return;
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
+    d3 P[4];
+#pragma unroll
+    for (int j = 0; j < NP; j++) P[j] = ldx(X, nV, v[j]);
+    FricRec r = {};
+    double d2;
+    if constexpr (KIND == K_EE) {
+        // near-parallel pairs are skipped (tangential_collisions.cpp:118-121);
+        // the rest are line-line (tangential edge_edge.hpp:27-31)
+        if (ee_cross2(P[0], P[1], P[2], P[3]) < C.eps[i]) {
+            out[i] = r;
+            return;
+        }
+        d2 = ee_dist2(P[0], P[1], P[2], P[3], EE_EA_EB);
+    } else if constexpr (KIND == K_VV) {
+        d2 = pp_dist2(P[0], P[1]);
+    } else if constexpr (KIND == K_EV) {
+        d2 = pe_dist2(P[0], P[1], P[2], point_edge_type(P[0], P[1], P[2]));
+    } else {
+        d2 = pt_dist2(P[0], P[1], P[2], P[3], point_triangle_type(P[0], P[1], P[2], P[3]));
+    }
+    d3 c0, c1;
+    fric_geometry<KIND>(P, c0, c1, r.beta);
+    r.P[0] = c0.x;
+    r.P[1] = c0.y;
+    r.P[2] = c0.z;
+    r.P[3] = c1.x;
+    r.P[4] = c1.y;
+    r.P[5] = c1.z;
+    // barrier_force_magnitude.cpp:5-15
+    const double N = -bar_f1(d2, bp) * 2 * sqrt(d2);
+    r.s = C.w[KIND][i] * N;
+    r.keep = 1.0;
+    out[i] = r;
+}
+
+template <int KIND>
+__device__ __forceinline__ void fric_local(const FricRec& r, const double* Vel, int nV,
+                                           const int* v, double ev, double ms, double mk,
+                                           double* u, double& norm)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    d3 w[4];
+#pragma unroll
+    for (int j = 0; j < NP; j++) w[j] = ldx(Vel, nV, v[j]);
+    const d3 rel = fric_relative_velocity<KIND>(w, r.beta);
+    u[0] = dot(d3 { r.P[0], r.P[1], r.P[2] }, rel);
+    u[1] = dot(d3 { r.P[3], r.P[4], r.P[5] }, rel);
+    norm = sqrt(u[0] * u[0] + u[1] * u[1]);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_fric_energy(CollView C, const FricRec* R, const double* Vel,
+                                                    int nV, FricParams fp, double* partial)
+{
+    const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    double e = 0;
+    if (i < C.n[KIND] && R[i].keep != 0) {
+        int v[4];
+        stencil_ids<KIND>(C, i, v);
+        double u[2], norm;
+        fric_local<KIND>(R[i], Vel, nV, v, fp.eps_v, fp.mu_s, fp.mu_k, u, norm);
+        e = R[i].s * smooth_mu_f0(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+    }
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
+    __shared__ double sh[PB / WAVE];
+    if (threadIdx.x % WAVE == 0) sh[threadIdx.x / WAVE] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int j = 0; j < PB / WAVE; j++) t += sh[j];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// gradient records: 3 NP values per collision, point major
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_fric_gradient_records(CollView C, const FricRec* R,
+                                                              const double* Vel, int nV,
+                                                              FricParams fp, double* rec)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
+    if (i >= C.n[KIND]) return;
+    double* o = rec + (int64_t)3 * NP * i;
+    const FricRec r = R[i];
+    if (r.keep == 0) {
+#pragma unroll
+        for (int j = 0; j < 3 * NP; j++) o[j] = 0.0;
+        return;
+    }
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
+    double u[2], norm, g[4];
+    fric_local<KIND>(r, Vel, nV, v, fp.eps_v, fp.mu_s, fp.mu_k, u, norm);
+    fric_coef<KIND>(r.beta, g);
+    const double f1x = smooth_mu_f1_over_x(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+    const double g0 = f1x * u[0] * r.s, g1 = f1x * u[1] * r.s;
+#pragma unroll
+    for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[3 * j + k] = (g[j] * r.P[k]) * g0 + (g[j] * r.P[3 + k]) * g1;
+}
+
+// 2 x 2 symmetric PSD projection (eigen decomposition in closed form)
+__device__ void psd2(double& a, double& b, double& d, int method)
+{
+    if (method == 0) return;
+    const double tr = 0.5 * (a + d), df = 0.5 * (a - d);
+    const double rad = sqrt(df * df + b * b);
+    const double l0 = tr - rad, l1 = tr + rad;
+    if (l0 >= 0) return; // already PSD
+    auto fix = [&](double l) { return method == 1 ? (l > 0 ? l : 0.0) : fabs(l); };
+    const double m0 = fix(l0), m1 = fix(l1);
+    // eigenvector of l1: (b, l1 - a) or (l1 - d, b), the longer of the two
+    double x, y;
+    if (fabs(l1 - a) >= fabs(l1 - d)) {
+        x = b;
+        y = l1 - a;
+    } else {
+        x = l1 - d;
+        y = b;
+    }
+    const double n = sqrt(x * x + y * y);
+    if (n == 0) { // a multiple of the identity
+        a = d = m1;
+        b = 0;
+        return;
+    }
+    x /= n;
+    y /= n;
+    // A = m1 e1 e1^T + m0 e0 e0^T with e0 = (-y, x)
+    a = m1 * x * x + m0 * y * y;
+    b = (m1 - m0) * x * y;
+    d = m1 * y * y + m0 * x * x;
+}
+
+// Hessian records in the 4-point Q basis (REC_LEN packed values)
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_fric_hessian_records(CollView C, const FricRec* R,
+                                                             const double* Vel, int nV,
+                                                             FricParams fp, double* rec)
+{
+    const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
+    if (i >= C.n[KIND]) return;
+    double* o = rec + (int64_t)REC_LEN * i;
+    const FricRec r = R[i];
+    double m00 = 0, m01 = 0, m11 = 0;
+    double sq[3] = { 0, 0, 0 };
+    if (r.keep != 0) {
+        int v[4];
+        stencil_ids<KIND>(C, i, v);
+        double u[2], norm, g[4];
+        fric_local<KIND>(r, Vel, nV, v, fp.eps_v, fp.mu_s, fp.mu_k, u, norm);
+        fric_coef<KIND>(r.beta, g);
+        const double f1x = smooth_mu_f1_over_x(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+        const double scale = r.s;
+        if (norm > fp.eps_v) { // sliding: f2 = 0, rank one along u_perp
+            if (!(fp.psd != 0 && scale <= 0)) {
+                const double cc = scale * f1x / (norm * norm);
+                m00 = cc * u[1] * u[1];
+                m01 = -cc * u[1] * u[0];
+                m11 = cc * u[0] * u[0];
+            }
+        } else if (norm == 0) {
+            if (!(fp.psd != 0 && scale <= 0)) m00 = m11 = scale * f1x;
+        } else {
+            const double f2 = smooth_mu_f2_x_minus_f1_over_x3(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+            m00 = (f2 * u[0] * u[0] + f1x) * scale;
+            m01 = (f2 * u[0] * u[1]) * scale;
+            m11 = (f2 * u[1] * u[1] + f1x) * scale;
+            psd2(m00, m01, m11, fp.psd);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double t = 0;
+#pragma unroll
+            for (int x = 0; x < 4; x++) t += QBasis<4>::q(x, k) * g[x];
+            sq[k] = t;
+        }
+    }
+    // A = P M P^T
+    double A[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double pa0 = r.P[a], pa1 = r.P[3 + a], pb0 = r.P[b], pb1 = r.P[3 + b];
+            A[a][b] = pa0 * (m00 * pb0 + m01 * pb1) + pa1 * (m01 * pb0 + m11 * pb1);
+        }
+#pragma unroll
+    for (int I = 0; I < 9; I++)
+#pragma unroll
+        for (int J = 0; J <= I; J++) o[sp(I, J)] = sq[I / 3] * sq[J / 3] * A[I % 3][J % 3];
+}
+
+} // namespace
+
+static void require_friction(Ctx& c)
+{
+    require_collisions(c);
+    if (!c.fric_valid) throw Error(IPCG_ERR_STATE, "no friction set (build it after the collisions)");
+    if (!c.has_vel) throw Error(IPCG_ERR_STATE, "no velocities uploaded");
+}
+
+// tangential_collisions.cpp:61-186 over the current collision set at the
+// positions in slot 0
+void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k)
+{
+    require_collisions(c);
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    c.fric.reserve(sizeof(FricRec) * (size_t)std::max<int64_t>(G.goff[4], 1));
+    FricRec* R = c.fric.as<FricRec>();
+    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const double* X = c.X[0].as<double>();
+    c.stage_begin("friction_build");
+    if (C.n[0]) hipLaunchKernelGGL(k_fric_build<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[0]);
+    if (C.n[1]) hipLaunchKernelGGL(k_fric_build<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[1]);
+    if (C.n[2]) hipLaunchKernelGGL(k_fric_build<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[2]);
+    if (C.n[3]) hipLaunchKernelGGL(k_fric_build<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[3]);
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    c.fric_mu_s = mu_s;
+    c.fric_mu_k = mu_k;
+    c.fric_valid = true;
+}
+
+// per collision (global order): keep flag, coordinates (2), basis (6,
+// column major), normal force magnitude; any pointer may be null
+void fric_download(Ctx& c, int32_t* keep, double* beta, double* basis, double* normal_force)
+{
+    require_collisions(c);
+    if (!c.fric_valid) throw Error(IPCG_ERR_STATE, "no friction set");
+    int64_t n = 0;
+    for (int k = 0; k < 4; k++) n += c.ncoll[k];
+    std::vector<FricRec> h((size_t)n);
+    if (n) IPCG_HIP(hipMemcpyAsync(h.data(), c.fric.p, sizeof(FricRec) * n, hipMemcpyDeviceToHost, c.stream));
+    IPCG_HIP(hipStreamSynchronize(c.stream));
+    // the weight is the collision's; the normal force is s / weight only
+    // when the weight is nonzero, so download the weights to divide
+    std::vector<double> w((size_t)n);
+    int64_t o = 0;
+    for (int k = 0; k < 4; k++) {
+        if (c.ncoll[k])
+            IPCG_HIP(hipMemcpyAsync(w.data() + o, c.c_w[k].p, sizeof(double) * c.ncoll[k], hipMemcpyDeviceToHost, c.stream));
+        o += c.ncoll[k];
+    }
+    IPCG_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t i = 0; i < n; i++) {
+        if (keep) keep[i] = h[i].keep != 0;
+        if (beta) {
+            beta[2 * i] = h[i].beta[0];
+            beta[2 * i + 1] = h[i].beta[1];
+        }
+        if (basis)
+            for (int j = 0; j < 6; j++) basis[6 * i + j] = h[i].P[j];
+        if (normal_force) normal_force[i] = h[i].keep != 0 && w[i] != 0 ? h[i].s / w[i] : 0.0;
+    }
+}
+
+double fric_energy(Ctx& c, double eps_v)
+{
+    require_friction(c);
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    const FricRec* R = c.fric.as<FricRec>();
+    const FricParams fp { eps_v, c.fric_mu_s, c.fric_mu_k, 0 };
+    int64_t nb[4], tot = 0;
+    for (int k = 0; k < 4; k++) {
+        nb[k] = (C.n[k] + PB - 1) / PB;
+        tot += nb[k];
+    }
+    c.red_partial.reserve(sizeof(double) * (size_t)std::max<int64_t>(tot, 1));
+    c.red_out.reserve(sizeof(double) * 2);
+    double* part = c.red_partial.as<double>();
+    const double* Vel = c.vel.as<double>();
+    c.stage_begin("friction_energy");
+    int64_t o = 0;
+    if (nb[0]) hipLaunchKernelGGL(k_fric_energy<K_VV>, dim3(nb[0]), dim3(PB), 0, s, C, R + G.goff[0], Vel, c.nV, fp, part + o);
+    o += nb[0];
+    if (nb[1]) hipLaunchKernelGGL(k_fric_energy<K_EV>, dim3(nb[1]), dim3(PB), 0, s, C, R + G.goff[1], Vel, c.nV, fp, part + o);
+    o += nb[1];
+    if (nb[2]) hipLaunchKernelGGL(k_fric_energy<K_EE>, dim3(nb[2]), dim3(PB), 0, s, C, R + G.goff[2], Vel, c.nV, fp, part + o);
+    o += nb[2];
+    if (nb[3]) hipLaunchKernelGGL(k_fric_energy<K_FV>, dim3(nb[3]), dim3(PB), 0, s, C, R + G.goff[3], Vel, c.nV, fp, part + o);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, part, tot, c.red_out.as<double>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    double* h = static_cast<double*>(c.pinned);
+    IPCG_HIP(hipMemcpyAsync(h, c.red_out.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    return h[0];
+}
+
+void fric_gradient(Ctx& c, double eps_v)
+{
+    require_friction(c);
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    require_incidences(c, C, G, M);
+    const FricRec* R = c.fric.as<FricRec>();
+    const FricParams fp { eps_v, c.fric_mu_s, c.fric_mu_k, 0 };
+    c.grad.reserve(sizeof(double) * 3 * (size_t)std::max(c.nV, 1));
+    const int npk[4] = { 2, 3, 4, 4 };
+    GradRecs GR;
+    int64_t nrec = 0;
+    for (int k = 0; k < 4; k++) nrec += 3 * (int64_t)npk[k] * C.n[k];
+    c.hrec.reserve(sizeof(double) * (size_t)std::max<int64_t>(nrec, 1));
+    int64_t ro = 0;
+    for (int k = 0; k < 4; k++) {
+        GR.rec[k] = c.hrec.as<double>() + ro;
+        ro += 3 * (int64_t)npk[k] * C.n[k];
+    }
+    const double* Vel = c.vel.as<double>();
+    c.stage_begin("friction_gradient");
+    if (C.n[0]) hipLaunchKernelGGL(k_fric_gradient_records<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[0], Vel, c.nV, fp, const_cast<double*>(GR.rec[0]));
+    if (C.n[1]) hipLaunchKernelGGL(k_fric_gradient_records<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[1], Vel, c.nV, fp, const_cast<double*>(GR.rec[1]));
+    if (C.n[2]) hipLaunchKernelGGL(k_fric_gradient_records<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[2], Vel, c.nV, fp, const_cast<double*>(GR.rec[2]));
+    if (C.n[3]) hipLaunchKernelGGL(k_fric_gradient_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[3], Vel, c.nV, fp, const_cast<double*>(GR.rec[3]));
+    const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
+    hipLaunchKernelGGL(k_gradient_rows, dim3(gw), dim3(256), 0, s, G, GR, c.inc_ptr.as<uint32_t>(),
+                       c.inc_vals.as<uint32_t>(), c.nV, c.grad.as<double>());
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+}
+
+void fric_hessian(Ctx& c, double eps_v, int psd)
+{
+    require_friction(c);
+    if (psd < 0 || psd > 2) throw Error(IPCG_ERR_ARG, "invalid PSD projection method");
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    const CollRanges G = hess_prepare(c, C);
+    const FricRec* R = c.fric.as<FricRec>();
+    const FricParams fp { eps_v, c.fric_mu_s, c.fric_mu_k, psd };
+    double* rec = c.hrec.as<double>();
+    const double* Vel = c.vel.as<double>();
+    c.stage_begin("friction_hessian_records");
+    if (C.n[0]) hipLaunchKernelGGL(k_fric_hessian_records<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[0], Vel, c.nV, fp, rec + REC_LEN * G.goff[0]);
+    if (C.n[1]) hipLaunchKernelGGL(k_fric_hessian_records<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[1], Vel, c.nV, fp, rec + REC_LEN * G.goff[1]);
+    if (C.n[2]) hipLaunchKernelGGL(k_fric_hessian_records<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[2], Vel, c.nV, fp, rec + REC_LEN * G.goff[2]);
+    if (C.n[3]) hipLaunchKernelGGL(k_fric_hessian_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[3], Vel, c.nV, fp, rec + REC_LEN * G.goff[3]);
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    hess_assemble(c);
 }
 
 } // namespace ipcg

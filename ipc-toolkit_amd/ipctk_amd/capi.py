@@ -38,6 +38,8 @@ EXPORTS = [
     "ipcg_candidates_per_vertex_safe_distances",
     "ipcg_candidates_noncandidate_conservative_stepsize", "ipcg_candidates_cfl_stepsize",
     "ipcg_collisions_set_area_weighting", "ipcg_collisions_set_type",
+    "ipcg_friction_build", "ipcg_friction_download", "ipcg_velocities_upload",
+    "ipcg_friction_energy", "ipcg_friction_gradient", "ipcg_friction_hessian",
 ]
 
 _lib = None
@@ -104,6 +106,12 @@ def load(path=LIB_PATH):
     L.ipcg_has_intersections.argtypes = [vp, C.POINTER(C.c_int)]
     L.ipcg_collisions_set_area_weighting.argtypes = [vp, C.c_int]
     L.ipcg_collisions_set_type.argtypes = [vp, C.c_int]
+    L.ipcg_friction_build.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.ipcg_friction_download.argtypes = [vp, _I, _D, _D, _D]
+    L.ipcg_velocities_upload.argtypes = [vp, _D, C.c_int]
+    L.ipcg_friction_energy.argtypes = [vp, C.c_double, _D]
+    L.ipcg_friction_gradient.argtypes = [vp, C.c_double, _D]
+    L.ipcg_friction_hessian.argtypes = [vp, C.c_double, C.c_int, _L]
     L.ipcg_barrier_gauss_newton_diagonal.argtypes = [vp, C.c_double, C.c_double, _D]
     L.ipcg_barrier_gauss_newton_quadratic_form.argtypes = [vp, C.c_double, C.c_double, _D, _D]
     L.ipcg_candidates_per_vertex_safe_distances.argtypes = [vp, C.c_double, C.c_double, _D]
@@ -327,6 +335,53 @@ class Context:
         self._check(self._L.ipcg_barrier_hessian(self._p, dhat, kappa, PSD[psd], C.byref(nnz)))
         if not download:
             return nnz.value
+        import scipy.sparse as sp
+        n = 3 * self.nV
+        rowptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        self._check(self._L.ipcg_hessian_download(self._p, rowptr.ctypes.data_as(_L),
+                                                  col.ctypes.data_as(_I), val.ctypes.data_as(_D)))
+        return sp.csr_matrix((val, col, rowptr), shape=(n, n))
+
+    # ---- friction ----------------------------------------------------------
+    def friction_build(self, dhat, kappa, mu_s, mu_k=None):
+        """TangentialCollisions::build over the current collision set at the
+        slot-0 positions, global coefficients mu_s, mu_k (default mu_s)"""
+        self._check(self._L.ipcg_friction_build(self._p, dhat, kappa, mu_s,
+                                                mu_s if mu_k is None else mu_k))
+
+    def friction_records(self):
+        """keep (bool), closest-point coordinates (n, 2), tangent bases
+        (n, 3, 2), normal force magnitudes (n,), in collision order"""
+        n = int(self.collision_counts().sum())
+        keep = np.zeros(n, np.int32)
+        beta = np.zeros((n, 2))
+        basis = np.zeros((n, 6))
+        force = np.zeros(n)
+        self._check(self._L.ipcg_friction_download(
+            self._p, keep.ctypes.data_as(_I), beta.ctypes.data_as(_D),
+            basis.ctypes.data_as(_D), force.ctypes.data_as(_D)))
+        return keep.astype(bool), beta, basis.reshape(n, 2, 3).transpose(0, 2, 1), force
+
+    def upload_velocities(self, V):
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        V, pV = _d(V)
+        self._check(self._L.ipcg_velocities_upload(self._p, pV, LAYOUT_AOS))
+
+    def friction_energy(self, eps_v):
+        e = C.c_double()
+        self._check(self._L.ipcg_friction_energy(self._p, eps_v, C.byref(e)))
+        return e.value
+
+    def friction_gradient(self, eps_v):
+        g = np.zeros(3 * self.nV)
+        self._check(self._L.ipcg_friction_gradient(self._p, eps_v, g.ctypes.data_as(_D)))
+        return g
+
+    def friction_hessian(self, eps_v, psd="clamp"):
+        nnz = C.c_int64()
+        self._check(self._L.ipcg_friction_hessian(self._p, eps_v, PSD[psd], C.byref(nnz)))
         import scipy.sparse as sp
         n = 3 * self.nV
         rowptr = np.zeros(n + 1, np.int64)
