@@ -603,6 +603,109 @@ private:
     }
 };
 
+// collisions/tangential/tangential_collisions.hpp: friction records over a
+// normal collision set, with one static and one kinetic coefficient for
+// every vertex (tangential_collisions.cpp:61-186; equal per-vertex
+// coefficients blend to themselves)
+class TangentialCollisions {
+public:
+    void build(const CollisionMesh& mesh, const MatrixXd& vertices,
+               const NormalCollisions& collisions, const BarrierPotential& normal_potential,
+               double mu_s, double mu_k)
+    {
+        if (collisions.mesh() != &mesh)
+            throw std::runtime_error("collisions were not built for this mesh");
+        if (!collisions.resident())
+            throw std::runtime_error("a later collision build replaced this set on the device");
+        mesh.set_positions(0, vertices);
+        ipcg_ctx* c = mesh.context();
+        detail::check(c, ipcg_friction_build(c, normal_potential.dhat(),
+                                             normal_potential.stiffness(), mu_s, mu_k));
+        const size_t n = collisions.size();
+        std::vector<int32_t> keep(n);
+        detail::check(c, ipcg_friction_download(c, keep.data(), nullptr, nullptr, nullptr));
+        m_size = 0;
+        for (int32_t k : keep) m_size += k != 0;
+        built_for = &mesh;
+        m_generation = mesh.collisions_generation();
+    }
+    void build(const CollisionMesh& mesh, const MatrixXd& vertices,
+               const NormalCollisions& collisions, const BarrierPotential& normal_potential,
+               double mu)
+    {
+        build(mesh, vertices, collisions, normal_potential, mu, mu);
+    }
+    size_t size() const { return m_size; }
+    bool empty() const { return m_size == 0; }
+    // the device holds this set (no later collision build on the mesh)
+    bool resident(const CollisionMesh& mesh) const
+    {
+        return built_for == &mesh && mesh.collisions_generation() == m_generation;
+    }
+
+private:
+    const CollisionMesh* built_for = nullptr;
+    uint64_t m_generation = 0;
+    size_t m_size = 0;
+};
+
+// potentials/friction_potential.hpp (tangential_potential.cpp:156-331),
+// isotropic, at velocities #V x 3
+class FrictionPotential {
+public:
+    explicit FrictionPotential(double eps_v) : m_eps_v(eps_v)
+    {
+        if (!(eps_v > 0)) throw std::invalid_argument("eps_v must be positive");
+    }
+    double eps_v() const { return m_eps_v; }
+
+    double operator()(const TangentialCollisions& collisions, const CollisionMesh& mesh,
+                      const MatrixXd& velocities) const
+    {
+        prepare(collisions, mesh, velocities);
+        double e = 0;
+        detail::check(mesh.context(), ipcg_friction_energy(mesh.context(), m_eps_v, &e));
+        return e;
+    }
+    std::vector<double> gradient(const TangentialCollisions& collisions,
+                                 const CollisionMesh& mesh, const MatrixXd& velocities) const
+    {
+        prepare(collisions, mesh, velocities);
+        std::vector<double> g(3 * (size_t)mesh.num_vertices());
+        detail::check(mesh.context(), ipcg_friction_gradient(mesh.context(), m_eps_v, g.data()));
+        return g;
+    }
+    SparseMatrix hessian(const TangentialCollisions& collisions, const CollisionMesh& mesh,
+                         const MatrixXd& velocities,
+                         PSDProjectionMethod project_hessian_to_psd = PSDProjectionMethod::NONE) const
+    {
+        prepare(collisions, mesh, velocities);
+        ipcg_ctx* c = mesh.context();
+        int64_t nnz = 0;
+        detail::check(c, ipcg_friction_hessian(c, m_eps_v, int(project_hessian_to_psd), &nnz));
+        SparseMatrix H;
+        H.rows = H.cols = 3 * (int64_t)mesh.num_vertices();
+        H.outer.resize(H.rows + 1);
+        H.inner.resize(nnz);
+        H.values.resize(nnz);
+        detail::check(c, ipcg_hessian_download(c, H.outer.data(), H.inner.data(), H.values.data()));
+        return H;
+    }
+
+private:
+    double m_eps_v;
+    static void prepare(const TangentialCollisions& collisions, const CollisionMesh& mesh,
+                        const MatrixXd& velocities)
+    {
+        if (!collisions.resident(mesh))
+            throw std::runtime_error("the friction set is not the one on the device for this mesh");
+        if (velocities.rows() != mesh.num_vertices() || (velocities.rows() && velocities.cols() != 3))
+            throw std::invalid_argument("velocities must be #V x 3");
+        detail::check(mesh.context(), ipcg_velocities_upload(mesh.context(), velocities.data(),
+                                                             IPCG_LAYOUT_AOS));
+    }
+};
+
 // ipc.cpp:45-101
 inline double compute_collision_free_stepsize(const CollisionMesh& mesh,
                                               const MatrixXd& vertices_t0,

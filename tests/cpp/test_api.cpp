@@ -280,6 +280,33 @@ static void improved_max_approx()
     CHECK(C.size() == 6 && C.vv_collisions.size() == 2 && C.ev_collisions.size() == 4);
 }
 
+// a codim point pair along z: the friction energy at velocity (vx, vy, *)
+// of the first point is mu N f0(|(vx, vy)|), N = -kappa b'(d^2) 2 d
+static void friction_point_pair()
+{
+    using namespace ipc;
+    const double d = 0.004, dhat = 0.01, kappa = 2.0, mu = 0.4, eps_v = 1e-3;
+    const MatrixXd V { { 0, 0, 0 }, { 0, 0, d } };
+    CollisionMesh mesh(V);
+    NormalCollisions C;
+    C.build(mesh, V, dhat);
+    CHECK(C.vv_collisions.size() == 1);
+    BarrierPotential B(dhat, kappa);
+    TangentialCollisions T;
+    T.build(mesh, V, C, B, mu);
+    CHECK(T.size() == 1);
+    const double x = d * d, t = dhat * dhat;
+    const double db = -2 * (x - t) * std::log(x / t) - (x - t) * (x - t) / x;
+    const double N = -kappa * db * 2 * d;
+    const MatrixXd Vel { { 3e-3, -4e-3, 0.5 }, { 0, 0, -0.1 } };
+    const double e = FrictionPotential(eps_v)(T, mesh, Vel);
+    CHECK(std::abs(e - mu * N * 5e-3) <= 1e-12 * mu * N * 5e-3);
+    const std::vector<double> g = FrictionPotential(eps_v).gradient(T, mesh, Vel);
+    // sliding: force mu N along u / |u| on the first point, opposite on the second
+    CHECK(std::abs(g[0] - mu * N * 0.6) <= 1e-12 * mu * N && std::abs(g[1] + mu * N * 0.8) <= 1e-12 * mu * N);
+    CHECK(std::abs(g[3] + g[0]) <= 1e-15 * mu * N && g[2] == 0.0);
+}
+
 int main()
 {
     try {
@@ -292,6 +319,7 @@ int main()
         candidate_helpers();
         area_weighting();
         improved_max_approx();
+        friction_point_pair();
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());
         failures++;
