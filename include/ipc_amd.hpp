@@ -8,6 +8,10 @@
 //   ipc::NormalCollisions       collisions/normal/normal_collisions.hpp:45-63
 //   ipc::BarrierPotential       potentials/barrier_potential.hpp,
 //                               potentials/potential.hpp:34-60
+//   ipc::TangentialCollisions   collisions/tangential/tangential_collisions.hpp
+//                               (global friction coefficients)
+//   ipc::FrictionPotential      potentials/friction_potential.hpp (isotropic)
+//   ipc::has_intersections      ipc.hpp (ipc.cpp:105-166)
 //   ipc::compute_collision_free_stepsize, ipc::is_step_collision_free
 //                               ipc.hpp (ipc.cpp:20-101)
 //   ipc::TightInclusionCCD      ccd/tight_inclusion_ccd.hpp:11-28 (parameters)
