@@ -9,7 +9,7 @@
 //   ipc::BarrierPotential       potentials/barrier_potential.hpp,
 //                               potentials/potential.hpp:34-60
 //   ipc::TangentialCollisions   collisions/tangential/tangential_collisions.hpp
-//                               (global friction coefficients)
+//                               (scalar or per-vertex friction coefficients)
 //   ipc::FrictionPotential      potentials/friction_potential.hpp (isotropic)
 //   ipc::has_intersections      ipc.hpp (ipc.cpp:105-166)
 //   ipc::compute_collision_free_stepsize, ipc::is_step_collision_free
@@ -608,36 +608,38 @@ private:
 };
 
 // collisions/tangential/tangential_collisions.hpp: friction records over a
-// normal collision set, with one static and one kinetic coefficient for
-// every vertex (tangential_collisions.cpp:61-186; equal per-vertex
-// coefficients blend to themselves)
+// normal collision set (tangential_collisions.cpp:61-186), with scalar or
+// per-vertex static and kinetic coefficients
 class TangentialCollisions {
 public:
     void build(const CollisionMesh& mesh, const MatrixXd& vertices,
                const NormalCollisions& collisions, const BarrierPotential& normal_potential,
                double mu_s, double mu_k)
     {
-        if (collisions.mesh() != &mesh)
-            throw std::runtime_error("collisions were not built for this mesh");
-        if (!collisions.resident())
-            throw std::runtime_error("a later collision build replaced this set on the device");
-        mesh.set_positions(0, vertices);
-        ipcg_ctx* c = mesh.context();
-        detail::check(c, ipcg_friction_build(c, normal_potential.dhat(),
-                                             normal_potential.stiffness(), mu_s, mu_k));
-        const size_t n = collisions.size();
-        std::vector<int32_t> keep(n);
-        detail::check(c, ipcg_friction_download(c, keep.data(), nullptr, nullptr, nullptr));
-        m_size = 0;
-        for (int32_t k : keep) m_size += k != 0;
-        built_for = &mesh;
-        m_generation = mesh.collisions_generation();
+        begin(mesh, vertices, collisions);
+        detail::check(mesh.context(), ipcg_friction_build(mesh.context(), normal_potential.dhat(),
+                                                          normal_potential.stiffness(), mu_s, mu_k));
+        finish(mesh, collisions);
     }
     void build(const CollisionMesh& mesh, const MatrixXd& vertices,
                const NormalCollisions& collisions, const BarrierPotential& normal_potential,
                double mu)
     {
         build(mesh, vertices, collisions, normal_potential, mu, mu);
+    }
+    // per-vertex coefficients, averaged per collision (default_blend_mu)
+    void build(const CollisionMesh& mesh, const MatrixXd& vertices,
+               const NormalCollisions& collisions, const BarrierPotential& normal_potential,
+               const std::vector<double>& mu_s, const std::vector<double>& mu_k)
+    {
+        if (mu_s.size() != (size_t)mesh.num_vertices() || mu_k.size() != mu_s.size())
+            throw std::invalid_argument("one friction coefficient per vertex");
+        begin(mesh, vertices, collisions);
+        detail::check(mesh.context(),
+                      ipcg_friction_build_per_vertex(mesh.context(), normal_potential.dhat(),
+                                                     normal_potential.stiffness(), mu_s.data(),
+                                                     mu_k.data()));
+        finish(mesh, collisions);
     }
     size_t size() const { return m_size; }
     bool empty() const { return m_size == 0; }
@@ -651,6 +653,25 @@ private:
     const CollisionMesh* built_for = nullptr;
     uint64_t m_generation = 0;
     size_t m_size = 0;
+    static void begin(const CollisionMesh& mesh, const MatrixXd& vertices,
+                      const NormalCollisions& collisions)
+    {
+        if (collisions.mesh() != &mesh)
+            throw std::runtime_error("collisions were not built for this mesh");
+        if (!collisions.resident())
+            throw std::runtime_error("a later collision build replaced this set on the device");
+        mesh.set_positions(0, vertices);
+    }
+    void finish(const CollisionMesh& mesh, const NormalCollisions& collisions)
+    {
+        std::vector<int32_t> keep(collisions.size());
+        detail::check(mesh.context(),
+                      ipcg_friction_download(mesh.context(), keep.data(), nullptr, nullptr, nullptr));
+        m_size = 0;
+        for (int32_t k : keep) m_size += k != 0;
+        built_for = &mesh;
+        m_generation = mesh.collisions_generation();
+    }
 };
 
 // potentials/friction_potential.hpp (tangential_potential.cpp:156-331),

@@ -216,8 +216,9 @@ int ipcg_barrier_gauss_newton_quadratic_form(ipcg_ctx* ctx, double dhat, double 
 
 /* ---- friction (tangential_collisions.cpp, tangential_potential.cpp) ------
  * TangentialCollisions::build(mesh, V, collisions, BarrierPotential(dhat,
- * kappa), mu_s, mu_k) with global coefficients (tangential_collisions.cpp:
- * 61-186; every per-vertex blend of equal coefficients returns them): over
+ * kappa), mu_s, mu_k) with the same coefficients at every vertex
+ * (tangential_collisions.cpp:61-186; every blend of equal coefficients
+ * returns them): over
  * the current collision set at the slot-0 positions (the lagged
  * positions), one record per collision: closest-point coordinates
  * (closest_point.cpp), tangent basis (tangent_basis.cpp), normal force
@@ -226,6 +227,12 @@ int ipcg_barrier_gauss_newton_quadratic_form(ipcg_ctx* ctx, double dhat, double 
  * below eps_x) are not kept (tangential_collisions.cpp:118-121).  A new
  * collision build or mesh upload discards the friction set. */
 int ipcg_friction_build(ipcg_ctx* ctx, double dhat, double kappa, double mu_s, double mu_k);
+/* the same with per-vertex coefficients (host arrays of nV doubles each):
+ * each collision takes the average (default_blend_mu,
+ * tangential_collisions.hpp:143-149) of its two sides' coefficients at
+ * their closest points (tangential_collisions.cpp:86-166) */
+int ipcg_friction_build_per_vertex(ipcg_ctx* ctx, double dhat, double kappa, const double* mu_s,
+                                   const double* mu_k);
 /* per collision in download order (vv, ev, ee, fv): keep flag,
  * closest-point coordinates (2), tangent basis (3 x 2, column major),
  * normal force magnitude (0 when not kept); any pointer may be NULL */

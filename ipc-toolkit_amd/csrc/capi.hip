@@ -175,7 +175,7 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
                        &c.adj_ev, &c.adj_vf_off, &c.adj_bnd, &c.np_rw, &c.np_flag, &c.np_pos, &c.np_rep,
                        &c.im_slot, &c.im_slot2, &c.im_vals, &c.im_vals2, &c.im_ukeys,
                        &c.im_start, &c.im_cnt, &c.im_off, &c.im_vv, &c.im_vvw, &c.im_ev,
-                       &c.im_evw, &c.im_ee, &c.fric, &c.vel,
+                       &c.im_evw, &c.im_ee, &c.fric, &c.vel, &c.fric_mu,
                        &c.np_cw };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
@@ -657,6 +657,19 @@ int ipcg_friction_build(ipcg_ctx* ctx, double dhat, double kappa, double mu_s, d
         if (!(dhat > 0) || !(kappa > 0) || !(mu_s >= 0) || !(mu_k >= 0))
             throw Error(IPCG_ERR_ARG, "dhat and kappa must be > 0, mu_s and mu_k >= 0");
         fric_build(c, dhat, kappa, mu_s, mu_k);
+    });
+}
+
+int ipcg_friction_build_per_vertex(ipcg_ctx* ctx, double dhat, double kappa, const double* mu_s,
+                                   const double* mu_k)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!(dhat > 0) || !(kappa > 0) || (c.nV > 0 && (!mu_s || !mu_k)))
+            throw Error(IPCG_ERR_ARG, "dhat and kappa must be > 0 and the coefficient arrays set");
+        for (int32_t i = 0; i < c.nV; i++)
+            if (!(mu_s[i] >= 0) || !(mu_k[i] >= 0))
+                throw Error(IPCG_ERR_ARG, "friction coefficients must be >= 0");
+        fric_build(c, dhat, kappa, 0.0, 0.0, mu_s, mu_k);
     });
 }
 

@@ -112,7 +112,7 @@ struct Ctx {
     DevBuf gn_out, gn_p;              // Gauss-Newton diagonal (3 nV) and direction (3 nV)
     DevBuf fric;                      // friction record per collision (global order)
     bool fric_valid = false;
-    double fric_mu_s = 0, fric_mu_k = 0;
+    DevBuf fric_mu;                   // per-vertex static, then kinetic coefficients
     DevBuf vel;                       // velocities (3 nV, vertex major)
     bool has_vel = false;
     bool inc_valid = false;
@@ -150,7 +150,8 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
                     double rescale, bool early_exit, double* const* toi_out);
 void bp_edge_face(Ctx& c, double inflation);
-void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k);
+void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k,
+                const double* mu_sv = nullptr, const double* mu_kv = nullptr);
 void fric_download(Ctx& c, int32_t* keep, double* beta, double* basis, double* normal_force);
 double fric_energy(Ctx& c, double eps_v);
 void fric_gradient(Ctx& c, double eps_v);

@@ -1273,16 +1273,36 @@ double pot_gn_quadratic_form(Ctx& c, double dhat, double kappa, const double* p_
 namespace {
 
 struct FricRec {
-    double P[6];    // tangent basis, column major 3 x 2
-    double beta[2]; // closest-point coordinates
-    double s;       // weight * normal force magnitude; 0: not kept
-    double keep;    // 1 kept, 0 skipped (near-parallel edge-edge)
+    double P[6];      // tangent basis, column major 3 x 2
+    double beta[2];   // closest-point coordinates
+    double s;         // weight * normal force magnitude; 0: not kept
+    double keep;      // 1 kept, 0 skipped (near-parallel edge-edge)
+    double mu_s, mu_k; // the collision's static / kinetic coefficients
 };
 
 struct FricParams {
-    double eps_v, mu_s, mu_k;
+    double eps_v;
     int psd;
 };
+
+// tangential_collisions.hpp:143-149 default_blend_mu
+__device__ __forceinline__ double blend_mu(double a, double b)
+{
+    return (a + b) / 2;
+}
+
+// tangential_collisions.cpp:86-166: each side's per-vertex coefficient at
+// its closest point, blended
+template <int KIND>
+__device__ __forceinline__ double collision_mu(const int* v, const double* beta, const double* m)
+{
+    if constexpr (KIND == K_VV) return blend_mu(m[v[0]], m[v[1]]);
+    if constexpr (KIND == K_EV) return blend_mu((m[v[2]] - m[v[1]]) * beta[0] + m[v[1]], m[v[0]]);
+    if constexpr (KIND == K_EE)
+        return blend_mu((m[v[1]] - m[v[0]]) * beta[0] + m[v[0]],
+                        (m[v[3]] - m[v[2]]) * beta[1] + m[v[2]]);
+    return blend_mu(m[v[1]] + beta[0] * (m[v[2]] - m[v[1]]) + beta[1] * (m[v[3]] - m[v[1]]), m[v[0]]);
+}
 
 // smooth_friction_mollifier.cpp:8-53
 __device__ __forceinline__ double sf_f0(double y, double ev)
@@ -1419,9 +1439,12 @@ __device__ __forceinline__ d3 fric_relative_velocity(const d3* v, const double* 
     return sub(v[0], t);
 }
 
+// msv / mkv: per-vertex coefficients, or null for the scalars ms / mk
 template <int KIND>
 __global__ __launch_bounds__(PB) void k_fric_build(CollView C, const double* X, int nV,
-                                                   BarrierParams bp, FricRec* out)
+                                                   BarrierParams bp, const double* msv,
+                                                   const double* mkv, double ms, double mk,
+                                                   FricRec* out)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
     const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
@@ -1460,6 +1483,8 @@ __global__ __launch_bounds__(PB) void k_fric_build(CollView C, const double* X, 
     const double N = -bar_f1(d2, bp) * 2 * sqrt(d2);
     r.s = C.w[KIND][i] * N;
     r.keep = 1.0;
+    r.mu_s = msv ? collision_mu<KIND>(v, r.beta, msv) : ms;
+    r.mu_k = mkv ? collision_mu<KIND>(v, r.beta, mkv) : mk;
     out[i] = r;
 }
 
@@ -1488,8 +1513,8 @@ __global__ __launch_bounds__(PB) void k_fric_energy(CollView C, const FricRec* R
         int v[4];
         stencil_ids<KIND>(C, i, v);
         double u[2], norm;
-        fric_local<KIND>(R[i], Vel, nV, v, fp.eps_v, fp.mu_s, fp.mu_k, u, norm);
-        e = R[i].s * smooth_mu_f0(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+        fric_local<KIND>(R[i], Vel, nV, v, fp.eps_v, R[i].mu_s, R[i].mu_k, u, norm);
+        e = R[i].s * smooth_mu_f0(norm, R[i].mu_s, R[i].mu_k, fp.eps_v);
     }
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
     __shared__ double sh[PB / WAVE];
@@ -1521,9 +1546,9 @@ __global__ __launch_bounds__(PB) void k_fric_gradient_records(CollView C, const 
     int v[4];
     stencil_ids<KIND>(C, i, v);
     double u[2], norm, g[4];
-    fric_local<KIND>(r, Vel, nV, v, fp.eps_v, fp.mu_s, fp.mu_k, u, norm);
+    fric_local<KIND>(r, Vel, nV, v, fp.eps_v, r.mu_s, r.mu_k, u, norm);
     fric_coef<KIND>(r.beta, g);
-    const double f1x = smooth_mu_f1_over_x(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+    const double f1x = smooth_mu_f1_over_x(norm, r.mu_s, r.mu_k, fp.eps_v);
     const double g0 = f1x * u[0] * r.s, g1 = f1x * u[1] * r.s;
 #pragma unroll
     for (int j = 0; j < NP; j++)
@@ -1580,9 +1605,9 @@ __global__ __launch_bounds__(PB) void k_fric_hessian_records(CollView C, const F
         int v[4];
         stencil_ids<KIND>(C, i, v);
         double u[2], norm, g[4];
-        fric_local<KIND>(r, Vel, nV, v, fp.eps_v, fp.mu_s, fp.mu_k, u, norm);
+        fric_local<KIND>(r, Vel, nV, v, fp.eps_v, r.mu_s, r.mu_k, u, norm);
         fric_coef<KIND>(r.beta, g);
-        const double f1x = smooth_mu_f1_over_x(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+        const double f1x = smooth_mu_f1_over_x(norm, r.mu_s, r.mu_k, fp.eps_v);
         const double scale = r.s;
         if (norm > fp.eps_v) { // sliding: f2 = 0, rank one along u_perp
             if (!(fp.psd != 0 && scale <= 0)) {
@@ -1594,7 +1619,7 @@ __global__ __launch_bounds__(PB) void k_fric_hessian_records(CollView C, const F
         } else if (norm == 0) {
             if (!(fp.psd != 0 && scale <= 0)) m00 = m11 = scale * f1x;
         } else {
-            const double f2 = smooth_mu_f2_x_minus_f1_over_x3(norm, fp.mu_s, fp.mu_k, fp.eps_v);
+            const double f2 = smooth_mu_f2_x_minus_f1_over_x3(norm, r.mu_s, r.mu_k, fp.eps_v);
             m00 = (f2 * u[0] * u[0] + f1x) * scale;
             m01 = (f2 * u[0] * u[1]) * scale;
             m11 = (f2 * u[1] * u[1] + f1x) * scale;
@@ -1633,8 +1658,10 @@ static void require_friction(Ctx& c)
 }
 
 // tangential_collisions.cpp:61-186 over the current collision set at the
-// positions in slot 0
-void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k)
+// positions in slot 0; mu_sv / mu_kv: per-vertex coefficients on the host
+// (nV each) or null for the scalars
+void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k, const double* mu_sv,
+                const double* mu_kv)
 {
     require_collisions(c);
     hipStream_t s = c.stream;
@@ -1645,15 +1672,23 @@ void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k)
     FricRec* R = c.fric.as<FricRec>();
     BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
     const double* X = c.X[0].as<double>();
+    const double *msv = nullptr, *mkv = nullptr;
+    if (mu_sv && mu_kv && c.nV) {
+        c.fric_mu.reserve(sizeof(double) * 2 * (size_t)c.nV);
+        IPCG_HIP(hipMemcpyAsync(c.fric_mu.p, mu_sv, sizeof(double) * c.nV, hipMemcpyHostToDevice, s));
+        IPCG_HIP(hipMemcpyAsync(c.fric_mu.as<double>() + c.nV, mu_kv, sizeof(double) * c.nV,
+                                hipMemcpyHostToDevice, s));
+        msv = c.fric_mu.as<double>();
+        mkv = msv + c.nV;
+    }
     c.stage_begin("friction_build");
-    if (C.n[0]) hipLaunchKernelGGL(k_fric_build<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[0]);
-    if (C.n[1]) hipLaunchKernelGGL(k_fric_build<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[1]);
-    if (C.n[2]) hipLaunchKernelGGL(k_fric_build<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[2]);
-    if (C.n[3]) hipLaunchKernelGGL(k_fric_build<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, R + G.goff[3]);
+    if (C.n[0]) hipLaunchKernelGGL(k_fric_build<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, msv, mkv, mu_s, mu_k, R + G.goff[0]);
+    if (C.n[1]) hipLaunchKernelGGL(k_fric_build<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, msv, mkv, mu_s, mu_k, R + G.goff[1]);
+    if (C.n[2]) hipLaunchKernelGGL(k_fric_build<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, msv, mkv, mu_s, mu_k, R + G.goff[2]);
+    if (C.n[3]) hipLaunchKernelGGL(k_fric_build<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, msv, mkv, mu_s, mu_k, R + G.goff[3]);
     IPCG_CHECK_LAUNCH();
     c.stage_end();
-    c.fric_mu_s = mu_s;
-    c.fric_mu_k = mu_k;
+    IPCG_HIP(hipStreamSynchronize(s)); // the host coefficient arrays may go away
     c.fric_valid = true;
 }
 
@@ -1698,7 +1733,7 @@ double fric_energy(Ctx& c, double eps_v)
     int64_t M;
     const CollRanges G = ranges(C, M);
     const FricRec* R = c.fric.as<FricRec>();
-    const FricParams fp { eps_v, c.fric_mu_s, c.fric_mu_k, 0 };
+    const FricParams fp { eps_v, 0 };
     int64_t nb[4], tot = 0;
     for (int k = 0; k < 4; k++) {
         nb[k] = (C.n[k] + PB - 1) / PB;
@@ -1735,7 +1770,7 @@ void fric_gradient(Ctx& c, double eps_v)
     const CollRanges G = ranges(C, M);
     require_incidences(c, C, G, M);
     const FricRec* R = c.fric.as<FricRec>();
-    const FricParams fp { eps_v, c.fric_mu_s, c.fric_mu_k, 0 };
+    const FricParams fp { eps_v, 0 };
     c.grad.reserve(sizeof(double) * 3 * (size_t)std::max(c.nV, 1));
     const int npk[4] = { 2, 3, 4, 4 };
     GradRecs GR;
@@ -1768,7 +1803,7 @@ void fric_hessian(Ctx& c, double eps_v, int psd)
     CollView C = view(c);
     const CollRanges G = hess_prepare(c, C);
     const FricRec* R = c.fric.as<FricRec>();
-    const FricParams fp { eps_v, c.fric_mu_s, c.fric_mu_k, psd };
+    const FricParams fp { eps_v, psd };
     double* rec = c.hrec.as<double>();
     const double* Vel = c.vel.as<double>();
     c.stage_begin("friction_hessian_records");

@@ -38,7 +38,7 @@ EXPORTS = [
     "ipcg_candidates_per_vertex_safe_distances",
     "ipcg_candidates_noncandidate_conservative_stepsize", "ipcg_candidates_cfl_stepsize",
     "ipcg_collisions_set_area_weighting", "ipcg_collisions_set_type",
-    "ipcg_friction_build", "ipcg_friction_download", "ipcg_velocities_upload",
+    "ipcg_friction_build", "ipcg_friction_build_per_vertex", "ipcg_friction_download", "ipcg_velocities_upload",
     "ipcg_friction_energy", "ipcg_friction_gradient", "ipcg_friction_hessian",
 ]
 
@@ -108,6 +108,7 @@ def load(path=LIB_PATH):
     L.ipcg_collisions_set_type.argtypes = [vp, C.c_int]
     L.ipcg_friction_build.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
     L.ipcg_friction_download.argtypes = [vp, _I, _D, _D, _D]
+    L.ipcg_friction_build_per_vertex.argtypes = [vp, C.c_double, C.c_double, _D, _D]
     L.ipcg_velocities_upload.argtypes = [vp, _D, C.c_int]
     L.ipcg_friction_energy.argtypes = [vp, C.c_double, _D]
     L.ipcg_friction_gradient.argtypes = [vp, C.c_double, _D]
@@ -347,9 +348,15 @@ class Context:
     # ---- friction ----------------------------------------------------------
     def friction_build(self, dhat, kappa, mu_s, mu_k=None):
         """TangentialCollisions::build over the current collision set at the
-        slot-0 positions, global coefficients mu_s, mu_k (default mu_s)"""
-        self._check(self._L.ipcg_friction_build(self._p, dhat, kappa, mu_s,
-                                                mu_s if mu_k is None else mu_k))
+        slot-0 positions; mu_s, mu_k (default mu_s): scalars or per-vertex
+        arrays"""
+        mu_k = mu_s if mu_k is None else mu_k
+        if np.ndim(mu_s) == 0 and np.ndim(mu_k) == 0:
+            self._check(self._L.ipcg_friction_build(self._p, dhat, kappa, mu_s, mu_k))
+            return
+        ms, pms = _d(np.broadcast_to(np.asarray(mu_s, float), (self.nV,)))
+        mk, pmk = _d(np.broadcast_to(np.asarray(mu_k, float), (self.nV,)))
+        self._check(self._L.ipcg_friction_build_per_vertex(self._p, dhat, kappa, pms, pmk))
 
     def friction_records(self):
         """keep (bool), closest-point coordinates (n, 2), tangent bases

@@ -298,7 +298,7 @@ int ipco_barrier_hessian(const ipco_collisions *c, const double *V, int nV,
 void ipco_csr_free(ipco_csr *m);
 /* ---- friction ------------------------------------------------------------
  * TangentialCollisions::build(mesh, V, collisions, BarrierPotential(dhat,
- * kappa), mu_s, mu_k) with global coefficients (tangential_collisions.cpp:
+ * kappa), mu_s, mu_k) with coefficients equal at every vertex (tangential_collisions.cpp:
  * 61-186; tangential_collision.cpp:8-29): one record per collision of `c`
  * (its order); near-parallel edge-edge collisions are not kept.  beta: the
  * closest-point coordinates (2), basis: the tangent basis (3 x 2 column
@@ -312,11 +312,17 @@ typedef struct ipco_friction {
     double *basis;
     double *normal_force;
     double *weight;
-    double mu_s, mu_k;
+    double *mu_s, *mu_k; /* per collision, blended from the vertices' */
 } ipco_friction;
 int ipco_friction_build(const ipco_collisions *c, const double *V, int nV, const int32_t *E,
                         const int32_t *F, double dhat, double kappa, double mu_s, double mu_k,
                         ipco_friction *out);
+/* the same with per-vertex coefficients (nV each) blended per collision:
+ * the value at each side's closest point, averaged
+ * (tangential_collisions.cpp:86-166, default_blend_mu) */
+int ipco_friction_build_pv(const ipco_collisions *c, const double *V, int nV, const int32_t *E,
+                           const int32_t *F, double dhat, double kappa, const double *mu_s,
+                           const double *mu_k, ipco_friction *out);
 void ipco_friction_free(ipco_friction *f);
 double ipco_friction_energy(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
                             int nV, const int32_t *E, const int32_t *F, double eps_v);

@@ -1605,12 +1605,51 @@ int ipco_friction_build(const ipco_collisions *c, const double *V, int nV, const
                         const int32_t *F, double dhat, double kappa, double mu_s, double mu_k,
                         ipco_friction *out)
 {
+    double *ms = (double *)malloc(sizeof(double) * (size_t)(nV > 0 ? nV : 1));
+    double *mk = (double *)malloc(sizeof(double) * (size_t)(nV > 0 ? nV : 1));
+    for (int i = 0; i < nV; i++) {
+        ms[i] = mu_s;
+        mk[i] = mu_k;
+    }
+    const int rc = ipco_friction_build_pv(c, V, nV, E, F, dhat, kappa, ms, mk, out);
+    free(ms);
+    free(mk);
+    return rc;
+}
+
+/* tangential_collisions.hpp:143-149 default_blend_mu */
+static double blend_mu(double a, double b)
+{
+    return (a + b) / 2;
+}
+
+/* tangential_collisions.cpp:86-166: the coefficient at the closest point
+ * of each side, blended */
+static double collision_mu(const stencil *s, const double *beta, const double *m)
+{
+    const int32_t *v = s->ids;
+    switch (s->kind) {
+    case 3: return blend_mu(m[v[0]], m[v[1]]);
+    case 2: return blend_mu((m[v[2]] - m[v[1]]) * beta[0] + m[v[1]], m[v[0]]);
+    case 1:
+        return blend_mu((m[v[1]] - m[v[0]]) * beta[0] + m[v[0]],
+                        (m[v[3]] - m[v[2]]) * beta[1] + m[v[2]]);
+    default:
+        return blend_mu(m[v[1]] + beta[0] * (m[v[2]] - m[v[1]]) + beta[1] * (m[v[3]] - m[v[1]]),
+                        m[v[0]]);
+    }
+}
+
+int ipco_friction_build_pv(const ipco_collisions *c, const double *V, int nV, const int32_t *E,
+                           const int32_t *F, double dhat, double kappa, const double *mu_s,
+                           const double *mu_k, ipco_friction *out)
+{
     (void)nV;
     const int64_t n = n_collisions(c);
     memset(out, 0, sizeof(*out));
     out->n = n;
-    out->mu_s = mu_s;
-    out->mu_k = mu_k;
+    out->mu_s = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    out->mu_k = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     out->keep = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
     out->beta = (double *)calloc(2 * (size_t)(n > 0 ? n : 1), sizeof(double));
     out->basis = (double *)calloc(6 * (size_t)(n > 0 ? n : 1), sizeof(double));
@@ -1643,6 +1682,8 @@ int ipco_friction_build(const ipco_collisions *c, const double *V, int nV, const
         /* barrier_force_magnitude.cpp:5-15 */
         out->normal_force[i] = -f1(d2, c->dmin, dhat, kappa) * 2 * sqrt(d2);
         out->weight[i] = s.w;
+        out->mu_s[i] = collision_mu(&s, out->beta + 2 * i, mu_s);
+        out->mu_k[i] = collision_mu(&s, out->beta + 2 * i, mu_k);
     }
     return 0;
 }
@@ -1654,6 +1695,8 @@ void ipco_friction_free(ipco_friction *f)
     free(f->basis);
     free(f->normal_force);
     free(f->weight);
+    free(f->mu_s);
+    free(f->mu_k);
     memset(f, 0, sizeof(*f));
 }
 
@@ -1689,7 +1732,7 @@ static void friction_local(const ipco_friction *fr, const stencil *s, int64_t i,
     tangent_project(P, r, u);
     const double norm = sqrt(u[0] * u[0] + u[1] * u[1]);
     const double scale = fr->weight[i] * fr->normal_force[i];
-    const double ms = fr->mu_s, mk = fr->mu_k;
+    const double ms = fr->mu_s[i], mk = fr->mu_k[i];
     (void)beta2;
     /* Gamma coefficients from the stored coordinates */
     switch (s->kind) {

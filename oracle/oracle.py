@@ -51,7 +51,8 @@ class _CSR(C.Structure):
 class _Friction(C.Structure):
     _fields_ = [("n", C.c_int64), ("keep", C.POINTER(C.c_uint8)), ("beta", C.POINTER(C.c_double)),
                 ("basis", C.POINTER(C.c_double)), ("normal_force", C.POINTER(C.c_double)),
-                ("weight", C.POINTER(C.c_double)), ("mu_s", C.c_double), ("mu_k", C.c_double)]
+                ("weight", C.POINTER(C.c_double)), ("mu_s", C.POINTER(C.c_double)),
+                ("mu_k", C.POINTER(C.c_double))]
 
 
 class CCDParams(C.Structure):
@@ -190,6 +191,8 @@ def lib():
         L.ipco_friction_build.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I, C.c_double,
                                           C.c_double, C.c_double, C.c_double, C.POINTER(_Friction)]
         L.ipco_friction_free.argtypes = [C.POINTER(_Friction)]
+        L.ipco_friction_build_pv.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
+                                             C.c_double, C.c_double, _D, _D, C.POINTER(_Friction)]
         L.ipco_friction_energy.argtypes = [C.POINTER(_Friction), C.POINTER(_Collisions), _D, C.c_int,
                                            _I, _I, C.c_double]
         L.ipco_friction_energy.restype = C.c_double
@@ -815,6 +818,8 @@ class Friction:
         self.basis = arr(s.basis, 6).reshape(n, 2, 3).transpose(0, 2, 1)  # n x 3 x 2
         self.normal_force = arr(s.normal_force, 1)
         self.weight = arr(s.weight, 1)
+        self.mu_s = arr(s.mu_s, 1)
+        self.mu_k = arr(s.mu_k, 1)
 
     def __del__(self):
         try:
@@ -824,12 +829,20 @@ class Friction:
 
 
 def friction_build(coll, V, E, F, dhat, kappa, mu_s, mu_k=None):
+    """mu_s / mu_k: scalars, or per-vertex arrays (blended per collision)"""
     V, pV = _d(V)
     E, pE = _i(np.asarray(E).reshape(-1, 2))
     F, pF = _i(np.asarray(F).reshape(-1, 3))
     s = _Friction()
-    lib().ipco_friction_build(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa, mu_s,
-                              mu_s if mu_k is None else mu_k, C.byref(s))
+    mu_k = mu_s if mu_k is None else mu_k
+    if np.ndim(mu_s) == 0 and np.ndim(mu_k) == 0:
+        lib().ipco_friction_build(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa, mu_s, mu_k,
+                                  C.byref(s))
+    else:
+        ms, pms = _d(np.broadcast_to(np.asarray(mu_s, float), (len(V),)))
+        mk, pmk = _d(np.broadcast_to(np.asarray(mu_k, float), (len(V),)))
+        lib().ipco_friction_build_pv(C.byref(coll._s), pV, len(V), pE, pF, dhat, kappa, pms, pmk,
+                                     C.byref(s))
     return Friction(s, coll, E, F)
 
 

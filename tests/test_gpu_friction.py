@@ -61,6 +61,28 @@ def test_friction(ctx, O, scenes, name, mus):
             assert sparse_rel_err(ctx.friction_hessian(EPS_V, psd), H_o) <= 1e-10, psd
 
 
+@pytest.mark.parametrize("name", ["spheres:4:3:0", "stack:32:0"])
+def test_per_vertex_coefficients(ctx, O, scenes, name):
+    s = make(scenes, name)
+    dhat = 3 * s.dhat
+    say("case per-vertex", name)
+    upload(ctx, s)
+    ctx.build_collisions(dhat)
+    ee, fv = ctx.candidates()
+    vv, ev = ctx.codim_candidates()
+    co = O.build_collisions(s.V, s.E, s.F, ee, fv, dhat, vv_cand=vv, ev_cand=ev)
+    m = np.random.default_rng(5).uniform(0.1, 0.9, len(s.V))
+    ctx.friction_build(dhat, s.kappa, m, 0.6 * m)
+    fr = O.friction_build(co, s.V, s.E, s.F, dhat, s.kappa, m, 0.6 * m)
+    Vel = np.random.default_rng(6).normal(0, 2 * EPS_V, s.V.shape)
+    ctx.upload_velocities(Vel)
+    e_o = O.friction_energy(fr, Vel, EPS_V)
+    assert abs(ctx.friction_energy(EPS_V) - e_o) <= 1e-12 * abs(e_o)
+    assert rel_err(ctx.friction_gradient(EPS_V), O.friction_gradient(fr, Vel, EPS_V)) <= 1e-12
+    H_o = O.friction_hessian(fr, Vel, EPS_V, "clamp")
+    assert sparse_rel_err(ctx.friction_hessian(EPS_V, "clamp"), H_o) <= 1e-10
+
+
 def test_requires_a_current_set(ctx, capi, scenes):
     s = make(scenes, "sheets:16:0")
     upload(ctx, s)

@@ -179,3 +179,31 @@ def test_near_parallel_edges_are_not_kept(O):
     fr = O.friction_build(co, V, E, Z3, 0.01, 1.0, 0.5)
     assert not fr.keep[0]
     assert O.friction_energy(fr, np.ones_like(V), EPS_V) == 0.0
+
+
+def test_per_vertex_coefficients(O):
+    """tangential_collisions.cpp:86-166 with default_blend_mu (the average):
+    each side's coefficient at its closest point, averaged"""
+    V, E, F = four_kinds(np.random.default_rng(1))
+    co = kinds_collisions(O, V, E, F, 0.02)
+    m = np.random.default_rng(2).uniform(0.1, 0.9, len(V))
+    fr = O.friction_build(co, V, E, F, 0.02, 1.0, m, 0.5 * m)
+    b = fr.beta
+    a, c = co.vv[0]
+    want = [(m[a] + m[c]) / 2]
+    e, v = co.ev[0]
+    want.append(((m[E[e, 1]] - m[E[e, 0]]) * b[1, 0] + m[E[e, 0]] + m[v]) / 2)
+    ea, eb = co.ee[0]
+    want.append((((m[E[ea, 1]] - m[E[ea, 0]]) * b[2, 0] + m[E[ea, 0]])
+                 + ((m[E[eb, 1]] - m[E[eb, 0]]) * b[2, 1] + m[E[eb, 0]])) / 2)
+    f, v = co.fv[0]
+    t = F[f]
+    want.append((m[t[0]] + b[3, 0] * (m[t[1]] - m[t[0]]) + b[3, 1] * (m[t[2]] - m[t[0]]) + m[v]) / 2)
+    assert np.allclose(fr.mu_s, want, rtol=1e-15)
+    assert np.allclose(fr.mu_k, 0.5 * np.array(want), rtol=1e-15)
+    # equal coefficients everywhere reproduce the scalar build exactly
+    f1 = O.friction_build(co, V, E, F, 0.02, 1.0, 0.4, 0.3)
+    f2 = O.friction_build(co, V, E, F, 0.02, 1.0, np.full(len(V), 0.4), np.full(len(V), 0.3))
+    assert np.array_equal(f1.mu_s, f2.mu_s) and np.all(f1.mu_s == 0.4)
+    Vel = np.random.default_rng(4).normal(0, 1e-3, V.shape)
+    assert O.friction_energy(f1, Vel, EPS_V) == O.friction_energy(f2, Vel, EPS_V)
