@@ -70,6 +70,9 @@ def parse():
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB MiB device buffer five times (torch), the known-size "
                         "reference for scaling the HBM byte counters of a profiler")
+    p.add_argument("--rows", action="store_true",
+                   help="time the collision-set variants, helpers and friction at the same "
+                        "workload instead (one JSON line per entry point)")
     p.add_argument("--verbose", action="store_true")
     return p.parse_args()
 
@@ -136,6 +139,63 @@ def copy_calibration(mib):
                       "GBps": round(2 * n / dt / 1e9, 1)}))
 
 
+def rows(args, ctx, sc, capi):
+    """The entry points beside the step on the same workload: per entry
+    point, the median wall time of `args.steps` synchronous calls after one
+    untimed call, and the HIP-event time of its stages per call."""
+    import numpy as np
+    dhat, kappa, eps_v = sc.dhat, sc.kappa, 1e-3
+
+    def timed(name, fn, **work):
+        fn()
+        ctx.timing(True)
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        st = ctx.stage_times()
+        ctx.timing(False)
+        out = {"row": name, "ms_per_call": round(1e3 * float(np.median(ts)), 3),
+               "calls": args.steps, "workload": sc.name,
+               "stages_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in st.items()}}
+        out.update(work)
+        print(json.dumps(out), flush=True)
+
+    ctx.build_collisions(dhat)
+    base = [int(x) for x in ctx.collision_counts()]
+    timed("has_intersections", ctx.has_intersections)
+    for label, aw, kind in (("collisions_ipc", False, capi.COLLISIONS_IPC),
+                            ("collisions_area_weighted", True, capi.COLLISIONS_IPC),
+                            ("collisions_improved_max_approx", True,
+                             capi.COLLISIONS_IMPROVED_MAX_APPROX),
+                            ("collisions_ogc", False, capi.COLLISIONS_OGC)):
+        ctx.set_area_weighting(aw)
+        ctx.set_collision_set_type(kind)
+        timed(label, lambda: ctx.build_collisions_from_candidates(dhat),
+              collisions=[int(x) for x in ctx.collision_counts()])
+    ctx.set_area_weighting(False)
+    ctx.set_collision_set_type(capi.COLLISIONS_IPC)
+    ctx.build_collisions_from_candidates(dhat)
+    p = np.random.default_rng(0).normal(size=3 * len(sc.V))
+    timed("gauss_newton_diagonal", lambda: ctx.gauss_newton_diagonal(dhat, kappa),
+          collisions=base, note="includes the 3 nV download")
+    timed("gauss_newton_quadratic_form", lambda: ctx.gauss_newton_quadratic_form(dhat, kappa, p),
+          collisions=base, note="includes the 3 nV upload")
+    r = 0.5 * dhat
+    timed("per_vertex_safe_distances", lambda: ctx.per_vertex_safe_distances(r, 0.0),
+          note="includes the nV download")
+    timed("noncandidate_conservative_stepsize",
+          lambda: ctx.noncandidate_conservative_stepsize(dhat))
+    timed("friction_build", lambda: ctx.friction_build(dhat, kappa, 0.5, 0.3), collisions=base)
+    ctx.upload_velocities(np.random.default_rng(1).normal(0, 2 * eps_v, sc.V.shape))
+    timed("friction_energy", lambda: ctx.friction_energy(eps_v), collisions=base)
+    timed("friction_gradient", lambda: ctx.friction_gradient(eps_v), collisions=base,
+          note="includes the 3 nV download")
+    timed("friction_hessian", lambda: ctx.friction_hessian(eps_v, "clamp", download=False),
+          collisions=base)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -157,6 +217,10 @@ def main():
     ctx.upload_positions(0, sc.V)
     ctx.upload_positions(1, sc.V1)
     ctx.synchronize()
+
+    if args.rows:
+        rows(args, ctx, sc, capi)
+        return
 
     for i in range(args.warmup):
         gpu_step(ctx, sc)

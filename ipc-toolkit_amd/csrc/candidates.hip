@@ -88,9 +88,13 @@ __global__ __launch_bounds__(CH_BLOCK) void k_safe_d2(CandLists L, const double*
         const d3 c = ldx(X, nV, v[2]);
         d2 = pe_dist2(a, b, c, point_edge_type(a, b, c));
     }
-    // NaN (degenerate) never lowers a minimum: its bits exceed +inf's
+    // NaN (degenerate) never lowers a minimum: its bits exceed +inf's.  The
+    // stored minimum only decreases, so a key not below a value read from it
+    // cannot lower it: the atomic is issued only for keys below the read.
     const unsigned long long key = (unsigned long long)__double_as_longlong(d2 + 0.0);
-    for (int k = 0; k < nv; k++) atomicMin(best + v[k], key);
+    for (int k = 0; k < nv; k++)
+        if (key < __hip_atomic_load(best + v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(best + v[k], key);
 }
 
 __global__ __launch_bounds__(CH_BLOCK) void k_mark(CandLists L, uint32_t* mark)
@@ -99,7 +103,8 @@ __global__ __launch_bounds__(CH_BLOCK) void k_mark(CandLists L, uint32_t* mark)
     if (i >= L.off[4]) return;
     int v[4], kind;
     const int nv = cand_vertices(L, i, v, kind);
-    for (int k = 0; k < nv; k++) mark[v[k]] = 1u;
+    for (int k = 0; k < nv; k++)
+        if (!mark[v[k]]) mark[v[k]] = 1u; // most vertices are marked many times
 }
 
 // largest squared displacement over marked vertices (block partials)

@@ -386,9 +386,11 @@ class Context:
         self._check(self._L.ipcg_friction_gradient(self._p, eps_v, g.ctypes.data_as(_D)))
         return g
 
-    def friction_hessian(self, eps_v, psd="clamp"):
+    def friction_hessian(self, eps_v, psd="clamp", download=True):
         nnz = C.c_int64()
         self._check(self._L.ipcg_friction_hessian(self._p, eps_v, PSD[psd], C.byref(nnz)))
+        if not download:
+            return nnz.value
         import scipy.sparse as sp
         n = 3 * self.nV
         rowptr = np.zeros(n + 1, np.int64)
