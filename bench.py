@@ -30,7 +30,8 @@ sys.path.insert(0, os.path.join(ROOT, "ipc-toolkit_amd"))
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix), TFLOP/s
 
-# Algorithmic work models (per unit).
+# Algorithmic work models, per unit of each stage (bytes a kernel must move
+# and FP64 flops it must execute, counted from the algorithm).
 # Hessian record of one collision: ids, weight (+ eps, dtype for EE), the
 # stencil's points, and the packed projected Hessian written out.
 REC_BYTES = {"ee": 8 + 8 + 8 + 4 + 4 * 24 + 45 * 8,
@@ -202,8 +203,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path would shard scenes or Morton ranges across GPUs, see DESIGN.md)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(several GPUs would each take whole scenes or a Morton range of one scene)",
               file=sys.stderr)
         sys.exit(2)
     from ipctk_amd import capi, scenes
@@ -254,7 +255,7 @@ def main():
 
     # Roofline of the dominant kernel.  Each stage below is one kernel
     # launch per call, timed with HIP events on the context stream.  Its
-    # algorithmic bytes and FP64 flops per launch follow DESIGN.md section 3;
+    # algorithmic bytes and FP64 flops per launch follow the models above;
     # the binding roof is the larger of bytes / HBM peak and flops / FP64 peak.
     per_launch = {k: v[0] / v[1] for k, v in stages.items()}
     per_step = {k: v[0] / args.steps for k, v in stages.items()}
