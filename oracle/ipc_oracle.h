@@ -313,6 +313,11 @@ typedef struct ipco_friction {
     double *normal_force;
     double *weight;
     double *mu_s, *mu_k; /* per collision, blended from the vertices' */
+    /* anisotropy per collision (tangential_collision.hpp:108-139): velocity
+     * scaling mu_aniso (2, default 1), elliptical mu_s_aniso / mu_k_aniso
+     * (2 each, default 0) and the lagged effective coefficients used by the
+     * potential (default mu_s, mu_k) */
+    double *mu_aniso, *mu_s_aniso, *mu_k_aniso, *mu_s_eff, *mu_k_eff;
 } ipco_friction;
 int ipco_friction_build(const ipco_collisions *c, const double *V, int nV, const int32_t *E,
                         const int32_t *F, double dhat, double kappa, double mu_s, double mu_k,
@@ -324,6 +329,21 @@ int ipco_friction_build_pv(const ipco_collisions *c, const double *V, int nV, co
                            const int32_t *F, double dhat, double kappa, const double *mu_s,
                            const double *mu_k, ipco_friction *out);
 void ipco_friction_free(ipco_friction *f);
+/* set the per-collision anisotropy (2 values per collision each; NULL keeps
+ * the current values) and reset the effective coefficients to mu_s, mu_k
+ * (reset_lagged_anisotropic_friction_coefficients) */
+void ipco_friction_set_anisotropy(ipco_friction *f, const double *mu_aniso,
+                                  const double *mu_s_aniso, const double *mu_k_aniso);
+/* update_lagged_anisotropic_friction_coefficients (tangential_collisions.cpp:
+ * 19-57) with the lagged positions the set was built at: tau = T^T v at the
+ * lagged basis, tau_aniso = mu_aniso * tau, effective coefficients from the
+ * elliptical model (smooth_mu.cpp anisotropic_mu_eff_from_tau_aniso) when
+ * mu_s_aniso is nonzero, else mu_s, mu_k */
+void ipco_friction_update_lagged_mu(ipco_friction *f, const ipco_collisions *c, const double *Vel,
+                                    const int32_t *E, const int32_t *F);
+/* smooth_mu.cpp:115-171: (mu_s_eff, mu_k_eff) for tau_aniso */
+void ipco_anisotropic_mu_eff(const double tau_aniso[2], const double mu_s_aniso[2],
+                             const double mu_k_aniso[2], double mu_s, double mu_k, double out[2]);
 double ipco_friction_energy(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,
                             int nV, const int32_t *E, const int32_t *F, double eps_v);
 void ipco_friction_gradient(const ipco_friction *fr, const ipco_collisions *c, const double *Vel,

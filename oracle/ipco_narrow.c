@@ -1650,6 +1650,12 @@ int ipco_friction_build_pv(const ipco_collisions *c, const double *V, int nV, co
     out->n = n;
     out->mu_s = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     out->mu_k = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    out->mu_aniso = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    out->mu_s_aniso = (double *)calloc(2 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    out->mu_k_aniso = (double *)calloc(2 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    out->mu_s_eff = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    out->mu_k_eff = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    for (int64_t i = 0; i < 2 * n; i++) out->mu_aniso[i] = 1.0;
     out->keep = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
     out->beta = (double *)calloc(2 * (size_t)(n > 0 ? n : 1), sizeof(double));
     out->basis = (double *)calloc(6 * (size_t)(n > 0 ? n : 1), sizeof(double));
@@ -1684,8 +1690,84 @@ int ipco_friction_build_pv(const ipco_collisions *c, const double *V, int nV, co
         out->weight[i] = s.w;
         out->mu_s[i] = collision_mu(&s, out->beta + 2 * i, mu_s);
         out->mu_k[i] = collision_mu(&s, out->beta + 2 * i, mu_k);
+        out->mu_s_eff[i] = out->mu_s[i];
+        out->mu_k_eff[i] = out->mu_k[i];
     }
     return 0;
+}
+
+/* smooth_mu.cpp:115-171 */
+void ipco_anisotropic_mu_eff(const double ta[2], const double msa[2], const double mka[2],
+                             double mu_s, double mu_k, double out[2])
+{
+    const int use_s = msa[0] * msa[0] + msa[1] * msa[1] > 0;
+    const int use_k = mka[0] * mka[0] + mka[1] * mka[1] > 0;
+    out[0] = mu_s;
+    out[1] = mu_k;
+    if (!use_s && !use_k) return;
+    double d[2] = { 1.0, 0.0 };
+    const double n = sqrt(ta[0] * ta[0] + ta[1] * ta[1]);
+    if (!(n < 1e-10)) {
+        d[0] = ta[0] / n;
+        d[1] = ta[1] / n;
+    }
+    if (use_s) {
+        const double a = msa[0] * d[0], b = msa[1] * d[1];
+        out[0] = sqrt(a * a + b * b);
+    }
+    if (use_k) {
+        const double a = mka[0] * d[0], b = mka[1] * d[1];
+        out[1] = sqrt(a * a + b * b);
+    }
+}
+
+void ipco_friction_set_anisotropy(ipco_friction *f, const double *mu_aniso,
+                                  const double *mu_s_aniso, const double *mu_k_aniso)
+{
+    for (int64_t i = 0; i < 2 * f->n; i++) {
+        if (mu_aniso) f->mu_aniso[i] = mu_aniso[i];
+        if (mu_s_aniso) f->mu_s_aniso[i] = mu_s_aniso[i];
+        if (mu_k_aniso) f->mu_k_aniso[i] = mu_k_aniso[i];
+    }
+    for (int64_t i = 0; i < f->n; i++) {
+        f->mu_s_eff[i] = f->mu_s[i];
+        f->mu_k_eff[i] = f->mu_k[i];
+    }
+}
+
+void ipco_friction_update_lagged_mu(ipco_friction *f, const ipco_collisions *c, const double *Vel,
+                                    const int32_t *E, const int32_t *F)
+{
+    for (int64_t i = 0; i < f->n; i++) {
+        f->mu_s_eff[i] = f->mu_s[i];
+        f->mu_k_eff[i] = f->mu_k[i];
+        if (!f->keep[i]) continue;
+        const double *msa = f->mu_s_aniso + 2 * i;
+        if (msa[0] * msa[0] + msa[1] * msa[1] == 0) continue;
+        stencil s;
+        double v[12], coef[4], tau[2] = { 0, 0 };
+        get_stencil(c, E, F, i, &s);
+        for (int a = 0; a < 4; a++)
+            for (int k = 0; k < 3; k++) v[3 * a + k] = a < s.nv ? Vel[3 * (size_t)s.ids[a] + k] : 0.0;
+        const double *beta = f->beta + 2 * i, *P = f->basis + 6 * i;
+        switch (s.kind) {
+        case 3: coef[0] = 1; coef[1] = -1; coef[2] = coef[3] = 0; break;
+        case 2: coef[0] = 1; coef[1] = beta[0] - 1; coef[2] = -beta[0]; coef[3] = 0; break;
+        case 1: coef[0] = 1 - beta[0]; coef[1] = beta[0]; coef[2] = beta[1] - 1; coef[3] = -beta[1]; break;
+        default: coef[0] = 1; coef[1] = beta[0] + beta[1] - 1; coef[2] = -beta[0]; coef[3] = -beta[1];
+        }
+        /* tau = T^T v with T = Gamma^T P */
+        for (int a = 0; a < s.nv; a++)
+            for (int k = 0; k < 3; k++) {
+                tau[0] += coef[a] * P[k] * v[3 * a + k];
+                tau[1] += coef[a] * P[3 + k] * v[3 * a + k];
+            }
+        const double ta[2] = { f->mu_aniso[2 * i] * tau[0], f->mu_aniso[2 * i + 1] * tau[1] };
+        double out[2];
+        ipco_anisotropic_mu_eff(ta, msa, f->mu_k_aniso + 2 * i, f->mu_s[i], f->mu_k[i], out);
+        f->mu_s_eff[i] = out[0];
+        f->mu_k_eff[i] = out[1];
+    }
 }
 
 void ipco_friction_free(ipco_friction *f)
@@ -1697,6 +1779,11 @@ void ipco_friction_free(ipco_friction *f)
     free(f->weight);
     free(f->mu_s);
     free(f->mu_k);
+    free(f->mu_aniso);
+    free(f->mu_s_aniso);
+    free(f->mu_k_aniso);
+    free(f->mu_s_eff);
+    free(f->mu_k_eff);
     memset(f, 0, sizeof(*f));
 }
 
@@ -1730,9 +1817,14 @@ static void friction_local(const ipco_friction *fr, const stencil *s, int64_t i,
     const double *P = fr->basis + 6 * i, *beta = fr->beta + 2 * i;
     relative_velocity(s->kind, v, beta, r);
     tangent_project(P, r, u);
+    /* tangential_potential.cpp:170-181: u_aniso = mu_aniso (.) u, T_aniso =
+     * T diag(mu_aniso), the lagged effective coefficients */
+    const double an[2] = { fr->mu_aniso[2 * i], fr->mu_aniso[2 * i + 1] };
+    u[0] = an[0] * u[0];
+    u[1] = an[1] * u[1];
     const double norm = sqrt(u[0] * u[0] + u[1] * u[1]);
     const double scale = fr->weight[i] * fr->normal_force[i];
-    const double ms = fr->mu_s[i], mk = fr->mu_k[i];
+    const double ms = fr->mu_s_eff[i], mk = fr->mu_k_eff[i];
     (void)beta2;
     /* Gamma coefficients from the stored coordinates */
     switch (s->kind) {
@@ -1748,7 +1840,7 @@ static void friction_local(const ipco_friction *fr, const stencil *s, int64_t i,
         const double g0 = f1x * u[0] * scale, g1 = f1x * u[1] * scale;
         for (int a = 0; a < s->nv; a++)
             for (int k = 0; k < 3; k++)
-                grad[3 * a + k] = (coef[a] * P[k]) * g0 + (coef[a] * P[3 + k]) * g1;
+                grad[3 * a + k] = (coef[a] * P[k] * an[0]) * g0 + (coef[a] * P[3 + k] * an[1]) * g1;
     }
     if (hess) {
         /* inner 2 x 2 matrix M, H = T M T^T with T[3a + k][c] = coef_a P[k][c] */
@@ -1773,8 +1865,8 @@ static void friction_local(const ipco_friction *fr, const stencil *s, int64_t i,
         }
         for (int a = 0; a < nd; a++)
             for (int b = 0; b < nd; b++) {
-                const double ta0 = coef[a / 3] * P[a % 3], ta1 = coef[a / 3] * P[3 + a % 3];
-                const double tb0 = coef[b / 3] * P[b % 3], tb1 = coef[b / 3] * P[3 + b % 3];
+                const double ta0 = coef[a / 3] * P[a % 3] * an[0], ta1 = coef[a / 3] * P[3 + a % 3] * an[1];
+                const double tb0 = coef[b / 3] * P[b % 3] * an[0], tb1 = coef[b / 3] * P[3 + b % 3] * an[1];
                 hess[a * nd + b] = ta0 * (M[0] * tb0 + M[1] * tb1) + ta1 * (M[2] * tb0 + M[3] * tb1);
             }
     }

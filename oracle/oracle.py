@@ -52,7 +52,9 @@ class _Friction(C.Structure):
     _fields_ = [("n", C.c_int64), ("keep", C.POINTER(C.c_uint8)), ("beta", C.POINTER(C.c_double)),
                 ("basis", C.POINTER(C.c_double)), ("normal_force", C.POINTER(C.c_double)),
                 ("weight", C.POINTER(C.c_double)), ("mu_s", C.POINTER(C.c_double)),
-                ("mu_k", C.POINTER(C.c_double))]
+                ("mu_k", C.POINTER(C.c_double)), ("mu_aniso", C.POINTER(C.c_double)),
+                ("mu_s_aniso", C.POINTER(C.c_double)), ("mu_k_aniso", C.POINTER(C.c_double)),
+                ("mu_s_eff", C.POINTER(C.c_double)), ("mu_k_eff", C.POINTER(C.c_double))]
 
 
 class CCDParams(C.Structure):
@@ -191,6 +193,10 @@ def lib():
         L.ipco_friction_build.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I, C.c_double,
                                           C.c_double, C.c_double, C.c_double, C.POINTER(_Friction)]
         L.ipco_friction_free.argtypes = [C.POINTER(_Friction)]
+        L.ipco_friction_set_anisotropy.argtypes = [C.POINTER(_Friction), _D, _D, _D]
+        L.ipco_friction_update_lagged_mu.argtypes = [C.POINTER(_Friction), C.POINTER(_Collisions),
+                                                     _D, _I, _I]
+        L.ipco_anisotropic_mu_eff.argtypes = [_D, _D, _D, C.c_double, C.c_double, _D]
         L.ipco_friction_build_pv.argtypes = [C.POINTER(_Collisions), _D, C.c_int, _I, _I,
                                              C.c_double, C.c_double, _D, _D, C.POINTER(_Friction)]
         L.ipco_friction_energy.argtypes = [C.POINTER(_Friction), C.POINTER(_Collisions), _D, C.c_int,
@@ -876,3 +882,40 @@ def friction_hessian(fr, Vel, eps_v, psd="clamp"):
     val = np.ctypeslib.as_array(m.val, shape=(max(nnz, 1),))[:nnz].copy()
     lib().ipco_csr_free(C.byref(m))
     return sp.csr_matrix((val, col, rowptr), shape=(n, n))
+
+
+def friction_set_anisotropy(fr, mu_aniso=None, mu_s_aniso=None, mu_k_aniso=None):
+    """per-collision (n, 2) arrays; None keeps the current values; resets the
+    effective coefficients"""
+    keep = []
+
+    def arg(a):
+        if a is None:
+            return None
+        a, p = _d(np.asarray(a, float).reshape(-1, 2))
+        keep.append(a)
+        return p
+    lib().ipco_friction_set_anisotropy(C.byref(fr._s), arg(mu_aniso), arg(mu_s_aniso),
+                                       arg(mu_k_aniso))
+
+
+def friction_update_lagged_mu(fr, Vel):
+    Vel, a = _fargs(fr, Vel)
+    lib().ipco_friction_update_lagged_mu(a[0], a[1], a[2], a[4], a[5])
+
+
+def friction_effective_mu(fr):
+    n = fr._s.n
+    if n == 0:
+        return np.zeros(0), np.zeros(0)
+    return (np.ctypeslib.as_array(fr._s.mu_s_eff, shape=(n,)).copy(),
+            np.ctypeslib.as_array(fr._s.mu_k_eff, shape=(n,)).copy())
+
+
+def anisotropic_mu_eff(tau_aniso, mu_s_aniso, mu_k_aniso, mu_s, mu_k):
+    t, pt = _d(np.asarray(tau_aniso, float))
+    a, pa = _d(np.asarray(mu_s_aniso, float))
+    b, pb = _d(np.asarray(mu_k_aniso, float))
+    out = np.zeros(2)
+    lib().ipco_anisotropic_mu_eff(pt, pa, pb, mu_s, mu_k, out.ctypes.data_as(_D))
+    return out

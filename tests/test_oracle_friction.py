@@ -207,3 +207,59 @@ def test_per_vertex_coefficients(O):
     assert np.array_equal(f1.mu_s, f2.mu_s) and np.all(f1.mu_s == 0.4)
     Vel = np.random.default_rng(4).normal(0, 1e-3, V.shape)
     assert O.friction_energy(f1, Vel, EPS_V) == O.friction_energy(f2, Vel, EPS_V)
+
+
+def test_anisotropic_mu_eff_reference_case(O):
+    # test_anisotropic_friction.cpp:23-79: the L2 (elliptical) projection
+    # sqrt((mu0 t0)^2 + (mu1 t1)^2) over directions, the principal axes
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        ms, mk = rng.uniform(0.1, 0.9, 2), rng.uniform(0.1, 0.9, 2)
+        for ang in np.arange(0, 2 * np.pi, np.pi / 4):
+            t = np.array([np.cos(ang), np.sin(ang)])
+            got = O.anisotropic_mu_eff(3.7 * t, ms, mk, 0.5, 0.4)
+            assert np.allclose(got, [np.hypot(*(ms * t)), np.hypot(*(mk * t))], rtol=1e-14)
+        assert np.allclose(O.anisotropic_mu_eff([2.0, 0], ms, mk, 0.5, 0.4), [ms[0], mk[0]])
+        assert np.allclose(O.anisotropic_mu_eff([0, -2.0], ms, mk, 0.5, 0.4), [ms[1], mk[1]])
+    # zero tangential velocity: the first axis; no anisotropy: the isotropic pair
+    assert np.allclose(O.anisotropic_mu_eff([0, 0], [0.3, 0.7], [0.2, 0.6], 0.5, 0.4), [0.3, 0.2])
+    assert np.array_equal(O.anisotropic_mu_eff([1, 1], [0, 0], [0, 0], 0.5, 0.4), [0.5, 0.4])
+
+
+def test_anisotropic_friction(O, scenes):
+    s = scenes.two_sheets(10, seed=2)
+    co = O.build_collisions(s.V, s.E, s.F, *O.candidates(s.V, s.E, s.F, 1.5 * s.dhat), 3 * s.dhat)
+    fr = O.friction_build(co, s.V, s.E, s.F, 3 * s.dhat, s.kappa, 0.6, 0.3)
+    n = len(fr.keep)
+    rng = np.random.default_rng(7)
+    Vel = rng.normal(0, 2 * EPS_V, s.V.shape)
+    e_iso = O.friction_energy(fr, Vel, EPS_V)
+    # defaults reproduce the isotropic potential exactly
+    O.friction_set_anisotropy(fr, np.ones((n, 2)), np.zeros((n, 2)), np.zeros((n, 2)))
+    O.friction_update_lagged_mu(fr, Vel)
+    assert O.friction_energy(fr, Vel, EPS_V) == e_iso
+    # velocity scaling and elliptical coefficients from the lagged velocities
+    a = rng.uniform(0.5, 2.0, (n, 2))
+    msa, mka = rng.uniform(0.2, 0.8, (n, 2)), rng.uniform(0.1, 0.5, (n, 2))
+    O.friction_set_anisotropy(fr, a, msa, mka)
+    O.friction_update_lagged_mu(fr, Vel)
+    ms_eff, mk_eff = O.friction_effective_mu(fr)
+    k = fr.keep
+    assert np.all(ms_eff[k] <= np.linalg.norm(msa[k], axis=1) + 1e-15)
+    assert np.all(ms_eff[k] >= msa[k].min(axis=1) - 1e-15)
+    # derivatives with the coefficients held (lagged)
+    g = O.friction_gradient(fr, Vel, EPS_V)
+    h = 1e-7 * 2 * EPS_V
+    for j in rng.choice(np.nonzero(g)[0], 15, replace=False):
+        Vp, Vm = Vel.copy(), Vel.copy()
+        Vp.flat[j] += h
+        Vm.flat[j] -= h
+        num = (O.friction_energy(fr, Vp, EPS_V) - O.friction_energy(fr, Vm, EPS_V)) / (2 * h)
+        assert abs(num - g[j]) <= 1e-5 * np.abs(g).max(), j
+    H = O.friction_hessian(fr, Vel, EPS_V, "none").toarray()
+    for j in rng.choice(np.nonzero(g)[0], 8, replace=False):
+        Vp, Vm = Vel.copy(), Vel.copy()
+        Vp.flat[j] += h
+        Vm.flat[j] -= h
+        num = (O.friction_gradient(fr, Vp, EPS_V) - O.friction_gradient(fr, Vm, EPS_V)) / (2 * h)
+        assert np.linalg.norm(num - H[:, j]) <= 1e-4 * np.abs(H).max(), j
