@@ -10,7 +10,7 @@
 //                               potentials/potential.hpp:34-60
 //   ipc::TangentialCollisions   collisions/tangential/tangential_collisions.hpp
 //                               (scalar or per-vertex friction coefficients)
-//   ipc::FrictionPotential      potentials/friction_potential.hpp (isotropic)
+//   ipc::FrictionPotential      potentials/friction_potential.hpp
 //   ipc::has_intersections      ipc.hpp (ipc.cpp:105-166)
 //   ipc::compute_collision_free_stepsize, ipc::is_step_collision_free
 //                               ipc.hpp (ipc.cpp:20-101)
@@ -643,6 +643,31 @@ public:
     }
     size_t size() const { return m_size; }
     bool empty() const { return m_size == 0; }
+
+    // tangential_collision.hpp:108-139: per collision of the normal set (in
+    // its order), 2 values each; an empty vector keeps the current values.
+    // Resets the effective coefficients (reset_lagged_anisotropic_friction_
+    // coefficients).
+    void set_anisotropy(const CollisionMesh& mesh, const std::vector<double>& mu_aniso,
+                        const std::vector<double>& mu_s_aniso = {},
+                        const std::vector<double>& mu_k_aniso = {}) const
+    {
+        if (!resident(mesh)) throw std::runtime_error("the friction set is not on the device");
+        auto ptr = [](const std::vector<double>& v) { return v.empty() ? nullptr : v.data(); };
+        detail::check(mesh.context(), ipcg_friction_set_anisotropy(mesh.context(), ptr(mu_aniso),
+                                                                   ptr(mu_s_aniso), ptr(mu_k_aniso)));
+    }
+    // tangential_collisions.cpp:19-57 at the lagged positions of the build
+    void update_lagged_anisotropic_friction_coefficients(const CollisionMesh& mesh,
+                                                         const MatrixXd& velocities) const
+    {
+        if (!resident(mesh)) throw std::runtime_error("the friction set is not on the device");
+        if (velocities.rows() != mesh.num_vertices() || (velocities.rows() && velocities.cols() != 3))
+            throw std::invalid_argument("velocities must be #V x 3");
+        detail::check(mesh.context(), ipcg_velocities_upload(mesh.context(), velocities.data(),
+                                                             IPCG_LAYOUT_AOS));
+        detail::check(mesh.context(), ipcg_friction_update_lagged_mu(mesh.context()));
+    }
     // the device holds this set (no later collision build on the mesh)
     bool resident(const CollisionMesh& mesh) const
     {
@@ -674,8 +699,8 @@ private:
     }
 };
 
-// potentials/friction_potential.hpp (tangential_potential.cpp:156-331),
-// isotropic, at velocities #V x 3
+// potentials/friction_potential.hpp (tangential_potential.cpp:156-331) at
+// velocities #V x 3, with the set's anisotropy and effective coefficients
 class FrictionPotential {
 public:
     explicit FrictionPotential(double eps_v) : m_eps_v(eps_v)

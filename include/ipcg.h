@@ -233,6 +233,20 @@ int ipcg_friction_build(ipcg_ctx* ctx, double dhat, double kappa, double mu_s, d
  * their closest points (tangential_collisions.cpp:86-166) */
 int ipcg_friction_build_per_vertex(ipcg_ctx* ctx, double dhat, double kappa, const double* mu_s,
                                    const double* mu_k);
+/* Anisotropic friction (tangential_collision.hpp:108-139): per collision in
+ * download order, 2 values each: mu_aniso scales the tangential velocity
+ * (u_aniso = mu_aniso * u, default 1), mu_s_aniso / mu_k_aniso give
+ * elliptical coefficients (default 0: isotropic).  Any pointer may be NULL
+ * to keep the current values; the effective coefficients reset to mu_s,
+ * mu_k (reset_lagged_anisotropic_friction_coefficients). */
+int ipcg_friction_set_anisotropy(ipcg_ctx* ctx, const double* mu_aniso, const double* mu_s_aniso,
+                                 const double* mu_k_aniso);
+/* update_lagged_anisotropic_friction_coefficients (tangential_collisions.cpp:
+ * 19-57) at the uploaded velocities, with the lagged positions the set was
+ * built at: tau = T^T v, the effective coefficients sqrt((mu0 t0)^2 +
+ * (mu1 t1)^2) along t = tau_aniso / |tau_aniso| (smooth_mu.cpp:115-171)
+ * where mu_s_aniso is nonzero */
+int ipcg_friction_update_lagged_mu(ipcg_ctx* ctx);
 /* per collision in download order (vv, ev, ee, fv): keep flag,
  * closest-point coordinates (2), tangent basis (3 x 2, column major),
  * normal force magnitude (0 when not kept); any pointer may be NULL */
@@ -242,7 +256,8 @@ int ipcg_friction_download(ipcg_ctx* ctx, int32_t* keep, double* closest_point,
 int ipcg_velocities_upload(ipcg_ctx* ctx, const double* V, int layout);
 /* FrictionPotential(eps_v) (friction_potential.cpp, smooth_mu.cpp,
  * smooth_friction_mollifier.cpp) at the uploaded velocities
- * (tangential_potential.cpp:156-331), isotropic: energy, gradient (host
+ * (tangential_potential.cpp:156-331) with the lagged effective
+ * coefficients and the velocity scaling above: energy, gradient (host
  * array of 3*nV doubles, or NULL to keep it on device) and Hessian with
  * the reference's projection of the inner 2 x 2 matrix (psd as for the
  * barrier; CSR read with ipcg_hessian_download). */

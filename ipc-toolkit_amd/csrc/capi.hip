@@ -673,6 +673,17 @@ int ipcg_friction_build_per_vertex(ipcg_ctx* ctx, double dhat, double kappa, con
     });
 }
 
+int ipcg_friction_set_anisotropy(ipcg_ctx* ctx, const double* mu_aniso, const double* mu_s_aniso,
+                                 const double* mu_k_aniso)
+{
+    return guard(ctx, [&](Ctx& c) { fric_set_anisotropy(c, mu_aniso, mu_s_aniso, mu_k_aniso); });
+}
+
+int ipcg_friction_update_lagged_mu(ipcg_ctx* ctx)
+{
+    return guard(ctx, [&](Ctx& c) { fric_update_lagged_mu(c); });
+}
+
 int ipcg_friction_download(ipcg_ctx* ctx, int32_t* keep, double* closest_point,
                            double* tangent_basis, double* normal_force)
 {

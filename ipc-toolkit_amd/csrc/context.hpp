@@ -152,6 +152,9 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
 void bp_edge_face(Ctx& c, double inflation);
 void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k,
                 const double* mu_sv = nullptr, const double* mu_kv = nullptr);
+void fric_set_anisotropy(Ctx& c, const double* mu_aniso, const double* mu_s_aniso,
+                         const double* mu_k_aniso);
+void fric_update_lagged_mu(Ctx& c);
 void fric_download(Ctx& c, int32_t* keep, double* beta, double* basis, double* normal_force);
 double fric_energy(Ctx& c, double eps_v);
 void fric_gradient(Ctx& c, double eps_v);

@@ -1278,6 +1278,9 @@ struct FricRec {
     double s;         // weight * normal force magnitude; 0: not kept
     double keep;      // 1 kept, 0 skipped (near-parallel edge-edge)
     double mu_s, mu_k; // the collision's static / kinetic coefficients
+    double an[2];      // mu_aniso: velocity scaling (default 1)
+    double msa[2], mka[2]; // mu_s_aniso / mu_k_aniso: elliptical coefficients (default 0)
+    double ms_eff, mk_eff; // lagged effective coefficients used by the potential
 };
 
 struct FricParams {
@@ -1485,6 +1488,9 @@ __global__ __launch_bounds__(PB) void k_fric_build(CollView C, const double* X, 
     r.keep = 1.0;
     r.mu_s = msv ? collision_mu<KIND>(v, r.beta, msv) : ms;
     r.mu_k = mkv ? collision_mu<KIND>(v, r.beta, mkv) : mk;
+    r.an[0] = r.an[1] = 1.0;
+    r.ms_eff = r.mu_s;
+    r.mk_eff = r.mu_k;
     out[i] = r;
 }
 
@@ -1498,8 +1504,9 @@ __device__ __forceinline__ void fric_local(const FricRec& r, const double* Vel, 
 #pragma unroll
     for (int j = 0; j < NP; j++) w[j] = ldx(Vel, nV, v[j]);
     const d3 rel = fric_relative_velocity<KIND>(w, r.beta);
-    u[0] = dot(d3 { r.P[0], r.P[1], r.P[2] }, rel);
-    u[1] = dot(d3 { r.P[3], r.P[4], r.P[5] }, rel);
+    // tangential_potential.cpp:170-181: u_aniso = mu_aniso (.) u
+    u[0] = r.an[0] * dot(d3 { r.P[0], r.P[1], r.P[2] }, rel);
+    u[1] = r.an[1] * dot(d3 { r.P[3], r.P[4], r.P[5] }, rel);
     norm = sqrt(u[0] * u[0] + u[1] * u[1]);
 }
 
@@ -1513,8 +1520,8 @@ __global__ __launch_bounds__(PB) void k_fric_energy(CollView C, const FricRec* R
         int v[4];
         stencil_ids<KIND>(C, i, v);
         double u[2], norm;
-        fric_local<KIND>(R[i], Vel, nV, v, fp.eps_v, R[i].mu_s, R[i].mu_k, u, norm);
-        e = R[i].s * smooth_mu_f0(norm, R[i].mu_s, R[i].mu_k, fp.eps_v);
+        fric_local<KIND>(R[i], Vel, nV, v, fp.eps_v, R[i].ms_eff, R[i].mk_eff, u, norm);
+        e = R[i].s * smooth_mu_f0(norm, R[i].ms_eff, R[i].mk_eff, fp.eps_v);
     }
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
     __shared__ double sh[PB / WAVE];
@@ -1546,14 +1553,78 @@ __global__ __launch_bounds__(PB) void k_fric_gradient_records(CollView C, const 
     int v[4];
     stencil_ids<KIND>(C, i, v);
     double u[2], norm, g[4];
-    fric_local<KIND>(r, Vel, nV, v, fp.eps_v, r.mu_s, r.mu_k, u, norm);
+    fric_local<KIND>(r, Vel, nV, v, fp.eps_v, r.ms_eff, r.mk_eff, u, norm);
     fric_coef<KIND>(r.beta, g);
-    const double f1x = smooth_mu_f1_over_x(norm, r.mu_s, r.mu_k, fp.eps_v);
+    const double f1x = smooth_mu_f1_over_x(norm, r.ms_eff, r.mk_eff, fp.eps_v);
     const double g0 = f1x * u[0] * r.s, g1 = f1x * u[1] * r.s;
 #pragma unroll
     for (int j = 0; j < NP; j++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) o[3 * j + k] = (g[j] * r.P[k]) * g0 + (g[j] * r.P[3 + k]) * g1;
+        for (int k = 0; k < 3; k++)
+            o[3 * j + k] = (g[j] * r.P[k] * r.an[0]) * g0 + (g[j] * r.P[3 + k] * r.an[1]) * g1;
+}
+
+// per-collision anisotropy (n x 2 each, null: keep), effective coefficients
+// reset to mu_s, mu_k (reset_lagged_anisotropic_friction_coefficients)
+__global__ void k_fric_set_aniso(FricRec* R, int64_t n, const double* an, const double* msa,
+                                 const double* mka)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    FricRec& r = R[i];
+    for (int k = 0; k < 2; k++) {
+        if (an) r.an[k] = an[2 * i + k];
+        if (msa) r.msa[k] = msa[2 * i + k];
+        if (mka) r.mka[k] = mka[2 * i + k];
+    }
+    r.ms_eff = r.mu_s;
+    r.mk_eff = r.mu_k;
+}
+
+// update_lagged_anisotropic_friction_coefficients (tangential_collisions.cpp:
+// 19-57) at the build positions: tau = T^T v, tau_aniso = mu_aniso (.) tau,
+// the elliptical model of smooth_mu.cpp:115-171 when mu_s_aniso is nonzero
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_fric_update_lagged(CollView C, FricRec* R, const double* Vel,
+                                                           int nV)
+{
+    constexpr int NP = StencilTraits<KIND>::NP;
+    const int64_t i = (int64_t)xcd_block() * PB + threadIdx.x;
+    if (i >= C.n[KIND]) return;
+    FricRec& r = R[i];
+    r.ms_eff = r.mu_s;
+    r.mk_eff = r.mu_k;
+    if (r.keep == 0 || r.msa[0] * r.msa[0] + r.msa[1] * r.msa[1] == 0) return;
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
+    double g[4];
+    fric_coef<KIND>(r.beta, g);
+    double t0 = 0, t1 = 0;
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        const d3 w = ldx(Vel, nV, v[j]);
+        t0 += g[j] * r.P[0] * w.x;
+        t0 += g[j] * r.P[1] * w.y;
+        t0 += g[j] * r.P[2] * w.z;
+        t1 += g[j] * r.P[3] * w.x;
+        t1 += g[j] * r.P[4] * w.y;
+        t1 += g[j] * r.P[5] * w.z;
+    }
+    const double ta0 = r.an[0] * t0, ta1 = r.an[1] * t1;
+    const bool use_k = r.mka[0] * r.mka[0] + r.mka[1] * r.mka[1] > 0;
+    double d0 = 1.0, d1 = 0.0;
+    const double nrm = sqrt(ta0 * ta0 + ta1 * ta1);
+    if (!(nrm < 1e-10)) {
+        d0 = ta0 / nrm;
+        d1 = ta1 / nrm;
+    }
+    double a = r.msa[0] * d0, b = r.msa[1] * d1;
+    r.ms_eff = sqrt(a * a + b * b);
+    if (use_k) {
+        a = r.mka[0] * d0;
+        b = r.mka[1] * d1;
+        r.mk_eff = sqrt(a * a + b * b);
+    }
 }
 
 // 2 x 2 symmetric PSD projection (eigen decomposition in closed form)
@@ -1605,9 +1676,9 @@ __global__ __launch_bounds__(PB) void k_fric_hessian_records(CollView C, const F
         int v[4];
         stencil_ids<KIND>(C, i, v);
         double u[2], norm, g[4];
-        fric_local<KIND>(r, Vel, nV, v, fp.eps_v, r.mu_s, r.mu_k, u, norm);
+        fric_local<KIND>(r, Vel, nV, v, fp.eps_v, r.ms_eff, r.mk_eff, u, norm);
         fric_coef<KIND>(r.beta, g);
-        const double f1x = smooth_mu_f1_over_x(norm, r.mu_s, r.mu_k, fp.eps_v);
+        const double f1x = smooth_mu_f1_over_x(norm, r.ms_eff, r.mk_eff, fp.eps_v);
         const double scale = r.s;
         if (norm > fp.eps_v) { // sliding: f2 = 0, rank one along u_perp
             if (!(fp.psd != 0 && scale <= 0)) {
@@ -1619,7 +1690,7 @@ __global__ __launch_bounds__(PB) void k_fric_hessian_records(CollView C, const F
         } else if (norm == 0) {
             if (!(fp.psd != 0 && scale <= 0)) m00 = m11 = scale * f1x;
         } else {
-            const double f2 = smooth_mu_f2_x_minus_f1_over_x3(norm, r.mu_s, r.mu_k, fp.eps_v);
+            const double f2 = smooth_mu_f2_x_minus_f1_over_x3(norm, r.ms_eff, r.mk_eff, fp.eps_v);
             m00 = (f2 * u[0] * u[0] + f1x) * scale;
             m01 = (f2 * u[0] * u[1]) * scale;
             m11 = (f2 * u[1] * u[1] + f1x) * scale;
@@ -1633,13 +1704,14 @@ __global__ __launch_bounds__(PB) void k_fric_hessian_records(CollView C, const F
             sq[k] = t;
         }
     }
-    // A = P M P^T
+    // A = P' M P'^T with P' = P diag(mu_aniso) (T_aniso = Gamma^T P')
     double A[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            const double pa0 = r.P[a], pa1 = r.P[3 + a], pb0 = r.P[b], pb1 = r.P[3 + b];
+            const double pa0 = r.P[a] * r.an[0], pa1 = r.P[3 + a] * r.an[1];
+            const double pb0 = r.P[b] * r.an[0], pb1 = r.P[3 + b] * r.an[1];
             A[a][b] = pa0 * (m00 * pb0 + m01 * pb1) + pa1 * (m01 * pb0 + m11 * pb1);
         }
 #pragma unroll
@@ -1814,6 +1886,49 @@ void fric_hessian(Ctx& c, double eps_v, int psd)
     IPCG_CHECK_LAUNCH();
     c.stage_end();
     hess_assemble(c);
+}
+
+
+// per-collision anisotropy from host arrays (2 per collision in global
+// order; null keeps the current values)
+void fric_set_anisotropy(Ctx& c, const double* an, const double* msa, const double* mka)
+{
+    require_collisions(c);
+    if (!c.fric_valid) throw Error(IPCG_ERR_STATE, "no friction set");
+    int64_t n = 0;
+    for (int k = 0; k < 4; k++) n += c.ncoll[k];
+    if (n == 0) return;
+    hipStream_t s = c.stream;
+    c.np_tmp.reserve(sizeof(double) * 6 * (size_t)n);
+    double* d = c.np_tmp.as<double>();
+    const double* src[3] = { an, msa, mka };
+    const double* dev[3] = { nullptr, nullptr, nullptr };
+    for (int k = 0; k < 3; k++)
+        if (src[k]) {
+            IPCG_HIP(hipMemcpyAsync(d + 2 * k * n, src[k], sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+            dev[k] = d + 2 * k * n;
+        }
+    hipLaunchKernelGGL(k_fric_set_aniso, dim3(grid_for(n, 256, INT32_MAX)), dim3(256), 0, s,
+                       c.fric.as<FricRec>(), n, dev[0], dev[1], dev[2]);
+    IPCG_CHECK_LAUNCH();
+    IPCG_HIP(hipStreamSynchronize(s)); // the host arrays may go away
+}
+
+void fric_update_lagged_mu(Ctx& c)
+{
+    require_friction(c);
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    int64_t M;
+    const CollRanges G = ranges(C, M);
+    FricRec* R = c.fric.as<FricRec>();
+    const double* Vel = c.vel.as<double>();
+    if (C.n[0]) hipLaunchKernelGGL(k_fric_update_lagged<K_VV>, dim3(grid_for(C.n[0], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[0], Vel, c.nV);
+    if (C.n[1]) hipLaunchKernelGGL(k_fric_update_lagged<K_EV>, dim3(grid_for(C.n[1], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[1], Vel, c.nV);
+    if (C.n[2]) hipLaunchKernelGGL(k_fric_update_lagged<K_EE>, dim3(grid_for(C.n[2], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[2], Vel, c.nV);
+    if (C.n[3]) hipLaunchKernelGGL(k_fric_update_lagged<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[3], Vel, c.nV);
+    IPCG_CHECK_LAUNCH();
+    IPCG_HIP(hipStreamSynchronize(s));
 }
 
 } // namespace ipcg

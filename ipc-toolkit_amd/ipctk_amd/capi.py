@@ -40,6 +40,7 @@ EXPORTS = [
     "ipcg_collisions_set_area_weighting", "ipcg_collisions_set_type",
     "ipcg_friction_build", "ipcg_friction_build_per_vertex", "ipcg_friction_download", "ipcg_velocities_upload",
     "ipcg_friction_energy", "ipcg_friction_gradient", "ipcg_friction_hessian",
+    "ipcg_friction_set_anisotropy", "ipcg_friction_update_lagged_mu",
 ]
 
 _lib = None
@@ -109,6 +110,8 @@ def load(path=LIB_PATH):
     L.ipcg_friction_build.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
     L.ipcg_friction_download.argtypes = [vp, _I, _D, _D, _D]
     L.ipcg_friction_build_per_vertex.argtypes = [vp, C.c_double, C.c_double, _D, _D]
+    L.ipcg_friction_set_anisotropy.argtypes = [vp, _D, _D, _D]
+    L.ipcg_friction_update_lagged_mu.argtypes = [vp]
     L.ipcg_velocities_upload.argtypes = [vp, _D, C.c_int]
     L.ipcg_friction_energy.argtypes = [vp, C.c_double, _D]
     L.ipcg_friction_gradient.argtypes = [vp, C.c_double, _D]
@@ -370,6 +373,22 @@ class Context:
             self._p, keep.ctypes.data_as(_I), beta.ctypes.data_as(_D),
             basis.ctypes.data_as(_D), force.ctypes.data_as(_D)))
         return keep.astype(bool), beta, basis.reshape(n, 2, 3).transpose(0, 2, 1), force
+
+    def friction_set_anisotropy(self, mu_aniso=None, mu_s_aniso=None, mu_k_aniso=None):
+        """per-collision (n, 2) arrays in collision order; None keeps"""
+        keep = []
+
+        def arg(a):
+            if a is None:
+                return None
+            a, p = _d(np.asarray(a, float).reshape(-1, 2))
+            keep.append(a)
+            return p
+        self._check(self._L.ipcg_friction_set_anisotropy(self._p, arg(mu_aniso), arg(mu_s_aniso),
+                                                         arg(mu_k_aniso)))
+
+    def friction_update_lagged_mu(self):
+        self._check(self._L.ipcg_friction_update_lagged_mu(self._p))
 
     def upload_velocities(self, V):
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)

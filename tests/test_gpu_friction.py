@@ -83,6 +83,37 @@ def test_per_vertex_coefficients(ctx, O, scenes, name):
     assert sparse_rel_err(ctx.friction_hessian(EPS_V, "clamp"), H_o) <= 1e-10
 
 
+@pytest.mark.parametrize("name", ["twisted:12:1", "stack:32:0"])
+def test_anisotropic(ctx, O, scenes, name):
+    s = make(scenes, name)
+    dhat = 3 * s.dhat
+    say("case anisotropic", name)
+    upload(ctx, s)
+    ctx.build_collisions(dhat)
+    ee, fv = ctx.candidates()
+    vv, ev = ctx.codim_candidates()
+    co = O.build_collisions(s.V, s.E, s.F, ee, fv, dhat, vv_cand=vv, ev_cand=ev)
+    ctx.friction_build(dhat, s.kappa, 0.6, 0.3)
+    fr = O.friction_build(co, s.V, s.E, s.F, dhat, s.kappa, 0.6, 0.3)
+    n = len(fr.keep)
+    rng = np.random.default_rng(8)
+    a = rng.uniform(0.5, 2.0, (n, 2))
+    msa, mka = rng.uniform(0.2, 0.8, (n, 2)), rng.uniform(0.1, 0.5, (n, 2))
+    msa[::3] = 0  # some collisions stay isotropic
+    ctx.friction_set_anisotropy(a, msa, mka)
+    O.friction_set_anisotropy(fr, a, msa, mka)
+    Vel = rng.normal(0, 2 * EPS_V, s.V.shape)
+    ctx.upload_velocities(Vel)
+    ctx.friction_update_lagged_mu()
+    O.friction_update_lagged_mu(fr, Vel)
+    e_o = O.friction_energy(fr, Vel, EPS_V)
+    assert abs(ctx.friction_energy(EPS_V) - e_o) <= 1e-12 * abs(e_o)
+    assert rel_err(ctx.friction_gradient(EPS_V), O.friction_gradient(fr, Vel, EPS_V)) <= 1e-12
+    for psd in ("clamp", "none"):
+        H_o = O.friction_hessian(fr, Vel, EPS_V, psd)
+        assert sparse_rel_err(ctx.friction_hessian(EPS_V, psd), H_o) <= 1e-10, psd
+
+
 def test_requires_a_current_set(ctx, capi, scenes):
     s = make(scenes, "sheets:16:0")
     upload(ctx, s)
