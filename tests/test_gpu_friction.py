@@ -114,6 +114,38 @@ def test_anisotropic(ctx, O, scenes, name):
         assert sparse_rel_err(ctx.friction_hessian(EPS_V, psd), H_o) <= 1e-10, psd
 
 
+def test_signed_weights(ctx, capi, O, scenes):
+    """friction over an IMPROVED_MAX_APPROX set: negative weights give
+    negative scales, which the projected Hessian drops (sliding and zero
+    velocity) or projects out (static)"""
+    s = make(scenes, "sheets:16:0")
+    dhat = 3 * s.dhat
+    say("case signed weights")
+    upload(ctx, s)
+    ctx.set_area_weighting(True)
+    ctx.set_collision_set_type(capi.COLLISIONS_IMPROVED_MAX_APPROX)
+    ctx.build_collisions(dhat)
+    ee, fv = ctx.candidates()
+    vv, ev = ctx.codim_candidates()
+    co = O.build_collisions(s.V, s.E, s.F, ee, fv, dhat, vv_cand=vv, ev_cand=ev,
+                            area_weighting=True, improved=True)
+    assert (co.ev_w < 0).any() or (co.ee_w < 0).any()
+    ctx.friction_build(dhat, s.kappa, 0.5)
+    fr = O.friction_build(co, s.V, s.E, s.F, dhat, s.kappa, 0.5)
+    rng = np.random.default_rng(9)
+    for scale in (0.0, 0.3 * EPS_V, 20 * EPS_V):
+        Vel = rng.normal(0, scale, s.V.shape) if scale else np.zeros_like(s.V)
+        ctx.upload_velocities(Vel)
+        e_o = O.friction_energy(fr, Vel, EPS_V)
+        assert abs(ctx.friction_energy(EPS_V) - e_o) <= 1e-12 * abs(e_o)
+        g_o = O.friction_gradient(fr, Vel, EPS_V)
+        if scale:
+            assert rel_err(ctx.friction_gradient(EPS_V), g_o) <= 1e-12
+        for psd in ("clamp", "abs", "none"):
+            H_o = O.friction_hessian(fr, Vel, EPS_V, psd)
+            assert sparse_rel_err(ctx.friction_hessian(EPS_V, psd), H_o) <= 1e-10, (scale, psd)
+
+
 def test_requires_a_current_set(ctx, capi, scenes):
     s = make(scenes, "sheets:16:0")
     upload(ctx, s)
