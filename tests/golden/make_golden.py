@@ -1,4 +1,5 @@
-"""Writes the regression fixtures of tests/test_golden.py.
+"""Writes the regression fixtures of tests/test_golden.py (and, under
+variants/, those of tests/test_golden_variants.py).
 
 Each fixture holds a small seeded scene (inputs) and the CPU oracle's
 outputs for the five calls of the contact step.  Regenerate with
@@ -47,8 +48,38 @@ def fixture(name, s, per_candidate_toi):
     print(name, s.sizes, "collisions", len(c), "nnz", H.nnz, "swept", len(see) + len(sfv))
 
 
+def variants(name, s, mult=3.0):
+    """collision-set variants and friction on one scene (tests/golden/variants/)"""
+    dhat = mult * s.dhat
+    ee, fv = O.candidates(s.V, s.E, s.F, 0.5 * dhat)
+    out = dict(V=s.V, E=s.E, F=s.F, dhat=dhat, kappa=s.kappa)
+    for tag, kw in (("imp", dict(area_weighting=True, improved=True)), ("ogc", dict(ogc=True))):
+        c = O.build_collisions(s.V, s.E, s.F, ee, fv, dhat, **kw)
+        for k in ("vv", "ev", "ee", "fv", "vv_w", "ev_w", "ee_w", "fv_w", "ee_eps_x", "ee_dtype"):
+            out[f"{tag}_{k}"] = getattr(c, k)
+    c = O.build_collisions(s.V, s.E, s.F, ee, fv, dhat)
+    fr = O.friction_build(c, s.V, s.E, s.F, dhat, s.kappa, 0.6, 0.3)
+    rng = np.random.default_rng(11)
+    n = len(fr.keep)
+    aniso = dict(mu_aniso=rng.uniform(0.5, 2.0, (n, 2)), mu_s_aniso=rng.uniform(0.2, 0.8, (n, 2)),
+                 mu_k_aniso=rng.uniform(0.1, 0.5, (n, 2)))
+    Vel = rng.normal(0, 2e-3, s.V.shape)
+    O.friction_set_anisotropy(fr, **aniso)
+    O.friction_update_lagged_mu(fr, Vel)
+    H = O.friction_hessian(fr, Vel, 1e-3, "clamp")
+    out.update(Vel=Vel, eps_v=1e-3, **aniso,
+               fric_energy=O.friction_energy(fr, Vel, 1e-3),
+               fric_gradient=O.friction_gradient(fr, Vel, 1e-3),
+               fric_H_indptr=H.indptr.astype(np.int64), fric_H_indices=H.indices.astype(np.int32),
+               fric_H_data=H.data)
+    os.makedirs(os.path.join(HERE, "variants"), exist_ok=True)
+    np.savez_compressed(os.path.join(HERE, "variants", name + ".npz"), **out)
+    print("variants", name, "improved", len(out["imp_ee"]), "ogc", len(out["ogc_fv"]), "nnz", H.nnz)
+
+
 if __name__ == "__main__":
     O.build()
+    variants("twisted_sheets_8_seed0", scenes.two_sheets(8, seed=0, twist=0.3))
     fixture("twisted_sheets_8_seed0", scenes.two_sheets(8, seed=0, twist=0.3), False)
     fixture("sphere_chain_4x3_seed0", scenes.sphere_chain(4, 3, seed=0), True)
     fixture("stacked_sheets_4x12_seed2", scenes.stacked_sheets(12, 4, seed=2, h=1.0 / 511), True)
