@@ -522,12 +522,19 @@ struct SparseMatrix {
 // potentials/barrier_potential.hpp (use_physical_barrier = false)
 class BarrierPotential {
 public:
-    BarrierPotential(double dhat, double stiffness = 1.0) : m_dhat(dhat), m_stiffness(stiffness)
+    BarrierPotential(double dhat, double stiffness = 1.0, bool use_physical_barrier = false)
+        : m_dhat(dhat), m_stiffness(stiffness), m_physical(use_physical_barrier)
     {
         if (!(dhat > 0)) throw std::invalid_argument("dhat must be positive");
     }
     double dhat() const { return m_dhat; }
     double stiffness() const { return m_stiffness; }
+    bool use_physical_barrier() const { return m_physical; }
+    // the device flag follows this potential for the calls it makes
+    void apply(const CollisionMesh& mesh) const
+    {
+        detail::check(mesh.context(), ipcg_barrier_set_physical(mesh.context(), m_physical ? 1 : 0));
+    }
 
     // potential.cpp:36-57 (over the collision set at `vertices`)
     double operator()(const NormalCollisions& collisions, const CollisionMesh& mesh,
@@ -595,15 +602,17 @@ public:
 
 private:
     double m_dhat, m_stiffness;
+    bool m_physical;
     // the device evaluates the collision set it holds at slot-0 positions
-    static void prepare(const NormalCollisions& collisions, const CollisionMesh& mesh,
-                        const MatrixXd& vertices)
+    void prepare(const NormalCollisions& collisions, const CollisionMesh& mesh,
+                 const MatrixXd& vertices) const
     {
         if (collisions.mesh() != &mesh)
             throw std::runtime_error("collisions were not built for this mesh");
         if (!collisions.resident())
             throw std::runtime_error("a later collision build replaced this set on the device");
         mesh.set_positions(0, vertices);
+        apply(mesh);
     }
 };
 
@@ -617,6 +626,7 @@ public:
                double mu_s, double mu_k)
     {
         begin(mesh, vertices, collisions);
+        normal_potential.apply(mesh);
         detail::check(mesh.context(), ipcg_friction_build(mesh.context(), normal_potential.dhat(),
                                                           normal_potential.stiffness(), mu_s, mu_k));
         finish(mesh, collisions);
@@ -635,6 +645,7 @@ public:
         if (mu_s.size() != (size_t)mesh.num_vertices() || mu_k.size() != mu_s.size())
             throw std::invalid_argument("one friction coefficient per vertex");
         begin(mesh, vertices, collisions);
+        normal_potential.apply(mesh);
         detail::check(mesh.context(),
                       ipcg_friction_build_per_vertex(mesh.context(), normal_potential.dhat(),
                                                      normal_potential.stiffness(), mu_s.data(),

@@ -189,6 +189,12 @@ int ipcg_collisions_download(ipcg_ctx* ctx, int32_t* vv, double* vv_w,
 
 /* ---- barrier potential (potential.cpp:36-222, barrier_potential.cpp) ----
  * Over the current collision set at slot-0 positions. */
+/* BarrierPotential use_physical_barrier (barrier_potential.cpp:33-99) for
+ * the calls that follow (energy, gradient, Hessian, Gauss-Newton helpers,
+ * the friction build's normal forces): the barrier and its derivatives
+ * scaled by dhat / ((2 dmin + dhat) dhat)^2 (barrier.hpp:133-137).
+ * Default off. */
+int ipcg_barrier_set_physical(ipcg_ctx* ctx, int use_physical_barrier);
 int ipcg_barrier_energy(ipcg_ctx* ctx, double dhat, double kappa, double* out);
 /* grad: host array of 3*nV doubles (RowMajor), or NULL to keep it on device */
 int ipcg_barrier_gradient(ipcg_ctx* ctx, double dhat, double kappa, double* grad);

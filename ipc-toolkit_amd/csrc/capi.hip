@@ -576,6 +576,11 @@ int ipcg_collisions_set_type(ipcg_ctx* ctx, int collision_set_type)
     });
 }
 
+int ipcg_barrier_set_physical(ipcg_ctx* ctx, int use_physical_barrier)
+{
+    return guard(ctx, [&](Ctx& c) { c.physical_barrier = use_physical_barrier != 0; });
+}
+
 int ipcg_collisions_build_from_candidates(ipcg_ctx* ctx, double dhat, double dmin)
 {
     return guard(ctx, [&](Ctx& c) {

@@ -55,6 +55,7 @@ struct Ctx {
     DevBuf v_area, e_area;     // CollisionMesh vertex / edge areas (rest positions)
     bool area_weighting = false; // NormalCollisions::use_area_weighting
     int coll_type = 0; // NormalCollisions::CollisionSetType: 0 IPC, 1 IMPROVED_MAX_APPROX, 2 OGC
+    bool physical_barrier = false; // BarrierPotential use_physical_barrier
     // CollisionMesh adjacencies (collision_mesh.cpp:247-291), CSR with int32
     // offsets, rows sorted and duplicate-free: vertex -> vertices sharing an
     // edge, vertex -> incident edges, edge -> vertices opposite it in its

@@ -825,6 +825,16 @@ CollView view(Ctx& c)
     return C;
 }
 
+BarrierParams barrier_params(const Ctx& c, double dhat, double kappa, int psd)
+{
+    BarrierParams bp { dhat, kappa, c.coll_dmin, psd };
+    if (c.physical_barrier) {
+        const double t = (2 * c.coll_dmin + dhat) * dhat;
+        bp.phys = dhat / (t * t);
+    }
+    return bp;
+}
+
 void require_collisions(Ctx& c)
 {
     c.require_mesh();
@@ -839,7 +849,7 @@ double pot_energy(Ctx& c, double dhat, double kappa)
     require_collisions(c);
     hipStream_t s = c.stream;
     CollView C = view(c);
-    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
     int64_t nb[4], tot = 0;
     for (int k = 0; k < 4; k++) {
         nb[k] = (C.n[k] + PB - 1) / PB;
@@ -928,7 +938,7 @@ void pot_gradient(Ctx& c, double dhat, double kappa)
     require_collisions(c);
     hipStream_t s = c.stream;
     CollView C = view(c);
-    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
     int64_t M;
     const CollRanges G = ranges(C, M);
     require_incidences(c, C, G, M);
@@ -1082,7 +1092,7 @@ void pot_hessian(Ctx& c, double dhat, double kappa, int psd)
     if (psd < 0 || psd > 2) throw Error(IPCG_ERR_ARG, "invalid PSD projection method");
     hipStream_t s = c.stream;
     CollView C = view(c);
-    BarrierParams bp { dhat, kappa, c.coll_dmin, psd };
+    const BarrierParams bp = barrier_params(c, dhat, kappa, psd);
     const double* X = c.X[0].as<double>();
     const CollRanges G = hess_prepare(c, C);
     // projected Hessian records, REC_LEN values per collision in global
@@ -1188,7 +1198,7 @@ void pot_gn_diagonal(Ctx& c, double dhat, double kappa)
     require_collisions(c);
     hipStream_t s = c.stream;
     CollView C = view(c);
-    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
     int64_t M;
     const CollRanges G = ranges(C, M);
     require_incidences(c, C, G, M);
@@ -1223,7 +1233,7 @@ double pot_gn_quadratic_form(Ctx& c, double dhat, double kappa, const double* p_
     require_collisions(c);
     hipStream_t s = c.stream;
     CollView C = view(c);
-    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
     const size_t pb = sizeof(double) * 3 * (size_t)std::max(c.nV, 1);
     c.gn_p.reserve(pb);
     if (c.nV)
@@ -1742,7 +1752,7 @@ void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k, con
     const CollRanges G = ranges(C, M);
     c.fric.reserve(sizeof(FricRec) * (size_t)std::max<int64_t>(G.goff[4], 1));
     FricRec* R = c.fric.as<FricRec>();
-    BarrierParams bp { dhat, kappa, c.coll_dmin, 0 };
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
     const double* X = c.X[0].as<double>();
     const double *msv = nullptr, *mkv = nullptr;
     if (mu_sv && mu_kv && c.nV) {

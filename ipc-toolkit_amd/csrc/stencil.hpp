@@ -16,19 +16,23 @@ enum StencilKind : int { K_VV = 0, K_EV = 1, K_EE = 2, K_FV = 3 };
 struct BarrierParams {
     double dhat, kappa, dmin;
     int psd;
+    // use_physical_barrier: b, b' and b'' scaled by dhat / units(dhat') with
+    // units = dhat'^2 for the clamped log barrier (barrier_potential.cpp:
+    // 62-99, barrier.hpp:133-137); 1 otherwise (an exact product)
+    double phys = 1.0;
 };
 
 IPCG_HD double bar_f(double d2, const BarrierParams& b)
 {
-    return b.kappa * barrier_b(d2 - b.dmin * b.dmin, (2 * b.dmin + b.dhat) * b.dhat);
+    return b.kappa * (barrier_b(d2 - b.dmin * b.dmin, (2 * b.dmin + b.dhat) * b.dhat) * b.phys);
 }
 IPCG_HD double bar_f1(double d2, const BarrierParams& b)
 {
-    return b.kappa * barrier_db(d2 - b.dmin * b.dmin, (2 * b.dmin + b.dhat) * b.dhat);
+    return b.kappa * (barrier_db(d2 - b.dmin * b.dmin, (2 * b.dmin + b.dhat) * b.dhat) * b.phys);
 }
 IPCG_HD double bar_f2(double d2, const BarrierParams& b)
 {
-    return b.kappa * barrier_d2b(d2 - b.dmin * b.dmin, (2 * b.dmin + b.dhat) * b.dhat);
+    return b.kappa * (barrier_d2b(d2 - b.dmin * b.dmin, (2 * b.dmin + b.dhat) * b.dhat) * b.phys);
 }
 
 // Squared distance of a stencil with its stored type (bit-exact path).

@@ -40,7 +40,7 @@ EXPORTS = [
     "ipcg_collisions_set_area_weighting", "ipcg_collisions_set_type",
     "ipcg_friction_build", "ipcg_friction_build_per_vertex", "ipcg_friction_download", "ipcg_velocities_upload",
     "ipcg_friction_energy", "ipcg_friction_gradient", "ipcg_friction_hessian",
-    "ipcg_friction_set_anisotropy", "ipcg_friction_update_lagged_mu",
+    "ipcg_friction_set_anisotropy", "ipcg_friction_update_lagged_mu", "ipcg_barrier_set_physical",
 ]
 
 _lib = None
@@ -112,6 +112,7 @@ def load(path=LIB_PATH):
     L.ipcg_friction_build_per_vertex.argtypes = [vp, C.c_double, C.c_double, _D, _D]
     L.ipcg_friction_set_anisotropy.argtypes = [vp, _D, _D, _D]
     L.ipcg_friction_update_lagged_mu.argtypes = [vp]
+    L.ipcg_barrier_set_physical.argtypes = [vp, C.c_int]
     L.ipcg_velocities_upload.argtypes = [vp, _D, C.c_int]
     L.ipcg_friction_energy.argtypes = [vp, C.c_double, _D]
     L.ipcg_friction_gradient.argtypes = [vp, C.c_double, _D]
@@ -278,6 +279,10 @@ class Context:
         COLLISIONS_IMPROVED_MAX_APPROX or COLLISIONS_OGC, for the builds that
         follow"""
         self._check(self._L.ipcg_collisions_set_type(self._p, int(collision_set_type)))
+
+    def set_physical_barrier(self, use=True):
+        """BarrierPotential use_physical_barrier for the calls that follow"""
+        self._check(self._L.ipcg_barrier_set_physical(self._p, int(bool(use))))
 
     def build_collisions_from_candidates(self, dhat, dmin=0.0):
         self._check(self._L.ipcg_collisions_build_from_candidates(self._p, dhat, dmin))
