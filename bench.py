@@ -65,9 +65,12 @@ def parse():
     p.add_argument("--n", type=int, default=512, help="vertices per sheet side")
     p.add_argument("--sheets", type=int, default=8)
     p.add_argument("--seed", type=int, default=0)
-    p.add_argument("--cpu-tile", type=int, default=96,
+    p.add_argument("--cpu-tile", type=int, default=64,
                    help="sheet side of the CPU-baseline tile (0: skip)")
-    p.add_argument("--cpu-steps", type=int, default=3)
+    p.add_argument("--cpu-steps", type=int, default=10,
+                   help="timed CPU-baseline steps on all host cores (after 2 warm-ups)")
+    p.add_argument("--cpu-steps-1core", type=int, default=1,
+                   help="timed CPU-baseline steps on one core")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB MiB device buffer five times (torch), the known-size "
                         "reference for scaling the HBM byte counters of a profiler")
@@ -101,28 +104,99 @@ def algorithmic_bytes(sc, counts, n_static, n_swept, nnz):
             + 24 * nV + 12 * nnz + 4 * (3 * nV + 1) + 8)
 
 
-def cpu_baseline(args, scenes):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_step(O, tile):
+    """The five calls on the CPU restatement; returns (pairs, results)."""
+    ee, fv = O.candidates(tile.V, tile.E, tile.F, 0.5 * tile.dhat)
+    coll = O.build_collisions(tile.V, tile.E, tile.F, ee, fv, tile.dhat)
+    e = O.barrier_energy(coll, tile.V, tile.E, tile.F, tile.dhat, tile.kappa)
+    g = O.barrier_gradient(coll, tile.V, tile.E, tile.F, tile.dhat, tile.kappa)
+    H = O.barrier_hessian(coll, tile.V, tile.E, tile.F, tile.dhat, tile.kappa, "clamp")
+    # the reference's own schedule: running-minimum tmax (candidates.cpp:269-287)
+    t = O.compute_collision_free_stepsize(tile.V, tile.V1, tile.E, tile.F, running_tmax=True)
+    return len(coll), (coll, e, g, H, t)
+
+
+def tile_parity(O, capi, tile, ref):
+    """The device's results on the CPU tile against the oracle's (the
+    checker): set equality and the relative error of every quantity."""
+    coll, e_o, g_o, H_o, _ = ref
+    ctx = capi.Context(0)
+    try:
+        ctx.upload_mesh(tile.V, tile.E, tile.F)
+        ctx.upload_positions(0, tile.V)
+        ctx.upload_positions(1, tile.V1)
+        ctx.build_collisions(tile.dhat)
+        cg = ctx.collisions()
+        same = all(np.array_equal(cg[k], getattr(coll, k)) and
+                   np.array_equal(cg[k + "_w"], getattr(coll, k + "_w"))
+                   for k in ("vv", "ev", "ee", "fv"))
+        same = same and np.array_equal(cg["ee_eps_x"], coll.ee_eps_x) and \
+            np.array_equal(cg["ee_dtype"], coll.ee_dtype)
+        e_g = ctx.energy(tile.dhat, tile.kappa)
+        g_g = ctx.gradient(tile.dhat, tile.kappa)
+        H_g = ctx.hessian(tile.dhat, tile.kappa, "clamp")
+        t_g = ctx.stepsize()
+    finally:
+        ctx.close()
+    D = H_g - H_o
+    t_all = O.compute_collision_free_stepsize(tile.V, tile.V1, tile.E, tile.F)
+    return {"collision_set_identical": bool(same),
+            "energy_rel_err": abs(e_g - e_o) / abs(e_o),
+            "gradient_rel_err": float(np.linalg.norm(g_g - g_o) / np.linalg.norm(g_o)),
+            "hessian_rel_err": float(np.sqrt(D.multiply(D).sum()) /
+                                     np.sqrt(H_o.multiply(H_o).sum())),
+            "step_equal_to_oracle": bool(t_g == t_all)}
+
+
+def cpu_baseline(args, scenes, capi):
+    """The oracle on a bounded tile of the workload: all host cores (median
+    of --cpu-steps after two warm-ups) and one core (fewer steps)."""
     import oracle as O
     tile = scenes.stacked_sheets(args.cpu_tile, args.sheets, seed=args.seed,
                                  h=1.0 / (args.n - 1))
     threads = O.get_max_threads()
-    times, pairs = [], 0
-    for _ in range(max(1, args.cpu_steps)):
-        t0 = time.perf_counter()
-        ee, fv = O.candidates(tile.V, tile.E, tile.F, 0.5 * tile.dhat)
-        coll = O.build_collisions(tile.V, tile.E, tile.F, ee, fv, tile.dhat)
-        O.barrier_energy(coll, tile.V, tile.E, tile.F, tile.dhat, tile.kappa)
-        O.barrier_gradient(coll, tile.V, tile.E, tile.F, tile.dhat, tile.kappa)
-        O.barrier_hessian(coll, tile.V, tile.E, tile.F, tile.dhat, tile.kappa, "clamp")
-        # the reference's own schedule: running-minimum tmax (candidates.cpp:269-287)
-        O.compute_collision_free_stepsize(tile.V, tile.V1, tile.E, tile.F, running_tmax=True)
-        times.append(time.perf_counter() - t0)
-        pairs = len(coll)
-    t = float(np.median(times))
-    return {"value": pairs / t, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{args.sheets} stacked {args.cpu_tile}x{args.cpu_tile} tile of the "
-                      f"workload (same spacing), {pairs} collision pairs, median of "
-                      f"{len(times)} steps, {t * 1e3:.1f} ms/step"}
+
+    def timed(steps, warm):
+        ts, pairs, res = [], 0, None
+        for i in range(warm + steps):
+            t0 = time.perf_counter()
+            pairs, res = cpu_step(O, tile)
+            if i >= warm:
+                ts.append(time.perf_counter() - t0)
+        return pairs, res, ts
+
+    pairs, res, ts = timed(max(1, args.cpu_steps), 2)
+    t, t99 = float(np.median(ts)), float(np.percentile(ts, 99))
+    O.set_num_threads(1)
+    try:
+        _, _, ts1 = timed(max(1, args.cpu_steps_1core), 0)
+    finally:
+        O.set_num_threads(threads)
+    t1 = float(np.median(ts1))
+    out = {"value": pairs / t, "unit": "pairs/s", "cores": threads, "kind": "port",
+           "cpu": cpu_model(), "nproc": os.cpu_count(),
+           "p50_ms": round(1e3 * t, 1), "p99_ms": round(1e3 * t99, 1),
+           "value_1core": pairs / t1, "ms_1core": round(1e3 * t1, 1),
+           "sample": f"{args.sheets} stacked {args.cpu_tile}x{args.cpu_tile} tile of the "
+                     f"workload (same spacing), {pairs} collision pairs; {len(ts)} timed steps "
+                     f"on {threads} threads after 2 warm-ups, {len(ts1)} on one thread; "
+                     f"the oracle (C, OpenMP, -O3 -ffp-contract=off), not a reference build"}
+    try:
+        out["device_vs_oracle"] = tile_parity(O, capi, tile, res)
+    except Exception as e:  # the check never blocks the line
+        out["device_vs_oracle"] = {"error": str(e)}
+    return out
 
 
 def copy_calibration(mib):
@@ -283,7 +357,7 @@ def main():
         t = per_launch[k] * 1e-3
         gbs, tfs = byt / t / 1e9, flo / t / 1e12
         if flo and tfs / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS:
-            return {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS,
+            return {"bound": "fp64_valu", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(tfs / FP64_PEAK_TFLOPS, 5),
                     "flops_per_launch": int(flo)}
         return {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -304,9 +378,9 @@ def main():
         if t.get("kernel") == dom:
             roof_dom["traffic"] = int(t["traffic_bytes_per_launch"])
             roof_dom["traffic_source"] = os.path.relpath(tpath, ROOT)
-    if roof_dom["bound"] == "mfma":
-        roof_dom["note"] = ("FP64 on the vector ALUs; the MI355X FP64 vector and "
-                            "matrix peaks are both 78.6 TFLOP/s")
+    if roof_dom["bound"] == "fp64_valu":
+        roof_dom["note"] = ("FP64 on the vector ALUs (no step is a dense contraction, so "
+                            "nothing runs on MFMA); the MI355X FP64 vector peak is 78.6 TFLOP/s")
     roof_dom["others"] = {k: {"bound": r["bound"], "frac": r["frac"]}
                           for k in sorted(known, key=per_step.get, reverse=True)[1:6]
                           for r in [roof(k)]}
@@ -332,7 +406,7 @@ def main():
     cpu = None
     if args.cpu_tile > 0:
         try:
-            cpu = cpu_baseline(args, scenes)
+            cpu = cpu_baseline(args, scenes, capi)
         except Exception as e:  # the baseline never blocks the GPU line
             cpu = {"value": None, "error": str(e)}
 

@@ -30,11 +30,18 @@ struct Error : std::runtime_error {
 
 constexpr int WAVE = 64;
 
-inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 1 << 20)
+// Workgroups of `block` threads for n items.  By default the grid covers
+// every item (one thread each); kernels that loop over their input with a
+// grid stride pass their own cap.  A launch beyond the 2^32 work-items of
+// one grid dimension is an error, never a silently truncated grid.
+inline unsigned grid_for(int64_t n, int block, int64_t max_blocks = 0)
 {
     int64_t g = (n + block - 1) / block;
     if (g < 1) g = 1;
-    if (g > max_blocks) g = max_blocks;
+    if (max_blocks > 0 && g > max_blocks) g = max_blocks;
+    if (g * (int64_t)block > (int64_t)UINT32_MAX)
+        throw Error(IPCG_ERR_ARG, "a launch over " + std::to_string(n)
+                                      + " items exceeds the 2^32 work-items of a grid");
     return (unsigned)g;
 }
 

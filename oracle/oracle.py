@@ -577,6 +577,46 @@ def build_collisions(V, E, F, ee_cand, fv_cand, dhat, dmin=0.0, V_rest=None, vv_
     return Collisions(out, (E, F))
 
 
+class CollisionArrays:
+    """A collision set given by explicit arrays (canonical order and layout
+    of Collisions), e.g. a subset of another set, for the potentials.  The
+    struct points into numpy arrays that this object keeps alive."""
+
+    def __init__(self, vv, vv_w, ev, ev_w, ee, ee_w, ee_eps_x, ee_dtype, fv, fv_w, dmin=0.0):
+        def ids(a):
+            return np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1, 2))
+
+        def vals(a, dt=np.float64):
+            return np.ascontiguousarray(np.asarray(a, dtype=dt).reshape(-1))
+
+        self.vv, self.vv_w = ids(vv), vals(vv_w)
+        self.ev, self.ev_w = ids(ev), vals(ev_w)
+        self.ee, self.ee_w = ids(ee), vals(ee_w)
+        self.ee_eps_x, self.ee_dtype = vals(ee_eps_x), vals(ee_dtype, np.int32)
+        self.fv, self.fv_w = ids(fv), vals(fv_w)
+        s = _Collisions()
+        s.nvv, s.nev, s.nee, s.nfv = len(self.vv), len(self.ev), len(self.ee), len(self.fv)
+        s.vv, s.vv_w = self.vv.ctypes.data_as(_I), self.vv_w.ctypes.data_as(_D)
+        s.ev, s.ev_w = self.ev.ctypes.data_as(_I), self.ev_w.ctypes.data_as(_D)
+        s.ee, s.ee_w = self.ee.ctypes.data_as(_I), self.ee_w.ctypes.data_as(_D)
+        s.ee_eps_x = self.ee_eps_x.ctypes.data_as(_D)
+        s.ee_dtype = self.ee_dtype.ctypes.data_as(_I)
+        s.fv, s.fv_w = self.fv.ctypes.data_as(_I), self.fv_w.ctypes.data_as(_D)
+        s.dmin = dmin
+        self._s = s
+
+    def __len__(self):
+        return len(self.vv) + len(self.ev) + len(self.ee) + len(self.fv)
+
+
+def collisions_subset(coll, keep_vv, keep_ev, keep_ee, keep_fv, dmin=0.0):
+    """The collisions of `coll` selected by one boolean mask per kind."""
+    return CollisionArrays(coll.vv[keep_vv], coll.vv_w[keep_vv], coll.ev[keep_ev],
+                           coll.ev_w[keep_ev], coll.ee[keep_ee], coll.ee_w[keep_ee],
+                           coll.ee_eps_x[keep_ee], coll.ee_dtype[keep_ee], coll.fv[keep_fv],
+                           coll.fv_w[keep_fv], dmin)
+
+
 def barrier_energy(coll, V, E, F, dhat, kappa):
     V, pV = _d(V)
     E, pE = _i(E)
