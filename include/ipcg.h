@@ -196,6 +196,19 @@ int ipcg_collisions_download(ipcg_ctx* ctx, int32_t* vv, double* vv_w,
  * Default off. */
 int ipcg_barrier_set_physical(ipcg_ctx* ctx, int use_physical_barrier);
 int ipcg_barrier_energy(ipcg_ctx* ctx, double dhat, double kappa, double* out);
+/* Allocation-free variant of ipcg_barrier_energy.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops).  Same kernels and the same value as
+ * ipcg_barrier_energy, but the block partial sums go to a file-scope device
+ * array of 2^20 doubles (8 MiB on every GPU the library runs on, held until
+ * the process ends) and the energy is written to energy_device (caller-owned
+ * device memory) on the context stream, without synchronizing: it is there
+ * once the stream reaches it (ipcg_ctx_synchronize).  Limit: the collision
+ * set may span at most 2^20 blocks of 128 collisions per kind
+ * (IPCG_ERR_ARG beyond).  Calls that overlap on one GPU share the array
+ * and race. */
+int ipcg_barrier_energy_noalloc(ipcg_ctx* ctx, double dhat, double kappa,
+                                double* energy_device);
 /* grad: host array of 3*nV doubles (RowMajor), or NULL to keep it on device */
 int ipcg_barrier_gradient(ipcg_ctx* ctx, double dhat, double kappa, double* grad);
 /* assembles the CSR on device; *nnz receives the entry count */

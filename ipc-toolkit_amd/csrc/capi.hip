@@ -136,6 +136,7 @@ int ipcg_ctx_create(int device, ipcg_ctx** out)
         c.device = device;
         IPCG_HIP(hipSetDevice(device));
         IPCG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        noalloc_init(device); // device address of the allocation-free energy's scratch
         IPCG_HIP(hipHostMalloc(&c.pinned, PINNED_BYTES, hipHostMallocDefault));
         c.counters.reserve(sizeof(unsigned long long) * 16);
     });
@@ -625,6 +626,11 @@ int ipcg_barrier_energy(ipcg_ctx* ctx, double dhat, double kappa, double* out)
         if (!out) throw Error(IPCG_ERR_ARG, "null output");
         *out = pot_energy(c, dhat, kappa);
     });
+}
+
+int ipcg_barrier_energy_noalloc(ipcg_ctx* ctx, double dhat, double kappa, double* energy_device)
+{
+    return guard(ctx, [&](Ctx& c) { pot_energy_noalloc(c, dhat, kappa, energy_device); });
 }
 
 int ipcg_barrier_gradient(ipcg_ctx* ctx, double dhat, double kappa, double* grad)

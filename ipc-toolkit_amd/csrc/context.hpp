@@ -144,6 +144,12 @@ struct Ctx {
 void bp_build(Ctx& c, bool swept, double inflation, int method);
 void np_build(Ctx& c, double dhat, double dmin);
 double pot_energy(Ctx& c, double dhat, double kappa);
+// allocation-free energy: block partials in a per-device static array of
+// NOALLOC_PARTIALS doubles; enqueues on the context stream and writes the
+// energy to device memory (no synchronization)
+constexpr int64_t NOALLOC_PARTIALS = 1 << 20;
+void noalloc_init(int device);
+void pot_energy_noalloc(Ctx& c, double dhat, double kappa, double* d_out);
 void pot_gradient(Ctx& c, double dhat, double kappa);
 void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
 // toi_out: null (step size) or 4 host arrays (ee, fv, vv, ev; any null)

@@ -18,6 +18,8 @@
 #include "sort.hpp"
 #include "stencil.hpp"
 
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace ipcg {
@@ -875,6 +877,76 @@ double pot_energy(Ctx& c, double dhat, double kappa)
     IPCG_HIP(hipMemcpyAsync(h, c.red_out.p, sizeof(double), hipMemcpyDeviceToHost, s));
     IPCG_HIP(hipStreamSynchronize(s));
     return h[0];
+}
+
+// ---- allocation-free energy -------------------------------------------------
+// The energy's block partial sums in a file-scope device array instead of a
+// context buffer, for callers that cannot allocate device memory (code under
+// graph capture, serving loops).  Every GPU the library runs on holds its own
+// copy until the process ends; its device address is looked up once per
+// device (noalloc_init, from context creation with the device current).
+// Calls on one GPU share the array: calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[MAX_DEVICES] = {};
+std::string g_scratch_err[MAX_DEVICES];
+} // namespace
+
+void noalloc_init(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES || g_scratch[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch[device] = static_cast<double*>(p);
+}
+
+void pot_energy_noalloc(Ctx& c, double dhat, double kappa, double* d_out)
+{
+    require_collisions(c);
+    if (!d_out) throw Error(IPCG_ERR_ARG, "null output pointer");
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c.device >= 0 && c.device < MAX_DEVICES) part = g_scratch[c.device];
+        if (!part)
+            throw Error(IPCG_ERR_STATE,
+                        "no scratch array on this device"
+                            + (c.device >= 0 && c.device < MAX_DEVICES
+                                   ? ": " + g_scratch_err[c.device] : std::string()));
+    }
+    hipStream_t s = c.stream;
+    CollView C = view(c);
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
+    int64_t nb[4], tot = 0;
+    for (int k = 0; k < 4; k++) {
+        nb[k] = (C.n[k] + PB - 1) / PB;
+        tot += nb[k];
+    }
+    if (tot > NOALLOC_PARTIALS)
+        throw Error(IPCG_ERR_ARG, "collision set too large for the allocation-free energy: "
+                                      + std::to_string(tot) + " blocks of "
+                                      + std::to_string(PB) + " collisions, at most "
+                                      + std::to_string(NOALLOC_PARTIALS));
+    const double* X = c.X[0].as<double>();
+    int64_t o = 0;
+    if (nb[0]) hipLaunchKernelGGL(k_energy<K_VV>, dim3(nb[0]), dim3(PB), 0, s, C, X, c.nV, bp, part + o);
+    o += nb[0];
+    if (nb[1]) hipLaunchKernelGGL(k_energy<K_EV>, dim3(nb[1]), dim3(PB), 0, s, C, X, c.nV, bp, part + o);
+    o += nb[1];
+    if (nb[2]) hipLaunchKernelGGL(k_energy<K_EE>, dim3(nb[2]), dim3(PB), 0, s, C, X, c.nV, bp, part + o);
+    o += nb[2];
+    if (nb[3]) hipLaunchKernelGGL(k_energy<K_FV>, dim3(nb[3]), dim3(PB), 0, s, C, X, c.nV, bp, part + o);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, part, tot, d_out);
+    IPCG_CHECK_LAUNCH();
 }
 
 // Global collision ranges per kind and the number of (vertex, collision)

@@ -41,6 +41,7 @@ EXPORTS = [
     "ipcg_friction_build", "ipcg_friction_build_per_vertex", "ipcg_friction_download", "ipcg_velocities_upload",
     "ipcg_friction_energy", "ipcg_friction_gradient", "ipcg_friction_hessian",
     "ipcg_friction_set_anisotropy", "ipcg_friction_update_lagged_mu", "ipcg_barrier_set_physical",
+    "ipcg_barrier_energy_noalloc",
 ]
 
 _lib = None
@@ -82,6 +83,7 @@ def load(path=LIB_PATH):
     L.ipcg_collisions_count.argtypes = [vp, _L]
     L.ipcg_collisions_download.argtypes = [vp, _I, _D, _I, _D, _I, _D, _D, _I, _I, _D]
     L.ipcg_barrier_energy.argtypes = [vp, C.c_double, C.c_double, _D]
+    L.ipcg_barrier_energy_noalloc.argtypes = [vp, C.c_double, C.c_double, vp]
     L.ipcg_barrier_gradient.argtypes = [vp, C.c_double, C.c_double, _D]
     L.ipcg_barrier_hessian.argtypes = [vp, C.c_double, C.c_double, C.c_int, _L]
     L.ipcg_hessian_download.argtypes = [vp, _L, _I, _D]
