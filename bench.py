@@ -74,6 +74,16 @@ def parse():
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only copy a MIB MiB device buffer five times (torch), the known-size "
                         "reference for scaling the HBM byte counters of a profiler")
+    p.add_argument("--forms", type=int, default=1,
+                   help="also measure the batched form (--batch-scenes scenes of uneven size in "
+                        "one context, beside the same scenes one context each) and the served "
+                        "form (Poisson requests); 0: skip")
+    p.add_argument("--batch-scenes", type=int, default=256)
+    p.add_argument("--batch-steps", type=int, default=3)
+    p.add_argument("--serve-scenes", type=int, default=64)
+    p.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
+    p.add_argument("--serve-seconds", type=float, default=4.0,
+                   help="span of the request arrivals")
     p.add_argument("--rows", action="store_true",
                    help="time the collision-set variants, helpers and friction at the same "
                         "workload instead (one JSON line per entry point)")
@@ -84,6 +94,11 @@ def parse():
 def log(args, *a):
     if args.verbose:
         print(*a, file=sys.stderr, flush=True)
+
+
+def progress(*a):
+    """a line on stderr for the long phases (the JSON result is on stdout)"""
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def gpu_step(ctx, sc):
@@ -197,6 +212,134 @@ def cpu_baseline(args, scenes, capi):
     except Exception as e:  # the check never blocks the line
         out["device_vs_oracle"] = {"error": str(e)}
     return out
+
+
+def batched_step(ctx, dhat, kappa):
+    ctx.build_collisions(dhat)
+    ctx.energy_batched(dhat, kappa)
+    ctx.gradient(dhat, kappa, download=False)
+    ctx.hessian(dhat, kappa, "clamp", download=False)
+    steps = ctx.stepsize_batched()
+    ctx.synchronize()
+    return steps
+
+
+def batched_form(args, capi, scenes):
+    """Many independent scenes of uneven size (sheet pairs of side
+    log-uniform in [32, 384] and rows of 2-16 icospheres of subdivision 3-5,
+    2.6k to 300k triangles each): the five calls once per step over all of
+    them in one batched context, against the same scenes each in its own
+    context stepped one after the other (the plain single-scene path)."""
+    dhat, kappa = 1e-3, 1e3
+    sc = scenes.batch_scenes(args.batch_scenes, seed=args.seed)
+    B = scenes.Batch(sc)
+    progress("batched form:", len(sc), "scenes generated", B.sizes)
+    ctx = capi.Context(0)
+    ctx.set_autotune(True)  # traversal workgroup sizes timed on the first step
+    ctx.upload_batch(B)
+    ctx.upload_positions(0, B.V)
+    ctx.upload_positions(1, B.V1)
+    progress("batched form: uploaded")
+    batched_step(ctx, dhat, kappa)  # warm-up: buffers reach their size, shapes are tuned
+    progress("batched form: warm-up step done")
+    shapes = {st: ctx.launch_shape(st)[0] for st in
+              ("bp_traverse_ee.static", "bp_traverse_fv.static", "bp_traverse_ee.swept",
+               "bp_traverse_fv.swept")}
+    lat = []
+    for _ in range(max(1, args.batch_steps)):
+        t0 = time.perf_counter()
+        batched_step(ctx, dhat, kappa)
+        lat.append(time.perf_counter() - t0)
+    pairs = int(ctx.collision_counts_batched().sum())
+    dev_bytes = ctx.device_bytes()
+    ctx.close()
+    t_b = float(np.median(lat))
+    # plain path: one resident context per scene, the step called per scene
+    progress("batched form: %d timed steps, %.1f ms median" % (len(lat), 1e3 * t_b))
+    ctxs = []
+    for k, s in enumerate(sc):
+        c = capi.Context(0)
+        c.upload_mesh(s.V, s.E, s.F)
+        c.upload_positions(0, s.V)
+        c.upload_positions(1, s.V1)
+        ctxs.append(c)
+        if k % 64 == 63:
+            progress("plain path: %d contexts uploaded" % (k + 1))
+
+    def plain_pass():
+        for s, c in zip(sc, ctxs):
+            c.build_collisions(dhat)
+            c.energy(dhat, kappa)
+            c.gradient(dhat, kappa, download=False)
+            c.hessian(dhat, kappa, "clamp", download=False)
+            c.stepsize()
+            c.synchronize()
+
+    plain_pass()
+    progress("plain path: warm-up pass done")
+    lat_p = []
+    for _ in range(max(1, args.batch_steps)):
+        t0 = time.perf_counter()
+        plain_pass()
+        lat_p.append(time.perf_counter() - t0)
+        progress("plain path: pass %.1f ms" % (1e3 * lat_p[-1]))
+    pairs_p = int(sum(int(c.collision_counts().sum()) for c in ctxs))
+    for c in ctxs:
+        c.close()
+    t_p = float(np.median(lat_p))
+    nV, nE, nF = B.sizes
+    return {"workload": f"batch_{args.batch_scenes}_scenes", "scenes": len(sc),
+            "vertices": nV, "faces": nF, "collisions": pairs, "collisions_plain": pairs_p,
+            "ms_per_step": round(1e3 * t_b, 3), "p50_ms": round(1e3 * t_b, 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "value": round(pairs / t_b, 1), "unit": "pairs/s",
+            "plain_ms_per_step": round(1e3 * t_p, 3), "plain_value": round(pairs_p / t_p, 1),
+            "speedup_vs_plain": round(t_p / t_b, 3), "device_bytes": dev_bytes,
+            "steps": len(lat), "tuned_workgroup_sizes": shapes}
+
+
+def served_form(args, capi, scenes):
+    """Scenes drawn as for the batched form send energy, gradient, Hessian
+    and step requests (uniform kind, uniform scene) with Poisson arrivals;
+    a step perturbs the scene's positions by N(0, (0.1 dhat)^2) and applies
+    them when the CCD step is 1.  Latency = finish - arrival."""
+    dhat, kappa = 1e-3, 1e3
+    sc = scenes.batch_scenes(args.serve_scenes, seed=args.seed + 1)
+    progress("served form:", len(sc), "scenes generated")
+    srv = capi.Server(hbm_budget_bytes=64 << 30)
+    ids = [srv.add_scene(s.V, s.E, s.F, dhat, kappa, 0.1 * dhat, seed=k)
+           for k, s in enumerate(sc)]
+    rng = np.random.default_rng(args.seed)
+    n = max(1, int(args.serve_rate * args.serve_seconds))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    who = rng.integers(len(sc), size=n)
+    kinds = rng.integers(4, size=n)
+    tickets = [srv.submit(ids[int(w)], int(k), float(a)) for a, w, k in zip(arrivals, who, kinds)]
+    progress("served form: %d requests over %.1f s submitted" % (n, float(arrivals[-1])))
+    srv.run()
+    progress("served form: done")
+    res = [srv.result(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r.finish - r.arrival for r in res])
+    pairs = sum(r.pairs for r in res)
+    wall = st["wall_seconds"]
+    per_kind = {name: round(1e3 * float(np.percentile(lat[kinds == k], 99)), 3)
+                for k, name in enumerate(("energy", "gradient", "hessian", "step"))
+                if np.any(kinds == k)}
+    return {"workload": f"serve_{args.serve_scenes}_scenes_poisson_{args.serve_rate:g}",
+            "requests": n, "rate_offered": args.serve_rate,
+            "rate_served": round(n / wall, 1), "wall_s": round(wall, 3),
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "p99_ms_by_kind": per_kind,
+            "value": round(pairs / wall, 1), "unit": "pairs/s",
+            "collision_pairs_served": int(pairs),
+            "iterations": st["iterations"], "requests_per_iteration": round(n / st["iterations"], 2),
+            "busy_fraction": round(st["busy_seconds"] / wall, 3),
+            "steps_accepted": int(sum(r.accepted for r in res)),
+            "peak_device_bytes": st["peak_device_bytes"],
+            "estimated_bytes": st["estimated_bytes"], "hbm_budget_bytes": 64 << 30}
 
 
 def copy_calibration(mib):
@@ -404,11 +547,23 @@ def main():
                      "gradient_device_ms": round(per_step.get("gradient", 0.0), 3)}
 
     cpu = None
+    progress("step: %.3f ms per step" % ms_step)
     if args.cpu_tile > 0:
         try:
+            progress("cpu baseline")
             cpu = cpu_baseline(args, scenes, capi)
         except Exception as e:  # the baseline never blocks the GPU line
             cpu = {"value": None, "error": str(e)}
+
+    forms = None
+    if args.forms:
+        forms = {}
+        for name, fn in (("batched", batched_form), ("served", served_form)):
+            try:
+                forms[name] = fn(args, capi, scenes)
+            except Exception as e:  # a form never blocks the line
+                forms[name] = {"error": str(e)}
+            log(args, name, forms[name])
 
     out = {
         "metric": "collision pairs/sec (build+E/grad/Hess+CCD) at 1e-10 rel err; p50/p99 latency",
@@ -436,6 +591,7 @@ def main():
                    "host_transfer": host_transfer},
         "roofline": roof_dom,
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out))
 

@@ -814,4 +814,55 @@ inline bool has_intersections(const CollisionMesh& mesh, const MatrixXd& vertice
     return hit != 0;
 }
 
+// ---- Eigen interoperability ------------------------------------------------
+// Where Eigen is available (the reference's own types), matrices convert to
+// and from it: the mirror's dense matrices are row-major copies of Eigen's
+// column-major ones, gradients become VectorXd and the Hessian (symmetric,
+// so CSR equals CSC) an Eigen::SparseMatrix<double> with int indices, as the
+// reference returns it.
+#if __has_include(<Eigen/Core>) && __has_include(<Eigen/Sparse>)
+} // namespace ipc
+#include <Eigen/Core>
+#include <Eigen/Sparse>
+namespace ipc {
+
+template <class T, class E> inline Matrix<T> from_eigen(const E& M)
+{
+    Matrix<T> out((int)M.rows(), (int)M.cols());
+    for (int i = 0; i < out.rows(); i++)
+        for (int j = 0; j < out.cols(); j++) out(i, j) = static_cast<T>(M(i, j));
+    return out;
+}
+
+inline MatrixXd from_eigen(const Eigen::MatrixXd& M) { return from_eigen<double>(M); }
+inline MatrixXi from_eigen(const Eigen::MatrixXi& M) { return from_eigen<int32_t>(M); }
+
+inline Eigen::MatrixXd to_eigen(const MatrixXd& M)
+{
+    Eigen::MatrixXd out(M.rows(), M.cols());
+    for (int i = 0; i < M.rows(); i++)
+        for (int j = 0; j < M.cols(); j++) out(i, j) = M(i, j);
+    return out;
+}
+
+inline Eigen::VectorXd to_eigen(const std::vector<double>& v)
+{
+    Eigen::VectorXd out((Eigen::Index)v.size());
+    std::copy(v.begin(), v.end(), out.data());
+    return out;
+}
+
+inline Eigen::SparseMatrix<double> to_eigen(const SparseMatrix& H)
+{
+    if (H.nonZeros() > INT32_MAX) throw std::runtime_error("Hessian too large for int indices");
+    Eigen::SparseMatrix<double> out((Eigen::Index)H.rows, (Eigen::Index)H.cols);
+    out.resizeNonZeros((Eigen::Index)H.nonZeros());
+    std::transform(H.outer.begin(), H.outer.end(), out.outerIndexPtr(),
+                   [](int64_t x) { return static_cast<int>(x); });
+    std::copy(H.inner.begin(), H.inner.end(), out.innerIndexPtr());
+    std::copy(H.values.begin(), H.values.end(), out.valuePtr());
+    return out;
+}
+#endif
+
 } // namespace ipc

@@ -56,6 +56,18 @@ int ipcg_ctx_synchronize(ipcg_ctx* ctx);
 /* the context's hipStream_t (as void*), for callers that order their own
  * work against the library's */
 void* ipcg_ctx_stream(ipcg_ctx* ctx);
+/* bytes of device memory the context holds (its buffers only grow) */
+int ipcg_ctx_device_bytes(ipcg_ctx* ctx, int64_t* bytes);
+/* Launch-shape autotuning (no reference counterpart): with enable != 0,
+ * the first broad phase of each kind ("bp_traverse_ee.static",
+ * "bp_traverse_fv.static", "bp_traverse_ee.swept", "bp_traverse_fv.swept")
+ * times the LBVH traversal at 64, 128 and 256 lanes per workgroup on the
+ * real input and keeps the fastest for the context; results do not depend
+ * on the choice.  Off (the default, and after turning it off): 128.
+ * ipcg_launch_shape reports a stage's chosen size (0: not tuned) and its
+ * measured time in milliseconds (ms may be NULL). */
+int ipcg_ctx_set_autotune(ipcg_ctx* ctx, int enable);
+int ipcg_launch_shape(ipcg_ctx* ctx, const char* stage, int32_t* block, double* ms);
 
 /* ---- collision mesh (collision_mesh.hpp:14-126, collision_filter.hpp) ---
  * rest_positions: nV x 3 in `layout`; used for the edge-edge mollifier
@@ -340,6 +352,110 @@ int ipcg_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
 int ipcg_codim_candidates_toi(ipcg_ctx* ctx, double min_distance, double tolerance,
                               int64_t max_iterations, double conservative_rescaling,
                               double* toi_vv, double* toi_ev);
+
+/* ---- batched scenes -------------------------------------------------------
+ * Many independent scenes in one context, each stage one launch over all of
+ * them.  No reference counterpart: each scene is what one CollisionMesh /
+ * NormalCollisions / BarrierPotential / compute_collision_free_stepsize
+ * call sequence (normal_collisions.cpp:20-158, potential.cpp:36-222,
+ * ipc.cpp:45-101) computes for that scene alone, and every per-scene result
+ * equals the one a context holding the scene alone returns.
+ *
+ * Scene s owns vertices [vertex_offsets[s], vertex_offsets[s+1]), edges
+ * [edge_offsets[s], ...) and faces [face_offsets[s], ...) of the
+ * concatenated arrays (n_scenes + 1 offsets each, starting at 0); edges and
+ * faces hold scene-local vertex ids.  Positions (both slots) are the
+ * concatenated nV x 3 arrays.  Pairs of different scenes never collide: the
+ * broad phase builds one LBVH per scene (each scene's Morton codes mapped
+ * by its own box) and every query walks its own scene's tree.  The
+ * ordinary entry points then work on the batch: collision lists,
+ * candidates and the gradient use global ids (scene s's gradient is rows
+ * [3 vertex_offsets[s], 3 vertex_offsets[s+1])), the Hessian is block
+ * diagonal with one block per scene.  LBVH broad phase only
+ * (IPCG_BP_SAP: IPCG_ERR_ARG). */
+int ipcg_mesh_upload_batched(ipcg_ctx* ctx, int32_t n_scenes, const int64_t* vertex_offsets,
+                             const double* rest_positions, int layout,
+                             const int64_t* edge_offsets, const int32_t* edges,
+                             const int64_t* face_offsets, const int32_t* faces);
+/* number of scenes of a batched context (0: an ordinary context) */
+int ipcg_batch_size(ipcg_ctx* ctx, int32_t* n_scenes);
+/* active[n_scenes] (NULL: all): the broad phase skips the queries of
+ * inactive scenes, so the calls that follow leave them without candidates
+ * and collisions (energy 0, step size 1); the results of active scenes do
+ * not depend on which others are active.  Invalidates the candidate and
+ * collision sets. */
+int ipcg_batch_set_active(ipcg_ctx* ctx, const uint8_t* active);
+/* counts[4 * n_scenes]: {vv, ev, ee, fv} of every scene of the current
+ * collision set */
+int ipcg_collisions_count_batched(ipcg_ctx* ctx, int64_t* counts);
+/* energies[n_scenes]: each scene's barrier energy, summed in the order a
+ * context holding the scene alone uses (the same value bit for bit) */
+int ipcg_barrier_energy_batched(ipcg_ctx* ctx, double dhat, double kappa, double* energies);
+/* steps[n_scenes]: compute_collision_free_stepsize of every scene (one
+ * running minimum per scene) */
+int ipcg_compute_collision_free_stepsize_batched(ipcg_ctx* ctx, double min_distance,
+                                                 double tolerance, int64_t max_iterations,
+                                                 double conservative_rescaling, double* steps);
+
+/* ---- serving --------------------------------------------------------------
+ * No reference counterpart: a request loop for callers that serve many
+ * simulations at once.  Simulated scenes send energy, gradient, Hessian and
+ * step requests with arrival times (seconds after ipcg_server_run starts).
+ * The server keeps every admitted scene's topology and positions resident
+ * on the device in one batched context and serves, per iteration, the
+ * earliest arrived request of every admitted scene (continuous batching:
+ * requests that arrive while an iteration runs join the next one).  Energy,
+ * gradient and Hessian requests share one collision build and evaluation
+ * (BarrierPotential with PSD clamp, inflation 0.5 dhat) over the scenes that
+ * asked; step requests share one CCD: the scene's positions are perturbed by
+ * N(0, sigma^2) per coordinate (its own generator, seeded per scene) and the
+ * perturbed positions replace the current ones only when
+ * compute_collision_free_stepsize returns 1 (min_distance 0, tolerance 1e-6,
+ * 1e7 iterations, rescaling 0.8).  A scene's results do not depend on the
+ * other scenes it shares launches with: each equals the result of serving
+ * the scene alone.  Admission: scenes enter the batch in submission order
+ * while the sum of their estimated device footprints stays within
+ * hbm_budget_bytes; when the admitted scenes have no request left, the
+ * batch is rebuilt with the scenes still waiting.  Every scene of a server
+ * shares dhat and kappa.  keep_results != 0 keeps every gradient and the
+ * Hessian block of every Hessian request on the host (for checks). */
+typedef struct ipcg_server ipcg_server;
+enum { IPCG_REQ_ENERGY = 0, IPCG_REQ_GRADIENT = 1, IPCG_REQ_HESSIAN = 2, IPCG_REQ_STEP = 3 };
+typedef struct ipcg_request_result {
+    double arrival, start, finish; /* seconds after the run started */
+    double value;     /* energy | gradient 2-norm | Hessian block nonzeros | step size */
+    int64_t pairs;    /* collisions of the scene (energy, gradient, Hessian requests) */
+    int32_t accepted; /* step requests: the perturbation was applied */
+    int32_t iteration;
+} ipcg_request_result;
+typedef struct ipcg_server_stats {
+    int64_t iterations, epochs, requests;
+    int64_t peak_device_bytes; /* the batched context's device memory, largest seen */
+    int64_t estimated_bytes;   /* sum of the admitted scenes' estimates, largest seen */
+    double busy_seconds, wall_seconds;
+} ipcg_server_stats;
+int ipcg_server_create(int device, int64_t hbm_budget_bytes, int keep_results,
+                       ipcg_server** out);
+int ipcg_server_destroy(ipcg_server* srv);
+const char* ipcg_server_last_error(const ipcg_server* srv);
+/* a scene: nV x 3 positions (AoS, also its rest positions), scene-local
+ * edges and faces, barrier parameters, step perturbation sigma and seed */
+int ipcg_server_add_scene(ipcg_server* srv, int32_t nV, const double* V, int32_t nE,
+                          const int32_t* E, int32_t nF, const int32_t* F, double dhat,
+                          double kappa, double sigma, uint64_t seed, int32_t* scene);
+int ipcg_server_submit(ipcg_server* srv, int32_t scene, int kind, double arrival,
+                       int64_t* ticket);
+/* serves every submitted request; returns when all are done */
+int ipcg_server_run(ipcg_server* srv);
+int ipcg_server_result(ipcg_server* srv, int64_t ticket, ipcg_request_result* out);
+/* kept results: the scene's gradient (3 nV), its Hessian block (CSR with
+ * 3 nV + 1 row pointers; nnz first, any pointer may be NULL) */
+int ipcg_server_gradient(ipcg_server* srv, int64_t ticket, double* grad);
+int ipcg_server_hessian(ipcg_server* srv, int64_t ticket, int64_t* nnz, int64_t* rowptr,
+                        int32_t* col, double* val);
+/* a scene's current positions (nV x 3) */
+int ipcg_server_positions(ipcg_server* srv, int32_t scene, double* V);
+int ipcg_server_get_stats(ipcg_server* srv, ipcg_server_stats* out);
 
 /* ---- timing of the last call's kernels (for benchmarks) ----------------- */
 /* Kernel time of named stages recorded with HIP events on the context

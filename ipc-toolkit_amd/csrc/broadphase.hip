@@ -101,6 +101,47 @@ __global__ void k_mesh_aabb_final(const double* partial, int nb, double* aabb)
     }
 }
 
+// Batched contexts: the mesh box of each scene's vertex boxes, one
+// workgroup per scene (min / max are exact, so the box equals the one a
+// context holding the scene alone computes).
+__global__ __launch_bounds__(BP_BLOCK) void k_scene_aabb(const double* X0, const double* X1,
+                                                         int nV, const int64_t* voff, double r,
+                                                         double* aabb)
+{
+    const int sc = blockIdx.x;
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int64_t i = voff[sc] + threadIdx.x; i < voff[sc + 1]; i += BP_BLOCK) {
+        double l[3], h[3];
+        vbox(X0, X1, nV, (int)i, r, l, h);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], l[k]);
+            hi[k] = fmax(hi[k], h[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, WAVE));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, WAVE));
+        }
+    __shared__ double sm[BP_BLOCK / WAVE][6];
+    const int w = threadIdx.x / WAVE;
+    if (threadIdx.x % WAVE == 0)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sm[w][k] = lo[k];
+            sm[w][3 + k] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = sm[0][threadIdx.x];
+        for (int j = 1; j < BP_BLOCK / WAVE; j++)
+            v = threadIdx.x < 3 ? fmin(v, sm[j][threadIdx.x]) : fmax(v, sm[j][threadIdx.x]);
+        aabb[6 * (int64_t)sc + threadIdx.x] = v;
+    }
+}
+
 __device__ __forceinline__ uint64_t expand_bits_2(uint64_t v)
 {
     v = (v | v << 32) & 0x1F00000000FFFFull;
@@ -125,16 +166,28 @@ __device__ __forceinline__ uint64_t morton3(double x, double y, double z)
 }
 
 // per primitive (ARITY vertices): FP64 box, Morton code, FP32 box
+// first vertex of primitive p (I: vertex ids per primitive; arity 1: a
+// vertex subset, or null for all vertices)
+template <int ARITY>
+__device__ __forceinline__ int first_vertex(const int32_t* I, int p)
+{
+    return (ARITY == 1 && !I) ? p : I[(int64_t)ARITY * p];
+}
+
+// vscene (batched contexts): scene per vertex, each primitive mapped by its
+// own scene's box; mort then receives a copy of the Morton codes
 template <int ARITY>
 __global__ __launch_bounds__(BP_BLOCK) void k_prim_boxes(
     const double* X0, const double* X1, int nV, const int32_t* I, int n, double r,
-    const double* aabb, float* box32, uint64_t* keys, uint32_t* ids)
+    const double* aabb, const int32_t* vscene, float* box32, uint64_t* keys, uint32_t* ids,
+    uint64_t* mort)
 {
     const int p = blockIdx.x * BP_BLOCK + threadIdx.x;
     if (p >= n) return;
     double lo[3], hi[3];
-    // I: vertex ids per primitive (arity 1: a vertex subset, or null for all)
-    vbox(X0, X1, nV, (ARITY == 1 && !I) ? p : I[(int64_t)ARITY * p], r, lo, hi);
+    const int v0 = first_vertex<ARITY>(I, p);
+    if (vscene) aabb += 6 * (int64_t)vscene[v0];
+    vbox(X0, X1, nV, v0, r, lo, hi);
 #pragma unroll
     for (int j = 1; j < ARITY; j++) {
         double l[3], h[3];
@@ -151,7 +204,9 @@ __global__ __launch_bounds__(BP_BLOCK) void k_prim_boxes(
         const double c = 0.5 * (lo[k] + hi[k]);
         m[k] = (c - aabb[k]) / (aabb[3 + k] - aabb[k]);
     }
-    keys[p] = morton3(m[0], m[1], m[2]);
+    const uint64_t code = morton3(m[0], m[1], m[2]);
+    keys[p] = code;
+    if (mort) mort[p] = code;
     ids[p] = (uint32_t)p;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -160,11 +215,46 @@ __global__ __launch_bounds__(BP_BLOCK) void k_prim_boxes(
     }
 }
 
+// Forest of a batched context: after the Morton sort, the scene of each
+// sorted primitive becomes the key of a second stable sort (primitives of
+// one scene keep their Morton order); then the Morton codes are gathered
+// back in the final order and the scene of every leaf is kept.
+template <int ARITY>
+__global__ __launch_bounds__(BP_BLOCK) void k_scene_keys(const uint32_t* ids, const int32_t* I,
+                                                         const int32_t* vscene, int n,
+                                                         uint64_t* keys)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (i < n) keys[i] = (uint64_t)vscene[first_vertex<ARITY>(I, (int)ids[i])];
+}
+
+__global__ __launch_bounds__(BP_BLOCK) void k_forest_keys(const uint32_t* ids, const uint64_t* mort,
+                                                          int n, uint64_t* keys, int32_t* lscene)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    lscene[i] = (int32_t)keys[i];
+    keys[i] = mort[ids[i]];
+}
+
+// Root entry per scene before the refit: none, a single leaf, or to be
+// completed (loff: the scene's first sorted leaf position, n_scenes + 1).
+__global__ void k_forest_roots(const int64_t* loff, int n_scenes, int32_t* root)
+{
+    const int sc = blockIdx.x * 256 + threadIdx.x;
+    if (sc >= n_scenes) return;
+    const int64_t m = loff[sc + 1] - loff[sc];
+    root[sc] = m == 0 ? ROOT_NONE : (m == 1 ? ~(int32_t)loff[sc] : ROOT_UNFINISHED);
+}
+
 // Common-prefix length of the sorted keys i and j extended by their
-// positions, so equal Morton codes still order strictly; -1 outside [0, n).
-__device__ __forceinline__ int delta(const uint64_t* codes, int n, int i, int j)
+// positions, so equal Morton codes still order strictly; -1 outside [0, n)
+// and, in a forest, across scenes (lscene: scene per sorted leaf).
+__device__ __forceinline__ int delta(const uint64_t* codes, const int32_t* lscene, int n, int i,
+                                     int j)
 {
     if (j < 0 || j >= n) return -1;
+    if (lscene && lscene[i] != lscene[j]) return -1;
     const uint64_t ci = codes[i], cj = codes[j];
     if (ci == cj) return 64 + __clz((unsigned)(i ^ j));
     return __clzll(ci ^ cj);
@@ -176,23 +266,30 @@ __device__ __forceinline__ int delta(const uint64_t* codes, int n, int i, int j)
 // build needs no inter-thread communication.  The tree shape differs from
 // the reference's Apetrei construction (lbvh.cpp:197-302); the candidate set
 // does not depend on it.
-__global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, int n, Node* nodes,
-                                                     int32_t* parent, int32_t* leaf_parent)
+//
+// In a forest (lscene non-null) the prefix is -1 across scenes, so every
+// scene's leaves form a tree of their own: internal node i exists when
+// leaves i and i + 1 belong to one scene, and the node at a scene's first
+// leaf is its root.
+__global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, const int32_t* lscene,
+                                                     int n, Node* nodes, int32_t* parent,
+                                                     int32_t* leaf_parent)
 {
     const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
     if (i >= n - 1) return;
-    const int d = delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1) >= 0 ? 1 : -1;
-    const int dmin = delta(codes, n, i, i - d);
+    if (lscene && lscene[i] != lscene[i + 1]) return; // no node between scenes
+    const int d = delta(codes, lscene, n, i, i + 1) - delta(codes, lscene, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = delta(codes, lscene, n, i, i - d);
     int lmax = 2;
-    while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    while (delta(codes, lscene, n, i, i + lmax * d) > dmin) lmax <<= 1;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+        if (delta(codes, lscene, n, i, i + (l + t) * d) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = delta(codes, n, i, j);
+    const int dnode = delta(codes, lscene, n, i, j);
     int sp = 0;
     for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
-        if (delta(codes, n, i, i + (sp + t) * d) > dnode) sp += t;
+        if (delta(codes, lscene, n, i, i + (sp + t) * d) > dnode) sp += t;
         if (t == 1) break;
     }
     const int g = i + sp * d + (d < 0 ? -1 : 0);
@@ -214,7 +311,7 @@ __global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, int 
     }
     N->r0 = g;
     N->r1 = hi;
-    if (i == 0) parent[0] = -1;
+    if (lscene ? (i == 0 || lscene[i - 1] != lscene[i]) : i == 0) parent[i] = -1; // a root
 }
 
 // Bottom-up box refit, one level of completed nodes per launch: a node's
@@ -262,6 +359,7 @@ __device__ __forceinline__ void refit_append(bool push, int32_t p, int32_t* next
 
 __global__ __launch_bounds__(RF_BLOCK) void k_refit_leaves(const uint32_t* ids,
                                                            const float* box32, int n,
+                                                           const int32_t* lscene,
                                                            const int32_t* leaf_parent,
                                                            Node* nodes, int32_t* visits,
                                                            int32_t* next, uint32_t* next_n)
@@ -269,7 +367,10 @@ __global__ __launch_bounds__(RF_BLOCK) void k_refit_leaves(const uint32_t* ids,
     const int i = blockIdx.x * RF_BLOCK + threadIdx.x;
     bool push = false;
     int32_t p = 0;
-    if (i < n) {
+    // a forest's single-leaf scenes have no parent node
+    const bool alone = lscene && i < n && (i == 0 || lscene[i - 1] != lscene[i])
+        && (i == n - 1 || lscene[i + 1] != lscene[i]);
+    if (i < n && !alone) {
         const uint32_t pid = ids[i];
         float lo[3], hi[3];
 #pragma unroll
@@ -285,7 +386,8 @@ __global__ __launch_bounds__(RF_BLOCK) void k_refit_leaves(const uint32_t* ids,
 
 __global__ __launch_bounds__(RF_BLOCK) void k_refit_level(const int32_t* cur,
                                                           const uint32_t* cur_n,
-                                                          const int32_t* parent, Node* nodes,
+                                                          const int32_t* parent,
+                                                          const int32_t* lscene, Node* nodes,
                                                           int32_t* visits, int32_t* next,
                                                           uint32_t* next_n, int32_t* root)
 {
@@ -302,7 +404,7 @@ __global__ __launch_bounds__(RF_BLOCK) void k_refit_level(const int32_t* cur,
             hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
         }
         p = parent[j];
-        if (p < 0) *root = j; // the tree is complete
+        if (p < 0) root[lscene ? lscene[j] : 0] = j; // the tree is complete
         else push = refit_child(nodes, p, j, lo, hi, visits);
     }
     refit_append(push, p, next, next_n);
@@ -421,19 +523,30 @@ __device__ __forceinline__ int2 leaf_pair(const Filter& flt, const Query& q, con
 
 // SWEPT only names the launches of the swept (CCD) broad phase apart from
 // the static ones in kernel profiles; the code is the same.
-template <int MODE, int SWEPT>
-__global__ __launch_bounds__(TR_BLOCK) void k_traverse(
+// Forest of a batched context (vscene non-null): every query walks the tree
+// of its own scene (root_p: one entry per scene; queries of scenes that
+// `active` leaves out are skipped).
+struct ForestArgs {
+    const int32_t* vscene;   // scene per vertex (null: one tree, root_p[0])
+    const uint8_t* active;   // per scene (null: all)
+    const uint32_t* tids;    // target tree: primitive per sorted leaf
+};
+
+// BLOCK: workgroup size (64, 128 or 256), chosen per launch by the
+// context's autotuner or TR_BLOCK.
+template <int MODE, int SWEPT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_traverse(
     const float* qbox, const uint32_t* qids, int nq, const Node* nodes, const int32_t* root_p,
     const int4* leaf, const float* tbox, int nleaves, Filter flt, const int32_t* qmap, int2* out,
-    Shards S, int* deep, int* status)
+    Shards S, int* deep, int* status, ForestArgs fa)
 {
     constexpr bool TRI = MODE == TR_EE || MODE == TR_VV;
     // The traversal stack of postponed right children: the first TR_LSTACK
     // entries in LDS (small enough for full occupancy), deeper ones in the
     // query's own HBM slots (deep: TR_STACK - TR_LSTACK ints per query).
-    __shared__ int stack[TR_LSTACK][TR_BLOCK];
+    __shared__ int stack[TR_LSTACK][BLOCK];
     const int t = threadIdx.x;
-    const int qpos = (int)xcd_block() * TR_BLOCK + t;
+    const int qpos = (int)xcd_block() * BLOCK + t;
     bool active = qpos < nq;
     float q[6] = { 0, 0, 0, 0, 0, 0 };
     Query qr { 0, { -1, -1, -1 } };
@@ -450,26 +563,8 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
             qr.v[0] = qr.id;
         }
     }
-    const int root = *root_p;
     const int nin = nleaves - 1; // internal nodes
-    if (root < -1 || root >= (nin > 0 ? nin : 1)) { // build did not finish: refuse to walk
-        if (t == 0) atomicOr(status, 2);
-        return;
-    }
     const int sh = wave_shard();
-    if (root == -1) { // single target leaf (lbvh.cpp:386-396)
-        int2 hit = make_int2(-1, -1);
-        if (!TRI && active) {
-            const int4 r = leaf[0];
-            const float* b = tbox;
-            if (overlap(q, b, b + 3)) hit = leaf_pair<MODE>(flt, qr, r);
-        }
-        const unsigned long long slot = shard_append(S, sh, hit.x >= 0 ? 1u : 0u);
-        if (hit.x >= 0 && (int64_t)slot < S.scap) out[sh * S.scap + (int64_t)slot] = hit;
-        return;
-    }
-    int node = root, sp = 0, steps = 0;
-    const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
     // Hits wait in registers (up to TR_BUF per lane) and the wave appends them
     // together when a lane could overflow or the walk ends: one atomic per
     // several iterations instead of one per iteration with a hit.
@@ -478,6 +573,35 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
     for (int k = 0; k < TR_BUF; k++) buf[k] = make_int2(-1, -1);
     unsigned nb = 0;
     int2* const reg = out + sh * S.scap;
+    int node = 0;
+    if (active) {
+        int root = root_p[0];
+        if (fa.vscene) {
+            const int sc = fa.vscene[qr.v[0]];
+            root = root_p[sc];
+            if (root == ROOT_NONE || (fa.active && !fa.active[sc])) active = false;
+        }
+        if (!active) {
+        } else if (root < -nleaves || root >= (nin > 0 ? nin : 1)) {
+            // the build did not finish: refuse to walk
+            atomicOr(status, 2);
+            active = false;
+        } else if (root < 0) { // single target leaf (lbvh.cpp:386-396)
+            if (!TRI) {
+                const int lp = ~root;
+                const float* b = tbox + (int64_t)6 * (fa.tids ? fa.tids[lp] : 0);
+                if (overlap(q, b, b + 3)) {
+                    buf[0] = leaf_pair<MODE>(flt, qr, leaf[lp]);
+                    nb = buf[0].x >= 0 ? 1u : 0u;
+                }
+            }
+            active = false;
+        } else {
+            node = root;
+        }
+    }
+    int sp = 0, steps = 0;
+    const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
     while (__any(active)) {
         if (active && ++steps > max_steps) {
             atomicOr(status, 2);
@@ -550,6 +674,79 @@ __global__ __launch_bounds__(TR_BLOCK) void k_traverse(
         for (int k = 0; k < TR_BUF; k++)
             if (nb > (unsigned)k && first + k < S.scap) reg[first + k] = buf[k];
     }
+}
+
+struct TraverseArgs {
+    const float* qbox;
+    const uint32_t* qids;
+    int nq;
+    const Node* nodes;
+    const int32_t* root;
+    const int4* leaf;
+    const float* tbox;
+    int nleaves;
+    Filter flt;
+    const int32_t* qmap;
+    int2* out;
+    Shards S;
+    int* deep;
+    int* status;
+    ForestArgs fa;
+};
+
+template <int MODE, int SWEPT, int BLOCK>
+void launch_traverse_b(const TraverseArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_traverse<MODE, SWEPT, BLOCK>), dim3(grid_for(a.nq, BLOCK)), dim3(BLOCK),
+                       0, s, a.qbox, a.qids, a.nq, a.nodes, a.root, a.leaf, a.tbox, a.nleaves,
+                       a.flt, a.qmap, a.out, a.S, a.deep, a.status, a.fa);
+}
+
+template <int MODE, int SWEPT> void launch_traverse(int block, const TraverseArgs& a, hipStream_t s)
+{
+    if (block == 64) launch_traverse_b<MODE, SWEPT, 64>(a, s);
+    else if (block == 256) launch_traverse_b<MODE, SWEPT, 256>(a, s);
+    else launch_traverse_b<MODE, SWEPT, TR_BLOCK>(a, s);
+}
+
+constexpr int TUNE_BLOCKS[3] = { 64, 128, 256 };
+
+// One traversal launch with the workgroup size tuned for this context: on
+// the first call of `stage` with autotuning on, every size runs once on the
+// real input (its output is discarded by resetting the list's shard
+// counters) and the fastest, by HIP events on the context stream, is kept.
+// Candidate sets do not depend on the size.
+template <class Launch>
+void tuned_traverse(Ctx& c, const std::string& stage, unsigned long long* cnt, Launch&& launch)
+{
+    hipStream_t s = c.stream;
+    auto it = c.launch_block.find(stage);
+    int block = it != c.launch_block.end() ? it->second : 0;
+    if (block == 0 && c.autotune) {
+        hipEvent_t e0, e1;
+        IPCG_HIP(hipEventCreate(&e0));
+        IPCG_HIP(hipEventCreate(&e1));
+        float best = INFINITY;
+        for (int b : TUNE_BLOCKS) {
+            IPCG_HIP(hipMemsetAsync(cnt, 0, SHARD_CNT_BYTES, s));
+            IPCG_HIP(hipEventRecord(e0, s));
+            launch(b);
+            IPCG_HIP(hipEventRecord(e1, s));
+            IPCG_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            IPCG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            c.tune_ms[stage + "@" + std::to_string(b)] = ms;
+            if (ms < best) {
+                best = ms;
+                block = b;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c.launch_block[stage] = block;
+        IPCG_HIP(hipMemsetAsync(cnt, 0, SHARD_CNT_BYTES, s));
+    }
+    launch(block ? block : TR_BLOCK);
 }
 
 // ---------------------------------------------------------------------------
@@ -736,37 +933,82 @@ __global__ __launch_bounds__(BP_BLOCK) void k_leaf_records(const uint32_t* ids, 
     rec[i] = make_int4(out_ids ? out_ids[p] : p, v[0], v[1], v[2]);
 }
 
-// out_ids: mesh id of each primitive (null: its index)
+// out_ids: mesh id of each primitive (null: its index).  Batched contexts
+// build a forest: `loff` holds each scene's first primitive (n_scenes + 1;
+// primitives of a scene are contiguous), c.vscene the scene per vertex.
 void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const double* X0,
-                const double* X1, double r, bool with_nodes, const int32_t* out_ids = nullptr)
+                const double* X1, double r, bool with_nodes, const int32_t* out_ids = nullptr,
+                const std::vector<int64_t>* loff = nullptr)
 {
     hipStream_t s = c.stream;
     T.n = n;
-    if (n <= 0) return;
+    const bool forest = loff != nullptr;
+    const int S = forest ? (int)loff->size() - 1 : 1;
+    T.root.reserve(sizeof(int32_t) * (size_t)S);
+    if (forest) { // root entries for scenes with no or one primitive
+        T.loff.reserve(sizeof(int64_t) * loff->size());
+        IPCG_HIP(hipMemcpyAsync(T.loff.p, loff->data(), sizeof(int64_t) * loff->size(),
+                                hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_forest_roots, dim3(grid_for(S, 256)), dim3(256), 0, s,
+                           T.loff.as<int64_t>(), S, T.root.as<int32_t>());
+        IPCG_CHECK_LAUNCH();
+    }
+    if (n <= 0) {
+        if (forest) IPCG_HIP(hipStreamSynchronize(s)); // the offsets' host copy
+        return;
+    }
     T.box.reserve(sizeof(float) * 6 * (size_t)n);
     T.keys.reserve(sizeof(uint64_t) * (size_t)n);
     T.keys2.reserve(sizeof(uint64_t) * (size_t)n);
     T.ids.reserve(sizeof(uint32_t) * (size_t)n);
     T.ids2.reserve(sizeof(uint32_t) * (size_t)n);
-    T.root.reserve(sizeof(int32_t));
+    const int32_t* vscene = forest ? c.vscene.as<int32_t>() : nullptr;
+    if (forest) {
+        T.mort.reserve(sizeof(uint64_t) * (size_t)n);
+        T.lscene.reserve(sizeof(int32_t) * (size_t)n);
+    }
+    uint64_t* mort = forest ? T.mort.as<uint64_t>() : nullptr;
     const double* aabb = c.aabb.as<double>();
     const unsigned g = grid_for(n, BP_BLOCK);
     if (arity == 1)
         hipLaunchKernelGGL(k_prim_boxes<1>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
-                           aabb, T.box.as<float>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+                           aabb, vscene, T.box.as<float>(), T.keys.as<uint64_t>(),
+                           T.ids.as<uint32_t>(), mort);
     else if (arity == 2)
         hipLaunchKernelGGL(k_prim_boxes<2>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
-                           aabb, T.box.as<float>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+                           aabb, vscene, T.box.as<float>(), T.keys.as<uint64_t>(),
+                           T.ids.as<uint32_t>(), mort);
     else
         hipLaunchKernelGGL(k_prim_boxes<3>, dim3(g), dim3(BP_BLOCK), 0, s, X0, X1, c.nV, I, n, r,
-                           aabb, T.box.as<float>(), T.keys.as<uint64_t>(), T.ids.as<uint32_t>());
+                           aabb, vscene, T.box.as<float>(), T.keys.as<uint64_t>(),
+                           T.ids.as<uint32_t>(), mort);
     IPCG_CHECK_LAUNCH();
     c.sort_tmp.reserve(radix_tmp_bytes(n));
     c.stage_begin("bp_sort");
     radix_sort_pairs(T.keys.as<uint64_t>(), T.ids.as<uint32_t>(), T.keys2.as<uint64_t>(),
                      T.ids2.as<uint32_t>(), n, 63, c.sort_tmp.p, s);
+    if (forest) { // stable second pass by scene, then the Morton codes in that order
+        if (arity == 1)
+            hipLaunchKernelGGL(k_scene_keys<1>, dim3(g), dim3(BP_BLOCK), 0, s,
+                               T.ids.as<uint32_t>(), I, vscene, n, T.keys.as<uint64_t>());
+        else if (arity == 2)
+            hipLaunchKernelGGL(k_scene_keys<2>, dim3(g), dim3(BP_BLOCK), 0, s,
+                               T.ids.as<uint32_t>(), I, vscene, n, T.keys.as<uint64_t>());
+        else
+            hipLaunchKernelGGL(k_scene_keys<3>, dim3(g), dim3(BP_BLOCK), 0, s,
+                               T.ids.as<uint32_t>(), I, vscene, n, T.keys.as<uint64_t>());
+        IPCG_CHECK_LAUNCH();
+        int bits = 1;
+        while ((1ll << bits) < S) bits++;
+        radix_sort_pairs(T.keys.as<uint64_t>(), T.ids.as<uint32_t>(), T.keys2.as<uint64_t>(),
+                         T.ids2.as<uint32_t>(), n, bits, c.sort_tmp.p, s);
+        hipLaunchKernelGGL(k_forest_keys, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(),
+                           mort, n, T.keys.as<uint64_t>(), T.lscene.as<int32_t>());
+        IPCG_CHECK_LAUNCH();
+    }
     c.stage_end();
     if (!with_nodes) return;
+    const int32_t* lscene = forest ? T.lscene.as<int32_t>() : nullptr;
     T.leaf.reserve(sizeof(int4) * (size_t)n);
     if (arity == 1)
         hipLaunchKernelGGL(k_leaf_records<1>, dim3(g), dim3(BP_BLOCK), 0, s, T.ids.as<uint32_t>(), I,
@@ -779,9 +1021,18 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
                            out_ids, n, T.leaf.as<int4>());
     IPCG_CHECK_LAUNCH();
     T.nodes.reserve(sizeof(Node) * (size_t)(n > 1 ? n - 1 : 1));
-    if (n == 1) { // single leaf: no internal node (lbvh.cpp:386-396)
+    if (n == 1 && !forest) { // single leaf: no internal node (lbvh.cpp:386-396)
         const int32_t one = -1;
         IPCG_HIP(hipMemcpyAsync(T.root.p, &one, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    // a forest of single-leaf scenes has no internal node either
+    bool any_node = !forest;
+    if (forest)
+        for (size_t k = 0; k + 1 < loff->size(); k++)
+            if ((*loff)[k + 1] - (*loff)[k] >= 2) any_node = true;
+    if (!any_node) {
         IPCG_HIP(hipStreamSynchronize(s));
         return;
     }
@@ -796,19 +1047,20 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     uint32_t* cnt = reinterpret_cast<uint32_t*>(work[1] + n);
     IPCG_HIP(hipMemsetAsync(visits, 0, sizeof(int32_t) * ni, s));
     IPCG_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(uint32_t), s));
-    IPCG_HIP(hipMemsetAsync(T.root.p, 0x80, sizeof(int32_t), s)); // "unfinished" sentinel
+    if (!forest) // "unfinished" sentinel
+        IPCG_HIP(hipMemsetAsync(T.root.p, 0x80, sizeof(int32_t), s));
     c.stage_begin("bp_build");
     hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, BP_BLOCK)), dim3(BP_BLOCK), 0, s,
-                       T.keys.as<uint64_t>(), n, T.nodes.as<Node>(), parent, leaf_parent);
+                       T.keys.as<uint64_t>(), lscene, n, T.nodes.as<Node>(), parent, leaf_parent);
     hipLaunchKernelGGL(k_refit_leaves, dim3(grid_for(n, RF_BLOCK)), dim3(RF_BLOCK), 0, s,
-                       T.ids.as<uint32_t>(), T.box.as<float>(), n, leaf_parent,
+                       T.ids.as<uint32_t>(), T.box.as<float>(), n, lscene, leaf_parent,
                        T.nodes.as<Node>(), visits, work[0], cnt);
     IPCG_CHECK_LAUNCH();
     // Round k completes the nodes of height k; their subtrees are disjoint and
     // hold at least k + 1 leaves each, so round k has at most n / (k + 1)
     // nodes.
     // Rounds are launched in batches sized by that bound (a round past the
-    // root finds an empty list) and the host checks for the end once per
+    // roots finds an empty list) and the host checks for the end once per
     // batch.  Every round completes at least one node, so n - 1 rounds bound
     // the loop.
     uint32_t* h = static_cast<uint32_t*>(c.pinned);
@@ -820,8 +1072,8 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
             uint32_t* nxt = cnt + (k & 1);
             IPCG_HIP(hipMemsetAsync(nxt, 0, sizeof(uint32_t), s));
             hipLaunchKernelGGL(k_refit_level, dim3(grid_for(bound, RF_BLOCK)), dim3(RF_BLOCK), 0,
-                               s, work[(k - 1) & 1], cur_n, parent, T.nodes.as<Node>(), visits,
-                               work[k & 1], nxt, T.root.as<int32_t>());
+                               s, work[(k - 1) & 1], cur_n, parent, lscene, T.nodes.as<Node>(),
+                               visits, work[k & 1], nxt, T.root.as<int32_t>());
         }
         IPCG_CHECK_LAUNCH();
         IPCG_HIP(hipMemcpyAsync(h, cnt + ((level + BATCH - 1) & 1), sizeof(uint32_t),
@@ -833,11 +1085,26 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     c.stage_end();
 }
 
+ForestArgs forest_args(const Ctx& c, const Tree& target)
+{
+    if (!c.batched()) return { nullptr, nullptr, target.ids.as<uint32_t>() };
+    return { c.vscene.as<int32_t>(), c.all_active ? nullptr : c.scene_active.as<uint8_t>(),
+             target.ids.as<uint32_t>() };
+}
+
 // mesh box of the vertex boxes (broad_phase.cpp:93-125) into c.aabb
 void mesh_bounds(Ctx& c, const double* X0, const double* X1, double inflation)
 {
     hipStream_t s = c.stream;
     c.stage_begin("bp_boxes");
+    if (c.batched()) { // one box per scene
+        c.aabb.reserve(sizeof(double) * 6 * (size_t)c.n_scenes);
+        hipLaunchKernelGGL(k_scene_aabb, dim3(c.n_scenes), dim3(BP_BLOCK), 0, s, X0, X1, c.nV,
+                           c.voff_d.as<int64_t>(), inflation, c.aabb.as<double>());
+        IPCG_CHECK_LAUNCH();
+        c.stage_end();
+        return;
+    }
     const int nb = (int)grid_for(c.nV, BP_BLOCK, 1024);
     c.aabb_partial.reserve(sizeof(double) * 6 * nb);
     c.aabb.reserve(sizeof(double) * 6);
@@ -859,6 +1126,8 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
     if (method != IPCG_BP_LBVH && method != IPCG_BP_SAP)
         throw Error(IPCG_ERR_ARG, "unsupported broad phase method");
     const bool sap = method == IPCG_BP_SAP;
+    if (sap && c.batched())
+        throw Error(IPCG_ERR_ARG, "a batched context supports the LBVH broad phase only");
     hipStream_t s = c.stream;
     const double* X0 = c.X[0].as<double>();
     const double* X1 = swept ? c.X[1].as<double>() : nullptr;
@@ -888,19 +1157,24 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         sap_list(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, axis);
     } else {
         // vertex order (queries of the face tree), edge tree, face tree
-        build_tree(c, c.tv, nullptr, 1, c.nV, X0, X1, inflation, false);
-        build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, true);
-        build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true);
+        const bool fo = c.batched();
+        build_tree(c, c.tv, nullptr, 1, c.nV, X0, X1, inflation, false, nullptr,
+                   fo ? &c.h_voff : nullptr);
+        build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, X1, inflation, true, nullptr,
+                   fo ? &c.h_eoff : nullptr);
+        build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true, nullptr,
+                   fo ? &c.h_foff : nullptr);
     }
     // codim lists (candidates.cpp:65-118): a tree over the codim vertices
     // (its own queries) and one over the codim edges; LBVH semantics with
     // either method
     const bool codim = c.n_cv > 0;
     if (codim) {
+        const bool fo = c.batched();
         build_tree(c, c.tcv, c.cv.as<int32_t>(), 1, c.n_cv, X0, X1, inflation, true,
-                   c.cv.as<int32_t>());
+                   c.cv.as<int32_t>(), fo ? &c.h_cvoff : nullptr);
         build_tree(c, c.tce, c.cev.as<int32_t>(), 2, c.n_ce, X0, X1, inflation, true,
-                   c.ce.as<int32_t>());
+                   c.ce.as<int32_t>(), fo ? &c.h_ceoff : nullptr);
     }
 
     c.counters.reserve(sizeof(unsigned long long) * 4);
@@ -950,42 +1224,55 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
             }
             c.stage_end();
         } else {
-            c.stage_begin("bp_traverse_ee");
-            if (c.nE >= 2)
-                hipLaunchKernelGGL((swept ? k_traverse<TR_EE, 1> : k_traverse<TR_EE, 0>), dim3(grid_for(c.nE, TR_BLOCK)),
-                                   dim3(TR_BLOCK), 0, s, c.te.box.as<float>(),
-                                   c.te.ids.as<uint32_t>(), c.nE, c.te.nodes.as<Node>(),
-                                   c.te.root.as<int32_t>(), c.te.leaf.as<int4>(),
-                                   c.te.box.as<float>(), c.nE, flt, nullptr,
-                                   c.cand_stage[0].as<int2>(), S0, deep, status);
-            c.stage_end();
-            c.stage_begin("bp_traverse_fv");
-            if (c.nF >= 1)
-                hipLaunchKernelGGL((swept ? k_traverse<TR_FV, 1> : k_traverse<TR_FV, 0>), dim3(grid_for(c.nV, TR_BLOCK)),
-                                   dim3(TR_BLOCK), 0, s, c.tv.box.as<float>(),
-                                   c.tv.ids.as<uint32_t>(), c.nV, c.tf.nodes.as<Node>(),
-                                   c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
-                                   c.tf.box.as<float>(), c.nF, flt, nullptr,
-                                   c.cand_stage[1].as<int2>(), S1, deep, status);
-            c.stage_end();
+            const std::string tag = swept ? ".swept" : ".static";
+            if (c.nE >= 2) {
+                const TraverseArgs a { c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
+                                       c.te.nodes.as<Node>(), c.te.root.as<int32_t>(),
+                                       c.te.leaf.as<int4>(), c.te.box.as<float>(), c.nE, flt,
+                                       nullptr, c.cand_stage[0].as<int2>(), S0, deep, status,
+                                       forest_args(c, c.te) };
+                c.stage_begin("bp_traverse_ee");
+                tuned_traverse(c, "bp_traverse_ee" + tag, cnt[0], [&](int b) {
+                    if (swept) launch_traverse<TR_EE, 1>(b, a, s);
+                    else launch_traverse<TR_EE, 0>(b, a, s);
+                });
+                c.stage_end();
+            }
+            if (c.nF >= 1) {
+                const TraverseArgs a { c.tv.box.as<float>(), c.tv.ids.as<uint32_t>(), c.nV,
+                                       c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(),
+                                       c.tf.leaf.as<int4>(), c.tf.box.as<float>(), c.nF, flt,
+                                       nullptr, c.cand_stage[1].as<int2>(), S1, deep, status,
+                                       forest_args(c, c.tf) };
+                c.stage_begin("bp_traverse_fv");
+                tuned_traverse(c, "bp_traverse_fv" + tag, cnt[1], [&](int b) {
+                    if (swept) launch_traverse<TR_FV, 1>(b, a, s);
+                    else launch_traverse<TR_FV, 0>(b, a, s);
+                });
+                c.stage_end();
+            }
         }
         if (codim) {
             c.stage_begin("bp_traverse_codim");
             const Shards S2 { cnt[2], c.scap_vv }, S3 { cnt[3], c.scap_ev };
-            if (c.n_cv >= 2)
-                hipLaunchKernelGGL((swept ? k_traverse<TR_VV, 1> : k_traverse<TR_VV, 0>), dim3(grid_for(c.n_cv, TR_BLOCK)),
-                                   dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
-                                   c.tcv.ids.as<uint32_t>(), c.n_cv, c.tcv.nodes.as<Node>(),
-                                   c.tcv.root.as<int32_t>(), c.tcv.leaf.as<int4>(),
-                                   c.tcv.box.as<float>(), c.n_cv, flt, c.cv.as<int32_t>(),
-                                   c.cand_stage[2].as<int2>(), S2, deep, status);
-            if (c.n_ce >= 1)
-                hipLaunchKernelGGL((swept ? k_traverse<TR_EV, 1> : k_traverse<TR_EV, 0>), dim3(grid_for(c.n_cv, TR_BLOCK)),
-                                   dim3(TR_BLOCK), 0, s, c.tcv.box.as<float>(),
-                                   c.tcv.ids.as<uint32_t>(), c.n_cv, c.tce.nodes.as<Node>(),
-                                   c.tce.root.as<int32_t>(), c.tce.leaf.as<int4>(),
-                                   c.tce.box.as<float>(), c.n_ce, flt, c.cv.as<int32_t>(),
-                                   c.cand_stage[3].as<int2>(), S3, deep, status);
+            if (c.n_cv >= 2) {
+                const TraverseArgs a { c.tcv.box.as<float>(), c.tcv.ids.as<uint32_t>(), c.n_cv,
+                                       c.tcv.nodes.as<Node>(), c.tcv.root.as<int32_t>(),
+                                       c.tcv.leaf.as<int4>(), c.tcv.box.as<float>(), c.n_cv, flt,
+                                       c.cv.as<int32_t>(), c.cand_stage[2].as<int2>(), S2, deep,
+                                       status, forest_args(c, c.tcv) };
+                if (swept) launch_traverse<TR_VV, 1>(TR_BLOCK, a, s);
+                else launch_traverse<TR_VV, 0>(TR_BLOCK, a, s);
+            }
+            if (c.n_ce >= 1) {
+                const TraverseArgs a { c.tcv.box.as<float>(), c.tcv.ids.as<uint32_t>(), c.n_cv,
+                                       c.tce.nodes.as<Node>(), c.tce.root.as<int32_t>(),
+                                       c.tce.leaf.as<int4>(), c.tce.box.as<float>(), c.n_ce, flt,
+                                       c.cv.as<int32_t>(), c.cand_stage[3].as<int2>(), S3, deep,
+                                       status, forest_args(c, c.tce) };
+                if (swept) launch_traverse<TR_EV, 1>(TR_BLOCK, a, s);
+                else launch_traverse<TR_EV, 0>(TR_BLOCK, a, s);
+            }
             c.stage_end();
         }
         IPCG_CHECK_LAUNCH();
@@ -1041,8 +1328,11 @@ void bp_edge_face(Ctx& c, double inflation)
         ~Tag() { c.stage_tag.clear(); }
     } tag(c);
     mesh_bounds(c, X0, nullptr, inflation);
-    build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, nullptr, inflation, false);
-    build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, nullptr, inflation, true);
+    const bool fo = c.batched();
+    build_tree(c, c.te, c.E.as<int32_t>(), 2, c.nE, X0, nullptr, inflation, false, nullptr,
+               fo ? &c.h_eoff : nullptr);
+    build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, nullptr, inflation, true, nullptr,
+               fo ? &c.h_foff : nullptr);
     Filter flt { c.E.as<int32_t>(), c.F.as<int32_t>(), c.has_patch ? c.patch.as<int32_t>() : nullptr,
                  c.n_dynamic, c.n_codim_filter };
     c.counters.reserve(sizeof(unsigned long long) * 4);
@@ -1058,11 +1348,11 @@ void bp_edge_face(Ctx& c, double inflation)
         IPCG_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
         const Shards S0 { cnt, c.scap_ef };
         c.stage_begin("bp_traverse_ef");
-        hipLaunchKernelGGL((k_traverse<TR_EF, 0>), dim3(grid_for(c.nE, TR_BLOCK)), dim3(TR_BLOCK), 0,
-                           s, c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
-                           c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
-                           c.tf.box.as<float>(), c.nF, flt, nullptr, c.cand_stage[0].as<int2>(), S0,
-                           c.tr_deep.as<int>(), status);
+        const TraverseArgs a { c.te.box.as<float>(), c.te.ids.as<uint32_t>(), c.nE,
+                               c.tf.nodes.as<Node>(), c.tf.root.as<int32_t>(), c.tf.leaf.as<int4>(),
+                               c.tf.box.as<float>(), c.nF, flt, nullptr, c.cand_stage[0].as<int2>(),
+                               S0, c.tr_deep.as<int>(), status, forest_args(c, c.tf) };
+        launch_traverse<TR_EF, 0>(TR_BLOCK, a, s);
         IPCG_CHECK_LAUNCH();
         c.stage_end();
         int* hs = static_cast<int*>(c.pinned);

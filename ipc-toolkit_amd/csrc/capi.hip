@@ -11,9 +11,6 @@
 
 using namespace ipcg;
 
-struct ipcg_ctx {
-    Ctx c;
-};
 
 namespace ipcg {
 
@@ -148,13 +145,10 @@ int ipcg_ctx_create(int device, ipcg_ctx** out)
     return IPCG_OK;
 }
 
-int ipcg_ctx_destroy(ipcg_ctx* ctx)
+// every device buffer a context owns
+static std::vector<DevBuf*> all_bufs(Ctx& c)
 {
-    if (!ctx) return IPCG_ERR_ARG;
-    Ctx& c = ctx->c;
-    (void)hipSetDevice(c.device);
-    if (c.stream) (void)hipStreamSynchronize(c.stream);
-    DevBuf* bufs[] = { &c.E, &c.F, &c.F2E, &c.patch, &c.rest, &c.X[0], &c.X[1],
+    std::vector<DevBuf*> bufs = { &c.E, &c.F, &c.F2E, &c.patch, &c.rest, &c.X[0], &c.X[1],
                        &c.tv.box, &c.tv.keys, &c.tv.keys2, &c.tv.ids, &c.tv.ids2, &c.tv.nodes, &c.tv.info, &c.tv.root,
                        &c.te.box, &c.te.keys, &c.te.keys2, &c.te.ids, &c.te.ids2, &c.te.nodes, &c.te.info, &c.te.root,
                        &c.tf.box, &c.tf.keys, &c.tf.keys2, &c.tf.ids, &c.tf.ids2, &c.tf.nodes, &c.tf.info, &c.tf.root,
@@ -182,7 +176,36 @@ int ipcg_ctx_destroy(ipcg_ctx* ctx)
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
                            &T->root, &T->sbox, &T->leaf })
             b->release();
-    for (DevBuf* b : bufs) b->release();
+    for (Tree* T : { &c.tcv, &c.tce })
+        for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
+                           &T->root, &T->sbox, &T->leaf })
+            bufs.push_back(b);
+    for (Tree* T : { &c.tv, &c.te, &c.tf, &c.tcv, &c.tce })
+        for (DevBuf* b : { &T->mort, &T->lscene, &T->loff }) bufs.push_back(b);
+    for (DevBuf* b : { &c.vscene, &c.voff_d, &c.scene_active, &c.coff_d, &c.scene_part,
+                       &c.ccd_sbound })
+        bufs.push_back(b);
+    return bufs;
+}
+
+extern "C++" {
+namespace ipcg {
+int64_t device_bytes(Ctx& c)
+{
+    int64_t n = 0;
+    for (DevBuf* b : all_bufs(c)) n += (int64_t)b->bytes;
+    return n;
+}
+} // namespace ipcg
+}
+
+int ipcg_ctx_destroy(ipcg_ctx* ctx)
+{
+    if (!ctx) return IPCG_ERR_ARG;
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (DevBuf* b : all_bufs(c)) b->release();
     for (auto& p : c.timer.pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -284,11 +307,14 @@ static void upload_adjacencies(Ctx& c, int32_t nV, const int32_t* edges, int32_t
     IPCG_HIP(hipStreamSynchronize(c.stream)); // the host vectors go out of scope
 }
 
-int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, int32_t nE,
-                     const int32_t* edges, int32_t nF, const int32_t* faces,
-                     const int32_t* vertex_patch)
+// The collision mesh (any number of scenes: their topology holds global
+// vertex ids); cv_out / ce_out receive the codim vertices and edges.
+static void mesh_upload(Ctx& c, int32_t nV, const double* rest, int layout, int32_t nE,
+                        const int32_t* edges, int32_t nF, const int32_t* faces,
+                        const int32_t* vertex_patch, std::vector<int32_t>* cv_out = nullptr,
+                        std::vector<int32_t>* ce_out = nullptr)
 {
-    return guard(ctx, [&](Ctx& c) {
+    {
         if (nV < 0 || nE < 0 || nF < 0 || (nV > 0 && !rest) || (nE > 0 && !edges)
             || (nF > 0 && !faces))
             throw Error(IPCG_ERR_ARG, "invalid mesh sizes or null arrays");
@@ -406,8 +432,191 @@ int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, 
         c.has_mesh = true;
         c.has_pos[0] = c.has_pos[1] = false;
         c.cand_valid = c.coll_valid = c.hess_valid = false;
+        c.coff_valid = false;
         c.fric_valid = c.has_vel = false;
         c.scap_ee = c.scap_fv = c.scap_vv = c.scap_ev = 0;
+        if (cv_out) *cv_out = cv;
+        if (ce_out) *ce_out = ce;
+    }
+}
+
+int ipcg_mesh_upload(ipcg_ctx* ctx, int32_t nV, const double* rest, int layout, int32_t nE,
+                     const int32_t* edges, int32_t nF, const int32_t* faces,
+                     const int32_t* vertex_patch)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.n_scenes = 0;
+        mesh_upload(c, nV, rest, layout, nE, edges, nF, faces, vertex_patch);
+    });
+}
+
+// first element of every scene among the sorted ids, n_scenes + 1 entries
+static std::vector<int64_t> scene_starts(const std::vector<int32_t>& ids,
+                                         const std::vector<int64_t>& voff)
+{
+    std::vector<int64_t> out(voff.size());
+    for (size_t k = 0; k < voff.size(); k++)
+        out[k] = std::lower_bound(ids.begin(), ids.end(), (int32_t)voff[k]) - ids.begin();
+    return out;
+}
+
+extern "C++" {
+namespace ipcg {
+
+void mesh_upload_batched(Ctx& c, int32_t n_scenes, const int64_t* vertex_offsets,
+                         const double* rest, int layout, const int64_t* edge_offsets,
+                         const int32_t* edges, const int64_t* face_offsets, const int32_t* faces)
+{
+        if (n_scenes <= 0 || !vertex_offsets || !edge_offsets || !face_offsets)
+            throw Error(IPCG_ERR_ARG, "a batch needs at least one scene and its offsets");
+        const int64_t* off[3] = { vertex_offsets, edge_offsets, face_offsets };
+        for (int a = 0; a < 3; a++) {
+            if (off[a][0] != 0) throw Error(IPCG_ERR_ARG, "scene offsets must start at 0");
+            for (int32_t k = 0; k < n_scenes; k++)
+                if (off[a][k + 1] < off[a][k])
+                    throw Error(IPCG_ERR_ARG, "scene offsets must not decrease");
+            if (off[a][n_scenes] >= INT32_MAX)
+                throw Error(IPCG_ERR_ARG, "too many vertices, edges or faces for one batch");
+        }
+        const int32_t nV = (int32_t)vertex_offsets[n_scenes];
+        const int32_t nE = (int32_t)edge_offsets[n_scenes];
+        const int32_t nF = (int32_t)face_offsets[n_scenes];
+        if ((nE > 0 && !edges) || (nF > 0 && !faces) || (nV > 0 && !rest))
+            throw Error(IPCG_ERR_ARG, "null mesh arrays");
+        // scene-local vertex ids -> global ids, each within its own scene
+        std::vector<int32_t> ge(2 * (size_t)nE), gf(3 * (size_t)nF);
+        for (int32_t k = 0; k < n_scenes; k++) {
+            const int64_t v0 = vertex_offsets[k], nv = vertex_offsets[k + 1] - v0;
+            for (int64_t i = 2 * edge_offsets[k]; i < 2 * edge_offsets[k + 1]; i++) {
+                if (edges[i] < 0 || edges[i] >= nv)
+                    throw Error(IPCG_ERR_ARG, "edge vertex id out of its scene's range");
+                ge[i] = (int32_t)(edges[i] + v0);
+            }
+            for (int64_t i = 3 * face_offsets[k]; i < 3 * face_offsets[k + 1]; i++) {
+                if (faces[i] < 0 || faces[i] >= nv)
+                    throw Error(IPCG_ERR_ARG, "face vertex id out of its scene's range");
+                gf[i] = (int32_t)(faces[i] + v0);
+            }
+        }
+        std::vector<int32_t> cv, ce;
+        c.n_scenes = 0;
+        mesh_upload(c, nV, rest, layout, nE, ge.data(), nF, gf.data(), nullptr, &cv, &ce);
+        const size_t S1 = (size_t)n_scenes + 1;
+        c.h_voff.assign(vertex_offsets, vertex_offsets + S1);
+        c.h_eoff.assign(edge_offsets, edge_offsets + S1);
+        c.h_foff.assign(face_offsets, face_offsets + S1);
+        // codim vertices are ascending vertex ids, codim edges ascending edge
+        // ids: each scene's are contiguous
+        c.h_cvoff = scene_starts(cv, c.h_voff);
+        c.h_ceoff = scene_starts(ce, c.h_eoff);
+        std::vector<int32_t> vs((size_t)std::max(nV, 1), 0);
+        for (int32_t k = 0; k < n_scenes; k++)
+            for (int64_t v = vertex_offsets[k]; v < vertex_offsets[k + 1]; v++) vs[v] = k;
+        std::vector<int64_t> so(3 * S1);
+        std::copy(c.h_voff.begin(), c.h_voff.end(), so.begin());
+        std::copy(c.h_eoff.begin(), c.h_eoff.end(), so.begin() + S1);
+        std::copy(c.h_foff.begin(), c.h_foff.end(), so.begin() + 2 * S1);
+        c.vscene.reserve(sizeof(int32_t) * vs.size());
+        c.voff_d.reserve(sizeof(int64_t) * so.size());
+        c.scene_active.reserve(S1);
+        IPCG_HIP(hipMemcpyAsync(c.vscene.p, vs.data(), sizeof(int32_t) * vs.size(),
+                                hipMemcpyHostToDevice, c.stream));
+        IPCG_HIP(hipMemcpyAsync(c.voff_d.p, so.data(), sizeof(int64_t) * so.size(),
+                                hipMemcpyHostToDevice, c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream)); // the host vectors go out of scope
+        c.all_active = true;
+        c.h_active.assign((size_t)n_scenes, 1);
+        c.n_scenes = n_scenes;
+}
+
+void batch_set_active(Ctx& c, const uint8_t* active)
+{
+        if (!c.batched()) throw Error(IPCG_ERR_STATE, "not a batched context");
+        if (active) c.h_active.assign(active, active + c.n_scenes);
+        else c.h_active.assign((size_t)c.n_scenes, 1);
+        c.all_active = std::all_of(c.h_active.begin(), c.h_active.end(), [](uint8_t a) { return a != 0; });
+        if (!c.all_active) {
+            IPCG_HIP(hipMemcpyAsync(c.scene_active.p, c.h_active.data(), c.h_active.size(),
+                                    hipMemcpyHostToDevice, c.stream));
+            IPCG_HIP(hipStreamSynchronize(c.stream));
+        }
+        c.cand_valid = c.coll_valid = c.hess_valid = false;
+        c.coff_valid = c.fric_valid = false;
+}
+
+} // namespace ipcg
+}
+
+int ipcg_mesh_upload_batched(ipcg_ctx* ctx, int32_t n_scenes, const int64_t* vertex_offsets,
+                             const double* rest, int layout, const int64_t* edge_offsets,
+                             const int32_t* edges, const int64_t* face_offsets,
+                             const int32_t* faces)
+{
+    return guard(ctx, [&](Ctx& c) {
+        mesh_upload_batched(c, n_scenes, vertex_offsets, rest, layout, edge_offsets, edges,
+                            face_offsets, faces);
+    });
+}
+
+int ipcg_ctx_device_bytes(ipcg_ctx* ctx, int64_t* bytes)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!bytes) throw Error(IPCG_ERR_ARG, "null output");
+        *bytes = device_bytes(c);
+    });
+}
+
+int ipcg_ctx_set_autotune(ipcg_ctx* ctx, int enable)
+{
+    return guard(ctx, [&](Ctx& c) {
+        c.autotune = enable != 0;
+        if (!c.autotune) c.launch_block.clear();
+    });
+}
+
+int ipcg_launch_shape(ipcg_ctx* ctx, const char* stage, int32_t* block, double* ms)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!stage || !block) throw Error(IPCG_ERR_ARG, "null argument");
+        auto it = c.launch_block.find(stage);
+        *block = it == c.launch_block.end() ? 0 : it->second;
+        if (ms) {
+            auto t = c.tune_ms.find(std::string(stage) + "@" + std::to_string(*block));
+            *ms = t == c.tune_ms.end() ? 0.0 : (double)t->second;
+        }
+    });
+}
+
+int ipcg_batch_size(ipcg_ctx* ctx, int32_t* n_scenes)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!n_scenes) throw Error(IPCG_ERR_ARG, "null output");
+        *n_scenes = c.n_scenes;
+    });
+}
+
+int ipcg_batch_set_active(ipcg_ctx* ctx, const uint8_t* active)
+{
+    return guard(ctx, [&](Ctx& c) { batch_set_active(c, active); });
+}
+
+int ipcg_collisions_count_batched(ipcg_ctx* ctx, int64_t* counts)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!counts) throw Error(IPCG_ERR_ARG, "null output");
+        scene_collision_offsets(c);
+        const size_t S1 = (size_t)c.n_scenes + 1;
+        for (int32_t sc = 0; sc < c.n_scenes; sc++)
+            for (int k = 0; k < 4; k++)
+                counts[4 * (size_t)sc + k] = c.h_coff[k * S1 + sc + 1] - c.h_coff[k * S1 + sc];
+    });
+}
+
+int ipcg_barrier_energy_batched(ipcg_ctx* ctx, double dhat, double kappa, double* energies)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!energies) throw Error(IPCG_ERR_ARG, "null output");
+        pot_energy_scenes(c, dhat, kappa, energies);
     });
 }
 
@@ -768,6 +977,19 @@ int ipcg_compute_collision_free_stepsize(ipcg_ctx* ctx, double min_distance, dou
         if (!step) throw Error(IPCG_ERR_ARG, "null output");
         bp_build(c, true, 0.5 * min_distance, method);
         *step = ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr);
+    });
+}
+
+int ipcg_compute_collision_free_stepsize_batched(ipcg_ctx* ctx, double min_distance,
+                                                 double tolerance, int64_t max_iterations,
+                                                 double rescale, double* steps)
+{
+    return guard(ctx, [&](Ctx& c) {
+        if (!c.batched()) throw Error(IPCG_ERR_STATE, "not a batched context");
+        check_ccd_args(min_distance, tolerance, rescale);
+        if (!steps) throw Error(IPCG_ERR_ARG, "null output");
+        bp_build(c, true, 0.5 * min_distance, IPCG_BP_LBVH);
+        ccd_stepsize(c, min_distance, tolerance, max_iterations, rescale, false, nullptr, steps);
     });
 }
 

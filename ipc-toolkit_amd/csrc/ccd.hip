@@ -55,7 +55,8 @@ struct CCDArgs {
     double min_distance, tolerance, rescale;
     long max_iter;
     bool bounded;          // hide outcomes >= running minimum
-    double* bound;         // running minimum (double bits)
+    double* bound;         // running minimum (double bits), one per scene
+    const int32_t* vscene; // batched contexts: scene per vertex (null: one scene)
     double* toi_out;       // per-candidate toi (null: not recorded)
     int* any_hit;          // early exit flag (null: no early exit)
 };
@@ -65,10 +66,11 @@ __device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
     atomicMin(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
 }
 
-__device__ __forceinline__ void report(const CCDArgs& A, int64_t ci, bool hit, double toi)
+__device__ __forceinline__ void report(const CCDArgs& A, int64_t ci, bool hit, double toi,
+                                       int scene)
 {
     if (hit) {
-        if (A.bounded) atomic_min_nonneg(A.bound, toi);
+        if (A.bounded) atomic_min_nonneg(A.bound + scene, toi);
         if (A.any_hit) atomicOr(A.any_hit, 1);
     }
     if (A.toi_out) A.toi_out[ci] = hit ? toi : INFINITY;
@@ -76,8 +78,9 @@ __device__ __forceinline__ void report(const CCDArgs& A, int64_t ci, bool hit, d
 
 // Strategy prologue (ccd_strategy + *_ccd of tight_inclusion_ccd.cpp):
 // gathers the stencil, returns 1 (hit at toi = 0), 0 (miss) or -1 (search
-// needed, q / eff / adj set).
-__device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double& eff, double& adj)
+// needed, q / eff / adj set); `scene` receives the candidate's scene.
+__device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double& eff, double& adj,
+                               int& scene)
 {
     // candidates [0, n_ee) edge-edge, then face-vertex, codim vertex-vertex
     // and codim edge-vertex; point-point and point-edge run as degenerate
@@ -112,6 +115,7 @@ __device__ int candidate_setup(const CCDArgs& A, int64_t ci, TIQuery& q, double&
         ids[3] = A.E[2 * (int64_t)c.x + 1];
     }
     const bool is_ee = kind != 1;
+    scene = A.vscene ? A.vscene[ids[0]] : 0;
     d3 p0[4];
     bool moving = false;
 #pragma unroll
@@ -159,15 +163,16 @@ __global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t*
     if (ci < A.n_ee + A.n_fv + A.n_vv + A.n_ev) {
         TIQuery q;
         double eff = 0, adj = 0;
-        const int pre = candidate_setup(A, ci, q, eff, adj);
+        int sc = 0;
+        const int pre = candidate_setup(A, ci, q, eff, adj, sc);
         if (pre >= 0) {
-            report(A, ci, pre == 1, 0.0);
+            report(A, ci, pre == 1, 0.0, sc);
         } else {
             ti_filter(q, eff > 0);
             bool box_in;
             double w[3];
             search = ti_inclusion(q, TIBox { 0.0, 0.0, 0.0 }, 1.0, 1.0, 1.0, eff, box_in, w);
-            if (!search) report(A, ci, false, INFINITY);
+            if (!search) report(A, ci, false, INFINITY, sc);
         }
     }
     const int sh = wave_shard();
@@ -191,15 +196,16 @@ __global__ __launch_bounds__(CB) void k_ccd_lds(CCDArgs A, const uint32_t* todo,
     const int64_t ci = todo[k];
     TIQuery q;
     double eff = 0, adj = 0, toi = INFINITY;
-    const int pre = candidate_setup(A, ci, q, eff, adj);
+    int sc = 0;
+    const int pre = candidate_setup(A, ci, q, eff, adj, sc);
     if (pre >= 0) {
-        report(A, ci, pre == 1, 0.0);
+        report(A, ci, pre == 1, 0.0, sc);
         return;
     }
-    TIBound B1 { A.bounded ? A.bound : nullptr, 1.0, SMALL_TOI };
+    TIBound B1 { A.bounded ? A.bound + sc : nullptr, 1.0, SMALL_TOI };
     int r = ti_query_lds(q, eff, adj, A.max_iter, false, B1, L, toi);
     if (r == TI_HIT && toi < SMALL_TOI) {
-        TIBound B2 { A.bounded ? A.bound : nullptr, (1.0 / A.rescale) * (1 + 1e-12), 0.0 };
+        TIBound B2 { A.bounded ? A.bound + sc : nullptr, (1.0 / A.rescale) * (1 + 1e-12), 0.0 };
         r = ti_query_lds(q, A.min_distance, adj, -1, true, B2, L, toi);
         if (r == TI_HIT) toi *= A.rescale;
     }
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(CB) void k_ccd_lds(CCDArgs A, const uint32_t* todo,
         spill[atomicAdd(n_spill, 1ull)] = (uint32_t)ci;
         return;
     }
-    report(A, ci, r == TI_HIT, toi);
+    report(A, ci, r == TI_HIT, toi, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -231,15 +237,16 @@ __global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, const uint32_t* tod
         const int64_t ci = todo[k];
         TIQuery q;
         double eff = 0, adj = 0, toi = INFINITY;
-        const int pre = candidate_setup(A, ci, q, eff, adj);
+        int sc = 0;
+        const int pre = candidate_setup(A, ci, q, eff, adj, sc);
         if (pre >= 0) {
-            report(A, ci, pre == 1, 0.0);
+            report(A, ci, pre == 1, 0.0, sc);
             continue;
         }
-        TIBound B1 { A.bounded ? A.bound : nullptr, 1.0, SMALL_TOI };
+        TIBound B1 { A.bounded ? A.bound + sc : nullptr, 1.0, SMALL_TOI };
         int r = ti_query(q, eff, adj, A.max_iter, false, B1, S, toi);
         if (r == TI_HIT && toi < SMALL_TOI) {
-            TIBound B2 { A.bounded ? A.bound : nullptr, (1.0 / A.rescale) * (1 + 1e-12), 0.0 };
+            TIBound B2 { A.bounded ? A.bound + sc : nullptr, (1.0 / A.rescale) * (1 + 1e-12), 0.0 };
             r = ti_query(q, A.min_distance, adj, -1, true, B2, S, toi);
             if (r == TI_HIT) toi *= A.rescale;
         }
@@ -247,7 +254,7 @@ __global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, const uint32_t* tod
             heavy[atomicAdd(n_heavy, 1ull)] = (uint32_t)ci;
             continue;
         }
-        report(A, ci, r == TI_HIT, toi);
+        report(A, ci, r == TI_HIT, toi, sc);
     }
 }
 
@@ -269,6 +276,7 @@ struct WFState {
     long max_itr;
     int stage, nz_iter, levels, axis;
     int use_skip, done, hit, restart;
+    int scene, pad_;
     int64_t seg_start, seg_count;   // boxes of the current level
     int64_t next_start, next_count; // boxes of the next level
 };
@@ -300,7 +308,7 @@ __global__ void k_wf_init(CCDArgs A, const uint32_t* heavy, int64_t first, int64
     WFState S;
     S.ci = heavy[first + i];
     double eff = 0, adj = 0;
-    const int pre = candidate_setup(A, S.ci, S.q, eff, adj);
+    const int pre = candidate_setup(A, S.ci, S.q, eff, adj, S.scene);
     S.done = 0;
     S.hit = 0;
     S.result = INFINITY;
@@ -328,7 +336,8 @@ __global__ void k_wf_init(CCDArgs A, const uint32_t* heavy, int64_t first, int64
 __device__ __forceinline__ double wf_bound(const CCDArgs& A, const WFState& S)
 {
     if (!A.bounded) return INFINITY;
-    const double g = __hip_atomic_load(A.bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double g = __hip_atomic_load(A.bound + S.scene, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
     if (S.stage == 1) return g > SMALL_TOI ? g : SMALL_TOI;
     return g * ((1.0 / A.rescale) * (1 + 1e-12));
 }
@@ -587,7 +596,7 @@ __global__ void k_wf_finish(CCDArgs A, const WFState* st, int64_t nb)
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
     const WFState& S = st[i];
-    report(A, S.ci, S.hit != 0, S.result);
+    report(A, S.ci, S.hit != 0, S.result, S.scene);
 }
 
 // n[0]: candidates still searching, n[1]: of those, restarting at a root box
@@ -670,8 +679,14 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
 
 } // namespace
 
+__global__ void k_fill_f64(double* p, int n, double v)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, double rescale,
-                    bool early_exit, double* const* toi_out)
+                    bool early_exit, double* const* toi_out, double* scene_steps)
 {
     c.require_mesh();
     c.require_pos(0);
@@ -682,7 +697,11 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     hipStream_t s = c.stream;
     const int64_t n = c.n_ee + c.n_fv + c.n_vv + c.n_ev;
     const bool per_cand = toi_out != nullptr;
-    if (n == 0) return 1.0;
+    const int nsc = c.batched() ? c.n_scenes : 1;
+    if (n == 0) {
+        if (scene_steps) std::fill(scene_steps, scene_steps + nsc, 1.0);
+        return 1.0;
+    }
     if (n >= (1ll << 32)) throw Error(IPCG_ERR_ARG, "too many CCD candidates");
     // device scalars: bound, work counter, heavy count, hit flag, error, roots
     c.ccd_status.reserve(512);
@@ -697,7 +716,14 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     unsigned long long* n_spill = reinterpret_cast<unsigned long long*>(st + 448);
     const double one = 1.0;
     IPCG_HIP(hipMemsetAsync(st, 0, 512, s));
-    IPCG_HIP(hipMemcpyAsync(bound, &one, sizeof(double), hipMemcpyHostToDevice, s));
+    if (c.batched()) { // one running minimum per scene
+        c.ccd_sbound.reserve(sizeof(double) * (size_t)nsc);
+        bound = c.ccd_sbound.as<double>();
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(nsc, 256)), dim3(256), 0, s, bound, nsc, 1.0);
+        IPCG_CHECK_LAUNCH();
+    } else {
+        IPCG_HIP(hipMemcpyAsync(bound, &one, sizeof(double), hipMemcpyHostToDevice, s));
+    }
     c.ccd_overflow.reserve(sizeof(uint32_t) * (size_t)n);
     uint32_t* heavy = c.ccd_overflow.as<uint32_t>();
     if (per_cand) c.ccd_toi.reserve(sizeof(double) * (size_t)n);
@@ -723,6 +749,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     A.max_iter = (long)max_iter;
     A.bounded = !per_cand;
     A.bound = bound;
+    A.vscene = c.batched() ? c.vscene.as<int32_t>() : nullptr;
     A.toi_out = per_cand ? c.ccd_toi.as<double>() : nullptr;
     A.any_hit = early_exit ? any_hit : nullptr;
     unsigned long long* h = static_cast<unsigned long long*>(c.pinned);
@@ -822,11 +849,14 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
         return 1.0;
     }
     double* hd = static_cast<double*>(c.pinned);
-    IPCG_HIP(hipMemcpyAsync(hd, bound, sizeof(double), hipMemcpyDeviceToHost, s));
-    IPCG_HIP(hipMemcpyAsync(hd + 1, any_hit, sizeof(int), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipMemcpyAsync(hd, any_hit, sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<double> per_scene((size_t)nsc);
+    IPCG_HIP(hipMemcpyAsync(per_scene.data(), bound, sizeof(double) * (size_t)nsc,
+                            hipMemcpyDeviceToHost, s));
     IPCG_HIP(hipStreamSynchronize(s));
-    if (early_exit) return reinterpret_cast<int*>(hd + 1)[0] ? 0.0 : 1.0;
-    return hd[0];
+    if (early_exit) return reinterpret_cast<int*>(hd)[0] ? 0.0 : 1.0;
+    if (scene_steps) std::copy(per_scene.begin(), per_scene.end(), scene_steps);
+    return *std::min_element(per_scene.begin(), per_scene.end());
 }
 
 } // namespace ipcg

@@ -40,7 +40,16 @@ struct Tree {
     DevBuf root;         // int32 root node (-1: single leaf)
     DevBuf sbox;         // sweep and prune: FP64 boxes in sorted order
     DevBuf leaf;         // int4 per sorted leaf: (mesh id, its vertices, -1 padded)
+    // forest of a batched context: Morton code per primitive, scene per
+    // sorted leaf; `root` then holds one entry per scene
+    DevBuf mort, lscene, loff;
 };
+
+// Root entries of a batched context's forest: an internal node (>= 0), a
+// single leaf (~position), no primitive of this kind in the scene, or not
+// yet completed by the refit.
+constexpr int32_t ROOT_NONE = INT32_MIN + 1;
+constexpr int32_t ROOT_UNFINISHED = INT32_MIN;
 
 struct Ctx {
     int device = 0;
@@ -64,6 +73,23 @@ struct Ctx {
     DevBuf adj_vv_off, adj_vv, adj_ve_off, adj_ve, adj_ev_off, adj_ev, adj_vf_off, adj_bnd;
     DevBuf X[2];
     bool has_pos[2] = { false, false };
+
+    // Batched context: independent scenes concatenated; scene s owns
+    // vertices [voff[s], voff[s+1]) and likewise edges, faces, codim
+    // vertices and codim edges.  n_scenes = 0: one scene, no scene arrays.
+    int32_t n_scenes = 0;
+    std::vector<int64_t> h_voff, h_eoff, h_foff, h_cvoff, h_ceoff;
+    DevBuf vscene;              // int32 scene per vertex
+    DevBuf voff_d;              // int64 vertex offsets (n_scenes + 1)
+    DevBuf scene_active;        // uint8 per scene: queries of inactive scenes are skipped
+    bool all_active = true;
+    std::vector<uint8_t> h_active;
+    bool batched() const { return n_scenes > 0; }
+    // per-scene collision offsets per kind (4 x (n_scenes + 1), kind-major),
+    // valid with the collision set
+    bool coff_valid = false;
+    std::vector<int64_t> h_coff;
+    DevBuf coff_d, scene_part;
 
     // codim vertices (in no edge) and codim edges (in no face), their
     // endpoint pairs (collision_mesh.cpp:145-182)
@@ -125,9 +151,16 @@ struct Ctx {
     // ccd
     DevBuf ccd_bound, ccd_slab, ccd_overflow, ccd_toi, ccd_status;
     DevBuf ccd_stage, ccd_todo; // root-pass survivors (sharded, packed)
+    DevBuf ccd_sbound;          // batched contexts: running minimum per scene
 
     // host pinned staging (PINNED_BYTES)
     void* pinned = nullptr;
+
+    // launch shapes: workgroup size per tuned stage (absent: the default),
+    // filled on first use when autotune is on; tune_ms: each size's time
+    bool autotune = false;
+    std::map<std::string, int> launch_block;
+    std::map<std::string, float> tune_ms;
 
     StageTimer timer;
     void stage_begin(const char* name);
@@ -149,13 +182,26 @@ double pot_energy(Ctx& c, double dhat, double kappa);
 // energy to device memory (no synchronization)
 constexpr int64_t NOALLOC_PARTIALS = 1 << 20;
 void noalloc_init(int device);
+// batched contexts: upload (scene-local topology, ragged offsets), the
+// active-scene mask, the bytes of device memory a context holds
+void mesh_upload_batched(Ctx& c, int32_t n_scenes, const int64_t* vertex_offsets,
+                         const double* rest, int layout, const int64_t* edge_offsets,
+                         const int32_t* edges, const int64_t* face_offsets, const int32_t* faces);
+void batch_set_active(Ctx& c, const uint8_t* active);
+int64_t device_bytes(Ctx& c);
+// batched contexts: per-scene collision offsets and energies (n_scenes
+// values); ccd_stepsize's scene_steps receives the per-scene step sizes
+void scene_collision_offsets(Ctx& c);
+void pot_energy_scenes(Ctx& c, double dhat, double kappa, double* out);
+
 void pot_energy_noalloc(Ctx& c, double dhat, double kappa, double* d_out);
 void pot_gradient(Ctx& c, double dhat, double kappa);
 void pot_hessian(Ctx& c, double dhat, double kappa, int psd);
 // toi_out: null (step size) or 4 host arrays (ee, fv, vv, ev; any null)
 // receiving per-candidate times of impact
 double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter,
-                    double rescale, bool early_exit, double* const* toi_out);
+                    double rescale, bool early_exit, double* const* toi_out,
+                    double* scene_steps = nullptr);
 void bp_edge_face(Ctx& c, double inflation);
 void fric_build(Ctx& c, double dhat, double kappa, double mu_s, double mu_k,
                 const double* mu_sv = nullptr, const double* mu_kv = nullptr);
@@ -174,3 +220,8 @@ double cand_noncandidate_stepsize(Ctx& c, double dhat);
 bool has_intersections(Ctx& c);
 
 } // namespace ipcg
+
+// the C ABI's opaque context
+struct ipcg_ctx {
+    ipcg::Ctx c;
+};

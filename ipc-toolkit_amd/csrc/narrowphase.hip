@@ -729,6 +729,7 @@ void np_build(Ctx& c, double dhat, double dmin)
         throw Error(IPCG_ERR_STATE, "no static candidate set (run the static broad phase)");
     hipStream_t s = c.stream;
     c.coll_valid = false;
+    c.coff_valid = false;
     const int64_t n = c.n_ee + c.n_fv + c.n_vv + c.n_ev;
     const int64_t waves = (n + WAVE - 1) / WAVE;
     const int64_t scap = std::max<int64_t>(WAVE * ((waves + SHARDS - 1) / SHARDS), WAVE);

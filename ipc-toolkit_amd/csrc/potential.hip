@@ -1,5 +1,5 @@
 // Barrier potential over the device collision set: energy (fixed-order
-// reduction), gradient (FP64 atomic scatter, RowMajor), and the
+// reduction), gradient (per-vertex sums in a fixed order, RowMajor), and the
 // PSD-projected Hessian assembled to CSR (potential.cpp:36-222).
 //
 // Hessian assembly (no global atomics).  Symbolic, once per collision set:
@@ -64,21 +64,22 @@ __device__ __forceinline__ void stencil_ids(const CollView& C, int64_t i, int* v
 }
 
 template <int KIND>
-__global__ __launch_bounds__(PB) void k_energy(CollView C, const double* X, int nV,
-                                               BarrierParams bp, double* partial)
+__device__ __forceinline__ double collision_energy(const CollView& C, const double* X, int nV,
+                                                   const BarrierParams& bp, int64_t i)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
-    const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
-    double e = 0;
-    if (i < C.n[KIND]) {
-        int v[4];
-        stencil_ids<KIND>(C, i, v);
-        d3 P[4];
+    int v[4];
+    stencil_ids<KIND>(C, i, v);
+    d3 P[4];
 #pragma unroll
-        for (int j = 0; j < NP; j++) P[j] = ldx(X, nV, v[j]);
-        e = stencil_energy(KIND, KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
-                           KIND == K_EE ? C.eps[i] : 0.0, bp);
-    }
+    for (int j = 0; j < NP; j++) P[j] = ldx(X, nV, v[j]);
+    return stencil_energy(KIND, KIND == K_EE ? C.dtype[i] : 0, P, C.w[KIND][i],
+                          KIND == K_EE ? C.eps[i] : 0.0, bp);
+}
+
+// sum of the block's lane values (waves reduce by shuffles, then in order)
+__device__ __forceinline__ void block_partial(double e, double* partial)
+{
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, WAVE);
     __shared__ double s[PB / WAVE];
     if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = e;
@@ -90,7 +91,75 @@ __global__ __launch_bounds__(PB) void k_energy(CollView C, const double* X, int 
     }
 }
 
-__global__ __launch_bounds__(256) void k_sum(const double* partial, int64_t n, double* out)
+template <int KIND>
+__global__ __launch_bounds__(PB) void k_energy(CollView C, const double* X, int nV,
+                                               BarrierParams bp, double* partial)
+{
+    const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    double e = 0;
+    if (i < C.n[KIND]) e = collision_energy<KIND>(C, X, nV, bp, i);
+    block_partial(e, partial);
+}
+
+// Batched contexts: the blocks of every (scene, kind) range, scene-major
+// (boff: first block of each of the 4 n_scenes ranges, then the total), so
+// each scene's partials are the ones a context holding the scene alone
+// computes, in the same order.
+__global__ __launch_bounds__(PB) void k_energy_scenes(CollView C, const double* X, int nV,
+                                                      BarrierParams bp, const int64_t* coff,
+                                                      const int64_t* boff, int S, double* partial)
+{
+    __shared__ int64_t first, last;
+    __shared__ int kind;
+    const int64_t b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = 4 * S; // the last range whose first block is <= b
+        while (hi - lo > 1) {
+            const int m = (lo + hi) >> 1;
+            if (boff[m] <= b) lo = m;
+            else hi = m;
+        }
+        const int sc = lo / 4, k = lo % 4;
+        kind = k;
+        first = coff[k * (int64_t)(S + 1) + sc] + (b - boff[lo]) * PB;
+        last = coff[k * (int64_t)(S + 1) + sc + 1];
+    }
+    __syncthreads();
+    const int64_t i = first + threadIdx.x;
+    double e = 0;
+    if (i < last) {
+        switch (kind) {
+        case K_VV: e = collision_energy<K_VV>(C, X, nV, bp, i); break;
+        case K_EV: e = collision_energy<K_EV>(C, X, nV, bp, i); break;
+        case K_EE: e = collision_energy<K_EE>(C, X, nV, bp, i); break;
+        default: e = collision_energy<K_FV>(C, X, nV, bp, i); break;
+        }
+    }
+    block_partial(e, partial);
+}
+
+// first collision of every scene per kind (kind-major, n_scenes + 1 each)
+// by binary search of the canonically sorted lists (soff: first vertex,
+// edge and face of every scene, n_scenes + 1 each)
+__global__ void k_scene_coff(CollView C, const int64_t* soff, int S, int64_t* coff)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 4 * (S + 1)) return;
+    const int k = t / (S + 1), sc = t % (S + 1);
+    const int64_t target = soff[(k == K_VV ? 0 : (k == K_FV ? 2 : 1)) * (int64_t)(S + 1) + sc];
+    const int2* ids = C.ids[k];
+    int64_t lo = 0, hi = C.n[k];
+    while (lo < hi) {
+        const int64_t m = (lo + hi) >> 1;
+        const int2 p = ids[m];
+        const int64_t key = k == K_EE ? (p.x < p.y ? p.x : p.y) : p.x;
+        if (key < target) lo = m + 1;
+        else hi = m;
+    }
+    coff[t] = lo;
+}
+
+__device__ __forceinline__ void sum_partials(const double* partial, int64_t n, double* out)
 {
     double t = 0;
     for (int64_t i = threadIdx.x; i < n; i += 256) t += partial[i];
@@ -99,6 +168,20 @@ __global__ __launch_bounds__(256) void k_sum(const double* partial, int64_t n, d
     if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = t;
     __syncthreads();
     if (threadIdx.x == 0) *out = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+__global__ __launch_bounds__(256) void k_sum(const double* partial, int64_t n, double* out)
+{
+    sum_partials(partial, n, out);
+}
+
+// one workgroup per scene over its partials (boff as for k_energy_scenes)
+__global__ __launch_bounds__(256) void k_sum_scenes(const double* partial, const int64_t* boff,
+                                                    double* out)
+{
+    const int sc = blockIdx.x;
+    sum_partials(partial + boff[4 * (int64_t)sc], boff[4 * (int64_t)sc + 4] - boff[4 * (int64_t)sc],
+                 out + sc);
 }
 
 // Atomic-free gradient: every collision writes its 3 NP derivatives to a
@@ -947,6 +1030,60 @@ void pot_energy_noalloc(Ctx& c, double dhat, double kappa, double* d_out)
     if (nb[3]) hipLaunchKernelGGL(k_energy<K_FV>, dim3(nb[3]), dim3(PB), 0, s, C, X, c.nV, bp, part + o);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, part, tot, d_out);
     IPCG_CHECK_LAUNCH();
+}
+
+// ---- batched contexts ------------------------------------------------------
+void scene_collision_offsets(Ctx& c)
+{
+    if (!c.batched()) throw Error(IPCG_ERR_STATE, "not a batched context");
+    require_collisions(c);
+    if (c.coff_valid) return;
+    const int S = c.n_scenes;
+    hipStream_t s = c.stream;
+    c.coff_d.reserve(sizeof(int64_t) * 4 * (size_t)(S + 1));
+    hipLaunchKernelGGL(k_scene_coff, dim3(grid_for(4 * (S + 1), 256)), dim3(256), 0, s, view(c),
+                       c.voff_d.as<int64_t>(), S, c.coff_d.as<int64_t>());
+    IPCG_CHECK_LAUNCH();
+    c.h_coff.assign(4 * (size_t)(S + 1), 0);
+    IPCG_HIP(hipMemcpyAsync(c.h_coff.data(), c.coff_d.p, sizeof(int64_t) * c.h_coff.size(),
+                            hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    c.coff_valid = true;
+}
+
+void pot_energy_scenes(Ctx& c, double dhat, double kappa, double* out)
+{
+    scene_collision_offsets(c);
+    const int S = c.n_scenes;
+    hipStream_t s = c.stream;
+    // blocks of PB collisions per (scene, kind), scene-major
+    std::vector<int64_t> boff(4 * (size_t)S + 1, 0);
+    for (int sc = 0; sc < S; sc++)
+        for (int k = 0; k < 4; k++) {
+            const int64_t n = c.h_coff[k * (size_t)(S + 1) + sc + 1] - c.h_coff[k * (size_t)(S + 1) + sc];
+            boff[4 * (size_t)sc + k + 1] = boff[4 * (size_t)sc + k] + (n + PB - 1) / PB;
+        }
+    const int64_t tot = boff.back();
+    c.red_partial.reserve(sizeof(double) * (size_t)std::max<int64_t>(tot, 1));
+    c.scene_part.reserve(sizeof(int64_t) * boff.size() + sizeof(double) * (size_t)S);
+    int64_t* boff_d = c.scene_part.as<int64_t>();
+    double* out_d = reinterpret_cast<double*>(boff_d + boff.size());
+    IPCG_HIP(hipMemcpyAsync(boff_d, boff.data(), sizeof(int64_t) * boff.size(),
+                            hipMemcpyHostToDevice, s));
+    const BarrierParams bp = barrier_params(c, dhat, kappa, 0);
+    c.stage_begin("energy");
+    if (tot > 0) {
+        if (tot > INT32_MAX) throw Error(IPCG_ERR_ARG, "too many collisions for one launch");
+        hipLaunchKernelGGL(k_energy_scenes, dim3((unsigned)tot), dim3(PB), 0, s, view(c),
+                           c.X[0].as<double>(), c.nV, bp, c.coff_d.as<int64_t>(), boff_d, S,
+                           c.red_partial.as<double>());
+    }
+    hipLaunchKernelGGL(k_sum_scenes, dim3(S), dim3(256), 0, s, c.red_partial.as<double>(), boff_d,
+                       out_d);
+    IPCG_CHECK_LAUNCH();
+    c.stage_end();
+    IPCG_HIP(hipMemcpyAsync(out, out_d, sizeof(double) * (size_t)S, hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
 }
 
 // Global collision ranges per kind and the number of (vertex, collision)

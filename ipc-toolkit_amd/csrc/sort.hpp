@@ -27,6 +27,4 @@ size_t rle_tmp_bytes(int64_t n);
 void run_length_encode(const uint64_t* keys, int64_t n, uint64_t* unique_keys,
                        uint32_t* run_start, uint32_t* n_unique_dev, void* tmp, hipStream_t s);
 
-// Min/max reduction of a double array into out[0] = min, out[1] = max over
-// `count` values strided by `stride` starting at `base` (device memory).
 } // namespace ipcg

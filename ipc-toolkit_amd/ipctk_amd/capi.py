@@ -41,8 +41,28 @@ EXPORTS = [
     "ipcg_friction_build", "ipcg_friction_build_per_vertex", "ipcg_friction_download", "ipcg_velocities_upload",
     "ipcg_friction_energy", "ipcg_friction_gradient", "ipcg_friction_hessian",
     "ipcg_friction_set_anisotropy", "ipcg_friction_update_lagged_mu", "ipcg_barrier_set_physical",
-    "ipcg_barrier_energy_noalloc",
+    "ipcg_barrier_energy_noalloc", "ipcg_ctx_device_bytes", "ipcg_ctx_set_autotune", "ipcg_launch_shape",
+    "ipcg_mesh_upload_batched", "ipcg_batch_size", "ipcg_batch_set_active",
+    "ipcg_collisions_count_batched", "ipcg_barrier_energy_batched",
+    "ipcg_compute_collision_free_stepsize_batched",
+    "ipcg_server_create", "ipcg_server_destroy", "ipcg_server_last_error", "ipcg_server_add_scene",
+    "ipcg_server_submit", "ipcg_server_run", "ipcg_server_result", "ipcg_server_gradient",
+    "ipcg_server_hessian", "ipcg_server_positions", "ipcg_server_get_stats",
 ]
+
+REQ_ENERGY, REQ_GRADIENT, REQ_HESSIAN, REQ_STEP = range(4)
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("arrival", C.c_double), ("start", C.c_double), ("finish", C.c_double),
+                ("value", C.c_double), ("pairs", C.c_int64), ("accepted", C.c_int32),
+                ("iteration", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("epochs", C.c_int64), ("requests", C.c_int64),
+                ("peak_device_bytes", C.c_int64), ("estimated_bytes", C.c_int64),
+                ("busy_seconds", C.c_double), ("wall_seconds", C.c_double)]
 
 _lib = None
 _D = C.POINTER(C.c_double)
@@ -125,10 +145,33 @@ def load(path=LIB_PATH):
     L.ipcg_candidates_noncandidate_conservative_stepsize.argtypes = [vp, C.c_double, _D]
     L.ipcg_candidates_cfl_stepsize.argtypes = [vp, C.c_double, C.c_double, C.c_double,
                                                C.c_int64, C.c_double, C.c_int, _D]
+    L.ipcg_ctx_device_bytes.argtypes = [vp, _L]
+    L.ipcg_ctx_set_autotune.argtypes = [vp, C.c_int]
+    L.ipcg_launch_shape.argtypes = [vp, C.c_char_p, _I, _D]
+    L.ipcg_mesh_upload_batched.argtypes = [vp, C.c_int32, _L, _D, C.c_int, _L, _I, _L, _I]
+    L.ipcg_batch_size.argtypes = [vp, _I]
+    L.ipcg_batch_set_active.argtypes = [vp, C.POINTER(C.c_uint8)]
+    L.ipcg_collisions_count_batched.argtypes = [vp, _L]
+    L.ipcg_barrier_energy_batched.argtypes = [vp, C.c_double, C.c_double, _D]
+    L.ipcg_compute_collision_free_stepsize_batched.argtypes = [vp, C.c_double, C.c_double,
+                                                               C.c_int64, C.c_double, _D]
+    L.ipcg_server_create.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(vp)]
+    L.ipcg_server_destroy.argtypes = [vp]
+    L.ipcg_server_last_error.argtypes = [vp]
+    L.ipcg_server_last_error.restype = C.c_char_p
+    L.ipcg_server_add_scene.argtypes = [vp, C.c_int32, _D, C.c_int32, _I, C.c_int32, _I,
+                                        C.c_double, C.c_double, C.c_double, C.c_uint64, _I]
+    L.ipcg_server_submit.argtypes = [vp, C.c_int32, C.c_int, C.c_double, _L]
+    L.ipcg_server_run.argtypes = [vp]
+    L.ipcg_server_result.argtypes = [vp, C.c_int64, C.POINTER(RequestResult)]
+    L.ipcg_server_gradient.argtypes = [vp, C.c_int64, _D]
+    L.ipcg_server_hessian.argtypes = [vp, C.c_int64, _L, _L, _I, _D]
+    L.ipcg_server_positions.argtypes = [vp, C.c_int32, _D]
+    L.ipcg_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     L.ipcg_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), _D, _L]
     L.ipcg_stage_times_reset.argtypes = [vp, C.c_int]
     for name in EXPORTS:
-        if name not in ("ipcg_last_error", "ipcg_ctx_stream"):
+        if name not in ("ipcg_last_error", "ipcg_ctx_stream", "ipcg_server_last_error"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -493,3 +536,152 @@ class Context:
 
     def synchronize(self):
         self._check(self._L.ipcg_ctx_synchronize(self._p))
+
+    def set_autotune(self, enable=True):
+        self._check(self._L.ipcg_ctx_set_autotune(self._p, int(bool(enable))))
+
+    def launch_shape(self, stage):
+        """(workgroup size, ms) the autotuner chose for a stage (0: untuned)"""
+        b = C.c_int32()
+        ms = C.c_double()
+        self._check(self._L.ipcg_launch_shape(self._p, stage.encode(), C.byref(b), C.byref(ms)))
+        return b.value, ms.value
+
+    def device_bytes(self):
+        v = C.c_int64()
+        self._check(self._L.ipcg_ctx_device_bytes(self._p, C.byref(v)))
+        return v.value
+
+    # ---- batched scenes ----------------------------------------------------
+    def upload_batch(self, batch):
+        """batch: ipctk_amd.scenes.Batch (scene-local topology with ragged
+        vertex / edge / face offsets)."""
+        voff = np.ascontiguousarray(batch.voff, dtype=np.int64)
+        eoff = np.ascontiguousarray(batch.eoff, dtype=np.int64)
+        foff = np.ascontiguousarray(batch.foff, dtype=np.int64)
+        rest, pr = _d(batch.rest)
+        E, pE = _i(batch.E_local)
+        F, pF = _i(batch.F_local)
+        self._check(self._L.ipcg_mesh_upload_batched(
+            self._p, len(voff) - 1, voff.ctypes.data_as(_L), pr, LAYOUT_AOS,
+            eoff.ctypes.data_as(_L), pE, foff.ctypes.data_as(_L), pF))
+        self.nV = int(voff[-1])
+        self.E, self.F = batch.E, batch.F
+        self.n_scenes = len(voff) - 1
+
+    def batch_size(self):
+        n = C.c_int32()
+        self._check(self._L.ipcg_batch_size(self._p, C.byref(n)))
+        return n.value
+
+    def set_active(self, active=None):
+        """active: n_scenes booleans (None: every scene); the scenes left out
+        report no candidates or collisions until the next call."""
+        if active is None:
+            self._check(self._L.ipcg_batch_set_active(self._p, None))
+            return
+        a = np.ascontiguousarray(np.asarray(active, dtype=bool).astype(np.uint8))
+        self._check(self._L.ipcg_batch_set_active(self._p, a.ctypes.data_as(
+            C.POINTER(C.c_uint8))))
+
+    def collision_counts_batched(self):
+        out = np.zeros((self.n_scenes, 4), np.int64)
+        self._check(self._L.ipcg_collisions_count_batched(self._p, out.ctypes.data_as(_L)))
+        return out
+
+    def energy_batched(self, dhat, kappa):
+        out = np.zeros(self.n_scenes)
+        self._check(self._L.ipcg_barrier_energy_batched(self._p, dhat, kappa,
+                                                        out.ctypes.data_as(_D)))
+        return out
+
+    def stepsize_batched(self, min_distance=0.0, tolerance=1e-6, max_iterations=10_000_000,
+                         rescale=0.8):
+        out = np.zeros(self.n_scenes)
+        self._check(self._L.ipcg_compute_collision_free_stepsize_batched(
+            self._p, min_distance, tolerance, max_iterations, rescale, out.ctypes.data_as(_D)))
+        return out
+
+
+class Server:
+    """The serving loop (ipcg_server_*): scenes, timed requests, one run."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, keep_results=False, device=0):
+        self._L = load()
+        self._p = C.c_void_p()
+        rc = self._L.ipcg_server_create(device, int(hbm_budget_bytes), int(bool(keep_results)),
+                                        C.byref(self._p))
+        if rc != OK:
+            raise IPCGError(rc, "ipcg_server_create failed")
+        self.nV = []
+        self.kinds = []
+
+    def close(self):
+        if self._p:
+            self._L.ipcg_server_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise IPCGError(rc, self._L.ipcg_server_last_error(self._p).decode())
+
+    def add_scene(self, V, E, F, dhat, kappa, sigma, seed):
+        V, pV = _d(np.asarray(V, dtype=np.float64).reshape(-1, 3))
+        E, pE = _i(np.asarray(E).reshape(-1, 2))
+        F, pF = _i(np.asarray(F).reshape(-1, 3))
+        sid = C.c_int32()
+        self._check(self._L.ipcg_server_add_scene(self._p, len(V), pV, len(E), pE, len(F), pF,
+                                                  dhat, kappa, sigma, C.c_uint64(seed),
+                                                  C.byref(sid)))
+        self.nV.append(len(V))
+        return sid.value
+
+    def submit(self, scene, kind, arrival):
+        t = C.c_int64()
+        self._check(self._L.ipcg_server_submit(self._p, scene, kind, float(arrival), C.byref(t)))
+        self.kinds.append((scene, kind))
+        return t.value
+
+    def run(self):
+        self._check(self._L.ipcg_server_run(self._p))
+
+    def result(self, ticket):
+        r = RequestResult()
+        self._check(self._L.ipcg_server_result(self._p, ticket, C.byref(r)))
+        return r
+
+    def gradient(self, ticket):
+        scene = self.kinds[ticket][0]
+        g = np.zeros(3 * self.nV[scene])
+        self._check(self._L.ipcg_server_gradient(self._p, ticket, g.ctypes.data_as(_D)))
+        return g
+
+    def hessian(self, ticket):
+        import scipy.sparse as sp
+        scene = self.kinds[ticket][0]
+        n = 3 * self.nV[scene]
+        nnz = C.c_int64()
+        self._check(self._L.ipcg_server_hessian(self._p, ticket, C.byref(nnz), None, None, None))
+        rowptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1))
+        self._check(self._L.ipcg_server_hessian(self._p, ticket, None,
+                                                rowptr.ctypes.data_as(_L),
+                                                col.ctypes.data_as(_I), val.ctypes.data_as(_D)))
+        return sp.csr_matrix((val[:nnz.value], col[:nnz.value], rowptr), shape=(n, n))
+
+    def positions(self, scene):
+        V = np.zeros((self.nV[scene], 3))
+        self._check(self._L.ipcg_server_positions(self._p, scene, V.ctypes.data_as(_D)))
+        return V
+
+    def stats(self):
+        st = ServerStats()
+        self._check(self._L.ipcg_server_get_stats(self._p, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

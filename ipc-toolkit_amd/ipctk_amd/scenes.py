@@ -9,6 +9,7 @@ Sheets are n x n vertex grids over the unit square, vertex (i, j) at
 (j h, i h) with h = 1/(n-1); each grid cell is split along its
 (i,j)-(i+1,j+1) diagonal into two counter-clockwise triangles.
 """
+import functools
 import math
 
 import numpy as np
@@ -122,6 +123,12 @@ def stacked_sheets(n=512, sheets=8, seed=0, dhat=1e-3, kappa=1e4, spacing=0.6,
 
 
 def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
+    V, F = _unit_icosphere(subdiv)
+    return V * radius + np.asarray(center, dtype=np.float64), F.copy()
+
+
+@functools.lru_cache(maxsize=None)
+def _unit_icosphere(subdiv):
     t = (1.0 + math.sqrt(5.0)) / 2.0
     V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
          (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -146,8 +153,11 @@ def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
             ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
             newF += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         F = newF
-    V = np.array(V) * radius + np.asarray(center, dtype=np.float64)
-    return V, np.array(F, dtype=np.int32)
+    V = np.array(V)
+    V.flags.writeable = False
+    F = np.array(F, dtype=np.int32)
+    F.flags.writeable = False
+    return V, F
 
 
 def _rotation(rng):
@@ -200,3 +210,68 @@ def sphere_chain(count=2, subdiv=3, radius=0.1, seed=0, dhat=1e-3, kappa=1e3,
         V1[k * nv:(k + 1) * nv, 0] += shifts[k]
     V1 += rng.normal(0.0, noise * dhat, V1.shape)
     return Scene(f"sphere_chain_{count}x{subdiv}", V, E, F, V1, dhat, kappa, seed=seed)
+
+
+MAX_BATCH_TRIS = 300_000
+
+
+def batch_scenes(count=256, seed=0, dhat=1e-3, kappa=1e3):
+    """`count` independent scenes of uneven size: each, with probability 1/2,
+    a sheet pair (two_sheets with side drawn log-uniform in [32, 384]
+    vertices) or a row of icospheres (sphere_chain: subdivision U{3,4,5},
+    radius 0.1, count U{2..16}, neighbouring surface gaps U(0.2, 1.5) dhat).
+    Draws whose triangle count exceeds MAX_BATCH_TRIS are drawn again, so
+    scene sizes run from 3.8k to 300k triangles (a sheet pair of side 384
+    would have 586k: the triangle range is the rule kept)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        sub_seed = int(rng.integers(1 << 31))
+        while True:
+            if rng.random() < 0.5:
+                n = int(round(math.exp(rng.uniform(math.log(32), math.log(384)))))
+                tris = 4 * (n - 1) ** 2
+                if tris > MAX_BATCH_TRIS:
+                    continue
+                s = two_sheets(n, seed=sub_seed, dhat=dhat, kappa=kappa)
+            else:
+                sub = int(rng.integers(3, 6))
+                cnt = int(rng.integers(2, 17))
+                tris = cnt * 20 * 4 ** sub
+                if tris > MAX_BATCH_TRIS:
+                    continue
+                s = sphere_chain(cnt, sub, radius=0.1, seed=sub_seed, dhat=dhat, kappa=kappa)
+            break
+        s.name = f"batch{k}_{s.name}"
+        out.append(s)
+    return out
+
+
+class Batch:
+    """Scenes concatenated into one vertex / edge / face numbering: scene s
+    owns vertices [voff[s], voff[s+1]) and likewise edges and faces; the
+    topology arrays hold global vertex ids."""
+
+    def __init__(self, scenes):
+        self.scenes = list(scenes)
+        nv = [len(s.V) for s in self.scenes]
+        ne = [len(s.E) for s in self.scenes]
+        nf = [len(s.F) for s in self.scenes]
+        self.voff = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+        self.eoff = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+        self.foff = np.concatenate([[0], np.cumsum(nf)]).astype(np.int64)
+        self.V = np.concatenate([s.V for s in self.scenes])
+        self.V1 = np.concatenate([s.V1 for s in self.scenes])
+        self.rest = np.concatenate([s.rest for s in self.scenes])
+        # scene-local topology (what the batched upload takes)
+        self.E_local = np.concatenate([s.E for s in self.scenes]).astype(np.int32)
+        self.F_local = np.concatenate([s.F for s in self.scenes]).astype(np.int32)
+        self.E = self.E_local + np.repeat(self.voff[:-1], ne)[:, None].astype(np.int32)
+        self.F = self.F_local + np.repeat(self.voff[:-1], nf)[:, None].astype(np.int32)
+
+    def __len__(self):
+        return len(self.scenes)
+
+    @property
+    def sizes(self):
+        return len(self.V), len(self.E), len(self.F)

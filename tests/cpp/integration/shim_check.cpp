@@ -1,0 +1,160 @@
+// The reference-side shim (ipc-toolkit_amd/integration/ipc_gpu.hpp) compiled
+// against a stand-in of the reference interface (stand_in/) and linked with
+// libipcg.so.  Without a HIP device it reports that and exits with 3; with
+// one it drives every override of the GPU broad phase through the
+// reference's BroadPhase interface, the GPU collision build into the
+// reference's collision vectors, the barrier and the step size through the
+// name dispatch a maintainer adds to compute_collision_free_stepsize.
+#include <ipc_gpu.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <utility>
+
+using namespace ipc;
+
+static int failures = 0;
+#define EXPECT(cond)                                                                         \
+    do {                                                                                     \
+        if (!(cond)) {                                                                       \
+            std::printf("FAILED: %s (line %d)\n", #cond, __LINE__);                          \
+            failures++;                                                                      \
+        }                                                                                    \
+    } while (0)
+
+// two stacked n x n unit sheets, the top one `gap` above the bottom one
+static void two_sheets(int n, double gap, Eigen::MatrixXd& V, Eigen::MatrixXi& E,
+                       Eigen::MatrixXi& F)
+{
+    const int nv = n * n, nf = 2 * (n - 1) * (n - 1);
+    V = Eigen::MatrixXd(2 * nv, 3);
+    F = Eigen::MatrixXi(2 * nf, 3);
+    std::map<std::pair<int, int>, int> edges;
+    for (int s = 0; s < 2; s++) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const int v = s * nv + i * n + j;
+                V(v, 0) = j / double(n - 1);
+                V(v, 1) = i / double(n - 1);
+                V(v, 2) = s * gap;
+            }
+        int f = s * nf;
+        for (int i = 0; i + 1 < n; i++)
+            for (int j = 0; j + 1 < n; j++) {
+                const int a = s * nv + i * n + j, b = a + 1, c = a + n + 1, d = a + n;
+                const int tri[2][3] = { { a, b, c }, { a, c, d } };
+                for (auto& t : tri) {
+                    for (int k = 0; k < 3; k++) {
+                        F(f, k) = t[k];
+                        const int x = t[k], y = t[(k + 1) % 3];
+                        edges.emplace(std::make_pair(std::min(x, y), std::max(x, y)), 0);
+                    }
+                    f++;
+                }
+            }
+    }
+    E = Eigen::MatrixXi(static_cast<Eigen::Index>(edges.size()), 2);
+    int e = 0;
+    for (auto& kv : edges) {
+        E(e, 0) = kv.first.first;
+        E(e, 1) = kv.first.second;
+        e++;
+    }
+}
+
+// the branch of compute_collision_free_stepsize (ipc.cpp:63-85 dispatches
+// its CUDA broad phase the same way)
+static double stepsize_by_name(BroadPhase* bp, const Eigen::MatrixXd& V0,
+                               const Eigen::MatrixXd& V1, const NarrowPhaseCCD& ccd)
+{
+    if (bp != nullptr && (bp->name() == "GPULBVH" || bp->name() == "GPUSweepAndPrune"))
+        return gpu::compute_collision_free_stepsize(static_cast<gpu::GPULBVH&>(*bp), V0, V1, 0,
+                                                    ccd);
+    return -1; // the reference's CPU path would run here
+}
+
+int main()
+{
+    const double dhat = 1e-3, kappa = 1e3;
+    Eigen::MatrixXd V, V1;
+    Eigen::MatrixXi E, F;
+    two_sheets(12, 0.5 * dhat, V, E, F);
+    V1 = V;
+    for (Eigen::Index v = V.rows() / 2; v < V.rows(); v++) V1(v, 2) -= 1e-3;
+    CollisionMesh mesh(V, E, F);
+    try {
+        gpu::Context probe(mesh);
+    } catch (const std::runtime_error& e) {
+        std::printf("no device: %s\n", e.what());
+        return 3;
+    }
+    gpu::Context ctx(mesh);
+    gpu::GPULBVH gpu_bp(ctx);
+    BroadPhase& bp = gpu_bp; // every call below goes through the reference interface
+    EXPECT(bp.name() == "GPULBVH");
+    bp.build(V, E, F, 0.5 * dhat);
+    std::vector<EdgeEdgeCandidate> ee;
+    std::vector<FaceVertexCandidate> fv;
+    std::vector<VertexVertexCandidate> vv;
+    std::vector<EdgeVertexCandidate> ev;
+    std::vector<EdgeFaceCandidate> ef;
+    std::vector<FaceFaceCandidate> ff;
+    bp.detect_edge_edge_candidates(ee);
+    bp.detect_face_vertex_candidates(fv);
+    bp.detect_vertex_vertex_candidates(vv);
+    bp.detect_edge_vertex_candidates(ev);
+    bp.detect_edge_face_candidates(ef);
+    EXPECT(!ee.empty() && !fv.empty() && !ef.empty());
+    EXPECT(vv.empty() && ev.empty()); // a surface mesh has no codim primitives
+    bool threw = false;
+    try {
+        bp.detect_face_face_candidates(ff);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    threw = false;
+    try {
+        bp.build(AABBs(), E, F, 3);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    bp.clear();
+    threw = false;
+    try {
+        bp.detect_edge_edge_candidates(ee);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+
+    NormalCollisions C;
+    gpu::build(C, ctx, V, dhat);
+    EXPECT(C.size() > 0);
+    int64_t n[4];
+    gpu::check(ctx.c, ipcg_collisions_count(ctx.c, n));
+    EXPECT((int64_t)C.vv_collisions.size() == n[0] && (int64_t)C.ev_collisions.size() == n[1]
+           && (int64_t)C.ee_collisions.size() == n[2] && (int64_t)C.fv_collisions.size() == n[3]);
+    const double e = gpu::energy(ctx, dhat, kappa);
+    const Eigen::VectorXd g = gpu::gradient(ctx, dhat, kappa);
+    const Eigen::SparseMatrix<double> H = gpu::hessian(ctx, dhat, kappa, PSDProjectionMethod::CLAMP);
+    double gn = 0;
+    for (Eigen::Index i = 0; i < g.size(); i++) gn += g.data()[i] * g.data()[i];
+    EXPECT(e > 0 && gn > 0 && H.nonZeros() > 0);
+
+    TightInclusionCCD ccd;
+    const double t = stepsize_by_name(&bp, V, V1, ccd);
+    double t_c = 0;
+    gpu::check(ctx.c, ipcg_compute_collision_free_stepsize(ctx.c, 0, ccd.tolerance,
+                                                           ccd.max_iterations,
+                                                           ccd.conservative_rescaling,
+                                                           IPCG_BP_LBVH, &t_c));
+    EXPECT(t > 0 && t < 1 && t == t_c);
+    std::printf("candidates ee %zu fv %zu ef %zu; collisions %zu; energy %.6e; nnz %ld; "
+                "step %.9g\n",
+                ee.size(), fv.size(), ef.size(), C.size(), e, (long)H.nonZeros(), t);
+    std::printf("%d failure(s)\n", failures);
+    return failures ? 1 : 0;
+}

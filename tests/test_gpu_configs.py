@@ -245,3 +245,26 @@ def test_config2_full_size(ctx, O, scenes):
     t_o = O.compute_collision_free_stepsize(s.V, s.V1, s.E, s.F)
     say(f"  step device {t_g!r} oracle {t_o!r} {time.time() - t0:.1f}s")
     assert t_g == t_o
+
+
+def test_ccd_tiers_on_config2_tile(ctx, O, scenes):
+    """the configuration-2 tile sends searches through all three CCD tiers
+    (LDS lists, per-lane HBM slabs, the wavefront): per-candidate TOIs
+    bit-identical to the oracle, and the step with the running minimum"""
+    s = make(scenes, "config2_tile:0")
+    say("case config2_tile tiers", s.sizes)
+    upload(ctx, s)
+    t = ctx.stepsize()
+    bounded = {k: ctx.counter("ccd." + k) for k in ("searched", "spilled", "wavefront")}
+    ctx.broad_phase(True, 0.0)
+    ee, fv = ctx.candidates()
+    te, tf = ctx.candidates_toi()
+    full = {k: ctx.counter("ccd." + k) for k in ("searched", "spilled", "wavefront")}
+    say("  step", t, "bounded", bounded, "per candidate", full)
+    assert bounded["spilled"] > 0 and bounded["wavefront"] > 0
+    assert full["spilled"] > 0 and full["wavefront"] > 0
+    oe = O.candidates_toi(s.V, s.V1, s.E, s.F, ee, True)
+    of = O.candidates_toi(s.V, s.V1, s.E, s.F, fv, False)
+    assert np.array_equal(te, oe) and np.array_equal(tf, of)
+    hits = np.concatenate([te, tf])
+    assert t == hits[np.isfinite(hits)].min()

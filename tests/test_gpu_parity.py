@@ -192,7 +192,7 @@ def test_layouts_and_device_upload(ctx, scenes):
                     ctx.gradient(s.dhat, s.kappa), ctx.stepsize()))
     for o in out[1:]:
         assert o[0] == out[0][0] and o[1] == out[0][1] and o[3] == out[0][3]
-        # the gradient's FP64 atomic sums may differ in the order of addition
+        # the gradient sums may differ in the order of addition
         assert rel_err(o[2], out[0][2]) <= 1e-14
 
 

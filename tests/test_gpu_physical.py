@@ -38,3 +38,35 @@ def test_physical_barrier(ctx, O, scenes, name, dmin):
     assert np.linalg.norm(d_g - d_o) <= 1e-12 * np.linalg.norm(d_o)
     ctx.set_physical_barrier(False)
     assert ctx.energy(dhat, s.kappa) == e0
+
+
+@pytest.mark.parametrize("name,dmin", [("sheets:16:0", 0.0), ("twisted:12:1", 2e-4)])
+def test_physical_barrier_friction_and_quadratic_form(ctx, O, scenes, name, dmin):
+    """with the physical barrier the friction build's normal forces and the
+    friction potential equal the oracle's at stiffness kappa * dhat / units,
+    and so does the Gauss-Newton quadratic form"""
+    s = make(scenes, name)
+    print("case", name, dmin, flush=True)
+    upload(ctx, s)
+    dhat = 2 * s.dhat
+    ctx.build_collisions(dhat, dmin)
+    ee, fv = ctx.candidates()
+    co = O.build_collisions(s.V, s.E, s.F, ee, fv, dhat, dmin)
+    t = (2 * dmin + dhat) * dhat
+    k_eff = s.kappa * dhat / (t * t)
+    ctx.set_physical_barrier(True)
+    p = np.random.default_rng(3).normal(size=3 * len(s.V))
+    q_o = O.gauss_newton_quadratic_form(co, s.V, s.E, s.F, dhat, k_eff, p)
+    assert abs(ctx.gauss_newton_quadratic_form(dhat, s.kappa, p) - q_o) <= 1e-12 * abs(q_o)
+    ctx.friction_build(dhat, s.kappa, 0.5, 0.3)
+    fr = O.friction_build(co, s.V, s.E, s.F, dhat, k_eff, 0.5, 0.3)
+    keep, beta, basis, force = ctx.friction_records()
+    assert np.array_equal(keep, fr.keep) and keep.any()
+    assert np.allclose(force[keep], fr.normal_force[keep], rtol=1e-13, atol=0)
+    eps_v = 1e-3
+    vel = np.random.default_rng(4).normal(0, 2 * eps_v, s.V.shape)
+    ctx.upload_velocities(vel)
+    e_o = O.friction_energy(fr, vel, eps_v)
+    assert abs(ctx.friction_energy(eps_v) - e_o) <= 1e-12 * abs(e_o)
+    assert rel_err(ctx.friction_gradient(eps_v), O.friction_gradient(fr, vel, eps_v)) <= 1e-12
+    ctx.set_physical_barrier(False)
