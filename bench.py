@@ -79,7 +79,7 @@ def parse():
                         "one context, beside the same scenes one context each) and the served "
                         "form (Poisson requests); 0: skip")
     p.add_argument("--batch-scenes", type=int, default=256)
-    p.add_argument("--batch-steps", type=int, default=3)
+    p.add_argument("--batch-steps", type=int, default=2)
     p.add_argument("--serve-scenes", type=int, default=64)
     p.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
     p.add_argument("--serve-seconds", type=float, default=4.0,

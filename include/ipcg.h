@@ -456,6 +456,9 @@ int ipcg_server_hessian(ipcg_server* srv, int64_t ticket, int64_t* nnz, int64_t*
 /* a scene's current positions (nV x 3) */
 int ipcg_server_positions(ipcg_server* srv, int32_t scene, double* V);
 int ipcg_server_get_stats(ipcg_server* srv, ipcg_server_stats* out);
+/* the server's batched context (stage times, counters; do not call other
+ * entry points on it while the server runs) */
+ipcg_ctx* ipcg_server_context(ipcg_server* srv);
 
 /* ---- timing of the last call's kernels (for benchmarks) ----------------- */
 /* Kernel time of named stages recorded with HIP events on the context

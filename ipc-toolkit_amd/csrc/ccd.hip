@@ -34,7 +34,14 @@ constexpr int CB = 64;
 constexpr int LDS_CAP = 6;
 constexpr int LANE_CAP = 256;
 constexpr int LANES = 131072; // 8 waves per CU at this kernel's register use
-constexpr int WF_BATCH = 4096;
+// Wavefront batches: searches advanced together (one round of launches and
+// one host check per level for the whole batch), from WF_BATCH, doubled
+// after every batch that fits the box budget up to WF_BATCH_MAX.  Workloads
+// with edges parallel and on top of each other send hundreds of thousands
+// of searches here; small batches would pay the per-level round trip for
+// every few thousand of them.
+constexpr int WF_BATCH = 65536;
+constexpr int WF_BATCH_MAX = 1 << 18;
 constexpr int64_t WF_MAX_BOXES = 1ll << 26;
 constexpr double SMALL_TOI = 1e-6; // tight_inclusion_ccd.hpp:14
 
@@ -184,8 +191,11 @@ __global__ __launch_bounds__(256) void k_ccd_root(CCDArgs A, Shards S, uint32_t*
 // ---------------------------------------------------------------------------
 // LDS tier
 // ---------------------------------------------------------------------------
+// ci_lo, ci_hi: only the candidates of this index range are searched (a
+// phase of the search; the others return at once)
 __global__ __launch_bounds__(CB) void k_ccd_lds(CCDArgs A, const uint32_t* todo, int64_t n_todo,
-                                                uint32_t* spill, unsigned long long* n_spill)
+                                                uint32_t* spill, unsigned long long* n_spill,
+                                                int64_t ci_lo, int64_t ci_hi)
 {
     __shared__ unsigned long long lists[3 * LDS_CAP * CB];
     const int64_t k = (int64_t)xcd_block() * CB + threadIdx.x;
@@ -194,6 +204,7 @@ __global__ __launch_bounds__(CB) void k_ccd_lds(CCDArgs A, const uint32_t* todo,
         return;
     const TILdsLists L { lists + threadIdx.x, CB, LDS_CAP };
     const int64_t ci = todo[k];
+    if (ci < ci_lo || ci >= ci_hi) return;
     TIQuery q;
     double eff = 0, adj = 0, toi = INFINITY;
     int sc = 0;
@@ -727,7 +738,6 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     c.ccd_overflow.reserve(sizeof(uint32_t) * (size_t)n);
     uint32_t* heavy = c.ccd_overflow.as<uint32_t>();
     if (per_cand) c.ccd_toi.reserve(sizeof(double) * (size_t)n);
-    c.ccd_slab.reserve(sizeof(TIBox) * 3 * (size_t)LANE_CAP * LANES);
 
     CCDArgs A;
     A.X0 = c.X[0].as<double>();
@@ -776,25 +786,33 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     c.stat["ccd.candidates"] = n;
     c.stat["ccd.searched"] = n_todo;
 
-    // One search of every surviving candidate: the LDS tier, the slab tier
-    // for its overflow, then the wavefront for searches that outgrew a slab.
-    // The root pass's staging array is free again and holds the spill list.
-    auto search = [&]() {
+    // One search of every surviving candidate of an index range: the LDS
+    // tier, the slab tier for its overflow, then the wavefront for searches
+    // that outgrew a slab.  The root pass's staging array is free again and
+    // holds the spill list.
+    auto search = [&](int64_t ci_lo, int64_t ci_hi) {
         uint32_t* spill = c.ccd_stage.as<uint32_t>();
+        IPCG_HIP(hipMemsetAsync(n_spill, 0, sizeof(unsigned long long), s));
+        IPCG_HIP(hipMemsetAsync(work, 0, sizeof(unsigned long long), s));
+        IPCG_HIP(hipMemsetAsync(n_heavy, 0, sizeof(unsigned long long), s));
         c.stage_begin("ccd_lds");
         if (n_todo > 0) {
             hipLaunchKernelGGL(k_ccd_lds, dim3((unsigned)((n_todo + CB - 1) / CB)), dim3(CB), 0, s,
-                               A, c.ccd_todo.as<uint32_t>(), n_todo, spill, n_spill);
+                               A, c.ccd_todo.as<uint32_t>(), n_todo, spill, n_spill, ci_lo,
+                               ci_hi);
             IPCG_CHECK_LAUNCH();
         }
         c.stage_end();
         IPCG_HIP(hipMemcpyAsync(h, n_spill, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));
         const int64_t n2 = (int64_t)h[0];
-        c.stat["ccd.spilled"] = n2;
+        c.stat["ccd.spilled"] += n2;
         c.stage_begin("ccd_lanes");
         if (n2 > 0) {
             const int64_t ln = std::min<int64_t>(LANES, ((n2 + CB - 1) / CB) * CB);
+            // slabs for the lanes launched only (a context that never
+            // spills many searches keeps a small slab)
+            c.ccd_slab.reserve(sizeof(TIBox) * 3 * (size_t)LANE_CAP * (size_t)ln);
             hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A, spill,
                                n2, c.ccd_slab.as<TIBox>(), work, heavy, n_heavy);
             IPCG_CHECK_LAUNCH();
@@ -807,13 +825,13 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
         if (nh > 0) {
             c.stage_begin("ccd_wavefront");
             DevBuf& wst = c.ccd_bound; // per-candidate wavefront state
-            wst.reserve(sizeof(WFState) * (size_t)WF_BATCH);
+            wst.reserve(sizeof(WFState) * (size_t)std::min<int64_t>(nh, WF_BATCH_MAX));
             WFState* states = wst.as<WFState>();
             // Batches of heavy candidates advance together.  A batch whose
             // level outgrows the box budget is abandoned (results are only
             // reported at the end of a batch, so this has no side effect)
             // and retried with a quarter of the candidates.
-            int64_t batch = WF_BATCH;
+            int64_t batch = std::min<int64_t>(WF_BATCH, nh);
             for (int64_t first = 0; first < nh;) {
                 const int64_t nb = std::min<int64_t>(batch, nh - first);
                 const bool ok = wf_batch(c, A, heavy, first, nb, states, early_exit, any_hit,
@@ -825,7 +843,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
                     continue;
                 }
                 first += nb;
-                batch = std::min<int64_t>(WF_BATCH, 2 * batch);
+                batch = std::min<int64_t>(WF_BATCH_MAX, 2 * batch);
             }
             c.stage_end();
             IPCG_HIP(hipMemcpyAsync(h, error, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -835,7 +853,18 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
         }
     };
     c.stat["ccd.wavefront"] = 0;
-    search();
+    c.stat["ccd.spilled"] = 0;
+    if (A.bounded) {
+        // Face-vertex and codim candidates first: their roots are isolated
+        // points, so they resolve fast and lower every scene's running
+        // minimum before the edge-edge searches start, which parallel
+        // overlapping edges can make long.  Outcomes below the minimum do not
+        // depend on the order, so neither does the result.
+        search(c.n_ee, n);
+        search(0, c.n_ee);
+    } else {
+        search(0, n);
+    }
     if (per_cand) { // toi_out: ee, fv, vv, ev (any null)
         const int64_t cnt[4] = { c.n_ee, c.n_fv, c.n_vv, c.n_ev };
         int64_t off = 0;

