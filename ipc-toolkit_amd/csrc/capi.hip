@@ -171,7 +171,7 @@ static std::vector<DevBuf*> all_bufs(Ctx& c)
                        &c.im_slot, &c.im_slot2, &c.im_vals, &c.im_vals2, &c.im_ukeys,
                        &c.im_start, &c.im_cnt, &c.im_off, &c.im_vv, &c.im_vvw, &c.im_ev,
                        &c.im_evw, &c.im_ee, &c.fric, &c.vel, &c.fric_mu,
-                       &c.np_cw };
+                       &c.np_cw, &c.wf_ev, &c.wf_live, &c.wf_lpre };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
                            &T->root, &T->sbox, &T->leaf })

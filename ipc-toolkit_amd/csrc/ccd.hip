@@ -13,9 +13,11 @@
 //  * wavefront path, for candidates whose levels outgrow the slab: all of
 //    them advance one level per round of short kernels.  Every box of the
 //    level is evaluated by its own lane (the inclusion test is the work),
-//    then one lane per candidate replays the sequential per-level rules over
-//    its boxes in order, then children are emitted through a scan and each
-//    candidate's next level is ordered (merge of the lower / upper t halves).
+//    the sequential per-level rules are resolved from each candidate's first
+//    deciding boxes (found by per-box atomic minimums over the t-sorted
+//    level), children are emitted through a scan and each candidate's next
+//    level is ordered (a merge of the lower / upper t halves, a lane per
+//    box).
 // The running minimum lives in device memory as the bit pattern of a
 // non-negative double (unsigned atomic min); every search hides outcomes at
 // or above it, which leaves the final minimum independent of scheduling.
@@ -288,6 +290,7 @@ struct WFState {
     int stage, nz_iter, levels, axis;
     int use_skip, done, hit, restart;
     int scene, pad_;
+    int64_t cut;                    // boxes of the level before this one may split
     int64_t seg_start, seg_count;   // boxes of the current level
     int64_t next_start, next_count; // boxes of the next level
 };
@@ -311,11 +314,18 @@ __device__ void wf_begin_query(WFState& S, double ms, double delta, long max_itr
     S.restart = 1;
 }
 
+// per candidate and level: first box (level index) that stops the scan,
+// first zero box out of tolerance, first zero box, first zero box past the
+// iteration limit
+constexpr int WF_EV = 4;
+constexpr unsigned long long WF_NONE = ~0ull;
+
 __global__ void k_wf_init(CCDArgs A, const uint32_t* heavy, int64_t first, int64_t nb,
-                          WFState* st, WFBox* boxes)
+                          WFState* st, WFBox* boxes, unsigned long long* ev)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
+    for (int k = 0; k < WF_EV; k++) ev[WF_EV * i + k] = WF_NONE;
     WFState S;
     S.ci = heavy[first + i];
     double eff = 0, adj = 0;
@@ -330,6 +340,7 @@ __global__ void k_wf_init(CCDArgs A, const uint32_t* heavy, int64_t first, int64
     S.axis = 0;
     S.seg_start = i;
     S.seg_count = 1;
+    S.cut = i;
     S.next_start = S.next_count = 0;
     if (pre >= 0) { // resolved by the prologue (cannot happen for lane overflows)
         S.done = 1;
@@ -353,16 +364,22 @@ __device__ __forceinline__ double wf_bound(const CCDArgs& A, const WFState& S)
     return g * ((1.0 / A.rescale) * (1 + 1e-12));
 }
 
+// Every box of the level by its own lane: skipped (its candidate is done,
+// or the box starts at or after the running minimum or the candidate's
+// earliest tolerance box), else the inclusion test.  live[j] = not skipped
+// (scanned for the iteration counts).  A level is sorted by t (the first
+// level is one box; a t split merges its halves in order, other splits keep
+// it); the per-level rules below rely on that and a level out of order is
+// reported as an error.
 __global__ void k_wf_eval(CCDArgs A, const WFState* st, const WFBox* boxes, int64_t n,
-                          uint8_t* flags, uint8_t* cmask)
+                          uint8_t* flags, uint32_t* live, int* error)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
-    cmask[j] = 0;
     const WFBox b = boxes[j];
     const WFState& S = st[b.owner];
     uint8_t f = 0;
-    if (S.done || b.t >= wf_bound(A, S)) {
+    if (S.done || b.t >= wf_bound(A, S) || b.t >= S.toi_skip) {
         f = F_SKIP;
     } else {
         bool box_in;
@@ -372,11 +389,54 @@ __global__ void k_wf_eval(CCDArgs A, const WFState* st, const WFBox* boxes, int6
             f = F_ZERO;
             if (box_in) f |= F_BOXIN;
             if (tw[0] <= S.delta && tw[1] <= S.delta && tw[2] <= S.delta) f |= F_TOL;
-            // max_itr bookkeeping needs the widest inclusion; it is recomputed
-            // by the deciding lane only for the first zero-in box of a level
         }
     }
     flags[j] = f;
+    live[j] = (f & F_SKIP) ? 0u : 1u;
+    if (j > S.seg_start && boxes[j - 1].t > b.t) atomicOr(error, 2);
+}
+
+// lowest level index of the lanes whose `hit` is set, per run of one
+// candidate inside the wave, into e (one atomic per run); called by every
+// lane of the wave
+__device__ __forceinline__ void wf_first(bool hit, int owner, int64_t j, unsigned long long* e)
+{
+    const unsigned long long m = __ballot(hit);
+    const unsigned long long below = m & ((1ull << lane_id()) - 1ull);
+    const int l = below ? 63 - __clzll((long long)below) : lane_id();
+    const int prev_owner = __shfl(owner, l, WAVE);
+    if (hit && (!below || prev_owner != owner)) atomicMin(e, (unsigned long long)j);
+}
+
+// The sequential per-level rules of ti_bfs, in parallel: the scan of a
+// level in order ends at the first zero box that is within tolerance or
+// inside the ε box (or any zero box once the level's widths are within
+// tolerance: c1); it finishes the search there if no zero box before it was
+// out of tolerance, else it sets the earliest tolerance box and everything
+// after it is skipped.  The iteration limit counts the scanned boxes.
+__global__ void k_wf_mark(const WFState* st, const WFBox* boxes, int64_t n, const uint8_t* flags,
+                          const uint32_t* lpre, unsigned long long* ev)
+{
+    const int64_t j0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool in_range = j0 < n;
+    const int64_t j = in_range ? j0 : n - 1;
+    const uint8_t f = in_range ? flags[j] : F_SKIP;
+    const bool zero = !(f & F_SKIP) && (f & F_ZERO);
+    const int owner = boxes[j].owner;
+    const WFState& S = st[owner];
+    const TIQuery& q = S.q;
+    const bool c1 = S.wt <= q.tol[0] && S.wu <= q.tol[1] && S.wv <= q.tol[2];
+    const bool tol = f & F_TOL, in = f & F_BOXIN;
+    unsigned long long* e = ev + (int64_t)WF_EV * owner;
+    bool past = false;
+    if (zero && S.max_itr > 0) {
+        const int64_t refine = S.refine + (int64_t)(lpre[j] - lpre[S.seg_start]) + 1;
+        past = refine > S.max_itr;
+    }
+    wf_first(zero && (c1 || tol || in), owner, j, e + 0);
+    wf_first(zero && !tol, owner, j, e + 1);
+    wf_first(zero, owner, j, e + 2);
+    wf_first(past, owner, j, e + 3);
 }
 
 // Ends the current search of a candidate with (hit, toi) and applies the
@@ -408,32 +468,34 @@ __device__ void wf_search_done(const CCDArgs& A, WFState& S, bool hit, double to
     S.result = hit ? toi * A.rescale : toi;
 }
 
-// one lane per candidate: replay ti_bfs's per-level rules over the level
+// one lane per candidate: the outcome of the level from its first events
+// (ti_bfs's rules); children may come only from boxes before S.cut
 __global__ void k_wf_decide(CCDArgs A, WFState* st, int64_t nb, const WFBox* boxes,
-                            const uint8_t* flags, uint8_t* cmask, int* error)
+                            const uint32_t* lpre, unsigned long long* ev, int* error)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
     WFState S = st[i];
     S.restart = 0;
     const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
+    unsigned long long* e = ev + (int64_t)WF_EV * i;
+    const unsigned long long e_stop = e[0], e_bad = e[1], e_zero = e[2], e_max = e[3];
+    for (int k = 0; k < WF_EV; k++) e[k] = WF_NONE;
+    S.cut = s0;
     if (S.done) {
-        for (int64_t j = s0; j < s1; j++) cmask[j] = 0;
         st[i] = S;
         return;
     }
     if (++S.levels > TI_MAX_LEVELS) {
         atomicOr(error, 1);
         S.done = 1;
-        for (int64_t j = s0; j < s1; j++) cmask[j] = 0;
         st[i] = S;
         return;
     }
-    const double bound = wf_bound(A, S);
     const TIQuery& q = S.q;
     const bool c1 = S.wt <= q.tol[0] && S.wu <= q.tol[1] && S.wv <= q.tol[2];
-    int axis = 0;
     {
+        int axis = 0;
         double best = -INFINITY;
         const double w3[3] = { S.wt, S.wu, S.wv };
         for (int k = 0; k < 3; k++) {
@@ -443,82 +505,80 @@ __global__ void k_wf_decide(CCDArgs A, WFState* st, int64_t nb, const WFBox* box
                 axis = k;
             }
         }
-    }
-    bool level_less_tol = true, level_root = false;
-    bool finished = false, fhit = false;
-    double ftoi = INFINITY, fout = S.delta;
-    int64_t nchild = 0;
-    int64_t j = s0;
-    for (; j < s1; j++) {
-        const uint8_t f = flags[j];
-        const WFBox b = boxes[j];
-        cmask[j] = 0;
-        if ((f & F_SKIP) || b.t >= S.toi_skip || b.t >= bound) continue;
-        S.refine++;
-        if (!(f & F_ZERO)) continue;
-        const bool tol_cond = f & F_TOL;
-        const bool box_in = f & F_BOXIN;
-        const bool c2 = box_in && level_less_tol;
-        if (!tol_cond) level_less_tol = false;
-        if (c1 || c2 || level_less_tol) {
-            finished = true;
-            fhit = true;
-            ftoi = b.t;
-            break;
-        }
-        if (S.max_itr > 0) {
-            if (!level_root) {
-                bool bi;
-                double tw[3];
-                const TIBox tb { b.t, b.u, b.v };
-                ti_inclusion(q, tb, S.wt, S.wu, S.wv, S.ms, bi, tw);
-                double m = S.delta;
-                for (int k = 0; k < 3; k++) m = tw[k] > m ? tw[k] : m;
-                S.temp_toi = b.t;
-                S.temp_out_tol = m;
-                level_root = true;
-            }
-            if (S.refine > S.max_itr) {
-                finished = true;
-                fhit = true;
-                ftoi = S.temp_toi;
-                fout = S.temp_out_tol;
-                break;
-            }
-        }
-        if (tol_cond || box_in) {
-            S.toi_skip = b.t < S.toi_skip ? b.t : S.toi_skip;
-            S.use_skip = 1;
-            continue;
-        }
-        uint8_t m = 0;
-        if (axis == 0) {
-            m = 3;
-        } else {
-            const double hu = axis == 1 ? b.u + 0.5 * S.wu : b.u;
-            const double hv = axis == 2 ? b.v + 0.5 * S.wv : b.v;
-            if (!q.vf || hu + hv <= 1) m |= 1; // upper half
-            if (!q.vf || b.u + b.v <= 1) m |= 2; // lower half
-        }
-        cmask[j] = m;
-        nchild += __popc(m);
-    }
-    for (j = j + 1; j < s1; j++) cmask[j] = 0;
-    if (finished) {
-        for (int64_t k = s0; k < s1; k++) cmask[k] = 0;
-        wf_search_done(A, S, fhit, ftoi, fout);
-    } else if (nchild == 0) {
-        wf_search_done(A, S, S.use_skip != 0, S.use_skip ? S.toi_skip : INFINITY, S.delta);
-    } else {
         S.axis = axis;
+    }
+    // scanned boxes up to and including level index j
+    auto scanned = [&](int64_t j) { return (int64_t)(lpre[j] - lpre[s0]) + 1; };
+    if (e_stop == WF_NONE && e_max == WF_NONE) { // every live box scanned
+        S.refine += (int64_t)(lpre[s1] - lpre[s0]);
+        S.cut = s1;
+    } else if (e_stop < e_max || (e_stop == e_max && (c1 || e_bad >= e_stop))) {
+        const int64_t j = (int64_t)e_stop;
+        S.refine += scanned(j);
+        const double t = boxes[j].t;
+        if (c1 || e_bad >= e_stop) { // no zero box before j out of tolerance
+            wf_search_done(A, S, true, t, S.delta);
+        } else { // earliest tolerance box; the rest of the level is skipped
+            S.toi_skip = t < S.toi_skip ? t : S.toi_skip;
+            S.use_skip = 1;
+            S.cut = j;
+        }
+    } else { // iteration limit: the level's first zero box
+        S.refine += scanned((int64_t)e_max);
+        const WFBox b = boxes[e_zero];
+        bool bi;
+        double tw[3];
+        const TIBox tb { b.t, b.u, b.v };
+        ti_inclusion(q, tb, S.wt, S.wu, S.wv, S.ms, bi, tw);
+        double m = S.delta;
+        for (int k = 0; k < 3; k++) m = tw[k] > m ? tw[k] : m;
+        S.temp_toi = b.t;
+        S.temp_out_tol = m;
+        wf_search_done(A, S, true, S.temp_toi, S.temp_out_tol);
     }
     st[i] = S;
 }
 
-__global__ void k_wf_counts(const uint8_t* cmask, int64_t n, uint32_t* cnt)
+// children of every box (split_and_push: the upper half, then the lower)
+__global__ void k_wf_counts(const WFState* st, const WFBox* boxes, const uint8_t* flags,
+                            int64_t n, uint8_t* cmask, uint32_t* cnt)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j < n) cnt[j] = (uint32_t)__popc(cmask[j]);
+    if (j >= n) return;
+    const uint8_t f = flags[j];
+    uint8_t m = 0;
+    if (!(f & F_SKIP) && (f & F_ZERO)) {
+        const WFBox b = boxes[j];
+        const WFState& S = st[b.owner];
+        if (j < S.cut && !S.done && !S.restart) {
+            if (S.axis == 0) {
+                m = 3;
+            } else {
+                const double hu = S.axis == 1 ? b.u + 0.5 * S.wu : b.u;
+                const double hv = S.axis == 2 ? b.v + 0.5 * S.wv : b.v;
+                if (!S.q.vf || hu + hv <= 1) m |= 1; // upper half
+                if (!S.q.vf || b.u + b.v <= 1) m |= 2; // lower half
+            }
+        }
+    }
+    cmask[j] = m;
+    cnt[j] = (uint32_t)__popc(m);
+}
+
+// a search whose level left no children ends (at its earliest tolerance
+// box, if any)
+__global__ void k_wf_nochild(CCDArgs A, WFState* st, int64_t nb, const uint32_t* off,
+                             int64_t nbox, int64_t total_children)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb) return;
+    WFState& S = st[i];
+    if (S.done || S.restart) return;
+    const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
+    const int64_t a = s0 < nbox ? (int64_t)off[s0] : total_children;
+    const int64_t b = s1 < nbox ? (int64_t)off[s1] : total_children;
+    if (b == a)
+        wf_search_done(A, S, S.use_skip != 0, S.use_skip ? S.toi_skip : INFINITY, S.delta);
 }
 
 __global__ void k_wf_emit(const WFState* st, const WFBox* boxes, const uint8_t* cmask,
@@ -565,41 +625,51 @@ __global__ void k_wf_segments(WFState* st, int64_t nb, const uint32_t* off, int6
     S.next_count = b - a;
 }
 
-// order each candidate's next level: a t split interleaves (upper, lower)
-// pairs; merge the two sorted halves (ties: upper half first, it was pushed
-// first); other splits are already in order.
-__global__ void k_wf_order(WFState* st, int64_t nb, const WFBox* in, WFBox* out)
+// Orders each candidate's next level, a lane per box: a t split
+// interleaves (upper, lower) pairs of two t-sorted halves, merged with ties
+// going to the upper half (it was pushed first): an upper box lands after
+// the lower boxes strictly below it, a lower box after the upper boxes at
+// or below it.  Other splits and root boxes are already in order.
+__global__ void k_wf_order(const WFState* st, int64_t n, const WFBox* in, WFBox* out)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const WFBox b = in[k];
+    const WFState& S = st[b.owner];
+    const bool was_split = !S.done && !S.restart;
+    if (!was_split || S.axis != 0) {
+        out[k] = b;
+        return;
+    }
+    const int64_t s = S.next_start, half = S.next_count / 2, i = k - s;
+    const bool upper = (i & 1) == 0;
+    const int64_t own = i >> 1;
+    // other half: element r at s + 2 r + (upper ? 1 : 0)
+    const int64_t base = s + (upper ? 1 : 0);
+    int64_t lo = 0, hi = half; // count of the other half before b
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const double t = in[base + 2 * mid].t;
+        const bool before = upper ? t < b.t : t <= b.t;
+        if (before) lo = mid + 1;
+        else hi = mid;
+    }
+    out[s + own + lo] = b;
+}
+
+// after the order: halve the split axis and make the next level current
+__global__ void k_wf_advance(WFState* st, int64_t nb)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
     WFState& S = st[i];
-    const int64_t s = S.next_start, n = S.next_count;
-    const bool was_split = !S.done && !S.restart;
-    if (was_split && S.axis == 0) {
-        int64_t a = 0, b = 0, k = s; // a: uppers at s+2a, b: lowers at s+2b+1
-        const int64_t half = n / 2;
-        while (a < half && b < half) {
-            const WFBox hu = in[s + 2 * a], lo = in[s + 2 * b + 1];
-            if (hu.t <= lo.t) {
-                out[k++] = hu;
-                a++;
-            } else {
-                out[k++] = lo;
-                b++;
-            }
-        }
-        while (a < half) out[k++] = in[s + 2 * a++];
-        while (b < half) out[k++] = in[s + 2 * b++ + 1];
-    } else {
-        for (int64_t k = s; k < s + n; k++) out[k] = in[k];
-    }
-    if (was_split) {
+    if (!S.done && !S.restart) {
         if (S.axis == 0) S.wt *= 0.5;
         else if (S.axis == 1) S.wu *= 0.5;
         else S.wv *= 0.5;
     }
-    S.seg_start = s;
-    S.seg_count = n;
+    S.seg_start = S.next_start;
+    S.seg_count = S.next_count;
 }
 
 __global__ void k_wf_finish(CCDArgs A, const WFState* st, int64_t nb)
@@ -630,9 +700,11 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
     hipStream_t s = c.stream;
     int64_t nbox = nb;
     c.blk_keys.reserve(sizeof(WFBox) * (size_t)std::max<int64_t>(nbox, 1024));
+    c.wf_ev.reserve(sizeof(unsigned long long) * WF_EV * (size_t)nb);
+    unsigned long long* ev = c.wf_ev.as<unsigned long long>();
     WFBox* cur = reinterpret_cast<WFBox*>(c.blk_keys.p);
     hipLaunchKernelGGL(k_wf_init, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, heavy, first, nb,
-                       states, cur);
+                       states, cur, ev);
     IPCG_CHECK_LAUNCH();
     for (;;) {
         if (early_exit) {
@@ -645,24 +717,36 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
         uint8_t* cmask = flags + ((nbox + 63) / 64) * 64;
         c.blk_start.reserve(sizeof(uint32_t) * (size_t)(nbox + 1));
         c.blk_vals2.reserve(sizeof(uint32_t) * (size_t)(nbox + 1));
+        c.wf_live.reserve(sizeof(uint32_t) * (size_t)(nbox + 1));
+        c.wf_lpre.reserve(sizeof(uint32_t) * (size_t)(nbox + 1));
         uint32_t* cnt = c.blk_start.as<uint32_t>();
         uint32_t* off = c.blk_vals2.as<uint32_t>();
+        uint32_t* live = c.wf_live.as<uint32_t>();
+        uint32_t* lpre = c.wf_lpre.as<uint32_t>();
         c.blk_tmp.reserve(scan_tmp_bytes(nbox + 1));
-        hipLaunchKernelGGL(k_wf_eval, dim3(grid_for(nbox, 256, INT32_MAX)), dim3(256), 0, s, A,
-                           states, cur, nbox, flags, cmask);
+        const unsigned gb = grid_for(nbox, 256, INT32_MAX);
+        hipLaunchKernelGGL(k_wf_eval, dim3(gb), dim3(256), 0, s, A, states, cur, nbox, flags, live,
+                           error);
+        exclusive_scan_u32(live, lpre, nbox, lpre + nbox, c.blk_tmp.p, s);
+        hipLaunchKernelGGL(k_wf_mark, dim3(gb), dim3(256), 0, s, states, cur, nbox, flags, lpre,
+                           ev);
         hipLaunchKernelGGL(k_wf_decide, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb,
-                           cur, flags, cmask, error);
-        hipLaunchKernelGGL(k_wf_counts, dim3(grid_for(nbox, 256, INT32_MAX)), dim3(256), 0, s,
-                           cmask, nbox, cnt);
+                           cur, lpre, ev, error);
+        hipLaunchKernelGGL(k_wf_counts, dim3(gb), dim3(256), 0, s, states, cur, flags, nbox, cmask,
+                           cnt);
         exclusive_scan_u32(cnt, off, nbox, off + nbox, c.blk_tmp.p, s);
         IPCG_CHECK_LAUNCH();
         IPCG_HIP(hipMemcpyAsync(h, off + nbox, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+        const int64_t children = reinterpret_cast<uint32_t*>(h)[0];
+        hipLaunchKernelGGL(k_wf_nochild, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb,
+                           off, nbox, children);
         IPCG_HIP(hipMemsetAsync(npend, 0, 2 * sizeof(unsigned long long), s));
         hipLaunchKernelGGL(k_wf_pending, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, npend);
+        IPCG_CHECK_LAUNCH();
         IPCG_HIP(hipMemcpyAsync(h + 1, npend, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                 s));
         IPCG_HIP(hipStreamSynchronize(s));
-        const int64_t children = reinterpret_cast<uint32_t*>(h)[0];
         const int64_t pending = (int64_t)h[1], restarts = (int64_t)h[2];
         if (pending == 0) break;
         const int64_t nnext = children + restarts; // a restart is one root box
@@ -672,12 +756,12 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
         WFBox* nxt = reinterpret_cast<WFBox*>(c.blk_keys2.p);
         WFBox* ord = reinterpret_cast<WFBox*>(c.blk_ukeys.p);
         IPCG_HIP(hipMemsetAsync(nroot, 0, sizeof(unsigned long long), s));
-        hipLaunchKernelGGL(k_wf_emit, dim3(grid_for(nbox, 256, INT32_MAX)), dim3(256), 0, s, states,
-                           cur, cmask, off, nbox, nxt);
+        hipLaunchKernelGGL(k_wf_emit, dim3(gb), dim3(256), 0, s, states, cur, cmask, off, nbox, nxt);
         hipLaunchKernelGGL(k_wf_segments, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, off,
                            nbox, nroot, children, nxt);
-        hipLaunchKernelGGL(k_wf_order, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, nxt,
-                           ord);
+        hipLaunchKernelGGL(k_wf_order, dim3(grid_for(nnext, 256, INT32_MAX)), dim3(256), 0, s,
+                           states, nnext, nxt, ord);
+        hipLaunchKernelGGL(k_wf_advance, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb);
         IPCG_CHECK_LAUNCH();
         std::swap(c.blk_keys, c.blk_ukeys);
         cur = reinterpret_cast<WFBox*>(c.blk_keys.p);
@@ -848,8 +932,9 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
             c.stage_end();
             IPCG_HIP(hipMemcpyAsync(h, error, sizeof(int), hipMemcpyDeviceToHost, s));
             IPCG_HIP(hipStreamSynchronize(s));
-            if (*reinterpret_cast<int*>(h))
-                throw Error(IPCG_ERR_NUMERIC, "CCD root search exceeded the level limit");
+            const int err = *reinterpret_cast<int*>(h);
+            if (err & 2) throw Error(IPCG_ERR_NUMERIC, "CCD wavefront level out of t order");
+            if (err) throw Error(IPCG_ERR_NUMERIC, "CCD root search exceeded the level limit");
         }
     };
     c.stat["ccd.wavefront"] = 0;
