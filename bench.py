@@ -458,7 +458,7 @@ def main():
     ctx.timing(False)
 
     ccd_work = {k: ctx.counter(k) for k in ("ccd.candidates", "ccd.searched", "ccd.spilled",
-                                              "ccd.wavefront")}
+                                              "ccd.spilled_deep", "ccd.wavefront")}
     ee_other = ctx.counter("hessian.ee_other")
     counts = ctx.collision_counts()
     ctx.broad_phase(False, 0.5 * sc.dhat)

@@ -158,7 +158,7 @@ static std::vector<DevBuf*> all_bufs(Ctx& c)
                        &c.np_start, &c.np_tmp, &c.np_raw, &c.red_partial, &c.red_out, &c.grad, &c.hq,
                        &c.blk_keys, &c.blk_keys2, &c.blk_vals, &c.blk_vals2, &c.blk_ukeys, &c.blk_start,
                        &c.blk_tmp, &c.blk_mask, &c.row_cnt, &c.rowptr, &c.col, &c.val, &c.ccd_bound,
-                       &c.ccd_slab, &c.ccd_overflow, &c.ccd_toi, &c.ccd_status,
+                       &c.ccd_overflow, &c.ccd_toi, &c.ccd_status,
                        &c.hp_mark, &c.hp_bptr, &c.hp_bcol, &c.cand_stage[0], &c.cand_stage[1],
                        &c.shard_cnt, &c.shard_off, &c.np_eec, &c.ccd_stage, &c.ccd_todo,
                        &c.tv.leaf, &c.te.leaf, &c.tf.leaf, &c.inc_keys, &c.inc_keys2,

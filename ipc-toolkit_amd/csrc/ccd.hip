@@ -27,6 +27,7 @@
 #include "ti.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace ipcg {
 
@@ -222,8 +223,13 @@ __global__ __launch_bounds__(CB) void k_ccd_lds(CCDArgs A, const uint32_t* todo,
         r = ti_query_lds(q, A.min_distance, adj, -1, true, B2, L, toi);
         if (r == TI_HIT) toi *= A.rescale;
     }
-    if (r == TI_OVERFLOW) {
+    if (r >= TI_OVERFLOW) {
         spill[atomicAdd(n_spill, 1ull)] = (uint32_t)ci;
+        // searches spilled for depth (the rest: for list length), counted
+        // per wave
+        const unsigned long long deep = __ballot(r == TI_DEEP);
+        if (deep && (unsigned)lane_id() == (unsigned)(__ffsll((long long)deep) - 1))
+            atomicAdd(n_spill + 1, (unsigned long long)__popcll(deep));
         return;
     }
     report(A, ci, r == TI_HIT, toi, sc);
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(CB) void k_ccd_lanes(CCDArgs A, const uint32_t* tod
             r = ti_query(q, A.min_distance, adj, -1, true, B2, S, toi);
             if (r == TI_HIT) toi *= A.rescale;
         }
-        if (r == TI_OVERFLOW) {
+        if (r >= TI_OVERFLOW) {
             heavy[atomicAdd(n_heavy, 1ull)] = (uint32_t)ci;
             continue;
         }
@@ -772,6 +778,25 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
     return true;
 }
 
+// The slab tier's box lists (3 x LANE_CAP boxes per lane, up to LANES
+// lanes: 2.4 GB) are one buffer per device shared by every context on it:
+// a context holds the lock while it enqueues its slab launch behind the
+// previous user's (stream event), so the memory does not grow with the
+// number of contexts.
+struct SharedSlab {
+    std::mutex mu;
+    DevBuf buf;
+    hipEvent_t last = nullptr;
+};
+constexpr int SLAB_DEVICES = 64;
+SharedSlab g_slab[SLAB_DEVICES];
+
+SharedSlab& shared_slab(int device)
+{
+    if (device < 0 || device >= SLAB_DEVICES) throw Error(IPCG_ERR_ARG, "device index out of range");
+    return g_slab[device];
+}
+
 } // namespace
 
 __global__ void k_fill_f64(double* p, int n, double v)
@@ -808,7 +833,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     int* error = reinterpret_cast<int*>(st + 256);
     unsigned long long* nroot = reinterpret_cast<unsigned long long*>(st + 320);
     unsigned long long* npend = reinterpret_cast<unsigned long long*>(st + 384); // [2]
-    unsigned long long* n_spill = reinterpret_cast<unsigned long long*>(st + 448);
+    unsigned long long* n_spill = reinterpret_cast<unsigned long long*>(st + 448); // [2]
     const double one = 1.0;
     IPCG_HIP(hipMemsetAsync(st, 0, 512, s));
     if (c.batched()) { // one running minimum per scene
@@ -876,7 +901,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     // holds the spill list.
     auto search = [&](int64_t ci_lo, int64_t ci_hi) {
         uint32_t* spill = c.ccd_stage.as<uint32_t>();
-        IPCG_HIP(hipMemsetAsync(n_spill, 0, sizeof(unsigned long long), s));
+        IPCG_HIP(hipMemsetAsync(n_spill, 0, 2 * sizeof(unsigned long long), s));
         IPCG_HIP(hipMemsetAsync(work, 0, sizeof(unsigned long long), s));
         IPCG_HIP(hipMemsetAsync(n_heavy, 0, sizeof(unsigned long long), s));
         c.stage_begin("ccd_lds");
@@ -887,19 +912,26 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
             IPCG_CHECK_LAUNCH();
         }
         c.stage_end();
-        IPCG_HIP(hipMemcpyAsync(h, n_spill, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipMemcpyAsync(h, n_spill, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                s));
         IPCG_HIP(hipStreamSynchronize(s));
         const int64_t n2 = (int64_t)h[0];
         c.stat["ccd.spilled"] += n2;
+        c.stat["ccd.spilled_deep"] += (int64_t)h[1];
         c.stage_begin("ccd_lanes");
         if (n2 > 0) {
             const int64_t ln = std::min<int64_t>(LANES, ((n2 + CB - 1) / CB) * CB);
-            // slabs for the lanes launched only (a context that never
-            // spills many searches keeps a small slab)
-            c.ccd_slab.reserve(sizeof(TIBox) * 3 * (size_t)LANE_CAP * (size_t)ln);
+            SharedSlab& P = shared_slab(c.device);
+            std::lock_guard<std::mutex> lock(P.mu);
+            // slabs for the lanes launched only; growing frees the old
+            // buffer, which waits for the device to finish with it
+            P.buf.reserve(sizeof(TIBox) * 3 * (size_t)LANE_CAP * (size_t)ln);
+            if (!P.last) IPCG_HIP(hipEventCreateWithFlags(&P.last, hipEventDisableTiming));
+            else IPCG_HIP(hipStreamWaitEvent(s, P.last, 0)); // the previous user's launch
             hipLaunchKernelGGL(k_ccd_lanes, dim3((unsigned)(ln / CB)), dim3(CB), 0, s, A, spill,
-                               n2, c.ccd_slab.as<TIBox>(), work, heavy, n_heavy);
+                               n2, P.buf.as<TIBox>(), work, heavy, n_heavy);
             IPCG_CHECK_LAUNCH();
+            IPCG_HIP(hipEventRecord(P.last, s));
         }
         c.stage_end();
         IPCG_HIP(hipMemcpyAsync(h, n_heavy, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -939,6 +971,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     };
     c.stat["ccd.wavefront"] = 0;
     c.stat["ccd.spilled"] = 0;
+    c.stat["ccd.spilled_deep"] = 0;
     if (A.bounded) {
         // Face-vertex and codim candidates first: their roots are isolated
         // points, so they resolve fast and lower every scene's running

@@ -149,7 +149,7 @@ struct Ctx {
     bool hess_valid = false;
 
     // ccd
-    DevBuf ccd_bound, ccd_slab, ccd_overflow, ccd_toi, ccd_status;
+    DevBuf ccd_bound, ccd_overflow, ccd_toi, ccd_status;
     DevBuf ccd_stage, ccd_todo; // root-pass survivors (sharded, packed)
     DevBuf ccd_sbound;          // batched contexts: running minimum per scene
     DevBuf wf_ev, wf_live, wf_lpre; // CCD wavefront: per-candidate events, scanned boxes

@@ -35,7 +35,9 @@ struct TISlab {
     int cap;     // capacity of each list
 };
 
-enum TIResult : int { TI_MISS = 0, TI_HIT = 1, TI_OVERFLOW = 2 };
+// TI_OVERFLOW: a level outgrew the lists; TI_DEEP: a box needs more
+// halvings than the packed LDS form holds (both: search again on the slab)
+enum TIResult : int { TI_MISS = 0, TI_HIT = 1, TI_OVERFLOW = 2, TI_DEEP = 3 };
 
 // Guards that keep every search finite on the GPU: a level cap (each level
 // halves one width, so a search that resolves needs far fewer) and a cap on
@@ -397,7 +399,7 @@ __device__ inline int ti_bfs_lds(const TIQuery& q, double co_tol, double ms, lon
                 use_skip = true;
                 continue;
             }
-            if (nk > TI_LDS_BITS) return TI_OVERFLOW;
+            if (nk > TI_LDS_BITS) return TI_DEEP;
             if (axis == 0) {
                 if (nlo >= L.cap || nhi >= L.cap) return TI_OVERFLOW;
                 L.at(2, nhi++) = ti_pack(2 * it + 1, iu, iv);
