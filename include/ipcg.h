@@ -408,7 +408,9 @@ int ipcg_compute_collision_free_stepsize_batched(ipcg_ctx* ctx, double min_dista
  * gradient and Hessian requests share one collision build and evaluation
  * (BarrierPotential with PSD clamp, inflation 0.5 dhat) over the scenes that
  * asked; step requests share one CCD: the scene's positions are perturbed by
- * N(0, sigma^2) per coordinate (its own generator, seeded per scene) and the
+ * N(0, sigma^2) per coordinate (drawn on the device from a counter-based
+ * generator keyed by the scene's seed, its step count and the coordinate,
+ * so a draw depends on nothing else) and the
  * perturbed positions replace the current ones only when
  * compute_collision_free_stepsize returns 1 (min_distance 0, tolerance 1e-6,
  * 1e7 iterations, rescaling 0.8).  A scene's results do not depend on the

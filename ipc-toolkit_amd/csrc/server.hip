@@ -33,20 +33,56 @@ namespace {
 constexpr int64_t BYTES_PER_TRI = 4096;
 constexpr int64_t BYTES_PER_VERTEX = 1024;
 
-uint64_t splitmix64(uint64_t& x)
+// Step perturbation, drawn on the device from a counter-based generator:
+// coordinate i of the scene's draw number k takes its two uniforms from
+// the splitmix64 finalizer of (seed, k, i), so a draw depends on nothing but
+// the scene's own seed and step count.  Box-Muller over two 53-bit
+// uniforms, u1 in (0, 1].
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
-    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
 
-// standard normal draw (Box-Muller over two 53-bit uniforms in (0, 1])
-double normal01(uint64_t& st)
+__global__ void k_perturb(const double* X0, double* X1, int64_t v0, int64_t nv, double sigma,
+                          uint64_t seed, uint64_t draw)
 {
-    const double u1 = ((splitmix64(st) >> 11) + 1) * (1.0 / 9007199254740992.0);
-    const double u2 = (splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= 3 * nv) return;
+    const int64_t v = i / 3, d = i - 3 * v;
+    const uint64_t key = seed * 0x9E3779B97F4A7C15ull + draw * 0xD1B54A32D192ED03ull
+                       + (uint64_t)i * 2ull;
+    const double u1 = (double)((mix64(key + 1) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(mix64(key + 2) >> 11) * (1.0 / 9007199254740992.0);
+    const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    const int64_t at = 4 * (v0 + v) + d;
+    X1[at] = X0[at] + sigma * z;
+}
+
+// per scene k: the squared 2-norm of the gradient rows [3 voff[k], 3
+// voff[k+1]) (a fixed reduction order per row count, so the same for the
+// scene in any batch) and the Hessian nonzeros of those rows
+__global__ __launch_bounds__(256) void k_scene_norms(const double* g, const int64_t* rowptr,
+                                                     const int64_t* voff, double* norm2,
+                                                     int64_t* nnz)
+{
+    __shared__ double part[256];
+    const int k = blockIdx.x;
+    const int64_t r0 = 3 * voff[k], r1 = 3 * voff[k + 1];
+    double acc = 0;
+    if (g)
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) acc += g[r] * g[r];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        norm2[k] = part[0];
+        nnz[k] = rowptr ? rowptr[r1] - rowptr[r0] : 0;
+    }
 }
 
 struct Scene {
@@ -54,7 +90,7 @@ struct Scene {
     std::vector<double> rest, V; // rest and current positions (nV x 3)
     std::vector<int32_t> E, F;   // scene-local topology
     double sigma = 0;            // perturbation standard deviation of a step
-    uint64_t rng = 0;
+    uint64_t seed = 0, draws = 0; // perturbation generator: seed, draws so far
     int64_t footprint = 0;
     std::deque<int64_t> pending; // tickets in arrival order
     int32_t slot = -1;           // index in the current batch
@@ -83,6 +119,7 @@ struct Server {
     std::vector<Req> reqs;
     std::vector<int32_t> resident; // scene ids in batch order
     std::vector<int64_t> voff;     // vertex offsets of the batch
+    DevBuf voff_d, red;            // device copy of voff; per-scene reductions
     bool ran = false;
     ipcg_server_stats stats {};
     std::string err;
@@ -102,10 +139,29 @@ std::vector<double> records(const double* V, int64_t n)
     return r;
 }
 
+// the resident scenes' current positions (device slot 0) into their host
+// copies
+void download_positions(Server& S)
+{
+    Ctx& c = S.ctx->c;
+    if (S.resident.empty() || !c.has_pos[0]) return;
+    std::vector<double> r(4 * (size_t)S.voff.back());
+    IPCG_HIP(hipMemcpyAsync(r.data(), c.X[0].p, sizeof(double) * r.size(), hipMemcpyDeviceToHost,
+                            c.stream));
+    IPCG_HIP(hipStreamSynchronize(c.stream));
+    for (int32_t id : S.resident) {
+        Scene& sc = S.scenes[id];
+        const int64_t v0 = S.voff[sc.slot];
+        for (int64_t i = 0; i < sc.nV; i++)
+            for (int d = 0; d < 3; d++) sc.V[3 * i + d] = r[4 * (v0 + i) + d];
+    }
+}
+
 // rebuild the batch from the scenes in `ids` (positions from the host)
 void rebuild(Server& S, const std::vector<int32_t>& ids)
 {
     Ctx& c = S.ctx->c;
+    download_positions(S);
     for (int32_t id : S.resident) S.scenes[id].slot = -1;
     S.resident = ids;
     const size_t n = ids.size();
@@ -135,6 +191,9 @@ void rebuild(Server& S, const std::vector<int32_t>& ids)
                                 hipMemcpyHostToDevice, c.stream));
         c.has_pos[slot] = true;
     }
+    S.voff_d.reserve(sizeof(int64_t) * voff.size());
+    IPCG_HIP(hipMemcpyAsync(S.voff_d.p, voff.data(), sizeof(int64_t) * voff.size(),
+                            hipMemcpyHostToDevice, c.stream));
     IPCG_HIP(hipStreamSynchronize(c.stream));
     S.voff = voff;
     S.stats.epochs++;
@@ -201,6 +260,24 @@ void serve_potentials(Server& S, const std::vector<int64_t>& batch, double (*now
     // block empty: nothing to launch
     if (want_g && total > 0) pot_gradient(c, S.dhat, S.kappa);
     if (want_h && total > 0) pot_hessian(c, S.dhat, S.kappa, IPCG_PSD_CLAMP);
+    // gradient norms and Hessian nonzeros of every scene: one launch, one copy
+    std::vector<double> norm2(n, 0.0);
+    std::vector<int64_t> nnz(n, 0);
+    if ((want_g || want_h) && total > 0) {
+        S.red.reserve((sizeof(double) + sizeof(int64_t)) * n);
+        double* d_norm = S.red.as<double>();
+        int64_t* d_nnz = reinterpret_cast<int64_t*>(d_norm + n);
+        hipLaunchKernelGGL(k_scene_norms, dim3((unsigned)n), dim3(256), 0, c.stream,
+                           want_g ? c.grad.as<double>() : nullptr,
+                           want_h ? c.rowptr.as<int64_t>() : nullptr, S.voff_d.as<int64_t>(),
+                           d_norm, d_nnz);
+        IPCG_CHECK_LAUNCH();
+        IPCG_HIP(hipMemcpyAsync(norm2.data(), d_norm, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                c.stream));
+        IPCG_HIP(hipMemcpyAsync(nnz.data(), d_nnz, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                                c.stream));
+        IPCG_HIP(hipStreamSynchronize(c.stream));
+    }
     for (int64_t t : batch) {
         Req& q = S.reqs[t];
         const Scene& sc = S.scenes[q.scene];
@@ -212,45 +289,42 @@ void serve_potentials(Server& S, const std::vector<int64_t>& batch, double (*now
         if (q.kind == IPCG_REQ_ENERGY) {
             q.value = energy[k];
         } else if (q.kind == IPCG_REQ_GRADIENT) {
-            std::vector<double> g((size_t)(r1 - r0), 0.0);
-            if (total > 0) {
-                IPCG_HIP(hipMemcpyAsync(g.data(), c.grad.as<double>() + r0,
-                                        sizeof(double) * g.size(), hipMemcpyDeviceToHost,
-                                        c.stream));
-                IPCG_HIP(hipStreamSynchronize(c.stream));
+            q.value = std::sqrt(norm2[k]);
+            if (S.keep) { // the scene's rows on the host
+                q.grad.assign((size_t)(r1 - r0), 0.0);
+                if (total > 0) {
+                    IPCG_HIP(hipMemcpyAsync(q.grad.data(), c.grad.as<double>() + r0,
+                                            sizeof(double) * q.grad.size(),
+                                            hipMemcpyDeviceToHost, c.stream));
+                    IPCG_HIP(hipStreamSynchronize(c.stream));
+                }
             }
-            double s2 = 0;
-            for (double x : g) s2 += x * x;
-            q.value = std::sqrt(s2);
-            if (S.keep) q.grad = std::move(g);
         } else if (total == 0) { // Hessian of no collision
             q.value = 0;
             if (S.keep) q.rowptr.assign((size_t)(r1 - r0 + 1), 0);
         } else { // Hessian: the scene's rows stay on the device unless kept
-            int64_t rp[2];
-            IPCG_HIP(hipMemcpyAsync(rp, c.rowptr.as<int64_t>() + r0, sizeof(int64_t),
-                                    hipMemcpyDeviceToHost, c.stream));
-            IPCG_HIP(hipMemcpyAsync(rp + 1, c.rowptr.as<int64_t>() + r1, sizeof(int64_t),
-                                    hipMemcpyDeviceToHost, c.stream));
-            IPCG_HIP(hipStreamSynchronize(c.stream));
-            q.value = (double)(rp[1] - rp[0]);
+            q.value = (double)nnz[k];
             if (S.keep) {
+                int64_t rp0 = 0;
+                IPCG_HIP(hipMemcpyAsync(&rp0, c.rowptr.as<int64_t>() + r0, sizeof(int64_t),
+                                        hipMemcpyDeviceToHost, c.stream));
+                IPCG_HIP(hipStreamSynchronize(c.stream));
                 q.rowptr.resize((size_t)(r1 - r0 + 1));
-                q.col.resize((size_t)(rp[1] - rp[0]));
+                q.col.resize((size_t)nnz[k]);
                 q.val.resize(q.col.size());
                 IPCG_HIP(hipMemcpyAsync(q.rowptr.data(), c.rowptr.as<int64_t>() + r0,
                                         sizeof(int64_t) * q.rowptr.size(), hipMemcpyDeviceToHost,
                                         c.stream));
                 if (!q.col.empty()) {
-                    IPCG_HIP(hipMemcpyAsync(q.col.data(), c.col.as<int32_t>() + rp[0],
+                    IPCG_HIP(hipMemcpyAsync(q.col.data(), c.col.as<int32_t>() + rp0,
                                             sizeof(int32_t) * q.col.size(),
                                             hipMemcpyDeviceToHost, c.stream));
-                    IPCG_HIP(hipMemcpyAsync(q.val.data(), c.val.as<double>() + rp[0],
+                    IPCG_HIP(hipMemcpyAsync(q.val.data(), c.val.as<double>() + rp0,
                                             sizeof(double) * q.val.size(), hipMemcpyDeviceToHost,
                                             c.stream));
                 }
                 IPCG_HIP(hipStreamSynchronize(c.stream));
-                for (int64_t& x : q.rowptr) x -= rp[0];
+                for (int64_t& x : q.rowptr) x -= rp0;
                 for (int32_t& x : q.col) x -= (int32_t)r0;
             }
         }
@@ -264,33 +338,29 @@ void serve_steps(Server& S, const std::vector<int64_t>& batch, double (*now)(voi
     Ctx& c = S.ctx->c;
     const size_t n = S.resident.size();
     std::vector<uint8_t> mask(n, 0);
-    std::vector<std::vector<double>> V1(batch.size());
-    for (size_t b = 0; b < batch.size(); b++) {
-        Scene& sc = S.scenes[S.reqs[batch[b]].scene];
+    // the perturbed positions of every stepping scene into slot 1
+    for (int64_t t : batch) {
+        Scene& sc = S.scenes[S.reqs[t].scene];
         mask[sc.slot] = 1;
-        V1[b] = sc.V;
-        for (double& x : V1[b]) x += sc.sigma * normal01(sc.rng);
-        const std::vector<double> r = records(V1[b].data(), sc.nV);
-        IPCG_HIP(hipMemcpyAsync(c.X[1].as<double>() + 4 * S.voff[sc.slot], r.data(),
-                                sizeof(double) * r.size(), hipMemcpyHostToDevice, c.stream));
-        IPCG_HIP(hipStreamSynchronize(c.stream)); // r goes out of scope
+        hipLaunchKernelGGL(k_perturb, dim3(grid_for(3 * (int64_t)sc.nV, 256)), dim3(256), 0,
+                           c.stream, c.X[0].as<double>(), c.X[1].as<double>(), S.voff[sc.slot],
+                           (int64_t)sc.nV, sc.sigma, sc.seed, sc.draws++);
     }
+    IPCG_CHECK_LAUNCH();
     batch_set_active(c, mask.data());
     bp_build(c, true, 0.0, IPCG_BP_LBVH);
     std::vector<double> steps(n, 1.0);
     ccd_stepsize(c, 0.0, 1e-6, 10000000, 0.8, false, nullptr, steps.data());
-    for (size_t b = 0; b < batch.size(); b++) {
-        Req& q = S.reqs[batch[b]];
+    for (int64_t t : batch) {
+        Req& q = S.reqs[t];
         Scene& sc = S.scenes[q.scene];
         q.value = steps[sc.slot];
         q.accepted = q.value >= 1.0;
-        if (q.accepted) { // the perturbed positions become the scene's state
+        if (q.accepted) // the perturbed positions become the scene's state
             IPCG_HIP(hipMemcpyAsync(c.X[0].as<double>() + 4 * S.voff[sc.slot],
                                     c.X[1].as<double>() + 4 * S.voff[sc.slot],
                                     sizeof(double) * 4 * (size_t)sc.nV, hipMemcpyDeviceToDevice,
                                     c.stream));
-            sc.V = std::move(V1[b]);
-        }
     }
     IPCG_HIP(hipStreamSynchronize(c.stream));
     const double f = now(clk);
@@ -403,6 +473,7 @@ void server_run(Server& S)
     }
     S.stats.busy_seconds = busy;
     S.stats.wall_seconds = seconds_since(t0);
+    download_positions(S);
 }
 
 } // namespace ipcg
@@ -501,7 +572,7 @@ int ipcg_server_add_scene(ipcg_server* srv, int32_t nV, const double* V, int32_t
         sc.E.assign(E, E + 2 * (size_t)nE);
         sc.F.assign(F, F + 3 * (size_t)nF);
         sc.sigma = sigma;
-        sc.rng = seed;
+        sc.seed = seed;
         sc.footprint = BYTES_PER_TRI * nF + BYTES_PER_VERTEX * nV;
         *scene = (int32_t)S.scenes.size();
         S.scenes.push_back(std::move(sc));
