@@ -681,7 +681,13 @@ __global__ __launch_bounds__(256) void k_incidence(CollView C, int64_t goff, int
     }
 }
 
-// first incidence of every vertex in the sorted keys (vptr[nV] = M)
+// First incidence of every vertex in the sorted keys (vptr[nV] = M): lane i
+// writes the vertices (keys[i-1], keys[i]] when they are few; the vertices
+// of a longer gap (no incidence: a masked scene of a batch, a free part of
+// the mesh) keep the fill value and k_gap_ptr finds them by binary search.
+constexpr int RUN_PTR_GAP = 32;
+constexpr uint32_t RUN_PTR_NONE = 0xFFFFFFFFu;
+
 __global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M, int nV,
                                                  uint32_t* vptr)
 {
@@ -689,7 +695,22 @@ __global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M
     if (i > M) return;
     const int64_t prev = i > 0 ? (int64_t)keys[i - 1] : -1;
     const int64_t cur = i < M ? (int64_t)keys[i] : (int64_t)nV;
-    for (int64_t v = prev + 1; v <= cur; v++) vptr[v] = (uint32_t)i;
+    const int64_t from = cur - prev <= RUN_PTR_GAP ? prev + 1 : cur;
+    for (int64_t v = from; v <= cur; v++) vptr[v] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_gap_ptr(const uint64_t* keys, int64_t M, int nV,
+                                                 uint32_t* vptr)
+{
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nV || vptr[v] != RUN_PTR_NONE) return;
+    int64_t lo = 0, hi = M; // first i with keys[i] >= v
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    vptr[v] = (uint32_t)lo;
 }
 
 // edge-edge collisions of the other (non-interior) types, whose Hessians
@@ -1132,8 +1153,12 @@ void require_incidences(Ctx& c, const CollView& C, const CollRanges& G, int64_t 
     while ((1ll << vb) <= c.nV) vb++;
     radix_sort_pairs(keys, vals, c.inc_keys2.as<uint64_t>(), c.inc_vals2.as<uint32_t>(), M, vb,
                      c.blk_tmp.p, s);
+    if (M >= (int64_t)RUN_PTR_NONE) throw Error(IPCG_ERR_ARG, "too many collision incidences");
+    IPCG_HIP(hipMemsetAsync(c.inc_ptr.p, 0xFF, sizeof(uint32_t) * (size_t)c.nV, s));
     hipLaunchKernelGGL(k_run_ptr, dim3(grid_for(M + 1, 256, INT32_MAX)), dim3(256), 0, s, keys, M,
                        c.nV, c.inc_ptr.as<uint32_t>());
+    hipLaunchKernelGGL(k_gap_ptr, dim3(grid_for(c.nV, 256)), dim3(256), 0, s, keys, M, c.nV,
+                       c.inc_ptr.as<uint32_t>());
     c.inc_vtx.reserve(sizeof(int4) * (size_t)mk);
     hipLaunchKernelGGL(k_inc_vertices, dim3(grid_for(M, 256, INT32_MAX)), dim3(256), 0, s, C, G,
                        vals, M, c.inc_vtx.as<int4>());
