@@ -1165,6 +1165,7 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
         build_tree(c, c.tf, c.F.as<int32_t>(), 3, c.nF, X0, X1, inflation, true, nullptr,
                    fo ? &c.h_foff : nullptr);
     }
+    c.row_order_valid = true; // c.aabb: the box of every scene
     // codim lists (candidates.cpp:65-118): a tree over the codim vertices
     // (its own queries) and one over the codim edges; LBVH semantics with
     // either method

@@ -171,7 +171,7 @@ static std::vector<DevBuf*> all_bufs(Ctx& c)
                        &c.im_slot, &c.im_slot2, &c.im_vals, &c.im_vals2, &c.im_ukeys,
                        &c.im_start, &c.im_cnt, &c.im_off, &c.im_vv, &c.im_vvw, &c.im_ev,
                        &c.im_evw, &c.im_ee, &c.fric, &c.vel, &c.fric_mu,
-                       &c.np_cw, &c.wf_ev, &c.wf_live, &c.wf_lpre };
+                       &c.np_cw, &c.wf_ev, &c.wf_live, &c.wf_lpre, &c.ro_keys, &c.ro_keys2, &c.ro_ids, &c.ro_ids2 };
     for (Tree* T : { &c.tcv, &c.tce })
         for (DevBuf* b : { &T->box, &T->keys, &T->keys2, &T->ids, &T->ids2, &T->nodes, &T->info,
                            &T->root, &T->sbox, &T->leaf })
@@ -433,6 +433,7 @@ static void mesh_upload(Ctx& c, int32_t nV, const double* rest, int layout, int3
         c.has_pos[0] = c.has_pos[1] = false;
         c.cand_valid = c.coll_valid = c.hess_valid = false;
         c.coff_valid = false;
+        c.row_order_valid = c.ro_ready = false;
         c.fric_valid = c.has_vel = false;
         c.scap_ee = c.scap_fv = c.scap_vv = c.scap_ev = 0;
         if (cv_out) *cv_out = cv;

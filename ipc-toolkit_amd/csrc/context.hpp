@@ -143,6 +143,11 @@ struct Ctx {
     DevBuf vel;                       // velocities (3 nV, vertex major)
     bool has_vel = false;
     bool inc_valid = false;
+    // aabb holds the box of every scene (set by a broad phase), so the
+    // per-vertex row kernels can visit the vertices in spatial order:
+    // ro_ids, built with the incidences (ro_ready)
+    bool row_order_valid = false, ro_ready = false;
+    DevBuf ro_keys, ro_keys2, ro_ids, ro_ids2;
     int64_t nblocks = 0;
     bool pattern_valid = false;
     int64_t nnz = 0;

@@ -543,12 +543,15 @@ struct GradRecs {
     const double* rec[4];
 };
 
+// order: vertex visited by each wave (null: vertex order)
 __global__ __launch_bounds__(256) void k_gradient_rows(CollRanges G, GradRecs R,
                                                        const uint32_t* vptr, const uint32_t* inc,
-                                                       int nV, double* grad)
+                                                       int nV, double* grad,
+                                                       const uint32_t* order)
 {
-    const int64_t v = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
-    if (v >= nV) return;
+    const int64_t w = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
+    if (w >= nV) return;
+    const int64_t v = order ? (int64_t)order[w] : w;
     const int l = lane_id();
     double gx = 0, gy = 0, gz = 0;
     for (uint32_t j = vptr[v] + l; j < vptr[v + 1]; j += WAVE) {
@@ -810,14 +813,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                                       const uint32_t* inc, const int4* inc_vtx,
                                                       const uint32_t* bptr,
                                                       const int32_t* bcol, int nV, double* bval,
-                                                      uint32_t* bmask)
+                                                      uint32_t* bmask, const uint32_t* order)
 {
     __shared__ double acc[256 / WAVE][RW][9];
     __shared__ uint32_t msk[256 / WAVE][RW];
     __shared__ int32_t cols[256 / WAVE][RW];
     const int wv = threadIdx.x / WAVE, l = lane_id();
-    const int64_t v = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
-    if (v >= nV) return;
+    const int64_t w = ((int64_t)xcd_block() * 256 + threadIdx.x) / WAVE; // uniform per wave
+    if (w >= nV) return;
+    const int64_t v = order ? (int64_t)order[w] : w; // vertex of this wave
     const uint32_t row_s = bptr[v], m = bptr[v + 1] - row_s;
     const uint32_t i0 = vptr[v], i1 = vptr[v + 1];
     for (uint32_t w0 = 0; w0 < m; w0 += RW) {
@@ -1123,6 +1127,78 @@ CollRanges ranges(const CollView& C, int64_t& M)
     return G;
 }
 
+// The order in which the per-vertex row kernels visit the vertices: Morton
+// order of the vertex positions with one scale for the three axes (cubic
+// cells of the scene's box, 13 bits per axis), scenes of a batch in turn.
+// The vertices of one collision are within dhat of each other, so they are
+// visited close together and the records they share are read while they are
+// still in L2.  No result depends on the order: every row is computed on
+// its own.
+__device__ __forceinline__ uint64_t spread3(uint64_t v)
+{
+    v = (v | v << 32) & 0x1F00000000FFFFull;
+    v = (v | v << 16) & 0x1F0000FF0000FFull;
+    v = (v | v << 8) & 0x100F00F00F00F00Full;
+    v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_row_keys(const double* X, int nV, const int32_t* vscene,
+                                                  const double* aabb, uint64_t* keys,
+                                                  uint32_t* ids)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nV) return;
+    const int sc = vscene ? vscene[i] : 0;
+    const double* b = aabb + 6 * (int64_t)sc;
+    double ext = b[3] - b[0];
+    ext = b[4] - b[1] > ext ? b[4] - b[1] : ext;
+    ext = b[5] - b[2] > ext ? b[5] - b[2] : ext;
+    const double inv = ext > 0 ? 8192.0 / ext : 0.0;
+    const d3 x = ldx(X, nV, (int)i);
+    const double xs[3] = { x.x, x.y, x.z };
+    uint64_t code = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double t = (xs[k] - b[k]) * inv;
+        t = t < 0.0 ? 0.0 : (t > 8191.0 ? 8191.0 : t);
+        code |= spread3((uint64_t)t) << (2 - k);
+    }
+    keys[i] = ((uint64_t)sc << 39) | code;
+    ids[i] = (uint32_t)i;
+}
+
+static void build_row_order(Ctx& c)
+{
+    c.ro_ready = false;
+    if (!c.row_order_valid || c.nV <= 1) return;
+    hipStream_t s = c.stream;
+    const int64_t n = c.nV;
+    c.ro_keys.reserve(sizeof(uint64_t) * (size_t)n);
+    c.ro_keys2.reserve(sizeof(uint64_t) * (size_t)n);
+    c.ro_ids.reserve(sizeof(uint32_t) * (size_t)n);
+    c.ro_ids2.reserve(sizeof(uint32_t) * (size_t)n);
+    c.blk_tmp.reserve(radix_tmp_bytes(n));
+    hipLaunchKernelGGL(k_row_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, c.X[0].as<double>(),
+                       c.nV, c.batched() ? c.vscene.as<int32_t>() : nullptr, c.aabb.as<double>(),
+                       c.ro_keys.as<uint64_t>(), c.ro_ids.as<uint32_t>());
+    IPCG_CHECK_LAUNCH();
+    int bits = 39;
+    if (c.batched())
+        while ((1ll << (bits - 39)) < c.n_scenes) bits++;
+    radix_sort_pairs(c.ro_keys.as<uint64_t>(), c.ro_ids.as<uint32_t>(), c.ro_keys2.as<uint64_t>(),
+                     c.ro_ids2.as<uint32_t>(), n, bits, c.blk_tmp.p, s);
+    c.ro_ready = true;
+}
+
+// the order in which the per-vertex row kernels visit the vertices (null:
+// vertex order)
+static const uint32_t* row_order(const Ctx& c)
+{
+    return c.ro_ready ? c.ro_ids.as<uint32_t>() : nullptr;
+}
+
 // (vertex, global collision g, point a) incidences sorted by vertex (stable,
 // so each vertex's incidences keep the collision order) and the first
 // incidence of every vertex; kept until the next collision build.
@@ -1163,6 +1239,7 @@ void require_incidences(Ctx& c, const CollView& C, const CollRanges& G, int64_t 
     hipLaunchKernelGGL(k_inc_vertices, dim3(grid_for(M, 256, INT32_MAX)), dim3(256), 0, s, C, G,
                        vals, M, c.inc_vtx.as<int4>());
     IPCG_CHECK_LAUNCH();
+    build_row_order(c);
     c.inc_valid = true;
     c.stage_end();
 }
@@ -1197,7 +1274,7 @@ void pot_gradient(Ctx& c, double dhat, double kappa)
     if (C.n[3]) hipLaunchKernelGGL(k_gradient_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
     const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_gradient_rows, dim3(gw), dim3(256), 0, s, G, R, c.inc_ptr.as<uint32_t>(),
-                       c.inc_vals.as<uint32_t>(), c.nV, g);
+                       c.inc_vals.as<uint32_t>(), c.nV, g, row_order(c));
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 }
@@ -1396,7 +1473,7 @@ static void hess_assemble(Ctx& c)
     const unsigned gwr = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_hessian_rows, dim3(gwr), dim3(256), 0, s, rec, c.inc_ptr.as<uint32_t>(),
                        c.inc_vals.as<uint32_t>(), c.inc_vtx.as<int4>(), bptr, bcol, c.nV, bval,
-                       bmask);
+                       bmask, row_order(c));
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 
@@ -1455,7 +1532,7 @@ void pot_gn_diagonal(Ctx& c, double dhat, double kappa)
     if (C.n[3]) hipLaunchKernelGGL(k_gn_diag_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, X, c.nV, bp, const_cast<double*>(R.rec[3]));
     const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_gradient_rows, dim3(gw), dim3(256), 0, s, G, R, c.inc_ptr.as<uint32_t>(),
-                       c.inc_vals.as<uint32_t>(), c.nV, c.gn_out.as<double>());
+                       c.inc_vals.as<uint32_t>(), c.nV, c.gn_out.as<double>(), row_order(c));
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 }
@@ -2106,7 +2183,7 @@ void fric_gradient(Ctx& c, double eps_v)
     if (C.n[3]) hipLaunchKernelGGL(k_fric_gradient_records<K_FV>, dim3(grid_for(C.n[3], PB, INT32_MAX)), dim3(PB), 0, s, C, R + G.goff[3], Vel, c.nV, fp, const_cast<double*>(GR.rec[3]));
     const unsigned gw = grid_for((int64_t)c.nV * WAVE, 256, INT32_MAX); // a wave per vertex
     hipLaunchKernelGGL(k_gradient_rows, dim3(gw), dim3(256), 0, s, G, GR, c.inc_ptr.as<uint32_t>(),
-                       c.inc_vals.as<uint32_t>(), c.nV, c.grad.as<double>());
+                       c.inc_vals.as<uint32_t>(), c.nV, c.grad.as<double>(), row_order(c));
     IPCG_CHECK_LAUNCH();
     c.stage_end();
 }

@@ -60,29 +60,47 @@ __global__ void k_perturb(const double* X0, double* X1, int64_t v0, int64_t nv, 
     X1[at] = X0[at] + sigma * z;
 }
 
-// per scene k: the squared 2-norm of the gradient rows [3 voff[k], 3
-// voff[k+1]) (a fixed reduction order per row count, so the same for the
-// scene in any batch) and the Hessian nonzeros of those rows
-__global__ __launch_bounds__(256) void k_scene_norms(const double* g, const int64_t* rowptr,
-                                                     const int64_t* voff, double* norm2,
-                                                     int64_t* nnz)
+// Per scene k: the squared 2-norm of the gradient rows [3 voff[k], 3
+// voff[k+1]) and the Hessian nonzeros of those rows.  The rows of a scene
+// are cut into chunks of NORM_CHUNK counted from its first row, each summed
+// by one block in a fixed order, and the chunks of a scene are added in
+// order, so the value is the same for the scene in any batch.
+constexpr int64_t NORM_CHUNK = 8192;
+
+__global__ __launch_bounds__(256) void k_norm_chunks(const double* g, const int64_t* voff,
+                                                     const int64_t* coff, int n, double* part)
 {
-    __shared__ double part[256];
-    const int k = blockIdx.x;
-    const int64_t r0 = 3 * voff[k], r1 = 3 * voff[k + 1];
+    __shared__ double red[256];
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = n; // scene of this chunk: last k with coff[k] <= b
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (coff[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const int64_t r0 = 3 * voff[lo] + (b - coff[lo]) * NORM_CHUNK;
+    const int64_t r1 = min(3 * voff[lo + 1], r0 + NORM_CHUNK);
     double acc = 0;
-    if (g)
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) acc += g[r] * g[r];
-    part[threadIdx.x] = acc;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) acc += g[r] * g[r];
+    red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        norm2[k] = part[0];
-        nnz[k] = rowptr ? rowptr[r1] - rowptr[r0] : 0;
-    }
+    if (threadIdx.x == 0) part[b] = red[0];
+}
+
+__global__ void k_scene_norms(const double* part, const int64_t* rowptr, const int64_t* voff,
+                              const int64_t* coff, int n, double* norm2, int64_t* nnz)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    double acc = 0;
+    if (part)
+        for (int64_t b = coff[k]; b < coff[k + 1]; b++) acc += part[b];
+    norm2[k] = acc;
+    nnz[k] = rowptr ? rowptr[3 * voff[k + 1]] - rowptr[3 * voff[k]] : 0;
 }
 
 struct Scene {
@@ -119,7 +137,8 @@ struct Server {
     std::vector<Req> reqs;
     std::vector<int32_t> resident; // scene ids in batch order
     std::vector<int64_t> voff;     // vertex offsets of the batch
-    DevBuf voff_d, red;            // device copy of voff; per-scene reductions
+    int64_t nchunks = 0;           // gradient-norm chunks of the batch
+    DevBuf voff_d, coff_d, red;    // voff, chunk offsets per scene; reductions
     bool ran = false;
     ipcg_server_stats stats {};
     std::string err;
@@ -191,9 +210,16 @@ void rebuild(Server& S, const std::vector<int32_t>& ids)
                                 hipMemcpyHostToDevice, c.stream));
         c.has_pos[slot] = true;
     }
+    std::vector<int64_t> coff(n + 1, 0);
+    for (size_t k = 0; k < n; k++)
+        coff[k + 1] = coff[k] + (3 * (voff[k + 1] - voff[k]) + NORM_CHUNK - 1) / NORM_CHUNK;
     S.voff_d.reserve(sizeof(int64_t) * voff.size());
+    S.coff_d.reserve(sizeof(int64_t) * coff.size());
     IPCG_HIP(hipMemcpyAsync(S.voff_d.p, voff.data(), sizeof(int64_t) * voff.size(),
                             hipMemcpyHostToDevice, c.stream));
+    IPCG_HIP(hipMemcpyAsync(S.coff_d.p, coff.data(), sizeof(int64_t) * coff.size(),
+                            hipMemcpyHostToDevice, c.stream));
+    S.nchunks = coff[n];
     IPCG_HIP(hipStreamSynchronize(c.stream));
     S.voff = voff;
     S.stats.epochs++;
@@ -264,13 +290,19 @@ void serve_potentials(Server& S, const std::vector<int64_t>& batch, double (*now
     std::vector<double> norm2(n, 0.0);
     std::vector<int64_t> nnz(n, 0);
     if ((want_g || want_h) && total > 0) {
-        S.red.reserve((sizeof(double) + sizeof(int64_t)) * n);
+        S.red.reserve((sizeof(double) + sizeof(int64_t)) * n
+                      + sizeof(double) * (size_t)std::max<int64_t>(S.nchunks, 1));
         double* d_norm = S.red.as<double>();
         int64_t* d_nnz = reinterpret_cast<int64_t*>(d_norm + n);
-        hipLaunchKernelGGL(k_scene_norms, dim3((unsigned)n), dim3(256), 0, c.stream,
-                           want_g ? c.grad.as<double>() : nullptr,
-                           want_h ? c.rowptr.as<int64_t>() : nullptr, S.voff_d.as<int64_t>(),
-                           d_norm, d_nnz);
+        double* d_part = reinterpret_cast<double*>(d_nnz + n);
+        if (want_g && S.nchunks > 0)
+            hipLaunchKernelGGL(k_norm_chunks, dim3((unsigned)S.nchunks), dim3(256), 0, c.stream,
+                               c.grad.as<double>(), S.voff_d.as<int64_t>(),
+                               S.coff_d.as<int64_t>(), (int)n, d_part);
+        hipLaunchKernelGGL(k_scene_norms, dim3(grid_for((int64_t)n, 256)), dim3(256), 0, c.stream,
+                           want_g ? d_part : nullptr, want_h ? c.rowptr.as<int64_t>() : nullptr,
+                           S.voff_d.as<int64_t>(), S.coff_d.as<int64_t>(), (int)n, d_norm,
+                           d_nnz);
         IPCG_CHECK_LAUNCH();
         IPCG_HIP(hipMemcpyAsync(norm2.data(), d_norm, sizeof(double) * n, hipMemcpyDeviceToHost,
                                 c.stream));
