@@ -26,7 +26,7 @@ constexpr int BP_BLOCK = 256;
 constexpr int TR_BLOCK = 128;
 constexpr int TR_STACK = 64;  // lbvh.cpp:378 MAX_STACK_SIZE
 constexpr int TR_LSTACK = 16; // entries per lane kept in LDS; the rest in HBM
-constexpr int TR_BUF = 4;     // hits a lane holds before the wave appends
+constexpr int TR_BUF = 4;     // overlapping leaves a lane holds before the wave settles them
 
 // AABB::from_point (+ swept union), aabb.cpp:21-55
 __device__ __forceinline__ void vbox(const double* X0, const double* X1, int nV, int i, double r,
@@ -565,14 +565,41 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
     }
     const int nin = nleaves - 1; // internal nodes
     const int sh = wave_shard();
-    // Hits wait in registers (up to TR_BUF per lane) and the wave appends them
-    // together when a lane could overflow or the walk ends: one atomic per
-    // several iterations instead of one per iteration with a hit.
-    int2 buf[TR_BUF];
+    // Overlapping leaves wait in registers (their positions, up to TR_BUF
+    // per lane) and the wave settles them together when a lane could
+    // overflow or the walks end: the leaf records are loaded for all of them
+    // at once (one memory round trip per flush instead of one per iteration
+    // with a hit), the filters drop pairs, and one append per wave writes
+    // the rest.
+    int buf[TR_BUF];
 #pragma unroll
-    for (int k = 0; k < TR_BUF; k++) buf[k] = make_int2(-1, -1);
+    for (int k = 0; k < TR_BUF; k++) buf[k] = -1;
     unsigned nb = 0;
     int2* const reg = out + sh * S.scap;
+    auto flush = [&]() {
+        int4 rec[TR_BUF]; // every load first, then the filters
+#pragma unroll
+        for (int k = 0; k < TR_BUF; k++) {
+            rec[k] = make_int4(-1, -1, -1, -1);
+            if (nb > (unsigned)k) rec[k] = leaf[buf[k]];
+        }
+        int2 hit[TR_BUF];
+#pragma unroll
+        for (int k = 0; k < TR_BUF; k++)
+            hit[k] = nb > (unsigned)k ? leaf_pair<MODE>(flt, qr, rec[k]) : make_int2(-1, -1);
+        unsigned nv = 0;
+#pragma unroll
+        for (int k = 0; k < TR_BUF; k++) nv += hit[k].x >= 0 ? 1u : 0u;
+        const int64_t first = (int64_t)shard_append(S, sh, nv);
+        int64_t at = first;
+#pragma unroll
+        for (int k = 0; k < TR_BUF; k++)
+            if (hit[k].x >= 0) {
+                if (at < S.scap) reg[at] = hit[k];
+                at++;
+            }
+        nb = 0;
+    };
     int node = 0;
     if (active) {
         int root = root_p[0];
@@ -591,8 +618,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
                 const int lp = ~root;
                 const float* b = tbox + (int64_t)6 * (fa.tids ? fa.tids[lp] : 0);
                 if (overlap(q, b, b + 3)) {
-                    buf[0] = leaf_pair<MODE>(flt, qr, leaf[lp]);
-                    nb = buf[0].x >= 0 ? 1u : 0u;
+                    buf[0] = lp;
+                    nb = 1;
                 }
             }
             active = false;
@@ -607,7 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
             atomicOr(status, 2);
             active = false;
         }
-        int2 h0 = make_int2(-1, -1), h1 = make_int2(-1, -1);
+        int h0 = -1, h1 = -1;
         bool nd_ok = true;
         if (active) {
             const Node nd = nodes[node];
@@ -623,8 +650,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
                 if (o1 && nd.r1 <= qpos) o1 = false;
             }
             const bool l0 = nd.c0 < 0, l1 = nd.c1 < 0;
-            if (o0 && l0) h0 = leaf_pair<MODE>(flt, qr, leaf[~nd.c0]);
-            if (o1 && l1) h1 = leaf_pair<MODE>(flt, qr, leaf[~nd.c1]);
+            if (o0 && l0) h0 = ~nd.c0;
+            if (o1 && l1) h1 = ~nd.c1;
             const bool t0 = o0 && !l0, t1 = o1 && !l1;
             if (!nd_ok) {
             } else if (!t0 && !t1) {
@@ -650,32 +677,20 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
         }
         // (nb <= TR_BUF - 2 here) queue h0 then h1; constant indices keep
         // the buffer in registers
-        if (h0.x >= 0) {
+        if (h0 >= 0) {
 #pragma unroll
             for (int k = 0; k < TR_BUF; k++) buf[k] = nb == (unsigned)k ? h0 : buf[k];
             nb++;
         }
-        if (h1.x >= 0) {
+        if (h1 >= 0) {
 #pragma unroll
             for (int k = 0; k < TR_BUF; k++) buf[k] = nb == (unsigned)k ? h1 : buf[k];
             nb++;
         }
-        if (__any(nb > TR_BUF - 2)) { // all lanes together
-            const int64_t first = (int64_t)shard_append(S, sh, nb);
-#pragma unroll
-            for (int k = 0; k < TR_BUF; k++)
-                if (nb > (unsigned)k && first + k < S.scap) reg[first + k] = buf[k];
-            nb = 0;
-        }
+        if (__any(nb > TR_BUF - 2)) flush(); // all lanes together
     }
-    if (__any(nb > 0)) {
-        const int64_t first = (int64_t)shard_append(S, sh, nb);
-#pragma unroll
-        for (int k = 0; k < TR_BUF; k++)
-            if (nb > (unsigned)k && first + k < S.scap) reg[first + k] = buf[k];
-    }
+    if (__any(nb > 0)) flush();
 }
-
 struct TraverseArgs {
     const float* qbox;
     const uint32_t* qids;
