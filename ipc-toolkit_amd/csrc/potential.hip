@@ -896,6 +896,34 @@ __global__ __launch_bounds__(256) void k_fill_rows(const uint32_t* bptr, const i
     if (v >= nV) return;
     const int l = lane_id();
     const uint32_t s = bptr[v], e = bptr[v + 1];
+    const int m = (int)(e - s);
+    if (m > 0 && m <= WAVE) {
+        // R of the three rows per pass, m lanes each (lane l: row k0 + l / m,
+        // block l % m)
+        const int R = WAVE / m < 3 ? WAVE / m : 3;
+        const int kk = l / m, b = l - kk * m;
+        for (int k0 = 0; k0 < 3; k0 += R) {
+            const int k = k0 + kk;
+            const bool on = kk < R && k < 3;
+            const uint32_t u = s + (uint32_t)b;
+            const uint32_t msk = on ? (bmask[u] >> (3 * k)) & 7u : 0u;
+            unsigned total;
+            const unsigned excl = wave_exclusive_sum(__popc(msk), total);
+            const unsigned first = __shfl(excl, on ? kk * m : 0, WAVE); // the row's first lane
+            if (msk) {
+                const int cb = bcol[u];
+                int64_t at = rowptr[3 * v + k] + (excl - first);
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (msk & (1u << j)) {
+                        col[at] = 3 * cb + j;
+                        val[at] = bval[9 * (int64_t)u + 3 * k + j];
+                        at++;
+                    }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         int64_t pos = rowptr[3 * v + k];
