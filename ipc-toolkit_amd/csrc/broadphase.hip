@@ -104,13 +104,21 @@ __global__ void k_mesh_aabb_final(const double* partial, int nb, double* aabb)
 // Batched contexts: the mesh box of each scene's vertex boxes, one
 // workgroup per scene (min / max are exact, so the box equals the one a
 // context holding the scene alone computes).
+// Scene boxes in two steps: workgroup (scene, chunk) reduces every
+// SCENE_CHUNKS-th stretch of the scene's vertices into a partial box, then a
+// lane per scene combines its partials (min / max are exact, so the box is
+// the one the scene alone gets).
+constexpr int SCENE_CHUNKS = 16;
+
 __global__ __launch_bounds__(BP_BLOCK) void k_scene_aabb(const double* X0, const double* X1,
                                                          int nV, const int64_t* voff, double r,
-                                                         double* aabb)
+                                                         double* part)
 {
-    const int sc = blockIdx.x;
+    const int sc = blockIdx.x / SCENE_CHUNKS, ch = blockIdx.x % SCENE_CHUNKS;
+    double* aabb = part + 6 * (int64_t)ch; // this chunk's partial of scene sc
     double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-    for (int64_t i = voff[sc] + threadIdx.x; i < voff[sc + 1]; i += BP_BLOCK) {
+    for (int64_t i = voff[sc] + (int64_t)ch * BP_BLOCK + threadIdx.x; i < voff[sc + 1];
+         i += (int64_t)SCENE_CHUNKS * BP_BLOCK) {
         double l[3], h[3];
         vbox(X0, X1, nV, (int)i, r, l, h);
 #pragma unroll
@@ -138,7 +146,20 @@ __global__ __launch_bounds__(BP_BLOCK) void k_scene_aabb(const double* X0, const
         double v = sm[0][threadIdx.x];
         for (int j = 1; j < BP_BLOCK / WAVE; j++)
             v = threadIdx.x < 3 ? fmin(v, sm[j][threadIdx.x]) : fmax(v, sm[j][threadIdx.x]);
-        aabb[6 * (int64_t)sc + threadIdx.x] = v;
+        aabb[6 * (int64_t)SCENE_CHUNKS * sc + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_scene_aabb_final(const double* part, int n_scenes, double* aabb)
+{
+    const int sc = blockIdx.x * 64 + threadIdx.x;
+    if (sc >= n_scenes) return;
+    const double* p = part + 6 * (int64_t)SCENE_CHUNKS * sc;
+    for (int k = 0; k < 6; k++) {
+        double v = p[k];
+        for (int c = 1; c < SCENE_CHUNKS; c++)
+            v = k < 3 ? fmin(v, p[6 * c + k]) : fmax(v, p[6 * c + k]);
+        aabb[6 * (int64_t)sc + k] = v;
     }
 }
 
@@ -1114,8 +1135,12 @@ void mesh_bounds(Ctx& c, const double* X0, const double* X1, double inflation)
     c.stage_begin("bp_boxes");
     if (c.batched()) { // one box per scene
         c.aabb.reserve(sizeof(double) * 6 * (size_t)c.n_scenes);
-        hipLaunchKernelGGL(k_scene_aabb, dim3(c.n_scenes), dim3(BP_BLOCK), 0, s, X0, X1, c.nV,
-                           c.voff_d.as<int64_t>(), inflation, c.aabb.as<double>());
+        c.aabb_partial.reserve(sizeof(double) * 6 * SCENE_CHUNKS * (size_t)c.n_scenes);
+        hipLaunchKernelGGL(k_scene_aabb, dim3(SCENE_CHUNKS * c.n_scenes), dim3(BP_BLOCK), 0, s, X0,
+                           X1, c.nV, c.voff_d.as<int64_t>(), inflation,
+                           c.aabb_partial.as<double>());
+        hipLaunchKernelGGL(k_scene_aabb_final, dim3((c.n_scenes + 63) / 64), dim3(64), 0, s,
+                           c.aabb_partial.as<double>(), c.n_scenes, c.aabb.as<double>());
         IPCG_CHECK_LAUNCH();
         c.stage_end();
         return;

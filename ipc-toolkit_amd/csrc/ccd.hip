@@ -573,9 +573,11 @@ __global__ void k_wf_counts(const WFState* st, const WFBox* boxes, const uint8_t
 
 // a search whose level left no children ends (at its earliest tolerance
 // box, if any)
+// off: the scan of the children counts, off[nbox] their total
 __global__ void k_wf_nochild(CCDArgs A, WFState* st, int64_t nb, const uint32_t* off,
-                             int64_t nbox, int64_t total_children)
+                             int64_t nbox)
 {
+    const int64_t total_children = (int64_t)off[nbox];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
     WFState& S = st[i];
@@ -741,18 +743,18 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
         hipLaunchKernelGGL(k_wf_counts, dim3(gb), dim3(256), 0, s, states, cur, flags, nbox, cmask,
                            cnt);
         exclusive_scan_u32(cnt, off, nbox, off + nbox, c.blk_tmp.p, s);
-        IPCG_CHECK_LAUNCH();
-        IPCG_HIP(hipMemcpyAsync(h, off + nbox, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        IPCG_HIP(hipStreamSynchronize(s));
-        const int64_t children = reinterpret_cast<uint32_t*>(h)[0];
         hipLaunchKernelGGL(k_wf_nochild, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb,
-                           off, nbox, children);
+                           off, nbox);
         IPCG_HIP(hipMemsetAsync(npend, 0, 2 * sizeof(unsigned long long), s));
         hipLaunchKernelGGL(k_wf_pending, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, npend);
         IPCG_CHECK_LAUNCH();
+        // one round trip per level: the children's total, the searches
+        // still pending and those restarting
+        IPCG_HIP(hipMemcpyAsync(h, off + nbox, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipMemcpyAsync(h + 1, npend, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                 s));
         IPCG_HIP(hipStreamSynchronize(s));
+        const int64_t children = reinterpret_cast<uint32_t*>(h)[0];
         const int64_t pending = (int64_t)h[1], restarts = (int64_t)h[2];
         if (pending == 0) break;
         const int64_t nnext = children + restarts; // a restart is one root box
