@@ -378,6 +378,54 @@ __device__ __forceinline__ void refit_append(bool push, int32_t p, int32_t* next
     if (push) next[base + slot] = p;
 }
 
+// The last levels, once the work list holds at most RF_TOP nodes: one
+// workgroup completes them all, level after level with a barrier between
+// levels (a workgroup's global writes are visible to it after the barrier),
+// instead of one launch per level.
+constexpr int RF_TOP = 4096;
+
+__global__ __launch_bounds__(RF_BLOCK) void k_refit_top(const int32_t* cur, const uint32_t* cur_n,
+                                                        const int32_t* parent,
+                                                        const int32_t* lscene, Node* nodes,
+                                                        int32_t* visits, int32_t* root,
+                                                        int64_t max_levels)
+{
+    __shared__ int32_t list[2][RF_TOP];
+    __shared__ uint32_t cnt[2];
+    const int t = threadIdx.x;
+    if (t == 0) {
+        const uint32_t n0 = *cur_n;
+        cnt[0] = n0 < (uint32_t)RF_TOP ? n0 : (uint32_t)RF_TOP; // the host checked n0 <= RF_TOP
+        cnt[1] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < cnt[0]; i += RF_BLOCK) list[0][i] = cur[i];
+    __syncthreads();
+    // each level completes at least one of the max_levels internal nodes
+    int c = 0;
+    for (int64_t level = 0; level < max_levels; c ^= 1, level++) {
+        const uint32_t m = cnt[c];
+        if (m == 0) break;
+        for (uint32_t i = t; i < m; i += RF_BLOCK) {
+            const int32_t j = list[c][i];
+            const Node* N = nodes + j;
+            float lo[3], hi[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                lo[k] = fminf(N->lo0[k], N->lo1[k]);
+                hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
+            }
+            const int32_t p = parent[j];
+            if (p < 0) root[lscene ? lscene[j] : 0] = j; // the tree is complete
+            else if (refit_child(nodes, p, j, lo, hi, visits))
+                list[c ^ 1][atomicAdd(&cnt[c ^ 1], 1u)] = p;
+        }
+        __syncthreads();
+        if (t == 0) cnt[c] = 0;
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(RF_BLOCK) void k_refit_leaves(const uint32_t* ids,
                                                            const float* box32, int n,
                                                            const int32_t* lscene,
@@ -1116,6 +1164,14 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
                                 hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));
         if (h[0] == 0) break;
+        if (h[0] <= (uint32_t)RF_TOP) { // the rest in one workgroup
+            const size_t k = level + BATCH; // the next round
+            hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(RF_BLOCK), 0, s, work[(k - 1) & 1],
+                               cnt + ((k - 1) & 1), parent, lscene, T.nodes.as<Node>(), visits,
+                               T.root.as<int32_t>(), (int64_t)ni);
+            IPCG_CHECK_LAUNCH();
+            break;
+        }
         if (level >= ni) throw Error(IPCG_ERR_NUMERIC, "LBVH refit did not converge");
     }
     c.stage_end();
