@@ -292,54 +292,14 @@ __device__ __forceinline__ int delta(const uint64_t* codes, const int32_t* lscen
 // scene's leaves form a tree of their own: internal node i exists when
 // leaves i and i + 1 belong to one scene, and the node at a scene's first
 // leaf is its root.
-__global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, const int32_t* lscene,
-                                                     int n, Node* nodes, int32_t* parent,
-                                                     int32_t* leaf_parent)
-{
-    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
-    if (i >= n - 1) return;
-    if (lscene && lscene[i] != lscene[i + 1]) return; // no node between scenes
-    const int d = delta(codes, lscene, n, i, i + 1) - delta(codes, lscene, n, i, i - 1) >= 0 ? 1 : -1;
-    const int dmin = delta(codes, lscene, n, i, i - d);
-    int lmax = 2;
-    while (delta(codes, lscene, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(codes, lscene, n, i, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = delta(codes, lscene, n, i, j);
-    int sp = 0;
-    for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
-        if (delta(codes, lscene, n, i, i + (sp + t) * d) > dnode) sp += t;
-        if (t == 1) break;
-    }
-    const int g = i + sp * d + (d < 0 ? -1 : 0);
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    Node* N = nodes + i;
-    if (lo == g) {
-        N->c0 = ~g;
-        leaf_parent[g] = i;
-    } else {
-        N->c0 = g;
-        parent[g] = i;
-    }
-    if (hi == g + 1) {
-        N->c1 = ~(g + 1);
-        leaf_parent[g + 1] = i;
-    } else {
-        N->c1 = g + 1;
-        parent[g + 1] = i;
-    }
-    N->r0 = g;
-    N->r1 = hi;
-    if (lscene ? (i == 0 || lscene[i - 1] != lscene[i]) : i == 0) parent[i] = -1; // a root
-}
 
-// Bottom-up box refit, one level of completed nodes per launch: a node's
-// box goes into its slot of the parent, and the second child to arrive puts
-// the parent on the next work list.  Boxes written in one launch are read
-// in the next, so no inter-workgroup fences are needed.  Appends to the next
-// list are gathered per workgroup (one global atomic per RF_BLOCK nodes).
+// Bottom-up box refit, one level of completed nodes per launch: leaf boxes
+// are written into their parents' slots by k_karras, which also starts the
+// first work list; then a completed node's box goes into its slot of the
+// parent, and the second child to arrive puts the parent on the next work
+// list.  Boxes written in one launch are read in the next, so no
+// inter-workgroup fences are needed.  Appends to the next list are gathered
+// per workgroup (one global atomic per workgroup).
 constexpr int RF_BLOCK = 1024;
 
 // Writes the child's box into parent p; returns whether p is now complete.
@@ -378,11 +338,78 @@ __device__ __forceinline__ void refit_append(bool push, int32_t p, int32_t* next
     if (push) next[base + slot] = p;
 }
 
-// The last levels, once the work list holds at most RF_TOP nodes: one
+// Also fills the box slot of every leaf child from the leaf's primitive box
+// and counts its leaf children in visits[i]; a node whose two children are
+// leaves is complete at once and goes on the first refit work list.
+__global__ __launch_bounds__(BP_BLOCK) void k_karras(const uint64_t* codes, const int32_t* lscene,
+                                                     int n, Node* nodes, int32_t* parent,
+                                                     const uint32_t* ids, const float* box32,
+                                                     int32_t* visits, int32_t* work,
+                                                     uint32_t* work_n)
+{
+    const int i = blockIdx.x * BP_BLOCK + threadIdx.x;
+    bool push = false;
+    // no node between scenes
+    if (i < n - 1 && !(lscene && lscene[i] != lscene[i + 1])) {
+        const int d = delta(codes, lscene, n, i, i + 1) - delta(codes, lscene, n, i, i - 1) >= 0 ? 1 : -1;
+        const int dmin = delta(codes, lscene, n, i, i - d);
+        int lmax = 2;
+        while (delta(codes, lscene, n, i, i + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (delta(codes, lscene, n, i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = delta(codes, lscene, n, i, j);
+        int sp = 0;
+        for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
+            if (delta(codes, lscene, n, i, i + (sp + t) * d) > dnode) sp += t;
+            if (t == 1) break;
+        }
+        const int g = i + sp * d + (d < 0 ? -1 : 0);
+        const int lo = i < j ? i : j, hi = i < j ? j : i;
+        Node* N = nodes + i;
+        int leaves = 0;
+        if (lo == g) {
+            N->c0 = ~g;
+            const uint32_t pid = ids[g];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                N->lo0[k] = box32[(int64_t)6 * pid + k];
+                N->hi0[k] = box32[(int64_t)6 * pid + 3 + k];
+            }
+            leaves++;
+        } else {
+            N->c0 = g;
+            parent[g] = i;
+        }
+        if (hi == g + 1) {
+            N->c1 = ~(g + 1);
+            const uint32_t pid = ids[g + 1];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                N->lo1[k] = box32[(int64_t)6 * pid + k];
+                N->hi1[k] = box32[(int64_t)6 * pid + 3 + k];
+            }
+            leaves++;
+        } else {
+            N->c1 = g + 1;
+            parent[g + 1] = i;
+        }
+        N->r0 = g;
+        N->r1 = hi;
+        visits[i] = leaves;
+        push = leaves == 2;
+        if (lscene ? (i == 0 || lscene[i - 1] != lscene[i]) : i == 0) parent[i] = -1; // a root
+    }
+    refit_append(push, i, work, work_n);
+}
+
+// The last levels, once the work list holds at most RF_TOP nodes (128 KB
+// of LDS for the two lists): one
 // workgroup completes them all, level after level with a barrier between
 // levels (a workgroup's global writes are visible to it after the barrier),
 // instead of one launch per level.
-constexpr int RF_TOP = 4096;
+constexpr int RF_TOP = 16384;
 
 __global__ __launch_bounds__(RF_BLOCK) void k_refit_top(const int32_t* cur, const uint32_t* cur_n,
                                                         const int32_t* parent,
@@ -426,32 +453,6 @@ __global__ __launch_bounds__(RF_BLOCK) void k_refit_top(const int32_t* cur, cons
     }
 }
 
-__global__ __launch_bounds__(RF_BLOCK) void k_refit_leaves(const uint32_t* ids,
-                                                           const float* box32, int n,
-                                                           const int32_t* lscene,
-                                                           const int32_t* leaf_parent,
-                                                           Node* nodes, int32_t* visits,
-                                                           int32_t* next, uint32_t* next_n)
-{
-    const int i = blockIdx.x * RF_BLOCK + threadIdx.x;
-    bool push = false;
-    int32_t p = 0;
-    // a forest's single-leaf scenes have no parent node
-    const bool alone = lscene && i < n && (i == 0 || lscene[i - 1] != lscene[i])
-        && (i == n - 1 || lscene[i + 1] != lscene[i]);
-    if (i < n && !alone) {
-        const uint32_t pid = ids[i];
-        float lo[3], hi[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            lo[k] = box32[(int64_t)6 * pid + k];
-            hi[k] = box32[(int64_t)6 * pid + 3 + k];
-        }
-        p = leaf_parent[i];
-        push = refit_child(nodes, p, ~i, lo, hi, visits);
-    }
-    refit_append(push, p, next, next_n);
-}
 
 __global__ __launch_bounds__(RF_BLOCK) void k_refit_level(const int32_t* cur,
                                                           const uint32_t* cur_n,
@@ -1120,25 +1121,21 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
         IPCG_HIP(hipStreamSynchronize(s));
         return;
     }
-    // int32 arrays: parent[n-1], leaf_parent[n], visits[n-1], two work lists
-    // of n, and two counters
+    // int32 arrays: parent[n-1], visits[n-1], two work lists of n, and two
+    // counters
     const size_t ni = (size_t)n - 1;
-    T.info.reserve(sizeof(int32_t) * (2 * ni + 3 * (size_t)n) + 256);
+    T.info.reserve(sizeof(int32_t) * (2 * ni + 2 * (size_t)n) + 256);
     int32_t* parent = T.info.as<int32_t>();
-    int32_t* leaf_parent = parent + ni;
-    int32_t* visits = leaf_parent + n;
+    int32_t* visits = parent + ni;
     int32_t* work[2] = { visits + ni, visits + ni + n };
     uint32_t* cnt = reinterpret_cast<uint32_t*>(work[1] + n);
-    IPCG_HIP(hipMemsetAsync(visits, 0, sizeof(int32_t) * ni, s));
     IPCG_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(uint32_t), s));
     if (!forest) // "unfinished" sentinel
         IPCG_HIP(hipMemsetAsync(T.root.p, 0x80, sizeof(int32_t), s));
     c.stage_begin("bp_build");
     hipLaunchKernelGGL(k_karras, dim3(grid_for(ni, BP_BLOCK)), dim3(BP_BLOCK), 0, s,
-                       T.keys.as<uint64_t>(), lscene, n, T.nodes.as<Node>(), parent, leaf_parent);
-    hipLaunchKernelGGL(k_refit_leaves, dim3(grid_for(n, RF_BLOCK)), dim3(RF_BLOCK), 0, s,
-                       T.ids.as<uint32_t>(), T.box.as<float>(), n, lscene, leaf_parent,
-                       T.nodes.as<Node>(), visits, work[0], cnt);
+                       T.keys.as<uint64_t>(), lscene, n, T.nodes.as<Node>(), parent,
+                       T.ids.as<uint32_t>(), T.box.as<float>(), visits, work[0], cnt);
     IPCG_CHECK_LAUNCH();
     // Round k completes the nodes of height k; their subtrees are disjoint and
     // hold at least k + 1 leaves each, so round k has at most n / (k + 1)
