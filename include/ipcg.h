@@ -56,7 +56,9 @@ int ipcg_ctx_synchronize(ipcg_ctx* ctx);
 /* the context's hipStream_t (as void*), for callers that order their own
  * work against the library's */
 void* ipcg_ctx_stream(ipcg_ctx* ctx);
-/* bytes of device memory the context holds (its buffers only grow) */
+/* bytes of device memory the context holds (its buffers only grow; the CCD
+ * slab tier's lists, one buffer per device shared by its contexts, are not
+ * counted) */
 int ipcg_ctx_device_bytes(ipcg_ctx* ctx, int64_t* bytes);
 /* Launch-shape autotuning (no reference counterpart): with enable != 0,
  * the first broad phase of each kind ("bp_traverse_ee.static",
