@@ -728,8 +728,14 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
                 if (sp == 0) active = false;
                 else {
                     --sp;
-                    node = sp < TR_LSTACK ? stack[sp][t]
-                                          : deep[(int64_t)qpos * (TR_STACK - TR_LSTACK) + sp - TR_LSTACK];
+                    // the LDS entries by an LDS read; the deep ones through a
+                    // load of their own (a shared select of the two addresses
+                    // would turn every pop into a flat load)
+                    if (sp < TR_LSTACK)
+                        node = stack[sp][t];
+                    else
+                        node = __builtin_nontemporal_load(
+                            &deep[(int64_t)qpos * (TR_STACK - TR_LSTACK) + sp - TR_LSTACK]);
                 }
             } else {
                 node = t0 ? nd.c0 : nd.c1;
