@@ -714,15 +714,16 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
                 active = false;
                 nd_ok = false;
             }
-            bool o0 = nd_ok && overlap(q, nd.lo0, nd.hi0), o1 = nd_ok && overlap(q, nd.lo1, nd.hi1);
+            // branch-free tests (bitwise, no short-circuit)
+            bool o0 = nd_ok & overlap(q, nd.lo0, nd.hi0), o1 = nd_ok & overlap(q, nd.lo1, nd.hi1);
             if (TRI) {
-                if (o0 && nd.r0 <= qpos) o0 = false;
-                if (o1 && nd.r1 <= qpos) o1 = false;
+                o0 = o0 & (nd.r0 > qpos);
+                o1 = o1 & (nd.r1 > qpos);
             }
             const bool l0 = nd.c0 < 0, l1 = nd.c1 < 0;
-            if (o0 && l0) h0 = ~nd.c0;
-            if (o1 && l1) h1 = ~nd.c1;
-            const bool t0 = o0 && !l0, t1 = o1 && !l1;
+            h0 = o0 & l0 ? ~nd.c0 : -1;
+            h1 = o1 & l1 ? ~nd.c1 : -1;
+            const bool t0 = o0 & !l0, t1 = o1 & !l1;
             if (!nd_ok) {
             } else if (!t0 && !t1) {
                 if (sp == 0) active = false;
