@@ -700,16 +700,15 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
     int sp = 0, steps = 0;
     const int max_steps = 2 * nleaves + 64; // each node is visited at most once per query
     while (__any(active)) {
-        if (active && ++steps > max_steps) {
-            atomicOr(status, 2);
-            active = false;
-        }
         int h0 = -1, h1 = -1;
         bool nd_ok = true;
         if (active) {
-            const Node nd = nodes[node];
-            // links must name an internal node or a leaf position
-            if (nd.c0 >= nin || nd.c1 >= nin || nd.c0 < -nleaves || nd.c1 < -nleaves) {
+            const Node nd = nodes[node]; // node: a valid index (checked when it was read)
+            // links must name an internal node or a leaf position, and a walk
+            // visits each node at most once: one rarely taken branch for both
+            ++steps;
+            if ((steps > max_steps) | (nd.c0 >= nin) | (nd.c1 >= nin) | (nd.c0 < -nleaves)
+                | (nd.c1 < -nleaves)) {
                 atomicOr(status, 2);
                 active = false;
                 nd_ok = false;
