@@ -751,17 +751,17 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
                 }
             }
         }
-        // (nb <= TR_BUF - 2 here) queue h0 then h1; constant indices keep
-        // the buffer in registers
-        if (h0 >= 0) {
+        // (nb <= TR_BUF - 2 here) queue h0 then h1 without branches:
+        // constant indices keep the buffer in registers, a missing hit
+        // matches no slot
+        {
+            const unsigned s0 = h0 >= 0 ? nb : (unsigned)TR_BUF;
+            nb += h0 >= 0 ? 1u : 0u;
+            const unsigned s1 = h1 >= 0 ? nb : (unsigned)TR_BUF;
+            nb += h1 >= 0 ? 1u : 0u;
 #pragma unroll
-            for (int k = 0; k < TR_BUF; k++) buf[k] = nb == (unsigned)k ? h0 : buf[k];
-            nb++;
-        }
-        if (h1 >= 0) {
-#pragma unroll
-            for (int k = 0; k < TR_BUF; k++) buf[k] = nb == (unsigned)k ? h1 : buf[k];
-            nb++;
+            for (int k = 0; k < TR_BUF; k++)
+                buf[k] = s0 == (unsigned)k ? h0 : (s1 == (unsigned)k ? h1 : buf[k]);
         }
         if (__any(nb > TR_BUF - 2)) flush(); // all lanes together
     }
