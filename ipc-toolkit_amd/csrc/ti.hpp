@@ -148,9 +148,12 @@ IPCG_HD void ti_filter(TIQuery& q, bool use_ms)
 }
 
 // Inclusion of the origin in the corner box of F over [t,t+wt]x[u,u+wu]x
-// [v,v+wv], inflated by err and ms.
-IPCG_HD bool ti_inclusion(const TIQuery& q, const TIBox& b, double wt, double wu, double wv,
-                          double ms, bool& box_in, double* width)
+// [v,v+wv], inflated by err and ms.  The kind is a template argument so the
+// eight corner evaluations of an axis are one straight block (shared
+// subexpressions computed once), not eight branches on q.vf.
+template <bool VF>
+IPCG_HD bool ti_inclusion_k(const TIQuery& q, const TIBox& b, double wt, double wu, double wv,
+                            double ms, bool& box_in, double* width)
 {
     box_in = true;
     const double t1 = b.t + wt, u1 = b.u + wu, v1 = b.v + wv;
@@ -161,7 +164,7 @@ IPCG_HD bool ti_inclusion(const TIQuery& q, const TIBox& b, double wt, double wu
             const double t = (c & 4) ? t1 : b.t;
             const double u = (c & 2) ? u1 : b.u;
             const double v = (c & 1) ? v1 : b.v;
-            const double f = q.vf ? ti_f_vf(q.s, q.e, d, t, u, v) : ti_f_ee(q.s, q.e, d, t, u, v);
+            const double f = VF ? ti_f_vf(q.s, q.e, d, t, u, v) : ti_f_ee(q.s, q.e, d, t, u, v);
             mn = (c == 0 || f < mn) ? f : mn;
             mx = (c == 0 || f > mx) ? f : mx;
         }
@@ -171,6 +174,13 @@ IPCG_HD bool ti_inclusion(const TIQuery& q, const TIBox& b, double wt, double wu
         if (!(mn + ms >= -eps && mx - ms <= eps)) box_in = false;
     }
     return true;
+}
+
+IPCG_HD bool ti_inclusion(const TIQuery& q, const TIBox& b, double wt, double wu, double wv,
+                          double ms, bool& box_in, double* width)
+{
+    return q.vf ? ti_inclusion_k<true>(q, b, wt, wu, wv, ms, box_in, width)
+                : ti_inclusion_k<false>(q, b, wt, wu, wv, ms, box_in, width);
 }
 
 // One breadth-first root search.  Returns TI_HIT (toi set), TI_MISS, or
