@@ -208,6 +208,18 @@ private:
     bool built = false;
 };
 
+/// SweepAndTiniestQueue (sweep_and_tiniest_queue.hpp, .cu:9-280) on this
+/// library, for builds without CUDA.  The overrides are GPULBVH's over the
+/// device sweep and prune (FP64 boxes): box overlap plus the mesh's filter,
+/// the sets the reference's STQ detect_* return.  name() is the one that
+/// compute_collision_free_stepsize dispatches on (ipc.cpp:63-85).
+class SweepAndTiniestQueue : public GPULBVH {
+public:
+    explicit SweepAndTiniestQueue(Context& ctx) : GPULBVH(ctx, IPCG_BP_SAP) { }
+
+    std::string name() const override { return "SweepAndTiniestQueue"; }
+};
+
 /// NormalCollisions::build (normal_collisions.cpp:20-158) entirely on the
 /// GPU: broad phase, narrow phase and merge; the collision vectors are
 /// filled in the reference's layout (canonical order, merged weights).
@@ -299,12 +311,46 @@ inline double compute_collision_free_stepsize(GPULBVH& bp, Eigen::ConstRef<Eigen
     return step;
 }
 
+/// The SweepAndTiniestQueue branch of compute_collision_free_stepsize
+/// (ipc.cpp:63-85): 3D only, no minimum distance whatever the caller asks,
+/// TightInclusionCCD's tolerance and iteration limit (the defaults for any
+/// other narrow phase), and the earliest time of impact scaled by 0.8 when
+/// it is below 1.  The device search runs with conservative rescaling 1 (no
+/// minimum separation inside the search, no rescaling of small times): the
+/// 0.8 of that branch is the only margin, as in the reference.
+inline double compute_collision_free_stepsize(SweepAndTiniestQueue& bp,
+                                              Eigen::ConstRef<Eigen::MatrixXd> V0,
+                                              Eigen::ConstRef<Eigen::MatrixXd> V1,
+                                              const NarrowPhaseCCD& narrow_phase_ccd)
+{
+    if (V0.cols() != 3)
+        throw std::runtime_error("Sweep and Tiniest Queue is only supported in 3D!");
+    double tol = TightInclusionCCD::DEFAULT_TOLERANCE;
+    long max_iter = TightInclusionCCD::DEFAULT_MAX_ITERATIONS;
+    if (const auto* ti = dynamic_cast<const TightInclusionCCD*>(&narrow_phase_ccd)) {
+        tol = ti->tolerance;
+        max_iter = ti->max_iterations;
+    }
+    Context& ctx = bp.context();
+    ctx.positions(0, V0);
+    ctx.positions(1, V1);
+    double toi = 1;
+    check(ctx.c, ipcg_compute_collision_free_stepsize(ctx.c, 0.0, tol, max_iter, 1.0,
+                                                      IPCG_BP_SAP, &toi));
+    return toi < 1.0 ? 0.8 * toi : 1.0;
+}
+
 } // namespace ipc::gpu
 
 // The branch a maintainer adds at the top of ipc::compute_collision_free_
 // stepsize (ipc.cpp:45-101), beside the reference's own dispatch on
-// broad_phase->name() for its CUDA broad phase (ipc.cpp:63-85):
+// broad_phase->name() for its CUDA broad phase (ipc.cpp:63-85).  The STQ
+// test comes first and by type: a CUDA build has its own class of that name.
 //
+//   if (auto* stq = dynamic_cast<gpu::SweepAndTiniestQueue*>(broad_phase)) {
+//       return gpu::compute_collision_free_stepsize(
+//           *stq, vertices_t0, vertices_t1, narrow_phase_ccd);
+//   }
 //   if (broad_phase != nullptr
 //       && (broad_phase->name() == "GPULBVH"
 //           || broad_phase->name() == "GPUSweepAndPrune")) {

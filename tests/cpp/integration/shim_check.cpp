@@ -7,6 +7,7 @@
 // name dispatch a maintainer adds to compute_collision_free_stepsize.
 #include <ipc_gpu.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -68,6 +69,8 @@ static void two_sheets(int n, double gap, Eigen::MatrixXd& V, Eigen::MatrixXi& E
 static double stepsize_by_name(BroadPhase* bp, const Eigen::MatrixXd& V0,
                                const Eigen::MatrixXd& V1, const NarrowPhaseCCD& ccd)
 {
+    if (auto* stq = dynamic_cast<gpu::SweepAndTiniestQueue*>(bp))
+        return gpu::compute_collision_free_stepsize(*stq, V0, V1, ccd);
     if (bp != nullptr && (bp->name() == "GPULBVH" || bp->name() == "GPUSweepAndPrune"))
         return gpu::compute_collision_free_stepsize(static_cast<gpu::GPULBVH&>(*bp), V0, V1, 0,
                                                     ccd);
@@ -152,9 +155,59 @@ int main()
                                                            ccd.conservative_rescaling,
                                                            IPCG_BP_LBVH, &t_c));
     EXPECT(t > 0 && t < 1 && t == t_c);
+
+    // SweepAndTiniestQueue: the same candidate sets as the LBVH, and the
+    // branch's rule on the step (ipc.cpp:63-85): no minimum distance, the
+    // earliest time of impact times 0.8 below 1, else 1
+    gpu::SweepAndTiniestQueue gpu_stq(ctx);
+    BroadPhase& stq = gpu_stq;
+    EXPECT(stq.name() == "SweepAndTiniestQueue");
+    using Pairs = std::vector<std::pair<int, int>>;
+    auto sorted = [](Pairs p) {
+        std::sort(p.begin(), p.end());
+        return p;
+    };
+    auto ee_pairs = [&](const std::vector<EdgeEdgeCandidate>& v) {
+        Pairs p;
+        for (const auto& c : v) p.emplace_back(c.edge0_id, c.edge1_id);
+        return sorted(p);
+    };
+    auto fv_pairs = [&](const std::vector<FaceVertexCandidate>& v) {
+        Pairs p;
+        for (const auto& c : v) p.emplace_back(c.face_id, c.vertex_id);
+        return sorted(p);
+    };
+    bp.build(V, E, F, 0.5 * dhat);
+    std::vector<EdgeEdgeCandidate> ee_l, ee_s;
+    std::vector<FaceVertexCandidate> fv_l, fv_s;
+    bp.detect_edge_edge_candidates(ee_l);
+    bp.detect_face_vertex_candidates(fv_l);
+    stq.build(V, E, F, 0.5 * dhat);
+    stq.detect_edge_edge_candidates(ee_s);
+    stq.detect_face_vertex_candidates(fv_s);
+    EXPECT(!ee_s.empty() && ee_pairs(ee_s) == ee_pairs(ee_l) && fv_pairs(fv_s) == fv_pairs(fv_l));
+    const double t_stq = stepsize_by_name(&stq, V, V1, ccd);
+    double toi = 0;
+    gpu::check(ctx.c, ipcg_compute_collision_free_stepsize(ctx.c, 0, ccd.tolerance,
+                                                           ccd.max_iterations, 1.0,
+                                                           IPCG_BP_LBVH, &toi));
+    EXPECT(toi > 0 && toi < 1 && t_stq == 0.8 * toi);
+    EXPECT(stepsize_by_name(&stq, V, V, ccd) == 1.0); // no motion, no impact
+    Eigen::MatrixXd V2(V.rows(), 2);
+    for (Eigen::Index i = 0; i < V.rows(); i++) {
+        V2(i, 0) = V(i, 0);
+        V2(i, 1) = V(i, 1);
+    }
+    threw = false;
+    try {
+        stepsize_by_name(&stq, V2, V2, ccd);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
     std::printf("candidates ee %zu fv %zu ef %zu; collisions %zu; energy %.6e; nnz %ld; "
-                "step %.9g\n",
-                ee.size(), fv.size(), ef.size(), C.size(), e, (long)H.nonZeros(), t);
+                "step %.9g; STQ step %.9g\n",
+                ee.size(), fv.size(), ef.size(), C.size(), e, (long)H.nonZeros(), t, t_stq);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
