@@ -18,6 +18,8 @@
 #include "geometry.hpp"
 #include "sort.hpp"
 
+#include <utility>
+
 namespace ipcg {
 
 namespace {
@@ -1076,8 +1078,11 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     IPCG_CHECK_LAUNCH();
     c.sort_tmp.reserve(radix_tmp_bytes(n));
     c.stage_begin("bp_sort");
-    radix_sort_pairs(T.keys.as<uint64_t>(), T.ids.as<uint32_t>(), T.keys2.as<uint64_t>(),
-                     T.ids2.as<uint32_t>(), n, 63, c.sort_tmp.p, s);
+    if (radix_sort_pairs_all(T.keys.as<uint64_t>(), T.ids.as<uint32_t>(), T.keys2.as<uint64_t>(),
+                             T.ids2.as<uint32_t>(), n, 63, c.sort_tmp.p, s)) {
+        std::swap(T.keys, T.keys2);
+        std::swap(T.ids, T.ids2);
+    }
     if (forest) { // stable second pass by scene, then the Morton codes in that order
         if (arity == 1)
             hipLaunchKernelGGL(k_scene_keys<1>, dim3(g), dim3(BP_BLOCK), 0, s,

@@ -20,6 +20,7 @@
 
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace ipcg {
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(64) void k_hessian_records(CollView C, const double
 // (vertex, global collision g, point a): key vertex, value g << 2 | a
 template <int KIND>
 __global__ __launch_bounds__(256) void k_incidence(CollView C, int64_t goff, int64_t ioff,
-                                                   uint64_t* keys, uint32_t* vals)
+                                                   uint32_t* keys, uint32_t* vals)
 {
     constexpr int NP = StencilTraits<KIND>::NP;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -679,7 +680,7 @@ __global__ __launch_bounds__(256) void k_incidence(CollView C, int64_t goff, int
     stencil_ids<KIND>(C, i, v);
 #pragma unroll
     for (int a = 0; a < NP; a++) {
-        keys[ioff + (int64_t)NP * i + a] = (uint64_t)v[a];
+        keys[ioff + (int64_t)NP * i + a] = (uint32_t)v[a];
         vals[ioff + (int64_t)NP * i + a] = (uint32_t)(((goff + i) << 2) | a);
     }
 }
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(256) void k_incidence(CollView C, int64_t goff, int
 constexpr int RUN_PTR_GAP = 32;
 constexpr uint32_t RUN_PTR_NONE = 0xFFFFFFFFu;
 
-__global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M, int nV,
+__global__ __launch_bounds__(256) void k_run_ptr(const uint32_t* keys, int64_t M, int nV,
                                                  uint32_t* vptr)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -702,7 +703,7 @@ __global__ __launch_bounds__(256) void k_run_ptr(const uint64_t* keys, int64_t M
     for (int64_t v = from; v <= cur; v++) vptr[v] = (uint32_t)i;
 }
 
-__global__ __launch_bounds__(256) void k_gap_ptr(const uint64_t* keys, int64_t M, int nV,
+__global__ __launch_bounds__(256) void k_gap_ptr(const uint32_t* keys, int64_t M, int nV,
                                                  uint32_t* vptr)
 {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1236,13 +1237,13 @@ void require_incidences(Ctx& c, const CollView& C, const CollRanges& G, int64_t 
     hipStream_t s = c.stream;
     c.stage_begin("incidence");
     const int64_t mk = std::max<int64_t>(M, 1);
-    c.inc_keys.reserve(sizeof(uint64_t) * mk);
-    c.inc_keys2.reserve(sizeof(uint64_t) * mk);
+    c.inc_keys.reserve(sizeof(uint32_t) * mk);
+    c.inc_keys2.reserve(sizeof(uint32_t) * mk);
     c.inc_vals.reserve(sizeof(uint32_t) * mk);
     c.inc_vals2.reserve(sizeof(uint32_t) * mk);
     c.inc_ptr.reserve(sizeof(uint32_t) * ((size_t)c.nV + 1));
     c.blk_tmp.reserve(radix_tmp_bytes(mk));
-    uint64_t* keys = c.inc_keys.as<uint64_t>();
+    uint32_t* keys = c.inc_keys.as<uint32_t>();
     uint32_t* vals = c.inc_vals.as<uint32_t>();
     int64_t io = 0;
     if (C.n[0]) hipLaunchKernelGGL(k_incidence<K_VV>, dim3(grid_for(C.n[0], 256, INT32_MAX)), dim3(256), 0, s, C, G.goff[0], io, keys, vals);
@@ -1255,8 +1256,13 @@ void require_incidences(Ctx& c, const CollView& C, const CollRanges& G, int64_t 
     IPCG_CHECK_LAUNCH();
     int vb = 1;
     while ((1ll << vb) <= c.nV) vb++;
-    radix_sort_pairs(keys, vals, c.inc_keys2.as<uint64_t>(), c.inc_vals2.as<uint32_t>(), M, vb,
-                     c.blk_tmp.p, s);
+    if (radix_sort_pairs_all(keys, vals, c.inc_keys2.as<uint32_t>(), c.inc_vals2.as<uint32_t>(),
+                             M, vb, c.blk_tmp.p, s)) { // sorted pairs in the second buffers
+        std::swap(c.inc_keys, c.inc_keys2);
+        std::swap(c.inc_vals, c.inc_vals2);
+        keys = c.inc_keys.as<uint32_t>();
+        vals = c.inc_vals.as<uint32_t>();
+    }
     if (M >= (int64_t)RUN_PTR_NONE) throw Error(IPCG_ERR_ARG, "too many collision incidences");
     IPCG_HIP(hipMemsetAsync(c.inc_ptr.p, 0xFF, sizeof(uint32_t) * (size_t)c.nV, s));
     hipLaunchKernelGGL(k_run_ptr, dim3(grid_for(M + 1, 256, INT32_MAX)), dim3(256), 0, s, keys, M,

@@ -7,9 +7,12 @@
 //      a 64-key chunk at a time by matching digits with 8 ballots, keeps a
 //      running per-digit count in LDS; wave counts are prefix-summed across
 //      the block, then every key is written to base + wave prefix + rank.
-// Passes whose digit is constant over all keys are skipped.
+// 64-bit or 32-bit keys.  Passes whose digit is constant over all keys are
+// skipped where the caller asks for it (a read-back of the varying bits).
 #include "shard.hpp"
 #include "sort.hpp"
+
+#include <utility>
 
 namespace ipcg {
 
@@ -131,9 +134,9 @@ constexpr int RS_WAVES = RS_BLOCK / WAVE;
 constexpr int RS_CHUNKS = 8; // 64-key chunks per wave
 constexpr int RS_TILE = RS_BLOCK * RS_CHUNKS;
 
-__global__ __launch_bounds__(RS_BLOCK) void k_radix_hist(const uint64_t* keys, int64_t n,
-                                                         int shift, uint32_t* counts,
-                                                         int64_t ntiles)
+template <class K>
+__global__ __launch_bounds__(RS_BLOCK) void k_radix_hist(const K* keys, int64_t n, int shift,
+                                                         uint32_t* counts, int64_t ntiles)
 {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
@@ -152,15 +155,16 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_hist(const uint64_t* keys, i
 // with ballots; the block turns the per-wave counts into offsets, places
 // the tile's keys in LDS in digit order, and writes each digit's run to its
 // global range so that neighbouring lanes write neighbouring addresses.
+template <class K>
 __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
-    const uint64_t* keys, const uint32_t* vals, uint64_t* okeys, uint32_t* ovals, int64_t n,
-    int shift, const uint32_t* offsets, int64_t ntiles)
+    const K* keys, const uint32_t* vals, K* okeys, uint32_t* ovals, int64_t n, int shift,
+    const uint32_t* offsets, int64_t ntiles)
 {
     __shared__ uint32_t wcnt[RS_WAVES][256];
     __shared__ uint32_t tile_off[256];
     __shared__ uint32_t dstart[256]; // first slot of each digit in the tile
     __shared__ uint32_t wtot[RS_WAVES];
-    __shared__ uint64_t sk[RS_TILE];
+    __shared__ K sk[RS_TILE];
     __shared__ uint32_t sv[RS_TILE];
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_BLOCK) (&wcnt[0][0])[i] = 0;
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
     const int64_t tile0 = (int64_t)blockIdx.x * RS_TILE;
     const int64_t base = tile0 + (int64_t)w * (RS_CHUNKS * WAVE);
     const unsigned long long lt = (1ull << lane) - 1ull;
-    uint64_t k[RS_CHUNKS];
+    K k[RS_CHUNKS];
     uint32_t v[RS_CHUNKS];
     uint32_t rank[RS_CHUNKS];
 #pragma unroll
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
     __syncthreads();
     const int tn = (int)(n - tile0 < RS_TILE ? n - tile0 : RS_TILE);
     for (int j = threadIdx.x; j < tn; j += RS_BLOCK) {
-        const uint64_t key = sk[j];
+        const K key = sk[j];
         const uint32_t d = (uint32_t)(key >> shift) & 0xFF;
         const int64_t pos = (int64_t)tile_off[d] + (j - (int)dstart[d]);
         okeys[pos] = key;
@@ -244,7 +248,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_radix_scatter(
 }
 
 // xor-fold of all keys against the first one: bits that vary
-__global__ __launch_bounds__(256) void k_vary_bits(const uint64_t* keys, int64_t n,
+template <class K>
+__global__ __launch_bounds__(256) void k_vary_bits(const K* keys, int64_t n,
                                                    unsigned long long* out)
 {
     const uint64_t k0 = keys[0];
@@ -301,10 +306,17 @@ size_t radix_tmp_bytes(int64_t n)
     return 2 * sizeof(uint32_t) * (size_t)nc + scan_tmp_bytes(nc) + 1024;
 }
 
-void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
-                      int64_t n, int key_bits, void* tmp, hipStream_t s)
+// LSD passes over the low key_bits bits.  skip_constant: one pass over the
+// keys and a read-back find the digits that vary, and only those passes run;
+// otherwise every pass runs without any host round trip (a constant digit
+// costs a stable identity pass).  Returns true when the sorted pairs ended
+// in keys_alt / vals_alt (copy_back false and an odd number of passes).
+template <class K>
+static bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
+                       int key_bits, void* tmp, hipStream_t s, bool skip_constant,
+                       bool copy_back)
 {
-    if (n <= 1) return;
+    if (n <= 1) return false;
     const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     const int64_t nc = 256 * ntiles;
     char* t = static_cast<char*>(tmp);
@@ -312,37 +324,54 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
     uint32_t* counts = reinterpret_cast<uint32_t*>(t + 256);
     uint32_t* offsets = counts + nc;
     void* stmp = offsets + nc;
-    // which bits vary at all (one small D2H read per sort)
-    IPCG_HIP(hipMemsetAsync(vary, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_vary_bits, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, keys, n,
-                       vary);
-    IPCG_CHECK_LAUNCH();
-    unsigned long long hv = 0;
-    IPCG_HIP(hipMemcpyAsync(&hv, vary, sizeof(hv), hipMemcpyDeviceToHost, s));
-    IPCG_HIP(hipStreamSynchronize(s));
-    uint64_t* ki = keys;
+    unsigned long long hv = ~0ull;
+    if (skip_constant) { // which bits vary at all (one small D2H read per sort)
+        IPCG_HIP(hipMemsetAsync(vary, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_vary_bits<K>, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, keys,
+                           n, vary);
+        IPCG_CHECK_LAUNCH();
+        IPCG_HIP(hipMemcpyAsync(&hv, vary, sizeof(hv), hipMemcpyDeviceToHost, s));
+        IPCG_HIP(hipStreamSynchronize(s));
+    }
+    K* ki = keys;
     uint32_t* vi = vals;
-    uint64_t* ko = keys_alt;
+    K* ko = keys_alt;
     uint32_t* vo = vals_alt;
     for (int shift = 0; shift < key_bits; shift += 8) {
         if (((hv >> shift) & 0xFFull) == 0) continue;
-        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, s, ki, n,
+        hipLaunchKernelGGL(k_radix_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, s, ki, n,
                            shift, counts, ntiles);
         exclusive_scan_u32(counts, offsets, nc, nullptr, stmp, s);
-        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, s, ki,
+        hipLaunchKernelGGL(k_radix_scatter<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, s, ki,
                            vi, ko, vo, n, shift, offsets, ntiles);
         IPCG_CHECK_LAUNCH();
-        uint64_t* tk = ki;
-        ki = ko;
-        ko = tk;
-        uint32_t* tv = vi;
-        vi = vo;
-        vo = tv;
+        std::swap(ki, ko);
+        std::swap(vi, vo);
     }
-    if (ki != keys) {
-        IPCG_HIP(hipMemcpyAsync(keys, ki, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
-        IPCG_HIP(hipMemcpyAsync(vals, vi, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
-    }
+    if (ki == keys) return false;
+    if (!copy_back) return true;
+    IPCG_HIP(hipMemcpyAsync(keys, ki, sizeof(K) * n, hipMemcpyDeviceToDevice, s));
+    IPCG_HIP(hipMemcpyAsync(vals, vi, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    return false;
+}
+
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                      int64_t n, int key_bits, void* tmp, hipStream_t s)
+{
+    radix_sort(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, s, true, true);
+}
+
+bool radix_sort_pairs_all(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                          uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s)
+{
+    return radix_sort(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, s, false, false);
+}
+
+bool radix_sort_pairs_all(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                          uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s)
+{
+    if (key_bits > 32) throw Error(IPCG_ERR_ARG, "32-bit keys hold at most 32 key bits");
+    return radix_sort(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, s, false, false);
 }
 
 size_t rle_tmp_bytes(int64_t n)

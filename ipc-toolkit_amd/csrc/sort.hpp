@@ -1,5 +1,5 @@
 // Hand-written device-wide primitives for gfx950: exclusive scan, stable
-// LSD radix sort of (uint64 key, uint32 value) pairs with 8-bit digits,
+// LSD radix sort of (uint64 or uint32 key, uint32 value) pairs with 8-bit digits,
 // run-length encoding of sorted keys.  Declarations; kernels in sort.hip.
 #pragma once
 
@@ -20,6 +20,13 @@ void exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* 
 size_t radix_tmp_bytes(int64_t n);
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                       int64_t n, int key_bits, void* tmp, hipStream_t s);
+// Every 8-bit pass below key_bits runs, with no host round trip and no copy
+// back: returns true when the sorted pairs are in keys_alt / vals_alt.  For
+// keys whose digits all vary (Morton codes, vertex ids).
+bool radix_sort_pairs_all(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                          uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s);
+bool radix_sort_pairs_all(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                          uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s);
 
 // Sorted keys -> unique keys, run starts (n_unique + 1 entries, last = n)
 // and the number of unique keys (device scalar).  tmp: rle_tmp_bytes(n).

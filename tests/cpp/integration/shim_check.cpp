@@ -156,7 +156,8 @@ int main()
                                                            IPCG_BP_LBVH, &t_c));
     EXPECT(t > 0 && t < 1 && t == t_c);
 
-    // SweepAndTiniestQueue: the same candidate sets as the LBVH, and the
+    // SweepAndTiniestQueue: the sets of the device sweep and prune (FP64
+    // boxes), inside those of the LBVH (FP32 outward-rounded boxes), and the
     // branch's rule on the step (ipc.cpp:63-85): no minimum distance, the
     // earliest time of impact times 0.8 below 1, else 1
     gpu::SweepAndTiniestQueue gpu_stq(ctx);
@@ -177,20 +178,36 @@ int main()
         for (const auto& c : v) p.emplace_back(c.face_id, c.vertex_id);
         return sorted(p);
     };
+    auto subset = [&](Pairs a, Pairs b) { // as unordered pairs: each broad phase
+        for (Pairs* p : { &a, &b }) {     // orders its edge-edge pairs its own way
+            for (auto& x : *p)
+                if (x.second < x.first) std::swap(x.first, x.second);
+            *p = sorted(*p);
+        }
+        return std::includes(b.begin(), b.end(), a.begin(), a.end());
+    };
+    gpu::GPULBVH gpu_sap(ctx, IPCG_BP_SAP);
+    BroadPhase& sap = gpu_sap;
+    EXPECT(sap.name() == "GPUSweepAndPrune");
+    std::vector<EdgeEdgeCandidate> ee_l, ee_p, ee_s;
+    std::vector<FaceVertexCandidate> fv_l, fv_p, fv_s;
     bp.build(V, E, F, 0.5 * dhat);
-    std::vector<EdgeEdgeCandidate> ee_l, ee_s;
-    std::vector<FaceVertexCandidate> fv_l, fv_s;
     bp.detect_edge_edge_candidates(ee_l);
     bp.detect_face_vertex_candidates(fv_l);
+    sap.build(V, E, F, 0.5 * dhat);
+    sap.detect_edge_edge_candidates(ee_p);
+    sap.detect_face_vertex_candidates(fv_p);
     stq.build(V, E, F, 0.5 * dhat);
     stq.detect_edge_edge_candidates(ee_s);
     stq.detect_face_vertex_candidates(fv_s);
-    EXPECT(!ee_s.empty() && ee_pairs(ee_s) == ee_pairs(ee_l) && fv_pairs(fv_s) == fv_pairs(fv_l));
+    EXPECT(!ee_s.empty() && ee_pairs(ee_s) == ee_pairs(ee_p) && fv_pairs(fv_s) == fv_pairs(fv_p));
+    EXPECT(subset(ee_pairs(ee_s), ee_pairs(ee_l)));
+    EXPECT(subset(fv_pairs(fv_s), fv_pairs(fv_l))); // (face, vertex): one order
     const double t_stq = stepsize_by_name(&stq, V, V1, ccd);
     double toi = 0;
     gpu::check(ctx.c, ipcg_compute_collision_free_stepsize(ctx.c, 0, ccd.tolerance,
                                                            ccd.max_iterations, 1.0,
-                                                           IPCG_BP_LBVH, &toi));
+                                                           IPCG_BP_SAP, &toi));
     EXPECT(toi > 0 && toi < 1 && t_stq == 0.8 * toi);
     EXPECT(stepsize_by_name(&stq, V, V, ccd) == 1.0); // no motion, no impact
     Eigen::MatrixXd V2(V.rows(), 2);
