@@ -250,15 +250,19 @@ IPCG_HD void to_qspace(const Red<R>& d, const int* plus, const int* minus,
 }
 
 // point-space gradient: g[plus[i]] += alpha g_i, g[minus[i]] -= alpha g_i
-template <int R>
+// (gp holds NP points; every point is addressed by a compile-time index so
+// the array stays in registers, each point updated in the same order)
+template <int NP, int R>
 IPCG_HD void to_points(const Red<R>& d, const int* plus, const int* minus,
                        double alpha, d3* gp)
 {
 #pragma unroll
-    for (int i = 0; i < R; i++) {
-        gp[plus[i]] = axpy3(gp[plus[i]], alpha, d.g[i]);
-        gp[minus[i]] = axpy3(gp[minus[i]], -alpha, d.g[i]);
-    }
+    for (int i = 0; i < R; i++)
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            if (plus[i] == j) gp[j] = axpy3(gp[j], alpha, d.g[i]);
+            if (minus[i] == j) gp[j] = axpy3(gp[j], -alpha, d.g[i]);
+        }
 }
 
 // Hq += alpha * (a b^T + b a^T) / (sym ? 2 : 1) on packed 3K storage, with

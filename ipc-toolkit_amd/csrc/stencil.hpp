@@ -83,8 +83,8 @@ IPCG_HD void with_distance_derivs(int dtype, const d3* P, Fn&& f)
     case EE_EA1_EB0:
     case EE_EA1_EB1: {
         const int a = (dtype >> 1) & 1, b = 2 + (dtype & 1);
-        Red<1> r;
-        pp_derivs(sub(P[a], P[b]), r);
+        Red<1> r; // the points picked by selects: P stays in registers
+        pp_derivs(sub(a ? P[1] : P[0], b == 3 ? P[3] : P[2]), r);
         const int pl[1] = { a }, mi[1] = { b };
         f(r, pl, mi);
         return;
@@ -92,8 +92,9 @@ IPCG_HD void with_distance_derivs(int dtype, const d3* P, Fn&& f)
     case EE_EA_EB0:
     case EE_EA_EB1: { // point eb0/eb1 against line ea
         const int p = dtype == EE_EA_EB0 ? 2 : 3;
+        const d3 Pp = p == 3 ? P[3] : P[2];
         Red<2> r;
-        pl_derivs(sub(P[0], P[p]), sub(P[1], P[p]), r);
+        pl_derivs(sub(P[0], Pp), sub(P[1], Pp), r);
         const int pl[2] = { 0, 1 }, mi[2] = { p, p };
         f(r, pl, mi);
         return;
@@ -101,8 +102,9 @@ IPCG_HD void with_distance_derivs(int dtype, const d3* P, Fn&& f)
     case EE_EA0_EB:
     case EE_EA1_EB: { // point ea0/ea1 against line eb
         const int p = dtype == EE_EA0_EB ? 0 : 1;
+        const d3 Pp = p ? P[1] : P[0];
         Red<2> r;
-        pl_derivs(sub(P[2], P[p]), sub(P[3], P[p]), r);
+        pl_derivs(sub(P[2], Pp), sub(P[3], Pp), r);
         const int pl[2] = { 2, 3 }, mi[2] = { p, p };
         f(r, pl, mi);
         return;
@@ -161,14 +163,14 @@ IPCG_HD void stencil_gradient(int dtype, const d3* P, double w, double eps_x,
     const double d2 = stencil_dist2(kind, dtype, P);
     const double f1 = bar_f1(d2, bp);
     with_distance_derivs<KIND>(dtype, P, [&](auto& red, const int* pl, const int* mi) {
-        to_points(red, pl, mi, w * M.m * f1, gp);
+        to_points<NP>(red, pl, mi, w * M.m * f1, gp);
     });
     if constexpr (KIND == K_EE) {
         if (M.active) {
             Red<2> X;
             cross2_derivs(sub(P[1], P[0]), sub(P[3], P[2]), X);
             const int pl[2] = { 1, 3 }, mi[2] = { 0, 2 };
-            to_points(X, pl, mi, w * bar_f(d2, bp) * M.m1, gp);
+            to_points<NP>(X, pl, mi, w * bar_f(d2, bp) * M.m1, gp);
         }
     }
 }
