@@ -722,8 +722,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(
                 o1 = o1 & (nd.r1 > qpos);
             }
             const bool l0 = nd.c0 < 0, l1 = nd.c1 < 0;
-            h0 = o0 & l0 ? ~nd.c0 : -1;
-            h1 = o1 & l1 ? ~nd.c1 : -1;
+            h0 = (o0 & l0) ? ~nd.c0 : -1;
+            h1 = (o1 & l1) ? ~nd.c1 : -1;
             const bool t0 = o0 & !l0, t1 = o1 & !l1;
             if (!nd_ok) {
             } else if (!t0 && !t1) {

@@ -22,6 +22,8 @@
 #include "shard.hpp"
 #include "sort.hpp"
 
+#include <utility>
+
 namespace ipcg {
 
 namespace {
@@ -806,8 +808,20 @@ void np_build(Ctx& c, double dhat, double dmin)
     uint32_t* vals = c.np_vals.as<uint32_t>();
     uint32_t* nu_dev = reinterpret_cast<uint32_t*>(c.counters.as<unsigned long long>() + 3);
     uint32_t* kept_dev = nu_dev + 1;
-    const int vbits = key_bits_for(std::max<int64_t>(c.nV, c.nE));
-    const int fbits = key_bits_for(std::max<int64_t>(c.nF, 1));
+    // (id << 32 | id) keys: only the digits that hold id bits can vary, so
+    // the sorts below run those passes without reading the keys back
+    const int ibits = key_bits_for(std::max<int64_t>(std::max<int64_t>(c.nV, c.nE), c.nF)) - 32;
+    const unsigned long long id_mask = (1ull << ibits) - 1ull;
+    const unsigned long long pair_mask = id_mask | (id_mask << 32);
+    auto sort_pairs = [&](int64_t m) {
+        if (radix_sort_pairs_masked(keys, vals, c.np_keys2.as<uint64_t>(),
+                                    c.np_vals2.as<uint32_t>(), m, pair_mask, c.np_tmp.p, s)) {
+            std::swap(c.np_keys, c.np_keys2); // sorted pairs in the second buffers
+            std::swap(c.np_vals, c.np_vals2);
+            keys = c.np_keys.as<uint64_t>();
+            vals = c.np_vals.as<uint32_t>();
+        }
+    };
     for (int k = 0; k < 4; k++) {
         const int64_t m0 = nraw[k], m = mtot[k];
         c.c_ids[k].reserve(sizeof(int2) * (size_t)std::max<int64_t>(m, 1));
@@ -827,8 +841,7 @@ void np_build(Ctx& c, double dhat, double dmin)
             if (m0) shard_compact(S.ee, scap, kcnt, kc[k].max, ee, koff, s);
             if (!real_w) { // one collision per edge-edge candidate: no duplicates
                 hipLaunchKernelGGL(k_ee_keys, dim3(g), dim3(256), 0, s, ee, m, keys, vals);
-                radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(),
-                                 m, vbits, c.np_tmp.p, s);
+                sort_pairs(m);
                 hipLaunchKernelGGL(k_ee_gather, dim3(g), dim3(256), 0, s, ee, vals, m,
                                    c.c_ids[2].as<int2>(), c.c_w[2].as<double>(),
                                    c.c_eps.as<double>(), c.c_dtype.as<int32_t>());
@@ -873,8 +886,7 @@ void np_build(Ctx& c, double dhat, double dmin)
                 }
             }
             hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, s, vals, m);
-            radix_sort_pairs(keys, vals, c.np_keys2.as<uint64_t>(), c.np_vals2.as<uint32_t>(), m,
-                             k == 3 ? fbits : vbits, c.np_tmp.p, s);
+            sort_pairs(m);
             if (k == 3) { // each (face, vertex) candidate is unique
                 hipLaunchKernelGGL(k_keys_to_pairs, dim3(g), dim3(256), 0, s, keys, m, A,
                                    c.c_ids[3].as<int2>(), c.c_w[3].as<double>());

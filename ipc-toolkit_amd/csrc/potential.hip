@@ -162,8 +162,18 @@ __global__ void k_scene_coff(CollView C, const int64_t* soff, int S, int64_t* co
 
 __device__ __forceinline__ void sum_partials(const double* partial, int64_t n, double* out)
 {
+    // a thread adds partial[tid], partial[tid + 256], ... in that order;
+    // eight loads are issued before their adds (one memory wait per eight)
     double t = 0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) t += partial[i];
+    int64_t i = threadIdx.x;
+    for (; i + 7 * 256 < n; i += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = partial[i + 256 * k];
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += v[k];
+    }
+    for (; i < n; i += 256) t += partial[i];
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, WAVE);
     __shared__ double s[4];
     if (threadIdx.x % WAVE == 0) s[threadIdx.x / WAVE] = t;

@@ -314,7 +314,7 @@ size_t radix_tmp_bytes(int64_t n)
 template <class K>
 static bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n,
                        int key_bits, void* tmp, hipStream_t s, bool skip_constant,
-                       bool copy_back)
+                       bool copy_back, unsigned long long bit_mask = ~0ull)
 {
     if (n <= 1) return false;
     const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
@@ -324,7 +324,7 @@ static bool radix_sort(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
     uint32_t* counts = reinterpret_cast<uint32_t*>(t + 256);
     uint32_t* offsets = counts + nc;
     void* stmp = offsets + nc;
-    unsigned long long hv = ~0ull;
+    unsigned long long hv = bit_mask; // digits without a bit here are constant
     if (skip_constant) { // which bits vary at all (one small D2H read per sort)
         IPCG_HIP(hipMemsetAsync(vary, 0, sizeof(unsigned long long), s));
         hipLaunchKernelGGL(k_vary_bits<K>, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, keys,
@@ -365,6 +365,13 @@ bool radix_sort_pairs_all(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                           uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s)
 {
     return radix_sort(keys, vals, keys_alt, vals_alt, n, key_bits, tmp, s, false, false);
+}
+
+bool radix_sort_pairs_masked(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                             uint32_t* vals_alt, int64_t n, unsigned long long bit_mask,
+                             void* tmp, hipStream_t s)
+{
+    return radix_sort(keys, vals, keys_alt, vals_alt, n, 64, tmp, s, false, false, bit_mask);
 }
 
 bool radix_sort_pairs_all(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,

@@ -27,6 +27,11 @@ bool radix_sort_pairs_all(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                           uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s);
 bool radix_sort_pairs_all(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                           uint32_t* vals_alt, int64_t n, int key_bits, void* tmp, hipStream_t s);
+// The same over the 8-bit digits that hold a bit of bit_mask (the others
+// are known to be constant), again without a host round trip.
+bool radix_sort_pairs_masked(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                             uint32_t* vals_alt, int64_t n, unsigned long long bit_mask,
+                             void* tmp, hipStream_t s);
 
 // Sorted keys -> unique keys, run starts (n_unique + 1 entries, last = n)
 // and the number of unique keys (device scalar).  tmp: rle_tmp_bytes(n).
