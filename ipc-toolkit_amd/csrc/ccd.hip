@@ -574,11 +574,18 @@ __global__ void k_wf_counts(const WFState* st, const WFBox* boxes, const uint8_t
 // a search whose level left no children ends (at its earliest tolerance
 // box, if any)
 // off: the scan of the children counts, off[nbox] their total
+// also clears the level's counters (npend[0..1] for k_wf_pending, nroot for
+// k_wf_segments), which run after it on the same stream
 __global__ void k_wf_nochild(CCDArgs A, WFState* st, int64_t nb, const uint32_t* off,
-                             int64_t nbox)
+                             int64_t nbox, unsigned long long* npend, unsigned long long* nroot)
 {
     const int64_t total_children = (int64_t)off[nbox];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) {
+        npend[0] = 0;
+        npend[1] = 0;
+        *nroot = 0;
+    }
     if (i >= nb) return;
     WFState& S = st[i];
     if (S.done || S.restart) return;
@@ -744,8 +751,7 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
                            cnt);
         exclusive_scan_u32(cnt, off, nbox, off + nbox, c.blk_tmp.p, s);
         hipLaunchKernelGGL(k_wf_nochild, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb,
-                           off, nbox);
-        IPCG_HIP(hipMemsetAsync(npend, 0, 2 * sizeof(unsigned long long), s));
+                           off, nbox, npend, nroot);
         hipLaunchKernelGGL(k_wf_pending, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, npend);
         IPCG_CHECK_LAUNCH();
         // one round trip per level: the children's total, the searches
@@ -763,7 +769,6 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
         c.blk_ukeys.reserve(sizeof(WFBox) * (size_t)std::max<int64_t>(nnext, 1));
         WFBox* nxt = reinterpret_cast<WFBox*>(c.blk_keys2.p);
         WFBox* ord = reinterpret_cast<WFBox*>(c.blk_ukeys.p);
-        IPCG_HIP(hipMemsetAsync(nroot, 0, sizeof(unsigned long long), s));
         hipLaunchKernelGGL(k_wf_emit, dim3(gb), dim3(256), 0, s, states, cur, cmask, off, nbox, nxt);
         hipLaunchKernelGGL(k_wf_segments, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, off,
                            nbox, nroot, children, nxt);

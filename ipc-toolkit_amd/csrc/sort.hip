@@ -111,6 +111,31 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_final(const T* in, T* out, 
     }
 }
 
+// n <= SCAN_TILE: the whole scan in one workgroup (one launch instead of
+// three)
+template <class T>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_one(const T* in, T* out, int64_t n,
+                                                         T* total)
+{
+    T v[SCAN_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        const int64_t i = (int64_t)threadIdx.x * SCAN_ITEMS + j;
+        v[j] = i < n ? in[i] : 0;
+        s += v[j];
+    }
+    T tot;
+    T off = block_exclusive_scan<T>(s, tot);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        const int64_t i = (int64_t)threadIdx.x * SCAN_ITEMS + j;
+        if (i < n) out[i] = off;
+        off += v[j];
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
 template <class T>
 void exclusive_scan(const T* in, T* out, int64_t n, T* total, void* tmp, hipStream_t s)
 {
@@ -118,6 +143,11 @@ void exclusive_scan(const T* in, T* out, int64_t n, T* total, void* tmp, hipStre
     T* partial = static_cast<T*>(tmp);
     if (n <= 0) {
         if (total) IPCG_HIP(hipMemsetAsync(total, 0, sizeof(T), s));
+        return;
+    }
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_one<T>, dim3(1), dim3(SCAN_BLOCK), 0, s, in, out, n, total);
+        IPCG_CHECK_LAUNCH();
         return;
     }
     hipLaunchKernelGGL(k_scan_partials<T>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n,
