@@ -456,30 +456,41 @@ __global__ __launch_bounds__(RF_BLOCK) void k_refit_top(const int32_t* cur, cons
 }
 
 
+// One level: the workgroups stride over the level's list (a grid of at
+// most RF_GRID workgroups, whatever the list's bound) and clear the counter
+// of the level after next (clear_n: read as input by the previous level,
+// appended to by the next one, so stream order makes the clear safe).
+constexpr unsigned RF_GRID = 512;
+
 __global__ __launch_bounds__(RF_BLOCK) void k_refit_level(const int32_t* cur,
                                                           const uint32_t* cur_n,
                                                           const int32_t* parent,
                                                           const int32_t* lscene, Node* nodes,
                                                           int32_t* visits, int32_t* next,
-                                                          uint32_t* next_n, int32_t* root)
+                                                          uint32_t* next_n, uint32_t* clear_n,
+                                                          int32_t* root)
 {
-    const uint32_t t = blockIdx.x * RF_BLOCK + threadIdx.x;
-    bool push = false;
-    int32_t p = 0;
-    if (t < *cur_n) {
-        const int32_t j = cur[t];
-        const Node* N = nodes + j;
-        float lo[3], hi[3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_n = 0;
+    const uint32_t m = *cur_n;
+    for (uint32_t base = blockIdx.x * RF_BLOCK; base < m; base += gridDim.x * RF_BLOCK) {
+        const uint32_t t = base + threadIdx.x; // the loop count is uniform per workgroup
+        bool push = false;
+        int32_t p = 0;
+        if (t < m) {
+            const int32_t j = cur[t];
+            const Node* N = nodes + j;
+            float lo[3], hi[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            lo[k] = fminf(N->lo0[k], N->lo1[k]);
-            hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
+            for (int k = 0; k < 3; k++) {
+                lo[k] = fminf(N->lo0[k], N->lo1[k]);
+                hi[k] = fmaxf(N->hi0[k], N->hi1[k]);
+            }
+            p = parent[j];
+            if (p < 0) root[lscene ? lscene[j] : 0] = j; // the tree is complete
+            else push = refit_child(nodes, p, j, lo, hi, visits);
         }
-        p = parent[j];
-        if (p < 0) root[lscene ? lscene[j] : 0] = j; // the tree is complete
-        else push = refit_child(nodes, p, j, lo, hi, visits);
+        refit_append(push, p, next, next_n);
     }
-    refit_append(push, p, next, next_n);
 }
 
 __device__ __forceinline__ bool overlap(const float* q, const float* lo, const float* hi)
@@ -1132,15 +1143,16 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
         IPCG_HIP(hipStreamSynchronize(s));
         return;
     }
-    // int32 arrays: parent[n-1], visits[n-1], two work lists of n, and two
-    // counters
+    // int32 arrays: parent[n-1], visits[n-1], two work lists of n, and three
+    // counters (level k reads cnt[(k-1) % 3], appends to cnt[k % 3] and
+    // clears cnt[(k+1) % 3]; the lists alternate)
     const size_t ni = (size_t)n - 1;
     T.info.reserve(sizeof(int32_t) * (2 * ni + 2 * (size_t)n) + 256);
     int32_t* parent = T.info.as<int32_t>();
     int32_t* visits = parent + ni;
     int32_t* work[2] = { visits + ni, visits + ni + n };
     uint32_t* cnt = reinterpret_cast<uint32_t*>(work[1] + n);
-    IPCG_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(uint32_t), s));
+    IPCG_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(uint32_t), s));
     if (!forest) // "unfinished" sentinel
         IPCG_HIP(hipMemsetAsync(T.root.p, 0x80, sizeof(int32_t), s));
     c.stage_begin("bp_build");
@@ -1160,22 +1172,21 @@ void build_tree(Ctx& c, Tree& T, const int32_t* I, int arity, int n, const doubl
     for (size_t level = 1;; level += BATCH) {
         for (size_t k = level; k < level + BATCH; k++) {
             const int64_t bound = std::max<int64_t>((int64_t)n / (int64_t)(k + 1), 1);
-            uint32_t* cur_n = cnt + ((k - 1) & 1);
-            uint32_t* nxt = cnt + (k & 1);
-            IPCG_HIP(hipMemsetAsync(nxt, 0, sizeof(uint32_t), s));
-            hipLaunchKernelGGL(k_refit_level, dim3(grid_for(bound, RF_BLOCK)), dim3(RF_BLOCK), 0,
-                               s, work[(k - 1) & 1], cur_n, parent, lscene, T.nodes.as<Node>(),
-                               visits, work[k & 1], nxt, T.root.as<int32_t>());
+            const unsigned grid = std::min<unsigned>(grid_for(bound, RF_BLOCK), RF_GRID);
+            hipLaunchKernelGGL(k_refit_level, dim3(grid), dim3(RF_BLOCK), 0, s,
+                               work[(k - 1) & 1], cnt + (k - 1) % 3, parent, lscene,
+                               T.nodes.as<Node>(), visits, work[k & 1], cnt + k % 3,
+                               cnt + (k + 1) % 3, T.root.as<int32_t>());
         }
         IPCG_CHECK_LAUNCH();
-        IPCG_HIP(hipMemcpyAsync(h, cnt + ((level + BATCH - 1) & 1), sizeof(uint32_t),
+        IPCG_HIP(hipMemcpyAsync(h, cnt + (level + BATCH - 1) % 3, sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s));
         IPCG_HIP(hipStreamSynchronize(s));
         if (h[0] == 0) break;
         if (h[0] <= (uint32_t)RF_TOP) { // the rest in one workgroup
             const size_t k = level + BATCH; // the next round
             hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(RF_BLOCK), 0, s, work[(k - 1) & 1],
-                               cnt + ((k - 1) & 1), parent, lscene, T.nodes.as<Node>(), visits,
+                               cnt + (k - 1) % 3, parent, lscene, T.nodes.as<Node>(), visits,
                                T.root.as<int32_t>(), (int64_t)ni);
             IPCG_CHECK_LAUNCH();
             break;
