@@ -377,10 +377,18 @@ __device__ __forceinline__ double wf_bound(const CCDArgs& A, const WFState& S)
 // level is one box; a t split merges its halves in order, other splits keep
 // it); the per-level rules below rely on that and a level out of order is
 // reported as an error.
+// also clears the level's counters (lvl[0..1] for k_wf_nochild, nroot for
+// k_wf_segments), which run after it on the same stream
 __global__ void k_wf_eval(CCDArgs A, const WFState* st, const WFBox* boxes, int64_t n,
-                          uint8_t* flags, uint32_t* live, int* error)
+                          uint8_t* flags, uint32_t* live, int* error, unsigned long long* lvl,
+                          unsigned long long* nroot)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j == 0) {
+        lvl[0] = 0;
+        lvl[1] = 0;
+        *nroot = 0;
+    }
     if (j >= n) return;
     const WFBox b = boxes[j];
     const WFState& S = st[b.owner];
@@ -572,28 +580,29 @@ __global__ void k_wf_counts(const WFState* st, const WFBox* boxes, const uint8_t
 }
 
 // a search whose level left no children ends (at its earliest tolerance
-// box, if any)
+// box, if any); then the level's counts for the host, in one read:
+// lvl[0] searches still pending, lvl[1] of those restarting at a root box,
+// lvl[2] the children's total
 // off: the scan of the children counts, off[nbox] their total
-// also clears the level's counters (npend[0..1] for k_wf_pending, nroot for
-// k_wf_segments), which run after it on the same stream
 __global__ void k_wf_nochild(CCDArgs A, WFState* st, int64_t nb, const uint32_t* off,
-                             int64_t nbox, unsigned long long* npend, unsigned long long* nroot)
+                             int64_t nbox, unsigned long long* lvl)
 {
     const int64_t total_children = (int64_t)off[nbox];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) {
-        npend[0] = 0;
-        npend[1] = 0;
-        *nroot = 0;
-    }
+    if (i == 0) lvl[2] = (unsigned long long)total_children;
     if (i >= nb) return;
     WFState& S = st[i];
-    if (S.done || S.restart) return;
-    const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
-    const int64_t a = s0 < nbox ? (int64_t)off[s0] : total_children;
-    const int64_t b = s1 < nbox ? (int64_t)off[s1] : total_children;
-    if (b == a)
-        wf_search_done(A, S, S.use_skip != 0, S.use_skip ? S.toi_skip : INFINITY, S.delta);
+    if (!S.done && !S.restart) {
+        const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
+        const int64_t a = s0 < nbox ? (int64_t)off[s0] : total_children;
+        const int64_t b = s1 < nbox ? (int64_t)off[s1] : total_children;
+        if (b == a)
+            wf_search_done(A, S, S.use_skip != 0, S.use_skip ? S.toi_skip : INFINITY, S.delta);
+    }
+    if (!S.done) {
+        atomicAdd(lvl, 1ull);
+        if (S.restart) atomicAdd(lvl + 1, 1ull);
+    }
 }
 
 __global__ void k_wf_emit(const WFState* st, const WFBox* boxes, const uint8_t* cmask,
@@ -695,17 +704,6 @@ __global__ void k_wf_finish(CCDArgs A, const WFState* st, int64_t nb)
     report(A, S.ci, S.hit != 0, S.result, S.scene);
 }
 
-// n[0]: candidates still searching, n[1]: of those, restarting at a root box
-__global__ void k_wf_pending(const WFState* st, int64_t nb, unsigned long long* n)
-{
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < nb && !st[i].done) {
-        atomicAdd(n, 1ull);
-        if (st[i].restart) atomicAdd(n + 1, 1ull);
-    }
-}
-
-
 // Runs one batch of heavy candidates to completion.  Returns false, with
 // nothing reported, when a level exceeds the box budget.
 bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, int64_t nb,
@@ -741,7 +739,7 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
         c.blk_tmp.reserve(scan_tmp_bytes(nbox + 1));
         const unsigned gb = grid_for(nbox, 256, INT32_MAX);
         hipLaunchKernelGGL(k_wf_eval, dim3(gb), dim3(256), 0, s, A, states, cur, nbox, flags, live,
-                           error);
+                           error, npend, nroot);
         exclusive_scan_u32(live, lpre, nbox, lpre + nbox, c.blk_tmp.p, s);
         hipLaunchKernelGGL(k_wf_mark, dim3(gb), dim3(256), 0, s, states, cur, nbox, flags, lpre,
                            ev);
@@ -751,17 +749,15 @@ bool wf_batch(Ctx& c, const CCDArgs& A, const uint32_t* heavy, int64_t first, in
                            cnt);
         exclusive_scan_u32(cnt, off, nbox, off + nbox, c.blk_tmp.p, s);
         hipLaunchKernelGGL(k_wf_nochild, dim3(grid_for(nb, 256)), dim3(256), 0, s, A, states, nb,
-                           off, nbox, npend, nroot);
-        hipLaunchKernelGGL(k_wf_pending, dim3(grid_for(nb, 256)), dim3(256), 0, s, states, nb, npend);
+                           off, nbox, npend);
         IPCG_CHECK_LAUNCH();
-        // one round trip per level: the children's total, the searches
-        // still pending and those restarting
-        IPCG_HIP(hipMemcpyAsync(h, off + nbox, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        IPCG_HIP(hipMemcpyAsync(h + 1, npend, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+        // one round trip per level: the searches still pending, those
+        // restarting and the children's total
+        IPCG_HIP(hipMemcpyAsync(h, npend, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                 s));
         IPCG_HIP(hipStreamSynchronize(s));
-        const int64_t children = reinterpret_cast<uint32_t*>(h)[0];
-        const int64_t pending = (int64_t)h[1], restarts = (int64_t)h[2];
+        const int64_t pending = (int64_t)h[0], restarts = (int64_t)h[1];
+        const int64_t children = (int64_t)h[2];
         if (pending == 0) break;
         const int64_t nnext = children + restarts; // a restart is one root box
         if (nnext > WF_MAX_BOXES) return false;
@@ -839,7 +835,7 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     int* any_hit = reinterpret_cast<int*>(st + 192);
     int* error = reinterpret_cast<int*>(st + 256);
     unsigned long long* nroot = reinterpret_cast<unsigned long long*>(st + 320);
-    unsigned long long* npend = reinterpret_cast<unsigned long long*>(st + 384); // [2]
+    unsigned long long* npend = reinterpret_cast<unsigned long long*>(st + 384); // [3]
     unsigned long long* n_spill = reinterpret_cast<unsigned long long*>(st + 448); // [2]
     const double one = 1.0;
     IPCG_HIP(hipMemsetAsync(st, 0, 512, s));
