@@ -467,8 +467,9 @@ struct CollRanges {
     int64_t goff[5]; // global collision index ranges per kind
 };
 
-constexpr int PH = 1024; // hash slots per wave
-constexpr int PU = 512;  // most distinct columns of a row on this path
+constexpr int PH_BITS = 9;
+constexpr int PH = 1 << PH_BITS; // hash slots per wave (LDS: 8 waves per SIMD fit)
+constexpr int PU = 256;          // most distinct columns of a row on this path
 
 __global__ __launch_bounds__(256) void k_pattern_rows(const uint32_t* vptr, const int4* inc_vtx,
                                                       int nV, int32_t* stage, uint32_t* cnt,
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(256) void k_pattern_rows(const uint32_t* vptr, cons
         for (int b = 0; b < 4; b++) {
             const int key = b == 0 ? vs.x : b == 1 ? vs.y : b == 2 ? vs.z : vs.w;
             if (key < 0) continue;
-            unsigned h = ((unsigned)key * 2654435761u) >> (32 - 10);
+            unsigned h = ((unsigned)key * 2654435761u) >> (32 - PH_BITS); // in [0, PH)
             for (int probe = 0; probe < PH; probe++) {
                 const int old = atomicCAS(&hs[wv][h], -1, key);
                 if (old == -1) {
