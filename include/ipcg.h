@@ -477,6 +477,9 @@ int ipcg_stage_times_reset(ipcg_ctx* ctx, int enable);
  * lists), "ccd.wavefront" (searches that outgrew a lane's HBM slab),
  * "hessian.ee_other" (edge-edge collisions of the last Hessian whose type
  * is not the interior one; their projection takes the 9 x 9 path),
+ * "hessian.pattern_sorted" (1 when a row of the last Hessian pattern had
+ * more than 256 distinct block columns and the pattern was built by a
+ * sort of all stencil pairs instead of the per-row hash),
  * "intersection.candidates" (edge-face candidates of the last
  * has_intersections), "intersection.deferred" (their orient3d calls the
  * floating-point filter left to the exact evaluation) */

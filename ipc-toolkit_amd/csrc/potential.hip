@@ -1434,7 +1434,11 @@ static CollRanges hess_prepare(Ctx& c, const CollView& C)
     // block pattern: from the incidences, or by sorting the stencil pairs
     // when a row outgrows the LDS path
     c.stage_begin("hessian_pattern");
-    if (!c.pattern_valid && !hess_pattern_incidences(c, M)) hess_pattern(c);
+    if (!c.pattern_valid) {
+        const bool hashed = hess_pattern_incidences(c, M);
+        if (!hashed) hess_pattern(c); // a row outgrew the LDS hash
+        c.stat["hessian.pattern_sorted"] = hashed ? 0 : 1;
+    }
     c.stage_end();
     c.hrec.reserve(sizeof(double) * REC_LEN * (size_t)std::max<int64_t>(G.goff[4], 1));
     return G;

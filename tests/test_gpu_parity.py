@@ -430,3 +430,52 @@ def test_codim_kats(ctx, O):
     assert t == pytest.approx((1 - (2e-3 + 1e-4)) / 4, rel=1.2e-5)
     ctx.build_collisions(0.25, 0.8)
     assert ctx.collision_counts().tolist() == [2, 4, 0, 0]
+
+
+
+def dense_row_scene(scenes):
+    """A 31 x 31 grid (patch 0) and one small triangle (patch 1) 0.3 dhat
+    above its centre, dhat = 12 grid cells: each vertex of the triangle is
+    within dhat of about 400 grid vertices, so its Hessian row has more
+    block columns than the per-row hash holds; the patch filter keeps the
+    grid's own pairs out, so the set stays small."""
+    n, h = 31, 1.0 / 30
+    dhat = 12 * h
+    Vg, Fg = scenes.grid_sheet(n, 0.0)
+    c = 0.5
+    Vt = np.array([[c - 0.3 * h, c - 0.2 * h, 0.3 * dhat], [c + 0.35 * h, c - 0.1 * h, 0.3 * dhat],
+                   [c, c + 0.3 * h, 0.3 * dhat]])
+    V, E, F = scenes.merge([(Vg, Fg), (Vt, np.array([[0, 1, 2]]))])
+    group = (np.arange(len(V)) >= len(Vg)).astype(np.int32)
+    return scenes.Scene("dense_rows", V, E, F, V.copy(), dhat, 1e3), group
+
+
+def test_hessian_pattern_of_dense_rows(capi, O, scenes):
+    """Rows with more distinct block columns than the per-row hash holds take
+    the sorted-pair pattern; the Hessians still match the oracle, and a
+    sparse scene keeps the hash path."""
+    s, group = dense_row_scene(scenes)
+    say("case dense rows", s.sizes)
+    ctx = capi.Context(0)
+    try:
+        ctx.upload_mesh(s.V, s.E, s.F, vertex_patch=group)
+        ctx.upload_positions(0, s.V)
+        ctx.build_collisions(s.dhat)
+        cg = ctx.collisions()
+        ee, fv = O.candidates(s.V, s.E, s.F, 0.5 * s.dhat, vertex_group=group)
+        co = O.build_collisions(s.V, s.E, s.F, ee, fv, s.dhat)
+        for k in ("vv", "ev", "ee", "fv"):
+            assert np.array_equal(getattr(co, k), cg[k]), k
+        for psd in ("none", "clamp"):
+            Ho = O.barrier_hessian(co, s.V, s.E, s.F, s.dhat, s.kappa, psd)
+            Hg = ctx.hessian(s.dhat, s.kappa, psd)
+            assert ctx.counter("hessian.pattern_sorted") == 1
+            assert np.diff(Hg.indptr).max() > 3 * 256  # a row over 256 block columns
+            assert sparse_rel_err(Hg, Ho) <= TOL_HESS, psd
+        s2 = make(scenes, "sheets:16:0")
+        upload(ctx, s2)
+        ctx.build_collisions(s2.dhat)
+        ctx.hessian(s2.dhat, s2.kappa, "clamp")
+        assert ctx.counter("hessian.pattern_sorted") == 0
+    finally:
+        ctx.close()
