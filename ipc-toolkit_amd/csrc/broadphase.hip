@@ -1392,16 +1392,16 @@ void bp_build(Ctx& c, bool swept, double inflation, int method)
             c.stage_end();
         }
         IPCG_CHECK_LAUNCH();
-        int* hs = static_cast<int*>(c.pinned);
-        IPCG_HIP(hipMemcpyAsync(hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
-        IPCG_HIP(hipStreamSynchronize(s));
-        const int st = hs[0];
+        // the lists' counts and the walk status in one read-back
+        ShardCounts kc[4];
+        int st = 0;
+        c.shard_sum.reserve(256);
+        shard_counts_n(cnt[0], lists, c.shard_sum.as<unsigned long long>(), c.pinned, s, kc,
+                       status, &st);
         if (st & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
         if (st & 1) throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
-        ShardCounts kc[4];
         bool again = false;
         for (int k = 0; k < lists; k++) {
-            kc[k] = shard_counts(cnt[k], c.pinned, s);
             if (kc[k].max > *scap[k]) {
                 *scap[k] = kc[k].max + kc[k].max / 4 + 1024;
                 again = true;
@@ -1471,12 +1471,13 @@ void bp_edge_face(Ctx& c, double inflation)
         launch_traverse<TR_EF, 0>(TR_BLOCK, a, s);
         IPCG_CHECK_LAUNCH();
         c.stage_end();
-        int* hs = static_cast<int*>(c.pinned);
-        IPCG_HIP(hipMemcpyAsync(hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
-        IPCG_HIP(hipStreamSynchronize(s));
-        if (hs[0] & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
-        if (hs[0] & 1) throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
-        const ShardCounts kc = shard_counts(cnt, c.pinned, s);
+        ShardCounts kc;
+        int st = 0;
+        c.shard_sum.reserve(256);
+        shard_counts_n(cnt, 1, c.shard_sum.as<unsigned long long>(), c.pinned, s, &kc, status,
+                       &st);
+        if (st & 2) throw Error(IPCG_ERR_NUMERIC, "LBVH build produced an invalid tree");
+        if (st & 1) throw Error(IPCG_ERR_NUMERIC, "LBVH traversal stack overflow (depth > 64)");
         if (kc.max > c.scap_ef) {
             c.scap_ef = kc.max + kc.max / 4 + 1024;
             continue;

@@ -162,7 +162,7 @@ static std::vector<DevBuf*> all_bufs(Ctx& c)
                        &c.hp_mark, &c.hp_bptr, &c.hp_bcol, &c.cand_stage[0], &c.cand_stage[1],
                        &c.shard_cnt, &c.shard_off, &c.np_eec, &c.ccd_stage, &c.ccd_todo,
                        &c.tv.leaf, &c.te.leaf, &c.tf.leaf, &c.inc_keys, &c.inc_keys2,
-                       &c.inc_vals, &c.inc_vals2, &c.inc_ptr, &c.inc_vtx, &c.ee_other, &c.hrec, &c.tr_deep,
+                       &c.inc_vals, &c.inc_vals2, &c.inc_ptr, &c.inc_vtx, &c.ee_other, &c.hrec, &c.tr_deep, &c.shard_sum,
                        &c.tv.sbox, &c.te.sbox, &c.tf.sbox, &c.cv, &c.ce, &c.cev,
                        &c.cand_vv, &c.cand_ev, &c.cand_stage[2], &c.cand_stage[3],
                        &c.cand_ef, &c.isect, &c.gn_out, &c.gn_p, &c.v_area, &c.e_area,

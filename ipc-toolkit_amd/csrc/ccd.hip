@@ -888,7 +888,8 @@ double ccd_stepsize(Ctx& c, double min_distance, double tol, int64_t max_iter, d
     hipLaunchKernelGGL(k_ccd_root, dim3(grid_for(n, 256, INT32_MAX)), dim3(256), 0, s, A, RS,
                        c.ccd_stage.as<uint32_t>());
     IPCG_CHECK_LAUNCH();
-    const ShardCounts rc = shard_counts(RS.cnt, c.pinned, s);
+    c.shard_sum.reserve(256);
+    const ShardCounts rc = shard_counts(RS.cnt, c.shard_sum.as<unsigned long long>(), c.pinned, s);
     if (rc.max > scap) throw Error(IPCG_ERR_NUMERIC, "CCD root pass outgrew its shard");
     c.ccd_todo.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(rc.total, 1));
     shard_compact(c.ccd_stage.as<uint32_t>(), scap, RS.cnt, rc.max, c.ccd_todo.as<uint32_t>(),

@@ -111,6 +111,7 @@ struct Ctx {
     int64_t scap_ee = 0, scap_fv = 0, scap_vv = 0, scap_ev = 0; // shard regions
     DevBuf cand_stage[4];              // sharded traversal output (ee, fv, vv, ev)
     DevBuf tr_deep;                    // traversal stack entries beyond LDS
+    DevBuf shard_sum;                  // per counter set: total, largest shard (u64)
     DevBuf shard_cnt, shard_off;       // shard counters / offsets of the append lists
 
     // collision set

@@ -772,8 +772,9 @@ void np_build(Ctx& c, double dhat, double dmin)
     c.stage_end();
     ShardCounts kc[4];
     int64_t nraw[4];
+    c.shard_sum.reserve(256);
+    shard_counts_n(S.cnt, 4, c.shard_sum.as<unsigned long long>(), c.pinned, s, kc);
     for (int k = 0; k < 4; k++) {
-        kc[k] = shard_counts(S.cnt + (int64_t)k * SHARDS * SHARD_PAD, c.pinned, s);
         if (kc[k].max > scap) throw Error(IPCG_ERR_NUMERIC, "collision stream outgrew its shard");
         nraw[k] = kc[k].total;
     }

@@ -47,8 +47,16 @@ struct ShardCounts {
     int64_t total = 0, max = 0;
 };
 
-// counts of the last producer launch (synchronises the stream)
-ShardCounts shard_counts(const unsigned long long* cnt_dev, void* pinned, hipStream_t s);
+// Counts of the last producer launch over nsets consecutive counter sets
+// (set k at cnt_dev + k * SHARDS * SHARD_PAD): each set's total and largest
+// shard are reduced on the device into sum_dev (2 * nsets u64) and read
+// back in one small copy with one synchronisation of the stream, together
+// with *extra_dev when it is given.
+ShardCounts shard_counts(const unsigned long long* cnt_dev, unsigned long long* sum_dev,
+                         void* pinned, hipStream_t s);
+void shard_counts_n(const unsigned long long* cnt_dev, int nsets, unsigned long long* sum_dev,
+                    void* pinned, hipStream_t s, ShardCounts* out, const int* extra_dev = nullptr,
+                    int* extra = nullptr);
 
 // exclusive offsets of the shard counts, each clipped to scap (SHARDS + 1
 // int64 on the device)

@@ -453,17 +453,59 @@ __global__ void k_shard_offsets(const unsigned long long* cnt, int64_t scap, int
     }
 }
 
-ShardCounts shard_counts(const unsigned long long* cnt_dev, void* pinned, hipStream_t s)
+// one workgroup per counter set: its total and its largest shard count
+__global__ __launch_bounds__(SHARDS) void k_shard_summary(const unsigned long long* cnt,
+                                                          unsigned long long* out)
 {
-    IPCG_HIP(hipMemcpyAsync(pinned, cnt_dev, SHARD_CNT_BYTES, hipMemcpyDeviceToHost, s));
-    IPCG_HIP(hipStreamSynchronize(s));
-    const unsigned long long* h = static_cast<const unsigned long long*>(pinned);
-    ShardCounts r;
-    for (int i = 0; i < SHARDS; i++) {
-        const int64_t v = (int64_t)h[(int64_t)i * SHARD_PAD];
-        r.total += v;
-        r.max = std::max(r.max, v);
+    __shared__ unsigned long long wsum[SHARDS / WAVE], wmax[SHARDS / WAVE];
+    const unsigned long long v =
+        cnt[(int64_t)blockIdx.x * SHARDS * SHARD_PAD + (int64_t)threadIdx.x * SHARD_PAD];
+    unsigned long long sm = v, mx = v;
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        sm += __shfl_xor(sm, o, WAVE);
+        const unsigned long long y = __shfl_xor(mx, o, WAVE);
+        mx = y > mx ? y : mx;
     }
+    if (threadIdx.x % WAVE == 0) {
+        wsum[threadIdx.x / WAVE] = sm;
+        wmax[threadIdx.x / WAVE] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0, m = 0;
+        for (int w = 0; w < SHARDS / WAVE; w++) {
+            t += wsum[w];
+            m = wmax[w] > m ? wmax[w] : m;
+        }
+        out[2 * blockIdx.x] = t;
+        out[2 * blockIdx.x + 1] = m;
+    }
+}
+
+void shard_counts_n(const unsigned long long* cnt_dev, int nsets, unsigned long long* sum_dev,
+                    void* pinned, hipStream_t s, ShardCounts* out, const int* extra_dev,
+                    int* extra)
+{
+    hipLaunchKernelGGL(k_shard_summary, dim3(nsets), dim3(SHARDS), 0, s, cnt_dev, sum_dev);
+    IPCG_CHECK_LAUNCH();
+    unsigned long long* h = static_cast<unsigned long long*>(pinned);
+    IPCG_HIP(hipMemcpyAsync(h, sum_dev, sizeof(unsigned long long) * 2 * nsets,
+                            hipMemcpyDeviceToHost, s));
+    if (extra_dev)
+        IPCG_HIP(hipMemcpyAsync(h + 2 * nsets, extra_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    IPCG_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < nsets; k++) {
+        out[k].total = (int64_t)h[2 * k];
+        out[k].max = (int64_t)h[2 * k + 1];
+    }
+    if (extra_dev && extra) *extra = *reinterpret_cast<const int*>(h + 2 * nsets);
+}
+
+ShardCounts shard_counts(const unsigned long long* cnt_dev, unsigned long long* sum_dev,
+                         void* pinned, hipStream_t s)
+{
+    ShardCounts r;
+    shard_counts_n(cnt_dev, 1, sum_dev, pinned, s, &r);
     return r;
 }
 
