@@ -33,30 +33,45 @@ struct CandLists {
 __device__ __forceinline__ int cand_vertices(const CandLists& L, int64_t i, int* v, int& kind)
 {
     kind = i < L.off[1] ? 0 : i < L.off[2] ? 1 : i < L.off[3] ? 2 : 3;
-    const int2 p = L.l[kind][i - L.off[kind]];
+    // list and offset by selects: a runtime index into the argument arrays
+    // would copy them to scratch memory
+    const int2* lst = kind == 0 ? L.l[0] : kind == 1 ? L.l[1] : kind == 2 ? L.l[2] : L.l[3];
+    const int64_t o = kind == 0 ? L.off[0] : kind == 1 ? L.off[1] : kind == 2 ? L.off[2] : L.off[3];
+    const int2 p = lst[i - o];
+    // every path sets all four entries (unused ones to -1), so the array
+    // stays in registers
+    int v0, v1, v2 = -1, v3 = -1, n;
     switch (kind) {
     case 0:
-        v[0] = L.E[2 * (int64_t)p.x];
-        v[1] = L.E[2 * (int64_t)p.x + 1];
-        v[2] = L.E[2 * (int64_t)p.y];
-        v[3] = L.E[2 * (int64_t)p.y + 1];
-        return 4;
+        v0 = L.E[2 * (int64_t)p.x];
+        v1 = L.E[2 * (int64_t)p.x + 1];
+        v2 = L.E[2 * (int64_t)p.y];
+        v3 = L.E[2 * (int64_t)p.y + 1];
+        n = 4;
+        break;
     case 1:
-        v[0] = p.y;
-        v[1] = L.F[3 * (int64_t)p.x];
-        v[2] = L.F[3 * (int64_t)p.x + 1];
-        v[3] = L.F[3 * (int64_t)p.x + 2];
-        return 4;
+        v0 = p.y;
+        v1 = L.F[3 * (int64_t)p.x];
+        v2 = L.F[3 * (int64_t)p.x + 1];
+        v3 = L.F[3 * (int64_t)p.x + 2];
+        n = 4;
+        break;
     case 2:
-        v[0] = p.x;
-        v[1] = p.y;
-        return 2;
+        v0 = p.x;
+        v1 = p.y;
+        n = 2;
+        break;
     default:
-        v[0] = p.y;
-        v[1] = L.E[2 * (int64_t)p.x];
-        v[2] = L.E[2 * (int64_t)p.x + 1];
-        return 3;
+        v0 = p.y;
+        v1 = L.E[2 * (int64_t)p.x];
+        v2 = L.E[2 * (int64_t)p.x + 1];
+        n = 3;
     }
+    v[0] = v0;
+    v[1] = v1;
+    v[2] = v2;
+    v[3] = v3;
+    return n;
 }
 
 __global__ __launch_bounds__(CH_BLOCK) void k_fill_u64(unsigned long long* a, int64_t n,
@@ -92,8 +107,10 @@ __global__ __launch_bounds__(CH_BLOCK) void k_safe_d2(CandLists L, const double*
     // stored minimum only decreases, so a key not below a value read from it
     // cannot lower it: the atomic is issued only for keys below the read.
     const unsigned long long key = (unsigned long long)__double_as_longlong(d2 + 0.0);
-    for (int k = 0; k < nv; k++)
-        if (key < __hip_atomic_load(best + v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+#pragma unroll
+    for (int k = 0; k < 4; k++) // compile-time indices keep v in registers
+        if (k < nv
+            && key < __hip_atomic_load(best + v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMin(best + v[k], key);
 }
 
@@ -103,8 +120,9 @@ __global__ __launch_bounds__(CH_BLOCK) void k_mark(CandLists L, uint32_t* mark)
     if (i >= L.off[4]) return;
     int v[4], kind;
     const int nv = cand_vertices(L, i, v, kind);
-    for (int k = 0; k < nv; k++)
-        if (!mark[v[k]]) mark[v[k]] = 1u; // most vertices are marked many times
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (k < nv && !mark[v[k]]) mark[v[k]] = 1u; // most vertices are marked many times
 }
 
 // largest squared displacement over marked vertices (block partials)
