@@ -972,15 +972,21 @@ double ipco_barrier_energy(const ipco_collisions *c, const double *V, int nV,
 {
     (void)nV;
     const int64_t n = n_collisions(c);
-    double sum = 0;
-#pragma omp parallel for schedule(static) reduction(+ : sum)
+    /* per-collision terms in parallel, summed in collision order: the
+       result does not depend on the thread count */
+    double *e = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!e) return NAN;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         stencil s;
         double x[12];
         get_stencil(c, E, F, i, &s);
         gather(V, &s, x);
-        sum += local_energy(&s, x, c->dmin, dhat, kappa);
+        e[i] = local_energy(&s, x, c->dmin, dhat, kappa);
     }
+    double sum = 0;
+    for (int64_t i = 0; i < n; i++) sum += e[i];
+    free(e);
     return sum;
 }
 
@@ -1312,8 +1318,10 @@ double ipco_gauss_newton_quadratic_form(const ipco_collisions *c, const double *
 {
     (void)nV;
     const int64_t n = n_collisions(c);
-    double sum = 0;
-#pragma omp parallel for schedule(static) reduction(+ : sum)
+    /* summed in collision order, as the energy */
+    double *e = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!e) return NAN;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         stencil s;
         double x[12], px[12], cc[4], t[3], q[3];
@@ -1326,8 +1334,11 @@ double ipco_gauss_newton_quadratic_form(const ipco_collisions *c, const double *
         const double d2 = dot_seq(t, t);
         const double gf = f1(d2, c->dmin, dhat, kappa), hf = f2(d2, c->dmin, dhat, kappa);
         const double qt = dot_seq(q, t);
-        sum += s.w * (4.0 * hf * (qt * qt) + 2.0 * gf * dot_seq(q, q));
+        e[i] = s.w * (4.0 * hf * (qt * qt) + 2.0 * gf * dot_seq(q, q));
     }
+    double sum = 0;
+    for (int64_t i = 0; i < n; i++) sum += e[i];
+    free(e);
     return sum;
 }
 
