@@ -489,21 +489,17 @@ __global__ void k_wf_decide(CCDArgs A, WFState* st, int64_t nb, const WFBox* box
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nb) return;
-    WFState S = st[i];
+    WFState& S = st[i]; // updated in place (one lane per search)
     S.restart = 0;
     const int64_t s0 = S.seg_start, s1 = S.seg_start + S.seg_count;
     unsigned long long* e = ev + (int64_t)WF_EV * i;
     const unsigned long long e_stop = e[0], e_bad = e[1], e_zero = e[2], e_max = e[3];
     for (int k = 0; k < WF_EV; k++) e[k] = WF_NONE;
     S.cut = s0;
-    if (S.done) {
-        st[i] = S;
-        return;
-    }
+    if (S.done) return;
     if (++S.levels > TI_MAX_LEVELS) {
         atomicOr(error, 1);
         S.done = 1;
-        st[i] = S;
         return;
     }
     const TIQuery& q = S.q;
@@ -550,7 +546,6 @@ __global__ void k_wf_decide(CCDArgs A, WFState* st, int64_t nb, const WFBox* box
         S.temp_out_tol = m;
         wf_search_done(A, S, true, S.temp_toi, S.temp_out_tol);
     }
-    st[i] = S;
 }
 
 // children of every box (split_and_push: the upper half, then the lower)
